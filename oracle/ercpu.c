@@ -1,0 +1,1469 @@
+/*
+ * ercpu.c -- the oracle: FP64 CPU restatement of ERmod's per-frame
+ * solute-solvent energy pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY: linked by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg as the checker; never by the product.
+ *
+ * Each function names the reference lines it restates.  Floating-point
+ * operations follow the reference's source order (left-to-right Fortran
+ * evaluation, sequential array sums); the file is compiled with
+ * -ffp-contract=off so that no multiply-add is fused behind our back.
+ *
+ * Pinning (tests/test_oracle_*.py): spline values against the reference's
+ * own spline_test.F90 known answers and against spline.F90 compiled from the
+ * reference (oracle/_ref); the hash against hash_real.c compiled from the
+ * reference; edges against their closed form; the binary search against
+ * numpy.searchsorted; the FFT against numpy.fft; the Green's function against
+ * an independent closed form; the real-space kernel against a brute-force
+ * double loop; reciprocal energies against a direct Ewald k-sum; the composed
+ * energy against the NaCl Madelung constant and against its independence of
+ * the Ewald screening parameter; histograms and the correlation matrix
+ * against brute-force counting.  Parity with the reference's exact FP64 bits
+ * (its -ffast-math build, its FFT library, its Fortran runtime erf/exp) is
+ * not pinnable without running the reference and is not claimed.
+ */
+#include "ercpu.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* engmain.F90:203 -- PI = 3.1415926535897932 promoted to real(8) */
+static const double PI_REF = 3.1415926535897932;
+
+/* ====================================================================== */
+/* spline.F90:26-59                                                       */
+/* ====================================================================== */
+void orc_spline_coeff(int32_t order, double *coeff) {
+  for (int i = 0; i <= order; ++i) {
+    double f = 1.0;
+    for (int k = 1; k <= i; ++k) f = f * (double)k;
+    for (int k = 1; k <= order - i; ++k) f = f * (double)k;
+    f = (double)order / f;
+    if (i % 2 == 1) f = -f;
+    coeff[i] = f;
+  }
+}
+
+/* integer power as repeated squaring, the lowering of Fortran x**n */
+static double ipow(double x, int n) {
+  double r = 1.0;
+  int first = 1;
+  double b = x;
+  while (n > 0) {
+    if (n & 1) {
+      r = first ? b : r * b;
+      first = 0;
+    }
+    n >>= 1;
+    if (n) b = b * b;
+  }
+  return first ? 1.0 : r;
+}
+
+double orc_spline_value(const double *coeff, int32_t order, double x) {
+  double f = 0.0;
+  if (x > 0.0 && x < (double)order) {
+    int k = (int)x;
+    for (int i = 0; i <= k; ++i) f = f + coeff[i] * ipow(x - (double)i, order - 1);
+  }
+  return f;
+}
+
+/* ====================================================================== */
+/* enginit edge construction, engproc.F90:163-218                         */
+/* ====================================================================== */
+static long fortran_nint(double x) { return (long)(x >= 0 ? floor(x + 0.5) : -floor(-x + 0.5)); }
+
+int32_t orc_build_edges(const orc_hist *h, int32_t self_energy, double *ercrd_out,
+                        int32_t cap, int32_t *pemax_out, int32_t *pesoft_out) {
+  enum { RGLMAX = 5, LARGE = 10000 };
+  double ecdmin = h->ecdmin, ecfmns = h->ecfmns, ecdcen = h->ecdcen;
+  double eccore = h->eccore, ecdmax = h->ecdmax;
+  double eclbin = h->eclbin, ecfbin = h->ecfbin, ec0bin = h->ec0bin;
+  double finfac = h->finfac;
+  int pecore = h->pecore;
+  double ectmvl = finfac * ecfbin;
+  ecdmin = ecdmin - ectmvl;
+  ecfmns = ecfmns - ectmvl;
+  double ecmns0 = ecdcen - ectmvl;
+  double ecpls0 = 2.0 * ecdcen - ecmns0;
+  double ecfpls = 2.0 * ecdcen - ecfmns;
+  eccore = eccore + ecfpls - ecdcen;
+  double cdrgvl[RGLMAX + 2] = {ecdmin, ecfmns, ecmns0, ecpls0, ecfpls, eccore, ecdmax};
+  int uprgcd[RGLMAX + 2];
+  uprgcd[0] = 1;
+  for (int regn = 1; regn <= RGLMAX; ++regn) {
+    double factor = (regn == 1 || regn == 5) ? eclbin : (regn == 2 || regn == 4) ? ecfbin : ec0bin;
+    double incre = cdrgvl[regn] - cdrgvl[regn - 1];
+    if (incre < factor) return ORC_E_ENG_ECD;
+    long iduv = fortran_nint(incre / factor);
+    uprgcd[regn] = uprgcd[regn - 1] + (int)iduv;
+  }
+  int pesoft = uprgcd[RGLMAX] - uprgcd[0];
+  int pemax = pesoft + pecore;
+  uprgcd[RGLMAX + 1] = pemax;
+  if (pemax > LARGE) return ORC_E_ENG_SIZ;
+  /* 1-based work array large enough for index pesoft+1 */
+  int top = (pemax > uprgcd[RGLMAX] ? pemax : uprgcd[RGLMAX]) + 1;
+  double *e = (double *)calloc((size_t)top + 1, sizeof(double));
+  if (!e) return ORC_E_ARG;
+  for (int r = 0; r <= RGLMAX + 1; ++r) e[uprgcd[r]] = cdrgvl[r];
+  int nreg = (pecore == 0) ? RGLMAX : RGLMAX + 1;
+  for (int regn = 1; regn <= nreg; ++regn) {
+    int minrg = uprgcd[regn - 1], maxrg = uprgcd[regn];
+    double incre = 0.0;
+    if (regn <= RGLMAX) incre = e[maxrg] - e[minrg];
+    if (regn == RGLMAX + 1) incre = log(e[maxrg] / e[minrg]);
+    double factor = incre / (double)(maxrg - minrg);
+    for (int iduv = minrg; iduv <= maxrg - 1; ++iduv) {
+      incre = factor * (double)(iduv - minrg);
+      if (regn <= RGLMAX) e[iduv] = e[minrg] + incre;
+      if (regn == RGLMAX + 1) e[iduv] = e[minrg] * exp(incre);
+    }
+  }
+  int n = self_energy ? pesoft : pemax;
+  if (n > cap) {
+    free(e);
+    return ORC_E_ARG;
+  }
+  for (int i = 0; i < n; ++i) ercrd_out[i] = e[i + 1];
+  free(e);
+  *pemax_out = n;
+  *pesoft_out = pesoft;
+  return ORC_OK;
+}
+
+/* engproc.F90:927-957: count of edges strictly below v */
+int32_t orc_binsearch(const double *coord, int32_t n, double v) {
+  if (v < coord[0]) return 0;
+  if (v > coord[n - 1]) return n;
+  int rmin = 1, rmax = n + 1;
+  while (rmax - rmin > 1) {
+    int rmid = (rmin + rmax - 1) / 2;
+    if (v > coord[rmid - 1])
+      rmin = rmid + 1;
+    else
+      rmax = rmid + 1;
+  }
+  return rmin - 1;
+}
+
+/* engproc.F90:1185-1223, 'intn' branch for one species (idpt 1-based) */
+void orc_repval(const double *crd, int32_t idmax, int32_t idsoft, int32_t idpt, double *out) {
+  if (idpt <= idsoft)
+    *out = (crd[idpt - 1] + crd[idpt]) / 2.0;
+  else if (idpt < idmax)
+    *out = sqrt(crd[idpt - 1] * crd[idpt]);
+  else
+    *out = crd[idpt - 1];
+}
+
+/* ====================================================================== */
+/* hash_real.c:5-15                                                       */
+/* ====================================================================== */
+uint64_t orc_hash_double(const double *v, int32_t n) {
+  uint64_t x = 0, b;
+  for (int32_t i = 0; i < n; ++i) {
+    memcpy(&b, &v[i], sizeof b);
+    x = ((x << 7) | (x >> 57)) ^ b;
+  }
+  return x;
+}
+
+/* ====================================================================== */
+/* engproc.F90:889-919                                                    */
+/* ====================================================================== */
+#define CM(m, i, j) ((m)[3 * ((j)-1) + ((i)-1)]) /* 1-based column-major */
+void orc_inverted_cell(const double *cell, double *invcl, double *volume) {
+  double v = CM(cell, 1, 1) * CM(cell, 2, 2) * CM(cell, 3, 3) + CM(cell, 1, 2) * CM(cell, 2, 3) * CM(cell, 3, 1) +
+             CM(cell, 1, 3) * CM(cell, 2, 1) * CM(cell, 3, 2) - CM(cell, 1, 3) * CM(cell, 2, 2) * CM(cell, 3, 1) -
+             CM(cell, 1, 2) * CM(cell, 2, 1) * CM(cell, 3, 3) - CM(cell, 1, 1) * CM(cell, 2, 3) * CM(cell, 3, 2);
+  CM(invcl, 1, 1) = CM(cell, 2, 2) * CM(cell, 3, 3) - CM(cell, 2, 3) * CM(cell, 3, 2);
+  CM(invcl, 1, 2) = CM(cell, 1, 3) * CM(cell, 3, 2) - CM(cell, 1, 2) * CM(cell, 3, 3);
+  CM(invcl, 1, 3) = CM(cell, 1, 2) * CM(cell, 2, 3) - CM(cell, 1, 3) * CM(cell, 2, 2);
+  CM(invcl, 2, 1) = CM(cell, 2, 3) * CM(cell, 3, 1) - CM(cell, 2, 1) * CM(cell, 3, 3);
+  CM(invcl, 2, 2) = CM(cell, 1, 1) * CM(cell, 3, 3) - CM(cell, 1, 3) * CM(cell, 3, 1);
+  CM(invcl, 2, 3) = CM(cell, 1, 3) * CM(cell, 2, 1) - CM(cell, 1, 1) * CM(cell, 2, 3);
+  CM(invcl, 3, 1) = CM(cell, 2, 1) * CM(cell, 3, 2) - CM(cell, 2, 2) * CM(cell, 3, 1);
+  CM(invcl, 3, 2) = CM(cell, 1, 2) * CM(cell, 3, 1) - CM(cell, 1, 1) * CM(cell, 3, 2);
+  CM(invcl, 3, 3) = CM(cell, 1, 1) * CM(cell, 2, 2) - CM(cell, 1, 2) * CM(cell, 2, 1);
+  for (int i = 0; i < 9; ++i) invcl[i] = invcl[i] / v;
+  *volume = v;
+}
+
+/* ====================================================================== */
+/* FFT: mixed-radix (2,3,5,7) Stockham autosort on interleaved complex.   */
+/* Library-convention wrapper for the reference's fft_iface.F90 calls.    */
+/* ====================================================================== */
+typedef struct {
+  int n, nf, fac[40];
+  double *tw; /* tw[2j], tw[2j+1] = cos, sin of -2 pi j / n */
+} fftplan;
+
+static int fft_plan(int n, fftplan *p) {
+  p->n = n;
+  p->nf = 0;
+  p->tw = NULL;
+  int m = n;
+  static const int radices[] = {4, 2, 3, 5, 7};
+  for (int ri = 0; ri < 5; ++ri) {
+    int r = radices[ri];
+    while (m % r == 0) {
+      p->fac[p->nf++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) return 0;
+  p->tw = (double *)malloc(sizeof(double) * 2 * n);
+  for (int j = 0; j < n; ++j) {
+    double ang = -2.0 * M_PI * (double)j / (double)n;
+    p->tw[2 * j] = cos(ang);
+    p->tw[2 * j + 1] = sin(ang);
+  }
+  return 1;
+}
+static void fft_unplan(fftplan *p) { free(p->tw); }
+
+/* one 1-D transform of length n on x (interleaved complex, stride in
+ * complex units); sign -1 forward, +1 backward; work: 4n doubles.
+ * Stockham autosort: stage with radix R and span p reads a[i + q m]
+ * (m = n/R), applies the twiddle w^(q k) with k = i mod p and w the
+ * (p R)-th root of unity, and writes the R-point DFT to b[(i-k)R + k + q p]. */
+static void fft1d(const fftplan *pl, double *x, long stride, int sign, double *work) {
+  int n = pl->n;
+  const double *tw = pl->tw;
+  double *a = work, *b = work + 2 * n;
+  for (int i = 0; i < n; ++i) {
+    a[2 * i] = x[2 * stride * i];
+    a[2 * i + 1] = x[2 * stride * i + 1];
+  }
+  double sg = (double)(-sign); /* tw holds e^{-i}; conjugate for sign +1 */
+  int p = 1;
+  for (int f = 0; f < pl->nf; ++f) {
+    int R = pl->fac[f];
+    int m = n / R;
+    int tstep = n / (p * R);
+    for (int i = 0; i < m; ++i) {
+      int k = i % p;
+      double xr[8], xi[8];
+      for (int q = 0; q < R; ++q) {
+        double re = a[2 * (i + q * m)], im = a[2 * (i + q * m) + 1];
+        if (q && k) {
+          int t = (q * k * tstep) % n;
+          double c = tw[2 * t], s = sg * tw[2 * t + 1];
+          double u = re * c - im * s;
+          im = re * s + im * c;
+          re = u;
+        }
+        xr[q] = re;
+        xi[q] = im;
+      }
+      int j = (i - k) * R + k;
+      if (R == 2) {
+        b[2 * j] = xr[0] + xr[1];
+        b[2 * j + 1] = xi[0] + xi[1];
+        b[2 * (j + p)] = xr[0] - xr[1];
+        b[2 * (j + p) + 1] = xi[0] - xi[1];
+      } else if (R == 4) {
+        /* y_q = sum_s x_s e^{sign 2 pi i s q / 4}; e^{sign i pi/2} = sign*i */
+        double ar = xr[0] + xr[2], ai = xi[0] + xi[2];
+        double br = xr[0] - xr[2], bi = xi[0] - xi[2];
+        double cr = xr[1] + xr[3], ci = xi[1] + xi[3];
+        double dr = xr[1] - xr[3], di = xi[1] - xi[3];
+        double s = (double)sign;
+        b[2 * j] = ar + cr;
+        b[2 * j + 1] = ai + ci;
+        b[2 * (j + 2 * p)] = ar - cr;
+        b[2 * (j + 2 * p) + 1] = ai - ci;
+        /* q=1: b + sign*i*d ; q=3: b - sign*i*d */
+        b[2 * (j + p)] = br - s * di;
+        b[2 * (j + p) + 1] = bi + s * dr;
+        b[2 * (j + 3 * p)] = br + s * di;
+        b[2 * (j + 3 * p) + 1] = bi - s * dr;
+      } else {
+        int rstep = n / R;
+        for (int q = 0; q < R; ++q) {
+          double sr = 0.0, si = 0.0;
+          for (int s2 = 0; s2 < R; ++s2) {
+            int t = ((s2 * q) % R) * rstep;
+            double c = tw[2 * t], s = sg * tw[2 * t + 1];
+            sr += xr[s2] * c - xi[s2] * s;
+            si += xr[s2] * s + xi[s2] * c;
+          }
+          b[2 * (j + q * p)] = sr;
+          b[2 * (j + q * p) + 1] = si;
+        }
+      }
+    }
+    double *t = a;
+    a = b;
+    b = t;
+    p *= R;
+  }
+  for (int i = 0; i < n; ++i) {
+    x[2 * stride * i] = a[2 * i];
+    x[2 * stride * i + 1] = a[2 * i + 1];
+  }
+}
+
+int32_t orc_fft_r2c_3d(const int32_t *K, const double *in, double *out) {
+  int K1 = K[0], K2 = K[1], K3 = K[2], H = K1 / 2 + 1;
+  fftplan p1 = {0}, p2 = {0}, p3 = {0};
+  if (!fft_plan(K1, &p1) || !fft_plan(K2, &p2) || !fft_plan(K3, &p3)) {
+    fft_unplan(&p1);
+    fft_unplan(&p2);
+    fft_unplan(&p3);
+    return ORC_E_UNSUPPORTED;
+  }
+  int nmax = K1 > K2 ? K1 : K2;
+  nmax = nmax > K3 ? nmax : K3;
+  double *work = (double *)malloc(sizeof(double) * 4 * nmax);
+  double *row = (double *)malloc(sizeof(double) * 2 * K1);
+  /* x: real rows -> half spectrum */
+  for (int z = 0; z < K3; ++z)
+    for (int y = 0; y < K2; ++y) {
+      const double *src = in + ((long)z * K2 + y) * K1;
+      for (int x = 0; x < K1; ++x) {
+        row[2 * x] = src[x];
+        row[2 * x + 1] = 0.0;
+      }
+      fft1d(&p1, row, 1, -1, work);
+      double *dst = out + 2 * ((long)z * K2 + y) * H;
+      for (int x = 0; x < H; ++x) {
+        dst[2 * x] = row[2 * x];
+        dst[2 * x + 1] = row[2 * x + 1];
+      }
+    }
+  /* y */
+  for (int z = 0; z < K3; ++z)
+    for (int x = 0; x < H; ++x) fft1d(&p2, out + 2 * ((long)z * K2 * H + x), H, -1, work);
+  /* z */
+  for (int y = 0; y < K2; ++y)
+    for (int x = 0; x < H; ++x) fft1d(&p3, out + 2 * ((long)y * H + x), (long)K2 * H, -1, work);
+  free(work);
+  free(row);
+  fft_unplan(&p1);
+  fft_unplan(&p2);
+  fft_unplan(&p3);
+  return ORC_OK;
+}
+
+int32_t orc_fft_c2r_3d(const int32_t *K, const double *in, double *out) {
+  int K1 = K[0], K2 = K[1], K3 = K[2], H = K1 / 2 + 1;
+  fftplan p1 = {0}, p2 = {0}, p3 = {0};
+  if (!fft_plan(K1, &p1) || !fft_plan(K2, &p2) || !fft_plan(K3, &p3)) {
+    fft_unplan(&p1);
+    fft_unplan(&p2);
+    fft_unplan(&p3);
+    return ORC_E_UNSUPPORTED;
+  }
+  int nmax = K1 > K2 ? K1 : K2;
+  nmax = nmax > K3 ? nmax : K3;
+  double *work = (double *)malloc(sizeof(double) * 4 * nmax);
+  double *row = (double *)malloc(sizeof(double) * 2 * K1);
+  long nc = (long)H * K2 * K3;
+  double *t = (double *)malloc(sizeof(double) * 2 * nc);
+  memcpy(t, in, sizeof(double) * 2 * nc);
+  for (int y = 0; y < K2; ++y)
+    for (int x = 0; x < H; ++x) fft1d(&p3, t + 2 * ((long)y * H + x), (long)K2 * H, +1, work);
+  for (int z = 0; z < K3; ++z)
+    for (int x = 0; x < H; ++x) fft1d(&p2, t + 2 * ((long)z * K2 * H + x), H, +1, work);
+  /* x: hermitian half rows -> real rows; the imaginary parts of the
+   * self-conjugate entries (x = 0 and x = K1/2) are ignored */
+  for (int z = 0; z < K3; ++z)
+    for (int y = 0; y < K2; ++y) {
+      const double *src = t + 2 * ((long)z * K2 + y) * H;
+      for (int x = 0; x < H; ++x) {
+        row[2 * x] = src[2 * x];
+        row[2 * x + 1] = src[2 * x + 1];
+      }
+      row[1] = 0.0;
+      if (K1 % 2 == 0) row[2 * (K1 / 2) + 1] = 0.0;
+      for (int x = H; x < K1; ++x) {
+        row[2 * x] = src[2 * (K1 - x)];
+        row[2 * x + 1] = -src[2 * (K1 - x) + 1];
+      }
+      fft1d(&p1, row, 1, +1, work);
+      double *dst = out + ((long)z * K2 + y) * K1;
+      for (int x = 0; x < K1; ++x) dst[x] = row[2 * x];
+    }
+  free(t);
+  free(work);
+  free(row);
+  fft_unplan(&p1);
+  fft_unplan(&p2);
+  fft_unplan(&p3);
+  return ORC_OK;
+}
+
+/* ====================================================================== */
+/* recpcal.F90:87-152, 408-442                                            */
+/* ====================================================================== */
+void orc_spline_axis(int32_t K, int32_t p, double *splfc) {
+  double coeff[32];
+  orc_spline_coeff(p, coeff);
+  for (int rci = 0; rci < K; ++rci) {
+    double re = 0.0, im = 0.0;
+    for (int spi = 0; spi <= p - 2; ++spi) {
+      double chr = orc_spline_value(coeff, p, (double)(spi + 1));
+      double rtp2 = 2.0 * PI_REF * (double)(spi * rci) / (double)K;
+      re = re + chr * cos(rtp2);
+      im = im + chr * sin(rtp2);
+    }
+    /* real(rcpi * conjg(rcpi)) */
+    splfc[rci] = re * re - im * (-im);
+  }
+}
+
+void orc_greenfunc(const int32_t *K, int32_t p, const double *invcl, double volume, double screen,
+                   const double *splfc1, const double *splfc2, const double *splfc3, double *engfac) {
+  int K1 = K[0], K2 = K[1], K3 = K[2], H = K1 / 2 + 1;
+  for (int rc3 = 0; rc3 < K3; ++rc3)
+    for (int rc2 = 0; rc2 < K2; ++rc2)
+      for (int rc1 = 0; rc1 < H; ++rc1) {
+        double factor = 0.0;
+        long idx = ((long)rc3 * K2 + rc2) * H + rc1;
+        if (rc1 == 0 && rc2 == 0 && rc3 == 0) {
+          engfac[idx] = 0.0; /* left unset by the reference, zeroed after */
+          continue;
+        }
+        int rci[3] = {rc1, rc2, rc3};
+        double inm[3];
+        int skip = 0;
+        for (int m = 0; m < 3; ++m) {
+          if ((p % 2 == 1) && (2 * abs(rci[m]) == K[m])) {
+            skip = 1;
+            break;
+          }
+          inm[m] = (rci[m] <= K[m] / 2) ? (double)rci[m] : (double)(rci[m] - K[m]);
+        }
+        if (!skip) {
+          double xst[3];
+          for (int m = 0; m < 3; ++m)
+            xst[m] = invcl[3 * m + 0] * inm[0] + invcl[3 * m + 1] * inm[1] + invcl[3 * m + 2] * inm[2];
+          double rtp2 = xst[0] * xst[0] + xst[1] * xst[1] + xst[2] * xst[2];
+          double chr = (PI_REF * PI_REF) * rtp2 / (screen * screen);
+          factor = exp(-chr) / rtp2 / PI_REF / volume;
+          rtp2 = splfc1[rc1] * splfc2[rc2] * splfc3[rc3];
+          factor = factor / rtp2;
+        }
+        engfac[idx] = factor;
+      }
+  engfac[0] = 0.0;
+}
+
+void orc_spline_site(const double *x, const double *invcl, const int32_t *K, int32_t p, const double *coeff,
+                     double *w, int32_t *g) {
+  double inm[3];
+  for (int k = 0; k < 3; ++k) {
+    /* dot_product(invcl(k,:), xst(:)) */
+    double f = invcl[k] * x[0] + invcl[3 + k] * x[1] + invcl[6 + k] * x[2];
+    f = f - floor(f);
+    inm[k] = f;
+  }
+  for (int m = 0; m < 3; ++m) {
+    double factor = inm[m] * (double)K[m];
+    int rci = (int)factor;
+    for (int spi = 0; spi < p; ++spi) {
+      double rtp2 = factor - (double)(rci - spi);
+      w[m * p + spi] = orc_spline_value(coeff, p, rtp2);
+    }
+    g[m] = rci;
+  }
+}
+
+/* ====================================================================== */
+/* pose: insertion.F90:103-366, bestfit.F90:49-100,158-203                */
+/* ====================================================================== */
+static void com_of(int n, const double *pts, const double *mass, double *c) {
+  double msum = 0.0;
+  for (int j = 0; j < n; ++j) msum = msum + mass[j];
+  for (int i = 0; i < 3; ++i) {
+    double d = 0.0;
+    for (int j = 0; j < n; ++j) d = d + pts[3 * j + i] * mass[j];
+    c[i] = d / msum;
+  }
+}
+
+static void qprod(const double *a, const double *b, double *p) {
+  double dot = a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+  p[0] = a[0] * b[0] - dot;
+  double cr[3] = {a[2] * b[3] - a[3] * b[2], a[3] * b[1] - a[1] * b[3], a[1] * b[2] - a[2] * b[1]};
+  for (int i = 0; i < 3; ++i) p[1 + i] = cr[i] + a[0] * b[1 + i] + b[0] * a[1 + i];
+}
+
+static void qrotate(const double *v, const double *q, double *res) {
+  double q2[4] = {0.0, v[0], v[1], v[2]};
+  double cj[4] = {q[0], -q[1], -q[2], -q[3]};
+  double t[4], r2[4];
+  qprod(q, q2, t);
+  qprod(t, cj, r2);
+  res[0] = r2[1];
+  res[1] = r2[2];
+  res[2] = r2[3];
+}
+
+void orc_place_solute(int32_t n, const double *bfcoord, const double *mass, const double *cell, const double *r,
+                      const double *q, double *out) {
+  double c[3];
+  memcpy(out, bfcoord, sizeof(double) * 3 * n);
+  /* set_solute_origin, INSORG_ORIGIN: COM to (0,0,0) */
+  com_of(n, out, mass, c);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + 0.0;
+  /* set_shift_com, INSPOS_RANDOM: COM = cell . r */
+  double com[3];
+  for (int i = 0; i < 3; ++i) com[i] = cell[i] * r[0] + cell[3 + i] * r[1] + cell[6 + i] * r[2];
+  com_of(n, out, mass, c);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + com[i];
+  /* apply_orientation, INSROT_RANDOM */
+  com_of(n, out, mass, c);
+  for (int j = 0; j < n; ++j) {
+    double v[3] = {out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]};
+    double t[3];
+    qrotate(v, q, t);
+    for (int i = 0; i < 3; ++i) out[3 * j + i] = t[i] + c[i];
+  }
+}
+
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+double orc_rng_uniform(uint64_t seed, uint64_t index, uint32_t draw) {
+  uint64_t key = mix64(seed + 0x9E3779B97F4A7C15ULL);
+  uint64_t ctr = index * 256ULL + (uint64_t)draw;
+  uint64_t z = mix64(key ^ mix64(ctr + 0x632BE59BD9B4E019ULL));
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+/* insertion.F90:153-163 (position) then 340-356 (quaternion by rejection) */
+void orc_random_pose(uint64_t seed, uint64_t index, double *r, double *q) {
+  for (int i = 0; i < 3; ++i) r[i] = orc_rng_uniform(seed, index, (uint32_t)i);
+  for (int a = 0; a < 63; ++a) {
+    double t[4];
+    for (int i = 0; i < 4; ++i) t[i] = orc_rng_uniform(seed, index, (uint32_t)(3 + 4 * a + i)) * 2.0 - 1.0;
+    double s = t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + t[3] * t[3];
+    if (s < 1.0) {
+      double nrm = sqrt(s);
+      for (int i = 0; i < 4; ++i) q[i] = t[i] / nrm;
+      return;
+    }
+  }
+  q[0] = 1.0;
+  q[1] = q[2] = q[3] = 0.0;
+}
+
+/* ====================================================================== */
+/* context                                                                */
+/* ====================================================================== */
+struct orc_ctx {
+  orc_system s;
+  /* owned copies of the system arrays */
+  int32_t *mol_begin, *ljtype, *sluvid, *uvspec, *uvmax, *uvsoft;
+  double *charge, *lj_eps, *lj_sig2, *mol_charge, *uvcrd, *escrd;
+  int32_t ermax;
+  /* tagpt (engproc.F90:336-347) */
+  int32_t slvmax, *tagpt, *kof; /* kof[mol] = index into tagpt or -1 */
+  /* PME */
+  int32_t p, K[3], H;
+  double coeff[33];
+  double *splfc[3], *engfac, *cnv, *rcp;
+  int32_t *slvtag, nslot;
+  double *splslv;
+  int32_t *grdslv;
+  int32_t have_cell;
+  double prevcl[9];
+  /* frame */
+  double cell[9], invcl[9], volume, L[3], stat_weight_system;
+  double *pos, *posn;
+  /* accumulators */
+  double *edens, *ecorr, *eself, *minuv, *maxuv, *slnuv, *flceng;
+  double engnorm, engsmpl, avslf, voffset;
+  int32_t voffset_init, warn_bin_first;
+  int32_t *insdst;
+  /* realcal_self cache (engproc.F90:712-730) */
+  uint64_t solute_hash;
+  double usreal;
+  /* scratch */
+  double *spos, *sposn;
+};
+
+static int32_t *dupi(const int32_t *src, long n) {
+  int32_t *d = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  if (src && n > 0) memcpy(d, src, sizeof(int32_t) * (size_t)n);
+  return d;
+}
+static double *dupd(const double *src, long n) {
+  double *d = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  if (src && n > 0) memcpy(d, src, sizeof(double) * (size_t)n);
+  return d;
+}
+#define DUPI(dst, src, n) ((dst) = dupi((src), (long)(n)))
+#define DUPD(dst, src, n) ((dst) = dupd((src), (long)(n)))
+
+int32_t orc_slvmax(const orc_ctx *c) { return c->slvmax; }
+const double *orc_flceng(const orc_ctx *c) { return c->flceng; }
+
+orc_ctx *orc_create(const orc_system *sys, int32_t *status) {
+  *status = ORC_OK;
+  if (sys->splodr < 2 || sys->splodr > 32 || sys->ms[0] < 2 || sys->ms[1] < 2 || sys->ms[2] < 2) {
+    *status = ORC_E_ARG;
+    return NULL;
+  }
+  orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+  c->s = *sys;
+  int na = sys->natom, nm = sys->nmol, nt = sys->ntype;
+  DUPI(c->mol_begin, sys->mol_begin, nm + 1);
+  DUPI(c->ljtype, sys->ljtype, na);
+  DUPI(c->sluvid, sys->sluvid, nm);
+  DUPI(c->uvspec, sys->uvspec, nm);
+  DUPI(c->uvmax, sys->uvmax, sys->numslv);
+  DUPI(c->uvsoft, sys->uvsoft, sys->numslv);
+  DUPD(c->charge, sys->charge, na);
+  DUPD(c->lj_eps, sys->lj_eps, nt * nt);
+  DUPD(c->lj_sig2, sys->lj_sig2, nt * nt);
+  DUPD(c->mol_charge, sys->mol_charge, nm);
+  c->ermax = 0;
+  for (int i = 0; i < sys->numslv; ++i) c->ermax += sys->uvmax[i];
+  DUPD(c->uvcrd, sys->uvcrd, c->ermax);
+  DUPD(c->escrd, sys->selfcal ? sys->escrd : NULL, sys->selfcal ? sys->esmax : 0);
+  c->s.mol_begin = c->mol_begin;
+  c->s.ljtype = c->ljtype;
+  c->s.sluvid = c->sluvid;
+  c->s.uvspec = c->uvspec;
+  c->s.uvmax = c->uvmax;
+  c->s.uvsoft = c->uvsoft;
+  c->s.charge = c->charge;
+  c->s.lj_eps = c->lj_eps;
+  c->s.lj_sig2 = c->lj_sig2;
+  c->s.mol_charge = c->mol_charge;
+  c->s.uvcrd = c->uvcrd;
+  c->s.escrd = c->escrd;
+  /* tagpt: physical molecules (solvent or solute) */
+  c->tagpt = (int32_t *)malloc(sizeof(int32_t) * nm);
+  c->kof = (int32_t *)malloc(sizeof(int32_t) * nm);
+  c->slvmax = 0;
+  for (int i = 0; i < nm; ++i) {
+    c->kof[i] = -1;
+    if (sys->sluvid[i] == 0 || sys->sluvid[i] == 1) {
+      c->kof[i] = c->slvmax;
+      c->tagpt[c->slvmax++] = i;
+    }
+  }
+  /* recpcal_init (recpcal.F90:42-85) */
+  c->p = sys->splodr;
+  for (int m = 0; m < 3; ++m) c->K[m] = sys->ms[m];
+  c->H = c->K[0] / 2 + 1;
+  orc_spline_coeff(c->p, c->coeff);
+  c->slvtag = (int32_t *)malloc(sizeof(int32_t) * nm);
+  c->nslot = 0;
+  for (int i = 0; i < nm; ++i) c->slvtag[i] = -1;
+  for (int k = 0; k < c->slvmax; ++k) {
+    int m = c->tagpt[k];
+    c->slvtag[m] = c->nslot;
+    c->nslot += c->mol_begin[m + 1] - c->mol_begin[m];
+  }
+  c->splslv = (double *)malloc(sizeof(double) * 3 * c->p * (c->nslot ? c->nslot : 1));
+  c->grdslv = (int32_t *)malloc(sizeof(int32_t) * 3 * (c->nslot ? c->nslot : 1));
+  for (int m = 0; m < 3; ++m) {
+    c->splfc[m] = (double *)malloc(sizeof(double) * c->K[m]);
+    orc_spline_axis(c->K[m], c->p, c->splfc[m]);
+  }
+  long nr = (long)c->K[0] * c->K[1] * c->K[2];
+  long nh = (long)c->H * c->K[1] * c->K[2];
+  c->engfac = (double *)malloc(sizeof(double) * nh);
+  c->cnv = (double *)malloc(sizeof(double) * nr);
+  c->rcp = (double *)malloc(sizeof(double) * 2 * nh);
+  c->pos = (double *)malloc(sizeof(double) * 3 * (na ? na : 1));
+  c->posn = (double *)malloc(sizeof(double) * 3 * (na ? na : 1));
+  int maxsite = 1;
+  for (int i = 0; i < nm; ++i) {
+    int ns = c->mol_begin[i + 1] - c->mol_begin[i];
+    if (ns > maxsite) maxsite = ns;
+  }
+  c->spos = (double *)malloc(sizeof(double) * 3 * maxsite);
+  c->sposn = (double *)malloc(sizeof(double) * 3 * maxsite);
+  /* accumulators */
+  int ns1 = sys->numslv + 1;
+  c->edens = (double *)calloc(c->ermax ? c->ermax : 1, sizeof(double));
+  c->ecorr = sys->corrcal ? (double *)calloc((size_t)c->ermax * c->ermax, sizeof(double)) : NULL;
+  c->eself = sys->selfcal ? (double *)calloc(sys->esmax ? sys->esmax : 1, sizeof(double)) : NULL;
+  c->minuv = (double *)malloc(sizeof(double) * ns1);
+  c->maxuv = (double *)malloc(sizeof(double) * ns1);
+  c->slnuv = (double *)calloc(ns1, sizeof(double));
+  c->flceng = (double *)calloc(ns1, sizeof(double));
+  c->insdst = (int32_t *)calloc(c->ermax ? c->ermax : 1, sizeof(int32_t));
+  for (int i = 0; i < ns1; ++i) {
+    c->minuv[i] = 1.0e50;
+    c->maxuv[i] = -1.0e50;
+  }
+  c->voffset = -1.0e50;
+  c->warn_bin_first = 1;
+  orc_engclear(c);
+  return c;
+}
+
+void orc_destroy(orc_ctx *c) {
+  if (!c) return;
+  void *ptrs[] = {c->mol_begin, c->ljtype, c->sluvid, c->uvspec, c->uvmax, c->uvsoft, c->charge, c->lj_eps,
+                  c->lj_sig2, c->mol_charge, c->uvcrd, c->escrd, c->tagpt, c->kof, c->slvtag, c->splslv,
+                  c->grdslv, c->splfc[0], c->splfc[1], c->splfc[2], c->engfac, c->cnv, c->rcp, c->pos,
+                  c->posn, c->spos, c->sposn, c->edens, c->ecorr, c->eself, c->minuv, c->maxuv, c->slnuv,
+                  c->flceng, c->insdst};
+  for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+  free(c);
+}
+
+/* engproc.F90:276-288 */
+void orc_engclear(orc_ctx *c) {
+  memset(c->edens, 0, sizeof(double) * c->ermax);
+  if (c->ecorr) memset(c->ecorr, 0, sizeof(double) * (size_t)c->ermax * c->ermax);
+  if (c->eself) memset(c->eself, 0, sizeof(double) * c->s.esmax);
+  memset(c->slnuv, 0, sizeof(double) * (c->s.numslv + 1));
+  c->avslf = 0.0;
+  c->engnorm = 0.0;
+  c->engsmpl = 0.0;
+}
+
+static double wrap1(double x, double L) { return x - L * floor(x * (1.0 / L)); }
+
+/* per-frame part of engconst (engproc.F90:352-383) plus the cuboid
+ * normalisation of realcal_prepare (realcal.F90:137-146, 789-885) */
+int32_t orc_set_frame(orc_ctx *c, const double *sitepos, const double *cell, double stat_weight_system) {
+  for (int i = 1; i <= 3; ++i)
+    for (int j = 1; j <= 3; ++j)
+      if (i != j && fabs(CM(cell, i, j)) > 1e-8) return ORC_E_NONCUBOID;
+  memcpy(c->cell, cell, sizeof(double) * 9);
+  orc_inverted_cell(c->cell, c->invcl, &c->volume);
+  for (int i = 0; i < 3; ++i) c->L[i] = fabs(CM(cell, i + 1, i + 1));
+  c->stat_weight_system = stat_weight_system;
+  int changed = !c->have_cell;
+  for (int i = 0; i < 9 && !changed; ++i) changed = (c->prevcl[i] != cell[i]);
+  if (changed)
+    orc_greenfunc(c->K, c->p, c->invcl, c->volume, c->s.screen, c->splfc[0], c->splfc[1], c->splfc[2], c->engfac);
+  memcpy(c->prevcl, cell, sizeof(double) * 9);
+  c->have_cell = 1;
+  memcpy(c->pos, sitepos, sizeof(double) * 3 * c->s.natom);
+  for (int a = 0; a < c->s.natom; ++a)
+    for (int i = 0; i < 3; ++i) c->posn[3 * a + i] = wrap1(c->pos[3 * a + i], c->L[i]);
+  /* recpcal_prepare_solvent for every physical molecule */
+  for (int k = 0; k < c->slvmax; ++k) {
+    int m = c->tagpt[k];
+    int slot = c->slvtag[m];
+    if (slot < 0) return ORC_E_RCP_CNS;
+    for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a, ++slot)
+      orc_spline_site(c->pos + 3 * a, c->invcl, c->K, c->p, c->coeff, c->splslv + (long)3 * c->p * slot,
+                      c->grdslv + 3 * slot);
+  }
+  return ORC_OK;
+}
+
+/* solute coordinates: caller-supplied (test particle) or from the frame */
+static int solute_coords(orc_ctx *c, int solute, const double *solute_pos, int *nsite) {
+  if (solute < 0 || solute >= c->s.nmol) return ORC_E_ARG;
+  int b = c->mol_begin[solute], e = c->mol_begin[solute + 1];
+  *nsite = e - b;
+  for (int j = 0; j < *nsite; ++j)
+    for (int i = 0; i < 3; ++i) {
+      double x = solute_pos ? solute_pos[3 * j + i] : c->pos[3 * (b + j) + i];
+      c->spos[3 * j + i] = x;
+      c->sposn[3 * j + i] = wrap1(x, c->L[i]);
+    }
+  return ORC_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* real-space kernel: realcal.F90:58-135, 373-784                          */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  double lw2, up2, el2, lw3, up3, lw6, up6, repA, repB, repC, attA, attB, attC;
+} ljconst;
+
+/* realcal.F90:888-904 */
+static void gmx_force_params(int pw, double lw, double up, double *A, double *B, double *C) {
+  double d = up - lw;
+  *A = -(double)pw * ((double)(pw + 4) * up - (double)(pw + 1) * lw) / (ipow(up, pw + 2) * ipow(d, 2)) / 3.0;
+  *B = (double)pw * ((double)(pw + 3) * up - (double)(pw + 1) * lw) / (ipow(up, pw + 2) * ipow(d, 3)) / 4.0;
+  *C = 1.0 / ipow(up, pw) - *A * ipow(d, 3) - *B * ipow(d, 4);
+}
+
+static void make_ljconst(const orc_system *s, ljconst *k) {
+  k->lw2 = ipow(s->lwljcut, 2);
+  k->up2 = ipow(s->upljcut, 2);
+  k->el2 = ipow(s->elecut, 2);
+  k->lw3 = ipow(s->lwljcut, 3);
+  k->up3 = ipow(s->upljcut, 3);
+  k->lw6 = k->lw3 * k->lw3;
+  k->up6 = k->up3 * k->up3;
+  gmx_force_params(12, s->lwljcut, s->upljcut, &k->repA, &k->repB, &k->repC);
+  gmx_force_params(6, s->lwljcut, s->upljcut, &k->attA, &k->attB, &k->attC);
+}
+
+/* LJ for a pair inside the lower cutoff (realcal.F90:708-734) */
+static double lj_low(const orc_system *s, const ljconst *k, double eps, double sig2, double dist) {
+  double invr2 = sig2 / dist;
+  double invr6 = invr2 * invr2 * invr2;
+  switch (s->ljswitch) {
+    case 0:
+    case 1:
+      return 4.0 * eps * invr6 * (invr6 - 1.0);
+    case 2: {
+      double sg6 = sig2 * sig2 * sig2;
+      double vdwa = invr6 * invr6 - sg6 * sg6 / (k->lw6 * k->up6);
+      double vdwb = invr6 - sg6 / (k->lw3 * k->up3);
+      return 4.0 * eps * (vdwa - vdwb);
+    }
+    default: {
+      double sg6 = sig2 * sig2 * sig2;
+      double vdwa = invr6 * invr6 - sg6 * sg6 * k->repC;
+      double vdwb = invr6 - sg6 * k->attC;
+      return 4.0 * eps * (vdwa - vdwb);
+    }
+  }
+}
+
+/* LJ in the switching region (realcal.F90:737-776) */
+static double lj_switch(const orc_system *s, const ljconst *k, double eps, double sig2, double dist) {
+  double invr2 = sig2 / dist;
+  double invr6 = invr2 * invr2 * invr2;
+  switch (s->ljswitch) {
+    case 0:
+      return 4.0 * eps * invr6 * (invr6 - 1.0) * (2.0 * dist + k->up2 - 3.0 * k->lw2) *
+             ipow(dist - k->up2, 2) / ipow(k->up2 - k->lw2, 3);
+    case 1: {
+      double sw = (sqrt(dist) - s->lwljcut) / (s->upljcut - s->lwljcut);
+      return 4.0 * eps * invr6 * (invr6 - 1.0) *
+             (1.0 - 10.0 * ipow(sw, 3) + 15.0 * ipow(sw, 4) - 6.0 * ipow(sw, 5));
+    }
+    case 2: {
+      double invr3 = sqrt(invr6);
+      double sg6 = sig2 * sig2 * sig2;
+      double sg3 = sqrt(sg6);
+      double vdwa = k->up6 / (k->up6 - k->lw6) * ipow(invr6 - sg6 / k->up6, 2);
+      double vdwb = k->up3 / (k->up3 - k->lw3) * ipow(invr3 - sg3 / k->up3, 2);
+      return 4.0 * eps * (vdwa - vdwb);
+    }
+    default: {
+      double sg6 = sig2 * sig2 * sig2;
+      double sw = sqrt(dist) - s->lwljcut;
+      double vdwa = invr6 * invr6 - sg6 * sg6 * (k->repA * ipow(sw, 3) + k->repB * ipow(sw, 4) + k->repC);
+      double vdwb = invr6 - sg6 * (k->attA * ipow(sw, 3) + k->attB * ipow(sw, 4) + k->attC);
+      return 4.0 * eps * (vdwa - vdwb);
+    }
+  }
+}
+
+static double minimg2(const double *a, const double *b, const double *half) {
+  double d2 = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double d = b[i] - a[i];
+    d = half[i] - fabs(half[i] - fabs(d));
+    d2 = d2 + d * d;
+  }
+  return d2;
+}
+
+typedef struct {
+  int n;
+  double *eps, *sig2, *dist;
+  int *bel;
+} plist;
+
+typedef struct {
+  int bs[3];
+  double unit[3];
+  int npill, *pill_y, *pill_z, *pill_xlen;
+  /* sorted solute and solvent sites */
+  int nu, nv;
+  int *u_atom, *u_psum; /* psum: ncell+1 */
+  double *u_pos;
+  int *v_atom, *v_bel, *v_psum;
+  double *v_pos;
+  plist low, sw, el;
+  double *el_q;
+} cellgrid;
+
+/* set_block_info (realcal.F90:373-437) */
+static void cells_setup(orc_ctx *c, cellgrid *g) {
+  const orc_system *s = &c->s;
+  int bmax = 0;
+  for (int i = 0; i < 3; ++i) {
+    g->bs[i] = (int)ceil(c->L[i] / s->block_threshold);
+    g->unit[i] = c->L[i] / (double)g->bs[i];
+    if (g->bs[i] > bmax) bmax = g->bs[i];
+  }
+  double *gd = (double *)malloc(sizeof(double) * 3 * bmax);
+  double *bd = (double *)malloc(sizeof(double) * 3 * bmax);
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < g->bs[j]; ++i) {
+      int a = i, b = ((-i) % g->bs[j] + g->bs[j]) % g->bs[j];
+      gd[j * bmax + i] = (double)(a < b ? a : b) * g->unit[j];
+    }
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < g->bs[j]; ++i) {
+      int n = g->bs[j];
+      double v0 = gd[j * bmax + i], v1 = gd[j * bmax + (i - 1 + n) % n], v2 = gd[j * bmax + (i + 1) % n];
+      double m = v0 < v1 ? v0 : v1;
+      bd[j * bmax + i] = m < v2 ? m : v2;
+    }
+  double cut = s->upljcut > s->elecut ? s->upljcut : s->elecut;
+  double cut2 = ipow(cut, 2);
+  g->npill = 0;
+  g->pill_y = (int *)malloc(sizeof(int) * g->bs[1] * g->bs[2]);
+  g->pill_z = (int *)malloc(sizeof(int) * g->bs[1] * g->bs[2]);
+  g->pill_xlen = (int *)malloc(sizeof(int) * g->bs[1] * g->bs[2]);
+  for (int k = 0; k < g->bs[2]; ++k)
+    for (int j = 0; j < g->bs[1]; ++j) {
+      double l = ipow(bd[1 * bmax + j], 2) + ipow(bd[2 * bmax + k], 2);
+      if (l < cut2) {
+        g->pill_y[g->npill] = j;
+        g->pill_z[g->npill] = k;
+        g->pill_xlen[g->npill] = (int)ceil(sqrt(cut2 - l) / g->unit[0]);
+        g->npill++;
+      }
+    }
+  free(gd);
+  free(bd);
+}
+
+static int cell_of(const cellgrid *g, const double *x, const double *L) {
+  int b[3];
+  for (int i = 0; i < 3; ++i) {
+    int t = (int)floor(x[i] * (1.0 / L[i]) * (double)g->bs[i]);
+    t %= g->bs[i];
+    if (t < 0) t += g->bs[i];
+    b[i] = t;
+  }
+  return b[0] + g->bs[0] * (b[1] + g->bs[1] * b[2]);
+}
+
+/* blockify + sort_block (realcal.F90:439-525): stable counting sort */
+static void cells_sort(const cellgrid *g, int n, const double *pos, const double *L, int *order, int *psum) {
+  int nc = g->bs[0] * g->bs[1] * g->bs[2];
+  int *cell = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i <= nc; ++i) psum[i] = 0;
+  for (int i = 0; i < n; ++i) {
+    cell[i] = cell_of(g, pos + 3 * i, L);
+    psum[cell[i] + 1]++;
+  }
+  for (int i = 0; i < nc; ++i) psum[i + 1] += psum[i];
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
+  for (int i = 0; i < nc; ++i) fill[i] = psum[i];
+  for (int i = 0; i < n; ++i) order[fill[cell[i]]++] = i;
+  free(fill);
+  free(cell);
+}
+
+static void plist_alloc(plist *l, int cap) {
+  l->eps = (double *)malloc(sizeof(double) * cap);
+  l->sig2 = (double *)malloc(sizeof(double) * cap);
+  l->dist = (double *)malloc(sizeof(double) * cap);
+  l->bel = (int *)malloc(sizeof(int) * cap);
+  l->n = 0;
+}
+static void plist_free(plist *l) {
+  free(l->eps);
+  free(l->sig2);
+  free(l->dist);
+  free(l->bel);
+}
+
+/* get_pair_energy_block (realcal.F90:585-784) over solvent cells [vb, ve) */
+static void pair_block(orc_ctx *c, cellgrid *g, const ljconst *k, int upos, int vb, int ve, double *eng) {
+  const orc_system *s = &c->s;
+  double half[3] = {0.5 * c->L[0], 0.5 * c->L[1], 0.5 * c->L[2]};
+  g->low.n = g->sw.n = g->el.n = 0;
+  int nt = s->ntype;
+  for (int ui = g->u_psum[upos]; ui < g->u_psum[upos + 1]; ++ui) {
+    int ua = g->u_atom[ui];
+    int tu = c->ljtype[ua];
+    const double *cu = g->u_pos + 3 * ui;
+    for (int vi = g->v_psum[vb]; vi < g->v_psum[ve]; ++vi) {
+      int va = g->v_atom[vi];
+      int tv = c->ljtype[va];
+      double dist = minimg2(cu, g->v_pos + 3 * vi, half);
+      double eps = c->lj_eps[tv * nt + tu];
+      if (eps > 0) {
+        if (dist <= k->lw2) {
+          int n = g->low.n++;
+          g->low.eps[n] = eps;
+          g->low.sig2[n] = c->lj_sig2[tv * nt + tu];
+          g->low.dist[n] = dist;
+          g->low.bel[n] = g->v_bel[vi];
+        } else if (dist <= k->up2) {
+          int n = g->sw.n++;
+          g->sw.eps[n] = eps;
+          g->sw.sig2[n] = c->lj_sig2[tv * nt + tu];
+          g->sw.dist[n] = dist;
+          g->sw.bel[n] = g->v_bel[vi];
+        }
+      }
+      if (dist <= k->el2) {
+        int n = g->el.n++;
+        g->el_q[n] = c->charge[ua] * c->charge[va];
+        g->el.dist[n] = dist;
+        g->el.bel[n] = g->v_bel[vi];
+      }
+    }
+  }
+  for (int i = 0; i < g->low.n; ++i) eng[g->low.bel[i]] += lj_low(s, k, g->low.eps[i], g->low.sig2[i], g->low.dist[i]);
+  for (int i = 0; i < g->sw.n; ++i) eng[g->sw.bel[i]] += lj_switch(s, k, g->sw.eps[i], g->sw.sig2[i], g->sw.dist[i]);
+  for (int i = 0; i < g->el.n; ++i) {
+    double r = sqrt(g->el.dist[i]);
+    eng[g->el.bel[i]] += g->el_q[i] * (1.0 - erf(s->screen * r)) / r;
+  }
+}
+
+int32_t orc_realcal_proc(orc_ctx *c, int32_t solute, const double *solute_pos, double *ereal) {
+  int ns, st = solute_coords(c, solute, solute_pos, &ns);
+  if (st) return st;
+  const orc_system *s = &c->s;
+  cellgrid g;
+  memset(&g, 0, sizeof g);
+  cells_setup(c, &g);
+  int nc = g.bs[0] * g.bs[1] * g.bs[2];
+  /* solvent atoms in tagpt order, excluding the solute (set_solv_atoms) */
+  int nv = 0;
+  for (int k = 0; k < c->slvmax; ++k) {
+    int m = c->tagpt[k];
+    if (m == solute) continue;
+    nv += c->mol_begin[m + 1] - c->mol_begin[m];
+  }
+  int *vat = (int *)malloc(sizeof(int) * (nv ? nv : 1));
+  int *vbel = (int *)malloc(sizeof(int) * (nv ? nv : 1));
+  double *vpos = (double *)malloc(sizeof(double) * 3 * (nv ? nv : 1));
+  int n = 0;
+  for (int k = 0; k < c->slvmax; ++k) {
+    int m = c->tagpt[k];
+    if (m == solute) continue;
+    for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a) {
+      vat[n] = a;
+      vbel[n] = k;
+      memcpy(vpos + 3 * n, c->posn + 3 * a, sizeof(double) * 3);
+      n++;
+    }
+  }
+  int *vord = (int *)malloc(sizeof(int) * (nv ? nv : 1));
+  int *uord = (int *)malloc(sizeof(int) * ns);
+  g.v_psum = (int *)malloc(sizeof(int) * (nc + 1));
+  g.u_psum = (int *)malloc(sizeof(int) * (nc + 1));
+  cells_sort(&g, nv, vpos, c->L, vord, g.v_psum);
+  cells_sort(&g, ns, c->sposn, c->L, uord, g.u_psum);
+  g.v_atom = (int *)malloc(sizeof(int) * (nv ? nv : 1));
+  g.v_bel = (int *)malloc(sizeof(int) * (nv ? nv : 1));
+  g.v_pos = (double *)malloc(sizeof(double) * 3 * (nv ? nv : 1));
+  for (int i = 0; i < nv; ++i) {
+    g.v_atom[i] = vat[vord[i]];
+    g.v_bel[i] = vbel[vord[i]];
+    memcpy(g.v_pos + 3 * i, vpos + 3 * vord[i], sizeof(double) * 3);
+  }
+  g.u_atom = (int *)malloc(sizeof(int) * ns);
+  g.u_pos = (double *)malloc(sizeof(double) * 3 * ns);
+  int b = c->mol_begin[solute];
+  for (int i = 0; i < ns; ++i) {
+    g.u_atom[i] = b + uord[i];
+    memcpy(g.u_pos + 3 * i, c->sposn + 3 * uord[i], sizeof(double) * 3);
+  }
+  /* list capacity: max solute cell x max solvent pillar (realcal.F90:100-109) */
+  int maxu = 0, maxv = 0;
+  for (int i = 0; i < nc; ++i) {
+    int d = g.u_psum[i + 1] - g.u_psum[i];
+    if (d > maxu) maxu = d;
+  }
+  for (int z = 0; z < g.bs[2]; ++z)
+    for (int y = 0; y < g.bs[1]; ++y) {
+      int base = g.bs[0] * (y + g.bs[1] * z);
+      int d = g.v_psum[base + g.bs[0]] - g.v_psum[base];
+      if (d > maxv) maxv = d;
+    }
+  int cap = maxu * maxv;
+  if (cap < 1) cap = 1;
+  plist_alloc(&g.low, cap);
+  plist_alloc(&g.sw, cap);
+  plist_alloc(&g.el, cap);
+  g.el_q = (double *)malloc(sizeof(double) * cap);
+  ljconst k;
+  make_ljconst(s, &k);
+  for (int i = 0; i < c->slvmax; ++i) ereal[i] = 0.0;
+  /* get_pair_energy (realcal.F90:527-582) */
+  for (int u3 = 0; u3 < g.bs[2]; ++u3)
+    for (int u2 = 0; u2 < g.bs[1]; ++u2)
+      for (int u1 = 0; u1 < g.bs[0]; ++u1) {
+        int upos = u1 + g.bs[0] * (u2 + g.bs[1] * u3);
+        if (g.u_psum[upos + 1] == g.u_psum[upos]) continue;
+        for (int i = 0; i < g.npill; ++i) {
+          int v2 = (u2 + g.pill_y[i]) % g.bs[1];
+          int v3 = (u3 + g.pill_z[i]) % g.bs[2];
+          int vbase = g.bs[0] * (v2 + g.bs[1] * v3);
+          int xlen = g.pill_xlen[i];
+          if (2 * xlen + 1 >= g.bs[0]) {
+            pair_block(c, &g, &k, upos, vbase, vbase + g.bs[0], ereal);
+          } else {
+            int vbeg = vbase + u1 - xlen, vend = vbase + u1 + xlen + 1, vline = vbase + g.bs[0];
+            if (vbeg < vbase) {
+              pair_block(c, &g, &k, upos, vbeg + g.bs[0], vline, ereal);
+              pair_block(c, &g, &k, upos, vbase, vend, ereal);
+            } else if (vend > vline) {
+              pair_block(c, &g, &k, upos, vbeg, vline, ereal);
+              pair_block(c, &g, &k, upos, vbase, vend - g.bs[0], ereal);
+            } else {
+              pair_block(c, &g, &k, upos, vbeg, vend, ereal);
+            }
+          }
+        }
+      }
+  plist_free(&g.low);
+  plist_free(&g.sw);
+  plist_free(&g.el);
+  free(g.el_q);
+  free(vat);
+  free(vbel);
+  free(vpos);
+  free(vord);
+  free(uord);
+  free(g.v_psum);
+  free(g.u_psum);
+  free(g.v_atom);
+  free(g.v_bel);
+  free(g.v_pos);
+  free(g.u_atom);
+  free(g.u_pos);
+  free(g.pill_y);
+  free(g.pill_z);
+  free(g.pill_xlen);
+  return ORC_OK;
+}
+
+/* The same sum as orc_realcal_proc by an O(N_s N_v) minimum-image double
+ * loop with no cell list: an independent check of the cell traversal. */
+int32_t orc_realcal_brute(orc_ctx *c, int32_t solute, const double *solute_pos, double *ereal) {
+  int ns, st = solute_coords(c, solute, solute_pos, &ns);
+  if (st) return st;
+  const orc_system *s = &c->s;
+  ljconst k;
+  make_ljconst(s, &k);
+  double half[3] = {0.5 * c->L[0], 0.5 * c->L[1], 0.5 * c->L[2]};
+  int b = c->mol_begin[solute], nt = s->ntype;
+  for (int kk = 0; kk < c->slvmax; ++kk) {
+    int m = c->tagpt[kk];
+    double e = 0.0;
+    if (m != solute) {
+      for (int j = 0; j < ns; ++j)
+        for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a) {
+          double dist = minimg2(c->sposn + 3 * j, c->posn + 3 * a, half);
+          int tu = c->ljtype[b + j], tv = c->ljtype[a];
+          double eps = c->lj_eps[tv * nt + tu], sig2 = c->lj_sig2[tv * nt + tu];
+          if (eps > 0) {
+            if (dist <= k.lw2)
+              e += lj_low(s, &k, eps, sig2, dist);
+            else if (dist <= k.up2)
+              e += lj_switch(s, &k, eps, sig2, dist);
+          }
+          if (dist <= k.el2) {
+            double r = sqrt(dist);
+            e += c->charge[b + j] * c->charge[a] * (1.0 - erf(s->screen * r)) / r;
+          }
+        }
+    }
+    ereal[kk] = e;
+  }
+  return ORC_OK;
+}
+
+/* realcal_self (realcal.F90:284-328) */
+int32_t orc_realcal_self(orc_ctx *c, int32_t solute, const double *solute_pos, double *pairep) {
+  int ns, st = solute_coords(c, solute, solute_pos, &ns);
+  if (st) return st;
+  double half[3] = {0.5 * c->L[0], 0.5 * c->L[1], 0.5 * c->L[2]};
+  int b = c->mol_begin[solute];
+  double a = c->s.screen, e = 0.0;
+  for (int i = 0; i < ns; ++i) {
+    double qi = c->charge[b + i];
+    e = e + (-qi * qi * a / sqrt(PI_REF));
+    for (int j = i + 1; j < ns; ++j) {
+      double dist = minimg2(c->sposn + 3 * i, c->sposn + 3 * j, half);
+      double r = sqrt(dist);
+      e = e + (-qi * c->charge[b + j] * erf(a * r) / r);
+    }
+  }
+  *pairep = e;
+  return ORC_OK;
+}
+
+/* recpcal_prepare_solute (recpcal.F90:342-406) and recpcal_energy for
+ * every physical molecule (recpcal.F90:451-502) */
+int32_t orc_recpcal(orc_ctx *c, int32_t solute, const double *solute_pos, double *erecp, double *self_energy,
+                    double *phi) {
+  int ns, st = solute_coords(c, solute, solute_pos, &ns);
+  if (st) return st;
+  int p = c->p, K1 = c->K[0], K2 = c->K[1], K3 = c->K[2], H = c->H;
+  long nr = (long)K1 * K2 * K3;
+  memset(c->cnv, 0, sizeof(double) * nr);
+  int b = c->mol_begin[solute];
+  double w[3 * 32];
+  int32_t g[3];
+  for (int sid = 0; sid < ns; ++sid) {
+    orc_spline_site(c->spos + 3 * sid, c->invcl, c->K, p, c->coeff, w, g);
+    double chr = c->charge[b + sid];
+    for (int c3 = 0; c3 < p; ++c3)
+      for (int c2 = 0; c2 < p; ++c2)
+        for (int c1 = 0; c1 < p; ++c1) {
+          int r1 = ((g[0] - c1) % K1 + K1) % K1;
+          int r2 = ((g[1] - c2) % K2 + K2) % K2;
+          int r3 = ((g[2] - c3) % K3 + K3) % K3;
+          double f = chr * w[0 * p + c1] * w[1 * p + c2] * w[2 * p + c3];
+          c->cnv[((long)r3 * K2 + r2) * K1 + r1] += f;
+        }
+  }
+  orc_fft_r2c_3d(c->K, c->cnv, c->rcp);
+  /* self energy on the half spectrum with symmetry weights (385-394) */
+  double s_mid = 0.0, s_0 = 0.0, s_last = 0.0;
+  int last = H - 1;
+  int even = (K1 % 2 == 0);
+  for (int r3 = 0; r3 < K3; ++r3)
+    for (int r2 = 0; r2 < K2; ++r2)
+      for (int r1 = 1; r1 <= (even ? last - 1 : last); ++r1) {
+        long i = ((long)r3 * K2 + r2) * H + r1;
+        double re = c->rcp[2 * i], im = c->rcp[2 * i + 1];
+        s_mid = s_mid + c->engfac[i] * (re * re - im * (-im));
+      }
+  for (int r3 = 0; r3 < K3; ++r3)
+    for (int r2 = 0; r2 < K2; ++r2) {
+      long i = ((long)r3 * K2 + r2) * H;
+      double re = c->rcp[2 * i], im = c->rcp[2 * i + 1];
+      s_0 = s_0 + c->engfac[i] * (re * re - im * (-im));
+      if (even) {
+        long j = i + last;
+        re = c->rcp[2 * j];
+        im = c->rcp[2 * j + 1];
+        s_last = s_last + c->engfac[j] * (re * re - im * (-im));
+      }
+    }
+  *self_energy = even ? s_mid + 0.5 * s_0 + 0.5 * s_last : s_mid + 0.5 * s_0;
+  long nh = (long)H * K2 * K3;
+  for (long i = 0; i < nh; ++i) {
+    c->rcp[2 * i] = c->engfac[i] * c->rcp[2 * i];
+    c->rcp[2 * i + 1] = c->engfac[i] * c->rcp[2 * i + 1];
+  }
+  orc_fft_c2r_3d(c->K, c->rcp, c->cnv);
+  if (phi) memcpy(phi, c->cnv, sizeof(double) * nr);
+  /* gather */
+  for (int k = 0; k < c->slvmax; ++k) {
+    int m = c->tagpt[k];
+    double e = 0.0;
+    if (m != solute) {
+      int svi = c->slvtag[m];
+      if (svi < 0) return ORC_E_RCP_CNS;
+      int nsm = c->mol_begin[m + 1] - c->mol_begin[m];
+      for (int sid = 0; sid < nsm; ++sid) {
+        int slot = svi + sid;
+        const double *sw = c->splslv + (long)3 * p * slot;
+        const int32_t *gg = c->grdslv + 3 * slot;
+        double chr = c->charge[c->mol_begin[m] + sid];
+        for (int c3 = 0; c3 < p; ++c3) {
+          double f1 = chr * sw[2 * p + c3];
+          int r3 = ((gg[2] - c3) % K3 + K3) % K3;
+          for (int c2 = 0; c2 < p; ++c2) {
+            double f2 = f1 * sw[1 * p + c2];
+            int r2 = ((gg[1] - c2) % K2 + K2) % K2;
+            for (int c1 = 0; c1 < p; ++c1) {
+              double f3 = f2 * sw[0 * p + c1];
+              int r1 = ((gg[0] - c1) % K1 + K1) % K1;
+              e = e + f3 * c->cnv[((long)r3 * K2 + r2) * K1 + r1];
+            }
+          }
+        }
+      }
+    }
+    erecp[k] = e;
+  }
+  return ORC_OK;
+}
+
+/* residual_ene (engproc.F90:859-869) */
+static double residual(const orc_ctx *c, int i, int j) {
+  double epcl = PI_REF * c->mol_charge[i] * c->mol_charge[j] / c->s.screen / c->s.screen / c->volume;
+  if (i == j) epcl = epcl / 2.0;
+  return -epcl;
+}
+
+/* get_uv_energy (engproc.F90:662-756), PME branch */
+int32_t orc_uv_energy(orc_ctx *c, int32_t solute, const double *solute_pos, double *uvengy) {
+  int ns, st = solute_coords(c, solute, solute_pos, &ns);
+  if (st) return st;
+  int n = c->slvmax;
+  double *ereal = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double *erecp = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double self_recp;
+  st = orc_recpcal(c, solute, solute_pos, erecp, &self_recp, NULL);
+  if (!st) st = orc_realcal_proc(c, solute, solute_pos, ereal);
+  if (st) {
+    free(ereal);
+    free(erecp);
+    return st;
+  }
+  uvengy[0] = self_recp;
+  for (int k = 0; k < n; ++k) uvengy[k + 1] = ereal[k];
+  /* realcal_self is reused while the solute hash is unchanged, and always
+   * after the first call for a rigid test particle (engproc.F90:712-730) */
+  double pairep;
+  uint64_t h = orc_hash_double(c->spos, 3 * ns);
+  if (h == c->solute_hash || (c->s.slttype == 2 && c->solute_hash != 0)) {
+    pairep = c->usreal;
+  } else {
+    orc_realcal_self(c, solute, solute_pos, &pairep);
+    c->usreal = pairep;
+  }
+  c->solute_hash = h;
+  double res = residual(c, solute, solute);
+  uvengy[0] = uvengy[0] + pairep + res;
+  for (int k = 0; k < n; ++k) {
+    int m = c->tagpt[k];
+    if (m == solute) continue;
+    double pe = 0.0 + residual(c, solute, m);
+    pe = pe + erecp[k];
+    uvengy[k + 1] = uvengy[k + 1] + pe;
+  }
+  free(ereal);
+  free(erecp);
+  return ORC_OK;
+}
+
+/* getiduv (engproc.F90:960-1020); returns status, writes the 0-based bin */
+static int getiduv(orc_ctx *c, int pti, double v, int *iduv) {
+  int idmin = 0, idnum, idpti;
+  if (pti > 0) {
+    for (int i = 0; i < pti - 1; ++i) idmin += c->uvmax[i];
+    idnum = c->uvmax[pti - 1];
+    idpti = orc_binsearch(c->uvcrd + idmin, idnum, v);
+  } else if (pti == 0) {
+    idnum = c->s.esmax;
+    idpti = orc_binsearch(c->escrd, idnum, v);
+  } else {
+    return ORC_E_ENG_BUG;
+  }
+  if (idpti <= 0) return ORC_E_ENG_MIN;
+  if (idpti > idnum) return ORC_E_ENG_BUG;
+  if (c->s.slttype == 1 && pti > 0 && idpti > c->uvsoft[pti - 1]) return ORC_E_ENG_SFT;
+  if (idpti == idnum && v < 1.0e3 && c->warn_bin_first) {
+    c->warn_bin_first = 0;
+    if (!c->s.force_calculation) return ORC_E_MBIN;
+  }
+  *iduv = idmin + idpti - 1;
+  return ORC_OK;
+}
+
+/* volcorrect (engproc.F90:871-887) */
+static double volcorrect(const orc_ctx *c, double weight) {
+  weight = weight * c->volume;
+  double tq = 0.0;
+  for (int i = 0; i < c->s.nmol; ++i)
+    if (c->sluvid[i] == 0 || c->sluvid[i] == 1) tq = tq + c->mol_charge[i];
+  double factor = PI_REF * ipow(tq, 2) / ipow(c->s.screen, 2) / c->volume / 2.0;
+  return weight * exp(factor / c->s.temp);
+}
+
+/* update_histogram (engproc.F90:758-856) */
+int32_t orc_update_histogram(orc_ctx *c, int32_t solute, double stat_weight_solute, const double *uvengy,
+                             int32_t *insdst) {
+  const orc_system *s = &c->s;
+  double engnmfc;
+  if (s->wgtslf == 0) {
+    engnmfc = 1.0;
+  } else {
+    if (!c->voffset_init) {
+      c->voffset = uvengy[0];
+      c->voffset_init = 1;
+    }
+    double f = uvengy[0] - c->voffset;
+    engnmfc = (s->slttype == 1) ? exp(f / s->temp) : exp(-f / s->temp);
+  }
+  double tw = c->stat_weight_system * stat_weight_solute;
+  if (s->estype == 2) tw = volcorrect(c, tw);
+  engnmfc = engnmfc * tw;
+  c->engnorm = c->engnorm + engnmfc;
+  c->engsmpl = c->engsmpl + 1.0;
+  c->avslf = c->avslf + tw;
+  int iduv;
+  if (s->selfcal) {
+    int st = getiduv(c, 0, uvengy[0], &iduv);
+    if (st) return st;
+    c->eself[iduv] = c->eself[iduv] + engnmfc;
+  }
+  if (uvengy[0] < c->minuv[0]) c->minuv[0] = uvengy[0];
+  if (uvengy[0] > c->maxuv[0]) c->maxuv[0] = uvengy[0];
+  memset(c->insdst, 0, sizeof(int32_t) * c->ermax);
+  for (int i = 0; i <= s->numslv; ++i) c->flceng[i] = 0.0;
+  for (int k = 0; k < c->slvmax; ++k) {
+    int m = c->tagpt[k];
+    if (m == solute) continue;
+    int pti = c->uvspec[m];
+    if (pti <= 0) return ORC_E_ENG_CNS;
+    double e = uvengy[k + 1];
+    int st = getiduv(c, pti, e, &iduv);
+    if (st) return st;
+    c->insdst[iduv] += 1;
+    c->flceng[pti] = c->flceng[pti] + e;
+    if (e < c->minuv[pti]) c->minuv[pti] = e;
+    if (e > c->maxuv[pti]) c->maxuv[pti] = e;
+  }
+  if (s->slttype == 1)
+    for (int i = 1; i <= s->numslv; ++i) c->slnuv[i] = c->slnuv[i] + c->flceng[i] * engnmfc;
+  for (int i = 0; i < c->ermax; ++i) {
+    int k = c->insdst[i];
+    if (k == 0) continue;
+    c->edens[i] = c->edens[i] + engnmfc * (double)k;
+  }
+  if (s->corrcal) {
+    for (int i = 0; i < c->ermax; ++i) {
+      int k = c->insdst[i];
+      if (k == 0) continue;
+      for (int j = 0; j < c->ermax; ++j) {
+        int q = c->insdst[j];
+        if (q == 0) continue;
+        /* ecorr(iduvp, iduv), column-major: element (j, i) at j + i*ermax */
+        c->ecorr[(long)i * c->ermax + j] = c->ecorr[(long)i * c->ermax + j] + engnmfc * (double)k * (double)q;
+      }
+    }
+  }
+  if (insdst) memcpy(insdst, c->insdst, sizeof(int32_t) * c->ermax);
+  return ORC_OK;
+}
+
+/* engstore normalisation for one process (engproc.F90:539-553) */
+int32_t orc_engstore(orc_ctx *c) {
+  for (int i = 0; i < c->ermax; ++i) c->edens[i] = c->edens[i] / c->engnorm;
+  if (c->ecorr)
+    for (long i = 0; i < (long)c->ermax * c->ermax; ++i) c->ecorr[i] = c->ecorr[i] / c->engnorm;
+  if (c->eself)
+    for (int i = 0; i < c->s.esmax; ++i) c->eself[i] = c->eself[i] / c->engnorm;
+  c->avslf = c->avslf / c->engnorm;
+  return ORC_OK;
+}
+
+void orc_get_acc(orc_ctx *c, orc_acc *o) {
+  o->edens = c->edens;
+  o->ecorr = c->ecorr;
+  o->eself = c->eself;
+  o->minuv = c->minuv;
+  o->maxuv = c->maxuv;
+  o->slnuv = c->slnuv;
+  o->engnorm = c->engnorm;
+  o->engsmpl = c->engsmpl;
+  o->avslf = c->avslf;
+  o->voffset = c->voffset;
+  o->ermax = c->ermax;
+  o->esmax = c->s.esmax;
+  o->numslv = c->s.numslv;
+  o->voffset_initialized = c->voffset_init;
+}
