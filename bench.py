@@ -1,0 +1,209 @@
+"""Benchmark: solute-solvent mol-pair energies/s and insertions/s of the
+reference-solvent insertion workload on one MI355X.
+
+Workload (``refs20``): 4000 rigid 3-site waters in a 49.3 A periodic box;
+per frame, 1000 insertions of a rigid 20-site test particle at uniform random
+positions and orientations (device-generated, counter-based); real-space
+cutoff 12 A, Ewald alpha 0.35, PME 48^3 with order-4 B-splines; 600-bin
+energy coordinate; per-insertion histograms accumulated into the 600 x 600
+correlation matrix on FP64 MFMA.  A step is one frame (1000 insertions x
+4000 molecules = 4e6 mol-pair energies).  Frames are synthetic and uploaded
+to HBM before the timed region.
+
+Prints one JSON line (rank 0).  ``roofline`` refers to the dominant kernel,
+timed with HIP events on the library's stream over the timed region;
+``cpu_baseline`` is the oracle (the FP64 CPU restatement) on a bounded sample
+of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8000.0          # GB/s, MI355X
+FP64_PEAK = 78.6           # TFLOP/s dense FP64 (vector and MFMA)
+FP32_PEAK = 157.3          # TFLOP/s dense FP32
+
+
+def stage_bytes(sy, prec_bytes, n_ins, slvmax, ermax):
+    """Algorithmic HBM bytes per launch (n_ins configurations) of each
+    device stage; the figures are derived in DESIGN.md."""
+    K1, K2, K3 = sy.system.ms
+    H = K1 // 2 + 1
+    T = prec_bytes
+    nsites = 3 * slvmax
+    p = sy.system.splodr
+    return {
+        # Hk written once (complex T)
+        "pme_zcol": n_ins * (2 * T * H * K2 * K3),
+        # Hk read once, phi written once
+        "pme_plane": n_ins * (2 * T * H * K2 * K3 + T * K1 * K2 * K3),
+        # phi read once, energies written once; solvent splines once per launch
+        "gather": n_ins * (T * K1 * K2 * K3 + 8 * slvmax) + nsites * (3 * p * T + 4 + T),
+        # energies read, histogram written
+        "bin": n_ins * (8 * slvmax + 8 * ermax),
+    }
+
+
+def pair_count(sy, pos, P, rc=12.0):
+    """Exact solute-site x solvent-site pairs within the cutoff for the
+    configurations P (numpy, minimum image)."""
+    s = sy.system
+    L = sy.L
+    phys = s.physical()
+    idx = np.concatenate([np.arange(s.mol_begin[m], s.mol_begin[m + 1]) for m in phys])
+    v = pos[idx]
+    tot = 0
+    for sp in P:
+        d = sp[:, None, :] - v[None, :, :]
+        d -= L * np.round(d / L)
+        tot += int(np.count_nonzero((d * d).sum(-1) <= rc * rc))
+    return tot / len(P)
+
+
+def cpu_baseline(sy, frame0, n_ins_max, seconds, seed):
+    import oracle as O
+    s = sy.system
+    o = O.Oracle(s)
+    o.set_frame(frame0, sy.cell)
+    m = sy.solute_mol
+    t0 = time.perf_counter()
+    done = 0
+    while done < n_ins_max:
+        r, q = O.random_pose(seed, done)
+        sp = O.place_solute(s.bfcoord, s.solute_mass(m), sy.cell, r, q)
+        u = o.uv_energy(m, sp)
+        o.update_histogram(m, u)
+        done += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    slv = o.slvmax
+    return {"value": done * slv / dt, "unit": "mol-pair energies/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (FP64 C restatement, 1 thread) on frame 0 of the same workload: "
+                      f"{done} insertions with update_histogram in {dt:.1f} s",
+            "insertions_per_s": done / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--insertions", type=int, default=1000)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261203)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs (frame sharding with "
+                                   "an all-reduce of the accumulators) are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from ermod_amd import synth
+    from ermod_amd.api import ErmGPU
+
+    sy = synth.refs20()
+    s = sy.system
+    nf = args.warmup + args.steps
+    nfu = min(nf, 8)   # distinct synthetic frames, cycled
+    t_gen = time.perf_counter()
+    frames = [sy.frame(i) for i in range(nfu)]
+    gen_s = time.perf_counter() - t_gen
+    dev = torch.device("cuda:0")
+    dframes = [torch.from_numpy(f.reshape(-1)).to(dev) for f in frames]
+    torch.cuda.synchronize()
+
+    def run(prec, steps, warmup):
+        g = ErmGPU(s, prec, max_batch=args.batch)
+        for i in range(warmup):
+            g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
+            g.insert_random(args.insertions, args.seed, first_index=i * args.insertions)
+        g.engclear()
+        torch.cuda.synchronize()
+        st = {k: 0.0 for k in ("prep", "pme_zcol", "pme_plane", "gather", "pair", "bin", "corr", "total")}
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
+            g.insert_random(args.insertions, args.seed, first_index=i * args.insertions)
+            for k, v in g.stage_times().items():
+                st[k] += v
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        acc = g.fetch()
+        bmax = args.batch if args.batch > 0 else 256
+        slv, erm = g.slvmax, g.ermax
+        g.close()
+        return dt, st, acc, bmax, slv, erm
+
+    dt, st, acc, bmax, slvmax, ermax = run(args.precision, args.steps, args.warmup)
+    n_ins = args.steps * args.insertions
+    value = n_ins * slvmax / dt
+    launches = args.steps * int(np.ceil(args.insertions / bmax))
+    T = 8 if args.precision == "fp64" else 4
+    sb = stage_bytes(sy, T, bmax, slvmax, ermax)
+    # dominant stage over the timed region
+    stages = {k: v for k, v in st.items() if k not in ("total",)}
+    dom = max(stages, key=stages.get)
+    avg_ms = stages[dom] / launches
+    if dom in sb:
+        ach = sb[dom] / (avg_ms * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None, "avg_launch_ms": avg_ms,
+                "bytes_per_launch": sb[dom], "configs_per_launch": bmax}
+    else:
+        roof = {"kernel": dom, "avg_launch_ms": avg_ms}
+    if dom == "pair":
+        import oracle as O
+        Ps = []
+        for i in range(4):
+            r, q = O.random_pose(args.seed, i)
+            Ps.append(O.place_solute(s.bfcoord, s.solute_mass(sy.solute_mol), sy.cell, r, q))
+        npair = pair_count(sy, frames[0], Ps, 12.0)
+        flops = 40.0 * npair * bmax
+        peak = FP64_PEAK if args.precision == "fp64" else FP32_PEAK
+        ach = flops / (avg_ms * 1e-3) / 1e12
+        roof = {"kernel": "pair", "bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": None, "avg_launch_ms": avg_ms,
+                "flops_per_launch": flops, "pairs_per_insertion": npair}
+    stage_share = {k: round(v / max(st["total"], 1e-9), 4) for k, v in stages.items()}
+
+    out = {"metric": "solute-solvent mol-pair energies/s", "value": value, "unit": "mol-pair energies/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if args.precision == "fp64" else "f32 pair/PME, f64 sums",
+           "data": "synthetic",
+           "config": {"workload": "refs20: 4000 waters in 49.3 A, rigid 20-site test particle, "
+                                  f"{args.insertions} random insertions per frame, cutoff 12 A, "
+                                  "alpha 0.35, PME 48^3 p4, 600 bins, correlation on FP64 MFMA",
+                      "frames": args.steps, "insertions_per_frame": args.insertions,
+                      "batch": bmax, "precision": args.precision},
+           "insertions_per_s": n_ins / dt,
+           "roofline": roof, "stage_ms": {k: round(v, 3) for k, v in st.items()},
+           "stage_share": stage_share, "engsmpl": acc["engsmpl"]}
+    if not args.no_fp32 and args.precision == "fp64":
+        dt32, st32, _, _, _, _ = run("fp32", max(1, args.steps // 2), 1)
+        n32 = max(1, args.steps // 2) * args.insertions
+        out["fp32"] = {"value": n32 * slvmax / dt32, "insertions_per_s": n32 / dt32,
+                       "stage_ms": {k: round(v, 3) for k, v in st32.items()}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sy, frames[0], 100000, args.cpu_seconds, args.seed)
+        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,305 @@
+"""ctypes front end of libermgpu.so (include/ermgpu.h).
+
+``ErmGPU`` mirrors the reference's engproc interface (engproc.F90): engclear,
+set_frame (the per-frame part of engconst), get_uv_energy, engconst's hot
+loop, update_histogram and engstore, plus realcal_proc / realcal_self /
+recpcal for stage-wise checks.  It raises ``ErmError`` carrying the library's
+status code; nothing here computes energies on the CPU -- a missing or
+unloadable library is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .system import ErmSystem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libermgpu.so")
+
+ERM_FP64, ERM_FP32 = 0, 1
+STATUS = {0: "ok", 1: "eng_min", 2: "eng_sft", 3: "mbin", 4: "eng_ecd", 5: "rcp_cns", 6: "noncuboid",
+          7: "eng_bug", 8: "eng_cns", 9: "eng_siz", 10: "arg", 11: "unsupported", 12: "device"}
+STAGES = ("prep", "pme_zcol", "pme_plane", "gather", "pair", "bin", "corr", "total")
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class ErmParams(C.Structure):
+    _fields_ = [("slttype", C.c_int32), ("estype", C.c_int32), ("ljswitch", C.c_int32),
+                ("splodr", C.c_int32), ("ms", C.c_int32 * 3), ("corrcal", C.c_int32),
+                ("selfcal", C.c_int32), ("wgtslf", C.c_int32), ("force_calculation", C.c_int32),
+                ("elecut", C.c_double), ("lwljcut", C.c_double), ("upljcut", C.c_double),
+                ("screen", C.c_double), ("temp", C.c_double), ("block_threshold", C.c_double),
+                ("precision", C.c_int32), ("max_batch", C.c_int32), ("device", C.c_int32)]
+
+
+class ErmHist(C.Structure):
+    _fields_ = [("ecdmin", C.c_double), ("ecfmns", C.c_double), ("ecdcen", C.c_double),
+                ("eccore", C.c_double), ("ecdmax", C.c_double), ("eclbin", C.c_double),
+                ("ecfbin", C.c_double), ("ec0bin", C.c_double), ("finfac", C.c_double),
+                ("pecore", C.c_int32)]
+
+
+class ErmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "erm_last_error",
+           "erm_set_topology", "erm_set_bins", "erm_engclear", "erm_set_frame", "erm_set_frame_device",
+           "erm_get_uv_energy", "erm_realcal_proc", "erm_realcal_self", "erm_recpcal", "erm_engconst",
+           "erm_insert_random", "erm_update_histogram", "erm_engstore", "erm_fetch", "erm_sizes",
+           "erm_stage_times")
+
+_lib = None
+
+
+def lib():
+    """Loads libermgpu.so; raises if it is not built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is not built: run `python -m ermod_amd.build`")
+        L = C.CDLL(LIBPATH)
+        vp = C.c_void_p
+        L.erm_build_edges.argtypes = [C.POINTER(ErmHist), C.c_int32, _f64p, C.c_int32, _i32p, _i32p]
+        L.erm_hash_double.argtypes = [_f64p, C.c_int32]
+        L.erm_hash_double.restype = C.c_uint64
+        L.erm_create.argtypes = [C.POINTER(ErmParams), C.POINTER(vp)]
+        L.erm_destroy.argtypes = [vp]
+        L.erm_destroy.restype = None
+        L.erm_last_error.argtypes = [vp]
+        L.erm_last_error.restype = C.c_char_p
+        L.erm_set_topology.argtypes = [vp, C.c_int32, C.c_int32, _i32p, _f64p, _i32p, C.c_int32, _f64p,
+                                       _f64p, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.erm_set_bins.argtypes = [vp, C.c_int32, _i32p, _i32p, _f64p, C.c_int32, _f64p]
+        L.erm_engclear.argtypes = [vp]
+        L.erm_set_frame.argtypes = [vp, _f64p, _f64p, C.c_double]
+        L.erm_set_frame_device.argtypes = [vp, C.c_void_p, _f64p, C.c_double]
+        L.erm_get_uv_energy.argtypes = [vp, C.c_int32, C.c_int32, _f64p, _f64p]
+        L.erm_realcal_proc.argtypes = [vp, C.c_int32, _f64p, _f64p]
+        L.erm_realcal_self.argtypes = [vp, C.c_int32, _f64p, _f64p]
+        L.erm_recpcal.argtypes = [vp, C.c_int32, _f64p, _f64p, _f64p]
+        L.erm_engconst.argtypes = [vp, C.c_int32, _i32p, _f64p, _f64p, _f64p]
+        L.erm_insert_random.argtypes = [vp, C.c_int32, C.c_uint64, C.c_uint64, _f64p]
+        L.erm_update_histogram.argtypes = [vp, C.c_int32, _i32p, _f64p, _f64p]
+        L.erm_engstore.argtypes = [vp]
+        L.erm_fetch.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.erm_sizes.argtypes = [vp, _i32p, _i32p]
+        L.erm_stage_times.argtypes = [vp, _f64p]
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a, t=_f64p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def hist_struct(h) -> ErmHist:
+    return ErmHist(h.ecdmin, h.ecfmns, h.ecdcen, h.eccore, h.ecdmax, h.eclbin, h.ecfbin, h.ec0bin,
+                   h.finfac, int(h.pecore))
+
+
+def build_edges(h, self_energy=False):
+    """enginit's bin edges of one species (engproc.F90:124-237)."""
+    out = np.zeros(10002)
+    pemax, pesoft = C.c_int32(), C.c_int32()
+    hs = hist_struct(h)
+    st = lib().erm_build_edges(C.byref(hs), int(self_energy), _p(out), len(out), C.byref(pemax),
+                               C.byref(pesoft))
+    if st:
+        raise ErmError(st, "erm_build_edges")
+    return out[:pemax.value].copy(), pemax.value, pesoft.value
+
+
+def hash_double(v):
+    a = _f64(v).ravel()
+    return lib().erm_hash_double(_p(a), len(a))
+
+
+class ErmGPU:
+    """One device context over an ErmSystem."""
+
+    def __init__(self, system: ErmSystem, precision: str = "fp64", max_batch: int = 0,
+                 device: int = 0):
+        s = system
+        self.system = s
+        self.precision = precision
+        P = ErmParams()
+        P.slttype, P.estype, P.ljswitch, P.splodr = s.slttype, s.estype, s.ljswitch, s.splodr
+        P.ms[:] = list(s.ms)
+        P.corrcal, P.selfcal, P.wgtslf = s.corrcal, s.selfcal, s.wgtslf
+        P.force_calculation = s.force_calculation
+        P.elecut, P.lwljcut, P.upljcut = s.elecut, s.lwljcut, s.upljcut
+        P.screen, P.temp, P.block_threshold = s.screen, s.temp, s.block_threshold
+        P.precision = ERM_FP32 if precision == "fp32" else ERM_FP64
+        P.max_batch = max_batch
+        P.device = device
+        h = C.c_void_p()
+        st = lib().erm_create(C.byref(P), C.byref(h))
+        if st:
+            raise ErmError(st, "erm_create")
+        self.h = h
+        self._keep = dict(mol_begin=_i32(s.mol_begin), charge=_f64(s.charge), ljtype=_i32(s.ljtype),
+                          lj_eps=_f64(s.lj_eps), lj_sig2=_f64(s.lj_sig2), sluvid=_i32(s.sluvid),
+                          uvspec=_i32(s.uvspec), mol_charge=_f64(s.mol_charge),
+                          sitemass=_f64(s.sitemass),
+                          bf=None if s.bfcoord is None else _f64(s.bfcoord))
+        k = self._keep
+        self._check(lib().erm_set_topology(h, s.natom, s.nmol, _p(k["mol_begin"], _i32p), _p(k["charge"]),
+                                           _p(k["ljtype"], _i32p), s.ntype, _p(k["lj_eps"]),
+                                           _p(k["lj_sig2"]), _p(k["sluvid"], _i32p),
+                                           _p(k["uvspec"], _i32p), _p(k["mol_charge"]),
+                                           _p(k["sitemass"]), _p(k["bf"])), "set_topology")
+        edges, uvmax, uvsoft = [], [], []
+        for hh in s.hists:
+            e, pemax, pesoft = build_edges(hh)
+            edges.append(e)
+            uvmax.append(pemax)
+            uvsoft.append(pesoft)
+        self.uvcrd = _f64(np.concatenate(edges))
+        self.uvmax = _i32(uvmax)
+        self.uvsoft = _i32(uvsoft)
+        self.escrd, self.esmax = None, 0
+        if s.selfcal:
+            e, self.esmax, _ = build_edges(s.self_hist, self_energy=True)
+            self.escrd = _f64(e)
+        self._check(lib().erm_set_bins(h, s.numslv, _p(self.uvmax, _i32p), _p(self.uvsoft, _i32p),
+                                       _p(self.uvcrd), self.esmax, _p(self.escrd)), "set_bins")
+        sv, em = C.c_int32(), C.c_int32()
+        lib().erm_sizes(h, C.byref(sv), C.byref(em))
+        self.slvmax, self.ermax = sv.value, em.value
+        self.test_mol = s.nmol - 1
+
+    # ------------------------------------------------------------------
+    def _check(self, st, where):
+        if st:
+            msg = lib().erm_last_error(self.h)
+            raise ErmError(st, f"{where}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().erm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------
+    def engclear(self):
+        self._check(lib().erm_engclear(self.h), "engclear")
+
+    def set_frame(self, sitepos, cell, weight=1.0):
+        pos = _f64(sitepos)
+        cm = _f64(np.asarray(cell).T.copy())
+        self._check(lib().erm_set_frame(self.h, _p(pos), _p(cm), float(weight)), "set_frame")
+
+    def set_frame_device(self, dev_ptr: int, cell, weight=1.0):
+        cm = _f64(np.asarray(cell).T.copy())
+        self._check(lib().erm_set_frame_device(self.h, C.c_void_p(dev_ptr), _p(cm), float(weight)),
+                    "set_frame_device")
+
+    def get_uv_energy(self, solute, solute_pos=None, n=None):
+        if solute_pos is not None:
+            sp = _f64(solute_pos)
+            n = sp.size // (3 * self.system.nsite(solute)) if n is None else n
+        else:
+            sp, n = None, (1 if n is None else n)
+        out = np.zeros((n, self.slvmax + 1))
+        self._check(lib().erm_get_uv_energy(self.h, int(solute), n, _p(sp), _p(out)), "get_uv_energy")
+        return out
+
+    def realcal_proc(self, solute, solute_pos=None):
+        out = np.zeros(self.slvmax)
+        sp = None if solute_pos is None else _f64(solute_pos)
+        self._check(lib().erm_realcal_proc(self.h, int(solute), _p(sp), _p(out)), "realcal_proc")
+        return out
+
+    def realcal_self(self, solute, solute_pos=None):
+        v = C.c_double()
+        sp = None if solute_pos is None else _f64(solute_pos)
+        self._check(lib().erm_realcal_self(self.h, int(solute), _p(sp), C.byref(v)), "realcal_self")
+        return v.value
+
+    def recpcal(self, solute, solute_pos=None):
+        out = np.zeros(self.slvmax)
+        v = C.c_double()
+        sp = None if solute_pos is None else _f64(solute_pos)
+        self._check(lib().erm_recpcal(self.h, int(solute), _p(sp), _p(out), C.byref(v)), "recpcal")
+        return out, v.value
+
+    def engconst(self, solutes=None, solute_pos=None, weights=None, want_uv=False):
+        """Hot loop of engconst: energies + histogram for n configurations."""
+        if solute_pos is not None:
+            sp = _f64(solute_pos)
+            n = sp.size // (3 * self.system.nsite(self.test_mol))
+        else:
+            sp = None
+        sl = None if solutes is None else _i32(solutes)
+        if sl is not None:
+            n = len(sl)
+        w = None if weights is None else _f64(weights)
+        out = np.zeros((n, self.slvmax + 1)) if want_uv else None
+        self._check(lib().erm_engconst(self.h, n, _p(sl, _i32p), _p(sp), _p(w), _p(out)), "engconst")
+        return out
+
+    def insert_random(self, n, seed, first_index=0, want_uv=False):
+        out = np.zeros((n, self.slvmax + 1)) if want_uv else None
+        self._check(lib().erm_insert_random(self.h, int(n), C.c_uint64(seed), C.c_uint64(first_index),
+                                            _p(out)), "insert_random")
+        return out
+
+    def update_histogram(self, uvengy, solutes=None, weights=None):
+        u = _f64(uvengy)
+        n = u.shape[0]
+        sl = None if solutes is None else _i32(solutes)
+        w = None if weights is None else _f64(weights)
+        self._check(lib().erm_update_histogram(self.h, n, _p(sl, _i32p), _p(w), _p(u)),
+                    "update_histogram")
+
+    def engstore(self):
+        self._check(lib().erm_engstore(self.h), "engstore")
+
+    def fetch(self):
+        E, ns = self.ermax, self.system.numslv
+        edens = np.zeros(E)
+        ecorr = np.zeros((E, E)) if self.system.corrcal else None
+        eself = np.zeros(max(self.esmax, 1))
+        mn, mx, sl = np.zeros(ns + 1), np.zeros(ns + 1), np.zeros(ns + 1)
+        sc = np.zeros(4)
+        self._check(lib().erm_fetch(self.h, _p(edens), _p(ecorr), _p(eself), _p(mn), _p(mx), _p(sl),
+                                    _p(sc)), "fetch")
+        out = dict(edens=edens, minuv=mn, maxuv=mx, slnuv=sl, engnorm=sc[0], engsmpl=sc[1],
+                   avslf=sc[2], voffset=sc[3])
+        if ecorr is not None:
+            out["ecorr"] = ecorr  # [iduv][iduvp]: the column-major ecorr(iduvp, iduv)
+        if self.esmax:
+            out["eself"] = eself[:self.esmax]
+        return out
+
+    def stage_times(self):
+        out = np.zeros(8)
+        lib().erm_stage_times(self.h, _p(out))
+        return dict(zip(STAGES, out))

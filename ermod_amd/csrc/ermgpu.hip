@@ -1,0 +1,1284 @@
+// ermgpu.hip -- host side of libermgpu.so: contexts, device buffers, the
+// per-frame and per-batch launch sequences and the C-ABI of ermgpu.h.
+#include "ermgpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ermgpu_kern.h"
+
+using namespace erm;
+
+namespace {
+
+const double PI_REF = 3.1415926535897932;
+
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// ---------------------------------------------------------------- host math
+// spline.F90:26-59 (coefficients) -- used for the B-spline moduli
+void spline_coeff(int order, double *c) {
+  for (int i = 0; i <= order; ++i) {
+    double f = 1.0;
+    for (int k = 1; k <= i; ++k) f = f * (double)k;
+    for (int k = 1; k <= order - i; ++k) f = f * (double)k;
+    f = (double)order / f;
+    if (i % 2 == 1) f = -f;
+    c[i] = f;
+  }
+}
+double ipow_h(double x, int n) {
+  double r = 1.0, b = x;
+  bool first = true;
+  while (n > 0) {
+    if (n & 1) {
+      r = first ? b : r * b;
+      first = false;
+    }
+    n >>= 1;
+    if (n) b = b * b;
+  }
+  return first ? 1.0 : r;
+}
+double spline_value_h(const double *c, int order, double x) {
+  double f = 0.0;
+  if (x > 0.0 && x < (double)order) {
+    int k = (int)x;
+    for (int i = 0; i <= k; ++i) f = f + c[i] * ipow_h(x - (double)i, order - 1);
+  }
+  return f;
+}
+
+// init_spline_axis (recpcal.F90:87-109)
+void spline_axis(int K, int p, const double *coeff, double *splfc) {
+  for (int rci = 0; rci < K; ++rci) {
+    double re = 0.0, im = 0.0;
+    for (int spi = 0; spi <= p - 2; ++spi) {
+      double chr = spline_value_h(coeff, p, (double)(spi + 1));
+      double rtp2 = 2.0 * PI_REF * (double)(spi * rci) / (double)K;
+      re = re + chr * cos(rtp2);
+      im = im + chr * sin(rtp2);
+    }
+    splfc[rci] = re * re - im * (-im);
+  }
+}
+
+// recpcal_spline_greenfunc (recpcal.F90:111-152); engfac[m3][m2][m1]
+void greenfunc(const int *K, int p, const double *invcl, double volume, double screen,
+               const std::vector<double> *splfc, std::vector<double> &engfac) {
+  const int K1 = K[0], K2 = K[1], K3 = K[2], H = K1 / 2 + 1;
+  engfac.assign((size_t)H * K2 * K3, 0.0);
+  for (int rc3 = 0; rc3 < K3; ++rc3)
+    for (int rc2 = 0; rc2 < K2; ++rc2)
+      for (int rc1 = 0; rc1 < H; ++rc1) {
+        if (rc1 == 0 && rc2 == 0 && rc3 == 0) continue;
+        const int rci[3] = {rc1, rc2, rc3};
+        double inm[3];
+        bool skip = false;
+        for (int m = 0; m < 3; ++m) {
+          if ((p % 2 == 1) && (2 * std::abs(rci[m]) == K[m])) {
+            skip = true;
+            break;
+          }
+          inm[m] = (rci[m] <= K[m] / 2) ? (double)rci[m] : (double)(rci[m] - K[m]);
+        }
+        double factor = 0.0;
+        if (!skip) {
+          double xst[3];
+          for (int m = 0; m < 3; ++m)
+            xst[m] = invcl[3 * m + 0] * inm[0] + invcl[3 * m + 1] * inm[1] + invcl[3 * m + 2] * inm[2];
+          double rtp2 = xst[0] * xst[0] + xst[1] * xst[1] + xst[2] * xst[2];
+          double chr = (PI_REF * PI_REF) * rtp2 / (screen * screen);
+          factor = exp(-chr) / rtp2 / PI_REF / volume;
+          rtp2 = splfc[0][rc1] * splfc[1][rc2] * splfc[2][rc3];
+          factor = factor / rtp2;
+        }
+        engfac[((size_t)rc3 * K2 + rc2) * H + rc1] = factor;
+      }
+  engfac[0] = 0.0;
+}
+
+// get_inverted_cell (engproc.F90:898-919), column-major
+void inverted_cell(const double *c, double *inv, double *vol) {
+#define M(i, j) c[3 * ((j)-1) + (i)-1]
+#define I(i, j) inv[3 * ((j)-1) + (i)-1]
+  double v = M(1, 1) * M(2, 2) * M(3, 3) + M(1, 2) * M(2, 3) * M(3, 1) + M(1, 3) * M(2, 1) * M(3, 2) -
+             M(1, 3) * M(2, 2) * M(3, 1) - M(1, 2) * M(2, 1) * M(3, 3) - M(1, 1) * M(2, 3) * M(3, 2);
+  I(1, 1) = M(2, 2) * M(3, 3) - M(2, 3) * M(3, 2);
+  I(1, 2) = M(1, 3) * M(3, 2) - M(1, 2) * M(3, 3);
+  I(1, 3) = M(1, 2) * M(2, 3) - M(1, 3) * M(2, 2);
+  I(2, 1) = M(2, 3) * M(3, 1) - M(2, 1) * M(3, 3);
+  I(2, 2) = M(1, 1) * M(3, 3) - M(1, 3) * M(3, 1);
+  I(2, 3) = M(1, 3) * M(2, 1) - M(1, 1) * M(2, 3);
+  I(3, 1) = M(2, 1) * M(3, 2) - M(2, 2) * M(3, 1);
+  I(3, 2) = M(1, 2) * M(3, 1) - M(1, 1) * M(3, 2);
+  I(3, 3) = M(1, 1) * M(2, 2) - M(1, 2) * M(2, 1);
+#undef M
+#undef I
+  for (int i = 0; i < 9; ++i) inv[i] = inv[i] / v;
+  *vol = v;
+}
+
+// calc_gmx_switching_force_params (realcal.F90:888-904)
+void gmx_force(int pw, double lw, double up, double *A, double *B, double *C) {
+  double d = up - lw;
+  *A = -(double)pw * ((double)(pw + 4) * up - (double)(pw + 1) * lw) / (ipow_h(up, pw + 2) * ipow_h(d, 2)) / 3.0;
+  *B = (double)pw * ((double)(pw + 3) * up - (double)(pw + 1) * lw) / (ipow_h(up, pw + 2) * ipow_h(d, 3)) / 4.0;
+  *C = 1.0 / ipow_h(up, pw) - *A * ipow_h(d, 3) - *B * ipow_h(d, 4);
+}
+
+bool fft_axis(int n, FftAxis &ax) {
+  ax.n = n;
+  ax.nfac = 0;
+  int m = n;
+  const int rad[4] = {4, 2, 3, 5};
+  for (int r : rad)
+    while (m % r == 0 && ax.nfac < MAXFAC) {
+      ax.fac[ax.nfac++] = r;
+      m /= r;
+    }
+  return m == 1;
+}
+
+// columns per z-column workgroup: two LDS buffers of ncol x K3 complex within
+// 56 KB (static LDS of the kernel on top stays under 64 KB)
+int zcol_width(int K3, size_t vb) {
+  return (int)std::min<size_t>(64, std::max<size_t>(1, (size_t)(56 * 1024) / (2 * (size_t)K3 * vb)));
+}
+
+long fortran_nint(double x) { return (long)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
+
+}  // namespace
+
+// ============================================================== the context
+struct erm_ctx {
+  erm_params P{};
+  std::string err;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[8] = {};
+  double stage_ms[8] = {};
+  bool fp32 = false;
+  int Bmax = 256;
+  // topology
+  bool have_topo = false, have_bins = false, have_frame = false;
+  int natom = 0, nmol = 0, ntype = 0;
+  std::vector<int> mol_begin, ljtype, sluvid, uvspec;
+  std::vector<double> charge, lj_eps, lj_sig2, mol_charge, sitemass, bfcoord;
+  int slvmax = 0;
+  std::vector<int> tagpt, kof, sltlist;
+  int slt_template = -1, ns = 0;
+  int nvt = 0;
+  std::vector<int> vtypes;  // compact index -> ljtype
+  double total_charge = 0.0;
+  // bins
+  int numslv = 0, ermax = 0, esmax = 0;
+  std::vector<int> uvmax, uvsoft, specoff;
+  std::vector<double> uvcrd, escrd;
+  // frame
+  double cell[9] = {}, invcl[9] = {}, volume = 0.0, L[3] = {};
+  bool have_prevcl = false;
+  double prevcl[9] = {};
+  double stat_weight_system = 1.0;
+  std::vector<int> jof_k;  // sorted index of tagpt index k
+  Params kp{};
+  // refs rigid: realcal_self of the first configuration is reused
+  bool have_usreal = false;
+  // device buffers
+  DBuf d_mol_begin, d_charge, d_tagpt, d_cvt, d_mol_charge, d_uvspec, d_sitemass, d_bf, d_qslt, d_qslt_t, d_sltlj;
+  DBuf d_pos, d_cellid, d_count, d_cell_start, d_fill, d_rm, d_mol_k, d_mol_ns, d_mol_first, d_mol_q,
+      d_mol_spec, d_mol_m, d_jof;
+  DBuf d_sx, d_sy, d_sz, d_sq, d_sct, d_sg, d_sw;
+  DBuf d_tw1, d_tw2, d_tw3, d_engfac;
+  DBuf d_edges, d_specoff, d_uvmax, d_uvsoft, d_escrd;
+  DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
+      d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
+  DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
+  std::vector<DBuf *> all() {
+    return {&d_mol_begin, &d_charge, &d_tagpt, &d_cvt, &d_mol_charge, &d_uvspec, &d_sitemass, &d_bf, &d_qslt, &d_qslt_t,
+            &d_sltlj, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
+            &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
+            &d_sw, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
+            &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
+            &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
+            &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm};
+  }
+};
+
+namespace {
+
+int fail(erm_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return fail((c), ERM_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(erm_ctx *c, DBuf &b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.bytes >= bytes) return ERM_OK;
+  b.release();
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, ERM_E_DEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return ERM_OK;
+}
+
+template <typename X> int upload(erm_ctx *c, DBuf &b, const X *src, size_t n) {
+  int s = ensure(c, b, n * sizeof(X));
+  if (s) return s;
+  if (n) HIPCHK(c, hipMemcpyAsync(b.p, src, n * sizeof(X), hipMemcpyHostToDevice, c->st));
+  return ERM_OK;
+}
+
+template <typename X> X *dp(DBuf &b) { return reinterpret_cast<X *>(b.p); }
+
+int sync(erm_ctx *c) {
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return ERM_OK;
+}
+
+int launch_check(erm_ctx *c, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, ERM_E_DEVICE, "launch %s: %s", what, hipGetErrorString(e));
+  return ERM_OK;
+}
+
+#define TRY(x)              \
+  do {                      \
+    int s_ = (x);           \
+    if (s_) return s_;      \
+  } while (0)
+
+const char *status_name(int code) {
+  switch (code) {
+    case ERM_E_ENG_MIN: return "eng_min: energy at or below the lowest bin edge";
+    case ERM_E_ENG_SFT: return "eng_sft: energy above the soft region";
+    case ERM_E_MBIN: return "mbin: energy in the overflow bin below 1e3";
+    case ERM_E_ENG_CNS: return "eng_cns: solvent molecule without species";
+    default: return "error";
+  }
+}
+
+int check_status(erm_ctx *c, int first) {
+  int h[8];
+  HIPCHK(c, hipMemcpyAsync(h, c->d_status.p, sizeof h, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  if (h[0]) {
+    double v;
+    memcpy(&v, &h[4], sizeof v);
+    return fail(c, h[0], "%s (configuration %d, sorted molecule %d, energy %.6g)", status_name(h[0]),
+                first + h[1], h[2], v);
+  }
+  return ERM_OK;
+}
+
+template <typename T> struct TP {
+  using V = typename C2<T>::type;
+};
+
+// ----------------------------------------------------------- setup helpers
+int build_tw(erm_ctx *c, DBuf &b, int n) {
+  if (c->fp32) {
+    std::vector<float2> t(n);
+    for (int j = 0; j < n; ++j) {
+      double a = -2.0 * M_PI * (double)j / (double)n;
+      t[j] = make_float2((float)cos(a), (float)sin(a));
+    }
+    return upload(c, b, t.data(), t.size());
+  }
+  std::vector<double2> t(n);
+  for (int j = 0; j < n; ++j) {
+    double a = -2.0 * M_PI * (double)j / (double)n;
+    t[j] = make_double2(cos(a), sin(a));
+  }
+  return upload(c, b, t.data(), t.size());
+}
+
+int upload_real(erm_ctx *c, DBuf &b, const std::vector<double> &v) {
+  if (c->fp32) {
+    std::vector<float> f(v.begin(), v.end());
+    return upload(c, b, f.data(), f.size());
+  }
+  return upload(c, b, v.data(), v.size());
+}
+
+// per-solute-type tables: charges and LJ rows against the solvent types
+int solute_tables(erm_ctx *c) {
+  int m = c->slt_template;
+  int a0 = c->mol_begin[m];
+  std::vector<double> q(c->ns), lj((size_t)2 * c->ns * c->nvt);
+  for (int s = 0; s < c->ns; ++s) {
+    q[s] = c->charge[a0 + s];
+    int tu = c->ljtype[a0 + s];
+    for (int v = 0; v < c->nvt; ++v) {
+      int tv = c->vtypes[v];
+      lj[((size_t)s * c->nvt + v) * 2] = c->lj_eps[(size_t)tv * c->ntype + tu];
+      lj[((size_t)s * c->nvt + v) * 2 + 1] = c->lj_sig2[(size_t)tv * c->ntype + tu];
+    }
+  }
+  TRY(upload(c, c->d_qslt, q.data(), q.size()));
+  TRY(upload_real(c, c->d_qslt_t, q));
+  TRY(upload_real(c, c->d_sltlj, lj));
+  return ERM_OK;
+}
+
+// Params fields that depend on the frame
+void frame_params(erm_ctx *c) {
+  Params &k = c->kp;
+  for (int d = 0; d < 3; ++d) {
+    k.L[d] = c->L[d];
+    k.halfL[d] = 0.5 * c->L[d];
+    int nb = (int)std::floor(c->L[d] / c->P.block_threshold);
+    nb = std::max(1, std::min(nb, 1000));
+    k.nb[d] = nb;
+    k.hcell[d] = c->L[d] / (double)nb;
+  }
+  // keep the pillar count within what one workgroup lists
+  while ((long)k.nb[1] * k.nb[2] * 2 > MAXSEG) {
+    for (int d = 1; d < 3; ++d) {
+      k.nb[d] = std::max(1, k.nb[d] / 2);
+      k.hcell[d] = c->L[d] / (double)k.nb[d];
+    }
+  }
+  for (int i = 0; i < 9; ++i) k.invcl[i] = c->invcl[i];
+  k.volume = c->volume;
+  k.stat_weight_system = c->stat_weight_system;
+}
+
+template <typename T> int frame_prep(erm_ctx *c) {
+  Params &k = c->kp;
+  const int ncell = k.nb[0] * k.nb[1] * k.nb[2];
+  const int S = c->slvmax;
+  int nsites = 0;
+  for (int kk = 0; kk < S; ++kk) {
+    int m = c->tagpt[kk];
+    nsites += c->mol_begin[m + 1] - c->mol_begin[m];
+  }
+  TRY(ensure(c, c->d_cellid, sizeof(int) * S));
+  TRY(ensure(c, c->d_count, sizeof(int) * (ncell + 1)));
+  TRY(ensure(c, c->d_cell_start, sizeof(int) * (ncell + 1)));
+  TRY(ensure(c, c->d_fill, sizeof(int) * (ncell + 1)));
+  TRY(ensure(c, c->d_rm, sizeof(unsigned long long)));
+  TRY(ensure(c, c->d_mol_k, sizeof(int) * S));
+  TRY(ensure(c, c->d_mol_ns, sizeof(int) * S));
+  TRY(ensure(c, c->d_mol_first, sizeof(int) * (S + 1)));
+  TRY(ensure(c, c->d_mol_q, sizeof(double) * S));
+  TRY(ensure(c, c->d_mol_spec, sizeof(int) * S));
+  TRY(ensure(c, c->d_mol_m, sizeof(int) * S));
+  TRY(ensure(c, c->d_jof, sizeof(int) * S));
+  TRY(ensure(c, c->d_sx, sizeof(double) * nsites));
+  TRY(ensure(c, c->d_sy, sizeof(double) * nsites));
+  TRY(ensure(c, c->d_sz, sizeof(double) * nsites));
+  TRY(ensure(c, c->d_sq, sizeof(T) * nsites));
+  TRY(ensure(c, c->d_sct, sizeof(int) * nsites));
+  TRY(ensure(c, c->d_sg, sizeof(int) * nsites));
+  TRY(ensure(c, c->d_sw, sizeof(T) * (size_t)nsites * 3 * k.p));
+  HIPCHK(c, hipMemsetAsync(c->d_count.p, 0, sizeof(int) * (ncell + 1), c->st));
+  unsigned long long zero = 0;
+  HIPCHK(c, hipMemcpyAsync(c->d_rm.p, &zero, sizeof zero, hipMemcpyHostToDevice, c->st));
+  const int tb = 256, gb = (S + tb - 1) / tb;
+  hipLaunchKernelGGL(k_mol_cell, dim3(gb), dim3(tb), 0, c->st, k, dp<double>(c->d_pos), dp<int>(c->d_tagpt),
+                     dp<int>(c->d_mol_begin), dp<int>(c->d_cellid), dp<int>(c->d_count),
+                     dp<unsigned long long>(c->d_rm));
+  TRY(launch_check(c, "k_mol_cell"));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_count), dp<int>(c->d_cell_start), ncell);
+  TRY(launch_check(c, "k_scan"));
+  HIPCHK(c, hipMemcpyAsync(c->d_fill.p, c->d_cell_start.p, sizeof(int) * ncell, hipMemcpyDeviceToDevice, c->st));
+  hipLaunchKernelGGL(k_mol_scatter, dim3(gb), dim3(tb), 0, c->st, k, dp<int>(c->d_cellid), dp<int>(c->d_fill),
+                     dp<int>(c->d_tagpt), dp<int>(c->d_mol_begin), dp<int>(c->d_mol_k), dp<int>(c->d_mol_ns));
+  TRY(launch_check(c, "k_mol_scatter"));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_mol_ns), dp<int>(c->d_mol_first), S);
+  TRY(launch_check(c, "k_scan"));
+  hipLaunchKernelGGL(k_site_fill<T>, dim3(gb), dim3(tb), 0, c->st, k, dp<double>(c->d_pos), dp<int>(c->d_tagpt),
+                     dp<int>(c->d_mol_begin), dp<double>(c->d_charge), dp<int>(c->d_cvt),
+                     dp<double>(c->d_mol_charge), dp<int>(c->d_uvspec), dp<int>(c->d_mol_k),
+                     dp<int>(c->d_mol_first), dp<double>(c->d_sx), dp<double>(c->d_sy), dp<double>(c->d_sz),
+                     dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sg), dp<T>(c->d_sw), dp<double>(c->d_mol_q),
+                     dp<int>(c->d_mol_spec), dp<int>(c->d_mol_m));
+  TRY(launch_check(c, "k_site_fill"));
+  // sorted index of every tagpt molecule (for soln-mode solutes)
+  std::vector<int> molk(S);
+  HIPCHK(c, hipMemcpyAsync(molk.data(), c->d_mol_k.p, sizeof(int) * S, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  c->jof_k.assign(S, -1);
+  for (int j = 0; j < S; ++j) c->jof_k[molk[j]] = j;
+  return ERM_OK;
+}
+
+// --------------------------------------------------------- batch pipeline
+enum { DO_RECIP = 1, DO_REAL = 2, DO_RESIDUAL = 4, DO_SELF = 8, DO_HIST = 16 };
+
+struct Batch {
+  int n;
+  int first;                       // global index of the first configuration
+  const double *host_pos;          // n * ns * 3 (refs) or NULL
+  const int *solutes;              // soln: n molecule indices, else NULL
+  const double *weights;           // n or NULL
+  bool random;                     // device poses
+  uint64_t seed, first_index;
+};
+
+
+// Energies of a batch of configurations into d_uv (sorted molecule order);
+// flags select the terms.  Stage events ev[0..6].
+template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
+  using V = typename C2<T>::type;
+  Params k = c->kp;
+  const int n = bt.n, ns = c->ns, S = c->slvmax;
+  const int K1 = k.K1, K2 = k.K2, K3 = k.K3, H = k.H, p = k.p;
+  const int smol = bt.solutes ? bt.solutes[0] : c->slt_template;
+  k.solute_charge = c->mol_charge[smol];
+  k.ns = ns;
+  TRY(ensure(c, c->d_spos, sizeof(double) * (size_t)n * ns * 3));
+  TRY(ensure(c, c->d_sref, sizeof(double) * 4 * n));
+  TRY(ensure(c, c->d_sg3, sizeof(int) * (size_t)n * ns));
+  TRY(ensure(c, c->d_sw3, sizeof(T) * (size_t)n * ns * p));
+  TRY(ensure(c, c->d_sqv, sizeof(T) * (size_t)n * ns));
+  TRY(ensure(c, c->d_w1h, sizeof(V) * (size_t)n * ns * H));
+  TRY(ensure(c, c->d_w2h, sizeof(V) * (size_t)n * ns * K2));
+  TRY(ensure(c, c->d_uv, sizeof(double) * (size_t)n * (S + 1)));
+  TRY(ensure(c, c->d_sltj, sizeof(int) * n));
+  TRY(ensure(c, c->d_selfreal, sizeof(double) * n));
+  TRY(ensure(c, c->d_usreal, sizeof(double)));
+  const size_t vb = sizeof(V);
+  const int ncolwg = zcol_width(K3, vb);
+  const int nzg = (H * K2 + ncolwg - 1) / ncolwg;
+  TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
+  if (flags & DO_RECIP) {
+    TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
+    TRY(ensure(c, c->d_phi, sizeof(T) * (size_t)n * K3 * K2 * K1));
+  }
+  HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+  // configurations
+  if (bt.random) {
+    hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, bt.seed, bt.first_index,
+                       dp<double>(c->d_bf), dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
+                       dp<double>(c->d_cellm), dp<double>(c->d_spos));
+    TRY(launch_check(c, "k_pose"));
+  } else if (bt.host_pos) {
+    HIPCHK(c, hipMemcpyAsync(c->d_spos.p, bt.host_pos, sizeof(double) * (size_t)n * ns * 3, hipMemcpyHostToDevice,
+                             c->st));
+  } else {
+    for (int b = 0; b < n; ++b)
+      HIPCHK(c, hipMemcpyAsync(dp<double>(c->d_spos) + (size_t)b * ns * 3,
+                               dp<double>(c->d_pos) + (size_t)3 * c->mol_begin[bt.solutes[b]],
+                               sizeof(double) * ns * 3, hipMemcpyDeviceToDevice, c->st));
+  }
+  std::vector<int> sltj(n, -1);
+  if (bt.solutes)
+    for (int b = 0; b < n; ++b) {
+      int kk = c->kof[bt.solutes[b]];
+      sltj[b] = (kk >= 0) ? c->jof_k[kk] : -1;
+    }
+  TRY(upload(c, c->d_sltj, sltj.data(), n));
+  hipLaunchKernelGGL(k_solute_prep<T>, dim3(n), dim3(128), 0, c->st, k, dp<double>(c->d_spos),
+                     dp<double>(c->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<double>(c->d_sref),
+                     dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h));
+  TRY(launch_check(c, "k_solute_prep"));
+  HIPCHK(c, hipEventRecord(c->ev[1], c->st));
+  if (flags & DO_RECIP) {
+    size_t lz = 2 * (size_t)ncolwg * K3 * vb;
+    hipLaunchKernelGGL(k_pme_zcol<T>, dim3(nzg, n), dim3(NTHR), lz, c->st, k, ncolwg, dp<T>(c->d_engfac),
+                       dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h),
+                       dp<V>(c->d_tw3), dp<V>(c->d_Hk), dp<double>(c->d_selfp));
+    TRY(launch_check(c, "k_pme_zcol"));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+    size_t lp = 2 * (size_t)K2 * H * vb;
+    hipLaunchKernelGGL(k_pme_plane<T>, dim3(K3, n), dim3(NTHR), lp, c->st, k, dp<V>(c->d_Hk), dp<V>(c->d_tw1),
+                       dp<V>(c->d_tw2), dp<T>(c->d_phi));
+    TRY(launch_check(c, "k_pme_plane"));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+    Params kg = k;
+    if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
+    hipLaunchKernelGGL(k_gather<T>, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, dp<T>(c->d_phi),
+                       dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), dp<int>(c->d_sg),
+                       dp<T>(c->d_sw), dp<T>(c->d_sq), dp<int>(c->d_sltj), dp<double>(c->d_uv));
+    TRY(launch_check(c, "k_gather"));
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->d_uv.p, 0, sizeof(double) * (size_t)n * (S + 1), c->st));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+  }
+  HIPCHK(c, hipEventRecord(c->ev[4], c->st));
+  if (flags & DO_REAL) {
+    double Rg = k.rcmax + 8.0;
+    double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
+    int nsplit = std::max(1, std::min(16, (int)std::ceil(frac * S / (2.0 * NTHR))));
+    size_t lpair = sizeof(T) * ((size_t)4 * ns + (size_t)2 * ns * c->nvt) + sizeof(int) * (2 * MAXSEG + 2) + 64;
+    hipLaunchKernelGGL(k_pair<T>, dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
+                       dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
+                       dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                       dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                       dp<double>(c->d_rm), dp<double>(c->d_uv));
+    TRY(launch_check(c, "k_pair"));
+  }
+  HIPCHK(c, hipEventRecord(c->ev[5], c->st));
+  if (flags & DO_SELF) {
+    // realcal_self: reused after the first configuration for a rigid test
+    // particle (engproc.F90:712-730), recomputed per configuration otherwise
+    int per_item = (c->P.slttype != 2);
+    if (per_item) {
+      for (int b = 0; b < n; ++b)
+        hipLaunchKernelGGL(k_selfreal, dim3(1), dim3(NTHR), 0, c->st, k, dp<double>(c->d_spos), b,
+                           dp<double>(c->d_qslt), dp<double>(c->d_selfreal) + b);
+    } else if (!c->have_usreal) {
+      hipLaunchKernelGGL(k_selfreal, dim3(1), dim3(NTHR), 0, c->st, k, dp<double>(c->d_spos), 0,
+                         dp<double>(c->d_qslt), dp<double>(c->d_usreal));
+      c->have_usreal = true;
+    }
+    TRY(launch_check(c, "k_selfreal"));
+    Params ku = k;
+    if (!(flags & DO_RECIP)) {
+      HIPCHK(c, hipMemsetAsync(c->d_selfp.p, 0, sizeof(double) * (size_t)n * nzg, c->st));
+    }
+    hipLaunchKernelGGL(k_u0, dim3((n + 63) / 64), dim3(64), 0, c->st, ku, n, nzg, dp<double>(c->d_selfp),
+                       per_item ? dp<double>(c->d_selfreal) : dp<double>(c->d_usreal), per_item,
+                       dp<double>(c->d_uv));
+    TRY(launch_check(c, "k_u0"));
+  }
+  return ERM_OK;
+}
+
+// update_histogram for the n configurations in d_uv (weights in d_wsol)
+int run_histogram(erm_ctx *c, int n, int first) {
+  Params k = c->kp;
+  const int E = c->ermax;
+  TRY(ensure(c, c->d_hist, sizeof(double) * (size_t)n * E));
+  TRY(ensure(c, c->d_w, sizeof(double) * n));
+  TRY(ensure(c, c->d_flceng, sizeof(double) * (size_t)n * MAXSPEC));
+  HIPCHK(c, hipMemsetAsync(c->d_status.p, 0, 32, c->st));
+  size_t lb = (size_t)E * (sizeof(double) + sizeof(int));
+  hipLaunchKernelGGL(k_bin, dim3(n), dim3(NTHR), lb, c->st, k, dp<double>(c->d_uv), dp<double>(c->d_edges),
+                     dp<int>(c->d_specoff), dp<int>(c->d_uvmax), dp<int>(c->d_uvsoft), dp<int>(c->d_mol_spec),
+                     dp<int>(c->d_sltj), dp<double>(c->d_hist), dp<double>(c->d_flceng),
+                     dp<unsigned long long>(c->d_mnmx), dp<int>(c->d_status));
+  TRY(launch_check(c, "k_bin"));
+  hipLaunchKernelGGL(k_scalars, dim3(1), dim3(1), 0, c->st, k, n, dp<double>(c->d_uv), dp<double>(c->d_wsol),
+                     dp<double>(c->d_escrd), dp<double>(c->d_scal), dp<double>(c->d_eself),
+                     dp<unsigned long long>(c->d_mnmx), dp<double>(c->d_w), dp<int>(c->d_status));
+  TRY(launch_check(c, "k_scalars"));
+  hipLaunchKernelGGL(k_edens, dim3((E + 255) / 256), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
+                     dp<double>(c->d_w), dp<double>(c->d_edens));
+  TRY(launch_check(c, "k_edens"));
+  if (c->P.slttype == 1) {
+    hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, dp<double>(c->d_flceng),
+                       dp<double>(c->d_w), dp<double>(c->d_slnuv));
+    TRY(launch_check(c, "k_slnuv"));
+  }
+  HIPCHK(c, hipEventRecord(c->ev[6], c->st));
+  if (c->P.corrcal) {
+    int nt = (E + 15) / 16;
+    int waves = nt * nt;
+    hipLaunchKernelGGL(k_corr, dim3((waves + 3) / 4), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
+                       dp<double>(c->d_w), dp<double>(c->d_ecorr));
+    TRY(launch_check(c, "k_corr"));
+  }
+  HIPCHK(c, hipEventRecord(c->ev[7], c->st));
+  return check_status(c, first);
+}
+
+int copy_out_uv(erm_ctx *c, int n, double *out) {
+  const int S = c->slvmax;
+  TRY(ensure(c, c->d_uvout, sizeof(double) * (size_t)n * (S + 1)));
+  hipLaunchKernelGGL(k_unsort, dim3((S + 255) / 256, n), dim3(256), 0, c->st, n, S, dp<int>(c->d_mol_k),
+                     dp<double>(c->d_uv), dp<double>(c->d_uvout));
+  TRY(launch_check(c, "k_unsort"));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_uvout.p, sizeof(double) * (size_t)n * (S + 1), hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+int stage_times(erm_ctx *c, bool hist) {
+  TRY(sync(c));
+  float ms;
+  for (int i = 0; i < 7; ++i) {
+    int a = i, b = i + 1;
+    if (!hist && b > 5) {
+      c->stage_ms[i] = 0;
+      continue;
+    }
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) == hipSuccess) c->stage_ms[i] += ms;
+  }
+  if (hipEventElapsedTime(&ms, c->ev[0], c->ev[hist ? 7 : 5]) == hipSuccess) c->stage_ms[7] += ms;
+  return ERM_OK;
+}
+
+int need(erm_ctx *c, bool frame, bool bins) {
+  if (!c) return ERM_E_ARG;
+  if (!c->have_topo) return fail(c, ERM_E_ARG, "erm_set_topology has not been called");
+  if (bins && !c->have_bins) return fail(c, ERM_E_ARG, "erm_set_bins has not been called");
+  if (frame && !c->have_frame) return fail(c, ERM_E_ARG, "erm_set_frame has not been called");
+  return ERM_OK;
+}
+
+// energies (and optionally histograms) over n configurations in batches
+int process(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
+  const int ns = c->ns;
+  for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
+  for (int s0 = 0; s0 < all.n; s0 += c->Bmax) {
+    Batch bt = all;
+    bt.n = std::min(c->Bmax, all.n - s0);
+    bt.first = all.first + s0;
+    if (all.host_pos) bt.host_pos = all.host_pos + (size_t)s0 * ns * 3;
+    if (all.solutes) bt.solutes = all.solutes + s0;
+    if (all.weights) bt.weights = all.weights + s0;
+    bt.first_index = all.first_index + (uint64_t)s0;
+    std::vector<double> ws(bt.n, 1.0);
+    if (bt.weights)
+      for (int b = 0; b < bt.n; ++b) ws[b] = bt.weights[b];
+    TRY(ensure(c, c->d_wsol, sizeof(double) * bt.n));
+    TRY(upload(c, c->d_wsol, ws.data(), bt.n));
+    int st = c->fp32 ? run_energies<float>(c, bt, flags) : run_energies<double>(c, bt, flags);
+    if (st) return st;
+    if (flags & DO_HIST) TRY(run_histogram(c, bt.n, bt.first));
+    TRY(stage_times(c, (flags & DO_HIST) != 0));
+    if (uv_out) TRY(copy_out_uv(c, bt.n, uv_out + (size_t)s0 * (c->slvmax + 1)));
+  }
+  return ERM_OK;
+}
+
+int check_solutes(erm_ctx *c, int n, const int *solutes, const double *pos) {
+  if (n < 0) return fail(c, ERM_E_ARG, "negative count");
+  if (c->P.slttype == 1) {
+    if (!solutes) return fail(c, ERM_E_ARG, "soln mode needs solute molecule indices");
+    for (int i = 0; i < n; ++i) {
+      int m = solutes[i];
+      if (m < 0 || m >= c->nmol || c->sluvid[m] != 1)
+        return fail(c, ERM_E_ARG, "molecule %d is not a solute", m);
+    }
+  } else {
+    if (!pos) return fail(c, ERM_E_ARG, "refs mode needs test-particle coordinates");
+  }
+  return ERM_OK;
+}
+
+template <typename T> int frame_prep_any(erm_ctx *c) { return frame_prep<T>(c); }
+
+int set_frame_common(erm_ctx *c, const double *cell, double w) {
+  TRY(need(c, false, false));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (i != j && std::fabs(cell[3 * j + i]) > 1e-8)
+        return fail(c, ERM_E_NONCUBOID, "non-cuboid cell: a non-diagonal cell is not supported");
+  memcpy(c->cell, cell, sizeof c->cell);
+  inverted_cell(c->cell, c->invcl, &c->volume);
+  for (int d = 0; d < 3; ++d) c->L[d] = std::fabs(cell[4 * d]);
+  c->stat_weight_system = w;
+  Params &k = c->kp;
+  // the cutoff must stay below half the box (single minimum image)
+  bool changed = !c->have_prevcl;
+  for (int i = 0; i < 9 && !changed; ++i) changed = (c->prevcl[i] != cell[i]);
+  if (changed) {
+    std::vector<double> splfc[3];
+    double coeff[MAXP + 1];
+    spline_coeff(k.p, coeff);
+    for (int d = 0; d < 3; ++d) {
+      splfc[d].resize(c->P.ms[d]);
+      spline_axis(c->P.ms[d], k.p, coeff, splfc[d].data());
+    }
+    std::vector<double> engfac;
+    greenfunc(c->P.ms, k.p, c->invcl, c->volume, c->P.screen, splfc, engfac);
+    TRY(upload_real(c, c->d_engfac, engfac));
+  }
+  memcpy(c->prevcl, cell, sizeof c->prevcl);
+  c->have_prevcl = true;
+  frame_params(c);
+  TRY(upload(c, c->d_cellm, c->cell, 9));
+  return ERM_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C-ABI
+extern "C" {
+
+int32_t erm_build_edges(const erm_hist *h, int32_t self_energy, double *ercrd_out, int32_t cap, int32_t *pemax_out,
+                        int32_t *pesoft_out) {
+  // enginit edge construction (engproc.F90:163-218)
+  if (!h || !ercrd_out || !pemax_out || !pesoft_out) return ERM_E_ARG;
+  const int RGLMAX = 5, LARGE = 10000;
+  double ecdmin = h->ecdmin, ecfmns = h->ecfmns, ecdcen = h->ecdcen, eccore = h->eccore, ecdmax = h->ecdmax;
+  double ectmvl = h->finfac * h->ecfbin;
+  ecdmin = ecdmin - ectmvl;
+  ecfmns = ecfmns - ectmvl;
+  double ecmns0 = ecdcen - ectmvl;
+  double ecpls0 = 2.0 * ecdcen - ecmns0;
+  double ecfpls = 2.0 * ecdcen - ecfmns;
+  eccore = eccore + ecfpls - ecdcen;
+  const double cdrgvl[7] = {ecdmin, ecfmns, ecmns0, ecpls0, ecfpls, eccore, ecdmax};
+  int uprgcd[7];
+  uprgcd[0] = 1;
+  for (int regn = 1; regn <= RGLMAX; ++regn) {
+    double factor = (regn == 1 || regn == 5) ? h->eclbin : (regn == 2 || regn == 4) ? h->ecfbin : h->ec0bin;
+    double incre = cdrgvl[regn] - cdrgvl[regn - 1];
+    if (incre < factor) return ERM_E_ENG_ECD;
+    uprgcd[regn] = uprgcd[regn - 1] + (int)fortran_nint(incre / factor);
+  }
+  int pesoft = uprgcd[RGLMAX] - uprgcd[0];
+  int pemax = pesoft + h->pecore;
+  uprgcd[RGLMAX + 1] = pemax;
+  if (pemax > LARGE) return ERM_E_ENG_SIZ;
+  std::vector<double> e((size_t)std::max(pemax, uprgcd[RGLMAX]) + 2, 0.0);
+  for (int r = 0; r <= RGLMAX + 1; ++r) e[uprgcd[r]] = cdrgvl[r];
+  int nreg = (h->pecore == 0) ? RGLMAX : RGLMAX + 1;
+  for (int regn = 1; regn <= nreg; ++regn) {
+    int minrg = uprgcd[regn - 1], maxrg = uprgcd[regn];
+    double incre = 0.0;
+    if (regn <= RGLMAX) incre = e[maxrg] - e[minrg];
+    if (regn == RGLMAX + 1) incre = log(e[maxrg] / e[minrg]);
+    double factor = incre / (double)(maxrg - minrg);
+    for (int iduv = minrg; iduv <= maxrg - 1; ++iduv) {
+      incre = factor * (double)(iduv - minrg);
+      if (regn <= RGLMAX) e[iduv] = e[minrg] + incre;
+      if (regn == RGLMAX + 1) e[iduv] = e[minrg] * exp(incre);
+    }
+  }
+  int n = self_energy ? pesoft : pemax;
+  if (n > cap) return ERM_E_ARG;
+  for (int i = 0; i < n; ++i) ercrd_out[i] = e[i + 1];
+  *pemax_out = n;
+  *pesoft_out = pesoft;
+  return ERM_OK;
+}
+
+uint64_t erm_hash_double(const double *v, int32_t n) {
+  uint64_t x = 0, b;
+  for (int32_t i = 0; i < n; ++i) {
+    memcpy(&b, &v[i], sizeof b);
+    x = ((x << 7) | (x >> 57)) ^ b;
+  }
+  return x;
+}
+
+int32_t erm_create(const erm_params *prm, erm_ctx **out) {
+  if (!prm || !out) return ERM_E_ARG;
+  *out = nullptr;
+  const erm_params &P = *prm;
+  if (P.slttype < 1 || P.slttype > 3) return ERM_E_ARG;
+  if (P.splodr < 2 || P.splodr > MAXP) return ERM_E_UNSUPPORTED;
+  if (P.ljswitch < 0 || P.ljswitch > 3) return ERM_E_ARG;
+  if (P.precision != ERM_FP64 && P.precision != ERM_FP32) return ERM_E_ARG;
+  FftAxis ax[3];
+  for (int d = 0; d < 3; ++d)
+    if (P.ms[d] < 2 || P.ms[d] > 512 || P.ms[d] < P.splodr || !fft_axis(P.ms[d], ax[d])) return ERM_E_UNSUPPORTED;
+  if (!(P.screen > 0) || !(P.elecut > 0) || !(P.upljcut >= P.lwljcut) || !(P.block_threshold > 0)) return ERM_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ERM_E_DEVICE;
+  if (P.device < 0 || P.device >= ndev) return ERM_E_ARG;
+  if (hipSetDevice(P.device) != hipSuccess) return ERM_E_DEVICE;
+  erm_ctx *c = new erm_ctx();
+  c->P = P;
+  c->fp32 = (P.precision == ERM_FP32);
+  c->Bmax = P.max_batch > 0 ? P.max_batch : 256;
+  if (hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return ERM_E_DEVICE;
+  }
+  for (int i = 0; i < 8; ++i)
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+      erm_destroy(c);
+      return ERM_E_DEVICE;
+    }
+  Params &k = c->kp;
+  memset(&k, 0, sizeof k);
+  k.K1 = P.ms[0];
+  k.K2 = P.ms[1];
+  k.K3 = P.ms[2];
+  k.H = k.K1 / 2 + 1;
+  k.p = P.splodr;
+  k.ax1 = ax[0];
+  k.ax2 = ax[1];
+  k.ax3 = ax[2];
+  spline_coeff(k.p, k.coeff);
+  k.screen = P.screen;
+  k.ljswitch = P.ljswitch;
+  k.lwljcut = P.lwljcut;
+  k.upljcut = P.upljcut;
+  k.lw2 = ipow_h(P.lwljcut, 2);
+  k.up2 = ipow_h(P.upljcut, 2);
+  k.el2 = ipow_h(P.elecut, 2);
+  k.rcmax = std::max(P.upljcut, P.elecut);
+  k.cut2 = ipow_h(k.rcmax, 2);
+  k.lw3 = ipow_h(P.lwljcut, 3);
+  k.up3 = ipow_h(P.upljcut, 3);
+  k.lw6 = k.lw3 * k.lw3;
+  k.up6 = k.up3 * k.up3;
+  gmx_force(12, P.lwljcut, P.upljcut, &k.repA, &k.repB, &k.repC);
+  gmx_force(6, P.lwljcut, P.upljcut, &k.attA, &k.attB, &k.attC);
+  k.slttype = P.slttype;
+  k.wgtslf = P.wgtslf;
+  k.estype = P.estype;
+  k.selfcal = P.selfcal;
+  k.force = P.force_calculation;
+  k.temp = P.temp;
+  // dynamic LDS above 64 KB for the PME plane kernel on large grids
+  size_t vb = c->fp32 ? sizeof(float2) : sizeof(double2);
+  size_t lp = 2 * (size_t)k.K2 * k.H * vb;
+  if (lp > 160 * 1024 || (size_t)((k.K2 + 1) / 2) * k.K1 > (size_t)k.K2 * k.H) {
+    erm_destroy(c);
+    return ERM_E_UNSUPPORTED;
+  }
+  if (lp > 64 * 1024) {
+    hipError_t e = c->fp32 ? hipFuncSetAttribute((const void *)k_pme_plane<float>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp)
+                           : hipFuncSetAttribute((const void *)k_pme_plane<double>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
+    if (e != hipSuccess) {
+      erm_destroy(c);
+      return ERM_E_UNSUPPORTED;
+    }
+  }
+  int s = build_tw(c, c->d_tw1, k.K1);
+  if (!s) s = build_tw(c, c->d_tw2, k.K2);
+  if (!s) s = build_tw(c, c->d_tw3, k.K3);
+  if (!s) s = ensure(c, c->d_status, 64);
+  if (s) {
+    erm_destroy(c);
+    return s;
+  }
+  *out = c;
+  return ERM_OK;
+}
+
+void erm_destroy(erm_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->P.device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (DBuf *b : c->all()) b->release();
+  for (int i = 0; i < 8; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char *erm_last_error(const erm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t erm_set_topology(erm_ctx *c, int32_t natom, int32_t nmol, const int32_t *mol_begin, const double *charge,
+                         const int32_t *ljtype, int32_t ntype, const double *lj_eps, const double *lj_sig2,
+                         const int32_t *sluvid, const int32_t *uvspec, const double *mol_charge,
+                         const double *sitemass, const double *bfcoord) {
+  if (!c) return ERM_E_ARG;
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (natom <= 0 || nmol <= 0 || ntype <= 0 || !mol_begin || !charge || !ljtype || !lj_eps || !lj_sig2 || !sluvid ||
+      !uvspec || !mol_charge || !sitemass)
+    return fail(c, ERM_E_ARG, "missing topology array");
+  if (mol_begin[0] != 0 || mol_begin[nmol] != natom) return fail(c, ERM_E_ARG, "mol_begin must span 0..natom");
+  for (int i = 0; i < nmol; ++i)
+    if (mol_begin[i + 1] <= mol_begin[i]) return fail(c, ERM_E_ARG, "molecule %d has no sites", i);
+  for (int a = 0; a < natom; ++a)
+    if (ljtype[a] < 0 || ljtype[a] >= ntype) return fail(c, ERM_E_ARG, "ljtype out of range at site %d", a);
+  c->natom = natom;
+  c->nmol = nmol;
+  c->ntype = ntype;
+  c->mol_begin.assign(mol_begin, mol_begin + nmol + 1);
+  c->charge.assign(charge, charge + natom);
+  c->ljtype.assign(ljtype, ljtype + natom);
+  c->lj_eps.assign(lj_eps, lj_eps + (size_t)ntype * ntype);
+  c->lj_sig2.assign(lj_sig2, lj_sig2 + (size_t)ntype * ntype);
+  c->sluvid.assign(sluvid, sluvid + nmol);
+  c->uvspec.assign(uvspec, uvspec + nmol);
+  c->mol_charge.assign(mol_charge, mol_charge + nmol);
+  c->sitemass.assign(sitemass, sitemass + natom);
+  // tagpt (engproc.F90:336-347) and the solute list
+  c->tagpt.clear();
+  c->sltlist.clear();
+  c->kof.assign(nmol, -1);
+  for (int i = 0; i < nmol; ++i) {
+    if (sluvid[i] < 0 || sluvid[i] > 3) return fail(c, ERM_E_ENG_BUG, "sluvid out of range");
+    if (sluvid[i] == 0 || sluvid[i] == 1) {
+      c->kof[i] = (int)c->tagpt.size();
+      c->tagpt.push_back(i);
+    }
+    if (sluvid[i] > 0) c->sltlist.push_back(i);
+  }
+  c->slvmax = (int)c->tagpt.size();
+  if (c->sltlist.empty()) return fail(c, ERM_E_ARG, "no solute or test particle");
+  if (c->slvmax == 0) return fail(c, ERM_E_ARG, "no physical molecule");
+  if (c->P.slttype != 1) {
+    if (c->sltlist.size() != 1 || c->sltlist[0] != nmol - 1)
+      return fail(c, ERM_E_ARG, "the test particle must be the single last molecule (eng_ins)");
+    for (int i = 0; i < nmol; ++i)
+      if (sluvid[i] == 1) return fail(c, ERM_E_ARG, "solute molecule in a refs system (eng_par)");
+  } else {
+    for (int i = 0; i < nmol; ++i)
+      if (sluvid[i] >= 2) return fail(c, ERM_E_ARG, "test particle in a soln system (eng_par)");
+  }
+  c->slt_template = c->sltlist[0];
+  c->ns = c->mol_begin[c->slt_template + 1] - c->mol_begin[c->slt_template];
+  for (int m : c->sltlist)
+    if (c->mol_begin[m + 1] - c->mol_begin[m] != c->ns) return fail(c, ERM_E_ARG, "solutes differ in size");
+  if (c->ns > 4096) return fail(c, ERM_E_UNSUPPORTED, "solute with more than 4096 sites");
+  // compact LJ types of the solvent sites (sluvid 0 molecules and other solutes)
+  std::vector<int> cv(ntype, -1), cvt(natom, 0);
+  c->vtypes.clear();
+  for (int k = 0; k < c->slvmax; ++k) {
+    int m = c->tagpt[k];
+    for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a) {
+      int t = ljtype[a];
+      if (cv[t] < 0) {
+        cv[t] = (int)c->vtypes.size();
+        c->vtypes.push_back(t);
+      }
+      cvt[a] = cv[t];
+    }
+  }
+  c->nvt = (int)c->vtypes.size();
+  size_t tbytes = c->fp32 ? 4 : 8;
+  if ((size_t)2 * c->ns * c->nvt * tbytes + 4 * (size_t)c->ns * tbytes > 100 * 1024)
+    return fail(c, ERM_E_UNSUPPORTED, "solute x solvent LJ table too large for LDS");
+  c->total_charge = 0.0;
+  for (int i = 0; i < nmol; ++i)
+    if (sluvid[i] == 0 || sluvid[i] == 1) c->total_charge = c->total_charge + mol_charge[i];
+  c->kp.total_charge = c->total_charge;
+  c->kp.slvmax = c->slvmax;
+  c->kp.ns = c->ns;
+  c->kp.nvt = c->nvt;
+  c->bfcoord.clear();
+  if (bfcoord) c->bfcoord.assign(bfcoord, bfcoord + 3 * c->ns);
+  TRY(upload(c, c->d_mol_begin, mol_begin, nmol + 1));
+  TRY(upload(c, c->d_charge, charge, natom));
+  TRY(upload(c, c->d_tagpt, c->tagpt.data(), c->slvmax));
+  TRY(upload(c, c->d_cvt, cvt.data(), natom));
+  TRY(upload(c, c->d_mol_charge, mol_charge, nmol));
+  TRY(upload(c, c->d_uvspec, uvspec, nmol));
+  TRY(upload(c, c->d_sitemass, sitemass, natom));
+  if (bfcoord) TRY(upload(c, c->d_bf, bfcoord, 3 * c->ns));
+  TRY(solute_tables(c));
+  TRY(ensure(c, c->d_pos, sizeof(double) * 3 * natom));
+  TRY(sync(c));
+  c->have_topo = true;
+  c->have_frame = false;
+  c->have_usreal = false;
+  return ERM_OK;
+}
+
+int32_t erm_set_bins(erm_ctx *c, int32_t numslv, const int32_t *uvmax, const int32_t *uvsoft, const double *uvcrd,
+                     int32_t esmax, const double *escrd) {
+  if (!c) return ERM_E_ARG;
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (numslv <= 0 || numslv > MAXSPEC || !uvmax || !uvsoft || !uvcrd)
+    return fail(c, ERM_E_ARG, "numslv must be 1..%d with edge arrays", MAXSPEC);
+  if (c->P.selfcal && (esmax <= 0 || !escrd)) return fail(c, ERM_E_ARG, "selfcal needs escrd");
+  c->numslv = numslv;
+  c->uvmax.assign(uvmax, uvmax + numslv);
+  c->uvsoft.assign(uvsoft, uvsoft + numslv);
+  c->specoff.assign(numslv, 0);
+  int E = 0;
+  for (int i = 0; i < numslv; ++i) {
+    if (uvmax[i] <= 0) return fail(c, ERM_E_ARG, "empty species grid");
+    c->specoff[i] = E;
+    E += uvmax[i];
+  }
+  if ((size_t)E * 12 > 120 * 1024) return fail(c, ERM_E_UNSUPPORTED, "ermax %d too large", E);
+  c->ermax = E;
+  c->uvcrd.assign(uvcrd, uvcrd + E);
+  c->esmax = c->P.selfcal ? esmax : 0;
+  c->escrd.clear();
+  if (c->P.selfcal) c->escrd.assign(escrd, escrd + esmax);
+  c->kp.ermax = E;
+  c->kp.numslv = numslv;
+  c->kp.esmax = c->esmax;
+  TRY(upload(c, c->d_edges, uvcrd, E));
+  TRY(upload(c, c->d_specoff, c->specoff.data(), numslv));
+  TRY(upload(c, c->d_uvmax, uvmax, numslv));
+  TRY(upload(c, c->d_uvsoft, uvsoft, numslv));
+  if (c->esmax) TRY(upload(c, c->d_escrd, escrd, c->esmax));
+  TRY(ensure(c, c->d_edens, sizeof(double) * E));
+  if (c->P.corrcal) TRY(ensure(c, c->d_ecorr, sizeof(double) * (size_t)E * E));
+  TRY(ensure(c, c->d_eself, sizeof(double) * std::max(1, c->esmax)));
+  TRY(ensure(c, c->d_slnuv, sizeof(double) * (numslv + 1)));
+  TRY(ensure(c, c->d_mnmx, sizeof(unsigned long long) * 2 * (numslv + 1)));
+  TRY(ensure(c, c->d_scal, sizeof(double) * 8));
+  c->have_bins = true;
+  // first-time state as enginit leaves it (engproc.F90:255-260)
+  std::vector<unsigned long long> mm(2 * (numslv + 1));
+  for (int i = 0; i <= numslv; ++i) {
+    mm[2 * i] = ~0ULL;
+    mm[2 * i + 1] = 0ULL;
+  }
+  TRY(upload(c, c->d_mnmx, mm.data(), mm.size()));
+  double sc[8] = {0, 0, 0, -1.0e50, 0, 0, 0, 0};
+  TRY(upload(c, c->d_scal, sc, 8));
+  TRY(sync(c));
+  return erm_engclear(c);
+}
+
+int32_t erm_engclear(erm_ctx *c) {
+  TRY(need(c, false, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  HIPCHK(c, hipMemsetAsync(c->d_edens.p, 0, sizeof(double) * c->ermax, c->st));
+  if (c->P.corrcal) HIPCHK(c, hipMemsetAsync(c->d_ecorr.p, 0, sizeof(double) * (size_t)c->ermax * c->ermax, c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_eself.p, 0, sizeof(double) * std::max(1, c->esmax), c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_slnuv.p, 0, sizeof(double) * (c->numslv + 1), c->st));
+  // avslf, engnorm, engsmpl (voffset persists across divisions)
+  HIPCHK(c, hipMemsetAsync(c->d_scal.p, 0, sizeof(double) * 3, c->st));
+  return sync(c);
+}
+
+static int set_frame_impl(erm_ctx *c, const double *pos, bool device_ptr, const double *cell, double w) {
+  if (!c || !pos || !cell) return ERM_E_ARG;
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  TRY(set_frame_common(c, cell, w));
+  HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * c->natom,
+                           device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+  int s = c->fp32 ? frame_prep<float>(c) : frame_prep<double>(c);
+  if (s) return s;
+  c->have_frame = true;
+  return ERM_OK;
+}
+
+int32_t erm_set_frame(erm_ctx *c, const double *sitepos, const double *cell, double w) {
+  return set_frame_impl(c, sitepos, false, cell, w);
+}
+
+int32_t erm_set_frame_device(erm_ctx *c, const double *d_sitepos, const double *cell, double w) {
+  return set_frame_impl(c, d_sitepos, true, cell, w);
+}
+
+int32_t erm_get_uv_energy(erm_ctx *c, int32_t solute, int32_t n, const double *solute_pos, double *uvengy) {
+  TRY(need(c, true, false));
+  if (!uvengy) return fail(c, ERM_E_ARG, "uvengy is NULL");
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  std::vector<int> sl;
+  if (c->P.slttype == 1) {
+    if (solute_pos) return fail(c, ERM_E_UNSUPPORTED, "soln mode takes solute coordinates from the frame");
+    sl.assign(std::max(n, 0), solute);
+  } else if (solute != c->slt_template) {
+    return fail(c, ERM_E_ARG, "molecule %d is not the test particle", solute);
+  }
+  TRY(check_solutes(c, n, sl.empty() ? nullptr : sl.data(), solute_pos));
+  Batch bt{n, 0, solute_pos, sl.empty() ? nullptr : sl.data(), nullptr, false, 0, 0};
+  return process(c, bt, DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF, uvengy);
+}
+
+static int piece(erm_ctx *c, int32_t solute, const double *pos, int flags, double *out, double *self_out,
+                 bool self_from_u0) {
+  TRY(need(c, true, false));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  int sl = solute;
+  if (c->P.slttype == 1) {
+    if (pos) return fail(c, ERM_E_UNSUPPORTED, "soln mode takes solute coordinates from the frame");
+    TRY(check_solutes(c, 1, &sl, nullptr));
+  } else {
+    if (solute != c->slt_template) return fail(c, ERM_E_ARG, "molecule %d is not the test particle", solute);
+    TRY(check_solutes(c, 1, nullptr, pos));
+  }
+  Batch bt{1, 0, c->P.slttype == 1 ? nullptr : pos, c->P.slttype == 1 ? &sl : nullptr, nullptr, false, 0, 0};
+  std::vector<double> uv(c->slvmax + 1);
+  TRY(process(c, bt, flags, uv.data()));
+  if (out) memcpy(out, uv.data() + 1, sizeof(double) * c->slvmax);
+  if (self_out && self_from_u0) *self_out = uv[0];
+  return ERM_OK;
+}
+
+int32_t erm_realcal_proc(erm_ctx *c, int32_t solute, const double *pos, double *ereal) {
+  return piece(c, solute, pos, DO_REAL, ereal, nullptr, false);
+}
+
+int32_t erm_recpcal(erm_ctx *c, int32_t solute, const double *pos, double *erecp, double *self_recp) {
+  TRY(piece(c, solute, pos, DO_RECIP, erecp, nullptr, false));
+  if (self_recp) {
+    // sum of the column partials of the single configuration
+    const int vb = c->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
+    const int ncolwg = zcol_width(c->kp.K3, vb);
+    const int nzg = (c->kp.H * c->kp.K2 + ncolwg - 1) / ncolwg;
+    std::vector<double> part(nzg);
+    HIPCHK(c, hipMemcpy(part.data(), c->d_selfp.p, sizeof(double) * nzg, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *self_recp = s;
+  }
+  return ERM_OK;
+}
+
+int32_t erm_realcal_self(erm_ctx *c, int32_t solute, const double *pos, double *self_real) {
+  TRY(need(c, true, false));
+  if (!self_real) return fail(c, ERM_E_ARG, "self_real is NULL");
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  const int ns = c->ns;
+  TRY(ensure(c, c->d_spos, sizeof(double) * ns * 3));
+  TRY(ensure(c, c->d_selfreal, sizeof(double)));
+  if (c->P.slttype == 1) {
+    int sl = solute;
+    TRY(check_solutes(c, 1, &sl, nullptr));
+    HIPCHK(c, hipMemcpyAsync(c->d_spos.p, dp<double>(c->d_pos) + (size_t)3 * c->mol_begin[sl],
+                             sizeof(double) * ns * 3, hipMemcpyDeviceToDevice, c->st));
+  } else {
+    if (!pos) return fail(c, ERM_E_ARG, "refs mode needs test-particle coordinates");
+    HIPCHK(c, hipMemcpyAsync(c->d_spos.p, pos, sizeof(double) * ns * 3, hipMemcpyHostToDevice, c->st));
+  }
+  hipLaunchKernelGGL(k_selfreal, dim3(1), dim3(NTHR), 0, c->st, c->kp, dp<double>(c->d_spos), 0,
+                     dp<double>(c->d_qslt), dp<double>(c->d_selfreal));
+  TRY(launch_check(c, "k_selfreal"));
+  HIPCHK(c, hipMemcpyAsync(self_real, c->d_selfreal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+int32_t erm_engconst(erm_ctx *c, int32_t n, const int32_t *solutes, const double *solute_pos, const double *weights,
+                     double *uvengy) {
+  TRY(need(c, true, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  TRY(check_solutes(c, n, solutes, solute_pos));
+  if (c->P.slttype == 1 && solute_pos) return fail(c, ERM_E_UNSUPPORTED, "soln mode takes coordinates from the frame");
+  Batch bt{n, 0, c->P.slttype == 1 ? nullptr : solute_pos, c->P.slttype == 1 ? solutes : nullptr, weights, false, 0, 0};
+  return process(c, bt, DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF | DO_HIST, uvengy);
+}
+
+int32_t erm_insert_random(erm_ctx *c, int32_t n, uint64_t seed, uint64_t first_index, double *uvengy) {
+  TRY(need(c, true, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (c->P.slttype != 2) return fail(c, ERM_E_ARG, "random insertion needs a rigid test particle (slttype 2)");
+  if (c->bfcoord.empty()) return fail(c, ERM_E_ARG, "random insertion needs bfcoord");
+  if (n < 0) return fail(c, ERM_E_ARG, "negative count");
+  Batch bt{n, 0, nullptr, nullptr, nullptr, true, seed, first_index};
+  return process(c, bt, DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF | DO_HIST, uvengy);
+}
+
+int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, const double *weights,
+                             const double *uvengy) {
+  TRY(need(c, true, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (!uvengy || n < 0) return fail(c, ERM_E_ARG, "uvengy is NULL");
+  if (c->P.slttype == 1 && !solutes) return fail(c, ERM_E_ARG, "soln mode needs solute indices");
+  const int S = c->slvmax;
+  for (int s0 = 0; s0 < n; s0 += c->Bmax) {
+    int nb = std::min(c->Bmax, n - s0);
+    // energies into the sorted order
+    std::vector<double> uvs((size_t)nb * (S + 1));
+    for (int b = 0; b < nb; ++b) {
+      const double *src = uvengy + (size_t)(s0 + b) * (S + 1);
+      double *dst = uvs.data() + (size_t)b * (S + 1);
+      dst[0] = src[0];
+      for (int kk = 0; kk < S; ++kk) dst[1 + c->jof_k[kk]] = src[1 + kk];
+    }
+    TRY(ensure(c, c->d_uv, sizeof(double) * uvs.size()));
+    TRY(upload(c, c->d_uv, uvs.data(), uvs.size()));
+    std::vector<int> sltj(nb, -1);
+    if (solutes)
+      for (int b = 0; b < nb; ++b) {
+        int kk = c->kof[solutes[s0 + b]];
+        sltj[b] = kk >= 0 ? c->jof_k[kk] : -1;
+      }
+    TRY(ensure(c, c->d_sltj, sizeof(int) * nb));
+    TRY(upload(c, c->d_sltj, sltj.data(), nb));
+    std::vector<double> ws(nb, 1.0);
+    if (weights)
+      for (int b = 0; b < nb; ++b) ws[b] = weights[s0 + b];
+    TRY(ensure(c, c->d_wsol, sizeof(double) * nb));
+    TRY(upload(c, c->d_wsol, ws.data(), nb));
+    HIPCHK(c, hipEventRecord(c->ev[5], c->st));
+    TRY(run_histogram(c, nb, s0));
+  }
+  return ERM_OK;
+}
+
+int32_t erm_engstore(erm_ctx *c) {
+  TRY(need(c, false, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  // single process: edens/engnorm, ecorr/engnorm, eself/engnorm, avslf/engnorm
+  const int E = c->ermax;
+  std::vector<double> sc(8), e(E);
+  HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scal.p, sizeof(double) * 8, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(e.data(), c->d_edens.p, sizeof(double) * E, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  const double en = sc[0];
+  for (double &v : e) v = v / en;
+  TRY(upload(c, c->d_edens, e.data(), E));
+  if (c->P.corrcal) {
+    std::vector<double> m((size_t)E * E);
+    HIPCHK(c, hipMemcpyAsync(m.data(), c->d_ecorr.p, sizeof(double) * m.size(), hipMemcpyDeviceToHost, c->st));
+    TRY(sync(c));
+    for (double &v : m) v = v / en;
+    TRY(upload(c, c->d_ecorr, m.data(), m.size()));
+  }
+  if (c->esmax) {
+    std::vector<double> s(c->esmax);
+    HIPCHK(c, hipMemcpyAsync(s.data(), c->d_eself.p, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->st));
+    TRY(sync(c));
+    for (double &v : s) v = v / en;
+    TRY(upload(c, c->d_eself, s.data(), s.size()));
+  }
+  sc[2] = sc[2] / en;
+  TRY(upload(c, c->d_scal, sc.data(), 8));
+  return sync(c);
+}
+
+int32_t erm_fetch(erm_ctx *c, double *edens, double *ecorr, double *eself, double *minuv, double *maxuv,
+                  double *slnuv, double *scalars) {
+  TRY(need(c, false, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  const int E = c->ermax;
+  if (edens) HIPCHK(c, hipMemcpyAsync(edens, c->d_edens.p, sizeof(double) * E, hipMemcpyDeviceToHost, c->st));
+  if (ecorr) {
+    if (!c->P.corrcal) return fail(c, ERM_E_ARG, "corrcal is off");
+    HIPCHK(c, hipMemcpyAsync(ecorr, c->d_ecorr.p, sizeof(double) * (size_t)E * E, hipMemcpyDeviceToHost, c->st));
+  }
+  if (eself && c->esmax)
+    HIPCHK(c, hipMemcpyAsync(eself, c->d_eself.p, sizeof(double) * c->esmax, hipMemcpyDeviceToHost, c->st));
+  if (slnuv)
+    HIPCHK(c, hipMemcpyAsync(slnuv, c->d_slnuv.p, sizeof(double) * (c->numslv + 1), hipMemcpyDeviceToHost, c->st));
+  std::vector<unsigned long long> mm(2 * (c->numslv + 1));
+  HIPCHK(c, hipMemcpyAsync(mm.data(), c->d_mnmx.p, sizeof(unsigned long long) * mm.size(), hipMemcpyDeviceToHost, c->st));
+  double sc[8];
+  HIPCHK(c, hipMemcpyAsync(sc, c->d_scal.p, sizeof sc, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  auto dec = [](unsigned long long u, double dflt, bool empty) {
+    if (empty) return dflt;
+    u = (u & 0x8000000000000000ULL) ? (u & 0x7FFFFFFFFFFFFFFFULL) : ~u;
+    double d;
+    memcpy(&d, &u, sizeof d);
+    return d;
+  };
+  for (int i = 0; i <= c->numslv; ++i) {
+    if (minuv) minuv[i] = dec(mm[2 * i], 1.0e50, mm[2 * i] == ~0ULL);
+    if (maxuv) maxuv[i] = dec(mm[2 * i + 1], -1.0e50, mm[2 * i + 1] == 0ULL);
+  }
+  if (scalars) {
+    scalars[0] = sc[0];
+    scalars[1] = sc[1];
+    scalars[2] = sc[2];
+    scalars[3] = sc[3];
+  }
+  return ERM_OK;
+}
+
+int32_t erm_sizes(const erm_ctx *c, int32_t *slvmax, int32_t *ermax) {
+  if (!c) return ERM_E_ARG;
+  if (slvmax) *slvmax = c->slvmax;
+  if (ermax) *ermax = c->ermax;
+  return ERM_OK;
+}
+
+int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
+  if (!c || !ms8) return ERM_E_ARG;
+  for (int i = 0; i < 8; ++i) ms8[i] = c->stage_ms[i];
+  return ERM_OK;
+}
+
+}  // extern "C"

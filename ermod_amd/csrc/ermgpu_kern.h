@@ -1,0 +1,1044 @@
+// ermgpu_kern.h -- kernels of the solute-solvent energy pipeline (gfx950).
+//
+// Data layout in HBM (per context):
+//   frame, sorted by cell of each physical molecule's first site:
+//     mol_*[j]   molecule j of the sorted order: tagpt index k, first sorted
+//                site, site count, molecular charge, species
+//     s*[t]      sorted sites, SoA: wrapped x/y/z (f64), charge (T), compact
+//                LJ type, PME grid base (packed 3 x 10 bits) and the 3 x p
+//                B-spline weights (T)
+//     cell_start[c]  first sorted molecule of cell c (x fastest)
+//   batch of B solute configurations b:
+//     spos[b][s][3] (f64), solute spline data, Hk[b][z][m2][m1] (complex T,
+//     half spectrum after the z-convolution), phi[b][z][y][x] (T),
+//     uv[b][0..slvmax] (f64, sorted molecule order), hist[b][ermax] (f64)
+#pragma once
+
+#include "ermgpu_dev.h"
+
+namespace erm {
+
+struct Params {
+  // grid / PME
+  int K1, K2, K3, H, p;
+  FftAxis ax1, ax2, ax3;
+  double coeff[MAXP + 1];
+  double L[3], halfL[3], invcl[9], volume, screen;
+  // real space
+  int ljswitch;
+  double lw2, up2, el2, cut2, rcmax, lwljcut, upljcut;
+  double lw3, up3, lw6, up6, repA, repB, repC, attA, attB, attC;
+  int nb[3];          // molecule cell grid
+  double hcell[3];
+  // sizes
+  int slvmax, ns, nvt, ermax, numslv, esmax;
+  int slttype, wgtslf, estype, selfcal, force;
+  double temp, stat_weight_system, total_charge, solute_charge;
+};
+
+// --------------------------------------------------------------------------
+// per-frame preparation
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wrap1(double x, double L) { return x - L * floor(x * (1.0 / L)); }
+
+__device__ __forceinline__ int cell_index(const Params &P, const double *w) {
+  int c[3];
+  for (int d = 0; d < 3; ++d) {
+    int t = (int)floor(w[d] * (1.0 / P.L[d]) * (double)P.nb[d]);
+    t %= P.nb[d];
+    if (t < 0) t += P.nb[d];
+    c[d] = t;
+  }
+  return c[0] + P.nb[0] * (c[1] + P.nb[1] * c[2]);
+}
+
+// cell of each physical molecule (by its first site) and the largest
+// distance of a site from its molecule's first site
+__global__ void k_mol_cell(Params P, const double *__restrict__ pos, const int *__restrict__ tagpt,
+                           const int *__restrict__ mol_begin, int *cellid, int *count,
+                           unsigned long long *rm_ord) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P.slvmax) return;
+  int m = tagpt[k];
+  int a0 = mol_begin[m], a1 = mol_begin[m + 1];
+  double w[3];
+  for (int d = 0; d < 3; ++d) w[d] = wrap1(pos[3 * a0 + d], P.L[d]);
+  int c = cell_index(P, w);
+  cellid[k] = c;
+  atomicAdd(&count[c], 1);
+  double rm = 0.0;
+  for (int a = a0 + 1; a < a1; ++a) {
+    double s2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      double x = pos[3 * a + d] - pos[3 * a0 + d];
+      x -= P.L[d] * rint(x / P.L[d]);
+      s2 += x * x;
+    }
+    rm = fmax(rm, sqrt(s2));
+  }
+  atomicMax(rm_ord, dbl_to_ord(rm));
+}
+
+// exclusive scan of n ints by one workgroup of 1024 threads; out[n] = total
+__global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *out, int n) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int b0 = t * chunk;
+  int s = 0;
+  for (int i = 0; i < chunk; ++i)
+    if (b0 + i < n) s += in[b0 + i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : part[t - 1];
+  for (int i = 0; i < chunk; ++i)
+    if (b0 + i < n) {
+      int v = in[b0 + i];
+      out[b0 + i] = run;
+      run += v;
+    }
+  if (t == 1023) out[n] = part[1023];
+}
+
+__global__ void k_mol_scatter(Params P, const int *__restrict__ cellid, int *fill,
+                              const int *__restrict__ tagpt, const int *__restrict__ mol_begin,
+                              int *mol_k, int *mol_ns) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P.slvmax) return;
+  int j = atomicAdd(&fill[cellid[k]], 1);
+  int m = tagpt[k];
+  mol_k[j] = k;
+  mol_ns[j] = mol_begin[m + 1] - mol_begin[m];
+}
+
+// sorted site arrays: wrapped coordinates (realcal.F90:789-885, cuboid),
+// charges, compact LJ types and B-spline weights (calc_spline_molecule,
+// recpcal.F90:408-442)
+template <typename T>
+__global__ void k_site_fill(Params P, const double *__restrict__ pos, const int *__restrict__ tagpt,
+                            const int *__restrict__ mol_begin, const double *__restrict__ charge,
+                            const int *__restrict__ cvt, const double *__restrict__ mol_charge,
+                            const int *__restrict__ uvspec, const int *__restrict__ mol_k,
+                            const int *__restrict__ mol_first, double *sx, double *sy, double *sz,
+                            T *sq, int *sct, int *sg, T *sw, double *mol_q, int *mol_spec,
+                            int *mol_m) {
+#pragma clang fp contract(off)
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P.slvmax) return;
+  int m = tagpt[mol_k[j]];
+  mol_q[j] = mol_charge[m];
+  mol_spec[j] = uvspec[m];
+  mol_m[j] = m;
+  int t = mol_first[j];
+  const int Ks[3] = {P.K1, P.K2, P.K3};
+  for (int a = mol_begin[m]; a < mol_begin[m + 1]; ++a, ++t) {
+    const double *x = pos + 3 * a;
+    sx[t] = wrap1(x[0], P.L[0]);
+    sy[t] = wrap1(x[1], P.L[1]);
+    sz[t] = wrap1(x[2], P.L[2]);
+    sq[t] = (T)charge[a];
+    sct[t] = cvt[a];
+    int g[3];
+    for (int d = 0; d < 3; ++d) {
+      double f = P.invcl[d] * x[0] + P.invcl[3 + d] * x[1] + P.invcl[6 + d] * x[2];
+      f = f - floor(f);
+      double factor = f * (double)Ks[d];
+      // f - floor(f) may round to 1.0: factor == K gives the weights of
+      // base K, i.e. of base 0 (indices are taken mod K); NaN maps to 0
+      int rci = (factor >= 0.0 && factor <= (double)Ks[d]) ? (int)factor : 0;
+      for (int s = 0; s < P.p; ++s)
+        sw[((size_t)t * 3 + d) * P.p + s] = (T)spline_value_dev(P.coeff, P.p, factor - (double)(rci - s));
+      g[d] = rci % Ks[d];
+    }
+    sg[t] = g[0] | (g[1] << 10) | (g[2] << 20);
+  }
+}
+
+// --------------------------------------------------------------------------
+// solute configurations
+// --------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double rng_uniform(uint64_t seed, uint64_t index, uint32_t draw) {
+  uint64_t key = mix64(seed + 0x9E3779B97F4A7C15ULL);
+  uint64_t ctr = index * 256ULL + (uint64_t)draw;
+  uint64_t z = mix64(key ^ mix64(ctr + 0x632BE59BD9B4E019ULL));
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+__device__ void com_dev(int n, const double *pts, const double *mass, double *c) {
+#pragma clang fp contract(off)
+  double msum = 0.0;
+  for (int j = 0; j < n; ++j) msum = msum + mass[j];
+  for (int i = 0; i < 3; ++i) {
+    double d = 0.0;
+    for (int j = 0; j < n; ++j) d = d + pts[3 * j + i] * mass[j];
+    c[i] = d / msum;
+  }
+}
+
+__device__ void qprod_dev(const double *a, const double *b, double *p) {
+#pragma clang fp contract(off)
+  double dot = a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+  p[0] = a[0] * b[0] - dot;
+  double cr[3] = {a[2] * b[3] - a[3] * b[2], a[3] * b[1] - a[1] * b[3], a[1] * b[2] - a[2] * b[1]};
+  for (int i = 0; i < 3; ++i) p[1 + i] = cr[i] + a[0] * b[1 + i] + b[0] * a[1 + i];
+}
+
+// random test-particle pose (insertion.F90:133-163, 325-366) placed with the
+// reference's operation sequence (set_solute_origin, set_shift_com,
+// apply_orientation; bestfit.F90:49-100, 174-203)
+__global__ void k_pose(Params P, int n, uint64_t seed, uint64_t first, const double *__restrict__ bf,
+                       const double *__restrict__ mass, const double *__restrict__ cellm, double *spos) {
+#pragma clang fp contract(off)
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const int ns = P.ns;
+  uint64_t index = first + (uint64_t)b;
+  double r[3], q[4] = {1.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < 3; ++i) r[i] = rng_uniform(seed, index, (uint32_t)i);
+  for (int a = 0; a < 63; ++a) {
+    double t[4];
+    for (int i = 0; i < 4; ++i) t[i] = rng_uniform(seed, index, (uint32_t)(3 + 4 * a + i)) * 2.0 - 1.0;
+    double s = t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + t[3] * t[3];
+    if (s < 1.0) {
+      double nrm = sqrt(s);
+      for (int i = 0; i < 4; ++i) q[i] = t[i] / nrm;
+      break;
+    }
+  }
+  double *out = spos + (size_t)b * ns * 3;
+  for (int i = 0; i < 3 * ns; ++i) out[i] = bf[i];
+  double c[3];
+  com_dev(ns, out, mass, c);
+  for (int j = 0; j < ns; ++j)
+    for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + 0.0;
+  double com[3];
+  for (int i = 0; i < 3; ++i) com[i] = cellm[i] * r[0] + cellm[3 + i] * r[1] + cellm[6 + i] * r[2];
+  com_dev(ns, out, mass, c);
+  for (int j = 0; j < ns; ++j)
+    for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + com[i];
+  com_dev(ns, out, mass, c);
+  double cj[4] = {q[0], -q[1], -q[2], -q[3]};
+  for (int j = 0; j < ns; ++j) {
+    double q2[4] = {0.0, out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]};
+    double t[4], r2[4];
+    qprod_dev(q, q2, t);
+    qprod_dev(t, cj, r2);
+    for (int i = 0; i < 3; ++i) out[3 * j + i] = r2[1 + i] + c[i];
+  }
+}
+
+// Per configuration: reference point C (mean of the sites) and radius, the
+// B-spline data of every solute site, and its separable spectra
+//   what_d(m) = sum_s w_s exp(-2 pi i m ((g_d - s) mod K_d) / K_d)
+// for axis 1 (m < H) and axis 2 (m < K2).
+template <typename T>
+__global__ __launch_bounds__(128) void k_solute_prep(Params P, const double *__restrict__ spos,
+                                                     const double *__restrict__ qsolute,
+                                                     const typename C2<T>::type *__restrict__ tw1,
+                                                     const typename C2<T>::type *__restrict__ tw2,
+                                                     double *sref, int *sg3, T *sw3, T *sqv,
+                                                     typename C2<T>::type *w1h,
+                                                     typename C2<T>::type *w2h) {
+#pragma clang fp contract(off)
+  using V = typename C2<T>::type;
+  const int b = blockIdx.x, ns = P.ns, p = P.p;
+  const double *x = spos + (size_t)b * ns * 3;
+  __shared__ double red[128 * 3];
+  __shared__ int gsh[3 * 64];
+  __shared__ double wsh[3 * MAXP * 64];
+  // reference point
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    s0 += x[3 * s];
+    s1 += x[3 * s + 1];
+    s2 += x[3 * s + 2];
+  }
+  red[threadIdx.x * 3] = s0;
+  red[threadIdx.x * 3 + 1] = s1;
+  red[threadIdx.x * 3 + 2] = s2;
+  __syncthreads();
+  __shared__ double cpt[4];
+  if (threadIdx.x == 0) {
+    double a = 0, bb = 0, c = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) {
+      a += red[3 * i];
+      bb += red[3 * i + 1];
+      c += red[3 * i + 2];
+    }
+    cpt[0] = a / ns;
+    cpt[1] = bb / ns;
+    cpt[2] = c / ns;
+  }
+  __syncthreads();
+  double rmax = 0.0;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    double d0 = x[3 * s] - cpt[0], d1 = x[3 * s + 1] - cpt[1], d2 = x[3 * s + 2] - cpt[2];
+    rmax = fmax(rmax, sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+  }
+  red[threadIdx.x] = rmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) r = fmax(r, red[i]);
+    sref[4 * b + 0] = cpt[0];
+    sref[4 * b + 1] = cpt[1];
+    sref[4 * b + 2] = cpt[2];
+    sref[4 * b + 3] = r;
+  }
+  // splines, in chunks of 64 sites through LDS
+  const int Ks[3] = {P.K1, P.K2, P.K3};
+  for (int s0c = 0; s0c < ns; s0c += 64) {
+    int nc = min(64, ns - s0c);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nc * 3; t += blockDim.x) {
+      int s = s0c + t / 3, d = t % 3;
+      const double *xs = x + 3 * s;
+      double f = P.invcl[d] * xs[0] + P.invcl[3 + d] * xs[1] + P.invcl[6 + d] * xs[2];
+      f = f - floor(f);
+      double factor = f * (double)Ks[d];
+      int rci = (factor >= 0.0 && factor <= (double)Ks[d]) ? (int)factor : 0;
+      for (int k = 0; k < p; ++k)
+        wsh[((t / 3) * 3 + d) * MAXP + k] = spline_value_dev(P.coeff, p, factor - (double)(rci - k));
+      gsh[(t / 3) * 3 + d] = rci % Ks[d];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+      int s = s0c + t;
+      sg3[(size_t)b * ns + s] = gsh[t * 3 + 2];
+      sqv[(size_t)b * ns + s] = (T)qsolute[s];
+      for (int k = 0; k < p; ++k) sw3[((size_t)b * ns + s) * p + k] = (T)wsh[(t * 3 + 2) * MAXP + k];
+    }
+    // spectra along axes 1 and 2
+    int nm = P.H + P.K2;
+    for (int t = threadIdx.x; t < nc * nm; t += blockDim.x) {
+      int sl = t / nm, mm = t - sl * nm;
+      int d = (mm < P.H) ? 0 : 1;
+      int m = (d == 0) ? mm : mm - P.H;
+      int K = Ks[d];
+      const V *tw = (d == 0) ? tw1 : tw2;
+      int g = gsh[sl * 3 + d];
+      V acc;
+      acc.x = 0;
+      acc.y = 0;
+      for (int k = 0; k < p; ++k) {
+        int gi = ((g - k) % K + K) % K;
+        V w = tw[(int)(((long long)m * gi) % K)];
+        T wt = (T)wsh[(sl * 3 + d) * MAXP + k];
+        acc.x += wt * w.x;
+        acc.y += wt * w.y;
+      }
+      int s = s0c + sl;
+      if (d == 0)
+        w1h[((size_t)b * ns + s) * P.H + m] = acc;
+      else
+        w2h[((size_t)b * ns + s) * P.K2 + m] = acc;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// PME: z columns.  For every half-spectrum column (m1, m2):
+//   G(z)  = sum_s q_s what1_s(m1) what2_s(m2) w3_s(z)   (sparse in z)
+//   Qhat  = FFT_z(G)                     (the r2c transform of the spread grid)
+//   E_self += weight(m1) * engfac * |Qhat|^2             (recpcal.F90:385-394)
+//   Hk(z) = IFFT_z(engfac * Qhat)                         (recpcal.F90:398,402)
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_pme_zcol(Params P, int ncolwg, const T *__restrict__ engfac,
+                                                   const int *__restrict__ sg3, const T *__restrict__ sw3,
+                                                   const T *__restrict__ sqv,
+                                                   const typename C2<T>::type *__restrict__ w1h,
+                                                   const typename C2<T>::type *__restrict__ w2h,
+                                                   const typename C2<T>::type *__restrict__ tw3,
+                                                   typename C2<T>::type *Hk, double *selfp) {
+  using V = typename C2<T>::type;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  V *buf0 = reinterpret_cast<V *>(lds_raw);
+  V *buf1 = buf0 + ncolwg * P.K3;
+  __shared__ double red[NTHR];
+  const int b = blockIdx.y, ns = P.ns, p = P.p, K3 = P.K3, H = P.H;
+  const int ncol_tot = H * P.K2;
+  const int c0 = blockIdx.x * ncolwg;
+  const int nc = min(ncolwg, ncol_tot - c0);
+  for (int t = threadIdx.x; t < ncolwg * K3; t += blockDim.x) {
+    buf0[t].x = 0;
+    buf0[t].y = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < nc) {
+    const int c = c0 + threadIdx.x;
+    const int m1 = c % H, m2 = c / H;
+    V *col = buf0 + threadIdx.x * K3;
+    for (int s = 0; s < ns; ++s) {
+      size_t bs = (size_t)b * ns + s;
+      V a = cmul(w1h[bs * H + m1], w2h[bs * P.K2 + m2]);
+      T q = sqv[bs];
+      a.x *= q;
+      a.y *= q;
+      int g = sg3[bs];
+      for (int k = 0; k < p; ++k) {
+        int z = ((g - k) % K3 + K3) % K3;
+        T w = sw3[bs * p + k];
+        col[z].x += a.x * w;
+        col[z].y += a.y * w;
+      }
+    }
+  }
+  __syncthreads();
+  V *res = lds_fft(buf0, buf1, ncolwg, K3, 1, P.ax3, -1, tw3);
+  // Green's function and self energy
+  double self = 0.0;
+  for (int t = threadIdx.x; t < nc * K3; t += blockDim.x) {
+    int cl = t / K3, m3 = t - cl * K3;
+    int c = c0 + cl;
+    int m1 = c % H, m2 = c / H;
+    T g = engfac[((size_t)m3 * P.K2 + m2) * H + m1];
+    V v = res[cl * K3 + m3];
+    double wgt = (m1 == 0 || (P.K1 % 2 == 0 && m1 == H - 1)) ? 0.5 : 1.0;
+    self += wgt * (double)g * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+    v.x *= g;
+    v.y *= g;
+    res[cl * K3 + m3] = v;
+  }
+  red[threadIdx.x] = self;
+  __syncthreads();
+  V *other = (res == buf0) ? buf1 : buf0;
+  V *out = lds_fft(res, other, ncolwg, K3, 1, P.ax3, +1, tw3);
+  for (int s = NTHR / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) selfp[(size_t)b * gridDim.x + blockIdx.x] = red[0];
+  // Hk[b][z][m2][m1]: consecutive columns are contiguous for fixed z
+  for (int t = threadIdx.x; t < nc * K3; t += blockDim.x) {
+    int z = t / nc, cl = t - z * nc;
+    Hk[((size_t)b * K3 + z) * ncol_tot + c0 + cl] = out[cl * K3 + z];
+  }
+}
+
+// --------------------------------------------------------------------------
+// PME: one z plane per workgroup.  y inverse transform of the K2 x H block,
+// then the x c2r transform, two real rows per complex FFT
+// (Z = X_y + i X_{y+1} with Hermitian extension), into phi[b][z][y][x].
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_pme_plane(Params P, const typename C2<T>::type *__restrict__ Hk,
+                                                    const typename C2<T>::type *__restrict__ tw1,
+                                                    const typename C2<T>::type *__restrict__ tw2,
+                                                    T *phi) {
+  using V = typename C2<T>::type;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int K1 = P.K1, K2 = P.K2, H = P.H;
+  const int blk = K2 * H;
+  V *A = reinterpret_cast<V *>(lds_raw);
+  V *A2 = A + blk;
+  const int z = blockIdx.x, b = blockIdx.y;
+  const V *src = Hk + ((size_t)b * P.K3 + z) * blk;
+  for (int t = threadIdx.x; t < blk; t += blockDim.x) A[t] = src[t];
+  __syncthreads();
+  // columns m1 (stride 1), elements m2 (stride H)
+  V *Y = lds_fft(A, A2, H, 1, H, P.ax2, +1, tw2);
+  V *Z = (Y == A) ? A2 : A;
+  const int npair = (K2 + 1) / 2;
+  for (int t = threadIdx.x; t < npair * K1; t += blockDim.x) {
+    int pr = t / K1, x = t - pr * K1;
+    int y0 = 2 * pr, y1 = y0 + 1;
+    V a, c;
+    if (x < H) {
+      a = Y[y0 * H + x];
+      c = (y1 < K2) ? Y[y1 * H + x] : V{0, 0};
+    } else {
+      V aa = Y[y0 * H + (K1 - x)];
+      V cc = (y1 < K2) ? Y[y1 * H + (K1 - x)] : V{0, 0};
+      a.x = aa.x;
+      a.y = -aa.y;
+      c.x = cc.x;
+      c.y = -cc.y;
+    }
+    if (x == 0 || (K1 % 2 == 0 && x == K1 / 2)) {
+      a.y = 0;
+      c.y = 0;
+    }
+    V zz;
+    zz.x = a.x - c.y;
+    zz.y = a.y + c.x;
+    Z[pr * K1 + x] = zz;
+  }
+  __syncthreads();
+  V *Zo = (Z == A) ? A2 : A;
+  V *R = lds_fft(Z, Zo, npair, K1, 1, P.ax1, +1, tw1);
+  T *dst = phi + ((size_t)b * P.K3 + z) * (size_t)K2 * K1;
+  for (int t = threadIdx.x; t < K2 * K1; t += blockDim.x) {
+    int y = t / K1, x = t - y * K1;
+    V v = R[(y >> 1) * K1 + x];
+    dst[t] = (y & 1) ? v.y : v.x;
+  }
+}
+
+// --------------------------------------------------------------------------
+// PME gather (recpcal_energy, recpcal.F90:451-502) plus the neutralising
+// background (residual_ene, engproc.F90:859-869); writes the sorted-order
+// energies uv[b][1 + j] = -pi Q_s Q_j / (alpha^2 V) + E_recip.  Per-row partial
+// sums in T, per-molecule accumulation in FP64.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_gather(Params P, const T *__restrict__ phi,
+                                                 const int *__restrict__ mol_first,
+                                                 const int *__restrict__ mol_ns,
+                                                 const double *__restrict__ mol_q,
+                                                 const int *__restrict__ sg, const T *__restrict__ sw,
+                                                 const T *__restrict__ sq, const int *__restrict__ slt_j,
+                                                 double *uv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= P.slvmax) return;
+  double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
+  if (j == slt_j[b]) {
+    u[j] = 0.0;
+    return;
+  }
+  const int K1 = P.K1, K2 = P.K2, K3 = P.K3, p = P.p;
+  const T *ph = phi + (size_t)b * K3 * K2 * K1;
+  double e = 0.0;
+  const int t0 = mol_first[j], nsm = mol_ns[j];
+  for (int t = t0; t < t0 + nsm; ++t) {
+    const int gp = sg[t];
+    const int g1 = gp & 1023, g2 = (gp >> 10) & 1023, g3 = (gp >> 20) & 1023;
+    const T *w = sw + (size_t)t * 3 * p;
+    const T chr = sq[t];
+    for (int c3 = 0; c3 < p; ++c3) {
+      T f1 = chr * w[2 * p + c3];
+      int r3 = g3 - c3;
+      r3 += (r3 < 0) ? K3 : 0;
+      for (int c2 = 0; c2 < p; ++c2) {
+        T f2 = f1 * w[p + c2];
+        int r2 = g2 - c2;
+        r2 += (r2 < 0) ? K2 : 0;
+        const T *row = ph + ((size_t)r3 * K2 + r2) * K1;
+        T rs = 0;
+        if (g1 >= p - 1) {
+          for (int c1 = 0; c1 < p; ++c1) rs += w[c1] * row[g1 - c1];
+        } else {
+          for (int c1 = 0; c1 < p; ++c1) {
+            int r1 = g1 - c1;
+            r1 += (r1 < 0) ? K1 : 0;
+            rs += w[c1] * row[r1];
+          }
+        }
+        e += (double)(f2 * rs);
+      }
+    }
+  }
+  double epcl = 3.1415926535897932 * P.solute_charge * mol_q[j] / P.screen / P.screen / P.volume;
+  u[j] = (0.0 - epcl) + e;
+}
+
+// --------------------------------------------------------------------------
+// Real space (realcal_proc, realcal.F90:58-135 / get_pair_energy_block
+// 585-784).  One workgroup slice per configuration: the molecules whose
+// first site lies within rc + R_solute + R_molecule of the configuration's
+// centre are found from the cell list by pillars (y, z) with an x range per
+// pillar; each candidate molecule is owned by one thread, which sums every
+// solute-site x molecule-site pair under the minimum image and adds the
+// result to uv[b][1 + j].
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T lj_low_t(const Params &P, T eps, T sig2, T dist) {
+  T invr2 = sig2 / dist;
+  T invr6 = invr2 * invr2 * invr2;
+  if (P.ljswitch <= 1) return (T)4 * eps * invr6 * (invr6 - (T)1);
+  T sg6 = sig2 * sig2 * sig2;
+  if (P.ljswitch == 2) {
+    T vdwa = invr6 * invr6 - sg6 * sg6 / (T)(P.lw6 * P.up6);
+    T vdwb = invr6 - sg6 / (T)(P.lw3 * P.up3);
+    return (T)4 * eps * (vdwa - vdwb);
+  }
+  T vdwa = invr6 * invr6 - sg6 * sg6 * (T)P.repC;
+  T vdwb = invr6 - sg6 * (T)P.attC;
+  return (T)4 * eps * (vdwa - vdwb);
+}
+
+template <typename T>
+__device__ __forceinline__ T lj_switch_t(const Params &P, T eps, T sig2, T dist) {
+  T invr2 = sig2 / dist;
+  T invr6 = invr2 * invr2 * invr2;
+  if (P.ljswitch == 0) {
+    T up2 = (T)P.up2, lw2 = (T)P.lw2;
+    T d = dist - up2, w = up2 - lw2;
+    return (T)4 * eps * invr6 * (invr6 - (T)1) * ((T)2 * dist + up2 - (T)3 * lw2) * (d * d) / (w * w * w);
+  }
+  if (P.ljswitch == 1) {
+    T sw = (sqrt(dist) - (T)P.lwljcut) / (T)(P.upljcut - P.lwljcut);
+    T sw3 = sw * sw * sw;
+    return (T)4 * eps * invr6 * (invr6 - (T)1) * ((T)1 - (T)10 * sw3 + (T)15 * (sw3 * sw) - (T)6 * (sw3 * sw * sw));
+  }
+  T sg6 = sig2 * sig2 * sig2;
+  if (P.ljswitch == 2) {
+    T invr3 = sqrt(invr6);
+    T sg3 = sqrt(sg6);
+    T a = invr6 - sg6 / (T)P.up6, c = invr3 - sg3 / (T)P.up3;
+    T vdwa = (T)(P.up6 / (P.up6 - P.lw6)) * (a * a);
+    T vdwb = (T)(P.up3 / (P.up3 - P.lw3)) * (c * c);
+    return (T)4 * eps * (vdwa - vdwb);
+  }
+  T sw = sqrt(dist) - (T)P.lwljcut;
+  T sw3 = sw * sw * sw, sw4 = sw3 * sw;
+  T vdwa = invr6 * invr6 - sg6 * sg6 * ((T)P.repA * sw3 + (T)P.repB * sw4 + (T)P.repC);
+  T vdwb = invr6 - sg6 * ((T)P.attA * sw3 + (T)P.attB * sw4 + (T)P.attC);
+  return (T)4 * eps * (vdwa - vdwb);
+}
+
+template <typename T> __device__ __forceinline__ T coul_t(T qq, T r, T alpha);
+template <> __device__ __forceinline__ float coul_t<float>(float qq, float r, float alpha) {
+  return qq * erfcf(alpha * r) / r;
+}
+// FP64 mirrors the reference's (1 - erf(alpha r)) / r
+template <> __device__ __forceinline__ double coul_t<double>(double qq, double r, double alpha) {
+  return qq * (1.0 - erf(alpha * r)) / r;
+}
+
+// inclusive prefix sum of a[0..n) in LDS, n <= 8 * blockDim.x; all threads call
+__device__ void block_inclusive_scan(int *a, int n) {
+  __shared__ int part[NTHR];
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int i = 0; i < per; ++i)
+    if (b0 + i < n) s += a[b0 + i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    int v = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = (threadIdx.x == 0) ? 0 : part[threadIdx.x - 1];
+  for (int i = 0; i < per; ++i)
+    if (b0 + i < n) {
+      run += a[b0 + i];
+      a[b0 + i] = run;
+    }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restrict__ spos,
+                                               const double *__restrict__ sref,
+                                               const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
+                                               const int *__restrict__ cell_start,
+                                               const int *__restrict__ mol_first,
+                                               const int *__restrict__ mol_ns, const double *__restrict__ sx,
+                                               const double *__restrict__ sy, const double *__restrict__ sz,
+                                               const T *__restrict__ sq, const int *__restrict__ sct,
+                                               const int *__restrict__ slt_j, const double *__restrict__ rm_ord_p,
+                                               double *uv) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int ns = P.ns, nvt = P.nvt;
+  T *ux = reinterpret_cast<T *>(lds_raw);
+  T *uy = ux + ns;
+  T *uz = uy + ns;
+  T *uq = uz + ns;
+  T *ulj = uq + ns;                       // [ns][nvt][2] (eps, sig2)
+  int *seg_beg = reinterpret_cast<int *>(ulj + 2 * ns * nvt);
+  int *seg_pre = seg_beg + MAXSEG;        // MAXSEG + 1 prefix sums
+  __shared__ int nseg;
+  const int b = blockIdx.x;
+  const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
+  const double Rs = sref[4 * b + 3];
+  const double *xs = spos + (size_t)b * ns * 3;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    ux[s] = (T)(xs[3 * s] - C0);
+    uy[s] = (T)(xs[3 * s + 1] - C1);
+    uz[s] = (T)(xs[3 * s + 2] - C2v);
+    uq[s] = slt_q[s];
+  }
+  for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
+  if (threadIdx.x == 0) nseg = 0;
+  __syncthreads();
+  // pillars within reach of the centre
+  const double Rm = ord_to_dbl(*(const unsigned long long *)rm_ord_p);
+  const double R = P.rcmax + Rs + Rm + 1e-6 * (1.0 + P.rcmax);
+  const double R2 = R * R;
+  const double Cw[3] = {wrap1(C0, P.L[0]), wrap1(C1, P.L[1]), wrap1(C2v, P.L[2])};
+  int lo[3], cnt[3];
+  for (int d = 1; d < 3; ++d) {
+    int n = (int)ceil(R / P.hcell[d]) + 1;
+    if (2 * n + 1 >= P.nb[d]) {
+      lo[d] = 0;
+      cnt[d] = P.nb[d];
+    } else {
+      lo[d] = (int)floor(Cw[d] / P.hcell[d]) - n;
+      cnt[d] = 2 * n + 1;
+    }
+  }
+  for (int t = threadIdx.x; t < cnt[1] * cnt[2]; t += blockDim.x) {
+    int oy = t % cnt[1], oz = t / cnt[1];
+    int yc = ((lo[1] + oy) % P.nb[1] + P.nb[1]) % P.nb[1];
+    int zc = ((lo[2] + oz) % P.nb[2] + P.nb[2]) % P.nb[2];
+    double dyz[2];
+    const int cc[2] = {yc, zc};
+    for (int e = 0; e < 2; ++e) {
+      int d = e + 1;
+      double ctr = ((double)cc[e] + 0.5) * P.hcell[d];
+      double dd = fabs(Cw[d] - ctr);
+      dd = fmin(dd, P.L[d] - dd);
+      dyz[e] = fmax(0.0, dd - 0.5 * P.hcell[d]);
+    }
+    double l2 = dyz[0] * dyz[0] + dyz[1] * dyz[1];
+    if (l2 > R2) continue;
+    double w = sqrt(R2 - l2);
+    int xa = (int)floor((Cw[0] - w) / P.hcell[0]);
+    int xb = (int)floor((Cw[0] + w) / P.hcell[0]);
+    int rowb = P.nb[0] * (yc + P.nb[1] * zc);
+    int n0 = P.nb[0];
+    int a0[2], a1[2], nsg = 0;
+    if (xb - xa + 1 >= n0) {
+      a0[0] = 0;
+      a1[0] = n0;
+      nsg = 1;
+    } else {
+      int xw = ((xa % n0) + n0) % n0;
+      int len = xb - xa + 1;
+      if (xw + len <= n0) {
+        a0[0] = xw;
+        a1[0] = xw + len;
+        nsg = 1;
+      } else {
+        a0[0] = xw;
+        a1[0] = n0;
+        a0[1] = 0;
+        a1[1] = xw + len - n0;
+        nsg = 2;
+      }
+    }
+    for (int g = 0; g < nsg; ++g) {
+      int mb = cell_start[rowb + a0[g]], me = cell_start[rowb + a1[g]];
+      if (me > mb) {
+        int slot = atomicAdd(&nseg, 1);
+        if (slot < MAXSEG) {
+          seg_beg[slot] = mb;
+          seg_pre[slot + 1] = me - mb;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int nsg = min(nseg, MAXSEG);
+  if (threadIdx.x == 0) seg_pre[0] = 0;
+  block_inclusive_scan(seg_pre + 1, nsg);
+  const int total = seg_pre[nsg];
+  const int sj = slt_j[b];
+  const T L0 = (T)P.L[0], L1 = (T)P.L[1], L2 = (T)P.L[2];
+  const T h0 = (T)P.halfL[0], h1 = (T)P.halfL[1], h2 = (T)P.halfL[2];
+  const T lw2 = (T)P.lw2, up2 = (T)P.up2, el2 = (T)P.el2, cut2 = (T)P.cut2, alpha = (T)P.screen;
+  double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
+  for (int idx = blockIdx.y * blockDim.x + threadIdx.x; idx < total; idx += gridDim.y * blockDim.x) {
+    int lo_s = 0, hi_s = nsg;  // last segment with seg_pre <= idx
+    while (hi_s - lo_s > 1) {
+      int mid = (lo_s + hi_s) >> 1;
+      if (seg_pre[mid] <= idx)
+        lo_s = mid;
+      else
+        hi_s = mid;
+    }
+    const int j = seg_beg[lo_s] + (idx - seg_pre[lo_s]);
+    if (j == sj) continue;
+    double e = 0.0;
+    const int t0 = mol_first[j], nsm = mol_ns[j];
+    for (int t = t0; t < t0 + nsm; ++t) {
+      double dx = sx[t] - C0, dy = sy[t] - C1, dz = sz[t] - C2v;
+      dx -= P.L[0] * rint(dx / P.L[0]);
+      dy -= P.L[1] * rint(dy / P.L[1]);
+      dz -= P.L[2] * rint(dz / P.L[2]);
+      const T vx = (T)dx, vy = (T)dy, vz = (T)dz;
+      const T qv = sq[t];
+      const int cv = sct[t];
+      for (int s = 0; s < ns; ++s) {
+        T ax = fabs(vx - ux[s]), ay = fabs(vy - uy[s]), az = fabs(vz - uz[s]);
+        ax = (ax > h0) ? L0 - ax : ax;
+        ay = (ay > h1) ? L1 - ay : ay;
+        az = (az > h2) ? L2 - az : az;
+        T r2 = ax * ax + ay * ay + az * az;
+        if (r2 <= cut2) {
+          T ep = 0;
+          T eps = ulj[(s * nvt + cv) * 2], sig2 = ulj[(s * nvt + cv) * 2 + 1];
+          if (eps > (T)0) {
+            if (r2 <= lw2)
+              ep += lj_low_t<T>(P, eps, sig2, r2);
+            else if (r2 <= up2)
+              ep += lj_switch_t<T>(P, eps, sig2, r2);
+          }
+          if (r2 <= el2) ep += coul_t<T>(uq[s] * qv, sqrt(r2), alpha);
+          e += (double)ep;
+        }
+      }
+    }
+    u[j] += e;
+  }
+}
+
+// realcal_self (realcal.F90:284-328) of configuration b, FP64
+__global__ __launch_bounds__(NTHR) void k_selfreal(Params P, const double *__restrict__ spos, int b,
+                                                   const double *__restrict__ qsolute, double *out) {
+  __shared__ double red[NTHR];
+  const int ns = P.ns;
+  const double *x = spos + (size_t)b * ns * 3;
+  double e = 0.0;
+  const double a = P.screen;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    double qi = qsolute[i];
+    e += -qi * qi * a / sqrt(3.1415926535897932);
+  }
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    for (int jj = i + 1; jj < ns; ++jj) {
+      double d2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        double dd = wrap1(x[3 * jj + d], P.L[d]) - wrap1(x[3 * i + d], P.L[d]);
+        dd = P.halfL[d] - fabs(P.halfL[d] - fabs(dd));
+        d2 += dd * dd;
+      }
+      double r = sqrt(d2);
+      e += -qsolute[i] * qsolute[jj] * erf(a * r) / r;
+    }
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = NTHR / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// uvengy(0) = self_recp + realcal_self + residual_self (engproc.F90:701-732)
+__global__ void k_u0(Params P, int n, int ngrp, const double *__restrict__ selfp,
+                     const double *__restrict__ selfreal, int per_item, double *uv) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  double s = 0.0;
+  for (int g = 0; g < ngrp; ++g) s += selfp[(size_t)b * ngrp + g];
+  double pairep = per_item ? selfreal[b] : selfreal[0];
+  double epcl = 3.1415926535897932 * P.solute_charge * P.solute_charge / P.screen / P.screen / P.volume;
+  epcl = epcl / 2.0;
+  double res = 0.0 - epcl;
+  uv[(size_t)b * (P.slvmax + 1)] = s + pairep + res;
+}
+
+// --------------------------------------------------------------------------
+// update_histogram (engproc.F90:758-856): bins of every molecule of
+// configuration b (getiduv / binsearch, engproc.F90:927-1020), the
+// instantaneous histogram, per-species sums and extrema.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void set_error(int *status, int code, int b, int j, double v) {
+  if (atomicCAS(&status[0], 0, code) == 0) {
+    status[1] = b;
+    status[2] = j;
+    *reinterpret_cast<double *>(&status[4]) = v;
+  }
+}
+
+__global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict__ uv,
+                                              const double *__restrict__ edges,
+                                              const int *__restrict__ spec_off, const int *__restrict__ uvmax,
+                                              const int *__restrict__ uvsoft, const int *__restrict__ mol_spec,
+                                              const int *__restrict__ slt_j, double *hist, double *flceng,
+                                              unsigned long long *mnmx, int *status) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *e = reinterpret_cast<double *>(lds_raw);
+  int *h = reinterpret_cast<int *>(e + P.ermax);
+  __shared__ double flc[MAXSPEC];
+  __shared__ unsigned long long mn[MAXSPEC], mx[MAXSPEC];
+  const int b = blockIdx.x;
+  for (int t = threadIdx.x; t < P.ermax; t += blockDim.x) {
+    e[t] = edges[t];
+    h[t] = 0;
+  }
+  if (threadIdx.x < MAXSPEC) {
+    flc[threadIdx.x] = 0.0;
+    mn[threadIdx.x] = ~0ULL;
+    mx[threadIdx.x] = 0ULL;
+  }
+  __syncthreads();
+  const double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
+  const int sj = slt_j[b];
+  for (int j = threadIdx.x; j < P.slvmax; j += blockDim.x) {
+    if (j == sj) continue;
+    int pti = mol_spec[j];
+    if (pti <= 0 || pti > P.numslv) {
+      set_error(status, 8, b, j, 0.0);
+      continue;
+    }
+    double v = u[j];
+    int off = spec_off[pti - 1], n = uvmax[pti - 1];
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (e[off + mid] < v)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo <= 0) {
+      set_error(status, 1, b, j, v);
+      continue;
+    }
+    if (P.slttype == 1 && lo > uvsoft[pti - 1]) {
+      set_error(status, 2, b, j, v);
+      continue;
+    }
+    if (lo == n && v < 1.0e3 && !P.force) set_error(status, 3, b, j, v);
+    atomicAdd(&h[off + lo - 1], 1);
+    atomicAdd(&flc[pti - 1], v);
+    atomicMin(&mn[pti - 1], dbl_to_ord(v));
+    atomicMax(&mx[pti - 1], dbl_to_ord(v));
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < P.ermax; t += blockDim.x) hist[(size_t)b * P.ermax + t] = (double)h[t];
+  if (threadIdx.x < P.numslv) {
+    flceng[(size_t)b * MAXSPEC + threadIdx.x] = flc[threadIdx.x];
+    atomicMin(&mnmx[2 * (threadIdx.x + 1)], mn[threadIdx.x]);
+    atomicMax(&mnmx[2 * (threadIdx.x + 1) + 1], mx[threadIdx.x]);
+  }
+}
+
+// Sequential scalar part of update_histogram over the configurations of a
+// batch, in order: weights (wgtslf, volcorrect), engnorm, engsmpl, avslf,
+// the self-energy histogram and its extrema (engproc.F90:776-808).
+// scal: [0] engnorm [1] engsmpl [2] avslf [3] voffset [4] voffset set
+__global__ void k_scalars(Params P, int n, const double *__restrict__ uv, const double *__restrict__ wsol,
+                          const double *__restrict__ escrd, double *scal, double *eself,
+                          unsigned long long *mnmx, double *wout, int *status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int b = 0; b < n; ++b) {
+    double u0 = uv[(size_t)b * (P.slvmax + 1)];
+    double engnmfc;
+    if (P.wgtslf == 0) {
+      engnmfc = 1.0;
+    } else {
+      if (scal[4] == 0.0) {
+        scal[3] = u0;
+        scal[4] = 1.0;
+      }
+      double f = u0 - scal[3];
+      engnmfc = (P.slttype == 1) ? exp(f / P.temp) : exp(-f / P.temp);
+    }
+    double tw = P.stat_weight_system * (wsol ? wsol[b] : 1.0);
+    if (P.estype == 2) {
+      tw = tw * P.volume;
+      double factor = 3.1415926535897932 * (P.total_charge * P.total_charge) / (P.screen * P.screen) /
+                      P.volume / 2.0;
+      tw = tw * exp(factor / P.temp);
+    }
+    engnmfc = engnmfc * tw;
+    scal[0] = scal[0] + engnmfc;
+    scal[1] = scal[1] + 1.0;
+    scal[2] = scal[2] + tw;
+    if (P.selfcal) {
+      int lo = 0, hi = P.esmax;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (escrd[mid] < u0)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo <= 0) {
+        set_error(status, 1, b, -1, u0);
+      } else {
+        if (lo == P.esmax && u0 < 1.0e3 && !P.force) set_error(status, 3, b, -1, u0);
+        eself[lo - 1] = eself[lo - 1] + engnmfc;
+      }
+    }
+    unsigned long long o = dbl_to_ord(u0);
+    if (o < mnmx[0]) mnmx[0] = o;
+    if (o > mnmx[1]) mnmx[1] = o;
+    wout[b] = engnmfc;
+  }
+}
+
+// edens(i) += engnmfc * insdst(i), configurations in order (engproc.F90:836-840)
+__global__ void k_edens(int n, int ermax, const double *__restrict__ hist, const double *__restrict__ w,
+                        double *edens) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ermax) return;
+  double e = edens[i];
+  for (int b = 0; b < n; ++b) {
+    double h = hist[(size_t)b * ermax + i];
+    if (h != 0.0) e = e + w[b] * h;
+  }
+  edens[i] = e;
+}
+
+// slnuv(:) += flceng(:, cntdst) * engnmfc (engproc.F90:832-834)
+__global__ void k_slnuv(int n, int numslv, const double *__restrict__ flceng, const double *__restrict__ w,
+                        double *slnuv) {
+  int s = threadIdx.x;
+  if (s >= numslv) return;
+  double v = slnuv[s + 1];
+  for (int b = 0; b < n; ++b) v = v + flceng[(size_t)b * MAXSPEC + s] * w[b];
+  slnuv[s + 1] = v;
+}
+
+// --------------------------------------------------------------------------
+// Correlation matrix on FP64 MFMA: ecorr(p, i) += sum_b (w_b h_b[i]) h_b[p]
+// (engproc.F90:841-853), one 16 x 16 tile per wave, v_mfma_f64_16x16x4f64
+// over the batch in steps of 4 configurations.  ecorr column-major
+// (element (p, i) at p + i * ermax).
+// --------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__restrict__ hist,
+                                              const double *__restrict__ w, double *ecorr) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nt = (ermax + 15) / 16;
+  if (wave >= nt * nt) return;
+  const int tp = wave % nt, ti = wave / nt;  // tile rows p, columns i
+  const int p0 = tp * 16, i0 = ti * 16;
+  const int r = lane & 15, kq = lane >> 4;
+  // D layout: lane holds column i0 + (lane & 15), rows p0 + 4 (lane >> 4) + v
+  dbl4 acc;
+  for (int v = 0; v < 4; ++v) {
+    int pr = p0 + 4 * kq + v, ic = i0 + r;
+    acc[v] = (pr < ermax && ic < ermax) ? ecorr[(size_t)ic * ermax + pr] : 0.0;
+  }
+  for (int b0 = 0; b0 < n; b0 += 4) {
+    int b = b0 + kq;
+    double a = 0.0, bb = 0.0;
+    if (b < n) {
+      if (p0 + r < ermax) a = hist[(size_t)b * ermax + p0 + r];
+      if (i0 + r < ermax) bb = w[b] * hist[(size_t)b * ermax + i0 + r];
+    }
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+  }
+  for (int v = 0; v < 4; ++v) {
+    int pr = p0 + 4 * kq + v, ic = i0 + r;
+    if (pr < ermax && ic < ermax) ecorr[(size_t)ic * ermax + pr] = acc[v];
+  }
+}
+
+// uvengy in tagpt order from the sorted order
+__global__ void k_unsort(int n, int slvmax, const int *__restrict__ mol_k, const double *__restrict__ uv,
+                         double *out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (b >= n) return;
+  const double *u = uv + (size_t)b * (slvmax + 1);
+  double *o = out + (size_t)b * (slvmax + 1);
+  if (j == 0) o[0] = u[0];
+  if (j < slvmax) o[1 + mol_k[j]] = u[1 + j];
+}
+
+}  // namespace erm

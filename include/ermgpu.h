@@ -1,0 +1,196 @@
+/*
+ * ermgpu.h -- C-ABI of the MI355X (gfx950) solute-solvent energy pipeline.
+ *
+ * Drop-in for ERmod's per-frame energy path: the module procedures of
+ * engproc.F90 / realcal.F90 / recpcal.F90 that engconst drives
+ * (engproc.F90:301-448).  A Fortran host binds every entry point through
+ * ISO_C_BINDING (see ermod_amd/fortran/ermgpu.F90); Python binds it through
+ * ctypes (ermod_amd/api.py).  No torch or HIP types appear here: plain
+ * pointers, sizes and int32 status codes.
+ *
+ * Conventions
+ *  - all indices are 0-based; mol_begin has nmol+1 entries;
+ *  - coordinates are xyz-interleaved doubles (site i at pos[3*i .. 3*i+2]);
+ *  - 3x3 matrices are column-major (m[3*j+i] = M(i+1,j+1)), the Fortran
+ *    layout of engmain's `cell` (columns are the box vectors);
+ *  - uvengy arrays hold slvmax+1 doubles: [0] is the solute self energy,
+ *    [k+1] the energy with the k-th physical molecule (tagpt order,
+ *    engproc.F90:336-347);
+ *  - host arrays are caller-owned and copied; device memory is owned by the
+ *    context; calls on one context are serialised on its HIP stream and must
+ *    come from one thread at a time;
+ *  - every call returns ERM_OK or an error code; nothing aborts the process.
+ *    erm_last_error() returns the message of the last failing call.
+ */
+#ifndef ERMGPU_H
+#define ERMGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1-9 are the hot path's halt_with_error / warning codes
+ * (mpiproc.F90:203-264) returned instead of aborting. */
+enum erm_status {
+  ERM_OK = 0,
+  ERM_E_ENG_MIN = 1,     /* energy at or below the lowest edge   engproc.F90:992-995   */
+  ERM_E_ENG_SFT = 2,     /* soln: above the soft region           engproc.F90:1000-1005 */
+  ERM_E_MBIN = 3,        /* overflow bin below 1e3 (warning)      engproc.F90:1010-1015 */
+  ERM_E_ENG_ECD = 4,     /* inconsistent &hist                    engproc.F90:185       */
+  ERM_E_RCP_CNS = 5,     /* molecule without spline slot          recpcal.F90:333       */
+  ERM_E_NONCUBOID = 6,   /* non-cuboid cell                       realcal.F90:859-864   */
+  ERM_E_ENG_BUG = 7,     /* internal inconsistency                eng_bug               */
+  ERM_E_ENG_CNS = 8,     /* solvent species <= 0                  engproc.F90:812       */
+  ERM_E_ENG_SIZ = 9,     /* too many bins                         engproc.F90:194       */
+  ERM_E_ARG = 10,        /* invalid argument or call order */
+  ERM_E_UNSUPPORTED = 11,/* option outside this implementation */
+  ERM_E_DEVICE = 12      /* HIP runtime error */
+};
+
+enum erm_precision { ERM_FP64 = 0, ERM_FP32 = 1 };
+
+/* &ene_param values the hot path reads (engmain.F90:331-343), after
+ * setparam's defaulting (setconf.F90:144-453). */
+typedef struct erm_params {
+  int32_t slttype;        /* 1 soln, 2 refs rigid, 3 refs flexible */
+  int32_t estype;         /* 1 NVT, 2 NPT (volcorrect weights) */
+  int32_t ljswitch;       /* 0 CHARMM pot, 1 GMX pot, 2 CHARMM force, 3 GMX force */
+  int32_t splodr;         /* PME B-spline order, 2..12 */
+  int32_t ms[3];          /* PME grid ms1max..ms3max; factors 2,3,5 only, <= 256 */
+  int32_t corrcal;        /* accumulate the correlation matrix */
+  int32_t selfcal;        /* accumulate the self-energy distribution */
+  int32_t wgtslf;         /* weight by the self energy */
+  int32_t force_calculation; /* continue past the mbin warning */
+  double elecut, lwljcut, upljcut, screen, temp, block_threshold;
+  int32_t precision;      /* ERM_FP64 or ERM_FP32 pair/PME arithmetic */
+  int32_t max_batch;      /* insertions per device batch; 0 picks a default */
+  int32_t device;         /* HIP device ordinal */
+} erm_params;
+
+/* &hist namelist of one species (engproc.F90:66-67) */
+typedef struct erm_hist {
+  double ecdmin, ecfmns, ecdcen, eccore, ecdmax;
+  double eclbin, ecfbin, ec0bin, finfac;
+  int32_t pecore;
+} erm_hist;
+
+typedef struct erm_ctx erm_ctx;
+
+/* ---- energy coordinate ------------------------------------------------ */
+/* Bin edges of one species from its &hist; replaces the edge construction of
+ * enginit (engproc.F90:124-237).  self_energy != 0 returns the pesoft edges
+ * of the solute self-energy coordinate. Host-only. */
+int32_t erm_build_edges(const erm_hist *h, int32_t self_energy, double *ercrd, int32_t cap,
+                        int32_t *pemax, int32_t *pesoft);
+
+/* hash of FP64 bit patterns; replaces hash_double_ (hash_real.c:5-15) used by
+ * get_solute_hash (engproc.F90:1176-1182). Host-only. */
+uint64_t erm_hash_double(const double *v, int32_t n);
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on params->device; replaces recpcal_init's setup
+ * (recpcal.F90:42-85) and the parameter part of enginit. */
+int32_t erm_create(const erm_params *params, erm_ctx **out);
+void erm_destroy(erm_ctx *ctx);
+const char *erm_last_error(const erm_ctx *ctx);
+
+/* Topology: the engmain arrays setparam fills (setconf.F90:706-834):
+ * scaled charges, 0-based LJ types, ntype*ntype LJ tables (epsilon and
+ * sigma^2, symmetric), sluvid, solvent species (uvspec, engproc.F90:96-103),
+ * molecular charges and site masses.  bfcoord (3*nsite of the test particle)
+ * is required for erm_insert_random, else may be NULL. */
+int32_t erm_set_topology(erm_ctx *ctx, int32_t natom, int32_t nmol, const int32_t *mol_begin,
+                         const double *charge, const int32_t *ljtype, int32_t ntype,
+                         const double *lj_eps, const double *lj_sig2, const int32_t *sluvid,
+                         const int32_t *uvspec, const double *mol_charge, const double *sitemass,
+                         const double *bfcoord);
+
+/* Energy coordinate: enginit's output (engproc.F90:227-247); uvcrd holds
+ * sum(uvmax) edges, escrd esmax edges (NULL when selfcal = 0). */
+int32_t erm_set_bins(erm_ctx *ctx, int32_t numslv, const int32_t *uvmax, const int32_t *uvsoft,
+                     const double *uvcrd, int32_t esmax, const double *escrd);
+
+/* Zeroes the accumulators of a division; replaces engclear
+ * (engproc.F90:276-288). */
+int32_t erm_engclear(erm_ctx *ctx);
+
+/* One frame: cell inverse and volume (get_inverted_cell, engproc.F90:898-919),
+ * Green's function when the cell changed (recpcal_spline_greenfunc,
+ * recpcal.F90:111-152), solvent B-spline weights (recpcal_prepare_solvent,
+ * recpcal.F90:328-340), coordinate normalisation (realcal_prepare,
+ * realcal.F90:137-146) and the cell list (set_block_info / blockify /
+ * sort_block, realcal.F90:373-525), built once per frame on the device.
+ * sitepos: natom*3 host doubles (test-particle slots ignored). */
+int32_t erm_set_frame(erm_ctx *ctx, const double *sitepos, const double *cell,
+                      double stat_weight_system);
+/* Same, with sitepos already in device memory (natom*3 doubles). */
+int32_t erm_set_frame_device(erm_ctx *ctx, const double *d_sitepos, const double *cell,
+                             double stat_weight_system);
+
+/* get_uv_energy (engproc.F90:662-756) for a batch of n solute
+ * configurations without touching the accumulators.  soln mode: solute is
+ * the molecule index and solute_pos may be NULL (coordinates of the frame).
+ * refs mode: solute is the test particle and solute_pos holds n*nsite*3
+ * doubles.  uvengy: n*(slvmax+1) doubles. */
+int32_t erm_get_uv_energy(erm_ctx *ctx, int32_t solute, int32_t n, const double *solute_pos,
+                          double *uvengy);
+
+/* The pieces of get_uv_energy for one configuration, as the reference's
+ * module procedures compute them:
+ *   realcal_proc  (realcal.F90:58-135)  -> ereal[slvmax]
+ *   realcal_self  (realcal.F90:284-328) -> *self_real
+ *   recpcal_prepare_solute + recpcal_energy (recpcal.F90:342-502)
+ *                                       -> erecp[slvmax], *self_recp  */
+int32_t erm_realcal_proc(erm_ctx *ctx, int32_t solute, const double *solute_pos, double *ereal);
+int32_t erm_realcal_self(erm_ctx *ctx, int32_t solute, const double *solute_pos, double *self_real);
+int32_t erm_recpcal(erm_ctx *ctx, int32_t solute, const double *solute_pos, double *erecp,
+                    double *self_recp);
+
+/* The hot loop of engconst (engproc.F90:388-392): get_uv_energy followed by
+ * update_histogram (engproc.F90:758-856) for every configuration.
+ * soln mode: n solute molecule indices in `solutes` (sltlist entries);
+ * refs mode: n test-particle configurations in solute_pos (n*nsite*3), with
+ * optional per-configuration weights (stat_weight_solute, NULL = 1).
+ * uvengy (optional) receives n*(slvmax+1) energies. */
+int32_t erm_engconst(erm_ctx *ctx, int32_t n, const int32_t *solutes, const double *solute_pos,
+                     const double *weights, double *uvengy);
+
+/* Test-particle insertion with device-generated random poses
+ * (instslt INSPOS_RANDOM + INSROT_RANDOM, insertion.F90:44-366): insertion
+ * `first_index + i` draws its position and quaternion from a counter-based
+ * generator keyed by `seed`, then get_uv_energy + update_histogram run on
+ * the device.  uvengy optional as above. */
+int32_t erm_insert_random(erm_ctx *ctx, int32_t n, uint64_t seed, uint64_t first_index,
+                          double *uvengy);
+
+/* update_histogram alone, for energies computed elsewhere (n configurations,
+ * uvengy n*(slvmax+1); solutes as in erm_engconst, NULL in refs mode). */
+int32_t erm_update_histogram(erm_ctx *ctx, int32_t n, const int32_t *solutes,
+                             const double *weights, const double *uvengy);
+
+/* engstore's normalisation (engproc.F90:539-553) on the device accumulators
+ * (one process: no reduction step). */
+int32_t erm_engstore(erm_ctx *ctx);
+
+/* Copies the accumulators to the host.  Any pointer may be NULL.
+ * edens[ermax]; ecorr[ermax*ermax] column-major as ecorr(iduvp, iduv);
+ * eself[esmax]; minuv/maxuv[numslv+1]; slnuv[numslv+1] (index 0 unused);
+ * scalars[4] = {engnorm, engsmpl, avslf, voffset}. */
+int32_t erm_fetch(erm_ctx *ctx, double *edens, double *ecorr, double *eself, double *minuv,
+                  double *maxuv, double *slnuv, double *scalars);
+
+/* sizes: slvmax (physical molecules), ermax (total bins) */
+int32_t erm_sizes(const erm_ctx *ctx, int32_t *slvmax, int32_t *ermax);
+
+/* Device time of the last engconst / insert_random call, split by stage
+ * (milliseconds, HIP events on the context's stream): [0] prep/pose,
+ * [1] PME columns, [2] PME planes, [3] gather, [4] pair, [5] binning,
+ * [6] correlation, [7] total. */
+int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

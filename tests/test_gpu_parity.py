@@ -1,0 +1,285 @@
+"""Parity of the device pipeline (libermgpu.so through its C-ABI) with the
+oracle on identical seeded inputs.
+
+Bars:
+* FP64: per-molecule energies within 1e-9 kcal/mol + 1e-11 relative of the
+  oracle (the two differ only in summation order, FFT algorithm and the
+  libm/ocml erf); histogram counts, edens and ecorr bit-exact with unit
+  weights;
+* FP32 (pair terms and PME in FP32, per-molecule sums in FP64): per-molecule
+  |dE| <= 1e-4 kcal/mol where |E| <= 100 kcal/mol, relative 1e-6 above
+  (overlapping insertions), and the bin-flip fraction is reported.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ermod_amd import synth
+from ermod_amd.api import ErmError, ErmGPU
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = dict(rtol=1e-11, atol=1e-9)
+
+
+def fp32_ok(e32, e64):
+    err = np.abs(e32 - e64)
+    lim = np.where(np.abs(e64) <= 100.0, 1e-4, 1e-6 * np.abs(e64))
+    return err <= lim, err
+
+
+def poses(sy, n, seed):
+    s = sy.system
+    m = sy.solute_mol
+    out = []
+    for i in range(n):
+        r, q = O.random_pose(seed, i)
+        out.append(O.place_solute(s.bfcoord, s.solute_mass(m), sy.cell, r, q))
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    sy = synth.tiny_refs()
+    return sy, sy.frame(0)
+
+
+@pytest.fixture(scope="module")
+def refs():
+    sy = synth.refs20()
+    return sy, sy.frame(0)
+
+
+def print_case(name):
+    print(f"\n[gpu case] {name}", flush=True)
+
+
+# ------------------------------------------------------------- stage parity
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_realcal_tiny(tiny, prec):
+    print_case(f"realcal tiny {prec}")
+    sy, pos = tiny
+    o = O.Oracle(sy.system)
+    o.set_frame(pos, sy.cell)
+    with ErmGPU(sy.system, prec) as g:
+        g.set_frame(pos, sy.cell)
+        for sp in poses(sy, 6, 1):
+            want = o.realcal(sy.solute_mol, sp)
+            got = g.realcal_proc(sy.solute_mol, sp)
+            if prec == "fp64":
+                np.testing.assert_allclose(got, want, **TOL64)
+            else:
+                ok, err = fp32_ok(got, want)
+                assert ok.all(), err.max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_recpcal_tiny(tiny, prec):
+    print_case(f"recpcal tiny {prec}")
+    sy, pos = tiny
+    o = O.Oracle(sy.system)
+    o.set_frame(pos, sy.cell)
+    with ErmGPU(sy.system, prec) as g:
+        g.set_frame(pos, sy.cell)
+        for sp in poses(sy, 4, 2):
+            want, wself = o.recpcal(sy.solute_mol, sp)
+            got, gself = g.recpcal(sy.solute_mol, sp)
+            if prec == "fp64":
+                np.testing.assert_allclose(got, want, **TOL64)
+                assert abs(gself - wself) <= 1e-9 + 1e-11 * abs(wself)
+            else:
+                ok, err = fp32_ok(got, want)
+                assert ok.all(), err.max()
+                assert abs(gself - wself) <= 1e-3
+
+
+def test_realcal_self(tiny):
+    print_case("realcal_self tiny")
+    sy, pos = tiny
+    o = O.Oracle(sy.system)
+    o.set_frame(pos, sy.cell)
+    with ErmGPU(sy.system, "fp64") as g:
+        g.set_frame(pos, sy.cell)
+        for sp in poses(sy, 3, 3):
+            assert abs(g.realcal_self(sy.solute_mol, sp) - o.realcal_self(sy.solute_mol, sp)) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_uv_energy_refs20(refs, prec):
+    """4000 waters, 49.3 A, 20-site test particle, PME 48^3."""
+    print_case(f"uv_energy refs20 {prec}")
+    sy, pos = refs
+    o = O.Oracle(sy.system)
+    o.set_frame(pos, sy.cell)
+    P = poses(sy, 5, 4)
+    with ErmGPU(sy.system, prec) as g:
+        g.set_frame(pos, sy.cell)
+        got = g.get_uv_energy(sy.solute_mol, P)
+    for i, sp in enumerate(P):
+        want = o.uv_energy(sy.solute_mol, sp)
+        if prec == "fp64":
+            np.testing.assert_allclose(got[i], want, **TOL64)
+        else:
+            ok, err = fp32_ok(got[i, 1:], want[1:])
+            assert ok.all(), (i, err.max())
+            assert abs(got[i, 0] - want[0]) < 1e-3
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_uv_energy_soln12(prec):
+    """Solution system: 12-site solute + 1000 waters, 31 A, PME 32^3."""
+    print_case(f"uv_energy soln12 {prec}")
+    sy = synth.soln12()
+    o = O.Oracle(sy.system)
+    with ErmGPU(sy.system, prec) as g:
+        for f in range(2):
+            pos = sy.frame(f)
+            o.set_frame(pos, sy.cell)
+            g.set_frame(pos, sy.cell)
+            want = o.uv_energy(sy.solute_mol)
+            got = g.get_uv_energy(sy.solute_mol)[0]
+            if prec == "fp64":
+                np.testing.assert_allclose(got, want, **TOL64)
+            else:
+                ok, err = fp32_ok(got[1:], want[1:])
+                assert ok.all(), err.max()
+
+
+# ------------------------------------------------------- histogram parity
+def test_engconst_refs_exact_counts(tiny):
+    """FP64: insdst / edens / ecorr bit-exact against the oracle."""
+    print_case("engconst tiny exact counts")
+    sy, _ = tiny
+    s = sy.system
+    o = O.Oracle(s)
+    with ErmGPU(s, "fp64", max_batch=7) as g:
+        for f in range(3):
+            pos = sy.frame(f)
+            o.set_frame(pos, sy.cell)
+            g.set_frame(pos, sy.cell)
+            P = poses(sy, 20, 10 + f)
+            u = g.engconst(solute_pos=P, want_uv=True)
+            for i, sp in enumerate(P):
+                want = o.uv_energy(sy.solute_mol, sp)
+                np.testing.assert_allclose(u[i], want, **TOL64)
+                o.update_histogram(sy.solute_mol, want)
+        a, b = o.acc(), g.fetch()
+    assert np.array_equal(a["edens"], b["edens"])
+    assert np.array_equal(a["ecorr"], b["ecorr"])
+    assert a["engnorm"] == b["engnorm"] and a["engsmpl"] == b["engsmpl"] and a["avslf"] == b["avslf"]
+    np.testing.assert_allclose(a["minuv"], b["minuv"], rtol=1e-11)
+    np.testing.assert_allclose(a["maxuv"], b["maxuv"], rtol=1e-11)
+
+
+def test_insert_random_matches_oracle(refs):
+    """Device-generated poses are the oracle's poses; counts bit-exact."""
+    print_case("insert_random refs20")
+    sy, pos = refs
+    s = sy.system
+    o = O.Oracle(s)
+    o.set_frame(pos, sy.cell)
+    n, seed = 48, 2026
+    with ErmGPU(s, "fp64", max_batch=32) as g:
+        g.set_frame(pos, sy.cell)
+        u = g.insert_random(n, seed, first_index=100, want_uv=True)
+        b = g.fetch()
+    for i in range(n):
+        r, q = O.random_pose(seed, 100 + i)
+        sp = O.place_solute(s.bfcoord, s.solute_mass(sy.solute_mol), sy.cell, r, q)
+        want = o.uv_energy(sy.solute_mol, sp)
+        np.testing.assert_allclose(u[i], want, **TOL64)
+        o.update_histogram(sy.solute_mol, want)
+    a = o.acc()
+    assert np.array_equal(a["edens"], b["edens"])
+    assert np.array_equal(a["ecorr"], b["ecorr"])
+
+
+def test_fp32_error_and_bin_flips(refs):
+    """FP32 vs FP64 on 64 insertions: per-molecule error bound and the
+    bin-flip fraction."""
+    print_case("fp32 vs fp64 refs20")
+    sy, pos = refs
+    out = {}
+    for prec in ("fp64", "fp32"):
+        with ErmGPU(sy.system, prec) as g:
+            g.set_frame(pos, sy.cell)
+            out[prec] = g.insert_random(64, 7, want_uv=True)
+    ok, err = fp32_ok(out["fp32"][:, 1:], out["fp64"][:, 1:])
+    edges = O.Oracle(sy.system).uvcrd
+    b64 = np.searchsorted(edges, out["fp64"][:, 1:], side="left")
+    b32 = np.searchsorted(edges, out["fp32"][:, 1:], side="left")
+    flips = np.count_nonzero(b64 != b32) / b64.size
+    print(f"max |dE| soft {err[np.abs(out['fp64'][:, 1:]) <= 100].max():.3e}  bin-flip fraction {flips:.2e}")
+    assert ok.all()
+    assert flips < 1e-3
+
+
+def test_soln_histogram_and_slnuv():
+    print_case("engconst soln12")
+    sy = synth.soln12()
+    s = sy.system
+    o = O.Oracle(s)
+    with ErmGPU(s, "fp64") as g:
+        for f in range(3):
+            pos = sy.frame(f)
+            o.set_frame(pos, sy.cell)
+            g.set_frame(pos, sy.cell)
+            g.engconst(solutes=[sy.solute_mol])
+            o.update_histogram(sy.solute_mol, o.uv_energy(sy.solute_mol))
+        a, b = o.acc(), g.fetch()
+    assert np.array_equal(a["edens"], b["edens"])
+    np.testing.assert_allclose(b["slnuv"][1:], a["slnuv"][1:], rtol=1e-10)
+    g2 = None
+    del g2
+
+
+def test_update_histogram_host_energies(tiny):
+    """update_histogram alone on energies from the oracle: exact counts."""
+    print_case("update_histogram tiny")
+    sy, pos = tiny
+    s = sy.system
+    o = O.Oracle(s)
+    o.set_frame(pos, sy.cell)
+    P = poses(sy, 10, 21)
+    U = np.array([o.uv_energy(sy.solute_mol, sp) for sp in P])
+    for u in U:
+        o.update_histogram(sy.solute_mol, u)
+    with ErmGPU(s, "fp64") as g:
+        g.set_frame(pos, sy.cell)
+        g.update_histogram(U)
+        b = g.fetch()
+    a = o.acc()
+    assert np.array_equal(a["edens"], b["edens"]) and np.array_equal(a["ecorr"], b["ecorr"])
+
+
+def test_engstore_normalisation(tiny):
+    print_case("engstore tiny")
+    sy, pos = tiny
+    s = sy.system
+    with ErmGPU(s, "fp64") as g:
+        g.set_frame(pos, sy.cell)
+        g.insert_random(30, 5)
+        raw = g.fetch()
+        g.engstore()
+        nrm = g.fetch()
+    assert np.array_equal(nrm["edens"], raw["edens"] / raw["engnorm"])
+    assert np.array_equal(nrm["ecorr"], raw["ecorr"] / raw["engnorm"])
+
+
+def test_error_codes(tiny):
+    """eng_min from an energy below the grid is returned, not aborted."""
+    print_case("error codes tiny")
+    sy, pos = tiny
+    s = sy.system
+    with ErmGPU(s, "fp64") as g:
+        g.set_frame(pos, sy.cell)
+        u = np.zeros((1, g.slvmax + 1))
+        u[0, 5] = -1e4
+        with pytest.raises(ErmError) as ei:
+            g.update_histogram(u)
+        assert ei.value.code == 1
+        cell = np.diag([14.0, 14.0, 14.0])
+        cell[0, 2] = 1.0
+        with pytest.raises(ErmError) as ei:
+            g.set_frame(pos, cell)
+        assert ei.value.code == 6
