@@ -49,7 +49,7 @@ def build_fortran(verbose: bool = False):
     lib = os.path.join(HERE, "libermgpu.so")
     cmds = [[FC, "-O2", "-c", mod, "-J", bdir, "-o", os.path.join(bdir, "ermgpu.o")],
             [FC, "-O2", ex, os.path.join(bdir, "ermgpu.o"), "-J", bdir, "-I", bdir, "-o", exe,
-             lib, "-Wl,-rpath," + HERE]]
+             lib, "-Wl,-rpath,$ORIGIN/.."]]
     for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)

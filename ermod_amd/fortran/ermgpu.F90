@@ -1,0 +1,216 @@
+! ermgpu.F90 -- ISO_C_BINDING interface of libermgpu.so (include/ermgpu.h)
+! for a Fortran host such as ERmod's engproc.  Every entry point of the
+! C-ABI is declared here with its C types; indices are 0-based, matrices
+! column-major (the Fortran layout of engmain's cell and ecorr).
+module ermgpu
+  use iso_c_binding, only: c_int32_t, c_int64_t, c_double, c_ptr, c_char, c_null_char, &
+                           c_f_pointer, c_null_ptr, c_associated
+  implicit none
+  private :: c_int32_t, c_int64_t, c_double, c_char, c_null_char, c_f_pointer
+
+  integer(c_int32_t), parameter :: ERM_OK = 0, ERM_E_ENG_MIN = 1, ERM_E_ENG_SFT = 2, &
+       ERM_E_MBIN = 3, ERM_E_ENG_ECD = 4, ERM_E_RCP_CNS = 5, ERM_E_NONCUBOID = 6, &
+       ERM_E_ENG_BUG = 7, ERM_E_ENG_CNS = 8, ERM_E_ENG_SIZ = 9, ERM_E_ARG = 10, &
+       ERM_E_UNSUPPORTED = 11, ERM_E_DEVICE = 12
+  integer(c_int32_t), parameter :: ERM_FP64 = 0, ERM_FP32 = 1
+
+  type, bind(C) :: erm_params
+     integer(c_int32_t) :: slttype, estype, ljswitch, splodr
+     integer(c_int32_t) :: ms(3)
+     integer(c_int32_t) :: corrcal, selfcal, wgtslf, force_calculation
+     real(c_double) :: elecut, lwljcut, upljcut, screen, temp, block_threshold
+     integer(c_int32_t) :: precision, max_batch, device
+  end type erm_params
+
+  type, bind(C) :: erm_hist
+     real(c_double) :: ecdmin, ecfmns, ecdcen, eccore, ecdmax
+     real(c_double) :: eclbin, ecfbin, ec0bin, finfac
+     integer(c_int32_t) :: pecore
+  end type erm_hist
+
+  interface
+     integer(c_int32_t) function erm_build_edges(h, self_energy, ercrd, cap, pemax, pesoft) &
+          bind(C, name="erm_build_edges")
+       import :: erm_hist, c_int32_t, c_double
+       type(erm_hist), intent(in) :: h
+       integer(c_int32_t), value :: self_energy, cap
+       real(c_double), intent(out) :: ercrd(*)
+       integer(c_int32_t), intent(out) :: pemax, pesoft
+     end function erm_build_edges
+
+     integer(c_int64_t) function erm_hash_double(v, n) bind(C, name="erm_hash_double")
+       import :: c_int64_t, c_int32_t, c_double
+       real(c_double), intent(in) :: v(*)
+       integer(c_int32_t), value :: n
+     end function erm_hash_double
+
+     integer(c_int32_t) function erm_create(params, ctx) bind(C, name="erm_create")
+       import :: erm_params, c_int32_t, c_ptr
+       type(erm_params), intent(in) :: params
+       type(c_ptr), intent(out) :: ctx
+     end function erm_create
+
+     subroutine erm_destroy(ctx) bind(C, name="erm_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end subroutine erm_destroy
+
+     type(c_ptr) function erm_last_error_c(ctx) bind(C, name="erm_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function erm_last_error_c
+
+     integer(c_int32_t) function erm_set_topology(ctx, natom, nmol, mol_begin, charge, ljtype, ntype, &
+          lj_eps, lj_sig2, sluvid, uvspec, mol_charge, sitemass, bfcoord) bind(C, name="erm_set_topology")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: natom, nmol, ntype
+       integer(c_int32_t), intent(in) :: mol_begin(*), ljtype(*), sluvid(*), uvspec(*)
+       real(c_double), intent(in) :: charge(*), lj_eps(*), lj_sig2(*), mol_charge(*), sitemass(*), bfcoord(*)
+     end function erm_set_topology
+
+     integer(c_int32_t) function erm_set_bins(ctx, numslv, uvmax, uvsoft, uvcrd, esmax, escrd) &
+          bind(C, name="erm_set_bins")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: numslv, esmax
+       integer(c_int32_t), intent(in) :: uvmax(*), uvsoft(*)
+       real(c_double), intent(in) :: uvcrd(*), escrd(*)
+     end function erm_set_bins
+
+     integer(c_int32_t) function erm_engclear(ctx) bind(C, name="erm_engclear")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+     end function erm_engclear
+
+     integer(c_int32_t) function erm_set_frame(ctx, sitepos, cell, weight) bind(C, name="erm_set_frame")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: sitepos(*), cell(3, 3)
+       real(c_double), value :: weight
+     end function erm_set_frame
+
+     integer(c_int32_t) function erm_set_frame_device(ctx, d_sitepos, cell, weight) &
+          bind(C, name="erm_set_frame_device")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx, d_sitepos
+       real(c_double), intent(in) :: cell(3, 3)
+       real(c_double), value :: weight
+     end function erm_set_frame_device
+
+     integer(c_int32_t) function erm_get_uv_energy(ctx, solute, n, solute_pos, uvengy) &
+          bind(C, name="erm_get_uv_energy")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: solute, n
+       real(c_double), intent(in) :: solute_pos(*)
+       real(c_double), intent(out) :: uvengy(*)
+     end function erm_get_uv_energy
+
+     integer(c_int32_t) function erm_realcal_proc(ctx, solute, solute_pos, ereal) &
+          bind(C, name="erm_realcal_proc")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: solute
+       real(c_double), intent(in) :: solute_pos(*)
+       real(c_double), intent(out) :: ereal(*)
+     end function erm_realcal_proc
+
+     integer(c_int32_t) function erm_realcal_self(ctx, solute, solute_pos, self_real) &
+          bind(C, name="erm_realcal_self")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: solute
+       real(c_double), intent(in) :: solute_pos(*)
+       real(c_double), intent(out) :: self_real
+     end function erm_realcal_self
+
+     integer(c_int32_t) function erm_recpcal(ctx, solute, solute_pos, erecp, self_recp) &
+          bind(C, name="erm_recpcal")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: solute
+       real(c_double), intent(in) :: solute_pos(*)
+       real(c_double), intent(out) :: erecp(*), self_recp
+     end function erm_recpcal
+
+     integer(c_int32_t) function erm_engconst(ctx, n, solutes, solute_pos, weights, uvengy) &
+          bind(C, name="erm_engconst")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       type(c_ptr), value :: solutes, solute_pos, weights, uvengy
+     end function erm_engconst
+
+     integer(c_int32_t) function erm_insert_random(ctx, n, seed, first_index, uvengy) &
+          bind(C, name="erm_insert_random")
+       import :: c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       integer(c_int64_t), value :: seed, first_index
+       type(c_ptr), value :: uvengy
+     end function erm_insert_random
+
+     integer(c_int32_t) function erm_update_histogram(ctx, n, solutes, weights, uvengy) &
+          bind(C, name="erm_update_histogram")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       type(c_ptr), value :: solutes, weights, uvengy
+     end function erm_update_histogram
+
+     integer(c_int32_t) function erm_engstore(ctx) bind(C, name="erm_engstore")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+     end function erm_engstore
+
+     integer(c_int32_t) function erm_fetch(ctx, edens, ecorr, eself, minuv, maxuv, slnuv, scalars) &
+          bind(C, name="erm_fetch")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, edens, ecorr, eself, minuv, maxuv, slnuv, scalars
+     end function erm_fetch
+
+     integer(c_int32_t) function erm_sizes(ctx, slvmax, ermax) bind(C, name="erm_sizes")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: slvmax, ermax
+     end function erm_sizes
+
+     integer(c_int32_t) function erm_stage_times(ctx, ms8) bind(C, name="erm_stage_times")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms8(8)
+     end function erm_stage_times
+  end interface
+
+contains
+
+  ! message of the last failing call as a Fortran string
+  function erm_last_error(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ''
+    if (.not. c_associated(erm_last_error_c(ctx))) return
+    call c_f_pointer(erm_last_error_c(ctx), s, [512])
+    do i = 1, 512
+       if (s(i) == c_null_char) exit
+       msg(i:i) = s(i)
+    end do
+  end function erm_last_error
+
+  ! representative energy of bin iduv (1-based) of one species, as repval
+  ! (engproc.F90:1185-1223) computes it for the engsln/engref files
+  real(c_double) function erm_repval(crd, idmax, idsoft, idpt) result(v)
+    real(c_double), intent(in) :: crd(*)
+    integer, intent(in) :: idmax, idsoft, idpt
+    if (idpt <= idsoft) then
+       v = (crd(idpt) + crd(idpt + 1)) / 2.0d0
+    else if (idpt < idmax) then
+       v = sqrt(crd(idpt) * crd(idpt + 1))
+    else
+       v = crd(idpt)
+    end if
+  end function erm_repval
+end module ermgpu

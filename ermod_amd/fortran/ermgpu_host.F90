@@ -1,0 +1,114 @@
+! ermgpu_host.F90 -- Fortran host of the energy pipeline through the
+! ISO_C_BINDING module ermgpu: the division loop of ERmod's trjmain
+! (enganal.F90:88-97) for a test-particle system -- engclear, per frame
+! set_frame + random insertions (engconst), engstore -- writing engref and
+! corref in the reference's formats (engproc.F90:609-644).
+!
+! usage: ermgpu_host <input stream file> <output directory>
+! The input holds, in order and without padding: natom nmol ntype nframes
+! nins (int32); the erm_params fields; the erm_hist fields of species 1;
+! mol_begin(nmol+1) charge(natom) ljtype(natom) lj_eps(ntype^2)
+! lj_sig2(ntype^2) sluvid(nmol) uvspec(nmol) mol_charge(nmol)
+! sitemass(natom) bfcoord(3*ns) cell(9) seed(int64); then nframes frames
+! of 3*natom coordinates.
+program ermgpu_host
+  use iso_c_binding
+  use ermgpu
+  implicit none
+  type(erm_params) :: prm
+  type(erm_hist) :: hst
+  type(c_ptr) :: ctx
+  integer(c_int32_t) :: natom, nmol, ntype, nframes, nins, st, pemax, pesoft, slvmax, ermax, ns
+  integer(c_int32_t), allocatable :: mol_begin(:), ljtype(:), sluvid(:), uvspec(:)
+  real(c_double), allocatable :: charge(:), lj_eps(:), lj_sig2(:), mol_charge(:), sitemass(:), bfcoord(:)
+  real(c_double), allocatable, target :: pos(:), uv(:, :), edens(:), ecorr(:, :), ercrd(:)
+  real(c_double), target :: scal(4), dummy(1)
+  real(c_double) :: cell(3, 3), factor
+  integer(c_int64_t) :: seed
+  integer :: f, iduv
+  character(len=1024) :: inp, outdir
+
+  call get_command_argument(1, inp)
+  call get_command_argument(2, outdir)
+  open(10, file=trim(inp), access='stream', form='unformatted', status='old', action='read')
+  read(10) natom, nmol, ntype, nframes, nins
+  read(10) prm%slttype, prm%estype, prm%ljswitch, prm%splodr, prm%ms, prm%corrcal, prm%selfcal, &
+       prm%wgtslf, prm%force_calculation, prm%elecut, prm%lwljcut, prm%upljcut, prm%screen, &
+       prm%temp, prm%block_threshold, prm%precision, prm%max_batch, prm%device
+  read(10) hst%ecdmin, hst%ecfmns, hst%ecdcen, hst%eccore, hst%ecdmax, hst%eclbin, hst%ecfbin, &
+       hst%ec0bin, hst%finfac, hst%pecore
+  allocate(mol_begin(nmol + 1), charge(natom), ljtype(natom), lj_eps(ntype * ntype), &
+       lj_sig2(ntype * ntype), sluvid(nmol), uvspec(nmol), mol_charge(nmol), sitemass(natom))
+  read(10) mol_begin, charge, ljtype, lj_eps, lj_sig2, sluvid, uvspec, mol_charge, sitemass
+  ns = mol_begin(nmol + 1) - mol_begin(nmol)
+  allocate(bfcoord(3 * ns), pos(3 * natom))
+  read(10) bfcoord, cell, seed
+
+  ! enginit: bin edges of the single solvent species
+  allocate(ercrd(10002))
+  st = erm_build_edges(hst, 0, ercrd, 10002, pemax, pesoft)
+  call check(st, 'erm_build_edges')
+  st = erm_create(prm, ctx)
+  call check(st, 'erm_create')
+  st = erm_set_topology(ctx, natom, nmol, mol_begin, charge, ljtype, ntype, lj_eps, lj_sig2, &
+       sluvid, uvspec, mol_charge, sitemass, bfcoord)
+  call check(st, 'erm_set_topology')
+  st = erm_set_bins(ctx, 1, [pemax], [pesoft], ercrd, 0, dummy)
+  call check(st, 'erm_set_bins')
+  st = erm_sizes(ctx, slvmax, ermax)
+  allocate(uv(slvmax + 1, nins), edens(ermax), ecorr(ermax, ermax))
+
+  ! one division: engclear, engconst per frame, engstore
+  st = erm_engclear(ctx)
+  call check(st, 'erm_engclear')
+  do f = 1, nframes
+     read(10) pos
+     st = erm_set_frame(ctx, pos, cell, 1.0d0)
+     call check(st, 'erm_set_frame')
+     if (f == 1) then
+        st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_loc(uv))
+     else
+        st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_null_ptr)
+     end if
+     call check(st, 'erm_insert_random')
+  end do
+  close(10)
+
+  st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(scal))
+  call check(st, 'erm_fetch')
+  open(11, file=trim(outdir)//'/raw.bin', access='stream', form='unformatted', status='replace')
+  write(11) slvmax, ermax, scal, edens, ecorr, uv
+  close(11)
+
+  st = erm_engstore(ctx)
+  call check(st, 'erm_engstore')
+  st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(scal))
+  call check(st, 'erm_fetch')
+  ! engref.tt (engproc.F90:609-622): representative energy, species, density
+  open(12, file=trim(outdir)//'/engref.tt', form='formatted', status='replace')
+  do iduv = 1, ermax
+     factor = erm_repval(ercrd, pemax, pesoft, iduv)
+     write(12, '(g15.7,i5,g25.15)') factor, 1, edens(iduv)
+  end do
+  close(12)
+  ! corref.tt (engproc.F90:625-640): one unformatted sequential record
+  open(13, file=trim(outdir)//'/corref.tt', form='unformatted', status='replace')
+  write(13) ecorr
+  close(13)
+  ! weight_refs (engproc.F90:588-601), wgtslf = 0
+  open(14, file=trim(outdir)//'/weight_refs', form='formatted', status='replace')
+  write(14, '(i5,e20.8)') 1, scal(1) / scal(2)
+  close(14)
+  call erm_destroy(ctx)
+  print '(a,i0,a,i0,a)', 'ermgpu_host: ', nframes, ' frames x ', nins, ' insertions done'
+
+contains
+  subroutine check(code, what)
+    integer(c_int32_t), intent(in) :: code
+    character(*), intent(in) :: what
+    if (code /= ERM_OK) then
+       print '(a,a,a,i0,a,a)', 'ermgpu_host: ', what, ' failed with ', code, ': ', trim(erm_last_error(ctx))
+       stop 1
+    end if
+  end subroutine check
+end program ermgpu_host

@@ -1,0 +1,94 @@
+"""The Fortran host (ermod_amd/fortran/ermgpu_host, built by amdflang over
+the ISO_C_BINDING module ermgpu) drives the device pipeline end to end;
+its raw accumulators match the oracle bit for bit and its engref/corref files
+carry the reference's formats."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ermod_amd import synth
+from ermod_amd.api import build_edges
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ermod_amd", "fortran", "ermgpu_host")
+
+
+def write_input(path, sy, frames, nins, seed):
+    s = sy.system
+    h = s.hists[0]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<5i", s.natom, s.nmol, s.ntype, len(frames), nins))
+        f.write(struct.pack("<4i3i4i6d3i", s.slttype, s.estype, s.ljswitch, s.splodr, *s.ms, s.corrcal,
+                            s.selfcal, s.wgtslf, s.force_calculation, s.elecut, s.lwljcut, s.upljcut,
+                            s.screen, s.temp, s.block_threshold, 0, 16, 0))
+        f.write(struct.pack("<9di", h.ecdmin, h.ecfmns, h.ecdcen, h.eccore, h.ecdmax, h.eclbin, h.ecfbin,
+                            h.ec0bin, h.finfac, h.pecore))
+        for a, t in ((s.mol_begin, np.int32), (s.charge, np.float64), (s.ljtype, np.int32),
+                     (s.lj_eps, np.float64), (s.lj_sig2, np.float64), (s.sluvid, np.int32),
+                     (s.uvspec, np.int32), (s.mol_charge, np.float64), (s.sitemass, np.float64),
+                     (s.bfcoord, np.float64), (sy.cell.T, np.float64)):
+            f.write(np.ascontiguousarray(a, dtype=t).tobytes())
+        f.write(struct.pack("<q", seed))
+        for fr in frames:
+            f.write(np.ascontiguousarray(fr, dtype=np.float64).tobytes())
+
+
+def test_fortran_host_end_to_end(tmp_path):
+    print("\n[gpu case] fortran host tiny_refs", flush=True)
+    assert os.path.exists(EXE), "build the Fortran host first (python -m ermod_amd.build)"
+    sy = synth.tiny_refs()
+    s = sy.system
+    frames = [sy.frame(i) for i in range(3)]
+    nins, seed = 12, 31
+    inp = tmp_path / "in.bin"
+    write_input(inp, sy, frames, nins, seed)
+    r = subprocess.run([EXE, str(inp), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(tmp_path / "raw.bin", "rb").read()
+    slvmax, ermax = struct.unpack("<2i", raw[:8])
+    off = 8
+    scal = np.frombuffer(raw, np.float64, 4, off); off += 32
+    edens = np.frombuffer(raw, np.float64, ermax, off); off += 8 * ermax
+    ecorr = np.frombuffer(raw, np.float64, ermax * ermax, off).reshape(ermax, ermax); off += 8 * ermax * ermax
+    uv = np.frombuffer(raw, np.float64, (slvmax + 1) * nins, off).reshape(nins, slvmax + 1)
+    o = O.Oracle(s)
+    m = sy.solute_mol
+    for fi, fr in enumerate(frames):
+        o.set_frame(fr, sy.cell)
+        for i in range(nins):
+            rr, q = O.random_pose(seed, fi * nins + i)
+            sp = O.place_solute(s.bfcoord, s.solute_mass(m), sy.cell, rr, q)
+            u = o.uv_energy(m, sp)
+            if fi == 0:
+                np.testing.assert_allclose(uv[i], u, rtol=1e-11, atol=1e-9)
+            o.update_histogram(m, u)
+    a = o.acc()
+    assert np.array_equal(edens, a["edens"]) and np.array_equal(ecorr, a["ecorr"])
+    assert scal[0] == a["engnorm"] and scal[1] == a["engsmpl"]
+    # engref.tt: (g15.7, i5, g25.15) per bin; corref.tt: one record of ermax^2 doubles
+    e, pemax, pesoft = build_edges(s.hists[0])
+    lines = open(tmp_path / "engref.tt").read().splitlines()
+    assert len(lines) == ermax
+    for iduv, line in enumerate(lines):
+        assert len(line) == 45
+        rep, spec, dens = float(line[:15]), int(line[15:20]), float(line[20:])
+        assert spec == 1
+        assert abs(dens - a["edens"][iduv] / a["engnorm"]) <= 1e-14 * max(1.0, abs(dens))
+        if iduv + 1 <= pesoft:
+            wr = (e[iduv] + e[iduv + 1]) / 2.0
+        elif iduv + 1 < pemax:
+            wr = np.sqrt(e[iduv] * e[iduv + 1])
+        else:
+            wr = e[iduv]
+        assert abs(rep - wr) <= 1e-6 * max(1.0, abs(wr))
+    rec = open(tmp_path / "corref.tt", "rb").read()
+    n1 = struct.unpack("<i", rec[:4])[0]
+    assert n1 == 8 * ermax * ermax and struct.unpack("<i", rec[-4:])[0] == n1
+    cor = np.frombuffer(rec[4:-4], np.float64).reshape(ermax, ermax)
+    assert np.array_equal(cor, a["ecorr"] / a["engnorm"])
