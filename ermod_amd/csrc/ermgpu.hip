@@ -934,11 +934,13 @@ int32_t erm_set_topology(erm_ctx *c, int32_t natom, int32_t nmol, const int32_t 
   for (int m : c->sltlist)
     if (c->mol_begin[m + 1] - c->mol_begin[m] != c->ns) return fail(c, ERM_E_ARG, "solutes differ in size");
   if (c->ns > 4096) return fail(c, ERM_E_UNSUPPORTED, "solute with more than 4096 sites");
-  // compact LJ types of the solvent sites (sluvid 0 molecules and other solutes)
+  // compact LJ types of the sites that act as solvent: every physical
+  // molecule, except the solute itself when it is the only one
   std::vector<int> cv(ntype, -1), cvt(natom, 0);
   c->vtypes.clear();
   for (int k = 0; k < c->slvmax; ++k) {
     int m = c->tagpt[k];
+    if (c->sltlist.size() == 1 && m == c->sltlist[0]) continue;
     for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a) {
       int t = ljtype[a];
       if (cv[t] < 0) {

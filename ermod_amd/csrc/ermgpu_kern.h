@@ -653,7 +653,6 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
   T *ulj = uq + ns;                       // [ns][nvt][2] (eps, sig2)
   int *seg_beg = reinterpret_cast<int *>(ulj + 2 * ns * nvt);
   int *seg_pre = seg_beg + MAXSEG;        // MAXSEG + 1 prefix sums
-  __shared__ int nseg;
   const int b = blockIdx.x;
   const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
   const double Rs = sref[4 * b + 3];
@@ -665,8 +664,6 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
     uq[s] = slt_q[s];
   }
   for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
-  if (threadIdx.x == 0) nseg = 0;
-  __syncthreads();
   // pillars within reach of the centre
   const double Rm = ord_to_dbl(*(const unsigned long long *)rm_ord_p);
   const double R = P.rcmax + Rs + Rm + 1e-6 * (1.0 + P.rcmax);
@@ -683,6 +680,14 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
       cnt[d] = 2 * n + 1;
     }
   }
+  // every pillar owns two fixed segment slots (empty when unused), so all
+  // workgroups of one configuration flatten the candidates in the same order
+  const int nsg = 2 * cnt[1] * cnt[2];
+  for (int t = threadIdx.x; t < nsg; t += blockDim.x) {
+    seg_beg[t] = 0;
+    seg_pre[t + 1] = 0;
+  }
+  __syncthreads();
   for (int t = threadIdx.x; t < cnt[1] * cnt[2]; t += blockDim.x) {
     int oy = t % cnt[1], oz = t / cnt[1];
     int yc = ((lo[1] + oy) % P.nb[1] + P.nb[1]) % P.nb[1];
@@ -703,39 +708,33 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
     int xb = (int)floor((Cw[0] + w) / P.hcell[0]);
     int rowb = P.nb[0] * (yc + P.nb[1] * zc);
     int n0 = P.nb[0];
-    int a0[2], a1[2], nsg = 0;
+    int a0[2], a1[2], npg = 0;
     if (xb - xa + 1 >= n0) {
       a0[0] = 0;
       a1[0] = n0;
-      nsg = 1;
+      npg = 1;
     } else {
       int xw = ((xa % n0) + n0) % n0;
       int len = xb - xa + 1;
       if (xw + len <= n0) {
         a0[0] = xw;
         a1[0] = xw + len;
-        nsg = 1;
+        npg = 1;
       } else {
         a0[0] = xw;
         a1[0] = n0;
         a0[1] = 0;
         a1[1] = xw + len - n0;
-        nsg = 2;
+        npg = 2;
       }
     }
-    for (int g = 0; g < nsg; ++g) {
+    for (int g = 0; g < npg; ++g) {
       int mb = cell_start[rowb + a0[g]], me = cell_start[rowb + a1[g]];
-      if (me > mb) {
-        int slot = atomicAdd(&nseg, 1);
-        if (slot < MAXSEG) {
-          seg_beg[slot] = mb;
-          seg_pre[slot + 1] = me - mb;
-        }
-      }
+      seg_beg[2 * t + g] = mb;
+      seg_pre[2 * t + g + 1] = me - mb;
     }
   }
   __syncthreads();
-  const int nsg = min(nseg, MAXSEG);
   if (threadIdx.x == 0) seg_pre[0] = 0;
   block_inclusive_scan(seg_pre + 1, nsg);
   const int total = seg_pre[nsg];
@@ -1008,10 +1007,11 @@ __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__
   const int tp = wave % nt, ti = wave / nt;  // tile rows p, columns i
   const int p0 = tp * 16, i0 = ti * 16;
   const int r = lane & 15, kq = lane >> 4;
-  // D layout: lane holds column i0 + (lane & 15), rows p0 + 4 (lane >> 4) + v
+  // D layout of the f64 16x16x4 MFMA: register v of a lane holds row
+  // (lane >> 4) + 4 v, column lane & 15
   dbl4 acc;
   for (int v = 0; v < 4; ++v) {
-    int pr = p0 + 4 * kq + v, ic = i0 + r;
+    int pr = p0 + kq + 4 * v, ic = i0 + r;
     acc[v] = (pr < ermax && ic < ermax) ? ecorr[(size_t)ic * ermax + pr] : 0.0;
   }
   for (int b0 = 0; b0 < n; b0 += 4) {
@@ -1024,7 +1024,7 @@ __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
   }
   for (int v = 0; v < 4; ++v) {
-    int pr = p0 + 4 * kq + v, ic = i0 + r;
+    int pr = p0 + kq + 4 * v, ic = i0 + r;
     if (pr < ermax && ic < ermax) ecorr[(size_t)ic * ermax + pr] = acc[v];
   }
 }

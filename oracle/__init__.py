@@ -308,6 +308,7 @@ class Oracle:
         out = dict(edens=np.ctypeslib.as_array(a.edens, (n,)).copy(),
                    minuv=np.ctypeslib.as_array(a.minuv, (ns + 1,)).copy(),
                    maxuv=np.ctypeslib.as_array(a.maxuv, (ns + 1,)).copy(),
+                   slnuv=np.ctypeslib.as_array(a.slnuv, (ns + 1,)).copy(),
                    engnorm=a.engnorm, engsmpl=a.engsmpl, avslf=a.avslf, voffset=a.voffset)
         if a.ecorr:
             out["ecorr"] = np.ctypeslib.as_array(a.ecorr, (n, n)).copy()  # [iduv][iduvp]

@@ -325,21 +325,30 @@ int32_t orc_fft_r2c_3d(const int32_t *K, const double *in, double *out) {
   nmax = nmax > K3 ? nmax : K3;
   double *work = (double *)malloc(sizeof(double) * 4 * nmax);
   double *row = (double *)malloc(sizeof(double) * 2 * K1);
-  /* x: real rows -> half spectrum */
-  for (int z = 0; z < K3; ++z)
-    for (int y = 0; y < K2; ++y) {
-      const double *src = in + ((long)z * K2 + y) * K1;
-      for (int x = 0; x < K1; ++x) {
-        row[2 * x] = src[x];
-        row[2 * x + 1] = 0.0;
-      }
-      fft1d(&p1, row, 1, -1, work);
-      double *dst = out + 2 * ((long)z * K2 + y) * H;
-      for (int x = 0; x < H; ++x) {
-        dst[2 * x] = row[2 * x];
-        dst[2 * x + 1] = row[2 * x + 1];
+  /* x: real rows -> half spectrum, two rows per complex transform:
+   * Z = FFT(a + i b), A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i */
+  long nrow = (long)K2 * K3;
+  for (long rw = 0; rw < nrow; rw += 2) {
+    const double *sa = in + rw * K1;
+    const double *sb = (rw + 1 < nrow) ? in + (rw + 1) * K1 : NULL;
+    for (int x = 0; x < K1; ++x) {
+      row[2 * x] = sa[x];
+      row[2 * x + 1] = sb ? sb[x] : 0.0;
+    }
+    fft1d(&p1, row, 1, -1, work);
+    double *da = out + 2 * rw * H;
+    double *db = sb ? out + 2 * (rw + 1) * H : NULL;
+    for (int k = 0; k < H; ++k) {
+      int nk = (K1 - k) % K1;
+      double zr = row[2 * k], zi = row[2 * k + 1], cr = row[2 * nk], ci = -row[2 * nk + 1];
+      da[2 * k] = 0.5 * (zr + cr);
+      da[2 * k + 1] = 0.5 * (zi + ci);
+      if (db) {
+        db[2 * k] = 0.5 * (zi - ci);
+        db[2 * k + 1] = -0.5 * (zr - cr);
       }
     }
+  }
   /* y */
   for (int z = 0; z < K3; ++z)
     for (int x = 0; x < H; ++x) fft1d(&p2, out + 2 * ((long)z * K2 * H + x), H, -1, work);
@@ -374,25 +383,33 @@ int32_t orc_fft_c2r_3d(const int32_t *K, const double *in, double *out) {
     for (int x = 0; x < H; ++x) fft1d(&p3, t + 2 * ((long)y * H + x), (long)K2 * H, +1, work);
   for (int z = 0; z < K3; ++z)
     for (int x = 0; x < H; ++x) fft1d(&p2, t + 2 * ((long)z * K2 * H + x), H, +1, work);
-  /* x: hermitian half rows -> real rows; the imaginary parts of the
-   * self-conjugate entries (x = 0 and x = K1/2) are ignored */
-  for (int z = 0; z < K3; ++z)
-    for (int y = 0; y < K2; ++y) {
-      const double *src = t + 2 * ((long)z * K2 + y) * H;
-      for (int x = 0; x < H; ++x) {
-        row[2 * x] = src[2 * x];
-        row[2 * x + 1] = src[2 * x + 1];
+  /* x: hermitian half rows -> real rows, two rows per complex transform
+   * (Z = A + i B with Hermitian extension; x = Re, y = Im of IFFT(Z)); the
+   * imaginary parts of the self-conjugate entries (x = 0, K1/2) are ignored */
+  long nrow = (long)K2 * K3;
+  for (long rw = 0; rw < nrow; rw += 2) {
+    const double *sa = t + 2 * rw * H;
+    const double *sb = (rw + 1 < nrow) ? t + 2 * (rw + 1) * H : NULL;
+    for (int x = 0; x < K1; ++x) {
+      int hx = x < H ? x : K1 - x;
+      double sg = x < H ? 1.0 : -1.0;
+      double ar = sa[2 * hx], ai = sg * sa[2 * hx + 1];
+      double br = sb ? sb[2 * hx] : 0.0, bi = sb ? sg * sb[2 * hx + 1] : 0.0;
+      if (x == 0 || (K1 % 2 == 0 && x == K1 / 2)) {
+        ai = 0.0;
+        bi = 0.0;
       }
-      row[1] = 0.0;
-      if (K1 % 2 == 0) row[2 * (K1 / 2) + 1] = 0.0;
-      for (int x = H; x < K1; ++x) {
-        row[2 * x] = src[2 * (K1 - x)];
-        row[2 * x + 1] = -src[2 * (K1 - x) + 1];
-      }
-      fft1d(&p1, row, 1, +1, work);
-      double *dst = out + ((long)z * K2 + y) * K1;
-      for (int x = 0; x < K1; ++x) dst[x] = row[2 * x];
+      row[2 * x] = ar - bi;
+      row[2 * x + 1] = ai + br;
     }
+    fft1d(&p1, row, 1, +1, work);
+    double *da = out + rw * K1;
+    double *db = sb ? out + (rw + 1) * K1 : NULL;
+    for (int x = 0; x < K1; ++x) {
+      da[x] = row[2 * x];
+      if (db) db[x] = row[2 * x + 1];
+    }
+  }
   free(t);
   free(work);
   free(row);
