@@ -34,7 +34,8 @@ class ErmParams(C.Structure):
                 ("selfcal", C.c_int32), ("wgtslf", C.c_int32), ("force_calculation", C.c_int32),
                 ("elecut", C.c_double), ("lwljcut", C.c_double), ("upljcut", C.c_double),
                 ("screen", C.c_double), ("temp", C.c_double), ("block_threshold", C.c_double),
-                ("precision", C.c_int32), ("max_batch", C.c_int32), ("device", C.c_int32)]
+                ("precision", C.c_int32), ("max_batch", C.c_int32), ("device", C.c_int32),
+                ("pme_path", C.c_int32)]
 
 
 class ErmHist(C.Structure):
@@ -134,7 +135,7 @@ class ErmGPU:
     """One device context over an ErmSystem."""
 
     def __init__(self, system: ErmSystem, precision: str = "fp64", max_batch: int = 0,
-                 device: int = 0):
+                 device: int = 0, pme_path: str = "auto"):
         s = system
         self.system = s
         self.precision = precision
@@ -148,6 +149,7 @@ class ErmGPU:
         P.precision = ERM_FP32 if precision == "fp32" else ERM_FP64
         P.max_batch = max_batch
         P.device = device
+        P.pme_path = {"auto": 0, "fused": 1, "unfused": 2}[pme_path]
         h = C.c_void_p()
         st = lib().erm_create(C.byref(P), C.byref(h))
         if st:

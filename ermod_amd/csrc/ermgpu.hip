@@ -17,7 +17,7 @@
 using namespace erm;
 
 namespace {
-
+size_t zcol_lds(int ncol, int K3, size_t vb);
 const double PI_REF = 3.1415926535897932;
 
 struct DBuf {
@@ -155,14 +155,23 @@ bool fft_axis(int n, FftAxis &ax) {
   return m == 1;
 }
 
-// columns per z-column workgroup: two LDS buffers of ncol x K3 complex within
-// 56 KB (static LDS of the kernel on top stays under 64 KB)
+// columns per z-column workgroup: two LDS buffers of ncol x K3 complex plus
+// the site chunk (ZSCH x ncol complex, ZSCH x (p + 1) words) within 56 KB
+// (the kernel's static LDS on top stays under 64 KB)
 int zcol_width(int K3, size_t vb) {
-  return (int)std::min<size_t>(64, std::max<size_t>(1, (size_t)(56 * 1024) / (2 * (size_t)K3 * vb)));
+  int n = 64;
+  while (n > 1 && zcol_lds(n, K3, vb) > (size_t)56 * 1024) --n;
+  return n;
 }
 
 long fortran_nint(double x) { return (long)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
 
+}  // namespace
+
+namespace {
+size_t zcol_lds(int ncol, int K3, size_t vb) {
+  return 2 * (size_t)ncol * K3 * vb + (size_t)ZSCH * ncol * vb + (size_t)ZSCH * (MAXP + 1) * 8 + 64;
+}
 }  // namespace
 
 // ============================================================== the context
@@ -203,6 +212,9 @@ struct erm_ctx {
   DBuf d_pos, d_cellid, d_count, d_cell_start, d_fill, d_rm, d_mol_k, d_mol_ns, d_mol_first, d_mol_q,
       d_mol_spec, d_mol_m, d_jof;
   DBuf d_sx, d_sy, d_sz, d_sq, d_sct, d_sg, d_sw;
+  DBuf d_zcount, d_zstart, d_zfill, d_zsite, d_zg, d_zq, d_zw, d_esite;
+  int nsites = 0;             // sites of the physical molecules
+  int slab_S = 0, slab_R = 0; // fused PME slab: planes per workgroup, registers per thread
   DBuf d_tw1, d_tw2, d_tw3, d_engfac;
   DBuf d_edges, d_specoff, d_uvmax, d_uvsoft, d_escrd;
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
@@ -212,7 +224,7 @@ struct erm_ctx {
     return {&d_mol_begin, &d_charge, &d_tagpt, &d_cvt, &d_mol_charge, &d_uvspec, &d_sitemass, &d_bf, &d_qslt, &d_qslt_t,
             &d_sltlj, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
             &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
-            &d_sw, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
+            &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
             &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
             &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm};
@@ -423,12 +435,53 @@ template <typename T> int frame_prep(erm_ctx *c) {
                      dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sg), dp<T>(c->d_sw), dp<double>(c->d_mol_q),
                      dp<int>(c->d_mol_spec), dp<int>(c->d_mol_m));
   TRY(launch_check(c, "k_site_fill"));
+  // sites sorted by PME grid base g3 for the fused slab kernel
+  const int K3 = k.K3;
+  c->nsites = nsites;
+  TRY(ensure(c, c->d_zcount, sizeof(int) * (K3 + 1)));
+  TRY(ensure(c, c->d_zstart, sizeof(int) * (K3 + 1)));
+  TRY(ensure(c, c->d_zfill, sizeof(int) * (K3 + 1)));
+  TRY(ensure(c, c->d_zsite, sizeof(int) * nsites));
+  TRY(ensure(c, c->d_zg, sizeof(int) * nsites));
+  TRY(ensure(c, c->d_zq, sizeof(T) * nsites));
+  TRY(ensure(c, c->d_zw, sizeof(T) * (size_t)nsites * 3 * k.p));
+  HIPCHK(c, hipMemsetAsync(c->d_zcount.p, 0, sizeof(int) * (K3 + 1), c->st));
+  const int gs = (nsites + tb - 1) / tb;
+  hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, dp<int>(c->d_sg), dp<int>(c->d_zcount));
+  TRY(launch_check(c, "k_zcount"));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_zcount), dp<int>(c->d_zstart), K3);
+  TRY(launch_check(c, "k_scan"));
+  HIPCHK(c, hipMemcpyAsync(c->d_zfill.p, c->d_zstart.p, sizeof(int) * K3, hipMemcpyDeviceToDevice, c->st));
+  hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, k.p, dp<int>(c->d_sg), dp<T>(c->d_sq),
+                     dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zg), dp<T>(c->d_zq),
+                     dp<T>(c->d_zw));
+  TRY(launch_check(c, "k_zscatter"));
+  std::vector<int> zst(K3 + 1);
+  HIPCHK(c, hipMemcpyAsync(zst.data(), c->d_zstart.p, sizeof(int) * (K3 + 1), hipMemcpyDeviceToHost, c->st));
+  // slab: at most 16 planes, sites per slab within the register budget
+  c->slab_S = 0;
+  c->slab_R = 0;
   // sorted index of every tagpt molecule (for soln-mode solutes)
   std::vector<int> molk(S);
   HIPCHK(c, hipMemcpyAsync(molk.data(), c->d_mol_k.p, sizeof(int) * S, hipMemcpyDeviceToHost, c->st));
   TRY(sync(c));
   c->jof_k.assign(S, -1);
   for (int j = 0; j < S; ++j) c->jof_k[molk[j]] = j;
+  for (int R : {16, 32}) {
+    if (c->P.pme_path == 2) break;
+    for (int Sl = std::min(16, K3); Sl >= 1 && !c->slab_S; --Sl) {
+      int mx = 0;
+      for (int zb = 0; zb < K3; zb += Sl) mx = std::max(mx, zst[std::min(zb + Sl, K3)] - zst[zb]);
+      // a slab recomputes p - 1 halo planes: keep that below 3x the owned planes
+      if (mx <= R * NTHR && (Sl + k.p - 1) <= 3 * Sl) {
+        c->slab_S = Sl;
+        c->slab_R = R;
+      }
+    }
+    if (c->slab_S) break;
+  }
+  if (c->P.pme_path == 1 && !c->slab_S)
+    return fail(c, ERM_E_UNSUPPORTED, "fused PME slab does not fit this grid / system");
   return ERM_OK;
 }
 
@@ -471,10 +524,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   const int ncolwg = zcol_width(K3, vb);
   const int nzg = (H * K2 + ncolwg - 1) / ncolwg;
   TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
-  if (flags & DO_RECIP) {
-    TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
-    TRY(ensure(c, c->d_phi, sizeof(T) * (size_t)n * K3 * K2 * K1));
-  }
+  if (flags & DO_RECIP) TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
   HIPCHK(c, hipEventRecord(c->ev[0], c->st));
   // configurations
   if (bt.random) {
@@ -504,23 +554,45 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   TRY(launch_check(c, "k_solute_prep"));
   HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   if (flags & DO_RECIP) {
-    size_t lz = 2 * (size_t)ncolwg * K3 * vb;
+    size_t lz = zcol_lds(ncolwg, K3, vb);
     hipLaunchKernelGGL(k_pme_zcol<T>, dim3(nzg, n), dim3(NTHR), lz, c->st, k, ncolwg, dp<T>(c->d_engfac),
                        dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h),
                        dp<V>(c->d_tw3), dp<V>(c->d_Hk), dp<double>(c->d_selfp));
     TRY(launch_check(c, "k_pme_zcol"));
     HIPCHK(c, hipEventRecord(c->ev[2], c->st));
     size_t lp = 2 * (size_t)K2 * H * vb;
-    hipLaunchKernelGGL(k_pme_plane<T>, dim3(K3, n), dim3(NTHR), lp, c->st, k, dp<V>(c->d_Hk), dp<V>(c->d_tw1),
-                       dp<V>(c->d_tw2), dp<T>(c->d_phi));
-    TRY(launch_check(c, "k_pme_plane"));
-    HIPCHK(c, hipEventRecord(c->ev[3], c->st));
     Params kg = k;
     if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
-    hipLaunchKernelGGL(k_gather<T>, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, dp<T>(c->d_phi),
-                       dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), dp<int>(c->d_sg),
-                       dp<T>(c->d_sw), dp<T>(c->d_sq), dp<int>(c->d_sltj), dp<double>(c->d_uv));
-    TRY(launch_check(c, "k_gather"));
+    if (c->slab_S > 0) {
+      const int nslab = (K3 + c->slab_S - 1) / c->slab_S;
+      TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
+      if (c->slab_R == 16)
+        hipLaunchKernelGGL((k_pme_slab<T, 16>), dim3(nslab, n), dim3(NTHR), lp, c->st, k, c->slab_S, c->nsites,
+                           dp<V>(c->d_Hk), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<int>(c->d_zstart),
+                           dp<int>(c->d_zg), dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite),
+                           dp<double>(c->d_esite));
+      else
+        hipLaunchKernelGGL((k_pme_slab<T, 32>), dim3(nslab, n), dim3(NTHR), lp, c->st, k, c->slab_S, c->nsites,
+                           dp<V>(c->d_Hk), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<int>(c->d_zstart),
+                           dp<int>(c->d_zg), dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite),
+                           dp<double>(c->d_esite));
+      TRY(launch_check(c, "k_pme_slab"));
+      HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+      hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
+                         dp<double>(c->d_esite), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
+                         dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
+      TRY(launch_check(c, "k_mol_sum"));
+    } else {
+      TRY(ensure(c, c->d_phi, sizeof(T) * (size_t)n * K3 * K2 * K1));
+      hipLaunchKernelGGL(k_pme_plane<T>, dim3(K3, n), dim3(NTHR), lp, c->st, k, dp<V>(c->d_Hk), dp<V>(c->d_tw1),
+                         dp<V>(c->d_tw2), dp<T>(c->d_phi));
+      TRY(launch_check(c, "k_pme_plane"));
+      HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+      hipLaunchKernelGGL(k_gather<T>, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, dp<T>(c->d_phi),
+                         dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), dp<int>(c->d_sg),
+                         dp<T>(c->d_sw), dp<T>(c->d_sq), dp<int>(c->d_sltj), dp<double>(c->d_uv));
+      TRY(launch_check(c, "k_gather"));
+    }
   } else {
     HIPCHK(c, hipMemsetAsync(c->d_uv.p, 0, sizeof(double) * (size_t)n * (S + 1), c->st));
     HIPCHK(c, hipEventRecord(c->ev[2], c->st));
@@ -531,7 +603,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     double Rg = k.rcmax + 8.0;
     double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
     int nsplit = std::max(1, std::min(16, (int)std::ceil(frac * S / (2.0 * NTHR))));
-    size_t lpair = sizeof(T) * ((size_t)4 * ns + (size_t)2 * ns * c->nvt) + sizeof(int) * (2 * MAXSEG + 2) + 64;
+    size_t lpair = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * c->nvt) + sizeof(int) * (2 * MAXSEG + 2) + 64;
     hipLaunchKernelGGL(k_pair<T>, dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
                        dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
                        dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
@@ -580,7 +652,7 @@ int run_histogram(erm_ctx *c, int n, int first) {
                      dp<int>(c->d_sltj), dp<double>(c->d_hist), dp<double>(c->d_flceng),
                      dp<unsigned long long>(c->d_mnmx), dp<int>(c->d_status));
   TRY(launch_check(c, "k_bin"));
-  hipLaunchKernelGGL(k_scalars, dim3(1), dim3(1), 0, c->st, k, n, dp<double>(c->d_uv), dp<double>(c->d_wsol),
+  hipLaunchKernelGGL(k_scalars, dim3(1), dim3(NTHR), 0, c->st, k, n, dp<double>(c->d_uv), dp<double>(c->d_wsol),
                      dp<double>(c->d_escrd), dp<double>(c->d_scal), dp<double>(c->d_eself),
                      dp<unsigned long long>(c->d_mnmx), dp<double>(c->d_w), dp<int>(c->d_status));
   TRY(launch_check(c, "k_scalars"));
@@ -784,6 +856,7 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
   if (P.splodr < 2 || P.splodr > MAXP) return ERM_E_UNSUPPORTED;
   if (P.ljswitch < 0 || P.ljswitch > 3) return ERM_E_ARG;
   if (P.precision != ERM_FP64 && P.precision != ERM_FP32) return ERM_E_ARG;
+  if (P.pme_path < 0 || P.pme_path > 2) return ERM_E_ARG;
   FftAxis ax[3];
   for (int d = 0; d < 3; ++d)
     if (P.ms[d] < 2 || P.ms[d] > 512 || P.ms[d] < P.splodr || !fft_axis(P.ms[d], ax[d])) return ERM_E_UNSUPPORTED;
@@ -845,13 +918,17 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
     return ERM_E_UNSUPPORTED;
   }
   if (lp > 64 * 1024) {
-    hipError_t e = c->fp32 ? hipFuncSetAttribute((const void *)k_pme_plane<float>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp)
-                           : hipFuncSetAttribute((const void *)k_pme_plane<double>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
-    if (e != hipSuccess) {
-      erm_destroy(c);
-      return ERM_E_UNSUPPORTED;
+    const void *fns_f[3] = {(const void *)k_pme_plane<float>, (const void *)k_pme_slab<float, 16>,
+                            (const void *)k_pme_slab<float, 32>};
+    const void *fns_d[3] = {(const void *)k_pme_plane<double>, (const void *)k_pme_slab<double, 16>,
+                            (const void *)k_pme_slab<double, 32>};
+    for (int i = 0; i < 3; ++i) {
+      hipError_t e = hipFuncSetAttribute(c->fp32 ? fns_f[i] : fns_d[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lp);
+      if (e != hipSuccess) {
+        erm_destroy(c);
+        return ERM_E_UNSUPPORTED;
+      }
     }
   }
   int s = build_tw(c, c->d_tw1, k.K1);

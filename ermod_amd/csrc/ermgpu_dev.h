@@ -15,6 +15,7 @@ constexpr int MAXFAC = 16;         // radix factors per FFT axis
 constexpr int MAXSPEC = 32;        // solvent species
 constexpr int MAXSEG = 2048;       // pillar segments per insertion (pair kernel)
 constexpr int MAXP = 12;           // spline order
+constexpr int ZSCH = 32;           // solute sites per LDS chunk of the z-column build
 
 template <typename T> struct C2;
 template <> struct C2<float> { using type = float2; };

@@ -366,6 +366,9 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol(Params P, int ncolwg, const T
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V *buf0 = reinterpret_cast<V *>(lds_raw);
   V *buf1 = buf0 + ncolwg * P.K3;
+  V *Acol = buf1 + ncolwg * P.K3;                       // [ZSCH][ncolwg]
+  T *wz = reinterpret_cast<T *>(Acol + ZSCH * ncolwg);  // [ZSCH][p]
+  int *gz = reinterpret_cast<int *>(wz + ZSCH * P.p);   // [ZSCH]
   __shared__ double red[NTHR];
   const int b = blockIdx.y, ns = P.ns, p = P.p, K3 = P.K3, H = P.H;
   const int ncol_tot = H * P.K2;
@@ -375,24 +378,46 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol(Params P, int ncolwg, const T
     buf0[t].x = 0;
     buf0[t].y = 0;
   }
-  __syncthreads();
-  if (threadIdx.x < nc) {
-    const int c = c0 + threadIdx.x;
-    const int m1 = c % H, m2 = c / H;
-    V *col = buf0 + threadIdx.x * K3;
-    for (int s = 0; s < ns; ++s) {
-      size_t bs = (size_t)b * ns + s;
-      V a = cmul(w1h[bs * H + m1], w2h[bs * P.K2 + m2]);
-      T q = sqv[bs];
-      a.x *= q;
-      a.y *= q;
-      int g = sg3[bs];
-      for (int k = 0; k < p; ++k) {
-        int z = ((g - k) % K3 + K3) % K3;
-        T w = sw3[bs * p + k];
-        col[z].x += a.x * w;
-        col[z].y += a.y * w;
+  // the column of (m1, m2) along z: sum_s q_s what1_s(m1) what2_s(m2) w3_s(z),
+  // sites in chunks of ZSCH through LDS, one thread per (column, z)
+  for (int s0 = 0; s0 < ns; s0 += ZSCH) {
+    const int nsc = min(ZSCH, ns - s0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nsc * ncolwg; t += blockDim.x) {
+      int sl = t / ncolwg, cl = t - sl * ncolwg;
+      int c = c0 + cl;
+      V a;
+      a.x = 0;
+      a.y = 0;
+      if (cl < nc) {
+        size_t bs = (size_t)b * ns + s0 + sl;
+        a = cmul(w1h[bs * H + c % H], w2h[bs * P.K2 + c / H]);
+        T q = sqv[bs];
+        a.x *= q;
+        a.y *= q;
       }
+      Acol[t] = a;
+    }
+    for (int t = threadIdx.x; t < nsc; t += blockDim.x) {
+      size_t bs = (size_t)b * ns + s0 + t;
+      gz[t] = sg3[bs];
+      for (int k = 0; k < p; ++k) wz[t * p + k] = sw3[bs * p + k];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ncolwg * K3; t += blockDim.x) {
+      int cl = t / K3, z = t - cl * K3;
+      V acc = buf0[t];
+      for (int sl = 0; sl < nsc; ++sl) {
+        int k = gz[sl] - z;
+        k += (k < 0) ? K3 : 0;
+        if (k < p) {
+          V a = Acol[sl * ncolwg + cl];
+          T w = wz[sl * p + k];
+          acc.x += a.x * w;
+          acc.y += a.y * w;
+        }
+      }
+      buf0[t] = acc;
     }
   }
   __syncthreads();
@@ -432,22 +457,20 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol(Params P, int ncolwg, const T
 // then the x c2r transform, two real rows per complex FFT
 // (Z = X_y + i X_{y+1} with Hermitian extension), into phi[b][z][y][x].
 // --------------------------------------------------------------------------
+// The inverse transform of one z plane in LDS: y inverse FFT of the K2 x H
+// block src[m2][m1], then the x c2r transform two real rows per complex FFT
+// (Z = X_y + i X_{y+1} with Hermitian extension).  Returns R with
+// phi(y, x) = (y & 1) ? R[(y >> 1) K1 + x].y : R[(y >> 1) K1 + x].x.
 template <typename T>
-__global__ __launch_bounds__(NTHR) void k_pme_plane(Params P, const typename C2<T>::type *__restrict__ Hk,
-                                                    const typename C2<T>::type *__restrict__ tw1,
-                                                    const typename C2<T>::type *__restrict__ tw2,
-                                                    T *phi) {
+__device__ typename C2<T>::type *plane_c2r(const Params &P, const typename C2<T>::type *__restrict__ src,
+                                           typename C2<T>::type *A, typename C2<T>::type *A2,
+                                           const typename C2<T>::type *__restrict__ tw1,
+                                           const typename C2<T>::type *__restrict__ tw2) {
   using V = typename C2<T>::type;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
   const int K1 = P.K1, K2 = P.K2, H = P.H;
   const int blk = K2 * H;
-  V *A = reinterpret_cast<V *>(lds_raw);
-  V *A2 = A + blk;
-  const int z = blockIdx.x, b = blockIdx.y;
-  const V *src = Hk + ((size_t)b * P.K3 + z) * blk;
   for (int t = threadIdx.x; t < blk; t += blockDim.x) A[t] = src[t];
   __syncthreads();
-  // columns m1 (stride 1), elements m2 (stride H)
   V *Y = lds_fft(A, A2, H, 1, H, P.ax2, +1, tw2);
   V *Z = (Y == A) ? A2 : A;
   const int npair = (K2 + 1) / 2;
@@ -477,13 +500,145 @@ __global__ __launch_bounds__(NTHR) void k_pme_plane(Params P, const typename C2<
   }
   __syncthreads();
   V *Zo = (Z == A) ? A2 : A;
-  V *R = lds_fft(Z, Zo, npair, K1, 1, P.ax1, +1, tw1);
+  return lds_fft(Z, Zo, npair, K1, 1, P.ax1, +1, tw1);
+}
+
+// --------------------------------------------------------------------------
+// PME: one z plane per workgroup into phi[b][z][y][x] (used when the fused
+// slab kernel below does not fit).
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_pme_plane(Params P, const typename C2<T>::type *__restrict__ Hk,
+                                                    const typename C2<T>::type *__restrict__ tw1,
+                                                    const typename C2<T>::type *__restrict__ tw2,
+                                                    T *phi) {
+  using V = typename C2<T>::type;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int K1 = P.K1, K2 = P.K2, H = P.H;
+  V *A = reinterpret_cast<V *>(lds_raw);
+  V *A2 = A + K2 * H;
+  const int z = blockIdx.x, b = blockIdx.y;
+  V *R = plane_c2r<T>(P, Hk + ((size_t)b * P.K3 + z) * K2 * H, A, A2, tw1, tw2);
   T *dst = phi + ((size_t)b * P.K3 + z) * (size_t)K2 * K1;
   for (int t = threadIdx.x; t < K2 * K1; t += blockDim.x) {
     int y = t / K1, x = t - y * K1;
     V v = R[(y >> 1) * K1 + x];
     dst[t] = (y & 1) ? v.y : v.x;
   }
+}
+
+// --------------------------------------------------------------------------
+// PME, fused inverse transform and gather (recpcal.F90:402 and 451-502).
+// Workgroup (slab, b) owns the solvent sites whose grid base g3 lies in
+// [zb, zb + S); it computes the planes zb - p + 1 .. zb + S - 1 one at a time
+// in LDS (plane_c2r) and, after each plane zz, adds that plane's share
+//   q w3[g3 - zz] sum_{c2,c1} w2[c2] w1[c1] phi_zz(g2 - c2, g1 - c1)
+// to every owned site whose support contains zz.  Thread t owns the slab
+// sites t, t + blockDim, ... (sites are sorted by g3, so a wave's sites are
+// active together); the per-site sums live in registers (FP64) and are
+// written once to e_site[b][t_site].  phi never leaves the CU.
+// --------------------------------------------------------------------------
+template <typename T, int MAXR>
+__global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
+                                                   const typename C2<T>::type *__restrict__ Hk,
+                                                   const typename C2<T>::type *__restrict__ tw1,
+                                                   const typename C2<T>::type *__restrict__ tw2,
+                                                   const int *__restrict__ z_start, const int *__restrict__ zg,
+                                                   const T *__restrict__ zq, const T *__restrict__ zw,
+                                                   const int *__restrict__ zsite, double *e_site) {
+  using V = typename C2<T>::type;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int K1 = P.K1, K2 = P.K2, K3 = P.K3, H = P.H, p = P.p;
+  V *A = reinterpret_cast<V *>(lds_raw);
+  V *A2 = A + K2 * H;
+  const int b = blockIdx.y;
+  const int zb = blockIdx.x * S, ze = min(zb + S, K3);
+  const int s_beg = z_start[zb], s_end = z_start[ze];
+  double acc[MAXR];
+  int gz[MAXR];
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) {
+    int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+    acc[r] = 0.0;
+    gz[r] = (idx < s_end) ? ((zg[idx] >> 20) & 1023) : -(1 << 20);
+  }
+  const size_t np = (size_t)nsites;
+  for (int zz = zb - p + 1; zz < ze; ++zz) {
+    const int z = ((zz % K3) + K3) % K3;
+    V *R = plane_c2r<T>(P, Hk + ((size_t)b * K3 + z) * K2 * H, A, A2, tw1, tw2);
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      const int c3 = gz[r] - zz;
+      if (c3 >= 0 && c3 < p) {
+        const int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+        const int gp = zg[idx];
+        const int g1 = gp & 1023, g2 = (gp >> 10) & 1023;
+        const T f1 = zq[idx] * zw[(size_t)(2 * p + c3) * np + idx];
+        T sum = 0;
+        for (int c2 = 0; c2 < p; ++c2) {
+          int y = g2 - c2;
+          y += (y < 0) ? K2 : 0;
+          const V *row = R + (y >> 1) * K1;
+          const bool odd = (y & 1);
+          T rs = 0;
+          for (int c1 = 0; c1 < p; ++c1) {
+            int x = g1 - c1;
+            x += (x < 0) ? K1 : 0;
+            V v = row[x];
+            rs += zw[(size_t)c1 * np + idx] * (odd ? v.y : v.x);
+          }
+          sum += zw[(size_t)(p + c2) * np + idx] * rs;
+        }
+        acc[r] += (double)(f1 * sum);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) {
+    int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+    if (idx < s_end) e_site[(size_t)b * np + zsite[idx]] = acc[r];
+  }
+}
+
+// per-molecule reciprocal energy from the per-site sums, plus the
+// neutralising background (residual_ene, engproc.F90:859-869):
+// uv[b][1 + j] = -pi Q_s Q_j / (alpha^2 V) + sum over the sites of j
+__global__ void k_mol_sum(Params P, int nsites, const double *__restrict__ e_site,
+                          const int *__restrict__ mol_first, const int *__restrict__ mol_ns,
+                          const double *__restrict__ mol_q, const int *__restrict__ slt_j, double *uv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= P.slvmax) return;
+  double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
+  if (j == slt_j[b]) {
+    u[j] = 0.0;
+    return;
+  }
+  const double *es = e_site + (size_t)b * nsites;
+  double e = 0.0;
+  for (int t = mol_first[j]; t < mol_first[j] + mol_ns[j]; ++t) e += es[t];
+  double epcl = 3.1415926535897932 * P.solute_charge * mol_q[j] / P.screen / P.screen / P.volume;
+  u[j] = (0.0 - epcl) + e;
+}
+
+// per-frame sort of the solvent sites by grid base g3 (z-sorted copies of
+// base, charge and B-spline weights, the latter as [3p][nsites])
+__global__ void k_zcount(int nsites, const int *__restrict__ sg, int *zcount) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nsites) atomicAdd(&zcount[(sg[t] >> 20) & 1023], 1);
+}
+
+template <typename T>
+__global__ void k_zscatter(int nsites, int p, const int *__restrict__ sg, const T *__restrict__ sq,
+                           const T *__restrict__ sw, int *zfill, int *zsite, int *zg, T *zq, T *zw) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsites) return;
+  int zt = atomicAdd(&zfill[(sg[t] >> 20) & 1023], 1);
+  zsite[zt] = t;
+  zg[zt] = sg[t];
+  zq[zt] = sq[t];
+  for (int k = 0; k < 3 * p; ++k) zw[(size_t)k * nsites + zt] = sw[(size_t)t * 3 * p + k];
 }
 
 // --------------------------------------------------------------------------
@@ -651,7 +806,8 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
   T *uz = uy + ns;
   T *uq = uz + ns;
   T *ulj = uq + ns;                       // [ns][nvt][2] (eps, sig2)
-  int *seg_beg = reinterpret_cast<int *>(ulj + 2 * ns * nvt);
+  T *ulo = ulj + 2 * ns * nvt;            // [ns][3] low parts of the offsets
+  int *seg_beg = reinterpret_cast<int *>(ulo + 3 * ns);
   int *seg_pre = seg_beg + MAXSEG;        // MAXSEG + 1 prefix sums
   const int b = blockIdx.x;
   const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
@@ -662,6 +818,10 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
     uy[s] = (T)(xs[3 * s + 1] - C1);
     uz[s] = (T)(xs[3 * s + 2] - C2v);
     uq[s] = slt_q[s];
+    // low parts of the solute offsets (zero in FP64)
+    ulo[3 * s] = (T)((xs[3 * s] - C0) - (double)ux[s]);
+    ulo[3 * s + 1] = (T)((xs[3 * s + 1] - C1) - (double)uy[s]);
+    ulo[3 * s + 2] = (T)((xs[3 * s + 2] - C2v) - (double)uz[s]);
   }
   for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
   // pillars within reach of the centre
@@ -761,11 +921,16 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
       dx -= P.L[0] * rint(dx / P.L[0]);
       dy -= P.L[1] * rint(dy / P.L[1]);
       dz -= P.L[2] * rint(dz / P.L[2]);
+      // each offset as hi + lo in T: (hi_v - hi_u) is exact for close pairs
+      // (Sterbenz), so FP32 distances keep ~1e-7 relative accuracy
       const T vx = (T)dx, vy = (T)dy, vz = (T)dz;
+      const T lx = (T)(dx - (double)vx), ly = (T)(dy - (double)vy), lz = (T)(dz - (double)vz);
       const T qv = sq[t];
       const int cv = sct[t];
       for (int s = 0; s < ns; ++s) {
-        T ax = fabs(vx - ux[s]), ay = fabs(vy - uy[s]), az = fabs(vz - uz[s]);
+        T ax = fabs(((vx - ux[s]) + lx) - ulo[3 * s]);
+        T ay = fabs(((vy - uy[s]) + ly) - ulo[3 * s + 1]);
+        T az = fabs(((vz - uz[s]) + lz) - ulo[3 * s + 2]);
         ax = (ax > h0) ? L0 - ax : ax;
         ay = (ay > h1) ? L1 - ay : ay;
         az = (az > h2) ? L2 - az : az;
@@ -912,58 +1077,90 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
   }
 }
 
-// Sequential scalar part of update_histogram over the configurations of a
-// batch, in order: weights (wgtslf, volcorrect), engnorm, engsmpl, avslf,
-// the self-energy histogram and its extrema (engproc.F90:776-808).
+// Scalar part of update_histogram over the configurations of a batch
+// (engproc.F90:776-808): each thread forms the weight of one configuration
+// (wgtslf, volcorrect) and its self-energy bin; thread 0 then adds them in
+// configuration order, as the reference's sequential loop does.
 // scal: [0] engnorm [1] engsmpl [2] avslf [3] voffset [4] voffset set
-__global__ void k_scalars(Params P, int n, const double *__restrict__ uv, const double *__restrict__ wsol,
-                          const double *__restrict__ escrd, double *scal, double *eself,
-                          unsigned long long *mnmx, double *wout, int *status) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int b = 0; b < n; ++b) {
-    double u0 = uv[(size_t)b * (P.slvmax + 1)];
-    double engnmfc;
-    if (P.wgtslf == 0) {
-      engnmfc = 1.0;
-    } else {
-      if (scal[4] == 0.0) {
-        scal[3] = u0;
-        scal[4] = 1.0;
-      }
-      double f = u0 - scal[3];
-      engnmfc = (P.slttype == 1) ? exp(f / P.temp) : exp(-f / P.temp);
+__global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double *__restrict__ uv,
+                                                  const double *__restrict__ wsol,
+                                                  const double *__restrict__ escrd, double *scal, double *eself,
+                                                  unsigned long long *mnmx, double *wout, int *status) {
+  __shared__ double wsh[NTHR], twsh[NTHR];
+  __shared__ int ish[NTHR];
+  __shared__ double voff;
+  __shared__ unsigned long long mn, mx;
+  if (threadIdx.x == 0) {
+    if (P.wgtslf && scal[4] == 0.0 && n > 0) {
+      scal[3] = uv[0];
+      scal[4] = 1.0;
     }
-    double tw = P.stat_weight_system * (wsol ? wsol[b] : 1.0);
-    if (P.estype == 2) {
-      tw = tw * P.volume;
-      double factor = 3.1415926535897932 * (P.total_charge * P.total_charge) / (P.screen * P.screen) /
-                      P.volume / 2.0;
-      tw = tw * exp(factor / P.temp);
-    }
-    engnmfc = engnmfc * tw;
-    scal[0] = scal[0] + engnmfc;
-    scal[1] = scal[1] + 1.0;
-    scal[2] = scal[2] + tw;
-    if (P.selfcal) {
-      int lo = 0, hi = P.esmax;
-      while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (escrd[mid] < u0)
-          lo = mid + 1;
-        else
-          hi = mid;
+    voff = scal[3];
+    mn = mnmx[0];
+    mx = mnmx[1];
+  }
+  __syncthreads();
+  for (int b0 = 0; b0 < n; b0 += NTHR) {
+    const int b = b0 + threadIdx.x;
+    if (b < n) {
+      double u0 = uv[(size_t)b * (P.slvmax + 1)];
+      double engnmfc = 1.0;
+      if (P.wgtslf) {
+        double f = u0 - voff;
+        engnmfc = (P.slttype == 1) ? exp(f / P.temp) : exp(-f / P.temp);
       }
-      if (lo <= 0) {
-        set_error(status, 1, b, -1, u0);
-      } else {
-        if (lo == P.esmax && u0 < 1.0e3 && !P.force) set_error(status, 3, b, -1, u0);
-        eself[lo - 1] = eself[lo - 1] + engnmfc;
+      double tw = P.stat_weight_system * (wsol ? wsol[b] : 1.0);
+      if (P.estype == 2) {
+        tw = tw * P.volume;
+        double factor = 3.1415926535897932 * (P.total_charge * P.total_charge) / (P.screen * P.screen) /
+                        P.volume / 2.0;
+        tw = tw * exp(factor / P.temp);
       }
+      engnmfc = engnmfc * tw;
+      int bin = -1;
+      if (P.selfcal) {
+        int lo = 0, hi = P.esmax;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (escrd[mid] < u0)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (lo <= 0) {
+          set_error(status, 1, b, -1, u0);
+        } else {
+          if (lo == P.esmax && u0 < 1.0e3 && !P.force) set_error(status, 3, b, -1, u0);
+          bin = lo - 1;
+        }
+      }
+      wsh[threadIdx.x] = engnmfc;
+      twsh[threadIdx.x] = tw;
+      ish[threadIdx.x] = bin;
+      wout[b] = engnmfc;
+      unsigned long long o = dbl_to_ord(u0);
+      atomicMin(&mn, o);
+      atomicMax(&mx, o);
     }
-    unsigned long long o = dbl_to_ord(u0);
-    if (o < mnmx[0]) mnmx[0] = o;
-    if (o > mnmx[1]) mnmx[1] = o;
-    wout[b] = engnmfc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int m = min(NTHR, n - b0);
+      double en = scal[0], es = scal[1], av = scal[2];
+      for (int i = 0; i < m; ++i) {
+        en = en + wsh[i];
+        es = es + 1.0;
+        av = av + twsh[i];
+        if (ish[i] >= 0) eself[ish[i]] = eself[ish[i]] + wsh[i];
+      }
+      scal[0] = en;
+      scal[1] = es;
+      scal[2] = av;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mnmx[0] = mn;
+    mnmx[1] = mx;
   }
 }
 
@@ -973,9 +1170,13 @@ __global__ void k_edens(int n, int ermax, const double *__restrict__ hist, const
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ermax) return;
   double e = edens[i];
-  for (int b = 0; b < n; ++b) {
-    double h = hist[(size_t)b * ermax + i];
-    if (h != 0.0) e = e + w[b] * h;
+  for (int b0 = 0; b0 < n; b0 += 8) {
+    double h[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) h[u] = (b0 + u < n) ? hist[(size_t)(b0 + u) * ermax + i] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (h[u] != 0.0) e = e + w[b0 + u] * h[u];
   }
   edens[i] = e;
 }

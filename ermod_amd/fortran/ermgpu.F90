@@ -19,7 +19,7 @@ module ermgpu
      integer(c_int32_t) :: ms(3)
      integer(c_int32_t) :: corrcal, selfcal, wgtslf, force_calculation
      real(c_double) :: elecut, lwljcut, upljcut, screen, temp, block_threshold
-     integer(c_int32_t) :: precision, max_batch, device
+     integer(c_int32_t) :: precision, max_batch, device, pme_path
   end type erm_params
 
   type, bind(C) :: erm_hist

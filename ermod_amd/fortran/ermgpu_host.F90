@@ -35,6 +35,7 @@ program ermgpu_host
   read(10) prm%slttype, prm%estype, prm%ljswitch, prm%splodr, prm%ms, prm%corrcal, prm%selfcal, &
        prm%wgtslf, prm%force_calculation, prm%elecut, prm%lwljcut, prm%upljcut, prm%screen, &
        prm%temp, prm%block_threshold, prm%precision, prm%max_batch, prm%device
+  prm%pme_path = 0
   read(10) hst%ecdmin, hst%ecfmns, hst%ecdcen, hst%eccore, hst%ecdmax, hst%eclbin, hst%ecfbin, &
        hst%ec0bin, hst%finfac, hst%pecore
   allocate(mol_begin(nmol + 1), charge(natom), ljtype(natom), lj_eps(ntype * ntype), &

@@ -307,3 +307,42 @@ def test_soln300_fp32_large_system():
     flips = np.count_nonzero(np.searchsorted(edges, res["fp32"][1:]) != np.searchsorted(edges, want[1:]))
     print(f"soln300 fp32 max |dE| {err.max():.3e} kcal/mol, bin flips {flips}/{len(err)}")
     assert ok.all(), err.max()
+
+
+@pytest.mark.parametrize("path", ["fused", "unfused"])
+def test_pme_paths_agree(refs, path):
+    """Both reciprocal-space paths (fused slab; plane + gather) match the
+    oracle in FP64 and FP32."""
+    print_case(f"pme path {path}")
+    sy, pos = refs
+    o = O.Oracle(sy.system)
+    o.set_frame(pos, sy.cell)
+    P = poses(sy, 3, 44)
+    for prec in ("fp64", "fp32"):
+        with ErmGPU(sy.system, prec, pme_path=path) as g:
+            g.set_frame(pos, sy.cell)
+            for sp in P:
+                want, _ = o.recpcal(sy.solute_mol, sp)
+                got, _ = g.recpcal(sy.solute_mol, sp)
+                if prec == "fp64":
+                    np.testing.assert_allclose(got, want, **TOL64)
+                else:
+                    ok, err = fp32_ok(got, want)
+                    assert ok.all(), err.max()
+
+
+def test_fp32_error_breakdown(refs):
+    """Real-space and reciprocal FP32 errors separately (printed)."""
+    print_case("fp32 error breakdown")
+    sy, pos = refs
+    P = poses(sy, 8, 45)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        with ErmGPU(sy.system, prec) as g:
+            g.set_frame(pos, sy.cell)
+            res[prec] = ([g.realcal_proc(sy.solute_mol, sp) for sp in P],
+                         [g.recpcal(sy.solute_mol, sp)[0] for sp in P])
+    er = max(np.max(np.abs(a - b)[np.abs(b) < 100]) for a, b in zip(res["fp32"][0], res["fp64"][0]))
+    ek = max(np.max(np.abs(a - b)) for a, b in zip(res["fp32"][1], res["fp64"][1]))
+    print(f"fp32 max |dE| real {er:.3e}  recip {ek:.3e}")
+    assert er < 1e-4 and ek < 1e-4
