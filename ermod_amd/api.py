@@ -149,7 +149,7 @@ class ErmGPU:
         P.precision = ERM_FP32 if precision == "fp32" else ERM_FP64
         P.max_batch = max_batch
         P.device = device
-        P.pme_path = {"auto": 0, "fused": 1, "unfused": 2}[pme_path]
+        P.pme_path = {"auto": 0, "fused": 1, "unfused": 2, "register": 3}[pme_path]
         h = C.c_void_p()
         st = lib().erm_create(C.byref(P), C.byref(h))
         if st:

@@ -217,6 +217,9 @@ struct erm_ctx {
   int slab_S = 0, slab_R = 0; // fused PME slab: planes per workgroup, registers per thread
   DBuf d_tw1, d_tw2, d_tw3, d_engfac;
   DBuf d_edges, d_specoff, d_uvmax, d_uvsoft, d_escrd;
+  DBuf d_w3h;
+  int xslab_Q = 0;            // planes per x-c2r group of the register path
+  size_t lds_cap = 64 * 1024; // dynamic LDS a kernel may request
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
       d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
   DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
@@ -225,7 +228,7 @@ struct erm_ctx {
             &d_sltlj, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
             &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
             &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
-            &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
+            &d_w3h, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
             &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm};
   }
@@ -384,6 +387,76 @@ void frame_params(erm_ctx *c) {
   k.stat_weight_system = c->stat_weight_system;
 }
 
+// grid lengths with compile-time register FFT kernels
+template <typename T> bool reg_len(int K) {
+  switch (K) {
+    case 16: case 24: case 32: case 48: case 64: return true;
+    case 96: return sizeof(T) == 4;
+    default: return false;
+  }
+}
+
+template <typename T> bool reg_path_ok(const erm_ctx *c) {
+  const Params &k = c->kp;
+  return reg_len<T>(k.K1) && reg_len<T>(k.K2) && reg_len<T>(k.K3) && (k.K1 % 2 == 0);
+}
+
+template <typename T, int K>
+void launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk) {
+  using V = typename C2<T>::type;
+  hipLaunchKernelGGL((k_pme_zcol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<T>(c->d_engfac),
+                     dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<V>(c->d_w3h), dp<V>(c->d_tw3),
+                     dp<V>(c->d_Hk), dp<double>(c->d_selfp));
+}
+template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, int n) {
+  using V = typename C2<T>::type;
+  int nblk = (k.K3 * k.H + NTHR - 1) / NTHR;
+  hipLaunchKernelGGL((k_pme_ycol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_tw2),
+                     dp<V>(c->d_Hk));
+}
+template <typename T, int K, int R> void launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
+  using V = typename C2<T>::type;
+  const int nslab = (k.K3 + c->slab_S - 1) / c->slab_S;
+  hipLaunchKernelGGL((k_pme_xslab<T, K, R>), dim3(nslab, n), dim3(NTHR), lds, c->st, k, c->slab_S, c->xslab_Q,
+                     c->nsites, dp<V>(c->d_Hk), dp<V>(c->d_tw1), dp<int>(c->d_zstart), dp<int>(c->d_zg),
+                     dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite), dp<double>(c->d_esite));
+}
+template <typename T, int K> void launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds) {
+  if (c->slab_R == 16)
+    launch_xslab_r<T, K, 16>(c, k, n, lds);
+  else
+    launch_xslab_r<T, K, 32>(c, k, n, lds);
+}
+
+#define ERM_KSWITCH(K, CALL)                  \
+  switch (K) {                                \
+    case 16: { constexpr int KK = 16; CALL; } break; \
+    case 24: { constexpr int KK = 24; CALL; } break; \
+    case 32: { constexpr int KK = 32; CALL; } break; \
+    case 48: { constexpr int KK = 48; CALL; } break; \
+    case 64: { constexpr int KK = 64; CALL; } break; \
+    case 96: if constexpr (sizeof(T) == 4) { constexpr int KK = 96; CALL; } break; \
+    default: break;                           \
+  }
+
+// raises the dynamic LDS limit of every xslab instantiation of precision T
+template <typename T> bool allow_big_lds() {
+  bool ok = true;
+  const int cap = 160 * 1024;
+  auto set = [&](const void *f) {
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, cap) != hipSuccess) ok = false;
+  };
+#define ERM_SETX(KV)                                   \
+  set((const void *)k_pme_xslab<T, KV, 16>);           \
+  set((const void *)k_pme_xslab<T, KV, 32>);
+  ERM_SETX(16) ERM_SETX(24) ERM_SETX(32) ERM_SETX(48) ERM_SETX(64)
+  if constexpr (sizeof(T) == 4) {
+    ERM_SETX(96)
+  }
+#undef ERM_SETX
+  return ok;
+}
+
 template <typename T> int frame_prep(erm_ctx *c) {
   Params &k = c->kp;
   const int ncell = k.nb[0] * k.nb[1] * k.nb[2];
@@ -480,8 +553,17 @@ template <typename T> int frame_prep(erm_ctx *c) {
     }
     if (c->slab_S) break;
   }
-  if (c->P.pme_path == 1 && !c->slab_S)
+  if ((c->P.pme_path == 1 || c->P.pme_path == 3) && !c->slab_S)
     return fail(c, ERM_E_UNSUPPORTED, "fused PME slab does not fit this grid / system");
+  // register path: planes per x group within the LDS the kernel may use
+  c->xslab_Q = 0;
+  if (c->slab_S && reg_path_ok<T>(c)) {
+    size_t plane_b = sizeof(T) * (size_t)k.K1 * k.K2;
+    int Q = (int)std::min<size_t>(8, c->lds_cap / plane_b);
+    if (Q >= 1) c->xslab_Q = Q;
+  }
+  if (c->P.pme_path == 3 && !c->xslab_Q)
+    return fail(c, ERM_E_UNSUPPORTED, "register-column PME path does not support this grid");
   return ERM_OK;
 }
 
@@ -516,13 +598,15 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   TRY(ensure(c, c->d_sqv, sizeof(T) * (size_t)n * ns));
   TRY(ensure(c, c->d_w1h, sizeof(V) * (size_t)n * ns * H));
   TRY(ensure(c, c->d_w2h, sizeof(V) * (size_t)n * ns * K2));
+  const bool regp = (flags & DO_RECIP) && c->xslab_Q > 0 && (c->P.pme_path == 0 || c->P.pme_path == 3);
+  if (regp) TRY(ensure(c, c->d_w3h, sizeof(V) * (size_t)n * ns * K3));
   TRY(ensure(c, c->d_uv, sizeof(double) * (size_t)n * (S + 1)));
   TRY(ensure(c, c->d_sltj, sizeof(int) * n));
   TRY(ensure(c, c->d_selfreal, sizeof(double) * n));
   TRY(ensure(c, c->d_usreal, sizeof(double)));
   const size_t vb = sizeof(V);
   const int ncolwg = zcol_width(K3, vb);
-  const int nzg = (H * K2 + ncolwg - 1) / ncolwg;
+  const int nzg = regp ? (H * K2 + NTHR - 1) / NTHR : (H * K2 + ncolwg - 1) / ncolwg;
   TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
   if (flags & DO_RECIP) TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
   HIPCHK(c, hipEventRecord(c->ev[0], c->st));
@@ -549,11 +633,29 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     }
   TRY(upload(c, c->d_sltj, sltj.data(), n));
   hipLaunchKernelGGL(k_solute_prep<T>, dim3(n), dim3(128), 0, c->st, k, dp<double>(c->d_spos),
-                     dp<double>(c->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<double>(c->d_sref),
-                     dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h));
+                     dp<double>(c->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<V>(c->d_tw3),
+                     dp<double>(c->d_sref), dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv),
+                     dp<V>(c->d_w1h), dp<V>(c->d_w2h), regp ? dp<V>(c->d_w3h) : (V *)nullptr);
   TRY(launch_check(c, "k_solute_prep"));
   HIPCHK(c, hipEventRecord(c->ev[1], c->st));
-  if (flags & DO_RECIP) {
+  if (regp) {
+    ERM_KSWITCH(K3, (launch_zcol_reg<T, KK>(c, k, n, nzg)));
+    TRY(launch_check(c, "k_pme_zcol_reg"));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+    ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
+    TRY(launch_check(c, "k_pme_ycol_reg"));
+    TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
+    const size_t lx = sizeof(T) * (size_t)c->xslab_Q * K1 * K2;
+    ERM_KSWITCH(K1, (launch_xslab<T, KK>(c, k, n, lx)));
+    TRY(launch_check(c, "k_pme_xslab"));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+    Params kg = k;
+    if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
+    hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
+                       dp<double>(c->d_esite), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
+                       dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
+    TRY(launch_check(c, "k_mol_sum"));
+  } else if (flags & DO_RECIP) {
     size_t lz = zcol_lds(ncolwg, K3, vb);
     hipLaunchKernelGGL(k_pme_zcol<T>, dim3(nzg, n), dim3(NTHR), lz, c->st, k, ncolwg, dp<T>(c->d_engfac),
                        dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h),
@@ -563,7 +665,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     size_t lp = 2 * (size_t)K2 * H * vb;
     Params kg = k;
     if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
-    if (c->slab_S > 0) {
+    if (c->slab_S > 0 && c->P.pme_path != 2) {
       const int nslab = (K3 + c->slab_S - 1) / c->slab_S;
       TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
       if (c->slab_R == 16)
@@ -856,7 +958,7 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
   if (P.splodr < 2 || P.splodr > MAXP) return ERM_E_UNSUPPORTED;
   if (P.ljswitch < 0 || P.ljswitch > 3) return ERM_E_ARG;
   if (P.precision != ERM_FP64 && P.precision != ERM_FP32) return ERM_E_ARG;
-  if (P.pme_path < 0 || P.pme_path > 2) return ERM_E_ARG;
+  if (P.pme_path < 0 || P.pme_path > 3) return ERM_E_ARG;
   FftAxis ax[3];
   for (int d = 0; d < 3; ++d)
     if (P.ms[d] < 2 || P.ms[d] > 512 || P.ms[d] < P.splodr || !fft_axis(P.ms[d], ax[d])) return ERM_E_UNSUPPORTED;
@@ -931,6 +1033,7 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
       }
     }
   }
+  c->lds_cap = (c->fp32 ? allow_big_lds<float>() : allow_big_lds<double>()) ? (size_t)150 * 1024 : (size_t)64 * 1024;
   int s = build_tw(c, c->d_tw1, k.K1);
   if (!s) s = build_tw(c, c->d_tw2, k.K2);
   if (!s) s = build_tw(c, c->d_tw3, k.K3);
@@ -1186,7 +1289,8 @@ int32_t erm_recpcal(erm_ctx *c, int32_t solute, const double *pos, double *erecp
     // sum of the column partials of the single configuration
     const int vb = c->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
     const int ncolwg = zcol_width(c->kp.K3, vb);
-    const int nzg = (c->kp.H * c->kp.K2 + ncolwg - 1) / ncolwg;
+    const bool regp = c->xslab_Q > 0 && (c->P.pme_path == 0 || c->P.pme_path == 3);
+    const int nzg = regp ? (c->kp.H * c->kp.K2 + NTHR - 1) / NTHR : (c->kp.H * c->kp.K2 + ncolwg - 1) / ncolwg;
     std::vector<double> part(nzg);
     HIPCHK(c, hipMemcpy(part.data(), c->d_selfp.p, sizeof(double) * nzg, hipMemcpyDeviceToHost));
     double s = 0.0;

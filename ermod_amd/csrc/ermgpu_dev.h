@@ -157,3 +157,115 @@ __device__ __forceinline__ double ord_to_dbl(unsigned long long u) {
 }
 
 }  // namespace erm
+
+namespace erm {
+
+// ---------------------------------------------------------------------------
+// Register-resident FFT of compile-time length N (factors 4, 2, 3, 5): one
+// thread transforms one column held in registers; all indices are
+// compile-time, so the arrays stay in VGPRs.  Stockham stages as in lds_fft;
+// twiddles come from the axis table tw (e^{-2 pi i j / N}), conjugated for
+// the backward transform.
+// ---------------------------------------------------------------------------
+template <int N> struct Fact {
+  static constexpr int nf() {
+    int n = N, c = 0;
+    for (int r : {4, 2, 3, 5})
+      while (n % r == 0) {
+        n /= r;
+        ++c;
+      }
+    return c;
+  }
+  static constexpr int f(int s) {
+    int n = N, c = 0;
+    for (int r : {4, 2, 3, 5})
+      while (n % r == 0) {
+        n /= r;
+        if (c == s) return r;
+        ++c;
+      }
+    return 1;
+  }
+  static constexpr int span(int s) {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= f(i);
+    return p;
+  }
+  static constexpr bool valid() {
+    int n = N;
+    for (int r : {4, 2, 3, 5})
+      while (n % r == 0) n /= r;
+    return n == 1;
+  }
+};
+
+template <int R, int SIGN, typename V>
+__device__ __forceinline__ void reg_butterfly(V (&x)[R], V *o, int p, const V *__restrict__ tw, int N) {
+  if constexpr (R == 4) {
+    V s0 = cadd(x[0], x[2]), d0 = csub(x[0], x[2]);
+    V s1 = cadd(x[1], x[3]), d1 = csub(x[1], x[3]);
+    V jd1;
+    jd1.x = -SIGN * d1.y;
+    jd1.y = SIGN * d1.x;
+    o[0] = cadd(s0, s1);
+    o[p] = cadd(d0, jd1);
+    o[2 * p] = csub(s0, s1);
+    o[3 * p] = csub(d0, jd1);
+  } else if constexpr (R == 2) {
+    o[0] = cadd(x[0], x[1]);
+    o[p] = csub(x[0], x[1]);
+  } else {
+    const int rs = N / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      V acc = x[0];
+#pragma unroll
+      for (int s = 1; s < R; ++s) {
+        V w = tw[((s * q) % R) * rs];
+        if (SIGN > 0) w.y = -w.y;
+        acc = cadd(acc, cmul(x[s], w));
+      }
+      o[q * p] = acc;
+    }
+  }
+}
+
+template <int N, int S, int SIGN, typename V>
+__device__ __forceinline__ void reg_stages(V (&a)[N], V (&b)[N], const V *__restrict__ tw) {
+  if constexpr (S < Fact<N>::nf()) {
+    constexpr int R = Fact<N>::f(S);
+    constexpr int p = Fact<N>::span(S);
+    constexpr int m = N / R;
+    constexpr int ts = N / (p * R);
+#pragma unroll
+    for (int i = 0; i < m; ++i) {
+      const int k = i % p;
+      V x[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        V v = a[i + q * m];
+        if (q && k) {
+          V w = tw[(q * k * ts) % N];
+          if (SIGN > 0) w.y = -w.y;
+          v = cmul(v, w);
+        }
+        x[q] = v;
+      }
+      reg_butterfly<R, SIGN>(x, b + (i - k) * R + k, p, tw, N);
+    }
+    reg_stages<N, S + 1, SIGN>(b, a, tw);
+  }
+}
+
+// in-place (from the caller's view) transform of x[N]
+template <int N, int SIGN, typename V> __device__ __forceinline__ void reg_fft(V (&x)[N], const V *__restrict__ tw) {
+  V y[N];
+  reg_stages<N, 0, SIGN>(x, y, tw);
+  if constexpr (Fact<N>::nf() % 2 == 1) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = y[i];
+  }
+}
+
+}  // namespace erm

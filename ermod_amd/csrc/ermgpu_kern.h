@@ -247,9 +247,11 @@ __global__ __launch_bounds__(128) void k_solute_prep(Params P, const double *__r
                                                      const double *__restrict__ qsolute,
                                                      const typename C2<T>::type *__restrict__ tw1,
                                                      const typename C2<T>::type *__restrict__ tw2,
+                                                     const typename C2<T>::type *__restrict__ tw3,
                                                      double *sref, int *sg3, T *sw3, T *sqv,
                                                      typename C2<T>::type *w1h,
-                                                     typename C2<T>::type *w2h) {
+                                                     typename C2<T>::type *w2h,
+                                                     typename C2<T>::type *w3h) {
 #pragma clang fp contract(off)
   using V = typename C2<T>::type;
   const int b = blockIdx.x, ns = P.ns, p = P.p;
@@ -319,14 +321,14 @@ __global__ __launch_bounds__(128) void k_solute_prep(Params P, const double *__r
       sqv[(size_t)b * ns + s] = (T)qsolute[s];
       for (int k = 0; k < p; ++k) sw3[((size_t)b * ns + s) * p + k] = (T)wsh[(t * 3 + 2) * MAXP + k];
     }
-    // spectra along axes 1 and 2
-    int nm = P.H + P.K2;
+    // spectra along the three axes (axis 3 only when w3h is requested)
+    int nm = P.H + P.K2 + (w3h ? P.K3 : 0);
     for (int t = threadIdx.x; t < nc * nm; t += blockDim.x) {
       int sl = t / nm, mm = t - sl * nm;
-      int d = (mm < P.H) ? 0 : 1;
-      int m = (d == 0) ? mm : mm - P.H;
+      int d = (mm < P.H) ? 0 : (mm < P.H + P.K2) ? 1 : 2;
+      int m = (d == 0) ? mm : (d == 1) ? mm - P.H : mm - P.H - P.K2;
       int K = Ks[d];
-      const V *tw = (d == 0) ? tw1 : tw2;
+      const V *tw = (d == 0) ? tw1 : (d == 1) ? tw2 : tw3;
       int g = gsh[sl * 3 + d];
       V acc;
       acc.x = 0;
@@ -341,8 +343,10 @@ __global__ __launch_bounds__(128) void k_solute_prep(Params P, const double *__r
       int s = s0c + sl;
       if (d == 0)
         w1h[((size_t)b * ns + s) * P.H + m] = acc;
-      else
+      else if (d == 1)
         w2h[((size_t)b * ns + s) * P.K2 + m] = acc;
+      else
+        w3h[((size_t)b * ns + s) * P.K3 + m] = acc;
     }
   }
 }
@@ -590,6 +594,202 @@ __global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
           sum += zw[(size_t)(p + c2) * np + idx] * rs;
         }
         acc[r] += (double)(f1 * sum);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) {
+    int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+    if (idx < s_end) e_site[(size_t)b * np + zsite[idx]] = acc[r];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Register-column PME path.  Three kernels, each thread owning one column in
+// registers (compile-time lengths, no LDS in the transforms):
+//  k_pme_zcol_reg  column (m1, m2): forward z transform of the solute's
+//                  separable spectrum, sum_s q_s what1_s(m1) what2_s(m2)
+//                  what3_s(m3) (a direct sum over sites: the r2c transform of
+//                  the spread grid without the grid), Green's function and
+//                  self energy (recpcal.F90:377-398), inverse z FFT -> Hk
+//  k_pme_ycol_reg  column (z, m1): inverse y FFT in place in Hk
+//  k_pme_xslab     slab of z planes: x c2r of row pairs into an LDS ring of
+//                  planes, then the gather into per-site FP64 sums
+// --------------------------------------------------------------------------
+template <typename T, int K3>
+__global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__restrict__ engfac,
+                                                       const T *__restrict__ sqv,
+                                                       const typename C2<T>::type *__restrict__ w1h,
+                                                       const typename C2<T>::type *__restrict__ w2h,
+                                                       const typename C2<T>::type *__restrict__ w3h,
+                                                       const typename C2<T>::type *__restrict__ tw3,
+                                                       typename C2<T>::type *Hk, double *selfp) {
+  using V = typename C2<T>::type;
+  __shared__ double red[NTHR];
+  const int b = blockIdx.y, ns = P.ns, H = P.H, K2 = P.K2;
+  const int ncol = H * K2;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = c < ncol;
+  const int m1 = act ? c % H : 0, m2 = act ? c / H : 0;
+  V X[K3];
+#pragma unroll
+  for (int m3 = 0; m3 < K3; ++m3) {
+    X[m3].x = 0;
+    X[m3].y = 0;
+  }
+  for (int s = 0; s < ns; ++s) {
+    const size_t bs = (size_t)b * ns + s;
+    V a = cmul(w1h[bs * H + m1], w2h[bs * K2 + m2]);
+    const T q = sqv[bs];
+    a.x *= q;
+    a.y *= q;
+    const V *w3 = w3h + bs * K3;
+#pragma unroll
+    for (int m3 = 0; m3 < K3; ++m3) {
+      const V w = w3[m3];
+      X[m3].x += a.x * w.x - a.y * w.y;
+      X[m3].y += a.x * w.y + a.y * w.x;
+    }
+  }
+  double self = 0.0;
+  const double wgt = (m1 == 0 || (P.K1 % 2 == 0 && m1 == H - 1)) ? 0.5 : 1.0;
+#pragma unroll
+  for (int m3 = 0; m3 < K3; ++m3) {
+    const T g = act ? engfac[((size_t)m3 * K2 + m2) * H + m1] : (T)0;
+    self += wgt * (double)g * ((double)X[m3].x * (double)X[m3].x + (double)X[m3].y * (double)X[m3].y);
+    X[m3].x *= g;
+    X[m3].y *= g;
+  }
+  red[threadIdx.x] = self;
+  reg_fft<K3, +1>(X, tw3);
+  __syncthreads();
+  for (int st = NTHR / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) selfp[(size_t)b * gridDim.x + blockIdx.x] = red[0];
+  if (act) {
+#pragma unroll
+    for (int z = 0; z < K3; ++z) Hk[((size_t)b * K3 + z) * ncol + c] = X[z];
+  }
+}
+
+template <typename T, int K2>
+__global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename C2<T>::type *__restrict__ tw2,
+                                                       typename C2<T>::type *Hk) {
+  using V = typename C2<T>::type;
+  const int b = blockIdx.y, H = P.H, K3 = P.K3;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;  // (z, m1)
+  if (c >= K3 * H) return;
+  const int z = c / H, m1 = c - z * H;
+  V *col = Hk + ((size_t)b * K3 + z) * K2 * H + m1;
+  V X[K2];
+#pragma unroll
+  for (int m2 = 0; m2 < K2; ++m2) X[m2] = col[(size_t)m2 * H];
+  reg_fft<K2, +1>(X, tw2);
+#pragma unroll
+  for (int y = 0; y < K2; ++y) col[(size_t)y * H] = X[y];
+}
+
+template <typename T, int K1, int MAXR>
+__global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int nsites,
+                                                    const typename C2<T>::type *__restrict__ Hk,
+                                                    const typename C2<T>::type *__restrict__ tw1,
+                                                    const int *__restrict__ z_start, const int *__restrict__ zg,
+                                                    const T *__restrict__ zq, const T *__restrict__ zw,
+                                                    const int *__restrict__ zsite, double *e_site) {
+  using V = typename C2<T>::type;
+  constexpr int H = K1 / 2 + 1;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *ring = reinterpret_cast<T *>(lds_raw);  // [Q][K2][K1]
+  const int K2 = P.K2, K3 = P.K3, p = P.p;
+  const int npair = (K2 + 1) / 2;
+  const int plane = K2 * K1;
+  const int b = blockIdx.y;
+  const int zb = blockIdx.x * S, ze = min(zb + S, K3);
+  const int s_beg = z_start[zb], s_end = z_start[ze];
+  double acc[MAXR];
+  int gz[MAXR];
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) {
+    int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+    acc[r] = 0.0;
+    gz[r] = (idx < s_end) ? ((zg[idx] >> 20) & 1023) : -(1 << 20);
+  }
+  const size_t np = (size_t)nsites;
+  for (int zz0 = zb - p + 1; zz0 < ze; zz0 += Q) {
+    const int nq = min(Q, ze - zz0);
+    // x c2r of row pairs (y0, y0 + 1) of the nq planes
+    for (int t = threadIdx.x; t < nq * npair; t += blockDim.x) {
+      const int q = t / npair, pr = t - q * npair;
+      const int z = (((zz0 + q) % K3) + K3) % K3;
+      const int y0 = 2 * pr, y1 = y0 + 1;
+      const V *r0 = Hk + (((size_t)b * K3 + z) * K2 + y0) * H;
+      const V *r1 = r0 + H;
+      const bool has1 = (y1 < K2);
+      V Z[K1];
+#pragma unroll
+      for (int x = 0; x < H; ++x) {
+        V a = r0[x];
+        V cc = has1 ? r1[x] : V{0, 0};
+        if (x == 0 || x == K1 / 2) {
+          a.y = 0;
+          cc.y = 0;
+        }
+        Z[x].x = a.x - cc.y;
+        Z[x].y = a.y + cc.x;
+        if (x > 0 && x < K1 - x) {
+          Z[K1 - x].x = a.x + cc.y;
+          Z[K1 - x].y = -a.y + cc.x;
+        }
+      }
+      reg_fft<K1, +1>(Z, tw1);
+      T *d0 = ring + (size_t)q * plane + (size_t)y0 * K1;
+#pragma unroll
+      for (int x = 0; x < K1; ++x) d0[x] = Z[x].x;
+      if (has1) {
+#pragma unroll
+        for (int x = 0; x < K1; ++x) d0[K1 + x] = Z[x].y;
+      }
+    }
+    __syncthreads();
+    // gather: owned sites whose support g3 - c3 meets planes [zz0, zz0 + nq)
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      const int g3 = gz[r];
+      const int clo = max(0, g3 - (zz0 + nq - 1)), chi = min(p - 1, g3 - zz0);
+      if (clo <= chi) {
+        const int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+        const int gp = zg[idx];
+        const int g1 = gp & 1023, g2 = (gp >> 10) & 1023;
+        T w1[MAXP], w2[MAXP];
+        for (int k = 0; k < p; ++k) {
+          w1[k] = zw[(size_t)k * np + idx];
+          w2[k] = zw[(size_t)(p + k) * np + idx];
+        }
+        const T qv = zq[idx];
+        for (int c3 = clo; c3 <= chi; ++c3) {
+          const T *ph = ring + (size_t)(g3 - c3 - zz0) * plane;
+          T sum = 0;
+          for (int c2 = 0; c2 < p; ++c2) {
+            int y = g2 - c2;
+            y += (y < 0) ? K2 : 0;
+            const T *row = ph + (size_t)y * K1;
+            T rs = 0;
+            if (g1 >= p - 1) {
+              for (int c1 = 0; c1 < p; ++c1) rs += w1[c1] * row[g1 - c1];
+            } else {
+              for (int c1 = 0; c1 < p; ++c1) {
+                int xx = g1 - c1;
+                xx += (xx < 0) ? K1 : 0;
+                rs += w1[c1] * row[xx];
+              }
+            }
+            sum += w2[c2] * rs;
+          }
+          acc[r] += (double)(qv * zw[(size_t)(2 * p + c3) * np + idx] * sum);
+        }
       }
     }
     __syncthreads();

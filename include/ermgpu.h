@@ -67,8 +67,9 @@ typedef struct erm_params {
   int32_t precision;      /* ERM_FP64 or ERM_FP32 pair/PME arithmetic */
   int32_t max_batch;      /* insertions per device batch; 0 picks a default */
   int32_t device;         /* HIP device ordinal */
-  int32_t pme_path;       /* 0 auto, 1 fused slab (inverse FFT + gather in LDS),
-                             2 separate plane and gather kernels */
+  int32_t pme_path;       /* reciprocal-space kernels: 0 auto, 1 fused slab with
+                             LDS FFTs, 2 separate plane and gather kernels,
+                             3 register-column FFTs + fused x/gather slab */
 } erm_params;
 
 /* &hist namelist of one species (engproc.F90:66-67) */

@@ -309,10 +309,10 @@ def test_soln300_fp32_large_system():
     assert ok.all(), err.max()
 
 
-@pytest.mark.parametrize("path", ["fused", "unfused"])
+@pytest.mark.parametrize("path", ["fused", "unfused", "register"])
 def test_pme_paths_agree(refs, path):
-    """Both reciprocal-space paths (fused slab; plane + gather) match the
-    oracle in FP64 and FP32."""
+    """Every reciprocal-space path (LDS slab; plane + gather; register
+    columns) matches the oracle in FP64 and FP32."""
     print_case(f"pme path {path}")
     sy, pos = refs
     o = O.Oracle(sy.system)
