@@ -55,7 +55,7 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_set_topology", "erm_set_bins", "erm_engclear", "erm_set_frame", "erm_set_frame_device",
            "erm_get_uv_energy", "erm_realcal_proc", "erm_realcal_self", "erm_recpcal", "erm_engconst",
            "erm_insert_random", "erm_update_histogram", "erm_engstore", "erm_fetch", "erm_sizes",
-           "erm_stage_times")
+           "erm_stage_times", "erm_solute_gradient")
 
 _lib = None
 
@@ -93,6 +93,8 @@ def lib():
         L.erm_fetch.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.erm_sizes.argtypes = [vp, _i32p, _i32p]
         L.erm_stage_times.argtypes = [vp, _f64p]
+        L.erm_solute_gradient.argtypes = [vp, C.c_int32, _i32p, _f64p, C.c_uint64, C.c_uint64, _f64p,
+                                          C.c_int32, _f64p, _f64p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -300,6 +302,32 @@ class ErmGPU:
         if self.esmax:
             out["eself"] = eself[:self.esmax]
         return out
+
+    def solute_gradient(self, solutes=None, solute_pos=None, n=None, seed=0, first_index=0,
+                        weights=None, cmbrule=0, q=None, eps=None, sigma=None):
+        """Weighted sums over configurations of the solute-solvent energy u and
+        its gradient with respect to the solute's per-site scaled charge, LJ
+        epsilon and sigma: (sum w, sum w u, sum w du/dq, sum w du/deps,
+        sum w du/dsigma).  soln: ``solutes``; refs: ``solute_pos`` or ``n``
+        random poses from (seed, first_index)."""
+        ns = self.system.nsite(int(self.system.solutes()[0]))
+        sl = None if solutes is None else _i32(solutes)
+        sp = None if solute_pos is None else _f64(solute_pos)
+        if sl is not None:
+            n = len(sl)
+        elif sp is not None:
+            n = sp.size // (3 * ns)
+        if n is None:
+            raise ValueError("give solutes, solute_pos or n")
+        w = None if weights is None else _f64(weights)
+        qq = None if q is None else _f64(q)
+        ee = None if eps is None else _f64(eps)
+        ss = None if sigma is None else _f64(sigma)
+        acc = np.zeros(2 + 3 * ns)
+        self._check(lib().erm_solute_gradient(self.h, int(n), _p(sl, _i32p), _p(sp), C.c_uint64(seed),
+                                              C.c_uint64(first_index), _p(w), int(cmbrule), _p(qq),
+                                              _p(ee), _p(ss), _p(acc)), "solute_gradient")
+        return acc[0], acc[1], acc[2:2 + ns], acc[2 + ns:2 + 2 * ns], acc[2 + 2 * ns:]
 
     def stage_times(self):
         out = np.zeros(8)

@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "ermgpu_kern.h"
+#include "ermgpu_grad.h"
 
 using namespace erm;
 
@@ -218,6 +218,10 @@ struct erm_ctx {
   DBuf d_tw1, d_tw2, d_tw3, d_engfac;
   DBuf d_edges, d_specoff, d_uvmax, d_uvsoft, d_escrd;
   DBuf d_w3h;
+  // solute gradient: per-frame solvent potential grid and batch buffers
+  DBuf d_ggrid, d_gHk, d_gtab, d_gq_t, d_gqd, d_gepsd, d_gpart, d_gphi, d_gub, d_gw, d_gout;
+  uint64_t frame_serial = 0, ggrid_serial = ~0ULL;
+  int ggrid_excl = -2;
   int xslab_Q = 0;            // planes per x-c2r group of the register path
   size_t lds_cap = 64 * 1024; // dynamic LDS a kernel may request
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
@@ -228,7 +232,8 @@ struct erm_ctx {
             &d_sltlj, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
             &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
             &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
-            &d_w3h, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
+            &d_w3h, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
+            &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
             &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm};
   }
@@ -454,6 +459,7 @@ template <typename T> bool allow_big_lds() {
     ERM_SETX(96)
   }
 #undef ERM_SETX
+  set((const void *)k_pair_grad<T>);
   return ok;
 }
 
@@ -581,6 +587,35 @@ struct Batch {
 };
 
 
+// solute configurations of a batch into d_spos (device poses, host
+// coordinates or the frame's solute molecules) and the sorted index of each
+// solute (d_sltj, -1 for a test particle)
+int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
+  const int n = bt.n, ns = c->ns;
+  if (bt.random) {
+    hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, bt.seed, bt.first_index,
+                       dp<double>(c->d_bf), dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
+                       dp<double>(c->d_cellm), dp<double>(c->d_spos));
+    TRY(launch_check(c, "k_pose"));
+  } else if (bt.host_pos) {
+    HIPCHK(c, hipMemcpyAsync(c->d_spos.p, bt.host_pos, sizeof(double) * (size_t)n * ns * 3, hipMemcpyHostToDevice,
+                             c->st));
+  } else {
+    for (int b = 0; b < n; ++b)
+      HIPCHK(c, hipMemcpyAsync(dp<double>(c->d_spos) + (size_t)b * ns * 3,
+                               dp<double>(c->d_pos) + (size_t)3 * c->mol_begin[bt.solutes[b]],
+                               sizeof(double) * ns * 3, hipMemcpyDeviceToDevice, c->st));
+  }
+  std::vector<int> sltj(n, -1);
+  if (bt.solutes)
+    for (int b = 0; b < n; ++b) {
+      int kk = c->kof[bt.solutes[b]];
+      sltj[b] = (kk >= 0) ? c->jof_k[kk] : -1;
+    }
+  TRY(upload(c, c->d_sltj, sltj.data(), n));
+  return ERM_OK;
+}
+
 // Energies of a batch of configurations into d_uv (sorted molecule order);
 // flags select the terms.  Stage events ev[0..6].
 template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
@@ -610,28 +645,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
   if (flags & DO_RECIP) TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
   HIPCHK(c, hipEventRecord(c->ev[0], c->st));
-  // configurations
-  if (bt.random) {
-    hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, bt.seed, bt.first_index,
-                       dp<double>(c->d_bf), dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
-                       dp<double>(c->d_cellm), dp<double>(c->d_spos));
-    TRY(launch_check(c, "k_pose"));
-  } else if (bt.host_pos) {
-    HIPCHK(c, hipMemcpyAsync(c->d_spos.p, bt.host_pos, sizeof(double) * (size_t)n * ns * 3, hipMemcpyHostToDevice,
-                             c->st));
-  } else {
-    for (int b = 0; b < n; ++b)
-      HIPCHK(c, hipMemcpyAsync(dp<double>(c->d_spos) + (size_t)b * ns * 3,
-                               dp<double>(c->d_pos) + (size_t)3 * c->mol_begin[bt.solutes[b]],
-                               sizeof(double) * ns * 3, hipMemcpyDeviceToDevice, c->st));
-  }
-  std::vector<int> sltj(n, -1);
-  if (bt.solutes)
-    for (int b = 0; b < n; ++b) {
-      int kk = c->kof[bt.solutes[b]];
-      sltj[b] = (kk >= 0) ? c->jof_k[kk] : -1;
-    }
-  TRY(upload(c, c->d_sltj, sltj.data(), n));
+  TRY(stage_configs(c, bt, k));
   hipLaunchKernelGGL(k_solute_prep<T>, dim3(n), dim3(128), 0, c->st, k, dp<double>(c->d_spos),
                      dp<double>(c->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<V>(c->d_tw3),
                      dp<double>(c->d_sref), dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv),
@@ -864,6 +878,7 @@ int set_frame_common(erm_ctx *c, const double *cell, double w) {
   inverted_cell(c->cell, c->invcl, &c->volume);
   for (int d = 0; d < 3; ++d) c->L[d] = std::fabs(cell[4 * d]);
   c->stat_weight_system = w;
+  ++c->frame_serial;
   Params &k = c->kp;
   // the cutoff must stay below half the box (single minimum image)
   bool changed = !c->have_prevcl;
@@ -884,6 +899,135 @@ int set_frame_common(erm_ctx *c, const double *cell, double w) {
   c->have_prevcl = true;
   frame_params(c);
   TRY(upload(c, c->d_cellm, c->cell, 9));
+  return ERM_OK;
+}
+
+// ---- solute gradient ---------------------------------------------------
+template <typename T, int K> void launch_xr2c(erm_ctx *c, const Params &k) {
+  using V = typename C2<T>::type;
+  const int nt = k.K3 * ((k.K2 + 1) / 2);
+  hipLaunchKernelGGL((k_xrow_r2c<T, K>), dim3((nt + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k,
+                     dp<T>(c->d_ggrid), dp<V>(c->d_tw1), dp<V>(c->d_gHk));
+}
+template <typename T, int K> void launch_xc2r(erm_ctx *c, const Params &k) {
+  using V = typename C2<T>::type;
+  const int nt = k.K3 * ((k.K2 + 1) / 2);
+  hipLaunchKernelGGL((k_xrow_c2r<T, K>), dim3((nt + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k,
+                     dp<V>(c->d_gHk), dp<V>(c->d_tw1), dp<T>(c->d_ggrid));
+}
+template <typename T, int K, int SIGN> void launch_ydense(erm_ctx *c, const Params &k) {
+  using V = typename C2<T>::type;
+  const int nt = k.K3 * k.H;
+  hipLaunchKernelGGL((k_pme_ycol_reg<T, K, SIGN>), dim3((nt + NTHR - 1) / NTHR, 1), dim3(NTHR), 0, c->st, k,
+                     dp<V>(c->d_tw2), dp<V>(c->d_gHk));
+}
+template <typename T, int K> void launch_zgreen(erm_ctx *c, const Params &k) {
+  using V = typename C2<T>::type;
+  const int nt = k.H * k.K2;
+  hipLaunchKernelGGL((k_zcol_green<T, K>), dim3((nt + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k,
+                     dp<T>(c->d_engfac), dp<V>(c->d_tw3), dp<V>(c->d_gHk));
+}
+
+// PME potential of every physical molecule except sorted molecule `excl`
+// on the real grid d_ggrid: spread, r2c, Green's function, c2r (the solvent
+// counterpart of recpcal_prepare_solute, recpcal.F90:342-406); once per frame
+template <typename T> int solvent_grid(erm_ctx *c, int excl) {
+  using V = typename C2<T>::type;
+  if (c->ggrid_serial == c->frame_serial && c->ggrid_excl == excl) return ERM_OK;
+  const Params &k = c->kp;
+  const size_t nr = (size_t)k.K1 * k.K2 * k.K3, nh = (size_t)k.H * k.K2 * k.K3;
+  TRY(ensure(c, c->d_ggrid, sizeof(T) * nr));
+  TRY(ensure(c, c->d_gHk, sizeof(V) * nh));
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((nr + NTHR - 1) / NTHR)), dim3(NTHR), 0, c->st,
+                     dp<T>(c->d_ggrid), nr, (T)0);
+  TRY(launch_check(c, "k_fill"));
+  hipLaunchKernelGGL(k_spread_slv<T>, dim3((c->slvmax + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k, excl,
+                     dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<int>(c->d_sg), dp<T>(c->d_sw),
+                     dp<T>(c->d_sq), dp<T>(c->d_ggrid));
+  TRY(launch_check(c, "k_spread_slv"));
+  ERM_KSWITCH(k.K1, (launch_xr2c<T, KK>(c, k)));
+  TRY(launch_check(c, "k_xrow_r2c"));
+  ERM_KSWITCH(k.K2, (launch_ydense<T, KK, -1>(c, k)));
+  TRY(launch_check(c, "k_pme_ycol_reg"));
+  ERM_KSWITCH(k.K3, (launch_zgreen<T, KK>(c, k)));
+  TRY(launch_check(c, "k_zcol_green"));
+  ERM_KSWITCH(k.K2, (launch_ydense<T, KK, +1>(c, k)));
+  TRY(launch_check(c, "k_pme_ycol_reg"));
+  ERM_KSWITCH(k.K1, (launch_xc2r<T, KK>(c, k)));
+  TRY(launch_check(c, "k_xrow_c2r"));
+  c->ggrid_serial = c->frame_serial;
+  c->ggrid_excl = excl;
+  return ERM_OK;
+}
+
+struct GradSetup {
+  std::vector<double> q, eps, sig, tab;  // per site; tab [ns][nvt][3]
+  double qslt = 0.0, rf = 0.0;
+  int excl = -1;
+};
+
+// one batch of the gradient: adds [sum w, sum w u, sum w grad] into acc
+template <typename T> int gradient_batch(erm_ctx *c, const Batch &bt, const GradSetup &g, double *acc) {
+  using V = typename C2<T>::type;
+  Params k = c->kp;
+  const int n = bt.n, ns = c->ns, p = k.p;
+  k.ns = ns;
+  TRY(ensure(c, c->d_spos, sizeof(double) * (size_t)n * ns * 3));
+  TRY(ensure(c, c->d_sref, sizeof(double) * 4 * n));
+  TRY(ensure(c, c->d_sg3, sizeof(int) * (size_t)n * ns));
+  TRY(ensure(c, c->d_sw3, sizeof(T) * (size_t)n * ns * p));
+  TRY(ensure(c, c->d_sqv, sizeof(T) * (size_t)n * ns));
+  TRY(ensure(c, c->d_w1h, sizeof(V) * (size_t)n * ns * k.H));
+  TRY(ensure(c, c->d_w2h, sizeof(V) * (size_t)n * ns * k.K2));
+  TRY(ensure(c, c->d_sltj, sizeof(int) * n));
+  TRY(stage_configs(c, bt, k));
+  hipLaunchKernelGGL(k_solute_prep<T>, dim3(n), dim3(128), 0, c->st, k, dp<double>(c->d_spos),
+                     dp<double>(c->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<V>(c->d_tw3),
+                     dp<double>(c->d_sref), dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv),
+                     dp<V>(c->d_w1h), dp<V>(c->d_w2h), (V *)nullptr);
+  TRY(launch_check(c, "k_solute_prep"));
+  // real space
+  double Rg = k.rcmax + 8.0;
+  double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
+  int nsplit = std::max(1, std::min(16, (int)std::ceil(frac * c->slvmax / (2.0 * NTHR))));
+  const int nsl = std::max(1, NTHR / ns);
+  size_t lds = sizeof(double) * 4 * (size_t)nsl * ns + sizeof(T) * ((size_t)7 * ns + (size_t)3 * ns * c->nvt) +
+               sizeof(int) * (2 * MAXSEG + 2) + 64;
+  if (lds > c->lds_cap) return fail(c, ERM_E_UNSUPPORTED, "solute too large for the gradient kernel's LDS");
+  TRY(ensure(c, c->d_gpart, sizeof(double) * (size_t)n * nsplit * 4 * ns));
+  hipLaunchKernelGGL(k_pair_grad<T>, dim3(n, nsplit), dim3(NTHR), lds, c->st, k, dp<double>(c->d_spos),
+                     dp<double>(c->d_sref), dp<T>(c->d_gq_t), dp<T>(c->d_gtab), dp<int>(c->d_cell_start),
+                     dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                     dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                     dp<double>(c->d_rm), dp<double>(c->d_gpart));
+  TRY(launch_check(c, "k_pair_grad"));
+  // reciprocal space
+  TRY(solvent_grid<T>(c, g.excl));
+  TRY(ensure(c, c->d_gphi, sizeof(double) * (size_t)n * ns));
+  hipLaunchKernelGGL(k_grad_recip<T>, dim3((n * ns + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k, n,
+                     dp<double>(c->d_spos), dp<T>(c->d_ggrid), dp<double>(c->d_gphi));
+  TRY(launch_check(c, "k_grad_recip"));
+  // weights and sums
+  std::vector<double> w(n, c->stat_weight_system);
+  if (bt.weights)
+    for (int b = 0; b < n; ++b) w[b] = c->stat_weight_system * bt.weights[b];
+  TRY(ensure(c, c->d_gw, sizeof(double) * n));
+  TRY(upload(c, c->d_gw, w.data(), n));
+  TRY(ensure(c, c->d_gub, sizeof(double) * n));
+  hipLaunchKernelGGL(k_grad_ub, dim3((n + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k, n, nsplit,
+                     dp<double>(c->d_gpart), dp<double>(c->d_gphi), dp<double>(c->d_gqd), g.rf, g.qslt,
+                     dp<double>(c->d_gub));
+  TRY(launch_check(c, "k_grad_ub"));
+  const int nout = 2 + 3 * ns;
+  TRY(ensure(c, c->d_gout, sizeof(double) * nout));
+  hipLaunchKernelGGL(k_grad_reduce, dim3((nout + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k, n, nsplit,
+                     dp<double>(c->d_gpart), dp<double>(c->d_gphi), dp<double>(c->d_gub), dp<double>(c->d_gw),
+                     dp<double>(c->d_gepsd), g.rf, dp<double>(c->d_gout));
+  TRY(launch_check(c, "k_grad_reduce"));
+  std::vector<double> out(nout);
+  HIPCHK(c, hipMemcpyAsync(out.data(), c->d_gout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  for (int i = 0; i < nout; ++i) acc[i] += out[i];
   return ERM_OK;
 }
 
@@ -1461,6 +1605,90 @@ int32_t erm_sizes(const erm_ctx *c, int32_t *slvmax, int32_t *ermax) {
 int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
   if (!c || !ms8) return ERM_E_ARG;
   for (int i = 0; i < 8; ++i) ms8[i] = c->stage_ms[i];
+  return ERM_OK;
+}
+
+int32_t erm_solute_gradient(erm_ctx *c, int32_t n, const int32_t *solutes, const double *solute_pos, uint64_t seed,
+                            uint64_t first_index, const double *weights, int32_t cmbrule, const double *q,
+                            const double *eps, const double *sigma, double *acc) {
+  TRY(need(c, true, false));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (!acc) return fail(c, ERM_E_ARG, "acc is NULL");
+  if (n < 0) return fail(c, ERM_E_ARG, "negative count");
+  if (cmbrule != 0 && cmbrule != 1) return fail(c, ERM_E_ARG, "cmbrule must be 0 or 1");
+  if (c->P.wgtslf != 0 || c->P.estype != 1)
+    return fail(c, ERM_E_UNSUPPORTED, "the gradient needs parameter-independent weights (wgtslf 0, estype 1)");
+  const int ns = c->ns;
+  if (ns > 1024) return fail(c, ERM_E_UNSUPPORTED, "gradient for solutes of more than 1024 sites");
+  const Params &k = c->kp;
+  bool regok = c->fp32 ? reg_path_ok<float>(c) : reg_path_ok<double>(c);
+  if (!regok) return fail(c, ERM_E_UNSUPPORTED, "gradient needs PME grid lengths 16, 24, 32, 48, 64 (96 in FP32)");
+  bool random = false;
+  if (c->P.slttype == 1) {
+    TRY(check_solutes(c, n, solutes, nullptr));
+    if (solute_pos) return fail(c, ERM_E_UNSUPPORTED, "soln mode takes coordinates from the frame");
+    if (c->sltlist.size() != 1) return fail(c, ERM_E_UNSUPPORTED, "gradient in soln mode needs a single solute");
+  } else {
+    if (!solute_pos) {
+      if (c->P.slttype != 2 || c->bfcoord.empty())
+        return fail(c, ERM_E_ARG, "random poses need a rigid test particle with bfcoord");
+      random = true;
+    }
+  }
+  // per-site parameters and the solute x solvent LJ table
+  GradSetup g;
+  const int a0 = c->mol_begin[c->slt_template], nt = c->ntype;
+  g.q.resize(ns);
+  g.eps.resize(ns);
+  g.sig.resize(ns);
+  for (int s = 0; s < ns; ++s) {
+    int t = c->ljtype[a0 + s];
+    g.q[s] = q ? q[s] : c->charge[a0 + s];
+    g.eps[s] = eps ? eps[s] : c->lj_eps[(size_t)t * nt + t];
+    g.sig[s] = sigma ? sigma[s] : std::sqrt(c->lj_sig2[(size_t)t * nt + t]);
+    if (!(g.eps[s] >= 0.0) || !(g.sig[s] > 0.0)) return fail(c, ERM_E_ARG, "site %d: eps < 0 or sigma <= 0", s);
+  }
+  g.tab.resize((size_t)3 * ns * c->nvt);
+  for (int s = 0; s < ns; ++s) {
+    int tu = c->ljtype[a0 + s];
+    for (int v = 0; v < c->nvt; ++v) {
+      int tv = c->vtypes[v];
+      double ev = c->lj_eps[(size_t)tv * nt + tv], sv = std::sqrt(c->lj_sig2[(size_t)tv * nt + tv]);
+      double eiv = std::sqrt(g.eps[s] * ev);
+      double s2 = cmbrule ? g.sig[s] * sv : ipow_h(0.5 * (g.sig[s] + sv), 2);
+      if (!eps && !sigma) {
+        // the topology's table must be this rule applied to its diagonals
+        double te = c->lj_eps[(size_t)tv * nt + tu], ts = c->lj_sig2[(size_t)tv * nt + tu];
+        if (std::fabs(te - eiv) > 1e-10 * std::fabs(eiv) + 1e-300 || std::fabs(ts - s2) > 1e-10 * s2)
+          return fail(c, ERM_E_UNSUPPORTED, "LJ tables do not follow combination rule %d", cmbrule);
+      }
+      double* e = &g.tab[((size_t)s * c->nvt + v) * 3];
+      e[0] = eiv;
+      e[1] = s2;
+      e[2] = cmbrule ? 1.0 / g.sig[s] : 2.0 / (g.sig[s] + sv);
+    }
+  }
+  for (int s = 0; s < ns; ++s) g.qslt += g.q[s];
+  double qsolv = c->total_charge;
+  if (c->P.slttype == 1) {
+    int m = c->sltlist[0];
+    qsolv = qsolv - c->mol_charge[m];
+    g.excl = c->jof_k[c->kof[m]];
+  }
+  g.rf = -3.1415926535897932 * qsolv / c->P.screen / c->P.screen / c->volume;
+  TRY(upload_real(c, c->d_gtab, g.tab));
+  TRY(upload_real(c, c->d_gq_t, g.q));
+  TRY(upload(c, c->d_gqd, g.q.data(), ns));
+  TRY(upload(c, c->d_gepsd, g.eps.data(), ns));
+  (void)k;
+  for (int s0 = 0; s0 < n; s0 += c->Bmax) {
+    Batch bt{std::min(c->Bmax, n - s0), s0, nullptr, nullptr, nullptr, random, seed, first_index + (uint64_t)s0};
+    if (c->P.slttype == 1) bt.solutes = solutes + s0;
+    else if (!random) bt.host_pos = solute_pos + (size_t)s0 * ns * 3;
+    if (weights) bt.weights = weights + s0;
+    int st = c->fp32 ? gradient_batch<float>(c, bt, g, acc) : gradient_batch<double>(c, bt, g, acc);
+    if (st) return st;
+  }
   return ERM_OK;
 }
 

@@ -613,7 +613,7 @@ __global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
 //                  what3_s(m3) (a direct sum over sites: the r2c transform of
 //                  the spread grid without the grid), Green's function and
 //                  self energy (recpcal.F90:377-398), inverse z FFT -> Hk
-//  k_pme_ycol_reg  column (z, m1): inverse y FFT in place in Hk
+//  k_pme_ycol_reg  column (z, m1): y FFT in place in Hk (inverse by default)
 //  k_pme_xslab     slab of z planes: x c2r of row pairs into an LDS ring of
 //                  planes, then the gather into per-site FP64 sums
 // --------------------------------------------------------------------------
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__rest
   }
 }
 
-template <typename T, int K2>
+template <typename T, int K2, int SIGN = +1>
 __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename C2<T>::type *__restrict__ tw2,
                                                        typename C2<T>::type *Hk) {
   using V = typename C2<T>::type;
@@ -687,7 +687,7 @@ __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename 
   V X[K2];
 #pragma unroll
   for (int m2 = 0; m2 < K2; ++m2) X[m2] = col[(size_t)m2 * H];
-  reg_fft<K2, +1>(X, tw2);
+  reg_fft<K2, SIGN>(X, tw2);
 #pragma unroll
   for (int y = 0; y < K2; ++y) col[(size_t)y * H] = X[y];
 }
@@ -988,47 +988,17 @@ __device__ void block_inclusive_scan(int *a, int n) {
   __syncthreads();
 }
 
-template <typename T>
-__global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restrict__ spos,
-                                               const double *__restrict__ sref,
-                                               const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
-                                               const int *__restrict__ cell_start,
-                                               const int *__restrict__ mol_first,
-                                               const int *__restrict__ mol_ns, const double *__restrict__ sx,
-                                               const double *__restrict__ sy, const double *__restrict__ sz,
-                                               const T *__restrict__ sq, const int *__restrict__ sct,
-                                               const int *__restrict__ slt_j, const double *__restrict__ rm_ord_p,
-                                               double *uv) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int ns = P.ns, nvt = P.nvt;
-  T *ux = reinterpret_cast<T *>(lds_raw);
-  T *uy = ux + ns;
-  T *uz = uy + ns;
-  T *uq = uz + ns;
-  T *ulj = uq + ns;                       // [ns][nvt][2] (eps, sig2)
-  T *ulo = ulj + 2 * ns * nvt;            // [ns][3] low parts of the offsets
-  int *seg_beg = reinterpret_cast<int *>(ulo + 3 * ns);
-  int *seg_pre = seg_beg + MAXSEG;        // MAXSEG + 1 prefix sums
-  const int b = blockIdx.x;
-  const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
-  const double Rs = sref[4 * b + 3];
-  const double *xs = spos + (size_t)b * ns * 3;
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-    ux[s] = (T)(xs[3 * s] - C0);
-    uy[s] = (T)(xs[3 * s + 1] - C1);
-    uz[s] = (T)(xs[3 * s + 2] - C2v);
-    uq[s] = slt_q[s];
-    // low parts of the solute offsets (zero in FP64)
-    ulo[3 * s] = (T)((xs[3 * s] - C0) - (double)ux[s]);
-    ulo[3 * s + 1] = (T)((xs[3 * s + 1] - C1) - (double)uy[s]);
-    ulo[3 * s + 2] = (T)((xs[3 * s + 2] - C2v) - (double)uz[s]);
-  }
-  for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
-  // pillars within reach of the centre
-  const double Rm = ord_to_dbl(*(const unsigned long long *)rm_ord_p);
-  const double R = P.rcmax + Rs + Rm + 1e-6 * (1.0 + P.rcmax);
+// Candidate molecules of a configuration with (wrapped) centre Cw and reach
+// R: pillars (y, z) of the molecule cell grid whose distance from Cw is at
+// most R, each with the x range of cells within reach, as up to two
+// contiguous ranges of cell-sorted molecules.  Every pillar owns two fixed
+// segment slots (empty when unused), so all workgroups of one configuration
+// flatten the candidates in the same order.  seg_beg[i] is the first
+// molecule of segment i and seg_pre[i] the number of candidates before it;
+// returns the segment count nsg (seg_pre[nsg] = total).  All threads call.
+__device__ int pillar_segments(const Params &P, const double Cw[3], double R, const int *__restrict__ cell_start,
+                               int *seg_beg, int *seg_pre) {
   const double R2 = R * R;
-  const double Cw[3] = {wrap1(C0, P.L[0]), wrap1(C1, P.L[1]), wrap1(C2v, P.L[2])};
   int lo[3], cnt[3];
   for (int d = 1; d < 3; ++d) {
     int n = (int)ceil(R / P.hcell[d]) + 1;
@@ -1040,8 +1010,6 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
       cnt[d] = 2 * n + 1;
     }
   }
-  // every pillar owns two fixed segment slots (empty when unused), so all
-  // workgroups of one configuration flatten the candidates in the same order
   const int nsg = 2 * cnt[1] * cnt[2];
   for (int t = threadIdx.x; t < nsg; t += blockDim.x) {
     seg_beg[t] = 0;
@@ -1097,6 +1065,50 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
   __syncthreads();
   if (threadIdx.x == 0) seg_pre[0] = 0;
   block_inclusive_scan(seg_pre + 1, nsg);
+  return nsg;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restrict__ spos,
+                                               const double *__restrict__ sref,
+                                               const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
+                                               const int *__restrict__ cell_start,
+                                               const int *__restrict__ mol_first,
+                                               const int *__restrict__ mol_ns, const double *__restrict__ sx,
+                                               const double *__restrict__ sy, const double *__restrict__ sz,
+                                               const T *__restrict__ sq, const int *__restrict__ sct,
+                                               const int *__restrict__ slt_j, const double *__restrict__ rm_ord_p,
+                                               double *uv) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int ns = P.ns, nvt = P.nvt;
+  T *ux = reinterpret_cast<T *>(lds_raw);
+  T *uy = ux + ns;
+  T *uz = uy + ns;
+  T *uq = uz + ns;
+  T *ulj = uq + ns;                       // [ns][nvt][2] (eps, sig2)
+  T *ulo = ulj + 2 * ns * nvt;            // [ns][3] low parts of the offsets
+  int *seg_beg = reinterpret_cast<int *>(ulo + 3 * ns);
+  int *seg_pre = seg_beg + MAXSEG;        // MAXSEG + 1 prefix sums
+  const int b = blockIdx.x;
+  const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
+  const double Rs = sref[4 * b + 3];
+  const double *xs = spos + (size_t)b * ns * 3;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    ux[s] = (T)(xs[3 * s] - C0);
+    uy[s] = (T)(xs[3 * s + 1] - C1);
+    uz[s] = (T)(xs[3 * s + 2] - C2v);
+    uq[s] = slt_q[s];
+    // low parts of the solute offsets (zero in FP64)
+    ulo[3 * s] = (T)((xs[3 * s] - C0) - (double)ux[s]);
+    ulo[3 * s + 1] = (T)((xs[3 * s + 1] - C1) - (double)uy[s]);
+    ulo[3 * s + 2] = (T)((xs[3 * s + 2] - C2v) - (double)uz[s]);
+  }
+  for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
+  // pillars within reach of the centre
+  const double Rm = ord_to_dbl(*(const unsigned long long *)rm_ord_p);
+  const double R = P.rcmax + Rs + Rm + 1e-6 * (1.0 + P.rcmax);
+  const double Cw[3] = {wrap1(C0, P.L[0]), wrap1(C1, P.L[1]), wrap1(C2v, P.L[2])};
+  const int nsg = pillar_segments(P, Cw, R, cell_start, seg_beg, seg_pre);
   const int total = seg_pre[nsg];
   const int sj = slt_j[b];
   const T L0 = (T)P.L[0], L1 = (T)P.L[1], L2 = (T)P.L[2];

@@ -169,6 +169,33 @@ def mol_charges(charge: np.ndarray, mol_begin: np.ndarray) -> np.ndarray:
     return out
 
 
+def site_lj(system: ErmSystem):
+    """Per-site LJ epsilon and sigma: the diagonals of the combined tables
+    (both combination rules give sigma_ii^2 = sigma_i^2)."""
+    t = system.ljtype
+    return system.lj_eps[t, t].copy(), np.sqrt(system.lj_sig2[t, t])
+
+
+def with_solute_params(system: ErmSystem, mol: int, q=None, eps=None, sigma=None,
+                       cmbrule: int = 0) -> ErmSystem:
+    """A copy of ``system`` whose molecule ``mol`` has per-site charges
+    ``q`` (scaled units), LJ ``eps`` and ``sigma`` (None keeps a field);
+    types and tables are rebuilt with combination rule ``cmbrule``
+    (setconf.F90:755-823), charges re-summed per molecule."""
+    b, e = int(system.mol_begin[mol]), int(system.mol_begin[mol + 1])
+    charge = system.charge.astype(np.float64).copy()
+    ep, sg = site_lj(system)
+    if q is not None:
+        charge[b:e] = np.asarray(q, dtype=np.float64)
+    if eps is not None:
+        ep[b:e] = np.asarray(eps, dtype=np.float64)
+    if sigma is not None:
+        sg[b:e] = np.asarray(sigma, dtype=np.float64)
+    ljtype, epsm, sig2 = lj_tables(ep, sg, cmbrule)
+    return dataclasses.replace(system, charge=charge, ljtype=ljtype, lj_eps=epsm, lj_sig2=sig2,
+                               mol_charge=mol_charges(charge, system.mol_begin))
+
+
 def cubic_cell(L: float) -> np.ndarray:
     """3x3 cell, columns = box vectors, stored column-major as 9 values."""
     return np.diag([L, L, L]).astype(np.float64)

@@ -193,6 +193,28 @@ int32_t erm_sizes(const erm_ctx *ctx, int32_t *slvmax, int32_t *ermax);
  * [6] correlation, [7] total. */
 int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
 
+/* ---- differentiation ---------------------------------------------------- */
+/* Gradient of the solute-solvent energy u = sum_{k>=1} uvengy(k) (the sum
+ * get_uv_energy, engproc.F90:662-756, leaves in uvengy(1:slvmax)) with
+ * respect to the solute's per-site scaled charge q_i, LJ epsilon_i and
+ * sigma_i.  The reference has no gradient; this is the adjoint of its energy:
+ * real space over the pairs of realcal.F90:527-784, reciprocal space as the
+ * solvent's PME potential at each solute site (recpcal.F90:342-502 is a
+ * symmetric bilinear form), residual_ene (engproc.F90:859-869).
+ * Configurations as in erm_engconst (soln: `solutes`, single solute type;
+ * refs: solute_pos, or NULL for device random poses from seed/first_index as
+ * in erm_insert_random).  q/eps/sigma (ns each, NULL = the topology's
+ * values) override the solute's parameters; solvent LJ parameters are the
+ * diagonals of the tables, combined by cmbrule (0 arithmetic sigma,
+ * 1 geometric; epsilon geometric, setconf.F90:810-820).  Adds into
+ * acc[2 + 3*ns]: {sum w, sum w u, sum w du/dq[ns], sum w du/deps[ns],
+ * sum w du/dsigma[ns]} with w = stat_weight_system * weights[i] (NULL = 1).
+ * Needs wgtslf = 0 and estype = 1 (weights independent of the parameters)
+ * and register-FFT grid lengths (16, 24, 32, 48, 64; 96 in FP32). */
+int32_t erm_solute_gradient(erm_ctx *ctx, int32_t n, const int32_t *solutes, const double *solute_pos,
+                            uint64_t seed, uint64_t first_index, const double *weights, int32_t cmbrule,
+                            const double *q, const double *eps, const double *sigma, double *acc);
+
 #ifdef __cplusplus
 }
 #endif

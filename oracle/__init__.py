@@ -96,6 +96,7 @@ def lib():
         L.orc_get_acc.argtypes = [vp, C.POINTER(OrcAcc)]
         L.orc_flceng.argtypes = [vp]
         L.orc_flceng.restype = _f64p
+        L.orc_solute_gradient.argtypes = [vp, C.c_int32, _f64p, C.c_int32, _f64p, _f64p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -287,6 +288,19 @@ class Oracle:
         check(lib().orc_recpcal(self.h, int(solute), P(sp), P(out), C.byref(self_e), P(phi)),
               "recpcal")
         return (out, self_e.value, phi) if want_phi else (out, self_e.value)
+
+    def solute_gradient(self, solute, solute_pos=None, cmbrule=0, q=None, eps=None, sigma=None):
+        """(u, du/dq, du/deps, du/dsigma) of one configuration; q in scaled
+        units, None parameters taken from the topology."""
+        ns = int(self.system.mol_begin[solute + 1] - self.system.mol_begin[solute])
+        out = np.zeros(1 + 3 * ns)
+        sp = None if solute_pos is None else f64(solute_pos)
+        qq = None if q is None else f64(q)
+        ee = None if eps is None else f64(eps)
+        ss = None if sigma is None else f64(sigma)
+        check(lib().orc_solute_gradient(self.h, int(solute), P(sp), int(cmbrule), P(qq), P(ee), P(ss),
+                                        P(out)), "solute_gradient")
+        return out[0], out[1:1 + ns], out[1 + ns:1 + 2 * ns], out[1 + 2 * ns:]
 
     def update_histogram(self, solute, uvengy, weight=1.0):
         ins = np.zeros(self.ermax, dtype=np.int32)

@@ -1484,3 +1484,178 @@ void orc_get_acc(orc_ctx *c, orc_acc *o) {
   o->numslv = c->s.numslv;
   o->voffset_initialized = c->voffset_init;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Gradient of the solute-solvent energy u = sum_{k>=1} uvengy(k) with     */
+/* respect to the solute's per-site charge q_i, LJ epsilon_i and sigma_i. */
+/* The reference has no gradient; this is the analytic derivative of the  */
+/* energy that get_uv_energy (engproc.F90:662-756) computes:              */
+/*  - real space (realcal.F90:676-779): pairs under the minimum image,    */
+/*    every LJ variant written as rep - att with rep proportional to      */
+/*    sigma^12 and att to sigma^6, so dU/d(sigma^2) = (6 rep - 3 att) /   */
+/*    sigma^2; epsilon_iv = sqrt(eps_i eps_v) (setconf.F90:820), sigma_iv */
+/*    arithmetic (cmbrule 0) or geometric (1) (setconf.F90:810-817);      */
+/*  - reciprocal space: the PME energy is the symmetric bilinear form     */
+/*    q_slt^T W^T Phi W q_slv (recpcal.F90:342-502), so dE/dq_i is the    */
+/*    solvent's PME potential interpolated at site i;                      */
+/*  - residual_ene (engproc.F90:859-869): -pi Q_slt Q_j / (alpha^2 V).     */
+/* Solvent LJ parameters are the diagonals of the type tables.            */
+/* ---------------------------------------------------------------------- */
+static void lj_parts(const orc_system *s, const ljconst *k, double eps, double sig2, double dist, double *rep,
+                     double *att) {
+  double invr2 = sig2 / dist;
+  double invr6 = invr2 * invr2 * invr2;
+  double sg6 = sig2 * sig2 * sig2;
+  int low = dist <= k->lw2;
+  switch (s->ljswitch) {
+    case 0:
+    case 1: {
+      double sw = 1.0;
+      if (!low) {
+        if (s->ljswitch == 0) {
+          sw = (2.0 * dist + k->up2 - 3.0 * k->lw2) * ipow(dist - k->up2, 2) / ipow(k->up2 - k->lw2, 3);
+        } else {
+          double x = (sqrt(dist) - s->lwljcut) / (s->upljcut - s->lwljcut);
+          sw = 1.0 - 10.0 * ipow(x, 3) + 15.0 * ipow(x, 4) - 6.0 * ipow(x, 5);
+        }
+      }
+      *rep = 4.0 * eps * invr6 * invr6 * sw;
+      *att = 4.0 * eps * invr6 * sw;
+      return;
+    }
+    case 2:
+      if (low) {
+        *rep = 4.0 * eps * (invr6 * invr6 - sg6 * sg6 / (k->lw6 * k->up6));
+        *att = 4.0 * eps * (invr6 - sg6 / (k->lw3 * k->up3));
+      } else {
+        double invr3 = sqrt(invr6), sg3 = sqrt(sg6);
+        *rep = 4.0 * eps * k->up6 / (k->up6 - k->lw6) * ipow(invr6 - sg6 / k->up6, 2);
+        *att = 4.0 * eps * k->up3 / (k->up3 - k->lw3) * ipow(invr3 - sg3 / k->up3, 2);
+      }
+      return;
+    default: {
+      double a = k->repC, b = k->attC;
+      if (!low) {
+        double sw = sqrt(dist) - s->lwljcut;
+        a = k->repA * ipow(sw, 3) + k->repB * ipow(sw, 4) + k->repC;
+        b = k->attA * ipow(sw, 3) + k->attB * ipow(sw, 4) + k->attC;
+      }
+      *rep = 4.0 * eps * (invr6 * invr6 - sg6 * sg6 * a);
+      *att = 4.0 * eps * (invr6 - sg6 * b);
+      return;
+    }
+  }
+}
+
+int32_t orc_solute_gradient(orc_ctx *c, int32_t solute, const double *solute_pos, int32_t cmbrule,
+                            const double *q_in, const double *eps_in, const double *sig_in, double *out) {
+  int ns, st = solute_coords(c, solute, solute_pos, &ns);
+  if (st) return st;
+  if (cmbrule != 0 && cmbrule != 1) return ORC_E_ARG;
+  const orc_system *s = &c->s;
+  const int nt = s->ntype, b0 = c->mol_begin[solute];
+  double *q = (double *)malloc(sizeof(double) * 3 * ns), *ep = q + ns, *sg = q + 2 * ns;
+  for (int i = 0; i < ns; ++i) {
+    int t = c->ljtype[b0 + i];
+    q[i] = q_in ? q_in[i] : c->charge[b0 + i];
+    ep[i] = eps_in ? eps_in[i] : c->lj_eps[t * nt + t];
+    sg[i] = sig_in ? sig_in[i] : sqrt(c->lj_sig2[t * nt + t]);
+  }
+  double *gq = out + 1, *ge = out + 1 + ns, *gs = out + 1 + 2 * ns;
+  double u = 0.0;
+  for (int i = 0; i < 3 * ns; ++i) out[1 + i] = 0.0;
+  double *elj = (double *)calloc((size_t)ns, sizeof(double));
+  /* real space */
+  ljconst k;
+  make_ljconst(s, &k);
+  double half[3] = {0.5 * c->L[0], 0.5 * c->L[1], 0.5 * c->L[2]};
+  double qsolv = 0.0;
+  for (int kk = 0; kk < c->slvmax; ++kk) {
+    int m = c->tagpt[kk];
+    if (m == solute) continue;
+    qsolv += c->mol_charge[m];
+    for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a) {
+      int tv = c->ljtype[a];
+      double ev = c->lj_eps[tv * nt + tv], sv = sqrt(c->lj_sig2[tv * nt + tv]);
+      for (int i = 0; i < ns; ++i) {
+        double dist = minimg2(c->sposn + 3 * i, c->posn + 3 * a, half);
+        double eiv = sqrt(ep[i] * ev);
+        double s2 = cmbrule ? sg[i] * sv : ipow(0.5 * (sg[i] + sv), 2);
+        if (eiv > 0 && dist <= k.up2) {
+          double rep, att;
+          lj_parts(s, &k, eiv, s2, dist, &rep, &att);
+          double f = cmbrule ? 1.0 / sg[i] : 2.0 / (sg[i] + sv);
+          u += rep - att;
+          elj[i] += rep - att;
+          gs[i] += (6.0 * rep - 3.0 * att) * f;
+        }
+        if (dist <= k.el2) {
+          double r = sqrt(dist);
+          double phi = c->charge[a] * (1.0 - erf(s->screen * r)) / r;
+          u += q[i] * phi;
+          gq[i] += phi;
+        }
+      }
+    }
+  }
+  for (int i = 0; i < ns; ++i) ge[i] = ep[i] > 0 ? elj[i] / (2.0 * ep[i]) : 0.0;
+  /* reciprocal space: solvent potential on the grid, gathered at the sites */
+  int p = c->p, K1 = c->K[0], K2 = c->K[1], K3 = c->K[2];
+  long nr = (long)K1 * K2 * K3, nh = (long)c->H * K2 * K3;
+  memset(c->cnv, 0, sizeof(double) * nr);
+  for (int kk = 0; kk < c->slvmax; ++kk) {
+    int m = c->tagpt[kk];
+    if (m == solute) continue;
+    int svi = c->slvtag[m];
+    if (svi < 0) {
+      free(q);
+      free(elj);
+      return ORC_E_RCP_CNS;
+    }
+    for (int sid = 0; sid < c->mol_begin[m + 1] - c->mol_begin[m]; ++sid) {
+      const double *sw = c->splslv + (long)3 * p * (svi + sid);
+      const int32_t *gg = c->grdslv + 3 * (svi + sid);
+      double chr = c->charge[c->mol_begin[m] + sid];
+      for (int c3 = 0; c3 < p; ++c3)
+        for (int c2 = 0; c2 < p; ++c2)
+          for (int c1 = 0; c1 < p; ++c1) {
+            int r1 = ((gg[0] - c1) % K1 + K1) % K1;
+            int r2 = ((gg[1] - c2) % K2 + K2) % K2;
+            int r3 = ((gg[2] - c3) % K3 + K3) % K3;
+            c->cnv[((long)r3 * K2 + r2) * K1 + r1] += chr * sw[c1] * sw[p + c2] * sw[2 * p + c3];
+          }
+    }
+  }
+  orc_fft_r2c_3d(c->K, c->cnv, c->rcp);
+  for (long i = 0; i < nh; ++i) {
+    c->rcp[2 * i] *= c->engfac[i];
+    c->rcp[2 * i + 1] *= c->engfac[i];
+  }
+  orc_fft_c2r_3d(c->K, c->rcp, c->cnv);
+  double w[3 * 32];
+  int32_t g[3];
+  for (int i = 0; i < ns; ++i) {
+    orc_spline_site(c->spos + 3 * i, c->invcl, c->K, p, c->coeff, w, g);
+    double phi = 0.0;
+    for (int c3 = 0; c3 < p; ++c3)
+      for (int c2 = 0; c2 < p; ++c2)
+        for (int c1 = 0; c1 < p; ++c1) {
+          int r1 = ((g[0] - c1) % K1 + K1) % K1;
+          int r2 = ((g[1] - c2) % K2 + K2) % K2;
+          int r3 = ((g[2] - c3) % K3 + K3) % K3;
+          phi += w[c1] * w[p + c2] * w[2 * p + c3] * c->cnv[((long)r3 * K2 + r2) * K1 + r1];
+        }
+    u += q[i] * phi;
+    gq[i] += phi;
+  }
+  /* residual */
+  double qslt = 0.0;
+  for (int i = 0; i < ns; ++i) qslt += q[i];
+  double rf = -PI_REF * qsolv / s->screen / s->screen / c->volume;
+  u += qslt * rf;
+  for (int i = 0; i < ns; ++i) gq[i] += rf;
+  out[0] = u;
+  free(q);
+  free(elj);
+  return ORC_OK;
+}

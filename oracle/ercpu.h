@@ -159,6 +159,16 @@ void orc_get_acc(orc_ctx *c, orc_acc *out);
 /* per-species sum of solute-solvent energy of the last update (flceng) */
 const double *orc_flceng(const orc_ctx *c);
 
+/* Solute-solvent energy u = sum_{k>=1} uvengy(k) of one configuration and
+ * its gradient with respect to the solute's per-site scaled charges q,
+ * LJ epsilon and sigma (NULL: the topology's values; solvent LJ parameters
+ * are the table diagonals, combined by cmbrule 0 arithmetic / 1 geometric).
+ * out: [u, du/dq (ns), du/deps (ns), du/dsigma (ns)]. Brute-force real
+ * space; reciprocal space through the solvent's PME potential. */
+int32_t orc_solute_gradient(orc_ctx *c, int32_t solute, const double *solute_pos,
+                            int32_t cmbrule, const double *q, const double *eps,
+                            const double *sigma, double *out);
+
 #ifdef __cplusplus
 }
 #endif
