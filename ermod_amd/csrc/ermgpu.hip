@@ -223,7 +223,7 @@ struct erm_ctx {
   uint64_t frame_serial = 0, ggrid_serial = ~0ULL;
   int ggrid_excl = -2;
   int xslab_Q = 0;            // planes per x-c2r group of the register path
-  size_t lds_cap = 64 * 1024; // dynamic LDS a kernel may request
+  size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
       d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
   DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
@@ -287,6 +287,18 @@ int sync(erm_ctx *c) {
 int launch_check(erm_ctx *c, const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, ERM_E_DEVICE, "launch %s: %s", what, hipGetErrorString(e));
+  return ERM_OK;
+}
+
+// raises the dynamic LDS limit of kernel f to `bytes` when it exceeds the
+// default 64 KB; a refused request is cleared from the runtime's error state
+int set_lds(erm_ctx *c, const void *f, size_t bytes, const char *what) {
+  if (bytes <= 64 * 1024) return ERM_OK;
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, ERM_E_UNSUPPORTED, "%s: %zu bytes of LDS refused (%s)", what, bytes, hipGetErrorString(e));
+  }
   return ERM_OK;
 }
 
@@ -407,11 +419,14 @@ template <typename T> bool reg_path_ok(const erm_ctx *c) {
 }
 
 template <typename T, int K>
-void launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk) {
+int launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk) {
   using V = typename C2<T>::type;
-  hipLaunchKernelGGL((k_pme_zcol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<T>(c->d_engfac),
+  const size_t lds = zcol_reg_lds<T>(k.H, k.K2, K);
+  TRY(set_lds(c, (const void *)k_pme_zcol_reg<T, K>, lds, "k_pme_zcol_reg"));
+  hipLaunchKernelGGL((k_pme_zcol_reg<T, K>), dim3(nblk, n), dim3(NTHR), lds, c->st, k, dp<T>(c->d_engfac),
                      dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<V>(c->d_w3h), dp<V>(c->d_tw3),
                      dp<V>(c->d_Hk), dp<double>(c->d_selfp));
+  return ERM_OK;
 }
 template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, int n) {
   using V = typename C2<T>::type;
@@ -419,18 +434,18 @@ template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, i
   hipLaunchKernelGGL((k_pme_ycol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_tw2),
                      dp<V>(c->d_Hk));
 }
-template <typename T, int K, int R> void launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
+template <typename T, int K, int R> int launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
   using V = typename C2<T>::type;
+  TRY(set_lds(c, (const void *)k_pme_xslab<T, K, R>, lds, "k_pme_xslab"));
   const int nslab = (k.K3 + c->slab_S - 1) / c->slab_S;
   hipLaunchKernelGGL((k_pme_xslab<T, K, R>), dim3(nslab, n), dim3(NTHR), lds, c->st, k, c->slab_S, c->xslab_Q,
                      c->nsites, dp<V>(c->d_Hk), dp<V>(c->d_tw1), dp<int>(c->d_zstart), dp<int>(c->d_zg),
                      dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite), dp<double>(c->d_esite));
+  return ERM_OK;
 }
-template <typename T, int K> void launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds) {
-  if (c->slab_R == 16)
-    launch_xslab_r<T, K, 16>(c, k, n, lds);
-  else
-    launch_xslab_r<T, K, 32>(c, k, n, lds);
+template <typename T, int K> int launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds) {
+  if (c->slab_R == 16) return launch_xslab_r<T, K, 16>(c, k, n, lds);
+  return launch_xslab_r<T, K, 32>(c, k, n, lds);
 }
 
 #define ERM_KSWITCH(K, CALL)                  \
@@ -443,25 +458,6 @@ template <typename T, int K> void launch_xslab(erm_ctx *c, const Params &k, int 
     case 96: if constexpr (sizeof(T) == 4) { constexpr int KK = 96; CALL; } break; \
     default: break;                           \
   }
-
-// raises the dynamic LDS limit of every xslab instantiation of precision T
-template <typename T> bool allow_big_lds() {
-  bool ok = true;
-  const int cap = 160 * 1024;
-  auto set = [&](const void *f) {
-    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, cap) != hipSuccess) ok = false;
-  };
-#define ERM_SETX(KV)                                   \
-  set((const void *)k_pme_xslab<T, KV, 16>);           \
-  set((const void *)k_pme_xslab<T, KV, 32>);
-  ERM_SETX(16) ERM_SETX(24) ERM_SETX(32) ERM_SETX(48) ERM_SETX(64)
-  if constexpr (sizeof(T) == 4) {
-    ERM_SETX(96)
-  }
-#undef ERM_SETX
-  set((const void *)k_pair_grad<T>);
-  return ok;
-}
 
 template <typename T> int frame_prep(erm_ctx *c) {
   Params &k = c->kp;
@@ -564,8 +560,10 @@ template <typename T> int frame_prep(erm_ctx *c) {
   // register path: planes per x group within the LDS the kernel may use
   c->xslab_Q = 0;
   if (c->slab_S && reg_path_ok<T>(c)) {
-    size_t plane_b = sizeof(T) * (size_t)k.K1 * k.K2;
+    // one complex plane per ring slot (the real rows are written in place)
+    size_t plane_b = 2 * sizeof(T) * (size_t)k.H * k.K2;
     int Q = (int)std::min<size_t>(8, c->lds_cap / plane_b);
+    Q = std::min(Q, NTHR / ((k.K2 + 1) / 2));
     if (Q >= 1) c->xslab_Q = Q;
   }
   if (c->P.pme_path == 3 && !c->xslab_Q)
@@ -653,14 +651,18 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   TRY(launch_check(c, "k_solute_prep"));
   HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   if (regp) {
-    ERM_KSWITCH(K3, (launch_zcol_reg<T, KK>(c, k, n, nzg)));
+    int rz = ERM_E_UNSUPPORTED;
+    ERM_KSWITCH(K3, (rz = launch_zcol_reg<T, KK>(c, k, n, nzg)));
+    TRY(rz);
     TRY(launch_check(c, "k_pme_zcol_reg"));
     HIPCHK(c, hipEventRecord(c->ev[2], c->st));
     ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
     TRY(launch_check(c, "k_pme_ycol_reg"));
     TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
-    const size_t lx = sizeof(T) * (size_t)c->xslab_Q * K1 * K2;
-    ERM_KSWITCH(K1, (launch_xslab<T, KK>(c, k, n, lx)));
+    const size_t lx = 2 * sizeof(T) * (size_t)c->xslab_Q * H * K2;
+    int rc = ERM_E_UNSUPPORTED;
+    ERM_KSWITCH(K1, (rc = launch_xslab<T, KK>(c, k, n, lx)));
+    TRY(rc);
     TRY(launch_check(c, "k_pme_xslab"));
     HIPCHK(c, hipEventRecord(c->ev[3], c->st));
     Params kg = k;
@@ -994,6 +996,7 @@ template <typename T> int gradient_batch(erm_ctx *c, const Batch &bt, const Grad
   size_t lds = sizeof(double) * 4 * (size_t)nsl * ns + sizeof(T) * ((size_t)7 * ns + (size_t)3 * ns * c->nvt) +
                sizeof(int) * (2 * MAXSEG + 2) + 64;
   if (lds > c->lds_cap) return fail(c, ERM_E_UNSUPPORTED, "solute too large for the gradient kernel's LDS");
+  TRY(set_lds(c, (const void *)k_pair_grad<T>, lds, "k_pair_grad"));
   TRY(ensure(c, c->d_gpart, sizeof(double) * (size_t)n * nsplit * 4 * ns));
   hipLaunchKernelGGL(k_pair_grad<T>, dim3(n, nsplit), dim3(NTHR), lds, c->st, k, dp<double>(c->d_spos),
                      dp<double>(c->d_sref), dp<T>(c->d_gq_t), dp<T>(c->d_gtab), dp<int>(c->d_cell_start),
@@ -1115,6 +1118,15 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
   c->P = P;
   c->fp32 = (P.precision == ERM_FP32);
   c->Bmax = P.max_batch > 0 ? P.max_batch : 256;
+  {
+    // LDS a workgroup may opt into (160 KB per CU on gfx950), with room for
+    // the kernels' static arrays
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, P.device) != hipSuccess)
+      optin = 0;
+    (void)hipGetLastError();
+    c->lds_cap = std::max<size_t>(64 * 1024, std::min<size_t>(150 * 1024, optin > 8192 ? (size_t)optin - 8192 : 0));
+  }
   if (hipStreamCreate(&c->st) != hipSuccess) {
     delete c;
     return ERM_E_DEVICE;
@@ -1177,7 +1189,6 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
       }
     }
   }
-  c->lds_cap = (c->fp32 ? allow_big_lds<float>() : allow_big_lds<double>()) ? (size_t)150 * 1024 : (size_t)64 * 1024;
   int s = build_tw(c, c->d_tw1, k.K1);
   if (!s) s = build_tw(c, c->d_tw2, k.K2);
   if (!s) s = build_tw(c, c->d_tw3, k.K3);

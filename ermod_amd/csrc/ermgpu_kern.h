@@ -617,6 +617,13 @@ __global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
 //  k_pme_xslab     slab of z planes: x c2r of row pairs into an LDS ring of
 //                  planes, then the gather into per-site FP64 sums
 // --------------------------------------------------------------------------
+constexpr int ZREG_SCH = 16;  // solute sites per LDS chunk of k_pme_zcol_reg
+
+// dynamic LDS of k_pme_zcol_reg: ZREG_SCH sites x (H + K2 + K3) spectra + charges
+template <typename T> __host__ __device__ inline size_t zcol_reg_lds(int H, int K2, int K3) {
+  return (size_t)ZREG_SCH * (H + K2 + K3) * 2 * sizeof(T) + ZREG_SCH * sizeof(T) + NTHR * sizeof(double);
+}
+
 template <typename T, int K3>
 __global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__restrict__ engfac,
                                                        const T *__restrict__ sqv,
@@ -626,8 +633,14 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__rest
                                                        const typename C2<T>::type *__restrict__ tw3,
                                                        typename C2<T>::type *Hk, double *selfp) {
   using V = typename C2<T>::type;
-  __shared__ double red[NTHR];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   const int b = blockIdx.y, ns = P.ns, H = P.H, K2 = P.K2;
+  const int nsp = H + K2 + K3;
+  V *l1 = reinterpret_cast<V *>(lds_raw);  // [ZREG_SCH][H]
+  V *l2 = l1 + ZREG_SCH * H;               // [ZREG_SCH][K2]
+  V *l3 = l2 + ZREG_SCH * K2;              // [ZREG_SCH][K3]
+  T *lq = reinterpret_cast<T *>(l3 + ZREG_SCH * K3);
+  double *red = reinterpret_cast<double *>(lds_raw + zcol_reg_lds<T>(H, K2, K3) - NTHR * sizeof(double));
   const int ncol = H * K2;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = c < ncol;
@@ -638,18 +651,35 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__rest
     X[m3].x = 0;
     X[m3].y = 0;
   }
-  for (int s = 0; s < ns; ++s) {
-    const size_t bs = (size_t)b * ns + s;
-    V a = cmul(w1h[bs * H + m1], w2h[bs * K2 + m2]);
-    const T q = sqv[bs];
-    a.x *= q;
-    a.y *= q;
-    const V *w3 = w3h + bs * K3;
+  for (int s0 = 0; s0 < ns; s0 += ZREG_SCH) {
+    const int nc = min(ZREG_SCH, ns - s0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nc * nsp; t += blockDim.x) {
+      const int sl = t / nsp, m = t - sl * nsp;
+      const size_t bs = (size_t)b * ns + s0 + sl;
+      if (m < H)
+        l1[sl * H + m] = w1h[bs * H + m];
+      else if (m < H + K2)
+        l2[sl * K2 + (m - H)] = w2h[bs * K2 + (m - H)];
+      else
+        l3[sl * K3 + (m - H - K2)] = w3h[bs * K3 + (m - H - K2)];
+    }
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) lq[t] = sqv[(size_t)b * ns + s0 + t];
+    __syncthreads();
+    for (int sl = 0; sl < nc; ++sl) {
+      V a = cmul(l1[sl * H + m1], l2[sl * K2 + m2]);
+      const T q = lq[sl];
+      a.x *= q;
+      a.y *= q;
+      const V *w3 = l3 + sl * K3;
 #pragma unroll
-    for (int m3 = 0; m3 < K3; ++m3) {
-      const V w = w3[m3];
-      X[m3].x += a.x * w.x - a.y * w.y;
-      X[m3].y += a.x * w.y + a.y * w.x;
+      for (int m3 = 0; m3 < K3; ++m3) {
+        const V w = w3[m3];
+        X[m3].x = fma(a.x, w.x, X[m3].x);
+        X[m3].x = fma(-a.y, w.y, X[m3].x);
+        X[m3].y = fma(a.x, w.y, X[m3].y);
+        X[m3].y = fma(a.y, w.x, X[m3].y);
+      }
     }
   }
   double self = 0.0;
@@ -692,6 +722,15 @@ __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename 
   for (int y = 0; y < K2; ++y) col[(size_t)y * H] = X[y];
 }
 
+// Slab of z planes [zb, zb + S) of configuration b.  The planes a site
+// touches, [g3 - p + 1, g3], are processed Q at a time: the Q complex planes
+// Hk[b][z][y][m1] are copied into LDS with coalesced loads; thread (q, pr)
+// holds rows y0 = pr and y1 = pr + ceil(K2/2) of plane q in registers, builds
+// the Hermitian-packed complex row y0 + i y1, transforms it (register FFT)
+// and writes the two real rows in place over the complex plane, with a row
+// stride of K1 + 1 (consecutive lanes on distinct LDS banks).  Each thread
+// then adds the planes' contributions to its MAXR owned sites (FP64 sums).
+// Needs Q * ceil(K2/2) <= blockDim.x (one row pair per thread and group).
 template <typename T, int K1, int MAXR>
 __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int nsites,
                                                     const typename C2<T>::type *__restrict__ Hk,
@@ -701,11 +740,12 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
                                                     const int *__restrict__ zsite, double *e_site) {
   using V = typename C2<T>::type;
   constexpr int H = K1 / 2 + 1;
+  constexpr int K1P = K1 + 1;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  T *ring = reinterpret_cast<T *>(lds_raw);  // [Q][K2][K1]
   const int K2 = P.K2, K3 = P.K3, p = P.p;
   const int npair = (K2 + 1) / 2;
-  const int plane = K2 * K1;
+  const int cplane = K2 * H;                  // complex elements per plane
+  V *stage = reinterpret_cast<V *>(lds_raw);  // [Q][K2][H] complex, then [Q][K2][K1P] real in place
   const int b = blockIdx.y;
   const int zb = blockIdx.x * S, ze = min(zb + S, K3);
   const int s_beg = z_start[zb], s_end = z_start[ze];
@@ -720,15 +760,22 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
   const size_t np = (size_t)nsites;
   for (int zz0 = zb - p + 1; zz0 < ze; zz0 += Q) {
     const int nq = min(Q, ze - zz0);
-    // x c2r of row pairs (y0, y0 + 1) of the nq planes
-    for (int t = threadIdx.x; t < nq * npair; t += blockDim.x) {
-      const int q = t / npair, pr = t - q * npair;
+    // coalesced copy of the nq complex planes
+    for (int t = threadIdx.x; t < nq * cplane; t += blockDim.x) {
+      const int q = t / cplane, e = t - q * cplane;
       const int z = (((zz0 + q) % K3) + K3) % K3;
-      const int y0 = 2 * pr, y1 = y0 + 1;
-      const V *r0 = Hk + (((size_t)b * K3 + z) * K2 + y0) * H;
-      const V *r1 = r0 + H;
-      const bool has1 = (y1 < K2);
-      V Z[K1];
+      stage[t] = Hk[((size_t)b * K3 + z) * cplane + e];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    const bool fft = t < nq * npair;
+    const int q = fft ? t / npair : 0, pr = fft ? t - q * npair : 0;
+    const int y0 = pr, y1 = pr + npair;
+    const bool has1 = fft && (y1 < K2);
+    V Z[K1];
+    if (fft) {
+      const V *r0 = stage + (size_t)q * cplane + (size_t)y0 * H;
+      const V *r1 = stage + (size_t)q * cplane + (size_t)(has1 ? y1 : y0) * H;
 #pragma unroll
       for (int x = 0; x < H; ++x) {
         V a = r0[x];
@@ -745,12 +792,17 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
         }
       }
       reg_fft<K1, +1>(Z, tw1);
-      T *d0 = ring + (size_t)q * plane + (size_t)y0 * K1;
+    }
+    __syncthreads();  // every complex row is in registers before the real rows overwrite them
+    T *real = reinterpret_cast<T *>(stage);
+    if (fft) {
+      T *d0 = real + (size_t)q * cplane * 2 + (size_t)y0 * K1P;
 #pragma unroll
       for (int x = 0; x < K1; ++x) d0[x] = Z[x].x;
       if (has1) {
+        T *d1 = real + (size_t)q * cplane * 2 + (size_t)y1 * K1P;
 #pragma unroll
-        for (int x = 0; x < K1; ++x) d0[K1 + x] = Z[x].y;
+        for (int x = 0; x < K1; ++x) d1[x] = Z[x].y;
       }
     }
     __syncthreads();
@@ -770,12 +822,12 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
         }
         const T qv = zq[idx];
         for (int c3 = clo; c3 <= chi; ++c3) {
-          const T *ph = ring + (size_t)(g3 - c3 - zz0) * plane;
+          const T *ph = real + (size_t)(g3 - c3 - zz0) * cplane * 2;
           T sum = 0;
           for (int c2 = 0; c2 < p; ++c2) {
             int y = g2 - c2;
             y += (y < 0) ? K2 : 0;
-            const T *row = ph + (size_t)y * K1;
+            const T *row = ph + (size_t)y * K1P;
             T rs = 0;
             if (g1 >= p - 1) {
               for (int c1 = 0; c1 < p; ++c1) rs += w1[c1] * row[g1 - c1];

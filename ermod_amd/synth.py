@@ -275,4 +275,44 @@ def tiny_soln(nwater=60, L=14.0, K=16, nsite=5, seed=SEED0 + 8, cut=6.5) -> Synt
                  elecut=cut, upljcut=cut, lwljcut=cut - 1.5, screen=0.5, block_threshold=3.0)
 
 
-WORKLOADS = {"soln12": soln12, "refs20": refs20, "soln300": soln300, "soln30": soln30}
+# ions (CHARMM-like Lennard-Jones parameters)
+ION_NA = dict(q=1.0, sig=2.43, eps=0.047, mass=ATOM_MASS["Na"])
+ION_CL = dict(q=-1.0, sig=4.04, eps=0.150, mass=ATOM_MASS["Cl"])
+
+
+def ion_kind(name, d) -> MolKind:
+    return MolKind(name, np.array([d["q"]]), np.array([d["sig"]]), np.array([d["eps"]]),
+                   np.array([d["mass"]]), np.zeros((1, 3)))
+
+
+def mixed_hists():
+    """150 bins per species (600 per solute over the four species): water and
+    cosolvent over [-22.5, 22.5] + 30 log bins to 1e4; the ions' linear range
+    reaches -302.5 (the reference stops with eng_min below the first edge)."""
+    neutral = Hist(ecdmin=-20.0, ecfmns=-5.0, ecdcen=0.0, eccore=15.0, eclbin=1.0, ecfbin=0.25,
+                   ec0bin=0.1, pecore=30)
+    ion = Hist(ecdmin=-300.0, ecfmns=-5.0, ecdcen=0.0, eccore=300.0, eclbin=20.0, ecfbin=0.25,
+               ec0bin=0.1, pecore=30)
+    return [neutral, neutral, ion, ion]
+
+
+MIXED_SOLUTE_SITES = (5, 20, 60, 150)
+
+
+def mixed(nsite=20, nwater=7000, ncosolv=800, nion=100, L=60.0, K=64, seed=SEED0 + 4) -> Synthetic:
+    """Heterogeneous insertion system: 7000 waters, 800 rigid 10-site neutral
+    cosolvent molecules, 100 Na+ and 100 Cl- in a 60 A box; one rigid test
+    particle of ``nsite`` sites (5, 20, 60 or 150).  The solvent of every
+    solute size is identical (same kinds, counts and seed), so frames agree
+    on all solvent sites."""
+    rng = np.random.default_rng(seed)
+    cos = solute_kind(rng, 10, radius=2.0, name="cosolvent")
+    radius = {5: 2.0, 20: 3.5, 60: 5.0, 150: 6.5}.get(nsite, 1.2 * nsite ** (1.0 / 3.0) + 1.0)
+    slt = solute_kind(np.random.default_rng([seed, nsite]), nsite, radius=radius)
+    kinds = [water_kind(), cos, ion_kind("Na", ION_NA), ion_kind("Cl", ION_CL), slt]
+    return build(kinds, [nwater, ncosolv, nion, nion, 1],
+                 [PT_SOLVENT, PT_SOLVENT, PT_SOLVENT, PT_SOLVENT, PT_TEST_RIGID], (K, K, K), 4, L, seed,
+                 SLT_REFS_RIGID, mixed_hists(), f"mixed{nsite}", corrcal=1)
+
+
+WORKLOADS = {"soln12": soln12, "refs20": refs20, "soln300": soln300, "soln30": soln30, "mixed": mixed}

@@ -1,0 +1,64 @@
+"""Heterogeneous insertion: a mixed solvent (water, 10-site cosolvent, Na+,
+Cl-: four species with their own bins) and four rigid test particles of 5,
+20, 60 and 150 sites, K = 64.  Per solute size: device random insertions
+against the oracle on the same poses (FP64 energies to the usual bar, every
+count bit-exact over the four species), FP32 error and bin flips.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ermod_amd import synth
+from ermod_amd.api import ErmGPU
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = dict(rtol=1e-11, atol=1e-9)
+
+
+def print_case(name):
+    print(f"\n[gpu case] {name}", flush=True)
+
+
+@pytest.mark.parametrize("nsite", synth.MIXED_SOLUTE_SITES)
+def test_mixed_insertions_fp64(nsite):
+    print_case(f"mixed solvent, {nsite}-site solute, fp64")
+    sy = synth.mixed(nsite)
+    s = sy.system
+    pos = sy.frame(0)
+    o = O.Oracle(s)
+    o.set_frame(pos, sy.cell)
+    n, seed, first = 12, 404, 5000
+    with ErmGPU(s, "fp64") as g:
+        g.set_frame(pos, sy.cell)
+        u = g.insert_random(n, seed, first_index=first, want_uv=True)
+        b = g.fetch()
+    for i in range(n):
+        r, q = O.random_pose(seed, first + i)
+        sp = O.place_solute(s.bfcoord, s.solute_mass(sy.solute_mol), sy.cell, r, q)
+        want = o.uv_energy(sy.solute_mol, sp)
+        np.testing.assert_allclose(u[i], want, **TOL64)
+        o.update_histogram(sy.solute_mol, want)
+    a = o.acc()
+    assert np.array_equal(a["edens"], b["edens"])
+    assert np.array_equal(a["ecorr"], b["ecorr"])
+    np.testing.assert_allclose(b["minuv"][1:], a["minuv"][1:], rtol=1e-11, atol=1e-9)
+    np.testing.assert_allclose(b["maxuv"][1:], a["maxuv"][1:], rtol=1e-11, atol=1e-9)
+
+
+@pytest.mark.parametrize("nsite", (5, 150))
+def test_mixed_fp32_error(nsite):
+    print_case(f"mixed solvent, {nsite}-site solute, fp32 vs fp64")
+    sy = synth.mixed(nsite)
+    pos = sy.frame(1)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        with ErmGPU(sy.system, prec) as g:
+            g.set_frame(pos, sy.cell)
+            out[prec] = g.insert_random(64, 9, want_uv=True)
+    e32, e64 = out["fp32"][:, 1:], out["fp64"][:, 1:]
+    err = np.abs(e32 - e64)
+    lim = np.where(np.abs(e64) <= 100.0, 1e-4, 1e-5 * np.abs(e64))
+    soft = np.abs(e64) <= 100.0
+    print(f"max |dE| soft {err[soft].max():.3e}")
+    assert (err <= lim).all()
