@@ -34,21 +34,22 @@ FP32_PEAK = 157.3          # TFLOP/s dense FP32
 
 
 def stage_bytes(sy, prec_bytes, n_ins, slvmax, ermax):
-    """Algorithmic HBM bytes per launch (n_ins configurations) of each
-    device stage; the figures are derived in DESIGN.md."""
+    """Algorithmic HBM bytes per launch (n_ins configurations) of the
+    register-path stages: Hk = the half-spectrum grid H*K2*K3 of complex T
+    per configuration (written by the z columns, read and rewritten by the y
+    transform, read by the x transform + gather, which also writes and
+    re-reads the per-site sums and writes the per-molecule energies; the
+    solvent's site data is read once per launch)."""
     K1, K2, K3 = sy.system.ms
     H = K1 // 2 + 1
     T = prec_bytes
     nsites = 3 * slvmax
     p = sy.system.splodr
+    hk = 2 * T * H * K2 * K3
     return {
-        # Hk written once (complex T)
-        "pme_zcol": n_ins * (2 * T * H * K2 * K3),
-        # Hk read once, phi written once
-        "pme_plane": n_ins * (2 * T * H * K2 * K3 + T * K1 * K2 * K3),
-        # phi read once, energies written once; solvent splines once per launch
-        "gather": n_ins * (T * K1 * K2 * K3 + 8 * slvmax) + nsites * (3 * p * T + 4 + T),
-        # energies read, histogram written
+        "pme_z": n_ins * hk,
+        "pme_y": n_ins * 2 * hk,
+        "pme_x_gather": n_ins * (hk + 16 * nsites + 8 * slvmax) + nsites * (3 * p * T + 8 + T),
         "bin": n_ins * (8 * slvmax + 8 * ermax),
     }
 
@@ -105,6 +106,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default batch and PME path)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs (frame sharding with "
@@ -127,14 +130,19 @@ def main():
     dframes = [torch.from_numpy(f.reshape(-1)).to(dev) for f in frames]
     torch.cuda.synchronize()
 
-    def run(prec, steps, warmup):
+    def run(prec, steps, warmup, graph=False):
         g = ErmGPU(s, prec, max_batch=args.batch)
+        tuning = None
+        g.set_frame_device(dframes[0].data_ptr(), sy.cell)
+        if not args.no_tune and args.batch == 0:
+            tuning, _ = g.autotune(n_trial=args.insertions, seed=args.seed + 1)
+        g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)
         for i in range(warmup):
             g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
             g.insert_random(args.insertions, args.seed, first_index=i * args.insertions)
         g.engclear()
         torch.cuda.synchronize()
-        st = {k: 0.0 for k in ("prep", "pme_zcol", "pme_plane", "gather", "pair", "bin", "corr", "total")}
+        st = {k: 0.0 for k in ("prep", "pme_z", "pme_y", "pme_x_gather", "pair", "bin", "corr", "total")}
         t0 = time.perf_counter()
         for i in range(warmup, warmup + steps):
             g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
@@ -144,12 +152,13 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         acc = g.fetch()
-        bmax = args.batch if args.batch > 0 else 256
+        tun = g.tuning()
         slv, erm = g.slvmax, g.ermax
         g.close()
-        return dt, st, acc, bmax, slv, erm
+        return dt, st, acc, tun, slv, erm
 
-    dt, st, acc, bmax, slvmax, ermax = run(args.precision, args.steps, args.warmup)
+    dt, st, acc, tun, slvmax, ermax = run(args.precision, args.steps, args.warmup)
+    bmax = tun["max_batch"]
     n_ins = args.steps * args.insertions
     value = n_ins * slvmax / dt
     launches = args.steps * int(np.ceil(args.insertions / bmax))
@@ -159,9 +168,13 @@ def main():
     stages = {k: v for k, v in st.items() if k not in ("total",)}
     dom = max(stages, key=stages.get)
     avg_ms = stages[dom] / launches
+    kname = {"pme_z": "k_pme_zcol_reg", "pme_y": "k_pme_ycol_reg", "pme_x_gather": "k_pme_xslab + k_mol_sum",
+             "bin": "k_bin", "prep": "k_pose + k_solute_prep", "corr": "k_corr"}.get(dom, dom)
+    if tun["pme_path"] != "register" and dom.startswith("pme"):
+        kname = dom + " (" + tun["pme_path"] + " path)"
     if dom in sb:
         ach = sb[dom] / (avg_ms * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
+        roof = {"kernel": kname, "stage": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": None, "avg_launch_ms": avg_ms,
                 "bytes_per_launch": sb[dom], "configs_per_launch": bmax}
     else:
@@ -190,14 +203,21 @@ def main():
                                   f"{args.insertions} random insertions per frame, cutoff 12 A, "
                                   "alpha 0.35, PME 48^3 p4, 600 bins, correlation on FP64 MFMA",
                       "frames": args.steps, "insertions_per_frame": args.insertions,
-                      "batch": bmax, "precision": args.precision},
+                      "batch": bmax, "pme_path": tun["pme_path"], "precision": args.precision,
+                      "tuning": "autotuned" if not args.no_tune and args.batch == 0 else "fixed"},
            "insertions_per_s": n_ins / dt,
            "roofline": roof, "stage_ms": {k: round(v, 3) for k, v in st.items()},
            "stage_share": stage_share, "engsmpl": acc["engsmpl"]}
+    if not args.no_graph:
+        dtg, _, accg, tung, _, _ = run(args.precision, args.steps, args.warmup, graph=True)
+        out["graph_replay"] = {"value": n_ins * slvmax / dtg, "insertions_per_s": n_ins / dtg,
+                               "ms_per_step": dtg / args.steps * 1e3, "use_graph": tung["use_graph"],
+                               "edens_equal": bool(np.array_equal(accg["edens"], acc["edens"]))}
     if not args.no_fp32 and args.precision == "fp64":
-        dt32, st32, _, _, _, _ = run("fp32", max(1, args.steps // 2), 1)
+        dt32, st32, _, tun32, _, _ = run("fp32", max(1, args.steps // 2), 1)
         n32 = max(1, args.steps // 2) * args.insertions
         out["fp32"] = {"value": n32 * slvmax / dt32, "insertions_per_s": n32 / dt32,
+                       "batch": tun32["max_batch"], "pme_path": tun32["pme_path"],
                        "stage_ms": {k: round(v, 3) for k, v in st32.items()}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sy, frames[0], 100000, args.cpu_seconds, args.seed)

@@ -22,7 +22,7 @@ LIBPATH = os.path.join(HERE, "libermgpu.so")
 ERM_FP64, ERM_FP32 = 0, 1
 STATUS = {0: "ok", 1: "eng_min", 2: "eng_sft", 3: "mbin", 4: "eng_ecd", 5: "rcp_cns", 6: "noncuboid",
           7: "eng_bug", 8: "eng_cns", 9: "eng_siz", 10: "arg", 11: "unsupported", 12: "device"}
-STAGES = ("prep", "pme_zcol", "pme_plane", "gather", "pair", "bin", "corr", "total")
+STAGES = ("prep", "pme_z", "pme_y", "pme_x_gather", "pair", "bin", "corr", "total")
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -55,7 +55,7 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_set_topology", "erm_set_bins", "erm_engclear", "erm_set_frame", "erm_set_frame_device",
            "erm_get_uv_energy", "erm_realcal_proc", "erm_realcal_self", "erm_recpcal", "erm_engconst",
            "erm_insert_random", "erm_update_histogram", "erm_engstore", "erm_fetch", "erm_sizes",
-           "erm_stage_times", "erm_solute_gradient")
+           "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune")
 
 _lib = None
 
@@ -93,6 +93,9 @@ def lib():
         L.erm_fetch.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.erm_sizes.argtypes = [vp, _i32p, _i32p]
         L.erm_stage_times.argtypes = [vp, _f64p]
+        L.erm_set_tuning.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+        L.erm_get_tuning.argtypes = [vp, _i32p]
+        L.erm_autotune.argtypes = [vp, C.c_int32, C.c_uint64, _f64p]
         L.erm_solute_gradient.argtypes = [vp, C.c_int32, _i32p, _f64p, C.c_uint64, C.c_uint64, _f64p,
                                           C.c_int32, _f64p, _f64p, _f64p, _f64p]
         _lib = L
@@ -328,6 +331,23 @@ class ErmGPU:
                                               C.c_uint64(first_index), _p(w), int(cmbrule), _p(qq),
                                               _p(ee), _p(ss), _p(acc)), "solute_gradient")
         return acc[0], acc[1], acc[2:2 + ns], acc[2 + ns:2 + 2 * ns], acc[2 + 2 * ns:]
+
+    def set_tuning(self, max_batch=0, pme_path="auto", use_graph=False):
+        pp = {"auto": 0, "fused": 1, "unfused": 2, "register": 3}[pme_path]
+        self._check(lib().erm_set_tuning(self.h, int(max_batch), pp, int(bool(use_graph))), "set_tuning")
+
+    def tuning(self):
+        out = np.zeros(4, dtype=np.int32)
+        self._check(lib().erm_get_tuning(self.h, _p(out, _i32p)), "get_tuning")
+        names = {0: "auto", 1: "fused", 2: "unfused", 3: "register"}
+        return dict(max_batch=int(out[0]), pme_path=names[int(out[1])], use_graph=bool(out[2]))
+
+    def autotune(self, n_trial=512, seed=12345):
+        """Times batch sizes, PME paths and graph replay on the current frame;
+        keeps the fastest (accumulators untouched).  Returns (tuning, ms)."""
+        ms = np.zeros(1)
+        self._check(lib().erm_autotune(self.h, int(n_trial), C.c_uint64(seed), _p(ms)), "autotune")
+        return self.tuning(), float(ms[0])
 
     def stage_times(self):
         out = np.zeros(8)

@@ -221,6 +221,16 @@ struct erm_ctx {
   // solute gradient: per-frame solvent potential grid and batch buffers
   DBuf d_ggrid, d_gHk, d_gtab, d_gq_t, d_gqd, d_gepsd, d_gpart, d_gphi, d_gub, d_gw, d_gout;
   uint64_t frame_serial = 0, ggrid_serial = ~0ULL;
+  // batching: device-side call arguments, per-batch stage events, graph replay
+  DBuf d_args;
+  uint64_t *h_args = nullptr;      // pinned {seed, first_index}
+  std::vector<hipEvent_t> bev;     // 8 events per batch of the last call
+  hipEvent_t *cur_ev = nullptr;    // events of the batch being enqueued
+  int wsol_ones = 0;               // d_wsol holds 1.0 for this many configurations
+  bool capturing = false, use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::string gkey;
+  uint64_t alloc_gen = 0;          // bumped by every device (re)allocation
   int ggrid_excl = -2;
   int xslab_Q = 0;            // planes per x-c2r group of the register path
   size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
@@ -232,7 +242,7 @@ struct erm_ctx {
             &d_sltlj, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
             &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
             &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
-            &d_w3h, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
+            &d_w3h, &d_args, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
             &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
             &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm};
@@ -267,6 +277,7 @@ int ensure(erm_ctx *c, DBuf &b, size_t bytes) {
     return fail(c, ERM_E_DEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
   }
   b.bytes = bytes;
+  ++c->alloc_gen;
   return ERM_OK;
 }
 
@@ -318,16 +329,25 @@ const char *status_name(int code) {
   }
 }
 
-int check_status(erm_ctx *c, int first) {
+// first error raised by the device during the call (configuration index
+// within the call); synchronises the stream
+int check_status(erm_ctx *c) {
   int h[8];
   HIPCHK(c, hipMemcpyAsync(h, c->d_status.p, sizeof h, hipMemcpyDeviceToHost, c->st));
   TRY(sync(c));
   if (h[0]) {
     double v;
     memcpy(&v, &h[4], sizeof v);
-    return fail(c, h[0], "%s (configuration %d, sorted molecule %d, energy %.6g)", status_name(h[0]),
-                first + h[1], h[2], v);
+    return fail(c, h[0], "%s (configuration %d, sorted molecule %d, energy %.6g)", status_name(h[0]), h[1],
+                h[2], v);
   }
+  return ERM_OK;
+}
+
+// stage event i of the batch being enqueued (none while a graph is captured)
+int mark(erm_ctx *c, int i) {
+  if (c->capturing || !c->cur_ev) return ERM_OK;
+  HIPCHK(c, hipEventRecord(c->cur_ev[i], c->st));
   return ERM_OK;
 }
 
@@ -591,9 +611,10 @@ struct Batch {
 int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
   const int n = bt.n, ns = c->ns;
   if (bt.random) {
-    hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, bt.seed, bt.first_index,
-                       dp<double>(c->d_bf), dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
-                       dp<double>(c->d_cellm), dp<double>(c->d_spos));
+    hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
+                       (uint64_t)bt.first, dp<double>(c->d_bf),
+                       dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template], dp<double>(c->d_cellm),
+                       dp<double>(c->d_spos));
     TRY(launch_check(c, "k_pose"));
   } else if (bt.host_pos) {
     HIPCHK(c, hipMemcpyAsync(c->d_spos.p, bt.host_pos, sizeof(double) * (size_t)n * ns * 3, hipMemcpyHostToDevice,
@@ -604,13 +625,20 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
                                dp<double>(c->d_pos) + (size_t)3 * c->mol_begin[bt.solutes[b]],
                                sizeof(double) * ns * 3, hipMemcpyDeviceToDevice, c->st));
   }
-  std::vector<int> sltj(n, -1);
-  if (bt.solutes)
+  if (bt.solutes) {
+    std::vector<int> sltj(n, -1);
     for (int b = 0; b < n; ++b) {
       int kk = c->kof[bt.solutes[b]];
       sltj[b] = (kk >= 0) ? c->jof_k[kk] : -1;
     }
-  TRY(upload(c, c->d_sltj, sltj.data(), n));
+    TRY(upload(c, c->d_sltj, sltj.data(), n));
+  } else {
+    // no solute among the physical molecules (test particle)
+    TRY(ensure(c, c->d_sltj, sizeof(int) * n));
+    hipLaunchKernelGGL(k_fill<int>, dim3((n + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, dp<int>(c->d_sltj),
+                       (size_t)n, -1);
+    TRY(launch_check(c, "k_fill"));
+  }
   return ERM_OK;
 }
 
@@ -642,29 +670,29 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   const int nzg = regp ? (H * K2 + NTHR - 1) / NTHR : (H * K2 + ncolwg - 1) / ncolwg;
   TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
   if (flags & DO_RECIP) TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
-  HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+  TRY(mark(c, 0));
   TRY(stage_configs(c, bt, k));
   hipLaunchKernelGGL(k_solute_prep<T>, dim3(n), dim3(128), 0, c->st, k, dp<double>(c->d_spos),
                      dp<double>(c->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<V>(c->d_tw3),
                      dp<double>(c->d_sref), dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv),
                      dp<V>(c->d_w1h), dp<V>(c->d_w2h), regp ? dp<V>(c->d_w3h) : (V *)nullptr);
   TRY(launch_check(c, "k_solute_prep"));
-  HIPCHK(c, hipEventRecord(c->ev[1], c->st));
+  TRY(mark(c, 1));
   if (regp) {
     int rz = ERM_E_UNSUPPORTED;
     ERM_KSWITCH(K3, (rz = launch_zcol_reg<T, KK>(c, k, n, nzg)));
     TRY(rz);
     TRY(launch_check(c, "k_pme_zcol_reg"));
-    HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+    TRY(mark(c, 2));
     ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
     TRY(launch_check(c, "k_pme_ycol_reg"));
+    TRY(mark(c, 3));
     TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
     const size_t lx = 2 * sizeof(T) * (size_t)c->xslab_Q * H * K2;
     int rc = ERM_E_UNSUPPORTED;
     ERM_KSWITCH(K1, (rc = launch_xslab<T, KK>(c, k, n, lx)));
     TRY(rc);
     TRY(launch_check(c, "k_pme_xslab"));
-    HIPCHK(c, hipEventRecord(c->ev[3], c->st));
     Params kg = k;
     if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
     hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
@@ -677,11 +705,12 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                        dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h),
                        dp<V>(c->d_tw3), dp<V>(c->d_Hk), dp<double>(c->d_selfp));
     TRY(launch_check(c, "k_pme_zcol"));
-    HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+    TRY(mark(c, 2));
     size_t lp = 2 * (size_t)K2 * H * vb;
     Params kg = k;
     if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
     if (c->slab_S > 0 && c->P.pme_path != 2) {
+      TRY(mark(c, 3));
       const int nslab = (K3 + c->slab_S - 1) / c->slab_S;
       TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
       if (c->slab_R == 16)
@@ -695,7 +724,6 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                            dp<int>(c->d_zg), dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite),
                            dp<double>(c->d_esite));
       TRY(launch_check(c, "k_pme_slab"));
-      HIPCHK(c, hipEventRecord(c->ev[3], c->st));
       hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
                          dp<double>(c->d_esite), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
                          dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
@@ -705,7 +733,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
       hipLaunchKernelGGL(k_pme_plane<T>, dim3(K3, n), dim3(NTHR), lp, c->st, k, dp<V>(c->d_Hk), dp<V>(c->d_tw1),
                          dp<V>(c->d_tw2), dp<T>(c->d_phi));
       TRY(launch_check(c, "k_pme_plane"));
-      HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+      TRY(mark(c, 3));
       hipLaunchKernelGGL(k_gather<T>, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, dp<T>(c->d_phi),
                          dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), dp<int>(c->d_sg),
                          dp<T>(c->d_sw), dp<T>(c->d_sq), dp<int>(c->d_sltj), dp<double>(c->d_uv));
@@ -713,10 +741,10 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     }
   } else {
     HIPCHK(c, hipMemsetAsync(c->d_uv.p, 0, sizeof(double) * (size_t)n * (S + 1), c->st));
-    HIPCHK(c, hipEventRecord(c->ev[2], c->st));
-    HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+    TRY(mark(c, 2));
+    TRY(mark(c, 3));
   }
-  HIPCHK(c, hipEventRecord(c->ev[4], c->st));
+  TRY(mark(c, 4));
   if (flags & DO_REAL) {
     double Rg = k.rcmax + 8.0;
     double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
@@ -729,7 +757,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                        dp<double>(c->d_rm), dp<double>(c->d_uv));
     TRY(launch_check(c, "k_pair"));
   }
-  HIPCHK(c, hipEventRecord(c->ev[5], c->st));
+  TRY(mark(c, 5));
   if (flags & DO_SELF) {
     // realcal_self: reused after the first configuration for a rigid test
     // particle (engproc.F90:712-730), recomputed per configuration otherwise
@@ -763,16 +791,15 @@ int run_histogram(erm_ctx *c, int n, int first) {
   TRY(ensure(c, c->d_hist, sizeof(double) * (size_t)n * E));
   TRY(ensure(c, c->d_w, sizeof(double) * n));
   TRY(ensure(c, c->d_flceng, sizeof(double) * (size_t)n * MAXSPEC));
-  HIPCHK(c, hipMemsetAsync(c->d_status.p, 0, 32, c->st));
   size_t lb = (size_t)E * (sizeof(double) + sizeof(int));
   hipLaunchKernelGGL(k_bin, dim3(n), dim3(NTHR), lb, c->st, k, dp<double>(c->d_uv), dp<double>(c->d_edges),
                      dp<int>(c->d_specoff), dp<int>(c->d_uvmax), dp<int>(c->d_uvsoft), dp<int>(c->d_mol_spec),
                      dp<int>(c->d_sltj), dp<double>(c->d_hist), dp<double>(c->d_flceng),
-                     dp<unsigned long long>(c->d_mnmx), dp<int>(c->d_status));
+                     dp<unsigned long long>(c->d_mnmx), dp<int>(c->d_status), first);
   TRY(launch_check(c, "k_bin"));
   hipLaunchKernelGGL(k_scalars, dim3(1), dim3(NTHR), 0, c->st, k, n, dp<double>(c->d_uv), dp<double>(c->d_wsol),
                      dp<double>(c->d_escrd), dp<double>(c->d_scal), dp<double>(c->d_eself),
-                     dp<unsigned long long>(c->d_mnmx), dp<double>(c->d_w), dp<int>(c->d_status));
+                     dp<unsigned long long>(c->d_mnmx), dp<double>(c->d_w), dp<int>(c->d_status), first);
   TRY(launch_check(c, "k_scalars"));
   hipLaunchKernelGGL(k_edens, dim3((E + 255) / 256), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
                      dp<double>(c->d_w), dp<double>(c->d_edens));
@@ -782,7 +809,7 @@ int run_histogram(erm_ctx *c, int n, int first) {
                        dp<double>(c->d_w), dp<double>(c->d_slnuv));
     TRY(launch_check(c, "k_slnuv"));
   }
-  HIPCHK(c, hipEventRecord(c->ev[6], c->st));
+  TRY(mark(c, 6));
   if (c->P.corrcal) {
     int nt = (E + 15) / 16;
     int waves = nt * nt;
@@ -790,8 +817,8 @@ int run_histogram(erm_ctx *c, int n, int first) {
                        dp<double>(c->d_w), dp<double>(c->d_ecorr));
     TRY(launch_check(c, "k_corr"));
   }
-  HIPCHK(c, hipEventRecord(c->ev[7], c->st));
-  return check_status(c, first);
+  TRY(mark(c, 7));
+  return ERM_OK;
 }
 
 int copy_out_uv(erm_ctx *c, int n, double *out) {
@@ -804,18 +831,30 @@ int copy_out_uv(erm_ctx *c, int n, double *out) {
   return sync(c);
 }
 
-int stage_times(erm_ctx *c, bool hist) {
-  TRY(sync(c));
+// stage times of the last call from the per-batch events (after a sync)
+int stage_times(erm_ctx *c, int nb, bool hist) {
+  for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
   float ms;
-  for (int i = 0; i < 7; ++i) {
-    int a = i, b = i + 1;
-    if (!hist && b > 5) {
-      c->stage_ms[i] = 0;
-      continue;
+  for (int bi = 0; bi < nb; ++bi) {
+    hipEvent_t *ev = &c->bev[8 * bi];
+    for (int i = 0; i < 7; ++i) {
+      int a = i, b = i + 1;
+      if (!hist && b > 5) continue;
+      if (hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess) c->stage_ms[i] += ms;
     }
-    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) == hipSuccess) c->stage_ms[i] += ms;
+    if (hipEventElapsedTime(&ms, ev[0], ev[hist ? 7 : 5]) == hipSuccess) c->stage_ms[7] += ms;
   }
-  if (hipEventElapsedTime(&ms, c->ev[0], c->ev[hist ? 7 : 5]) == hipSuccess) c->stage_ms[7] += ms;
+  (void)hipGetLastError();
+  return ERM_OK;
+}
+
+// 8 stage events for each of nb batches
+int batch_events(erm_ctx *c, int nb) {
+  while ((int)c->bev.size() < 8 * nb) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c->bev.push_back(e);
+  }
   return ERM_OK;
 }
 
@@ -827,29 +866,128 @@ int need(erm_ctx *c, bool frame, bool bins) {
   return ERM_OK;
 }
 
-// energies (and optionally histograms) over n configurations in batches
-int process(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
+// work of one call that precedes its batches: clear the error status, copy
+// the generator arguments, unit weights when none are given
+int begin_call(erm_ctx *c, const Batch &all) {
+  TRY(ensure(c, c->d_status, 64));
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(32), 0, c->st, dp<int>(c->d_status), (size_t)8, 0);
+  TRY(launch_check(c, "k_fill"));
+  if (all.random) {
+    TRY(ensure(c, c->d_args, 2 * sizeof(uint64_t)));
+    if (!c->h_args) HIPCHK(c, hipHostMalloc((void **)&c->h_args, 2 * sizeof(uint64_t), hipHostMallocDefault));
+    c->h_args[0] = all.seed;
+    c->h_args[1] = all.first_index;
+    HIPCHK(c, hipMemcpyAsync(c->d_args.p, c->h_args, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, c->st));
+  }
+  if (!all.weights) {
+    const int need_n = std::min(all.n, c->Bmax);
+    if (c->wsol_ones < need_n) {
+      TRY(ensure(c, c->d_wsol, sizeof(double) * c->Bmax));
+      hipLaunchKernelGGL(k_fill<double>, dim3((c->Bmax + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st,
+                         dp<double>(c->d_wsol), (size_t)c->Bmax, 1.0);
+      TRY(launch_check(c, "k_fill"));
+      c->wsol_ones = c->Bmax;
+    }
+  }
+  return ERM_OK;
+}
+
+// enqueues every batch of a call on the context's stream; with uv_out the
+// energies are copied out per batch (synchronising)
+int enqueue_batches(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
   const int ns = c->ns;
-  for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
-  for (int s0 = 0; s0 < all.n; s0 += c->Bmax) {
+  int bi = 0;
+  for (int s0 = 0; s0 < all.n; s0 += c->Bmax, ++bi) {
     Batch bt = all;
     bt.n = std::min(c->Bmax, all.n - s0);
     bt.first = all.first + s0;
     if (all.host_pos) bt.host_pos = all.host_pos + (size_t)s0 * ns * 3;
     if (all.solutes) bt.solutes = all.solutes + s0;
     if (all.weights) bt.weights = all.weights + s0;
-    bt.first_index = all.first_index + (uint64_t)s0;
-    std::vector<double> ws(bt.n, 1.0);
-    if (bt.weights)
-      for (int b = 0; b < bt.n; ++b) ws[b] = bt.weights[b];
-    TRY(ensure(c, c->d_wsol, sizeof(double) * bt.n));
-    TRY(upload(c, c->d_wsol, ws.data(), bt.n));
+    c->cur_ev = (8 * (bi + 1) <= (int)c->bev.size()) ? &c->bev[8 * bi] : nullptr;
+    if (bt.weights) {
+      TRY(upload(c, c->d_wsol, bt.weights, bt.n));
+      c->wsol_ones = 0;
+    }
     int st = c->fp32 ? run_energies<float>(c, bt, flags) : run_energies<double>(c, bt, flags);
     if (st) return st;
     if (flags & DO_HIST) TRY(run_histogram(c, bt.n, bt.first));
-    TRY(stage_times(c, (flags & DO_HIST) != 0));
     if (uv_out) TRY(copy_out_uv(c, bt.n, uv_out + (size_t)s0 * (c->slvmax + 1)));
   }
+  c->cur_ev = nullptr;
+  return ERM_OK;
+}
+
+// energies (and optionally histograms) over n configurations in batches;
+// one synchronisation at the end (the error status and the stage times)
+int process(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
+  for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
+  const int nb = (all.n + c->Bmax - 1) / c->Bmax;
+  TRY(batch_events(c, nb));
+  TRY(begin_call(c, all));
+  int st = enqueue_batches(c, all, flags, uv_out);
+  c->cur_ev = nullptr;
+  if (st) return st;
+  TRY(check_status(c));
+  return stage_times(c, nb, (flags & DO_HIST) != 0);
+}
+
+// identity of a captured call: everything its kernels take by value or by
+// device address
+std::string graph_key(erm_ctx *c, const Batch &all, int flags) {
+  std::string k(reinterpret_cast<const char *>(&c->kp), sizeof(Params));
+  char buf[256];
+  snprintf(buf, sizeof buf, "|%d|%d|%d|%d|%d|%d|%d|%d|%d|%llu|%d", all.n, c->Bmax, flags, c->P.pme_path, c->slab_S,
+           c->slab_R, c->xslab_Q, (int)c->have_usreal, c->wsol_ones, (unsigned long long)c->alloc_gen,
+           (int)all.random);
+  return k + buf;
+}
+
+// random insertions replayed from a captured graph: the first call with a
+// new key runs eagerly and then captures its batches; later calls copy the
+// generator arguments and launch the graph
+int process_graph(erm_ctx *c, const Batch &all, int flags) {
+  std::string key = graph_key(c, all, flags);
+  if (c->gexec && key == c->gkey) {
+    for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
+    TRY(begin_call(c, all));
+    HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+    HIPCHK(c, hipEventRecord(c->ev[7], c->st));
+    TRY(check_status(c));
+    float ms;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[7]) == hipSuccess) c->stage_ms[7] = ms;
+    return ERM_OK;
+  }
+  TRY(process(c, all, flags, nullptr));
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  key = graph_key(c, all, flags);
+  const uint64_t gen = c->alloc_gen;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipGetLastError();
+    c->use_graph = false;
+    return ERM_OK;
+  }
+  c->capturing = true;
+  int st = enqueue_batches(c, all, flags, nullptr);
+  c->capturing = false;
+  hipError_t e = hipStreamEndCapture(c->st, &g);
+  bool ok = (st == ERM_OK && e == hipSuccess && g && c->alloc_gen == gen);
+  if (ok) ok = (hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0) == hipSuccess);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  if (!ok) {
+    // the eager results of this call stand; later calls run eagerly
+    c->gexec = nullptr;
+    c->use_graph = false;
+    c->err = "graph capture failed; eager execution";
+    return ERM_OK;
+  }
+  c->gkey = key;
   return ERM_OK;
 }
 
@@ -1208,6 +1346,9 @@ void erm_destroy(erm_ctx *c) {
   for (DBuf *b : c->all()) b->release();
   for (int i = 0; i < 8; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  for (hipEvent_t e : c->bev) (void)hipEventDestroy(e);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->h_args) (void)hipHostFree(c->h_args);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1495,7 +1636,9 @@ int32_t erm_insert_random(erm_ctx *c, int32_t n, uint64_t seed, uint64_t first_i
   if (c->bfcoord.empty()) return fail(c, ERM_E_ARG, "random insertion needs bfcoord");
   if (n < 0) return fail(c, ERM_E_ARG, "negative count");
   Batch bt{n, 0, nullptr, nullptr, nullptr, true, seed, first_index};
-  return process(c, bt, DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF | DO_HIST, uvengy);
+  const int flags = DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF | DO_HIST;
+  if (c->use_graph && !uvengy && n > 0) return process_graph(c, bt, flags);
+  return process(c, bt, flags, uvengy);
 }
 
 int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, const double *weights,
@@ -1505,6 +1648,10 @@ int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, cons
   if (!uvengy || n < 0) return fail(c, ERM_E_ARG, "uvengy is NULL");
   if (c->P.slttype == 1 && !solutes) return fail(c, ERM_E_ARG, "soln mode needs solute indices");
   const int S = c->slvmax;
+  TRY(ensure(c, c->d_status, 64));
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(32), 0, c->st, dp<int>(c->d_status), (size_t)8, 0);
+  TRY(launch_check(c, "k_fill"));
+  c->cur_ev = nullptr;
   for (int s0 = 0; s0 < n; s0 += c->Bmax) {
     int nb = std::min(c->Bmax, n - s0);
     // energies into the sorted order
@@ -1530,10 +1677,10 @@ int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, cons
       for (int b = 0; b < nb; ++b) ws[b] = weights[s0 + b];
     TRY(ensure(c, c->d_wsol, sizeof(double) * nb));
     TRY(upload(c, c->d_wsol, ws.data(), nb));
-    HIPCHK(c, hipEventRecord(c->ev[5], c->st));
+    c->wsol_ones = 0;
     TRY(run_histogram(c, nb, s0));
   }
-  return ERM_OK;
+  return check_status(c);
 }
 
 int32_t erm_engstore(erm_ctx *c) {
@@ -1616,6 +1763,93 @@ int32_t erm_sizes(const erm_ctx *c, int32_t *slvmax, int32_t *ermax) {
 int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
   if (!c || !ms8) return ERM_E_ARG;
   for (int i = 0; i < 8; ++i) ms8[i] = c->stage_ms[i];
+  return ERM_OK;
+}
+
+int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t use_graph) {
+  if (!c) return ERM_E_ARG;
+  if (max_batch < 0 || max_batch > 4096 || pme_path < 0 || pme_path > 3 || use_graph < 0 || use_graph > 1)
+    return fail(c, ERM_E_ARG, "tuning out of range");
+  if (max_batch > 0) c->Bmax = max_batch;
+  if ((pme_path == 1 || pme_path == 3) && c->have_frame && !(pme_path == 1 ? c->slab_S : c->xslab_Q))
+    return fail(c, ERM_E_UNSUPPORTED, "PME path %d does not fit this grid / system", pme_path);
+  c->P.pme_path = pme_path;
+  c->use_graph = use_graph != 0;
+  return ERM_OK;
+}
+
+int32_t erm_get_tuning(const erm_ctx *c, int32_t *out4) {
+  if (!c || !out4) return ERM_E_ARG;
+  out4[0] = c->Bmax;
+  out4[1] = c->P.pme_path;
+  out4[2] = c->use_graph ? 1 : 0;
+  out4[3] = c->fp32 ? 1 : 0;
+  return ERM_OK;
+}
+
+int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms) {
+  TRY(need(c, true, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (c->P.slttype != 2 || c->bfcoord.empty())
+    return fail(c, ERM_E_UNSUPPORTED, "autotuning times random insertions of a rigid test particle");
+  if (n_trial <= 0) return fail(c, ERM_E_ARG, "n_trial must be positive");
+  const bool had_usreal = c->have_usreal;
+  const int flags = DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF;  // accumulators untouched
+  std::vector<int> paths;
+  if (c->xslab_Q) paths.push_back(3);
+  if (c->slab_S) paths.push_back(1);
+  paths.push_back(2);
+  double best = 1e300;
+  int bestB = c->Bmax, bestP = c->P.pme_path;
+  auto timed = [&](double *ms) -> int {
+    Batch bt{n_trial, 0, nullptr, nullptr, nullptr, true, seed, 0};
+    TRY(process(c, bt, flags, nullptr));  // warm-up (allocations, first launches)
+    HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+    TRY(process(c, bt, flags, nullptr));
+    HIPCHK(c, hipEventRecord(c->ev[7], c->st));
+    TRY(sync(c));
+    float f = 0;
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev[0], c->ev[7]));
+    *ms = f;
+    return ERM_OK;
+  };
+  c->use_graph = false;
+  for (int B : {64, 128, 256, 512, 1024}) {
+    if (B > 4 * n_trial && B > 64) break;
+    for (int pp : paths) {
+      c->Bmax = B;
+      c->P.pme_path = pp;
+      double ms;
+      TRY(timed(&ms));
+      if (ms < best) {
+        best = ms;
+        bestB = B;
+        bestP = pp;
+      }
+    }
+  }
+  c->Bmax = bestB;
+  c->P.pme_path = bestP;
+  // graph replay against eager launches for the chosen shape
+  c->use_graph = true;
+  {
+    Batch bt{n_trial, 0, nullptr, nullptr, nullptr, true, seed, 0};
+    TRY(process_graph(c, bt, flags));  // eager run + capture
+    if (c->use_graph && c->gexec) {
+      HIPCHK(c, hipEventRecord(c->ev[1], c->st));
+      TRY(process_graph(c, bt, flags));
+      HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+      TRY(sync(c));
+      float f = 0;
+      HIPCHK(c, hipEventElapsedTime(&f, c->ev[1], c->ev[2]));
+      if (f >= best)
+        c->use_graph = false;
+      else
+        best = f;
+    }
+  }
+  c->have_usreal = had_usreal;  // the trial's realcal_self must not be reused
+  if (best_ms) *best_ms = best;
   return ERM_OK;
 }
 

@@ -197,13 +197,18 @@ __device__ void qprod_dev(const double *a, const double *b, double *p) {
 // random test-particle pose (insertion.F90:133-163, 325-366) placed with the
 // reference's operation sequence (set_solute_origin, set_shift_com,
 // apply_orientation; bestfit.F90:49-100, 174-203)
-__global__ void k_pose(Params P, int n, uint64_t seed, uint64_t first, const double *__restrict__ bf,
-                       const double *__restrict__ mass, const double *__restrict__ cellm, double *spos) {
+// args = {seed, first index of the call}; insertion b of this launch is
+// number args[1] + offset + b of the generator stream (device-side arguments,
+// so a captured graph replays with new seeds)
+__global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
+                       const double *__restrict__ bf, const double *__restrict__ mass,
+                       const double *__restrict__ cellm, double *spos) {
 #pragma clang fp contract(off)
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int ns = P.ns;
-  uint64_t index = first + (uint64_t)b;
+  const uint64_t seed = args[0];
+  uint64_t index = args[1] + offset + (uint64_t)b;
   double r[3], q[4] = {1.0, 0.0, 0.0, 0.0};
   for (int i = 0; i < 3; ++i) r[i] = rng_uniform(seed, index, (uint32_t)i);
   for (int a = 0; a < 63; ++a) {
@@ -1282,7 +1287,7 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
                                               const int *__restrict__ spec_off, const int *__restrict__ uvmax,
                                               const int *__restrict__ uvsoft, const int *__restrict__ mol_spec,
                                               const int *__restrict__ slt_j, double *hist, double *flceng,
-                                              unsigned long long *mnmx, int *status) {
+                                              unsigned long long *mnmx, int *status, int first) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double *e = reinterpret_cast<double *>(lds_raw);
   int *h = reinterpret_cast<int *>(e + P.ermax);
@@ -1305,7 +1310,7 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
     if (j == sj) continue;
     int pti = mol_spec[j];
     if (pti <= 0 || pti > P.numslv) {
-      set_error(status, 8, b, j, 0.0);
+      set_error(status, 8, first + b, j, 0.0);
       continue;
     }
     double v = u[j];
@@ -1319,14 +1324,14 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
         hi = mid;
     }
     if (lo <= 0) {
-      set_error(status, 1, b, j, v);
+      set_error(status, 1, first + b, j, v);
       continue;
     }
     if (P.slttype == 1 && lo > uvsoft[pti - 1]) {
-      set_error(status, 2, b, j, v);
+      set_error(status, 2, first + b, j, v);
       continue;
     }
-    if (lo == n && v < 1.0e3 && !P.force) set_error(status, 3, b, j, v);
+    if (lo == n && v < 1.0e3 && !P.force) set_error(status, 3, first + b, j, v);
     atomicAdd(&h[off + lo - 1], 1);
     atomicAdd(&flc[pti - 1], v);
     atomicMin(&mn[pti - 1], dbl_to_ord(v));
@@ -1349,7 +1354,7 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
 __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double *__restrict__ uv,
                                                   const double *__restrict__ wsol,
                                                   const double *__restrict__ escrd, double *scal, double *eself,
-                                                  unsigned long long *mnmx, double *wout, int *status) {
+                                                  unsigned long long *mnmx, double *wout, int *status, int first) {
   __shared__ double wsh[NTHR], twsh[NTHR];
   __shared__ int ish[NTHR];
   __shared__ double voff;
@@ -1392,9 +1397,9 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
             hi = mid;
         }
         if (lo <= 0) {
-          set_error(status, 1, b, -1, u0);
+          set_error(status, 1, first + b, -1, u0);
         } else {
-          if (lo == P.esmax && u0 < 1.0e3 && !P.force) set_error(status, 3, b, -1, u0);
+          if (lo == P.esmax && u0 < 1.0e3 && !P.force) set_error(status, 3, first + b, -1, u0);
           bin = lo - 1;
         }
       }

@@ -188,10 +188,28 @@ int32_t erm_fetch(erm_ctx *ctx, double *edens, double *ecorr, double *eself, dou
 int32_t erm_sizes(const erm_ctx *ctx, int32_t *slvmax, int32_t *ermax);
 
 /* Device time of the last engconst / insert_random call, split by stage
- * (milliseconds, HIP events on the context's stream): [0] prep/pose,
- * [1] PME columns, [2] PME planes, [3] gather, [4] pair, [5] binning,
- * [6] correlation, [7] total. */
+ * (milliseconds, HIP events on the context's stream, summed over batches):
+ * [0] poses and solute splines, [1] PME z columns, [2] PME y transform
+ * (planes, for the unfused path), [3] PME x transform and gather, [4] real
+ * space and self terms, [5] binning, [6] correlation, [7] total.  A call
+ * replayed from a graph reports the total only. */
 int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
+
+/* ---- batching and tuning ------------------------------------------------- */
+/* No counterpart in the reference (its engconst runs one insertion at a
+ * time).  max_batch: configurations per device batch (0 keeps the current);
+ * pme_path as in erm_params; use_graph = 1 replays erm_insert_random calls
+ * without uvengy from a captured HIP graph (the first call of a given shape
+ * runs eagerly and captures; a change of frame geometry, batch shape or
+ * buffers re-captures). */
+int32_t erm_set_tuning(erm_ctx *ctx, int32_t max_batch, int32_t pme_path, int32_t use_graph);
+/* out4 = {max_batch, pme_path, use_graph, precision} */
+int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out4);
+/* Times n_trial random insertions (seed) on the current frame for batch
+ * sizes 64..1024 and every applicable PME path, then graph replay against
+ * eager launches, and keeps the fastest; accumulators are untouched.
+ * best_ms (optional) receives the chosen configuration's time. */
+int32_t erm_autotune(erm_ctx *ctx, int32_t n_trial, uint64_t seed, double *best_ms);
 
 /* ---- differentiation ---------------------------------------------------- */
 /* Gradient of the solute-solvent energy u = sum_{k>=1} uvengy(k) (the sum
