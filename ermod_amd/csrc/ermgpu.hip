@@ -224,6 +224,7 @@ struct erm_ctx {
   // batching: device-side call arguments, per-batch stage events, graph replay
   DBuf d_args;
   uint64_t *h_args = nullptr;      // pinned {seed, first_index}
+  bool args_ready = false;         // d_args holds the current call's arguments
   std::vector<hipEvent_t> bev;     // 8 events per batch of the last call
   hipEvent_t *cur_ev = nullptr;    // events of the batch being enqueued
   int wsol_ones = 0;               // d_wsol holds 1.0 for this many configurations
@@ -611,6 +612,7 @@ struct Batch {
 int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
   const int n = bt.n, ns = c->ns;
   if (bt.random) {
+    if (!c->args_ready || !c->d_args.p) return fail(c, ERM_E_ENG_BUG, "random poses without generator arguments");
     hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
                        (uint64_t)bt.first, dp<double>(c->d_bf),
                        dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template], dp<double>(c->d_cellm),
@@ -866,19 +868,26 @@ int need(erm_ctx *c, bool frame, bool bins) {
   return ERM_OK;
 }
 
+// generator arguments {seed, first index of the call} for k_pose, through a
+// pinned staging buffer (the previous call has synchronised before it is
+// rewritten)
+int upload_args(erm_ctx *c, uint64_t seed, uint64_t first_index) {
+  TRY(ensure(c, c->d_args, 2 * sizeof(uint64_t)));
+  if (!c->h_args) HIPCHK(c, hipHostMalloc((void **)&c->h_args, 2 * sizeof(uint64_t), hipHostMallocDefault));
+  c->h_args[0] = seed;
+  c->h_args[1] = first_index;
+  HIPCHK(c, hipMemcpyAsync(c->d_args.p, c->h_args, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, c->st));
+  c->args_ready = true;
+  return ERM_OK;
+}
+
 // work of one call that precedes its batches: clear the error status, copy
 // the generator arguments, unit weights when none are given
 int begin_call(erm_ctx *c, const Batch &all) {
   TRY(ensure(c, c->d_status, 64));
   hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(32), 0, c->st, dp<int>(c->d_status), (size_t)8, 0);
   TRY(launch_check(c, "k_fill"));
-  if (all.random) {
-    TRY(ensure(c, c->d_args, 2 * sizeof(uint64_t)));
-    if (!c->h_args) HIPCHK(c, hipHostMalloc((void **)&c->h_args, 2 * sizeof(uint64_t), hipHostMallocDefault));
-    c->h_args[0] = all.seed;
-    c->h_args[1] = all.first_index;
-    HIPCHK(c, hipMemcpyAsync(c->d_args.p, c->h_args, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, c->st));
-  }
+  if (all.random) TRY(upload_args(c, all.seed, all.first_index));
   if (!all.weights) {
     const int need_n = std::min(all.n, c->Bmax);
     if (c->wsol_ones < need_n) {
@@ -1925,9 +1934,11 @@ int32_t erm_solute_gradient(erm_ctx *c, int32_t n, const int32_t *solutes, const
   TRY(upload_real(c, c->d_gq_t, g.q));
   TRY(upload(c, c->d_gqd, g.q.data(), ns));
   TRY(upload(c, c->d_gepsd, g.eps.data(), ns));
+  if (random) TRY(upload_args(c, seed, first_index));
   (void)k;
   for (int s0 = 0; s0 < n; s0 += c->Bmax) {
-    Batch bt{std::min(c->Bmax, n - s0), s0, nullptr, nullptr, nullptr, random, seed, first_index + (uint64_t)s0};
+    // k_pose numbers insertion b of this batch first_index + s0 + b
+    Batch bt{std::min(c->Bmax, n - s0), s0, nullptr, nullptr, nullptr, random, seed, first_index};
     if (c->P.slttype == 1) bt.solutes = solutes + s0;
     else if (!random) bt.host_pos = solute_pos + (size_t)s0 * ns * 3;
     if (weights) bt.weights = weights + s0;

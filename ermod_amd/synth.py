@@ -287,10 +287,12 @@ def ion_kind(name, d) -> MolKind:
 
 def mixed_hists():
     """150 bins per species (600 per solute over the four species): water and
-    cosolvent over [-22.5, 22.5] + 30 log bins to 1e4; the ions' linear range
-    reaches -302.5 (the reference stops with eng_min below the first edge)."""
-    neutral = Hist(ecdmin=-20.0, ecfmns=-5.0, ecdcen=0.0, eccore=15.0, eclbin=1.0, ecfbin=0.25,
-                   ec0bin=0.1, pecore=30)
+    cosolvent linear over [-102.5, 52.5] (0.1 near 0, 0.25, then 5) + 32 log
+    bins to 1e4; the ions' linear range reaches -302.5 (the reference stops
+    with eng_min below the first edge; a 150-site solute binds a cosolvent
+    molecule by more than 30 kcal/mol)."""
+    neutral = Hist(ecdmin=-100.0, ecfmns=-5.0, ecdcen=0.0, eccore=45.0, eclbin=5.0, ecfbin=0.25,
+                   ec0bin=0.1, pecore=32)
     ion = Hist(ecdmin=-300.0, ecfmns=-5.0, ecdcen=0.0, eccore=300.0, eclbin=20.0, ecfbin=0.25,
                ec0bin=0.1, pecore=30)
     return [neutral, neutral, ion, ion]
