@@ -136,7 +136,7 @@ def main():
         g.set_frame_device(dframes[0].data_ptr(), sy.cell)
         if not args.no_tune and args.batch == 0:
             tuning, _ = g.autotune(n_trial=args.insertions, seed=args.seed + 1)
-        g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)
+        g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)  # keeps batch and slab shape
         for i in range(warmup):
             g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
             g.insert_random(args.insertions, args.seed, first_index=i * args.insertions)
@@ -203,7 +203,8 @@ def main():
                                   f"{args.insertions} random insertions per frame, cutoff 12 A, "
                                   "alpha 0.35, PME 48^3 p4, 600 bins, correlation on FP64 MFMA",
                       "frames": args.steps, "insertions_per_frame": args.insertions,
-                      "batch": bmax, "pme_path": tun["pme_path"], "precision": args.precision,
+                      "batch": bmax, "pme_path": tun["pme_path"], "slab_planes": tun["slab_planes"],
+                      "group_planes": tun["group_planes"], "precision": args.precision,
                       "tuning": "autotuned" if not args.no_tune and args.batch == 0 else "fixed"},
            "insertions_per_s": n_ins / dt,
            "roofline": roof, "stage_ms": {k: round(v, 3) for k, v in st.items()},

@@ -234,6 +234,8 @@ struct erm_ctx {
   uint64_t alloc_gen = 0;          // bumped by every device (re)allocation
   int ggrid_excl = -2;
   int xslab_Q = 0;            // planes per x-c2r group of the register path
+  int tune_S = 16, tune_Q = 8;  // caps on slab depth and planes per group (autotuned)
+  std::vector<int> zst;         // first z-sorted site of every plane (host copy)
   size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
       d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
@@ -480,6 +482,44 @@ template <typename T, int K> int launch_xslab(erm_ctx *c, const Params &k, int n
     default: break;                           \
   }
 
+// Slab shape of the fused PME kernels from the frame's z-sorted site counts:
+// the deepest slab (at most tune_S planes) whose busiest slab fits the
+// per-thread site registers (R = 16 or 32), with at most 3x halo planes;
+// for the register path, tune_Q planes per group within the LDS budget and
+// one row pair per thread.
+template <typename T> int choose_slab(erm_ctx *c) {
+  const Params &k = c->kp;
+  const int K3 = k.K3;
+  const std::vector<int> &zst = c->zst;
+  c->slab_S = 0;
+  c->slab_R = 0;
+  for (int R : {16, 32}) {
+    if (c->P.pme_path == 2) break;
+    for (int Sl = std::min(c->tune_S, K3); Sl >= 1 && !c->slab_S; --Sl) {
+      int mx = 0;
+      for (int zb = 0; zb < K3; zb += Sl) mx = std::max(mx, zst[std::min(zb + Sl, K3)] - zst[zb]);
+      if (mx <= R * NTHR && (Sl + k.p - 1) <= 3 * Sl) {
+        c->slab_S = Sl;
+        c->slab_R = R;
+      }
+    }
+    if (c->slab_S) break;
+  }
+  if ((c->P.pme_path == 1 || c->P.pme_path == 3) && !c->slab_S)
+    return fail(c, ERM_E_UNSUPPORTED, "fused PME slab does not fit this grid / system");
+  c->xslab_Q = 0;
+  if (c->slab_S && reg_path_ok<T>(c)) {
+    // one complex plane per group slot (the real rows are written in place)
+    size_t plane_b = 2 * sizeof(T) * (size_t)k.H * k.K2;
+    int Q = (int)std::min<size_t>((size_t)c->tune_Q, c->lds_cap / plane_b);
+    Q = std::min(Q, NTHR / ((k.K2 + 1) / 2));
+    if (Q >= 1) c->xslab_Q = Q;
+  }
+  if (c->P.pme_path == 3 && !c->xslab_Q)
+    return fail(c, ERM_E_UNSUPPORTED, "register-column PME path does not support this grid");
+  return ERM_OK;
+}
+
 template <typename T> int frame_prep(erm_ctx *c) {
   Params &k = c->kp;
   const int ncell = k.nb[0] * k.nb[1] * k.nb[2];
@@ -552,44 +592,15 @@ template <typename T> int frame_prep(erm_ctx *c) {
                      dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zg), dp<T>(c->d_zq),
                      dp<T>(c->d_zw));
   TRY(launch_check(c, "k_zscatter"));
-  std::vector<int> zst(K3 + 1);
-  HIPCHK(c, hipMemcpyAsync(zst.data(), c->d_zstart.p, sizeof(int) * (K3 + 1), hipMemcpyDeviceToHost, c->st));
-  // slab: at most 16 planes, sites per slab within the register budget
-  c->slab_S = 0;
-  c->slab_R = 0;
+  c->zst.assign(K3 + 1, 0);
+  HIPCHK(c, hipMemcpyAsync(c->zst.data(), c->d_zstart.p, sizeof(int) * (K3 + 1), hipMemcpyDeviceToHost, c->st));
   // sorted index of every tagpt molecule (for soln-mode solutes)
   std::vector<int> molk(S);
   HIPCHK(c, hipMemcpyAsync(molk.data(), c->d_mol_k.p, sizeof(int) * S, hipMemcpyDeviceToHost, c->st));
   TRY(sync(c));
   c->jof_k.assign(S, -1);
   for (int j = 0; j < S; ++j) c->jof_k[molk[j]] = j;
-  for (int R : {16, 32}) {
-    if (c->P.pme_path == 2) break;
-    for (int Sl = std::min(16, K3); Sl >= 1 && !c->slab_S; --Sl) {
-      int mx = 0;
-      for (int zb = 0; zb < K3; zb += Sl) mx = std::max(mx, zst[std::min(zb + Sl, K3)] - zst[zb]);
-      // a slab recomputes p - 1 halo planes: keep that below 3x the owned planes
-      if (mx <= R * NTHR && (Sl + k.p - 1) <= 3 * Sl) {
-        c->slab_S = Sl;
-        c->slab_R = R;
-      }
-    }
-    if (c->slab_S) break;
-  }
-  if ((c->P.pme_path == 1 || c->P.pme_path == 3) && !c->slab_S)
-    return fail(c, ERM_E_UNSUPPORTED, "fused PME slab does not fit this grid / system");
-  // register path: planes per x group within the LDS the kernel may use
-  c->xslab_Q = 0;
-  if (c->slab_S && reg_path_ok<T>(c)) {
-    // one complex plane per ring slot (the real rows are written in place)
-    size_t plane_b = 2 * sizeof(T) * (size_t)k.H * k.K2;
-    int Q = (int)std::min<size_t>(8, c->lds_cap / plane_b);
-    Q = std::min(Q, NTHR / ((k.K2 + 1) / 2));
-    if (Q >= 1) c->xslab_Q = Q;
-  }
-  if (c->P.pme_path == 3 && !c->xslab_Q)
-    return fail(c, ERM_E_UNSUPPORTED, "register-column PME path does not support this grid");
-  return ERM_OK;
+  return choose_slab<T>(c);
 }
 
 // --------------------------------------------------------- batch pipeline
@@ -1775,24 +1786,31 @@ int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
   return ERM_OK;
 }
 
-int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t use_graph) {
+int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t use_graph, int32_t slab_planes,
+                       int32_t group_planes) {
   if (!c) return ERM_E_ARG;
-  if (max_batch < 0 || max_batch > 4096 || pme_path < 0 || pme_path > 3 || use_graph < 0 || use_graph > 1)
+  if (max_batch < 0 || max_batch > 4096 || pme_path < 0 || pme_path > 3 || use_graph < 0 || use_graph > 1 ||
+      slab_planes < 0 || slab_planes > 16 || group_planes < 0 || group_planes > 8)
     return fail(c, ERM_E_ARG, "tuning out of range");
   if (max_batch > 0) c->Bmax = max_batch;
-  if ((pme_path == 1 || pme_path == 3) && c->have_frame && !(pme_path == 1 ? c->slab_S : c->xslab_Q))
-    return fail(c, ERM_E_UNSUPPORTED, "PME path %d does not fit this grid / system", pme_path);
+  if (slab_planes > 0) c->tune_S = slab_planes;
+  if (group_planes > 0) c->tune_Q = group_planes;
   c->P.pme_path = pme_path;
   c->use_graph = use_graph != 0;
+  if (c->have_frame) TRY(c->fp32 ? choose_slab<float>(c) : choose_slab<double>(c));
   return ERM_OK;
 }
 
-int32_t erm_get_tuning(const erm_ctx *c, int32_t *out4) {
-  if (!c || !out4) return ERM_E_ARG;
-  out4[0] = c->Bmax;
-  out4[1] = c->P.pme_path;
-  out4[2] = c->use_graph ? 1 : 0;
-  out4[3] = c->fp32 ? 1 : 0;
+int32_t erm_get_tuning(const erm_ctx *c, int32_t *out8) {
+  if (!c || !out8) return ERM_E_ARG;
+  out8[0] = c->Bmax;
+  out8[1] = c->P.pme_path;
+  out8[2] = c->use_graph ? 1 : 0;
+  out8[3] = c->fp32 ? 1 : 0;
+  out8[4] = c->slab_S;
+  out8[5] = c->xslab_Q;
+  out8[6] = c->tune_S;
+  out8[7] = c->tune_Q;
   return ERM_OK;
 }
 
@@ -1809,7 +1827,7 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
   if (c->slab_S) paths.push_back(1);
   paths.push_back(2);
   double best = 1e300;
-  int bestB = c->Bmax, bestP = c->P.pme_path;
+  int bestB = c->Bmax, bestP = c->P.pme_path, bestS = c->tune_S, bestQ = c->tune_Q;
   auto timed = [&](double *ms) -> int {
     Batch bt{n_trial, 0, nullptr, nullptr, nullptr, true, seed, 0};
     TRY(process(c, bt, flags, nullptr));  // warm-up (allocations, first launches)
@@ -1823,22 +1841,38 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
     return ERM_OK;
   };
   c->use_graph = false;
-  for (int B : {64, 128, 256, 512, 1024}) {
-    if (B > 4 * n_trial && B > 64) break;
+  auto apply = [&](int B, int pp, int Sc, int Qc) -> int {
+    c->Bmax = B;
+    c->P.pme_path = pp;
+    c->tune_S = Sc;
+    c->tune_Q = Qc;
+    return c->fp32 ? choose_slab<float>(c) : choose_slab<double>(c);
+  };
+  for (int B : {128, 256, 512, 1024}) {
+    if (B > 4 * n_trial && B > 128) break;
     for (int pp : paths) {
-      c->Bmax = B;
-      c->P.pme_path = pp;
-      double ms;
-      TRY(timed(&ms));
-      if (ms < best) {
-        best = ms;
-        bestB = B;
-        bestP = pp;
+      for (int Sc : {16, 8}) {
+        for (int Qc : {8, 4, 2}) {
+          if (pp != 3 && Qc != 8) continue;
+          if (pp == 2 && Sc != 16) continue;
+          if (apply(B, pp, Sc, Qc) != ERM_OK) {
+            c->err.clear();
+            continue;
+          }
+          double ms;
+          TRY(timed(&ms));
+          if (ms < best) {
+            best = ms;
+            bestB = B;
+            bestP = pp;
+            bestS = Sc;
+            bestQ = Qc;
+          }
+        }
       }
     }
   }
-  c->Bmax = bestB;
-  c->P.pme_path = bestP;
+  TRY(apply(bestB, bestP, bestS, bestQ));
   // graph replay against eager launches for the chosen shape
   c->use_graph = true;
   {

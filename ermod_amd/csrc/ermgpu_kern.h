@@ -765,11 +765,27 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
   const size_t np = (size_t)nsites;
   for (int zz0 = zb - p + 1; zz0 < ze; zz0 += Q) {
     const int nq = min(Q, ze - zz0);
-    // coalesced copy of the nq complex planes
-    for (int t = threadIdx.x; t < nq * cplane; t += blockDim.x) {
-      const int q = t / cplane, e = t - q * cplane;
-      const int z = (((zz0 + q) % K3) + K3) % K3;
-      stage[t] = Hk[((size_t)b * K3 + z) * cplane + e];
+    // coalesced copy of the nq complex planes, XU loads in flight per thread
+    {
+      constexpr int XU = 8;
+      const int tot = nq * cplane;
+      for (int t0 = threadIdx.x; t0 < tot; t0 += XU * (int)blockDim.x) {
+        V tmp[XU];
+#pragma unroll
+        for (int u = 0; u < XU; ++u) {
+          const int t = t0 + u * (int)blockDim.x;
+          if (t < tot) {
+            const int q = t / cplane, e = t - q * cplane;
+            const int z = (((zz0 + q) % K3) + K3) % K3;
+            tmp[u] = Hk[((size_t)b * K3 + z) * cplane + e];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < XU; ++u) {
+          const int t = t0 + u * (int)blockDim.x;
+          if (t < tot) stage[t] = tmp[u];
+        }
+      }
     }
     __syncthreads();
     const int t = threadIdx.x;

@@ -201,14 +201,18 @@ int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
  * pme_path as in erm_params; use_graph = 1 replays erm_insert_random calls
  * without uvengy from a captured HIP graph (the first call of a given shape
  * runs eagerly and captures; a change of frame geometry, batch shape or
- * buffers re-captures). */
-int32_t erm_set_tuning(erm_ctx *ctx, int32_t max_batch, int32_t pme_path, int32_t use_graph);
-/* out4 = {max_batch, pme_path, use_graph, precision} */
-int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out4);
+ * buffers re-captures); slab_planes (1..16) caps the z planes of one fused
+ * PME slab and group_planes (1..8) the planes the register path transforms
+ * per group (0 keeps either). */
+int32_t erm_set_tuning(erm_ctx *ctx, int32_t max_batch, int32_t pme_path, int32_t use_graph,
+                       int32_t slab_planes, int32_t group_planes);
+/* out8 = {max_batch, pme_path, use_graph, precision, slab planes in use,
+ * group planes in use, slab cap, group cap} */
+int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out8);
 /* Times n_trial random insertions (seed) on the current frame for batch
- * sizes 64..1024 and every applicable PME path, then graph replay against
- * eager launches, and keeps the fastest; accumulators are untouched.
- * best_ms (optional) receives the chosen configuration's time. */
+ * sizes 128..1024, every applicable PME path and slab / group shapes, then
+ * graph replay against eager launches, and keeps the fastest; accumulators
+ * are untouched.  best_ms (optional) receives the chosen time. */
 int32_t erm_autotune(erm_ctx *ctx, int32_t n_trial, uint64_t seed, double *best_ms);
 
 /* ---- differentiation ---------------------------------------------------- */
