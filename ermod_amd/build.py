@@ -17,7 +17,7 @@ FC = "/opt/rocm/bin/amdflang"
 SOURCES = [os.path.join(HERE, "csrc", "ermgpu.hip")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("ermgpu_dev.h", "ermgpu_kern.h", "ermgpu_grad.h")] + [
     os.path.join(ROOT, "include", "ermgpu.h")]
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(ROOT, "include")]
 

@@ -467,6 +467,7 @@ template <typename T, int K, int R> int launch_xslab_r(erm_ctx *c, const Params 
   return ERM_OK;
 }
 template <typename T, int K> int launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds) {
+  if (c->slab_R == 8) return launch_xslab_r<T, K, 8>(c, k, n, lds);
   if (c->slab_R == 16) return launch_xslab_r<T, K, 16>(c, k, n, lds);
   return launch_xslab_r<T, K, 32>(c, k, n, lds);
 }
@@ -493,8 +494,11 @@ template <typename T> int choose_slab(erm_ctx *c) {
   const std::vector<int> &zst = c->zst;
   c->slab_S = 0;
   c->slab_R = 0;
-  for (int R : {16, 32}) {
+  // R = 8 only for the register path with shallow slabs (fewer live registers)
+  const bool reg8 = c->tune_S <= 8 && (c->P.pme_path == 0 || c->P.pme_path == 3);
+  for (int R : {8, 16, 32}) {
     if (c->P.pme_path == 2) break;
+    if (R == 8 && !reg8) continue;
     for (int Sl = std::min(c->tune_S, K3); Sl >= 1 && !c->slab_S; --Sl) {
       int mx = 0;
       for (int zb = 0; zb < K3; zb += Sl) mx = std::max(mx, zst[std::min(zb + Sl, K3)] - zst[zb]);

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace erm {
 
 constexpr int NTHR = 256;          // threads per workgroup of the batched kernels
@@ -162,36 +164,79 @@ namespace erm {
 
 // ---------------------------------------------------------------------------
 // Register-resident FFT of compile-time length N (factors 4, 2, 3, 5): one
-// thread transforms one column held in registers; all indices are
-// compile-time, so the arrays stay in VGPRs.  Stockham stages as in lds_fft;
-// twiddles come from the axis table tw (e^{-2 pi i j / N}), conjugated for
-// the backward transform.
+// thread transforms one column held in registers.  Mixed-radix decimation in
+// frequency, in place: a stage of radix R on a block of N = R M reads
+// x[n1 + M n2] (n2 < R), takes the R-point DFT, multiplies output k2 by
+// w_N^(n1 k2) and writes it back to x[n1 + M k2]; each of the R blocks of
+// length M is then transformed the same way.  Position p of the result holds
+// X[rev_N(p)] with rev_N(p) = R rev_M(p mod M) + p / M, undone by renaming
+// registers.  Every index and twiddle is a compile-time constant: twiddles
+// come from a constexpr sine/cosine (octant reduction of the rational
+// angle, Taylor series to order 25 on [0, pi/4]), so no table is read and
+// only N complex values are live.
 // ---------------------------------------------------------------------------
+struct CRoot {
+  double re, im;
+};
+
+constexpr double kTwoPiOver4 = 1.57079632679489661923132169163975144;  // pi / 2
+
+constexpr double taylor_sin(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int k = 1; k <= 12; ++k) {
+    term *= -x2 / (double)((2 * k) * (2 * k + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k <= 12; ++k) {
+    term *= -x2 / (double)((2 * k - 1) * (2 * k));
+    sum += term;
+  }
+  return sum;
+}
+
+// e^{-2 pi i j / N}
+constexpr CRoot unit_root(int j, int N) {
+  j %= N;
+  if (j < 0) j += N;
+  const int q = (4 * j) / N;          // quadrant
+  const int r = 4 * j - q * N;        // angle within the quadrant = (pi/2) r / N
+  double c, s;
+  if (2 * r <= N) {
+    const double t = kTwoPiOver4 * (double)r / (double)N;
+    c = taylor_cos(t);
+    s = taylor_sin(t);
+  } else {
+    const double t = kTwoPiOver4 * (double)(N - r) / (double)N;
+    c = taylor_sin(t);
+    s = taylor_cos(t);
+  }
+  double cq = c, sq = s;  // angle q pi/2 + theta
+  if (q == 1) {
+    cq = -s;
+    sq = c;
+  } else if (q == 2) {
+    cq = -c;
+    sq = -s;
+  } else if (q == 3) {
+    cq = s;
+    sq = -c;
+  }
+  return CRoot{cq, -sq};
+}
+
+constexpr int first_radix(int n) { return n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : n % 5 == 0 ? 5 : n; }
+
+constexpr int dif_rev(int n, int p) {
+  if (n == 1) return 0;
+  const int R = first_radix(n), M = n / R;
+  return R * dif_rev(M, p % M) + p / M;
+}
+
 template <int N> struct Fact {
-  static constexpr int nf() {
-    int n = N, c = 0;
-    for (int r : {4, 2, 3, 5})
-      while (n % r == 0) {
-        n /= r;
-        ++c;
-      }
-    return c;
-  }
-  static constexpr int f(int s) {
-    int n = N, c = 0;
-    for (int r : {4, 2, 3, 5})
-      while (n % r == 0) {
-        n /= r;
-        if (c == s) return r;
-        ++c;
-      }
-    return 1;
-  }
-  static constexpr int span(int s) {
-    int p = 1;
-    for (int i = 0; i < s; ++i) p *= f(i);
-    return p;
-  }
   static constexpr bool valid() {
     int n = N;
     for (int r : {4, 2, 3, 5})
@@ -200,72 +245,142 @@ template <int N> struct Fact {
   }
 };
 
-template <int R, int SIGN, typename V>
-__device__ __forceinline__ void reg_butterfly(V (&x)[R], V *o, int p, const V *__restrict__ tw, int N) {
-  if constexpr (R == 4) {
-    V s0 = cadd(x[0], x[2]), d0 = csub(x[0], x[2]);
-    V s1 = cadd(x[1], x[3]), d1 = csub(x[1], x[3]);
-    V jd1;
-    jd1.x = -SIGN * d1.y;
-    jd1.y = SIGN * d1.x;
-    o[0] = cadd(s0, s1);
-    o[p] = cadd(d0, jd1);
-    o[2 * p] = csub(s0, s1);
-    o[3 * p] = csub(d0, jd1);
-  } else if constexpr (R == 2) {
-    o[0] = cadd(x[0], x[1]);
-    o[p] = csub(x[0], x[1]);
+// w_N^m for the transform direction SIGN (-1 forward, +1 backward)
+template <typename V, int SIGN, int M_, int N> __host__ __device__ __forceinline__ V root_v() {
+  constexpr CRoot w = unit_root(M_, N);
+  V r;
+  r.x = (decltype(r.x))w.re;
+  r.y = (decltype(r.x))(SIGN < 0 ? w.im : -w.im);
+  return r;
+}
+
+template <typename V> __host__ __device__ __forceinline__ V cmul_h(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+// v times w_N^m (m = 0 leaves v unchanged)
+template <int SIGN, int M_, int N, typename V> __host__ __device__ __forceinline__ V rot(V v) {
+  if constexpr (M_ % N == 0) {
+    return v;
+  } else if constexpr (4 * (M_ % N) == N) {  // w^(N/4) = SIGN i
+    V r;
+    r.x = -SIGN * v.y;
+    r.y = SIGN * v.x;
+    return r;
+  } else if constexpr (2 * (M_ % N) == N) {  // w^(N/2) = -1
+    V r;
+    r.x = -v.x;
+    r.y = -v.y;
+    return r;
+  } else if constexpr (4 * (M_ % N) == 3 * N) {  // w^(3N/4) = -SIGN i
+    V r;
+    r.x = SIGN * v.y;
+    r.y = -SIGN * v.x;
+    return r;
   } else {
-    const int rs = N / R;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      V acc = x[0];
-#pragma unroll
-      for (int s = 1; s < R; ++s) {
-        V w = tw[((s * q) % R) * rs];
-        if (SIGN > 0) w.y = -w.y;
-        acc = cadd(acc, cmul(x[s], w));
-      }
-      o[q * p] = acc;
-    }
+    return cmul_h(v, root_v<V, SIGN, M_ % N, N>());
   }
 }
 
-template <int N, int S, int SIGN, typename V>
-__device__ __forceinline__ void reg_stages(V (&a)[N], V (&b)[N], const V *__restrict__ tw) {
-  if constexpr (S < Fact<N>::nf()) {
-    constexpr int R = Fact<N>::f(S);
-    constexpr int p = Fact<N>::span(S);
-    constexpr int m = N / R;
-    constexpr int ts = N / (p * R);
-#pragma unroll
-    for (int i = 0; i < m; ++i) {
-      const int k = i % p;
-      V x[R];
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        V v = a[i + q * m];
-        if (q && k) {
-          V w = tw[(q * k * ts) % N];
-          if (SIGN > 0) w.y = -w.y;
-          v = cmul(v, w);
-        }
-        x[q] = v;
-      }
-      reg_butterfly<R, SIGN>(x, b + (i - k) * R + k, p, tw, N);
-    }
-    reg_stages<N, S + 1, SIGN>(b, a, tw);
+template <typename V> __host__ __device__ __forceinline__ V cadd_h(V a, V b) {
+  V r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+
+// output q of an R-point DFT: sum_s v[s] w_R^(s q)
+template <int R, int SIGN, int Q, typename V, int... S>
+__host__ __device__ __forceinline__ V dft_out(const V (&v)[R], std::integer_sequence<int, S...>) {
+  V acc = v[0];
+  ((acc = (S == 0) ? acc : cadd_h(acc, rot<SIGN, (S * Q) % R, R>(v[S]))), ...);
+  return acc;
+}
+
+template <int R, int SIGN, typename V, int... Q>
+__host__ __device__ __forceinline__ void small_dft(const V (&v)[R], V (&o)[R], std::integer_sequence<int, Q...>) {
+  if constexpr (R == 4) {
+    const V s0 = cadd_h(v[0], v[2]), s1 = cadd_h(v[1], v[3]);
+    V d0, d1;
+    d0.x = v[0].x - v[2].x;
+    d0.y = v[0].y - v[2].y;
+    d1.x = v[1].x - v[3].x;
+    d1.y = v[1].y - v[3].y;
+    const V jd = rot<SIGN, 1, 4>(d1);
+    o[0] = cadd_h(s0, s1);
+    o[1] = cadd_h(d0, jd);
+    o[2].x = s0.x - s1.x;
+    o[2].y = s0.y - s1.y;
+    o[3].x = d0.x - jd.x;
+    o[3].y = d0.y - jd.y;
+  } else if constexpr (R == 2) {
+    o[0] = cadd_h(v[0], v[1]);
+    o[1].x = v[0].x - v[1].x;
+    o[1].y = v[0].y - v[1].y;
+  } else {
+    ((o[Q] = dft_out<R, SIGN, Q>(v, std::make_integer_sequence<int, R>{})), ...);
   }
 }
 
-// in-place (from the caller's view) transform of x[N]
-template <int N, int SIGN, typename V> __device__ __forceinline__ void reg_fft(V (&x)[N], const V *__restrict__ tw) {
-  V y[N];
-  reg_stages<N, 0, SIGN>(x, y, tw);
-  if constexpr (Fact<N>::nf() % 2 == 1) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = y[i];
+template <int TOT, int N, int BASE, int SIGN, typename V> __host__ __device__ __forceinline__ void dif(V (&x)[TOT]);
+
+// one column n1 of a DIF stage: R-point DFT of x[n1 + M k], twiddled by
+// w_N^(n1 k), back into the same slots
+template <int TOT, int N, int BASE, int SIGN, int N1, typename V, int... K>
+__host__ __device__ __forceinline__ void dif_col(V (&x)[TOT], std::integer_sequence<int, K...>) {
+  constexpr int R = sizeof...(K), M = N / R;
+  const V v[R] = {x[BASE + N1 + M * K]...};
+  V o[R];
+  small_dft<R, SIGN>(v, o, std::make_integer_sequence<int, R>{});
+  ((x[BASE + N1 + M * K] = rot<SIGN, N1 * K, N>(o[K])), ...);
+}
+
+template <int TOT, int N, int BASE, int SIGN, typename V, int... N1>
+__host__ __device__ __forceinline__ void dif_cols(V (&x)[TOT], std::integer_sequence<int, N1...>) {
+  (dif_col<TOT, N, BASE, SIGN, N1>(x, std::make_integer_sequence<int, first_radix(N)>{}), ...);
+}
+
+template <int TOT, int N, int BASE, int SIGN, typename V, int... K>
+__host__ __device__ __forceinline__ void dif_blocks(V (&x)[TOT], std::integer_sequence<int, K...>) {
+  constexpr int M = N / first_radix(N);
+  (dif<TOT, M, BASE + M * K, SIGN>(x), ...);
+}
+
+template <int TOT, int N, int BASE, int SIGN, typename V> __host__ __device__ __forceinline__ void dif(V (&x)[TOT]) {
+  if constexpr (N > 1) {
+    constexpr int R = first_radix(N), M = N / R;
+    static_assert(R * M == N && R > 1, "FFT length must factor into 2, 3, 4, 5");
+    dif_cols<TOT, N, BASE, SIGN>(x, std::make_integer_sequence<int, M>{});
+    dif_blocks<TOT, N, BASE, SIGN>(x, std::make_integer_sequence<int, R>{});
   }
+}
+
+template <int N, int P> struct DifRev {
+  static constexpr int value = dif_rev(N, P);
+};
+
+template <int N, typename V, int... P>
+__host__ __device__ __forceinline__ void dif_unscramble(V (&x)[N], std::integer_sequence<int, P...>) {
+  const V y[N] = {x[P]...};
+  ((x[DifRev<N, P>::value] = y[P]), ...);
+}
+
+// in-place transform of x[N] (natural order in and out), unnormalised;
+// SIGN = -1 forward (e^{-i}), +1 backward
+template <int N, int SIGN, typename V> __host__ __device__ __forceinline__ void reg_fft(V (&x)[N]) {
+  static_assert(Fact<N>::valid(), "FFT length must factor into 2, 3, 4, 5");
+  dif<N, N, 0, SIGN>(x);
+  dif_unscramble<N>(x, std::make_integer_sequence<int, N>{});
+}
+
+// the same with the (unused) twiddle table of the axis, for the kernels that
+// hold one
+template <int N, int SIGN, typename V>
+__host__ __device__ __forceinline__ void reg_fft(V (&x)[N], const V *__restrict__) {
+  reg_fft<N, SIGN>(x);
 }
 
 }  // namespace erm
