@@ -108,6 +108,10 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default batch and PME path)")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay pass")
+    ap.add_argument("--slab-planes", type=int, default=0, help="fixed slab depth (with --batch)")
+    ap.add_argument("--group-planes", type=int, default=0, help="fixed planes per x group (with --batch)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="after the run, one device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs (frame sharding with "
@@ -136,6 +140,8 @@ def main():
         g.set_frame_device(dframes[0].data_ptr(), sy.cell)
         if not args.no_tune and args.batch == 0:
             tuning, _ = g.autotune(n_trial=args.insertions, seed=args.seed + 1)
+        else:
+            g.set_tuning(max_batch=args.batch, slab_planes=args.slab_planes, group_planes=args.group_planes)
         g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)  # keeps batch and slab shape
         for i in range(warmup):
             g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
@@ -223,6 +229,14 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sy, frames[0], 100000, args.cpu_seconds, args.seed)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 2 * n * 8
     print(json.dumps(out), flush=True)
 
 

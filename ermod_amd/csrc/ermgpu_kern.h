@@ -765,27 +765,14 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
   const size_t np = (size_t)nsites;
   for (int zz0 = zb - p + 1; zz0 < ze; zz0 += Q) {
     const int nq = min(Q, ze - zz0);
-    // coalesced copy of the nq complex planes, XU loads in flight per thread
-    {
-      constexpr int XU = 8;
-      const int tot = nq * cplane;
-      for (int t0 = threadIdx.x; t0 < tot; t0 += XU * (int)blockDim.x) {
-        V tmp[XU];
-#pragma unroll
-        for (int u = 0; u < XU; ++u) {
-          const int t = t0 + u * (int)blockDim.x;
-          if (t < tot) {
-            const int q = t / cplane, e = t - q * cplane;
-            const int z = (((zz0 + q) % K3) + K3) % K3;
-            tmp[u] = Hk[((size_t)b * K3 + z) * cplane + e];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < XU; ++u) {
-          const int t = t0 + u * (int)blockDim.x;
-          if (t < tot) stage[t] = tmp[u];
-        }
-      }
+    // coalesced copy of the nq complex planes (plane by plane, unrolled so
+    // that several loads are in flight)
+    for (int q = 0; q < nq; ++q) {
+      const int z = (((zz0 + q) % K3) + K3) % K3;
+      const V *src = Hk + ((size_t)b * K3 + z) * cplane;
+      V *dst = stage + (size_t)q * cplane;
+#pragma unroll 4
+      for (int e = threadIdx.x; e < cplane; e += blockDim.x) dst[e] = src[e];
     }
     __syncthreads();
     const int t = threadIdx.x;
