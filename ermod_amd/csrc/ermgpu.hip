@@ -208,7 +208,8 @@ struct erm_ctx {
   // refs rigid: realcal_self of the first configuration is reused
   bool have_usreal = false;
   // device buffers
-  DBuf d_mol_begin, d_charge, d_tagpt, d_cvt, d_mol_charge, d_uvspec, d_sitemass, d_bf, d_qslt, d_qslt_t, d_sltlj;
+  DBuf d_mol_begin, d_charge, d_tagpt, d_cvt, d_mol_charge, d_uvspec, d_sitemass, d_bf, d_qslt, d_qslt_t, d_sltlj,
+      d_sltlj64;
   DBuf d_pos, d_cellid, d_count, d_cell_start, d_fill, d_rm, d_mol_k, d_mol_ns, d_mol_first, d_mol_q,
       d_mol_spec, d_mol_m, d_jof;
   DBuf d_sx, d_sy, d_sz, d_sq, d_sct, d_sg, d_sw;
@@ -242,7 +243,7 @@ struct erm_ctx {
   DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
   std::vector<DBuf *> all() {
     return {&d_mol_begin, &d_charge, &d_tagpt, &d_cvt, &d_mol_charge, &d_uvspec, &d_sitemass, &d_bf, &d_qslt, &d_qslt_t,
-            &d_sltlj, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
+            &d_sltlj, &d_sltlj64, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
             &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
             &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
             &d_w3h, &d_args, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
@@ -401,6 +402,7 @@ int solute_tables(erm_ctx *c) {
   TRY(upload(c, c->d_qslt, q.data(), q.size()));
   TRY(upload_real(c, c->d_qslt_t, q));
   TRY(upload_real(c, c->d_sltlj, lj));
+  TRY(upload(c, c->d_sltlj64, lj.data(), lj.size()));
   return ERM_OK;
 }
 
@@ -771,7 +773,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                        dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
                        dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
                        dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
-                       dp<double>(c->d_rm), dp<double>(c->d_uv));
+                       dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
     TRY(launch_check(c, "k_pair"));
   }
   TRY(mark(c, 5));
@@ -1855,8 +1857,8 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
   for (int B : {128, 256, 512, 1024}) {
     if (B > 4 * n_trial && B > 128) break;
     for (int pp : paths) {
-      for (int Sc : {16, 8}) {
-        for (int Qc : {8, 4, 2}) {
+      for (int Sc : {16, 8, 5}) {
+        for (int Qc : {8, 5, 3}) {
           if (pp != 3 && Qc != 8) continue;
           if (pp == 2 && Sc != 16) continue;
           if (apply(B, pp, Sc, Qc) != ERM_OK) {

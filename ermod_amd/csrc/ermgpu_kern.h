@@ -1138,7 +1138,8 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
                                                const double *__restrict__ sy, const double *__restrict__ sz,
                                                const T *__restrict__ sq, const int *__restrict__ sct,
                                                const int *__restrict__ slt_j, const double *__restrict__ rm_ord_p,
-                                               double *uv) {
+                                               const double *__restrict__ slt_lj64,
+                                               const double *__restrict__ slt_q64, double *uv) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int ns = P.ns, nvt = P.nvt;
   T *ux = reinterpret_cast<T *>(lds_raw);
@@ -1210,6 +1211,30 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
         if (r2 <= cut2) {
           T ep = 0;
           T eps = ulj[(s * nvt + cv) * 2], sig2 = ulj[(s * nvt + cv) * 2 + 1];
+          if constexpr (sizeof(T) == 4) {
+            // r < 2^(-1/6) sigma (a repulsive overlap, where the r^-12 term
+            // amplifies FP32 rounding): the pair is recomputed in FP64 from
+            // the FP64 displacement and parameters
+            if (eps > (T)0 && sig2 > (T)1.2599211 * r2) {
+              const double ox = (double)ux[s] + (double)ulo[3 * s];
+              const double oy = (double)uy[s] + (double)ulo[3 * s + 1];
+              const double oz = (double)uz[s] + (double)ulo[3 * s + 2];
+              double bx = fabs(dx - ox), by = fabs(dy - oy), bz = fabs(dz - oz);
+              bx = (bx > P.halfL[0]) ? P.L[0] - bx : bx;
+              by = (by > P.halfL[1]) ? P.L[1] - by : by;
+              bz = (bz > P.halfL[2]) ? P.L[2] - bz : bz;
+              const double r2d = bx * bx + by * by + bz * bz;
+              const double e64 = slt_lj64[(s * nvt + cv) * 2], s64 = slt_lj64[(s * nvt + cv) * 2 + 1];
+              double epd = 0.0;
+              if (r2d <= P.lw2)
+                epd = lj_low_t<double>(P, e64, s64, r2d);
+              else if (r2d <= P.up2)
+                epd = lj_switch_t<double>(P, e64, s64, r2d);
+              if (r2d <= P.el2) epd += coul_t<double>(slt_q64[s] * (double)qv, sqrt(r2d), P.screen);
+              e += epd;
+              continue;
+            }
+          }
           if (eps > (T)0) {
             if (r2 <= lw2)
               ep += lj_low_t<T>(P, eps, sig2, r2);
