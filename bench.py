@@ -54,6 +54,26 @@ def stage_bytes(sy, prec_bytes, n_ins, slvmax, ermax):
     }
 
 
+def measured_traffic(stage, precision, configs):
+    """HBM bytes per launch of the x-slab stage from the committed counter
+    summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, each
+    calibrated on a device copy of known size), scaled to `configs`
+    configurations per launch; None when no summary covers the case."""
+    path = os.path.join(ROOT, "profiles", "refs20_fp64_hbm_traffic_pmc.md")
+    if stage != "pme_x_gather" or precision != "fp64" or not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if "=" in line and " " not in line.strip():
+            k, v = line.strip().split("=", 1)
+            kv[k] = float(v)
+    keys = ("xslab_read_bytes_per_launch", "xslab_write_bytes_per_launch",
+            "mol_sum_read_bytes_per_launch", "mol_sum_write_bytes_per_launch")
+    if not all(k in kv for k in keys) or not kv.get("configs_per_launch"):
+        return None
+    return sum(kv[k] for k in keys) * configs / kv["configs_per_launch"]
+
+
 def pair_count(sy, pos, P, rc=12.0):
     """Exact solute-site x solvent-site pairs within the cutoff for the
     configurations P (numpy, minimum image)."""
@@ -181,7 +201,8 @@ def main():
     if dom in sb:
         ach = sb[dom] / (avg_ms * 1e-3) / 1e9
         roof = {"kernel": kname, "stage": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None, "avg_launch_ms": avg_ms,
+                "frac": ach / HBM_PEAK, "traffic": measured_traffic(dom, args.precision, bmax),
+                "avg_launch_ms": avg_ms,
                 "bytes_per_launch": sb[dom], "configs_per_launch": bmax}
     else:
         roof = {"kernel": dom, "avg_launch_ms": avg_ms}
