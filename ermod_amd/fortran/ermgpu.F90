@@ -181,6 +181,36 @@ module ermgpu
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: ms8(8)
      end function erm_stage_times
+
+     integer(c_int32_t) function erm_set_tuning(ctx, max_batch, pme_path, use_graph, slab_planes, &
+          group_planes) bind(C, name="erm_set_tuning")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: max_batch, pme_path, use_graph, slab_planes, group_planes
+     end function erm_set_tuning
+
+     integer(c_int32_t) function erm_get_tuning(ctx, out8) bind(C, name="erm_get_tuning")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: out8(8)
+     end function erm_get_tuning
+
+     integer(c_int32_t) function erm_autotune(ctx, n_trial, seed, best_ms) bind(C, name="erm_autotune")
+       import :: c_ptr, c_int32_t, c_int64_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n_trial
+       integer(c_int64_t), value :: seed
+       real(c_double), intent(out) :: best_ms
+     end function erm_autotune
+
+     integer(c_int32_t) function erm_solute_gradient(ctx, n, solutes, solute_pos, seed, first_index, &
+          weights, cmbrule, q, eps, sigma, acc) bind(C, name="erm_solute_gradient")
+       import :: c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n, cmbrule
+       integer(c_int64_t), value :: seed, first_index
+       type(c_ptr), value :: solutes, solute_pos, weights, q, eps, sigma, acc
+     end function erm_solute_gradient
   end interface
 
 contains
