@@ -518,7 +518,7 @@ template <typename T> int choose_slab(erm_ctx *c) {
     // one complex plane per group slot (the real rows are written in place)
     size_t plane_b = 2 * sizeof(T) * (size_t)k.H * k.K2;
     int Q = (int)std::min<size_t>((size_t)c->tune_Q, c->lds_cap / plane_b);
-    Q = std::min(Q, NTHR / ((k.K2 + 1) / 2));
+    Q = std::min(Q, NTHR / k.K2);  // one row per thread
     if (Q >= 1) c->xslab_Q = Q;
   }
   if (c->P.pme_path == 3 && !c->xslab_Q)
@@ -1858,7 +1858,7 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
     if (B > 4 * n_trial && B > 128) break;
     for (int pp : paths) {
       for (int Sc : {16, 8, 5}) {
-        for (int Qc : {8, 5, 3}) {
+        for (int Qc : {5, 3, 2}) {
           if (pp != 3 && Qc != 8) continue;
           if (pp == 2 && Sc != 16) continue;
           if (apply(B, pp, Sc, Qc) != ERM_OK) {

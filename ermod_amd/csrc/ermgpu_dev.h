@@ -376,6 +376,38 @@ template <int N, int SIGN, typename V> __host__ __device__ __forceinline__ void 
   dif_unscramble<N>(x, std::make_integer_sequence<int, N>{});
 }
 
+// Inverse real transform of one Hermitian row through a half-length complex
+// FFT: for k < M = N/2 with X[0] and X[M] taken as real,
+//   E[k] = X[k] + conj X[M-k],  O[k] = (X[k] - conj X[M-k]) e^{+2 pi i k/N},
+//   z = IFFT_M(E + i O)  gives  x[2m] = Re z[m], x[2m+1] = Im z[m]
+// (unnormalised, e^{+i}), i.e. x[n] = sum_{k<N} X[k] e^{2 pi i k n / N}.
+template <typename V> __host__ __device__ __forceinline__ V c2r_one(V a, V b, V w) {
+  V e, d, z;
+  e.x = a.x + b.x;
+  e.y = a.y - b.y;
+  d.x = a.x - b.x;
+  d.y = a.y + b.y;
+  const V o = cmul_h(d, w);
+  z.x = e.x - o.y;
+  z.y = e.y + o.x;
+  return z;
+}
+
+template <int N, typename V, int... K>
+__host__ __device__ __forceinline__ void c2r_pack(const V *__restrict__ X, V (&Z)[N / 2],
+                                                  std::integer_sequence<int, K...>) {
+  constexpr int M = N / 2;
+  V x0 = X[0], xm = X[M];
+  x0.y = 0;
+  xm.y = 0;
+  ((Z[K] = c2r_one(K == 0 ? x0 : X[K], K == 0 ? xm : X[M - K], root_v<V, +1, K, N>())), ...);
+}
+
+template <int N, typename V> __host__ __device__ __forceinline__ void c2r_half(const V *__restrict__ X, V (&Z)[N / 2]) {
+  c2r_pack<N>(X, Z, std::make_integer_sequence<int, N / 2>{});
+  reg_fft<N / 2, +1>(Z);
+}
+
 // the same with the (unused) twiddle table of the axis, for the kernels that
 // hold one
 template <int N, int SIGN, typename V>

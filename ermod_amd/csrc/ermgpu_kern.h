@@ -729,13 +729,14 @@ __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename 
 
 // Slab of z planes [zb, zb + S) of configuration b.  The planes a site
 // touches, [g3 - p + 1, g3], are processed Q at a time: the Q complex planes
-// Hk[b][z][y][m1] are copied into LDS with coalesced loads; thread (q, pr)
-// holds rows y0 = pr and y1 = pr + ceil(K2/2) of plane q in registers, builds
-// the Hermitian-packed complex row y0 + i y1, transforms it (register FFT)
-// and writes the two real rows in place over the complex plane, with a row
-// stride of K1 + 1 (consecutive lanes on distinct LDS banks).  Each thread
-// then adds the planes' contributions to its MAXR owned sites (FP64 sums).
-// Needs Q * ceil(K2/2) <= blockDim.x (one row pair per thread and group).
+// Hk[b][z][y][m1] are copied into LDS with coalesced loads; thread (q, y)
+// holds row y of plane q in registers and inverts it through a half-length
+// complex FFT (c2r_half), then writes the real row in place over the complex
+// plane as (x[2m], x[2m+1]) pairs with a row stride of K1 + 2 values
+// (consecutive lanes on distinct LDS banks; the real plane fills exactly
+// the complex plane's bytes).  Each thread then adds the planes'
+// contributions to its MAXR owned sites (FP64 sums).
+// Needs Q * K2 <= blockDim.x (one row per thread and group).
 template <typename T, int K1, int MAXR>
 __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int nsites,
                                                     const typename C2<T>::type *__restrict__ Hk,
@@ -745,10 +746,9 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
                                                     const int *__restrict__ zsite, double *e_site) {
   using V = typename C2<T>::type;
   constexpr int H = K1 / 2 + 1;
-  constexpr int K1P = K1 + 1;
+  constexpr int K1P = K1 + 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int K2 = P.K2, K3 = P.K3, p = P.p;
-  const int npair = (K2 + 1) / 2;
   const int cplane = K2 * H;                  // complex elements per plane
   V *stage = reinterpret_cast<V *>(lds_raw);  // [Q][K2][H] complex, then [Q][K2][K1P] real in place
   const int b = blockIdx.y;
@@ -776,42 +776,16 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
     }
     __syncthreads();
     const int t = threadIdx.x;
-    const bool fft = t < nq * npair;
-    const int q = fft ? t / npair : 0, pr = fft ? t - q * npair : 0;
-    const int y0 = pr, y1 = pr + npair;
-    const bool has1 = fft && (y1 < K2);
-    V Z[K1];
-    if (fft) {
-      const V *r0 = stage + (size_t)q * cplane + (size_t)y0 * H;
-      const V *r1 = stage + (size_t)q * cplane + (size_t)(has1 ? y1 : y0) * H;
-#pragma unroll
-      for (int x = 0; x < H; ++x) {
-        V a = r0[x];
-        V cc = has1 ? r1[x] : V{0, 0};
-        if (x == 0 || x == K1 / 2) {
-          a.y = 0;
-          cc.y = 0;
-        }
-        Z[x].x = a.x - cc.y;
-        Z[x].y = a.y + cc.x;
-        if (x > 0 && x < K1 - x) {
-          Z[K1 - x].x = a.x + cc.y;
-          Z[K1 - x].y = -a.y + cc.x;
-        }
-      }
-      reg_fft<K1, +1>(Z, tw1);
-    }
+    const bool fft = t < nq * K2;
+    const int q = fft ? t / K2 : 0, y = fft ? t - q * K2 : 0;
+    V Z[K1 / 2];
+    if (fft) c2r_half<K1>(stage + (size_t)q * cplane + (size_t)y * H, Z);
     __syncthreads();  // every complex row is in registers before the real rows overwrite them
     T *real = reinterpret_cast<T *>(stage);
     if (fft) {
-      T *d0 = real + (size_t)q * cplane * 2 + (size_t)y0 * K1P;
+      V *d = reinterpret_cast<V *>(real + (size_t)q * cplane * 2 + (size_t)y * K1P);
 #pragma unroll
-      for (int x = 0; x < K1; ++x) d0[x] = Z[x].x;
-      if (has1) {
-        T *d1 = real + (size_t)q * cplane * 2 + (size_t)y1 * K1P;
-#pragma unroll
-        for (int x = 0; x < K1; ++x) d1[x] = Z[x].y;
-      }
+      for (int m = 0; m < K1 / 2; ++m) d[m] = Z[m];
     }
     __syncthreads();
     // gather: owned sites whose support g3 - c3 meets planes [zz0, zz0 + nq)
