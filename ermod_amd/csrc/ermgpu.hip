@@ -518,7 +518,6 @@ template <typename T> int choose_slab(erm_ctx *c) {
     // one complex plane per group slot (the real rows are written in place)
     size_t plane_b = 2 * sizeof(T) * (size_t)k.H * k.K2;
     int Q = (int)std::min<size_t>((size_t)c->tune_Q, c->lds_cap / plane_b);
-    Q = std::min(Q, NTHR / k.K2);  // one row per thread
     if (Q >= 1) c->xslab_Q = Q;
   }
   if (c->P.pme_path == 3 && !c->xslab_Q)
@@ -1796,7 +1795,7 @@ int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t 
                        int32_t group_planes) {
   if (!c) return ERM_E_ARG;
   if (max_batch < 0 || max_batch > 4096 || pme_path < 0 || pme_path > 3 || use_graph < 0 || use_graph > 1 ||
-      slab_planes < 0 || slab_planes > 16 || group_planes < 0 || group_planes > 8)
+      slab_planes < 0 || slab_planes > 16 || group_planes < 0 || group_planes > 16)
     return fail(c, ERM_E_ARG, "tuning out of range");
   if (max_batch > 0) c->Bmax = max_batch;
   if (slab_planes > 0) c->tune_S = slab_planes;
@@ -1857,8 +1856,8 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
   for (int B : {128, 256, 512, 1024}) {
     if (B > 4 * n_trial && B > 128) break;
     for (int pp : paths) {
-      for (int Sc : {16, 8, 5}) {
-        for (int Qc : {5, 3, 2}) {
+      for (int Sc : {16, 8, 5, 4}) {
+        for (int Qc : {16, 8, 5}) {
           if (pp != 3 && Qc != 8) continue;
           if (pp == 2 && Sc != 16) continue;
           if (apply(B, pp, Sc, Qc) != ERM_OK) {

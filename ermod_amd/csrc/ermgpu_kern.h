@@ -731,12 +731,11 @@ __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename 
 // touches, [g3 - p + 1, g3], are processed Q at a time: the Q complex planes
 // Hk[b][z][y][m1] are copied into LDS with coalesced loads; thread (q, y)
 // holds row y of plane q in registers and inverts it through a half-length
-// complex FFT (c2r_half), then writes the real row in place over the complex
-// plane as (x[2m], x[2m+1]) pairs with a row stride of K1 + 2 values
-// (consecutive lanes on distinct LDS banks; the real plane fills exactly
-// the complex plane's bytes).  Each thread then adds the planes'
+// complex FFT (c2r_half), then writes the real row in place over its own
+// complex row as (x[2m], x[2m+1]) pairs with a row stride of K1 + 2 values
+// (consecutive lanes on distinct LDS banks; each real row fills exactly its
+// complex row's bytes).  Each thread then adds the planes'
 // contributions to its MAXR owned sites (FP64 sums).
-// Needs Q * K2 <= blockDim.x (one row per thread and group).
 template <typename T, int K1, int MAXR>
 __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int nsites,
                                                     const typename C2<T>::type *__restrict__ Hk,
@@ -775,14 +774,14 @@ __global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int 
       for (int e = threadIdx.x; e < cplane; e += blockDim.x) dst[e] = src[e];
     }
     __syncthreads();
-    const int t = threadIdx.x;
-    const bool fft = t < nq * K2;
-    const int q = fft ? t / K2 : 0, y = fft ? t - q * K2 : 0;
-    V Z[K1 / 2];
-    if (fft) c2r_half<K1>(stage + (size_t)q * cplane + (size_t)y * H, Z);
-    __syncthreads();  // every complex row is in registers before the real rows overwrite them
+    // the real row (q, y) occupies exactly the bytes of the complex row
+    // (q, y) (K1P = 2H), which only this thread reads: no barrier between
+    // the transform and its write, any number of rows per thread
     T *real = reinterpret_cast<T *>(stage);
-    if (fft) {
+    for (int t = threadIdx.x; t < nq * K2; t += blockDim.x) {
+      const int q = t / K2, y = t - q * K2;
+      V Z[K1 / 2];
+      c2r_half<K1>(stage + (size_t)q * cplane + (size_t)y * H, Z);
       V *d = reinterpret_cast<V *>(real + (size_t)q * cplane * 2 + (size_t)y * K1P);
 #pragma unroll
       for (int m = 0; m < K1 / 2; ++m) d[m] = Z[m];

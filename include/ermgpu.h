@@ -202,7 +202,7 @@ int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
  * without uvengy from a captured HIP graph (the first call of a given shape
  * runs eagerly and captures; a change of frame geometry, batch shape or
  * buffers re-captures); slab_planes (1..16) caps the z planes of one fused
- * PME slab and group_planes (1..8) the planes the register path transforms
+ * PME slab and group_planes (1..16) the planes the register path transforms
  * per group (0 keeps either). */
 int32_t erm_set_tuning(erm_ctx *ctx, int32_t max_batch, int32_t pme_path, int32_t use_graph,
                        int32_t slab_planes, int32_t group_planes);
