@@ -236,6 +236,7 @@ struct erm_ctx {
   int ggrid_excl = -2;
   int xslab_Q = 0;            // planes per x-c2r group of the register path
   int tune_S = 16, tune_Q = 8;  // caps on slab depth and planes per group (autotuned)
+  int xslab_T = NTHR;           // threads per x-slab workgroup, 256 or 512 (autotuned)
   std::vector<int> zst;         // first z-sorted site of every plane (host copy)
   size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
@@ -459,16 +460,24 @@ template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, i
   hipLaunchKernelGGL((k_pme_ycol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_tw2),
                      dp<V>(c->d_Hk));
 }
-template <typename T, int K, int R> int launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
+template <typename T, int K, int R, int XT = NTHR>
+int launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
   using V = typename C2<T>::type;
-  TRY(set_lds(c, (const void *)k_pme_xslab<T, K, R>, lds, "k_pme_xslab"));
+  TRY(set_lds(c, (const void *)k_pme_xslab<T, K, R, XT>, lds, "k_pme_xslab"));
   const int nslab = (k.K3 + c->slab_S - 1) / c->slab_S;
-  hipLaunchKernelGGL((k_pme_xslab<T, K, R>), dim3(nslab, n), dim3(NTHR), lds, c->st, k, c->slab_S, c->xslab_Q,
+  hipLaunchKernelGGL((k_pme_xslab<T, K, R, XT>), dim3(nslab, n), dim3(XT), lds, c->st, k, c->slab_S, c->xslab_Q,
                      c->nsites, dp<V>(c->d_Hk), dp<V>(c->d_tw1), dp<int>(c->d_zstart), dp<int>(c->d_zg),
                      dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite), dp<double>(c->d_esite));
   return ERM_OK;
 }
+// slab_R counts sites per thread of a 256-thread workgroup; a 512-thread
+// workgroup owns the same sites with half as many per thread
 template <typename T, int K> int launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds) {
+  if (c->xslab_T == 512) {
+    if (c->slab_R == 8) return launch_xslab_r<T, K, 4, 512>(c, k, n, lds);
+    if (c->slab_R == 16) return launch_xslab_r<T, K, 8, 512>(c, k, n, lds);
+    return launch_xslab_r<T, K, 16, 512>(c, k, n, lds);
+  }
   if (c->slab_R == 8) return launch_xslab_r<T, K, 8>(c, k, n, lds);
   if (c->slab_R == 16) return launch_xslab_r<T, K, 16>(c, k, n, lds);
   return launch_xslab_r<T, K, 32>(c, k, n, lds);
@@ -962,9 +971,9 @@ int process(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
 std::string graph_key(erm_ctx *c, const Batch &all, int flags) {
   std::string k(reinterpret_cast<const char *>(&c->kp), sizeof(Params));
   char buf[256];
-  snprintf(buf, sizeof buf, "|%d|%d|%d|%d|%d|%d|%d|%d|%d|%llu|%d", all.n, c->Bmax, flags, c->P.pme_path, c->slab_S,
-           c->slab_R, c->xslab_Q, (int)c->have_usreal, c->wsol_ones, (unsigned long long)c->alloc_gen,
-           (int)all.random);
+  snprintf(buf, sizeof buf, "|%d|%d|%d|%d|%d|%d|%d|%d|%d|%d|%llu|%d", all.n, c->Bmax, flags, c->P.pme_path,
+           c->slab_S, c->slab_R, c->xslab_Q, c->xslab_T, (int)c->have_usreal, c->wsol_ones,
+           (unsigned long long)c->alloc_gen, (int)all.random);
   return k + buf;
 }
 
@@ -1792,14 +1801,16 @@ int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
 }
 
 int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t use_graph, int32_t slab_planes,
-                       int32_t group_planes) {
+                       int32_t group_planes, int32_t slab_threads) {
   if (!c) return ERM_E_ARG;
   if (max_batch < 0 || max_batch > 4096 || pme_path < 0 || pme_path > 3 || use_graph < 0 || use_graph > 1 ||
-      slab_planes < 0 || slab_planes > 16 || group_planes < 0 || group_planes > 16)
+      slab_planes < 0 || slab_planes > 16 || group_planes < 0 || group_planes > 16 ||
+      (slab_threads != 0 && slab_threads != 256 && slab_threads != 512))
     return fail(c, ERM_E_ARG, "tuning out of range");
   if (max_batch > 0) c->Bmax = max_batch;
   if (slab_planes > 0) c->tune_S = slab_planes;
   if (group_planes > 0) c->tune_Q = group_planes;
+  if (slab_threads > 0) c->xslab_T = slab_threads;
   c->P.pme_path = pme_path;
   c->use_graph = use_graph != 0;
   if (c->have_frame) TRY(c->fp32 ? choose_slab<float>(c) : choose_slab<double>(c));
@@ -1811,7 +1822,7 @@ int32_t erm_get_tuning(const erm_ctx *c, int32_t *out8) {
   out8[0] = c->Bmax;
   out8[1] = c->P.pme_path;
   out8[2] = c->use_graph ? 1 : 0;
-  out8[3] = c->fp32 ? 1 : 0;
+  out8[3] = c->xslab_T;
   out8[4] = c->slab_S;
   out8[5] = c->xslab_Q;
   out8[6] = c->tune_S;
@@ -1832,7 +1843,7 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
   if (c->slab_S) paths.push_back(1);
   paths.push_back(2);
   double best = 1e300;
-  int bestB = c->Bmax, bestP = c->P.pme_path, bestS = c->tune_S, bestQ = c->tune_Q;
+  int bestB = c->Bmax, bestP = c->P.pme_path, bestS = c->tune_S, bestQ = c->tune_Q, bestT = c->xslab_T;
   auto timed = [&](double *ms) -> int {
     Batch bt{n_trial, 0, nullptr, nullptr, nullptr, true, seed, 0};
     TRY(process(c, bt, flags, nullptr));  // warm-up (allocations, first launches)
@@ -1846,11 +1857,12 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
     return ERM_OK;
   };
   c->use_graph = false;
-  auto apply = [&](int B, int pp, int Sc, int Qc) -> int {
+  auto apply = [&](int B, int pp, int Sc, int Qc, int Tc) -> int {
     c->Bmax = B;
     c->P.pme_path = pp;
     c->tune_S = Sc;
     c->tune_Q = Qc;
+    c->xslab_T = Tc;
     return c->fp32 ? choose_slab<float>(c) : choose_slab<double>(c);
   };
   for (int B : {128, 256, 512, 1024}) {
@@ -1858,26 +1870,29 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
     for (int pp : paths) {
       for (int Sc : {16, 8, 5, 4}) {
         for (int Qc : {16, 8, 5}) {
-          if (pp != 3 && Qc != 8) continue;
-          if (pp == 2 && Sc != 16) continue;
-          if (apply(B, pp, Sc, Qc) != ERM_OK) {
-            c->err.clear();
-            continue;
-          }
-          double ms;
-          TRY(timed(&ms));
-          if (ms < best) {
-            best = ms;
-            bestB = B;
-            bestP = pp;
-            bestS = Sc;
-            bestQ = Qc;
+          for (int Tc : {256, 512}) {
+            if (pp != 3 && (Qc != 8 || Tc != 256)) continue;
+            if (pp == 2 && Sc != 16) continue;
+            if (apply(B, pp, Sc, Qc, Tc) != ERM_OK) {
+              c->err.clear();
+              continue;
+            }
+            double ms;
+            TRY(timed(&ms));
+            if (ms < best) {
+              best = ms;
+              bestB = B;
+              bestP = pp;
+              bestS = Sc;
+              bestQ = Qc;
+              bestT = Tc;
+            }
           }
         }
       }
     }
   }
-  TRY(apply(bestB, bestP, bestS, bestQ));
+  TRY(apply(bestB, bestP, bestS, bestQ, bestT));
   // graph replay against eager launches for the chosen shape
   c->use_graph = true;
   {

@@ -736,8 +736,8 @@ __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename 
 // (consecutive lanes on distinct LDS banks; each real row fills exactly its
 // complex row's bytes).  Each thread then adds the planes'
 // contributions to its MAXR owned sites (FP64 sums).
-template <typename T, int K1, int MAXR>
-__global__ __launch_bounds__(NTHR) void k_pme_xslab(Params P, int S, int Q, int nsites,
+template <typename T, int K1, int MAXR, int XT = NTHR>
+__global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int nsites,
                                                     const typename C2<T>::type *__restrict__ Hk,
                                                     const typename C2<T>::type *__restrict__ tw1,
                                                     const int *__restrict__ z_start, const int *__restrict__ zg,
