@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--slab-planes", type=int, default=0, help="fixed slab depth (with --batch)")
     ap.add_argument("--group-planes", type=int, default=0, help="fixed planes per x group (with --batch)")
     ap.add_argument("--slab-threads", type=int, default=0, help="fixed x-slab workgroup size (with --batch)")
+    ap.add_argument("--column-threads", type=int, default=0, help="fixed z-column workgroup size (with --batch)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="after the run, one device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -163,7 +164,7 @@ def main():
             tuning, _ = g.autotune(n_trial=args.insertions, seed=args.seed + 1)
         else:
             g.set_tuning(max_batch=args.batch, slab_planes=args.slab_planes, group_planes=args.group_planes,
-                         slab_threads=args.slab_threads)
+                         slab_threads=args.slab_threads, column_threads=args.column_threads)
         g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)  # keeps batch and slab shape
         for i in range(warmup):
             g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
@@ -234,6 +235,7 @@ def main():
                       "frames": args.steps, "insertions_per_frame": args.insertions,
                       "batch": bmax, "pme_path": tun["pme_path"], "slab_planes": tun["slab_planes"],
                       "group_planes": tun["group_planes"], "slab_threads": tun["slab_threads"],
+                      "column_threads": tun["column_threads"],
                       "precision": args.precision,
                       "tuning": "autotuned" if not args.no_tune and args.batch == 0 else "fixed"},
            "insertions_per_s": n_ins / dt,

@@ -93,7 +93,8 @@ def lib():
         L.erm_fetch.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.erm_sizes.argtypes = [vp, _i32p, _i32p]
         L.erm_stage_times.argtypes = [vp, _f64p]
-        L.erm_set_tuning.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.erm_set_tuning.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_int32]
         L.erm_get_tuning.argtypes = [vp, _i32p]
         L.erm_autotune.argtypes = [vp, C.c_int32, C.c_uint64, _f64p]
         L.erm_solute_gradient.argtypes = [vp, C.c_int32, _i32p, _f64p, C.c_uint64, C.c_uint64, _f64p,
@@ -333,18 +334,19 @@ class ErmGPU:
         return acc[0], acc[1], acc[2:2 + ns], acc[2 + ns:2 + 2 * ns], acc[2 + 2 * ns:]
 
     def set_tuning(self, max_batch=0, pme_path="auto", use_graph=False, slab_planes=0, group_planes=0,
-                   slab_threads=0):
+                   slab_threads=0, column_threads=0):
         pp = {"auto": 0, "fused": 1, "unfused": 2, "register": 3}[pme_path]
         self._check(lib().erm_set_tuning(self.h, int(max_batch), pp, int(bool(use_graph)), int(slab_planes),
-                                         int(group_planes), int(slab_threads)), "set_tuning")
+                                         int(group_planes), int(slab_threads), int(column_threads)),
+                    "set_tuning")
 
     def tuning(self):
-        out = np.zeros(8, dtype=np.int32)
+        out = np.zeros(9, dtype=np.int32)
         self._check(lib().erm_get_tuning(self.h, _p(out, _i32p)), "get_tuning")
         names = {0: "auto", 1: "fused", 2: "unfused", 3: "register"}
         return dict(max_batch=int(out[0]), pme_path=names[int(out[1])], use_graph=bool(out[2]),
                     slab_threads=int(out[3]), slab_planes=int(out[4]), group_planes=int(out[5]),
-                    slab_cap=int(out[6]), group_cap=int(out[7]))
+                    slab_cap=int(out[6]), group_cap=int(out[7]), column_threads=int(out[8]))
 
     def autotune(self, n_trial=512, seed=12345):
         """Times batch sizes, PME paths and graph replay on the current frame;

@@ -237,6 +237,7 @@ struct erm_ctx {
   int xslab_Q = 0;            // planes per x-c2r group of the register path
   int tune_S = 16, tune_Q = 8;  // caps on slab depth and planes per group (autotuned)
   int xslab_T = NTHR;           // threads per x-slab workgroup, 256 or 512 (autotuned)
+  int zcol_T = NTHR;            // threads per z-column workgroup, 256 or 512 (autotuned)
   std::vector<int> zst;         // first z-sorted site of every plane (host copy)
   size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
@@ -444,15 +445,20 @@ template <typename T> bool reg_path_ok(const erm_ctx *c) {
   return reg_len<T>(k.K1) && reg_len<T>(k.K2) && reg_len<T>(k.K3) && (k.K1 % 2 == 0);
 }
 
-template <typename T, int K>
-int launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk) {
+template <typename T, int K, int XT>
+int launch_zcol_reg_t(erm_ctx *c, const Params &k, int n, int nblk) {
   using V = typename C2<T>::type;
-  const size_t lds = zcol_reg_lds<T>(k.H, k.K2, K);
-  TRY(set_lds(c, (const void *)k_pme_zcol_reg<T, K>, lds, "k_pme_zcol_reg"));
-  hipLaunchKernelGGL((k_pme_zcol_reg<T, K>), dim3(nblk, n), dim3(NTHR), lds, c->st, k, dp<T>(c->d_engfac),
+  const size_t lds = zcol_reg_lds<T>(k.H, k.K2, K, XT);
+  TRY(set_lds(c, (const void *)k_pme_zcol_reg<T, K, XT>, lds, "k_pme_zcol_reg"));
+  hipLaunchKernelGGL((k_pme_zcol_reg<T, K, XT>), dim3(nblk, n), dim3(XT), lds, c->st, k, dp<T>(c->d_engfac),
                      dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<V>(c->d_w3h), dp<V>(c->d_tw3),
                      dp<V>(c->d_Hk), dp<double>(c->d_selfp));
   return ERM_OK;
+}
+// z columns with the tuned workgroup size (256 or 512)
+template <typename T, int K> int launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk) {
+  if (c->zcol_T == 512) return launch_zcol_reg_t<T, K, 512>(c, k, n, nblk);
+  return launch_zcol_reg_t<T, K, NTHR>(c, k, n, nblk);
 }
 template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, int n) {
   using V = typename C2<T>::type;
@@ -694,7 +700,8 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   TRY(ensure(c, c->d_usreal, sizeof(double)));
   const size_t vb = sizeof(V);
   const int ncolwg = zcol_width(K3, vb);
-  const int nzg = regp ? (H * K2 + NTHR - 1) / NTHR : (H * K2 + ncolwg - 1) / ncolwg;
+  const int zt = (c->zcol_T == 512) ? 512 : NTHR;
+  const int nzg = regp ? (H * K2 + zt - 1) / zt : (H * K2 + ncolwg - 1) / ncolwg;
   TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
   if (flags & DO_RECIP) TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
   TRY(mark(c, 0));
@@ -971,8 +978,8 @@ int process(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
 std::string graph_key(erm_ctx *c, const Batch &all, int flags) {
   std::string k(reinterpret_cast<const char *>(&c->kp), sizeof(Params));
   char buf[256];
-  snprintf(buf, sizeof buf, "|%d|%d|%d|%d|%d|%d|%d|%d|%d|%d|%llu|%d", all.n, c->Bmax, flags, c->P.pme_path,
-           c->slab_S, c->slab_R, c->xslab_Q, c->xslab_T, (int)c->have_usreal, c->wsol_ones,
+  snprintf(buf, sizeof buf, "|%d|%d|%d|%d|%d|%d|%d|%d|%d|%d|%d|%llu|%d", all.n, c->Bmax, flags, c->P.pme_path,
+           c->slab_S, c->slab_R, c->xslab_Q, c->xslab_T, c->zcol_T, (int)c->have_usreal, c->wsol_ones,
            (unsigned long long)c->alloc_gen, (int)all.random);
   return k + buf;
 }
@@ -1620,7 +1627,8 @@ int32_t erm_recpcal(erm_ctx *c, int32_t solute, const double *pos, double *erecp
     const int vb = c->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
     const int ncolwg = zcol_width(c->kp.K3, vb);
     const bool regp = c->xslab_Q > 0 && (c->P.pme_path == 0 || c->P.pme_path == 3);
-    const int nzg = regp ? (c->kp.H * c->kp.K2 + NTHR - 1) / NTHR : (c->kp.H * c->kp.K2 + ncolwg - 1) / ncolwg;
+    const int zt = (c->zcol_T == 512) ? 512 : NTHR;
+    const int nzg = regp ? (c->kp.H * c->kp.K2 + zt - 1) / zt : (c->kp.H * c->kp.K2 + ncolwg - 1) / ncolwg;
     std::vector<double> part(nzg);
     HIPCHK(c, hipMemcpy(part.data(), c->d_selfp.p, sizeof(double) * nzg, hipMemcpyDeviceToHost));
     double s = 0.0;
@@ -1801,12 +1809,14 @@ int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
 }
 
 int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t use_graph, int32_t slab_planes,
-                       int32_t group_planes, int32_t slab_threads) {
+                       int32_t group_planes, int32_t slab_threads, int32_t column_threads) {
   if (!c) return ERM_E_ARG;
   if (max_batch < 0 || max_batch > 4096 || pme_path < 0 || pme_path > 3 || use_graph < 0 || use_graph > 1 ||
       slab_planes < 0 || slab_planes > 16 || group_planes < 0 || group_planes > 16 ||
-      (slab_threads != 0 && slab_threads != 256 && slab_threads != 512))
+      (slab_threads != 0 && slab_threads != 256 && slab_threads != 512) ||
+      (column_threads != 0 && column_threads != 256 && column_threads != 512))
     return fail(c, ERM_E_ARG, "tuning out of range");
+  if (column_threads > 0) c->zcol_T = column_threads;
   if (max_batch > 0) c->Bmax = max_batch;
   if (slab_planes > 0) c->tune_S = slab_planes;
   if (group_planes > 0) c->tune_Q = group_planes;
@@ -1817,8 +1827,10 @@ int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t 
   return ERM_OK;
 }
 
-int32_t erm_get_tuning(const erm_ctx *c, int32_t *out8) {
-  if (!c || !out8) return ERM_E_ARG;
+int32_t erm_get_tuning(const erm_ctx *c, int32_t *out9) {
+  if (!c || !out9) return ERM_E_ARG;
+  int32_t *out8 = out9;
+  out9[8] = c->zcol_T;
   out8[0] = c->Bmax;
   out8[1] = c->P.pme_path;
   out8[2] = c->use_graph ? 1 : 0;
@@ -1843,7 +1855,7 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
   if (c->slab_S) paths.push_back(1);
   paths.push_back(2);
   double best = 1e300;
-  int bestB = c->Bmax, bestP = c->P.pme_path, bestS = c->tune_S, bestQ = c->tune_Q, bestT = c->xslab_T;
+  int bestB = c->Bmax, bestP = c->P.pme_path, bestS = c->tune_S, bestQ = c->tune_Q, bestT = c->xslab_T, bestZ = 0;
   auto timed = [&](double *ms) -> int {
     Batch bt{n_trial, 0, nullptr, nullptr, nullptr, true, seed, 0};
     TRY(process(c, bt, flags, nullptr));  // warm-up (allocations, first launches)
@@ -1893,6 +1905,19 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
     }
   }
   TRY(apply(bestB, bestP, bestS, bestQ, bestT));
+  if (bestP == 3) {  // z-column workgroup size for the chosen shape
+    const int z0 = c->zcol_T;
+    double ms;
+    for (int Zc : {256, 512}) {
+      c->zcol_T = Zc;
+      TRY(timed(&ms));
+      if (ms < best) {
+        best = ms;
+        bestZ = Zc;
+      }
+    }
+    c->zcol_T = bestZ ? bestZ : z0;
+  }
   // graph replay against eager launches for the chosen shape
   c->use_graph = true;
   {

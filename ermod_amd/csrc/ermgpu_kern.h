@@ -624,13 +624,14 @@ __global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
 // --------------------------------------------------------------------------
 constexpr int ZREG_SCH = 16;  // solute sites per LDS chunk of k_pme_zcol_reg
 
-// dynamic LDS of k_pme_zcol_reg: ZREG_SCH sites x (H + K2 + K3) spectra + charges
-template <typename T> __host__ __device__ inline size_t zcol_reg_lds(int H, int K2, int K3) {
-  return (size_t)ZREG_SCH * (H + K2 + K3) * 2 * sizeof(T) + ZREG_SCH * sizeof(T) + NTHR * sizeof(double);
+// dynamic LDS of k_pme_zcol_reg with XT threads: ZREG_SCH sites x
+// (H + K2 + K3) spectra + charges + the self-energy reduction
+template <typename T> __host__ __device__ inline size_t zcol_reg_lds(int H, int K2, int K3, int XT = NTHR) {
+  return (size_t)ZREG_SCH * (H + K2 + K3) * 2 * sizeof(T) + ZREG_SCH * sizeof(T) + XT * sizeof(double);
 }
 
-template <typename T, int K3>
-__global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__restrict__ engfac,
+template <typename T, int K3, int XT = NTHR>
+__global__ __launch_bounds__(XT) void k_pme_zcol_reg(Params P, const T *__restrict__ engfac,
                                                        const T *__restrict__ sqv,
                                                        const typename C2<T>::type *__restrict__ w1h,
                                                        const typename C2<T>::type *__restrict__ w2h,
@@ -645,7 +646,7 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__rest
   V *l2 = l1 + ZREG_SCH * H;               // [ZREG_SCH][K2]
   V *l3 = l2 + ZREG_SCH * K2;              // [ZREG_SCH][K3]
   T *lq = reinterpret_cast<T *>(l3 + ZREG_SCH * K3);
-  double *red = reinterpret_cast<double *>(lds_raw + zcol_reg_lds<T>(H, K2, K3) - NTHR * sizeof(double));
+  double *red = reinterpret_cast<double *>(lds_raw + zcol_reg_lds<T>(H, K2, K3, XT) - XT * sizeof(double));
   const int ncol = H * K2;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = c < ncol;
@@ -699,7 +700,7 @@ __global__ __launch_bounds__(NTHR) void k_pme_zcol_reg(Params P, const T *__rest
   red[threadIdx.x] = self;
   reg_fft<K3, +1>(X, tw3);
   __syncthreads();
-  for (int st = NTHR / 2; st > 0; st >>= 1) {
+  for (int st = XT / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }

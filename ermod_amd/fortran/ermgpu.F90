@@ -183,16 +183,17 @@ module ermgpu
      end function erm_stage_times
 
      integer(c_int32_t) function erm_set_tuning(ctx, max_batch, pme_path, use_graph, slab_planes, &
-          group_planes, slab_threads) bind(C, name="erm_set_tuning")
+          group_planes, slab_threads, column_threads) bind(C, name="erm_set_tuning")
        import :: c_ptr, c_int32_t
        type(c_ptr), value :: ctx
-       integer(c_int32_t), value :: max_batch, pme_path, use_graph, slab_planes, group_planes, slab_threads
+       integer(c_int32_t), value :: max_batch, pme_path, use_graph, slab_planes, group_planes, slab_threads, &
+                                    column_threads
      end function erm_set_tuning
 
-     integer(c_int32_t) function erm_get_tuning(ctx, out8) bind(C, name="erm_get_tuning")
+     integer(c_int32_t) function erm_get_tuning(ctx, out9) bind(C, name="erm_get_tuning")
        import :: c_ptr, c_int32_t
        type(c_ptr), value :: ctx
-       integer(c_int32_t), intent(out) :: out8(8)
+       integer(c_int32_t), intent(out) :: out9(9)
      end function erm_get_tuning
 
      integer(c_int32_t) function erm_autotune(ctx, n_trial, seed, best_ms) bind(C, name="erm_autotune")

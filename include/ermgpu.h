@@ -203,13 +203,15 @@ int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
  * runs eagerly and captures; a change of frame geometry, batch shape or
  * buffers re-captures); slab_planes (1..16) caps the z planes of one fused
  * PME slab and group_planes (1..16) the planes the register path transforms
- * per group, slab_threads (256 or 512) the workgroup size of the register
- * path's x slab (0 keeps any of them). */
+ * per group, slab_threads and column_threads (256 or 512) the workgroup
+ * sizes of the register path's x slab and z columns (0 keeps any of them). */
 int32_t erm_set_tuning(erm_ctx *ctx, int32_t max_batch, int32_t pme_path, int32_t use_graph,
-                       int32_t slab_planes, int32_t group_planes, int32_t slab_threads);
-/* out8 = {max_batch, pme_path, use_graph, slab threads (256 / 512), slab
- * planes in use, group planes in use, slab cap, group cap} */
-int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out8);
+                       int32_t slab_planes, int32_t group_planes, int32_t slab_threads,
+                       int32_t column_threads);
+/* out9 = {max_batch, pme_path, use_graph, slab threads (256 / 512), slab
+ * planes in use, group planes in use, slab cap, group cap, z-column
+ * threads (256 / 512)} */
+int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out9);
 /* Times n_trial random insertions (seed) on the current frame for batch
  * sizes 128..1024, every applicable PME path and slab / group shapes, then
  * graph replay against eager launches, and keeps the fastest; accumulators
