@@ -1436,17 +1436,18 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
 }
 
 // edens(i) += engnmfc * insdst(i), configurations in order (engproc.F90:836-840)
-__global__ void k_edens(int n, int ermax, const double *__restrict__ hist, const double *__restrict__ w,
-                        double *edens) {
+__global__ __launch_bounds__(256) void k_edens(int n, int ermax, const double *__restrict__ hist,
+                                               const double *__restrict__ w, double *edens) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ermax) return;
+  // the reference's sequential order per bin; 32 loads in flight per thread
   double e = edens[i];
-  for (int b0 = 0; b0 < n; b0 += 8) {
-    double h[8];
+  for (int b0 = 0; b0 < n; b0 += 32) {
+    double h[32];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) h[u] = (b0 + u < n) ? hist[(size_t)(b0 + u) * ermax + i] : 0.0;
+    for (int u = 0; u < 32; ++u) h[u] = (b0 + u < n) ? hist[(size_t)(b0 + u) * ermax + i] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < 32; ++u)
       if (h[u] != 0.0) e = e + w[b0 + u] * h[u];
   }
   edens[i] = e;
