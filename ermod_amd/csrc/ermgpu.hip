@@ -1880,8 +1880,8 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
   for (int B : {128, 256, 512, 1024}) {
     if (B > 4 * n_trial && B > 128) break;
     for (int pp : paths) {
-      for (int Sc : {16, 8, 5, 4}) {
-        for (int Qc : {16, 8, 5}) {
+      for (int Sc : {16, 8, 5, 4, 3}) {
+        for (int Qc : {16, 8, 5, 4, 3}) {
           for (int Tc : {256, 512}) {
             if (pp != 3 && (Qc != 8 || Tc != 256)) continue;
             if (pp == 2 && Sc != 16) continue;
