@@ -114,6 +114,45 @@ def cpu_baseline(sy, frame0, n_ins_max, seconds, seed):
             "insertions_per_s": done / dt}
 
 
+def mixed_pass(torch, dev, frames=4, per_frame=5000, prec="fp32"):
+    """The heterogeneous insertion workload: 7000 waters + 800 10-site
+    cosolvent molecules + 100 Na+ + 100 Cl- in a 60 A box (K = 64), per frame
+    5000 insertions split over four rigid solutes of 5, 20, 60 and 150 sites,
+    one context each (600 bins apiece), run concurrently on four streams
+    from one device copy of the frame; FP32.  Each context is autotuned on
+    the first frame (untimed)."""
+    from ermod_amd import synth
+    from ermod_amd.api import ErmGPU, insert_random_multi
+    sizes = synth.MIXED_SOLUTE_SITES
+    syss = [synth.mixed(ns) for ns in sizes]
+    big = max(syss, key=lambda sy: sy.system.natom)
+    host = [big.frame(i) for i in range(frames + 1)]
+    dpos = [torch.from_numpy(f.reshape(-1)).to(dev) for f in host]
+    share = [per_frame // len(sizes)] * len(sizes)
+    engines = [ErmGPU(sy.system, prec) for sy in syss]
+    try:
+        for e, sy in zip(engines, syss):
+            e.set_frame_device(dpos[0].data_ptr(), sy.cell)
+            e.autotune(n_trial=share[0], seed=99)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(1, frames + 1):
+            for e, sy in zip(engines, syss):
+                e.set_frame_device(dpos[f].data_ptr(), sy.cell)
+            insert_random_multi(engines, share, [11, 12, 13, 14], [f * s for s in share])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        nmol = syss[0].system.nmol - 1
+        nins = frames * sum(share)
+        return {"workload": "7000 waters + 800 cosolvent + 100 Na+ + 100 Cl- in 60 A, K 64; solutes of "
+                            "5/20/60/150 sites, 1250 insertions each per frame, concurrent contexts",
+                "precision": prec, "frames": frames, "mol_pairs_per_s": nins * nmol / dt,
+                "insertions_per_s": nins / dt, "ms_per_frame": dt / frames * 1e3}
+    finally:
+        for e in engines:
+            e.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +171,8 @@ def main():
     ap.add_argument("--group-planes", type=int, default=0, help="fixed planes per x group (with --batch)")
     ap.add_argument("--slab-threads", type=int, default=0, help="fixed x-slab workgroup size (with --batch)")
     ap.add_argument("--column-threads", type=int, default=0, help="fixed z-column workgroup size (with --batch)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the heterogeneous pass (four solute sizes in a mixed solvent, concurrently)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="after the run, one device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -252,6 +293,8 @@ def main():
         out["fp32"] = {"value": n32 * slvmax / dt32, "insertions_per_s": n32 / dt32,
                        "batch": tun32["max_batch"], "pme_path": tun32["pme_path"],
                        "stage_ms": {k: round(v, 3) for k, v in st32.items()}}
+    if not args.no_mixed and args.precision == "fp64":
+        out["heterogeneous"] = mixed_pass(torch, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sy, frames[0], 100000, args.cpu_seconds, args.seed)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

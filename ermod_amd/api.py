@@ -55,7 +55,8 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_set_topology", "erm_set_bins", "erm_engclear", "erm_set_frame", "erm_set_frame_device",
            "erm_get_uv_energy", "erm_realcal_proc", "erm_realcal_self", "erm_recpcal", "erm_engconst",
            "erm_insert_random", "erm_update_histogram", "erm_engstore", "erm_fetch", "erm_sizes",
-           "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune")
+           "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune",
+           "erm_insert_random_multi")
 
 _lib = None
 
@@ -97,6 +98,8 @@ def lib():
                                      C.c_int32]
         L.erm_get_tuning.argtypes = [vp, _i32p]
         L.erm_autotune.argtypes = [vp, C.c_int32, C.c_uint64, _f64p]
+        L.erm_insert_random_multi.argtypes = [C.POINTER(vp), C.c_int32, _i32p, C.POINTER(C.c_uint64),
+                                              C.POINTER(C.c_uint64)]
         L.erm_solute_gradient.argtypes = [vp, C.c_int32, _i32p, _f64p, C.c_uint64, C.c_uint64, _f64p,
                                           C.c_int32, _f64p, _f64p, _f64p, _f64p]
         _lib = L
@@ -130,6 +133,20 @@ def build_edges(h, self_energy=False):
     if st:
         raise ErmError(st, "erm_build_edges")
     return out[:pemax.value].copy(), pemax.value, pesoft.value
+
+
+def insert_random_multi(engines, n, seeds, first_indices):
+    """erm_insert_random on several contexts at once (solutes of different
+    sizes on one frame, each on its own stream)."""
+    k = len(engines)
+    hs = (C.c_void_p * k)(*[e.h for e in engines])
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    ss = (C.c_uint64 * k)(*[int(x) for x in seeds])
+    ff = (C.c_uint64 * k)(*[int(x) for x in first_indices])
+    st = lib().erm_insert_random_multi(hs, k, _p(nn, _i32p), ss, ff)
+    if st:
+        msgs = [lib().erm_last_error(e.h) for e in engines]
+        raise ErmError(st, "insert_random_multi: " + "; ".join(m.decode() for m in msgs if m))
 
 
 def hash_double(v):

@@ -1683,6 +1683,46 @@ int32_t erm_insert_random(erm_ctx *c, int32_t n, uint64_t seed, uint64_t first_i
   return process(c, bt, flags, uvengy);
 }
 
+int32_t erm_insert_random_multi(erm_ctx *const *ctxs, int32_t nctx, const int32_t *n, const uint64_t *seed,
+                                const uint64_t *first_index) {
+  if (!ctxs || !n || !seed || !first_index || nctx < 0) return ERM_E_ARG;
+  const int flags = DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF | DO_HIST;
+  for (int i = 0; i < nctx; ++i) {
+    erm_ctx *c = ctxs[i];
+    TRY(need(c, true, true));
+    if (c->P.slttype != 2 || c->bfcoord.empty())
+      return fail(c, ERM_E_ARG, "random insertion needs a rigid test particle with bfcoord");
+    if (n[i] < 0) return fail(c, ERM_E_ARG, "negative count");
+    if (c->P.device != ctxs[0]->P.device) return fail(c, ERM_E_ARG, "contexts on different devices");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == c) return fail(c, ERM_E_ARG, "a context appears twice");
+  }
+  if (nctx == 0) return ERM_OK;
+  if (hipSetDevice(ctxs[0]->P.device) != hipSuccess) return fail(ctxs[0], ERM_E_DEVICE, "hipSetDevice");
+  // enqueue every context's batches on its own stream, then wait for each
+  int first_err = ERM_OK;
+  std::vector<int> nbs(nctx, 0), ok(nctx, 0);
+  for (int i = 0; i < nctx; ++i) {
+    erm_ctx *c = ctxs[i];
+    Batch bt{n[i], 0, nullptr, nullptr, nullptr, true, seed[i], first_index[i]};
+    for (int k = 0; k < 8; ++k) c->stage_ms[k] = 0;
+    nbs[i] = (bt.n + c->Bmax - 1) / c->Bmax;
+    int st = batch_events(c, nbs[i]);
+    if (!st) st = begin_call(c, bt);
+    if (!st) st = enqueue_batches(c, bt, flags, nullptr);
+    c->cur_ev = nullptr;
+    if (st && !first_err) first_err = st;
+    ok[i] = (st == ERM_OK);
+  }
+  for (int i = 0; i < nctx; ++i) {
+    erm_ctx *c = ctxs[i];
+    int st = ok[i] ? check_status(c) : sync(c);
+    if (ok[i] && !st) st = stage_times(c, nbs[i], true);
+    if (st && !first_err) first_err = st;
+  }
+  return first_err;
+}
+
 int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, const double *weights,
                              const double *uvengy) {
   TRY(need(c, true, true));

@@ -151,6 +151,15 @@ module ermgpu
        type(c_ptr), value :: uvengy
      end function erm_insert_random
 
+     integer(c_int32_t) function erm_insert_random_multi(ctxs, nctx, n, seed, first_index) &
+          bind(C, name="erm_insert_random_multi")
+       import :: c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), intent(in) :: ctxs(*)
+       integer(c_int32_t), value :: nctx
+       integer(c_int32_t), intent(in) :: n(*)
+       integer(c_int64_t), intent(in) :: seed(*), first_index(*)
+     end function erm_insert_random_multi
+
      integer(c_int32_t) function erm_update_histogram(ctx, n, solutes, weights, uvengy) &
           bind(C, name="erm_update_histogram")
        import :: c_ptr, c_int32_t

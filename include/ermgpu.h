@@ -168,6 +168,15 @@ int32_t erm_engconst(erm_ctx *ctx, int32_t n, const int32_t *solutes, const doub
 int32_t erm_insert_random(erm_ctx *ctx, int32_t n, uint64_t seed, uint64_t first_index,
                           double *uvengy);
 
+/* erm_insert_random on nctx contexts at once (no counterpart in the
+ * reference, which runs one solute species per run): each context's batches
+ * are enqueued on its own stream, so solutes of different sizes run
+ * concurrently, and one wait per context follows.  n, seed, first_index
+ * hold one value per context.  Returns the first error; each context's
+ * message is in its own erm_last_error. */
+int32_t erm_insert_random_multi(erm_ctx *const *ctxs, int32_t nctx, const int32_t *n, const uint64_t *seed,
+                                const uint64_t *first_index);
+
 /* update_histogram alone, for energies computed elsewhere (n configurations,
  * uvengy n*(slvmax+1); solutes as in erm_engconst, NULL in refs mode). */
 int32_t erm_update_histogram(erm_ctx *ctx, int32_t n, const int32_t *solutes,
