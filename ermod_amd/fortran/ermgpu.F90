@@ -240,6 +240,124 @@ contains
     end do
   end function erm_last_error
 
+  ! engstore's file output (engproc.F90:560-656), called after erm_engstore
+  ! for division `division` of `engdiv`: engsln|engref.NN and, with corrcal,
+  ! corsln|corref.NN, with selfcal slfeng.NN ('.tt' when engdiv = 1); at the
+  ! last division aveuv.tt (soln), weight_soln|weight_refs and uvrange.tt.
+  ! avediv(engdiv, 2) and aveuv(engdiv, numslv) carry the per-division
+  ! values across calls.  Files go to directory `dir`.
+  subroutine erm_store_division(ctx, division, engdiv, slttype, wgtslf, corrcal, temp, numslv, uvmax, &
+       uvsoft, uvcrd, esmax, escrd, avediv, aveuv, dir)
+    use iso_c_binding, only: c_loc
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: division, engdiv, slttype, wgtslf, corrcal, numslv, esmax
+    integer(c_int32_t), intent(in) :: uvmax(numslv), uvsoft(numslv)
+    real(c_double), intent(in) :: temp, uvcrd(*), escrd(*)
+    real(c_double), intent(inout) :: avediv(engdiv, 2), aveuv(engdiv, numslv)
+    character(*), intent(in) :: dir
+    integer(c_int32_t) :: slvmax, ermax, st
+    real(c_double), allocatable, target :: edens(:), ecorr(:, :), eself(:), minuv(:), maxuv(:), slnuv(:)
+    real(c_double), target :: scal(4)
+    real(c_double) :: factor
+    character(len=3) :: suff
+    character(len=6) :: stem
+    integer :: k, iduv, pti, idmin, u
+    st = erm_sizes(ctx, slvmax, ermax)
+    allocate(edens(ermax), eself(max(esmax, 1)), minuv(0:numslv), maxuv(0:numslv), slnuv(0:numslv))
+    if (corrcal /= 0) then
+       allocate(ecorr(ermax, ermax))
+       st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_loc(eself), c_loc(minuv), c_loc(maxuv), &
+                      c_loc(slnuv), c_loc(scal))
+    else
+       st = erm_fetch(ctx, c_loc(edens), c_null_ptr, c_loc(eself), c_loc(minuv), c_loc(maxuv), &
+                      c_loc(slnuv), c_loc(scal))
+    end if
+    ! scal = engnorm, engsmpl (raw), avslf / engnorm, voffset
+    avediv(division, 1) = scal(1) / scal(2)
+    if (slttype == 1) then
+       aveuv(division, 1:numslv) = slnuv(1:numslv) / scal(1)
+       avediv(division, 2) = scal(4) - temp * log(scal(3))
+    else
+       avediv(division, 2) = scal(4) + temp * log(scal(3))
+    end if
+    if (division == engdiv) then
+       if (slttype == 1) then
+          open(newunit=u, file=dir//'/aveuv.tt', action='write')
+          do k = 1, engdiv
+             write(u, '(i5, 9999f15.5)') k, aveuv(k, 1:numslv)
+          end do
+          endfile(u)
+          close(u)
+          open(newunit=u, file=dir//'/weight_soln', action='write')
+       else
+          open(newunit=u, file=dir//'/weight_refs', action='write')
+       end if
+       do k = 1, engdiv
+          if (wgtslf == 0) then
+             write(u, '(i5,e20.8)') k, avediv(k, 1)
+          else
+             write(u, '(i5,e20.8,e20.7)') k, avediv(k, 1), avediv(k, 2)
+          end if
+       end do
+       endfile(u)
+       close(u)
+       open(newunit=u, file=dir//'/uvrange.tt', action='write')
+       write(u, '(A)') ' species     minimum        maximum'
+       do pti = 0, numslv
+          if (maxuv(pti) < 1.0d5) then
+             write(u, '(i5,2f15.5)') pti, minuv(pti), maxuv(pti)
+          else
+             write(u, '(i5,f15.5,g18.5)') pti, minuv(pti), maxuv(pti)
+          end if
+       end do
+       endfile(u)
+       close(u)
+    end if
+    if (engdiv == 1) then
+       suff = '.tt'
+    else
+       write(suff, '(a1,i2.2)') '.', mod(division, 100)
+    end if
+    stem = 'engref'
+    if (slttype == 1) stem = 'engsln'
+    open(newunit=u, file=dir//'/'//stem//suff, form='formatted', action='write')
+    idmin = 0
+    pti = 1
+    do iduv = 1, ermax
+       do while (iduv > idmin + uvmax(pti))
+          idmin = idmin + uvmax(pti)
+          pti = pti + 1
+       end do
+       ! the species' edges from uvcrd(idmin + 1) on (sequence association,
+       ! like the reference's uvcrd(iduv + 1) at the last linear bin)
+       factor = erm_repval(uvcrd(idmin + 1), uvmax(pti), uvsoft(pti), iduv - idmin)
+       write(u, '(g15.7,i5,g25.15)') factor, pti, edens(iduv)
+    end do
+    endfile(u)
+    close(u)
+    if (corrcal /= 0) then
+       stem = 'corref'
+       if (slttype == 1) stem = 'corsln'
+       open(newunit=u, file=dir//'/'//stem//suff, form='unformatted', action='write')
+       write(u) ecorr
+       endfile(u)
+       close(u)
+    end if
+    if (esmax > 0) then
+       open(newunit=u, file=dir//'/slfeng'//suff, form='formatted', action='write')
+       do iduv = 1, esmax
+          if (iduv < esmax) then
+             factor = (escrd(iduv) + escrd(iduv + 1)) / 2.0d0
+          else
+             factor = escrd(esmax)
+          end if
+          write(u, '(g15.7,g25.15)') factor, eself(iduv)
+       end do
+       endfile(u)
+       close(u)
+    end if
+  end subroutine erm_store_division
+
   ! representative energy of bin iduv (1-based) of one species, as repval
   ! (engproc.F90:1185-1223) computes it for the engsln/engref files
   real(c_double) function erm_repval(crd, idmax, idsoft, idpt) result(v)

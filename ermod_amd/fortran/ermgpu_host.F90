@@ -1,16 +1,17 @@
 ! ermgpu_host.F90 -- Fortran host of the energy pipeline through the
 ! ISO_C_BINDING module ermgpu: the division loop of ERmod's trjmain
-! (enganal.F90:88-97) for a test-particle system -- engclear, per frame
-! set_frame + random insertions (engconst), engstore -- writing engref and
-! corref in the reference's formats (engproc.F90:609-644).
+! (enganal.F90:88-97) for a test-particle system -- per division engclear,
+! per frame set_frame + random insertions (engconst), engstore and the
+! reference's output files (erm_store_division, engproc.F90:560-656).
 !
 ! usage: ermgpu_host <input stream file> <output directory>
 ! The input holds, in order and without padding: natom nmol ntype nframes
-! nins (int32); the erm_params fields; the erm_hist fields of species 1;
+! nins ndiv (int32); the erm_params fields; the erm_hist fields of species 1
+! (also used for the self-energy coordinate when selfcal = 1);
 ! mol_begin(nmol+1) charge(natom) ljtype(natom) lj_eps(ntype^2)
 ! lj_sig2(ntype^2) sluvid(nmol) uvspec(nmol) mol_charge(nmol)
 ! sitemass(natom) bfcoord(3*ns) cell(9) seed(int64); then nframes frames
-! of 3*natom coordinates.
+! of 3*natom coordinates, split evenly over the ndiv divisions.
 program ermgpu_host
   use iso_c_binding
   use ermgpu
@@ -18,20 +19,22 @@ program ermgpu_host
   type(erm_params) :: prm
   type(erm_hist) :: hst
   type(c_ptr) :: ctx
-  integer(c_int32_t) :: natom, nmol, ntype, nframes, nins, st, pemax, pesoft, slvmax, ermax, ns
+  integer(c_int32_t) :: natom, nmol, ntype, nframes, nins, ndiv, st, pemax, pesoft, slvmax, ermax, ns
+  integer(c_int32_t) :: esmax, essoft
   integer(c_int32_t), allocatable :: mol_begin(:), ljtype(:), sluvid(:), uvspec(:)
   real(c_double), allocatable :: charge(:), lj_eps(:), lj_sig2(:), mol_charge(:), sitemass(:), bfcoord(:)
-  real(c_double), allocatable, target :: pos(:), uv(:, :), edens(:), ecorr(:, :), ercrd(:)
-  real(c_double), target :: scal(4), dummy(1)
-  real(c_double) :: cell(3, 3), factor
+  real(c_double), allocatable, target :: pos(:), uv(:, :), edens(:), ecorr(:, :), ercrd(:), escrd(:)
+  real(c_double), allocatable :: avediv(:, :), aveuv(:, :)
+  real(c_double), target :: scal(4)
+  real(c_double) :: cell(3, 3)
   integer(c_int64_t) :: seed
-  integer :: f, iduv
+  integer :: f, div, fpd
   character(len=1024) :: inp, outdir
 
   call get_command_argument(1, inp)
   call get_command_argument(2, outdir)
   open(10, file=trim(inp), access='stream', form='unformatted', status='old', action='read')
-  read(10) natom, nmol, ntype, nframes, nins
+  read(10) natom, nmol, ntype, nframes, nins, ndiv
   read(10) prm%slttype, prm%estype, prm%ljswitch, prm%splodr, prm%ms, prm%corrcal, prm%selfcal, &
        prm%wgtslf, prm%force_calculation, prm%elecut, prm%lwljcut, prm%upljcut, prm%screen, &
        prm%temp, prm%block_threshold, prm%precision, prm%max_batch, prm%device
@@ -45,63 +48,59 @@ program ermgpu_host
   allocate(bfcoord(3 * ns), pos(3 * natom))
   read(10) bfcoord, cell, seed
 
-  ! enginit: bin edges of the single solvent species
-  allocate(ercrd(10002))
+  ! enginit: bin edges of the single solvent species (and of the self energy)
+  allocate(ercrd(10002), escrd(10002))
   st = erm_build_edges(hst, 0, ercrd, 10002, pemax, pesoft)
   call check(st, 'erm_build_edges')
+  esmax = 0
+  if (prm%selfcal /= 0) then
+     st = erm_build_edges(hst, 1, escrd, 10002, esmax, essoft)
+     call check(st, 'erm_build_edges (self)')
+  end if
   st = erm_create(prm, ctx)
   call check(st, 'erm_create')
   st = erm_set_topology(ctx, natom, nmol, mol_begin, charge, ljtype, ntype, lj_eps, lj_sig2, &
        sluvid, uvspec, mol_charge, sitemass, bfcoord)
   call check(st, 'erm_set_topology')
-  st = erm_set_bins(ctx, 1, [pemax], [pesoft], ercrd, 0, dummy)
+  st = erm_set_bins(ctx, 1, [pemax], [pesoft], ercrd, esmax, escrd)
   call check(st, 'erm_set_bins')
   st = erm_sizes(ctx, slvmax, ermax)
   allocate(uv(slvmax + 1, nins), edens(ermax), ecorr(ermax, ermax))
+  allocate(avediv(ndiv, 2), aveuv(ndiv, 1))
 
-  ! one division: engclear, engconst per frame, engstore
-  st = erm_engclear(ctx)
-  call check(st, 'erm_engclear')
-  do f = 1, nframes
-     read(10) pos
-     st = erm_set_frame(ctx, pos, cell, 1.0d0)
-     call check(st, 'erm_set_frame')
-     if (f == 1) then
-        st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_loc(uv))
-     else
-        st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_null_ptr)
+  fpd = nframes / ndiv
+  do div = 1, ndiv
+     st = erm_engclear(ctx)
+     call check(st, 'erm_engclear')
+     do f = (div - 1) * fpd + 1, div * fpd
+        read(10) pos
+        st = erm_set_frame(ctx, pos, cell, 1.0d0)
+        call check(st, 'erm_set_frame')
+        if (f == 1) then
+           st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_loc(uv))
+        else
+           st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_null_ptr)
+        end if
+        call check(st, 'erm_insert_random')
+     end do
+     if (div == ndiv) then
+        ! raw accumulators of the last division, before normalisation
+        st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                       c_loc(scal))
+        call check(st, 'erm_fetch')
+        open(11, file=trim(outdir)//'/raw.bin', access='stream', form='unformatted', status='replace')
+        write(11) slvmax, ermax, scal, edens, ecorr, uv
+        close(11)
      end if
-     call check(st, 'erm_insert_random')
+     st = erm_engstore(ctx)
+     call check(st, 'erm_engstore')
+     call erm_store_division(ctx, div, int(ndiv), int(prm%slttype), int(prm%wgtslf), int(prm%corrcal), &
+          prm%temp, 1, [pemax], [pesoft], ercrd, int(esmax), escrd, avediv, aveuv, trim(outdir))
   end do
   close(10)
-
-  st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(scal))
-  call check(st, 'erm_fetch')
-  open(11, file=trim(outdir)//'/raw.bin', access='stream', form='unformatted', status='replace')
-  write(11) slvmax, ermax, scal, edens, ecorr, uv
-  close(11)
-
-  st = erm_engstore(ctx)
-  call check(st, 'erm_engstore')
-  st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(scal))
-  call check(st, 'erm_fetch')
-  ! engref.tt (engproc.F90:609-622): representative energy, species, density
-  open(12, file=trim(outdir)//'/engref.tt', form='formatted', status='replace')
-  do iduv = 1, ermax
-     factor = erm_repval(ercrd, pemax, pesoft, iduv)
-     write(12, '(g15.7,i5,g25.15)') factor, 1, edens(iduv)
-  end do
-  close(12)
-  ! corref.tt (engproc.F90:625-640): one unformatted sequential record
-  open(13, file=trim(outdir)//'/corref.tt', form='unformatted', status='replace')
-  write(13) ecorr
-  close(13)
-  ! weight_refs (engproc.F90:588-601), wgtslf = 0
-  open(14, file=trim(outdir)//'/weight_refs', form='formatted', status='replace')
-  write(14, '(i5,e20.8)') 1, scal(1) / scal(2)
-  close(14)
   call erm_destroy(ctx)
-  print '(a,i0,a,i0,a)', 'ermgpu_host: ', nframes, ' frames x ', nins, ' insertions done'
+  print '(a,i0,a,i0,a,i0,a)', 'ermgpu_host: ', ndiv, ' divisions, ', nframes, ' frames x ', nins, &
+       ' insertions done'
 
 contains
   subroutine check(code, what)

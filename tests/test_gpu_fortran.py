@@ -1,7 +1,9 @@
 """The Fortran host (ermod_amd/fortran/ermgpu_host, built by amdflang over
 the ISO_C_BINDING module ermgpu) drives the device pipeline end to end;
-its raw accumulators match the oracle bit for bit and its engref/corref files
-carry the reference's formats."""
+its raw accumulators match the oracle bit for bit and its output files
+(engref, corref, slfeng per division; weight_refs and uvrange.tt) carry the
+reference's formats (engproc.F90:560-656) and the oracle's values."""
+import dataclasses
 import os
 import struct
 import subprocess
@@ -19,11 +21,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "ermod_amd", "fortran", "ermgpu_host")
 
 
-def write_input(path, sy, frames, nins, seed):
-    s = sy.system
+def write_input(path, sy, frames, nins, seed, ndiv=1, s=None):
+    s = s or sy.system
     h = s.hists[0]
     with open(path, "wb") as f:
-        f.write(struct.pack("<5i", s.natom, s.nmol, s.ntype, len(frames), nins))
+        f.write(struct.pack("<6i", s.natom, s.nmol, s.ntype, len(frames), nins, ndiv))
         f.write(struct.pack("<4i3i4i6d3i", s.slttype, s.estype, s.ljswitch, s.splodr, *s.ms, s.corrcal,
                             s.selfcal, s.wgtslf, s.force_calculation, s.elecut, s.lwljcut, s.upljcut,
                             s.screen, s.temp, s.block_threshold, 0, 16, 0))
@@ -92,3 +94,53 @@ def test_fortran_host_end_to_end(tmp_path):
     assert n1 == 8 * ermax * ermax and struct.unpack("<i", rec[-4:])[0] == n1
     cor = np.frombuffer(rec[4:-4], np.float64).reshape(ermax, ermax)
     assert np.array_equal(cor, a["ecorr"] / a["engnorm"])
+
+
+def test_fortran_host_divisions_and_files(tmp_path):
+    """Two divisions with the self-energy distribution: per-division files,
+    the weights of both divisions and the overall energy range."""
+    print("\n[gpu case] fortran host tiny_refs, 2 divisions, selfcal", flush=True)
+    sy = synth.tiny_refs()
+    s = dataclasses.replace(sy.system, selfcal=1, self_hist=sy.system.hists[0])
+    frames = [sy.frame(i) for i in range(4)]
+    nins, seed = 10, 77
+    inp = tmp_path / "in.bin"
+    write_input(inp, sy, frames, nins, seed, ndiv=2, s=s)
+    r = subprocess.run([EXE, str(inp), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = O.Oracle(s)
+    m = sy.solute_mol
+    per_div = []
+    for div in range(2):
+        o.engclear()
+        for fi in range(2 * div, 2 * div + 2):
+            o.set_frame(frames[fi], sy.cell)
+            for i in range(nins):
+                rr, q = O.random_pose(seed, fi * nins + i)
+                sp = O.place_solute(s.bfcoord, s.solute_mass(m), sy.cell, rr, q)
+                o.update_histogram(m, o.uv_energy(m, sp))
+        a = o.acc()
+        per_div.append({k: (v.copy() if hasattr(v, "copy") else v) for k, v in a.items()})
+    for div, a in enumerate(per_div, start=1):
+        lines = open(tmp_path / f"engref.{div:02d}").read().splitlines()
+        dens = np.array([float(l[20:]) for l in lines])
+        np.testing.assert_allclose(dens, a["edens"] / a["engnorm"], rtol=1e-14, atol=1e-300)
+        slf = open(tmp_path / f"slfeng.{div:02d}").read().splitlines()
+        assert len(slf) == len(a["eself"]) and all(len(l) == 40 for l in slf)
+        np.testing.assert_allclose([float(l[15:]) for l in slf], a["eself"] / a["engnorm"], rtol=1e-14,
+                                   atol=1e-300)
+        rec = open(tmp_path / f"corref.{div:02d}", "rb").read()
+        cor = np.frombuffer(rec[4:-4], np.float64).reshape(a["ecorr"].shape)
+        assert np.array_equal(cor, a["ecorr"] / a["engnorm"])
+    w = open(tmp_path / "weight_refs").read().splitlines()
+    assert len(w) == 2 and all(len(l) == 25 for l in w)
+    for div, (line, a) in enumerate(zip(w, per_div), start=1):
+        assert int(line[:5]) == div
+        assert float(line[5:]) == pytest.approx(a["engnorm"] / a["engsmpl"], rel=1e-7)
+    uvr = open(tmp_path / "uvrange.tt").read().splitlines()
+    assert uvr[0] == " species     minimum        maximum"
+    a = per_div[-1]
+    for pti, line in enumerate(uvr[1:]):
+        assert int(line[:5]) == pti
+        mn = float(line[5:20])
+        assert mn == pytest.approx(a["minuv"][pti], abs=1e-5)
