@@ -35,7 +35,7 @@ class ErmParams(C.Structure):
                 ("elecut", C.c_double), ("lwljcut", C.c_double), ("upljcut", C.c_double),
                 ("screen", C.c_double), ("temp", C.c_double), ("block_threshold", C.c_double),
                 ("precision", C.c_int32), ("max_batch", C.c_int32), ("device", C.c_int32),
-                ("pme_path", C.c_int32)]
+                ("pme_path", C.c_int32), ("cltype", C.c_int32), ("boxshp", C.c_int32)]
 
 
 class ErmHist(C.Structure):
@@ -173,6 +173,7 @@ class ErmGPU:
         P.max_batch = max_batch
         P.device = device
         P.pme_path = {"auto": 0, "fused": 1, "unfused": 2, "register": 3}[pme_path]
+        P.cltype, P.boxshp = s.cltype, s.boxshp
         h = C.c_void_p()
         st = lib().erm_create(C.byref(P), C.byref(h))
         if st:

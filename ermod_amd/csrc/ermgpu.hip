@@ -113,6 +113,92 @@ void greenfunc(const int *K, int p, const double *invcl, double volume, double s
   engfac[0] = 0.0;
 }
 
+// calc_gfb_pppm (recpcal.F90:168-186): coefficients b_l of the alias sum
+// sum_m W^2(k + 2 pi m / h) = sum_l b_l sin^(2l)(k h / 2) for order p;
+// after the inner loop its index is 0, as in the reference
+void pppm_gfb(int p, double *b) {
+  for (int i = 0; i < p; ++i) b[i] = 0.0;
+  b[0] = 1.0;
+  for (int m = 1; m < p; ++m) {
+    int l = m;
+    for (; l >= 1; --l)
+      b[l] = 4.0 * (b[l] * (double)(l - m) * ((double)(l - m) - 0.5) -
+                    b[l - 1] * (double)(l - m - 1) * (double)(l - m - 1));
+    b[0] = 4.0 * (b[0] * (double)(l - m) * ((double)(l - m) - 0.5));
+  }
+  double f = 1.0;
+  for (int i = 2; i <= 2 * p - 1; ++i) f *= (double)i;
+  const double gi = 1.0 / f;
+  for (int i = 0; i < p; ++i) b[i] = b[i] * gi;
+}
+
+// recpcal_pppm_greenfunc (recpcal.F90:205-325): the optimal influence
+// function on the (K1/2+1, K2, K3) half spectrum, engfac[m3][m2][m1].  The
+// alias range per axis is int(cell column . m_max) with m_max(d) = alpha /
+// (pi K_d) (-ln 1e-7)^(1/4); the z alias offset is mz * K2 as in the
+// reference (recpcal.F90:293), identical to mz * K3 when K2 = K3.
+// Returns false for an empty (negative) alias range.
+bool pppm_greenfunc(const int *K, int p, const double *cell, const double *invcl, double volume, double screen,
+                    std::vector<double> &engfac) {
+  const int K1 = K[0], K2 = K[1], K3 = K[2], H = K1 / 2 + 1;
+  engfac.assign((size_t)H * K2 * K3, 0.0);
+  double b[MAXP];
+  pppm_gfb(p, b);
+  const double hoc = std::pow(-std::log(0.0000001), 0.25);
+  double mm[3];
+  for (int d = 0; d < 3; ++d) mm[d] = screen / (PI_REF * (double)K[d]) * hoc;
+  int lim[3];
+  for (int j = 0; j < 3; ++j) lim[j] = (int)(cell[3 * j] * mm[0] + cell[3 * j + 1] * mm[1] + cell[3 * j + 2] * mm[2]);
+  if (lim[0] < 0 || lim[1] < 0 || lim[2] < 0) return false;
+  auto fold = [](int r, int Kd) { return (r <= Kd / 2) ? (double)r : (double)(r - Kd); };
+  auto sinc2p = [p](double t) { return (t == 0.0) ? 1.0 : ipow_h(std::sin(t) / t, 2 * p); };
+  auto poly = [&b, p](double x) {
+    double a = 0.0;
+    for (int l = p - 1; l >= 0; --l) a = b[l] + a * x;
+    return a;
+  };
+  for (int rc3 = 0; rc3 < K3; ++rc3) {
+    const double n3 = fold(rc3, K3), s3 = ipow_h(std::sin(PI_REF * n3 / (double)K3), 2);
+    for (int rc2 = 0; rc2 < K2; ++rc2) {
+      const double n2 = fold(rc2, K2), s2 = ipow_h(std::sin(PI_REF * n2 / (double)K2), 2);
+      for (int rc1 = 0; rc1 < H; ++rc1) {
+        if (rc1 == 0 && rc2 == 0 && rc3 == 0) continue;
+        const double n1 = fold(rc1, K1), s1 = ipow_h(std::sin(PI_REF * n1 / (double)K1), 2);
+        const double nv[3] = {n1, n2, n3};
+        double kv[3];
+        for (int i = 0; i < 3; ++i)
+          kv[i] = 2.0 * PI_REF * (invcl[3 * i] * nv[0] + invcl[3 * i + 1] * nv[1] + invcl[3 * i + 2] * nv[2]);
+        const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+        if (!(k2 > 0.0)) continue;
+        double acc = 0.0;
+        for (int mx = -lim[0]; mx <= lim[0]; ++mx) {
+          const double u1 = sinc2p(PI_REF * (n1 / (double)K1 + (double)mx));
+          for (int my = -lim[1]; my <= lim[1]; ++my) {
+            const double u2 = sinc2p(PI_REF * (n2 / (double)K2 + (double)my));
+            for (int mz = -lim[2]; mz <= lim[2]; ++mz) {
+              const double u3 = sinc2p(PI_REF * (n3 / (double)K3 + (double)mz));
+              const double shift[3] = {(double)(mx * K1), (double)(my * K2), (double)(mz * K2)};
+              double km[3], g[3];
+              for (int i = 0; i < 3; ++i) {
+                km[i] = kv[i] + 2.0 * PI_REF * (invcl[3 * i] * shift[0] + invcl[3 * i + 1] * shift[1] +
+                                                invcl[3 * i + 2] * shift[2]);
+                g[i] = std::exp(-0.25 * ipow_h(km[i] / screen, 2));
+              }
+              const double dot = kv[0] * km[0] + kv[1] * km[1] + kv[2] * km[2];
+              const double kmm = km[0] * km[0] + km[1] * km[1] + km[2] * km[2];
+              acc = acc + dot * (g[0] * g[1] * g[2]) * (u1 * u2 * u3) / kmm;
+            }
+          }
+        }
+        const double den = ipow_h(poly(s1) * poly(s2) * poly(s3), 2);
+        engfac[((size_t)rc3 * K2 + rc2) * H + rc1] = acc * 4.0 * PI_REF / (k2 * den) / volume;
+      }
+    }
+  }
+  engfac[0] = 0.0;
+  return true;
+}
+
 // get_inverted_cell (engproc.F90:898-919), column-major
 void inverted_cell(const double *c, double *inv, double *vol) {
 #define M(i, j) c[3 * ((j)-1) + (i)-1]
@@ -540,6 +626,18 @@ template <typename T> int choose_slab(erm_ctx *c) {
   return ERM_OK;
 }
 
+// sorted index of every tagpt molecule (for soln-mode solutes); waits for
+// the frame's preparation
+int mol_index(erm_ctx *c) {
+  const int S = c->slvmax;
+  std::vector<int> molk(S);
+  HIPCHK(c, hipMemcpyAsync(molk.data(), c->d_mol_k.p, sizeof(int) * S, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  c->jof_k.assign(S, -1);
+  for (int j = 0; j < S; ++j) c->jof_k[molk[j]] = j;
+  return ERM_OK;
+}
+
 template <typename T> int frame_prep(erm_ctx *c) {
   Params &k = c->kp;
   const int ncell = k.nb[0] * k.nb[1] * k.nb[2];
@@ -591,9 +689,10 @@ template <typename T> int frame_prep(erm_ctx *c) {
                      dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sg), dp<T>(c->d_sw), dp<double>(c->d_mol_q),
                      dp<int>(c->d_mol_spec), dp<int>(c->d_mol_m));
   TRY(launch_check(c, "k_site_fill"));
+  c->nsites = nsites;
+  if (c->P.cltype == ERM_EL_COULOMB) return mol_index(c);
   // sites sorted by PME grid base g3 for the fused slab kernel
   const int K3 = k.K3;
-  c->nsites = nsites;
   TRY(ensure(c, c->d_zcount, sizeof(int) * (K3 + 1)));
   TRY(ensure(c, c->d_zstart, sizeof(int) * (K3 + 1)));
   TRY(ensure(c, c->d_zfill, sizeof(int) * (K3 + 1)));
@@ -614,12 +713,7 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(launch_check(c, "k_zscatter"));
   c->zst.assign(K3 + 1, 0);
   HIPCHK(c, hipMemcpyAsync(c->zst.data(), c->d_zstart.p, sizeof(int) * (K3 + 1), hipMemcpyDeviceToHost, c->st));
-  // sorted index of every tagpt molecule (for soln-mode solutes)
-  std::vector<int> molk(S);
-  HIPCHK(c, hipMemcpyAsync(molk.data(), c->d_mol_k.p, sizeof(int) * S, hipMemcpyDeviceToHost, c->st));
-  TRY(sync(c));
-  c->jof_k.assign(S, -1);
-  for (int j = 0; j < S; ++j) c->jof_k[molk[j]] = j;
+  TRY(mol_index(c));
   return choose_slab<T>(c);
 }
 
@@ -679,6 +773,10 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
 // flags select the terms.  Stage events ev[0..6].
 template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   using V = typename C2<T>::type;
+  // bare Coulomb: no reciprocal space, and realcal_self and residual_ene
+  // return 0 (realcal.F90:294, engproc.F90:865), so uvengy(0) = 0
+  const bool bare = (c->P.cltype == ERM_EL_COULOMB);
+  if (bare) flags &= ~(DO_RECIP | DO_SELF | DO_RESIDUAL);
   Params k = c->kp;
   const int n = bt.n, ns = c->ns, S = c->slvmax;
   const int K1 = k.K1, K2 = k.K2, K3 = k.K3, H = k.H, p = k.p;
@@ -774,21 +872,40 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
       TRY(launch_check(c, "k_gather"));
     }
   } else {
-    HIPCHK(c, hipMemsetAsync(c->d_uv.p, 0, sizeof(double) * (size_t)n * (S + 1), c->st));
+    const size_t nuv = (size_t)n * (S + 1);
+    hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((nuv + NTHR - 1) / NTHR)), dim3(NTHR), 0, c->st,
+                       dp<double>(c->d_uv), nuv, 0.0);
+    TRY(launch_check(c, "k_fill"));
     TRY(mark(c, 2));
     TRY(mark(c, 3));
   }
   TRY(mark(c, 4));
-  if (flags & DO_REAL) {
+  if ((flags & DO_REAL) && bare && c->P.boxshp == ERM_SYS_NONPERIODIC) {
+    size_t lo = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * c->nvt) + 64;
+    TRY(set_lds(c, (const void *)k_pair_open<T>, lo, "k_pair_open"));
+    hipLaunchKernelGGL(k_pair_open<T>, dim3(n, (S + NTHR - 1) / NTHR), dim3(NTHR), lo, c->st, k,
+                       dp<double>(c->d_spos), dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj),
+                       dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                       dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                       dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+    TRY(launch_check(c, "k_pair_open"));
+  } else if (flags & DO_REAL) {
     double Rg = k.rcmax + 8.0;
     double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
     int nsplit = std::max(1, std::min(16, (int)std::ceil(frac * S / (2.0 * NTHR))));
     size_t lpair = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * c->nvt) + sizeof(int) * (2 * MAXSEG + 2) + 64;
-    hipLaunchKernelGGL(k_pair<T>, dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
-                       dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
-                       dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
-                       dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
-                       dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+    if (bare)
+      hipLaunchKernelGGL((k_pair<T, true>), dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
+                         dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
+                         dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                         dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+    else
+      hipLaunchKernelGGL((k_pair<T, false>), dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
+                         dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
+                         dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                         dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
     TRY(launch_check(c, "k_pair"));
   }
   TRY(mark(c, 5));
@@ -1051,6 +1168,20 @@ template <typename T> int frame_prep_any(erm_ctx *c) { return frame_prep<T>(c); 
 
 int set_frame_common(erm_ctx *c, const double *cell, double w) {
   TRY(need(c, false, false));
+  if (c->P.boxshp == ERM_SYS_NONPERIODIC) {
+    // no box: the cell is not read; coordinates stay raw and every
+    // molecule falls in one cell of unit length
+    memset(c->cell, 0, sizeof c->cell);
+    memset(c->invcl, 0, sizeof c->invcl);
+    c->volume = 0.0;
+    for (int d = 0; d < 3; ++d) c->L[d] = 1.0;
+    c->stat_weight_system = w;
+    ++c->frame_serial;
+    frame_params(c);
+    TRY(upload(c, c->d_cellm, c->cell, 9));
+    return ERM_OK;
+  }
+  if (!cell) return fail(c, ERM_E_ARG, "cell is NULL");
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
       if (i != j && std::fabs(cell[3 * j + i]) > 1e-8)
@@ -1064,7 +1195,12 @@ int set_frame_common(erm_ctx *c, const double *cell, double w) {
   // the cutoff must stay below half the box (single minimum image)
   bool changed = !c->have_prevcl;
   for (int i = 0; i < 9 && !changed; ++i) changed = (c->prevcl[i] != cell[i]);
-  if (changed) {
+  if (changed && c->P.cltype == ERM_EL_PPPM) {
+    std::vector<double> engfac;
+    if (!pppm_greenfunc(c->P.ms, k.p, c->cell, c->invcl, c->volume, c->P.screen, engfac))
+      return fail(c, ERM_E_ARG, "PPPM: negative alias range for this cell");
+    TRY(upload_real(c, c->d_engfac, engfac));
+  } else if (changed && c->P.cltype == ERM_EL_PME) {
     std::vector<double> splfc[3];
     double coeff[MAXP + 1];
     spline_coeff(k.p, coeff);
@@ -1285,16 +1421,29 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
   if (P.ljswitch < 0 || P.ljswitch > 3) return ERM_E_ARG;
   if (P.precision != ERM_FP64 && P.precision != ERM_FP32) return ERM_E_ARG;
   if (P.pme_path < 0 || P.pme_path > 3) return ERM_E_ARG;
+  // cltype / boxshp (setconf.F90:326-339, 368-372; insertion.F90:155)
+  if (P.cltype == 1) return ERM_E_UNSUPPORTED;
+  if (P.cltype != ERM_EL_COULOMB && P.cltype != ERM_EL_PME && P.cltype != ERM_EL_PPPM) return ERM_E_ARG;
+  if (P.boxshp != ERM_SYS_NONPERIODIC && P.boxshp != ERM_SYS_PERIODIC) return ERM_E_ARG;
+  if (P.boxshp == ERM_SYS_NONPERIODIC && (P.cltype != ERM_EL_COULOMB || P.estype == 2 || P.slttype != 1))
+    return ERM_E_ARG;
+  if (P.cltype == ERM_EL_PPPM && P.splodr % 2 == 1) return ERM_E_ARG;
+  const bool bare = (P.cltype == ERM_EL_COULOMB);
+  // bare Coulomb has no grid: a small placeholder keeps the buffers sized
+  int msv[3];
+  for (int d = 0; d < 3; ++d) msv[d] = bare ? 16 : P.ms[d];
   FftAxis ax[3];
   for (int d = 0; d < 3; ++d)
-    if (P.ms[d] < 2 || P.ms[d] > 512 || P.ms[d] < P.splodr || !fft_axis(P.ms[d], ax[d])) return ERM_E_UNSUPPORTED;
-  if (!(P.screen > 0) || !(P.elecut > 0) || !(P.upljcut >= P.lwljcut) || !(P.block_threshold > 0)) return ERM_E_ARG;
+    if (msv[d] < 2 || msv[d] > 512 || msv[d] < P.splodr || !fft_axis(msv[d], ax[d])) return ERM_E_UNSUPPORTED;
+  if ((!bare && !(P.screen > 0)) || !(P.elecut > 0) || !(P.upljcut >= P.lwljcut) || !(P.block_threshold > 0))
+    return ERM_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ERM_E_DEVICE;
   if (P.device < 0 || P.device >= ndev) return ERM_E_ARG;
   if (hipSetDevice(P.device) != hipSuccess) return ERM_E_DEVICE;
   erm_ctx *c = new erm_ctx();
   c->P = P;
+  for (int d = 0; d < 3; ++d) c->P.ms[d] = msv[d];
   c->fp32 = (P.precision == ERM_FP32);
   c->Bmax = P.max_batch > 0 ? P.max_batch : 256;
   {
@@ -1317,9 +1466,12 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
     }
   Params &k = c->kp;
   memset(&k, 0, sizeof k);
-  k.K1 = P.ms[0];
-  k.K2 = P.ms[1];
-  k.K3 = P.ms[2];
+  k.K1 = msv[0];
+  k.K2 = msv[1];
+  k.K3 = msv[2];
+  k.cltype = P.cltype;
+  k.boxshp = P.boxshp;
+  k.elecut = P.elecut;
   k.H = k.K1 / 2 + 1;
   k.p = P.splodr;
   k.ax1 = ax[0];
@@ -1561,7 +1713,7 @@ int32_t erm_engclear(erm_ctx *c) {
 }
 
 static int set_frame_impl(erm_ctx *c, const double *pos, bool device_ptr, const double *cell, double w) {
-  if (!c || !pos || !cell) return ERM_E_ARG;
+  if (!c || !pos) return ERM_E_ARG;
   if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
   TRY(set_frame_common(c, cell, w));
   HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * c->natom,
@@ -1991,6 +2143,7 @@ int32_t erm_solute_gradient(erm_ctx *c, int32_t n, const int32_t *solutes, const
   if (cmbrule != 0 && cmbrule != 1) return fail(c, ERM_E_ARG, "cmbrule must be 0 or 1");
   if (c->P.wgtslf != 0 || c->P.estype != 1)
     return fail(c, ERM_E_UNSUPPORTED, "the gradient needs parameter-independent weights (wgtslf 0, estype 1)");
+  if (c->P.cltype == ERM_EL_COULOMB) return fail(c, ERM_E_UNSUPPORTED, "the gradient covers PME and PPPM");
   const int ns = c->ns;
   if (ns > 1024) return fail(c, ERM_E_UNSUPPORTED, "gradient for solutes of more than 1024 sites");
   const Params &k = c->kp;

@@ -34,6 +34,9 @@ struct Params {
   int slvmax, ns, nvt, ermax, numslv, esmax;
   int slttype, wgtslf, estype, selfcal, force;
   double temp, stat_weight_system, total_charge, solute_charge;
+  // electrostatics (0 bare Coulomb, 2 PME, 3 PPPM) and boundary (0 none)
+  int cltype, boxshp;
+  double elecut;
 };
 
 // --------------------------------------------------------------------------
@@ -63,7 +66,7 @@ __global__ void k_mol_cell(Params P, const double *__restrict__ pos, const int *
   int a0 = mol_begin[m], a1 = mol_begin[m + 1];
   double w[3];
   for (int d = 0; d < 3; ++d) w[d] = wrap1(pos[3 * a0 + d], P.L[d]);
-  int c = cell_index(P, w);
+  int c = P.boxshp ? cell_index(P, w) : 0;
   cellid[k] = c;
   atomicAdd(&count[c], 1);
   double rm = 0.0;
@@ -71,7 +74,7 @@ __global__ void k_mol_cell(Params P, const double *__restrict__ pos, const int *
     double s2 = 0.0;
     for (int d = 0; d < 3; ++d) {
       double x = pos[3 * a + d] - pos[3 * a0 + d];
-      x -= P.L[d] * rint(x / P.L[d]);
+      if (P.boxshp) x -= P.L[d] * rint(x / P.L[d]);
       s2 += x * x;
     }
     rm = fmax(rm, sqrt(s2));
@@ -139,11 +142,13 @@ __global__ void k_site_fill(Params P, const double *__restrict__ pos, const int 
   const int Ks[3] = {P.K1, P.K2, P.K3};
   for (int a = mol_begin[m]; a < mol_begin[m + 1]; ++a, ++t) {
     const double *x = pos + 3 * a;
-    sx[t] = wrap1(x[0], P.L[0]);
-    sy[t] = wrap1(x[1], P.L[1]);
-    sz[t] = wrap1(x[2], P.L[2]);
+    // raw coordinates without a box (realcal_prepare, realcal.F90:137-146)
+    sx[t] = P.boxshp ? wrap1(x[0], P.L[0]) : x[0];
+    sy[t] = P.boxshp ? wrap1(x[1], P.L[1]) : x[1];
+    sz[t] = P.boxshp ? wrap1(x[2], P.L[2]) : x[2];
     sq[t] = (T)charge[a];
     sct[t] = cvt[a];
+    if (P.cltype == 0) continue;  // no reciprocal space
     int g[3];
     for (int d = 0; d < 3; ++d) {
       double f = P.invcl[d] * x[0] + P.invcl[3 + d] * x[1] + P.invcl[6 + d] * x[2];
@@ -303,6 +308,7 @@ __global__ __launch_bounds__(128) void k_solute_prep(Params P, const double *__r
     sref[4 * b + 2] = cpt[2];
     sref[4 * b + 3] = r;
   }
+  if (P.cltype == 0) return;  // bare Coulomb: no reciprocal space
   // splines, in chunks of 64 sites through LDS
   const int Ks[3] = {P.K1, P.K2, P.K3};
   for (int s0c = 0; s0c < ns; s0c += 64) {
@@ -997,6 +1003,33 @@ template <> __device__ __forceinline__ double coul_t<double>(double qq, double r
   return qq * (1.0 - erf(alpha * r)) / r;
 }
 
+// LJ of one pair as realcal_bare writes it (realcal.F90:213-266): zero
+// beyond upljcut, the potential switches applied as a factor beyond
+// lwljcut, the force switches branching on the distance rst
+template <typename T>
+__device__ __forceinline__ T lj_bare_t(const Params &P, T eps, T sig2, T dis2, T rst) {
+  if (rst > (T)P.upljcut) return (T)0;
+  if (P.ljswitch >= 2)
+    return (rst <= (T)P.lwljcut) ? lj_low_t<T>(P, eps, sig2, dis2) : lj_switch_t<T>(P, eps, sig2, dis2);
+  T invr2 = sig2 / dis2;
+  T invr6 = invr2 * invr2 * invr2;
+  T eplj = (T)4 * eps * invr6 * (invr6 - (T)1);
+  if (rst > (T)P.lwljcut) {
+    T swth;
+    if (P.ljswitch == 0) {
+      T up2 = (T)P.up2, lw2 = (T)P.lw2;
+      T d = dis2 - up2, w = up2 - lw2;
+      swth = ((T)2 * dis2 + up2 - (T)3 * lw2) * (d * d) / (w * w * w);
+    } else {
+      T sw = (rst - (T)P.lwljcut) / (T)(P.upljcut - P.lwljcut);
+      T sw3 = sw * sw * sw;
+      swth = (T)1 - (T)10 * sw3 + (T)15 * (sw3 * sw) - (T)6 * (sw3 * sw * sw);
+    }
+    eplj = swth * eplj;
+  }
+  return eplj;
+}
+
 // inclusive prefix sum of a[0..n) in LDS, n <= 8 * blockDim.x; all threads call
 __device__ void block_inclusive_scan(int *a, int n) {
   __shared__ int part[NTHR];
@@ -1102,7 +1135,9 @@ __device__ int pillar_segments(const Params &P, const double Cw[3], double R, co
   return nsg;
 }
 
-template <typename T>
+// BARE: bare Coulomb q q / r below elecut and the LJ of realcal_bare
+// (realcal.F90:155-281) in place of the screened real-space pair
+template <typename T, bool BARE = false>
 __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restrict__ spos,
                                                const double *__restrict__ sref,
                                                const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
@@ -1200,14 +1235,27 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
               const double r2d = bx * bx + by * by + bz * bz;
               const double e64 = slt_lj64[(s * nvt + cv) * 2], s64 = slt_lj64[(s * nvt + cv) * 2 + 1];
               double epd = 0.0;
-              if (r2d <= P.lw2)
-                epd = lj_low_t<double>(P, e64, s64, r2d);
-              else if (r2d <= P.up2)
-                epd = lj_switch_t<double>(P, e64, s64, r2d);
-              if (r2d <= P.el2) epd += coul_t<double>(slt_q64[s] * (double)qv, sqrt(r2d), P.screen);
+              if constexpr (BARE) {
+                const double rd = sqrt(r2d);
+                epd = lj_bare_t<double>(P, e64, s64, r2d, rd);
+                if (rd < P.elecut) epd += slt_q64[s] * (double)qv / rd;
+              } else {
+                if (r2d <= P.lw2)
+                  epd = lj_low_t<double>(P, e64, s64, r2d);
+                else if (r2d <= P.up2)
+                  epd = lj_switch_t<double>(P, e64, s64, r2d);
+                if (r2d <= P.el2) epd += coul_t<double>(slt_q64[s] * (double)qv, sqrt(r2d), P.screen);
+              }
               e += epd;
               continue;
             }
+          }
+          if constexpr (BARE) {
+            const T rst = sqrt(r2);
+            ep = lj_bare_t<T>(P, eps, sig2, r2, rst);
+            if (rst < (T)P.elecut) ep += uq[s] * qv / rst;
+            e += (double)ep;
+            continue;
           }
           if (eps > (T)0) {
             if (r2 <= lw2)
@@ -1222,6 +1270,78 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
     }
     u[j] += e;
   }
+}
+
+// realcal_bare without a box (realcal.F90:155-281 with boxshp = 0): the
+// solute with every site of every physical molecule, raw coordinates, no
+// Coulomb cutoff, LJ up to upljcut.  Grid (configuration, molecule blocks),
+// one molecule per thread.  Offsets as hi + lo pairs and the FP64 overlap
+// recomputation as in k_pair.
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_pair_open(Params P, const double *__restrict__ spos,
+                                                    const double *__restrict__ sref,
+                                                    const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
+                                                    const int *__restrict__ mol_first,
+                                                    const int *__restrict__ mol_ns, const double *__restrict__ sx,
+                                                    const double *__restrict__ sy, const double *__restrict__ sz,
+                                                    const T *__restrict__ sq, const int *__restrict__ sct,
+                                                    const int *__restrict__ slt_j,
+                                                    const double *__restrict__ slt_lj64,
+                                                    const double *__restrict__ slt_q64, double *uv) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int ns = P.ns, nvt = P.nvt;
+  T *ux = reinterpret_cast<T *>(lds_raw);
+  T *uy = ux + ns;
+  T *uz = uy + ns;
+  T *uq = uz + ns;
+  T *ulj = uq + ns;             // [ns][nvt][2] (eps, sig2)
+  T *ulo = ulj + 2 * ns * nvt;  // [ns][3] low parts of the offsets
+  const int b = blockIdx.x;
+  const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
+  const double *xs = spos + (size_t)b * ns * 3;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    ux[s] = (T)(xs[3 * s] - C0);
+    uy[s] = (T)(xs[3 * s + 1] - C1);
+    uz[s] = (T)(xs[3 * s + 2] - C2v);
+    uq[s] = slt_q[s];
+    ulo[3 * s] = (T)((xs[3 * s] - C0) - (double)ux[s]);
+    ulo[3 * s + 1] = (T)((xs[3 * s + 1] - C1) - (double)uy[s]);
+    ulo[3 * s + 2] = (T)((xs[3 * s + 2] - C2v) - (double)uz[s]);
+  }
+  for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
+  __syncthreads();
+  const int j = blockIdx.y * blockDim.x + threadIdx.x;
+  if (j >= P.slvmax || j == slt_j[b]) return;
+  double e = 0.0;
+  const int t0 = mol_first[j], nsm = mol_ns[j];
+  for (int t = t0; t < t0 + nsm; ++t) {
+    const double dx = sx[t] - C0, dy = sy[t] - C1, dz = sz[t] - C2v;
+    const T vx = (T)dx, vy = (T)dy, vz = (T)dz;
+    const T lx = (T)(dx - (double)vx), ly = (T)(dy - (double)vy), lz = (T)(dz - (double)vz);
+    const T qv = sq[t];
+    const int cv = sct[t];
+    for (int s = 0; s < ns; ++s) {
+      const T ax = ((vx - ux[s]) + lx) - ulo[3 * s];
+      const T ay = ((vy - uy[s]) + ly) - ulo[3 * s + 1];
+      const T az = ((vz - uz[s]) + lz) - ulo[3 * s + 2];
+      const T r2 = ax * ax + ay * ay + az * az;
+      const T eps = ulj[(s * nvt + cv) * 2], sig2 = ulj[(s * nvt + cv) * 2 + 1];
+      if constexpr (sizeof(T) == 4) {
+        if (eps > (T)0 && sig2 > (T)1.2599211 * r2) {
+          const double bx = dx - ((double)ux[s] + (double)ulo[3 * s]);
+          const double by = dy - ((double)uy[s] + (double)ulo[3 * s + 1]);
+          const double bz = dz - ((double)uz[s] + (double)ulo[3 * s + 2]);
+          const double r2d = bx * bx + by * by + bz * bz, rd = sqrt(r2d);
+          e += lj_bare_t<double>(P, slt_lj64[(s * nvt + cv) * 2], slt_lj64[(s * nvt + cv) * 2 + 1], r2d, rd) +
+               slt_q64[s] * (double)qv / rd;
+          continue;
+        }
+      }
+      const T rst = sqrt(r2);
+      e += (double)(lj_bare_t<T>(P, eps, sig2, r2, rst) + uq[s] * qv / rst);
+    }
+  }
+  uv[(size_t)b * (P.slvmax + 1) + 1 + j] += e;
 }
 
 // realcal_self (realcal.F90:284-328) of configuration b, FP64
@@ -1383,9 +1503,11 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
       double tw = P.stat_weight_system * (wsol ? wsol[b] : 1.0);
       if (P.estype == 2) {
         tw = tw * P.volume;
-        double factor = 3.1415926535897932 * (P.total_charge * P.total_charge) / (P.screen * P.screen) /
-                        P.volume / 2.0;
-        tw = tw * exp(factor / P.temp);
+        if (P.cltype != 0) {
+          double factor = 3.1415926535897932 * (P.total_charge * P.total_charge) / (P.screen * P.screen) /
+                          P.volume / 2.0;
+          tw = tw * exp(factor / P.temp);
+        }
       }
       engnmfc = engnmfc * tw;
       int bin = -1;

@@ -20,6 +20,8 @@ module ermgpu
      integer(c_int32_t) :: corrcal, selfcal, wgtslf, force_calculation
      real(c_double) :: elecut, lwljcut, upljcut, screen, temp, block_threshold
      integer(c_int32_t) :: precision, max_batch, device, pme_path
+     ! engmain's cltype (0 bare Coulomb, 2 PME, 3 PPPM) and boxshp
+     integer(c_int32_t) :: cltype, boxshp
   end type erm_params
 
   type, bind(C) :: erm_hist

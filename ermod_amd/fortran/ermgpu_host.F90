@@ -6,7 +6,8 @@
 !
 ! usage: ermgpu_host <input stream file> <output directory>
 ! The input holds, in order and without padding: natom nmol ntype nframes
-! nins ndiv (int32); the erm_params fields; the erm_hist fields of species 1
+! nins ndiv (int32); the erm_params fields except pme_path (cltype and
+! boxshp last); the erm_hist fields of species 1
 ! (also used for the self-energy coordinate when selfcal = 1);
 ! mol_begin(nmol+1) charge(natom) ljtype(natom) lj_eps(ntype^2)
 ! lj_sig2(ntype^2) sluvid(nmol) uvspec(nmol) mol_charge(nmol)
@@ -37,7 +38,7 @@ program ermgpu_host
   read(10) natom, nmol, ntype, nframes, nins, ndiv
   read(10) prm%slttype, prm%estype, prm%ljswitch, prm%splodr, prm%ms, prm%corrcal, prm%selfcal, &
        prm%wgtslf, prm%force_calculation, prm%elecut, prm%lwljcut, prm%upljcut, prm%screen, &
-       prm%temp, prm%block_threshold, prm%precision, prm%max_batch, prm%device
+       prm%temp, prm%block_threshold, prm%precision, prm%max_batch, prm%device, prm%cltype, prm%boxshp
   prm%pme_path = 0
   read(10) hst%ecdmin, hst%ecfmns, hst%ecdcen, hst%eccore, hst%ecdmax, hst%eclbin, hst%ecfbin, &
        hst%ec0bin, hst%finfac, hst%pecore

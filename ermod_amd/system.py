@@ -26,6 +26,9 @@ SGMCNV = 1.7817974362806784
 
 SLT_SOLN, SLT_REFS_RIGID, SLT_REFS_FLEX = 1, 2, 3
 ES_NVT, ES_NPT = 1, 2
+# cltype and boxshp (engmain.F90:306,314); Ewald (1) is refused by the reference
+EL_COULOMB, EL_PME, EL_PPPM = 0, 2, 3
+SYS_NONPERIODIC, SYS_PERIODIC = 0, 1
 PT_SOLVENT, PT_SOLUTE, PT_TEST_RIGID, PT_TEST_FLEX = 0, 1, 2, 3
 
 # getmass (setconf.F90:999-1053)
@@ -70,6 +73,8 @@ class ErmSystem:
     screen: float = 0.35
     temp: float = 300.0 * GAS_CONST / CAL_PER_J
     block_threshold: float = 4.0
+    cltype: int = EL_PME           # EL_COULOMB (bare), EL_PME or EL_PPPM
+    boxshp: int = SYS_PERIODIC     # SYS_NONPERIODIC with bare Coulomb only
     # topology
     mol_begin: np.ndarray = None      # int32 nmol+1
     charge: np.ndarray = None         # f64 natom, scaled

@@ -70,7 +70,17 @@ typedef struct erm_params {
   int32_t pme_path;       /* reciprocal-space kernels: 0 auto, 1 fused slab with
                              LDS FFTs, 2 separate plane and gather kernels,
                              3 register-column FFTs + fused x/gather slab */
+  int32_t cltype;         /* electrostatics, the reference's numbering
+                             (engmain.F90:314): ERM_EL_COULOMB bare 1/r
+                             (realcal_bare), ERM_EL_PME, ERM_EL_PPPM; Ewald (1)
+                             is refused as by setconf.F90:329-331 */
+  int32_t boxshp;         /* ERM_SYS_PERIODIC, or ERM_SYS_NONPERIODIC (bare
+                             Coulomb, estype 1 and slttype 1 only,
+                             setconf.F90:368-372, insertion.F90:155) */
 } erm_params;
+
+enum { ERM_EL_COULOMB = 0, ERM_EL_PME = 2, ERM_EL_PPPM = 3 };
+enum { ERM_SYS_NONPERIODIC = 0, ERM_SYS_PERIODIC = 1 };
 
 /* &hist namelist of one species (engproc.F90:66-67) */
 typedef struct erm_hist {

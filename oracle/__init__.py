@@ -37,7 +37,7 @@ class OrcSystem(C.Structure):
                 ("mol_begin", _i32p), ("charge", _f64p), ("ljtype", _i32p), ("lj_eps", _f64p),
                 ("lj_sig2", _f64p), ("sluvid", _i32p), ("uvspec", _i32p), ("mol_charge", _f64p),
                 ("numslv", C.c_int32), ("uvmax", _i32p), ("uvsoft", _i32p), ("uvcrd", _f64p),
-                ("esmax", C.c_int32), ("escrd", _f64p)]
+                ("esmax", C.c_int32), ("escrd", _f64p), ("cltype", C.c_int32), ("boxshp", C.c_int32)]
 
 
 class OrcAcc(C.Structure):
@@ -75,6 +75,9 @@ def lib():
         L.orc_spline_axis.argtypes = [C.c_int32, C.c_int32, _f64p]
         L.orc_greenfunc.argtypes = [_i32p, C.c_int32, _f64p, C.c_double, C.c_double, _f64p, _f64p,
                                     _f64p, _f64p]
+        L.orc_gfb_pppm.argtypes = [C.c_int32, _f64p]
+        L.orc_pppm_greenfunc.argtypes = [_i32p, C.c_int32, _f64p, _f64p, C.c_double, C.c_double, _f64p]
+        L.orc_realcal_bare.argtypes = [vp, C.c_int32, _f64p, _f64p]
         L.orc_spline_site.argtypes = [_f64p, _f64p, _i32p, C.c_int32, _f64p, _f64p, _i32p]
         L.orc_place_solute.argtypes = [C.c_int32, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.orc_rng_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
@@ -133,6 +136,25 @@ def spline_coeff(order):
 def spline_value(order, x):
     c = spline_coeff(order)
     return np.array([lib().orc_spline_value(P(c), order, float(v)) for v in np.atleast_1d(x)])
+
+
+def gfb_pppm(order):
+    """calc_gfb_pppm (recpcal.F90:168-186): gf_b(1..order)."""
+    out = np.zeros(order)
+    lib().orc_gfb_pppm(order, P(out))
+    return out
+
+
+def pppm_greenfunc(K, order, cell, screen):
+    """recpcal_pppm_greenfunc on the (K3, K2, K1/2+1) half spectrum."""
+    cell = np.asarray(cell, dtype=np.float64)
+    inv = np.linalg.inv(cell)
+    out = np.zeros((K[2], K[1], K[0] // 2 + 1))
+    invcm = f64(inv.T.copy())
+    cm = f64(cell.T.copy())
+    check(lib().orc_pppm_greenfunc(P(i32(K), _i32p), order, P(cm), P(invcm), float(np.linalg.det(cell)),
+                                   float(screen), P(out)), "pppm_greenfunc")
+    return out
 
 
 def hist_struct(h) -> OrcHist:
@@ -237,6 +259,7 @@ class Oracle:
         st.uvcrd = P(self.uvcrd)
         st.esmax = self.esmax
         st.escrd = P(self.escrd)
+        st.cltype, st.boxshp = s.cltype, s.boxshp
         self._st = st
         status = C.c_int32()
         self.h = lib().orc_create(C.byref(st), C.byref(status))
@@ -271,6 +294,12 @@ class Oracle:
         sp = None if solute_pos is None else f64(solute_pos)
         fn = lib().orc_realcal_brute if brute else lib().orc_realcal_proc
         check(fn(self.h, int(solute), P(sp), P(out)), "realcal")
+        return out
+
+    def realcal_bare(self, solute, solute_pos=None):
+        out = np.zeros(self.slvmax)
+        sp = None if solute_pos is None else f64(solute_pos)
+        check(lib().orc_realcal_bare(self.h, int(solute), P(sp), P(out)), "realcal_bare")
         return out
 
     def realcal_self(self, solute, solute_pos=None):

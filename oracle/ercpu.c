@@ -475,6 +475,112 @@ void orc_greenfunc(const int32_t *K, int32_t p, const double *invcl, double volu
   engfac[0] = 0.0;
 }
 
+/* calc_gfb_pppm (recpcal.F90:168-186): the coefficients of
+ * sum_l gf_b(l) x^(l-1), x = sin^2(k h / 2), the alias sum of the squared
+ * charge-assignment function; the inner loop's index is 0 after it ends */
+void orc_gfb_pppm(int32_t p, double *gf_b) {
+  for (int i = 1; i < p; ++i) gf_b[i] = 0.0;
+  gf_b[0] = 1.0;
+  for (int m = 1; m <= p - 1; ++m) {
+    int l;
+    for (l = m; l >= 1; --l)
+      gf_b[l] = 4.0 * (gf_b[l] * (double)(l - m) * ((double)(l - m) - 0.5) -
+                       gf_b[l - 1] * (double)(l - m - 1) * (double)(l - m - 1));
+    gf_b[0] = 4.0 * (gf_b[0] * (double)(l - m) * ((double)(l - m) - 0.5));
+  }
+  double fact = 1.0;
+  for (int i = 1; i <= 2 * p - 1; ++i) fact *= (double)i;
+  double gaminv = 1.0 / fact;
+  for (int m = 0; m < p; ++m) gf_b[m] = gf_b[m] * gaminv;
+}
+
+/* gf_denom (recpcal.F90:189-202) */
+static double gf_denom(const double *gf_b, int p, double x, double y, double z) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int l = p - 1; l >= 0; --l) {
+    s0 = gf_b[l] + s0 * x;
+    s1 = gf_b[l] + s1 * y;
+    s2 = gf_b[l] + s2 * z;
+  }
+  double pr = s0 * s1 * s2;
+  return pr * pr;
+}
+
+/* recpcal_pppm_greenfunc (recpcal.F90:205-325).  The z alias offset is
+ * mz * K2, as the reference writes it (recpcal.F90:293); it equals mz * K3
+ * for K2 = K3. */
+int32_t orc_pppm_greenfunc(const int32_t *K, int32_t p, const double *cell, const double *invcl, double volume,
+                           double screen, double *engfac) {
+  int K1 = K[0], K2 = K[1], K3 = K[2], H = K1 / 2 + 1;
+  if (p % 2 == 1) return ORC_E_ARG;
+  double gf_b[32];
+  orc_gfb_pppm(p, gf_b);
+  const double EPS_HOC = 0.0000001;
+  double m_max[3];
+  for (int d = 0; d < 3; ++d) m_max[d] = screen / (PI_REF * (double)K[d]) * pow(-log(EPS_HOC), 0.25);
+  int mmax[3];
+  for (int j = 0; j < 3; ++j)
+    mmax[j] = (int)(cell[3 * j] * m_max[0] + cell[3 * j + 1] * m_max[1] + cell[3 * j + 2] * m_max[2]);
+  if (mmax[0] < 0 || mmax[1] < 0 || mmax[2] < 0) return ORC_E_ARG;
+  for (int rc3 = 0; rc3 < K3; ++rc3) {
+    double inm3 = (rc3 <= K3 / 2) ? (double)rc3 : (double)(rc3 - K3);
+    double s3 = sin(PI_REF * inm3 / (double)K3);
+    s3 = s3 * s3;
+    for (int rc2 = 0; rc2 < K2; ++rc2) {
+      double inm2 = (rc2 <= K2 / 2) ? (double)rc2 : (double)(rc2 - K2);
+      double s2 = sin(PI_REF * inm2 / (double)K2);
+      s2 = s2 * s2;
+      for (int rc1 = 0; rc1 < H; ++rc1) {
+        long idx = ((long)rc3 * K2 + rc2) * H + rc1;
+        if (rc1 == 0 && rc2 == 0 && rc3 == 0) {
+          engfac[idx] = 0.0;
+          continue;
+        }
+        double inm1 = (rc1 <= K1 / 2) ? (double)rc1 : (double)(rc1 - K1);
+        double s1 = sin(PI_REF * inm1 / (double)K1);
+        s1 = s1 * s1;
+        const double inm[3] = {inm1, inm2, inm3};
+        double k[3];
+        for (int i = 0; i < 3; ++i)
+          k[i] = 2.0 * PI_REF * (invcl[3 * i] * inm[0] + invcl[3 * i + 1] * inm[1] + invcl[3 * i + 2] * inm[2]);
+        double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        double factor = 0.0;
+        if (k2 > 0.0) {
+          double sum = 0.0;
+          for (int mx = -mmax[0]; mx <= mmax[0]; ++mx) {
+            double t1 = PI_REF * (inm1 / (double)K1 + (double)mx);
+            double u1 = (t1 == 0.0) ? 1.0 : ipow(sin(t1) / t1, 2 * p);
+            for (int my = -mmax[1]; my <= mmax[1]; ++my) {
+              double t2 = PI_REF * (inm2 / (double)K2 + (double)my);
+              double u2 = (t2 == 0.0) ? 1.0 : ipow(sin(t2) / t2, 2 * p);
+              for (int mz = -mmax[2]; mz <= mmax[2]; ++mz) {
+                double t3 = PI_REF * (inm3 / (double)K3 + (double)mz);
+                double u3 = (t3 == 0.0) ? 1.0 : ipow(sin(t3) / t3, 2 * p);
+                const double mN[3] = {(double)(mx * K1), (double)(my * K2), (double)(mz * K2)};
+                double km[3], gam = 1.0;
+                for (int i = 0; i < 3; ++i) {
+                  km[i] = k[i] + 2.0 * PI_REF * (invcl[3 * i] * mN[0] + invcl[3 * i + 1] * mN[1] +
+                                                 invcl[3 * i + 2] * mN[2]);
+                }
+                double g[3];
+                for (int i = 0; i < 3; ++i) g[i] = exp(-0.25 * ipow(km[i] / screen, 2));
+                gam = g[0] * g[1] * g[2];
+                double kkm = k[0] * km[0] + k[1] * km[1] + k[2] * km[2];
+                double kmkm = km[0] * km[0] + km[1] * km[1] + km[2] * km[2];
+                sum = sum + kkm * gam * (u1 * u2 * u3) / kmkm;
+              }
+            }
+          }
+          factor = sum * 4.0 * PI_REF / (k2 * gf_denom(gf_b, p, s1, s2, s3)) / volume;
+        }
+        engfac[idx] = factor;
+      }
+    }
+  }
+  engfac[0] = 0.0;
+  return ORC_OK;
+}
+
 void orc_spline_site(const double *x, const double *invcl, const int32_t *K, int32_t p, const double *coeff,
                      double *w, int32_t *g) {
   double inm[3];
@@ -637,6 +743,12 @@ orc_ctx *orc_create(const orc_system *sys, int32_t *status) {
     *status = ORC_E_ARG;
     return NULL;
   }
+  /* cltype / boxshp consistency (setconf.F90:326-339, 368-372) */
+  if ((sys->cltype != 0 && sys->cltype != 2 && sys->cltype != 3) || (sys->boxshp != 0 && sys->boxshp != 1) ||
+      (sys->boxshp == 0 && (sys->cltype != 0 || sys->estype == 2)) || (sys->cltype == 3 && sys->splodr % 2 == 1)) {
+    *status = ORC_E_ARG;
+    return NULL;
+  }
   orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
   c->s = *sys;
   int na = sys->natom, nm = sys->nmol, nt = sys->ntype;
@@ -757,22 +869,33 @@ static double wrap1(double x, double L) { return x - L * floor(x * (1.0 / L)); }
 /* per-frame part of engconst (engproc.F90:352-383) plus the cuboid
  * normalisation of realcal_prepare (realcal.F90:137-146, 789-885) */
 int32_t orc_set_frame(orc_ctx *c, const double *sitepos, const double *cell, double stat_weight_system) {
+  c->stat_weight_system = stat_weight_system;
+  if (c->s.boxshp == 0) {
+    /* non-periodic: raw coordinates, no cell (realcal.F90:137-146) */
+    memcpy(c->pos, sitepos, sizeof(double) * 3 * c->s.natom);
+    memcpy(c->posn, sitepos, sizeof(double) * 3 * c->s.natom);
+    return ORC_OK;
+  }
   for (int i = 1; i <= 3; ++i)
     for (int j = 1; j <= 3; ++j)
       if (i != j && fabs(CM(cell, i, j)) > 1e-8) return ORC_E_NONCUBOID;
   memcpy(c->cell, cell, sizeof(double) * 9);
   orc_inverted_cell(c->cell, c->invcl, &c->volume);
   for (int i = 0; i < 3; ++i) c->L[i] = fabs(CM(cell, i + 1, i + 1));
-  c->stat_weight_system = stat_weight_system;
   int changed = !c->have_cell;
   for (int i = 0; i < 9 && !changed; ++i) changed = (c->prevcl[i] != cell[i]);
-  if (changed)
+  if (changed && c->s.cltype == 2)
     orc_greenfunc(c->K, c->p, c->invcl, c->volume, c->s.screen, c->splfc[0], c->splfc[1], c->splfc[2], c->engfac);
+  if (changed && c->s.cltype == 3) {
+    int st = orc_pppm_greenfunc(c->K, c->p, c->cell, c->invcl, c->volume, c->s.screen, c->engfac);
+    if (st) return st;
+  }
   memcpy(c->prevcl, cell, sizeof(double) * 9);
   c->have_cell = 1;
   memcpy(c->pos, sitepos, sizeof(double) * 3 * c->s.natom);
   for (int a = 0; a < c->s.natom; ++a)
     for (int i = 0; i < 3; ++i) c->posn[3 * a + i] = wrap1(c->pos[3 * a + i], c->L[i]);
+  if (c->s.cltype == 0) return ORC_OK;
   /* recpcal_prepare_solvent for every physical molecule */
   for (int k = 0; k < c->slvmax; ++k) {
     int m = c->tagpt[k];
@@ -794,7 +917,7 @@ static int solute_coords(orc_ctx *c, int solute, const double *solute_pos, int *
     for (int i = 0; i < 3; ++i) {
       double x = solute_pos ? solute_pos[3 * j + i] : c->pos[3 * (b + j) + i];
       c->spos[3 * j + i] = x;
-      c->sposn[3 * j + i] = wrap1(x, c->L[i]);
+      c->sposn[3 * j + i] = c->s.boxshp ? wrap1(x, c->L[i]) : x;
     }
   return ORC_OK;
 }
@@ -1197,6 +1320,93 @@ int32_t orc_realcal_brute(orc_ctx *c, int32_t solute, const double *solute_pos, 
   return ORC_OK;
 }
 
+/* LJ of one pair as realcal_bare writes it (realcal.F90:213-266): cut
+ * beyond upljcut, switched beyond lwljcut, comparisons on the distance */
+static double lj_bare(const orc_system *s, const ljconst *k, double eps, double sig2, double dis2, double rst) {
+  if (rst > s->upljcut) return 0.0;
+  double invr2 = sig2 / dis2;
+  double invr6 = invr2 * invr2 * invr2;
+  switch (s->ljswitch) {
+    case 0:
+    case 1: {
+      double eplj = 4.0 * eps * invr6 * (invr6 - 1.0);
+      if (rst > s->lwljcut) {
+        double swth;
+        if (s->ljswitch == 0) {
+          double lw2 = ipow(s->lwljcut, 2), up2 = ipow(s->upljcut, 2);
+          swth = (2.0 * dis2 + up2 - 3.0 * lw2) * ipow(dis2 - up2, 2) / ipow(up2 - lw2, 3);
+        } else {
+          double swfac = (rst - s->lwljcut) / (s->upljcut - s->lwljcut);
+          swth = 1.0 - 10.0 * ipow(swfac, 3) + 15.0 * ipow(swfac, 4) - 6.0 * ipow(swfac, 5);
+        }
+        eplj = swth * eplj;
+      }
+      return eplj;
+    }
+    case 2: {
+      double sg6 = sig2 * sig2 * sig2, vdwa, vdwb;
+      if (rst <= s->lwljcut) {
+        vdwa = invr6 * invr6 - sg6 * sg6 / (k->lw6 * k->up6);
+        vdwb = invr6 - sg6 / (k->lw3 * k->up3);
+      } else {
+        double invr3 = sqrt(invr6), sg3 = sqrt(sg6);
+        vdwa = k->up6 / (k->up6 - k->lw6) * ipow(invr6 - sg6 / k->up6, 2);
+        vdwb = k->up3 / (k->up3 - k->lw3) * ipow(invr3 - sg3 / k->up3, 2);
+      }
+      return 4.0 * eps * (vdwa - vdwb);
+    }
+    default: {
+      double sg6 = sig2 * sig2 * sig2, vdwa, vdwb;
+      if (rst <= s->lwljcut) {
+        vdwa = invr6 * invr6 - sg6 * sg6 * k->repC;
+        vdwb = invr6 - sg6 * k->attC;
+      } else {
+        double swfac = rst - s->lwljcut;
+        vdwa = invr6 * invr6 - sg6 * sg6 * (k->repA * ipow(swfac, 3) + k->repB * ipow(swfac, 4) + k->repC);
+        vdwb = invr6 - sg6 * (k->attA * ipow(swfac, 3) + k->attB * ipow(swfac, 4) + k->attC);
+      }
+      return 4.0 * eps * (vdwa - vdwb);
+    }
+  }
+}
+
+/* realcal_bare (realcal.F90:155-281) for every physical molecule but the
+ * solute: sites of both molecules in the normalised cell, cuboid minimum
+ * image and Coulomb cutoff elecut when periodic; raw coordinates and no
+ * Coulomb cutoff otherwise */
+int32_t orc_realcal_bare(orc_ctx *c, int32_t solute, const double *solute_pos, double *ereal) {
+  int ns, st = solute_coords(c, solute, solute_pos, &ns);
+  if (st) return st;
+  const orc_system *s = &c->s;
+  ljconst k;
+  make_ljconst(s, &k);
+  const double reelcut = s->boxshp ? s->elecut : 1.0e50;
+  const double half[3] = {0.5 * c->L[0], 0.5 * c->L[1], 0.5 * c->L[2]};
+  int b = c->mol_begin[solute], nt = s->ntype;
+  for (int kk = 0; kk < c->slvmax; ++kk) {
+    int m = c->tagpt[kk];
+    double pairep = 0.0;
+    if (m != solute) {
+      for (int j = 0; j < ns; ++j)
+        for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a) {
+          double dis2 = 0.0;
+          for (int i = 0; i < 3; ++i) {
+            double x = c->sposn[3 * j + i] - c->posn[3 * a + i];
+            if (s->boxshp) x = half[i] - fabs(half[i] - fabs(x));
+            dis2 = dis2 + x * x;
+          }
+          double rst = sqrt(dis2);
+          int tu = c->ljtype[b + j], tv = c->ljtype[a];
+          double eplj = lj_bare(s, &k, c->lj_eps[tv * nt + tu], c->lj_sig2[tv * nt + tu], dis2, rst);
+          double epcl = (rst >= reelcut) ? 0.0 : c->charge[b + j] * c->charge[a] / rst;
+          pairep = pairep + eplj + epcl;
+        }
+    }
+    ereal[kk] = pairep;
+  }
+  return ORC_OK;
+}
+
 /* realcal_self (realcal.F90:284-328) */
 int32_t orc_realcal_self(orc_ctx *c, int32_t solute, const double *solute_pos, double *pairep) {
   int ns, st = solute_coords(c, solute, solute_pos, &ns);
@@ -1314,11 +1524,17 @@ static double residual(const orc_ctx *c, int i, int j) {
   return -epcl;
 }
 
-/* get_uv_energy (engproc.F90:662-756), PME branch */
+/* get_uv_energy (engproc.F90:662-756): the PME / PPPM branch, or with bare
+ * Coulomb realcal_bare per molecule and a zero self energy (realcal_self
+ * and residual_ene return 0, realcal.F90:294, engproc.F90:865) */
 int32_t orc_uv_energy(orc_ctx *c, int32_t solute, const double *solute_pos, double *uvengy) {
   int ns, st = solute_coords(c, solute, solute_pos, &ns);
   if (st) return st;
   int n = c->slvmax;
+  if (c->s.cltype == 0) {
+    uvengy[0] = 0.0;
+    return orc_realcal_bare(c, solute, solute_pos, uvengy + 1);
+  }
   double *ereal = (double *)malloc(sizeof(double) * (n ? n : 1));
   double *erecp = (double *)malloc(sizeof(double) * (n ? n : 1));
   double self_recp;
@@ -1383,6 +1599,7 @@ static int getiduv(orc_ctx *c, int pti, double v, int *iduv) {
 /* volcorrect (engproc.F90:871-887) */
 static double volcorrect(const orc_ctx *c, double weight) {
   weight = weight * c->volume;
+  if (c->s.cltype == 0) return weight;
   double tq = 0.0;
   for (int i = 0; i < c->s.nmol; ++i)
     if (c->sluvid[i] == 0 || c->sluvid[i] == 1) tq = tq + c->mol_charge[i];
@@ -1552,6 +1769,7 @@ int32_t orc_solute_gradient(orc_ctx *c, int32_t solute, const double *solute_pos
   int ns, st = solute_coords(c, solute, solute_pos, &ns);
   if (st) return st;
   if (cmbrule != 0 && cmbrule != 1) return ORC_E_ARG;
+  if (c->s.cltype != 2 && c->s.cltype != 3) return ORC_E_UNSUPPORTED;
   const orc_system *s = &c->s;
   const int nt = s->ntype, b0 = c->mol_begin[solute];
   double *q = (double *)malloc(sizeof(double) * 3 * ns), *ep = q + ns, *sg = q + 2 * ns;

@@ -73,6 +73,9 @@ typedef struct orc_system {
   const double *uvcrd;        /* ermax = sum(uvmax) */
   int32_t esmax;
   const double *escrd;        /* esmax (may be NULL when selfcal = 0) */
+  /* electrostatics and boundary (engmain.F90:306,314) */
+  int32_t cltype;             /* 0 bare Coulomb, 2 PME, 3 PPPM (1 Ewald: refused) */
+  int32_t boxshp;             /* 0 non-periodic (bare Coulomb only), 1 periodic */
 } orc_system;
 
 typedef struct orc_ctx orc_ctx;
@@ -108,6 +111,13 @@ void orc_spline_axis(int32_t K, int32_t p, double *splfc);
 void orc_greenfunc(const int32_t *K, int32_t p, const double *invcl, double volume,
                    double screen, const double *splfc1, const double *splfc2,
                    const double *splfc3, double *engfac /* (K1/2+1)*K2*K3 */);
+/* PPPM (recpcal.F90:153-325): the denominator coefficients gf_b (p of
+ * them, calc_gfb_pppm) and the optimal influence function on the
+ * (K1/2+1, K2, K3) half spectrum.  Returns ORC_E_ARG for an odd order or a
+ * negative alias range. */
+void orc_gfb_pppm(int32_t p, double *gf_b);
+int32_t orc_pppm_greenfunc(const int32_t *K, int32_t p, const double *cell, const double *invcl,
+                           double volume, double screen, double *engfac);
 void orc_spline_site(const double *x, const double *invcl, const int32_t *K,
                      int32_t p, const double *coeff, double *w /* p*3 */,
                      int32_t *g /* 3 */);
@@ -138,6 +148,11 @@ int32_t orc_realcal_proc(orc_ctx *c, int32_t solute, const double *solute_pos,
                          double *ereal /* slvmax */);
 int32_t orc_realcal_brute(orc_ctx *c, int32_t solute, const double *solute_pos,
                           double *ereal /* slvmax */);
+/* realcal_bare (realcal.F90:155-281): bare Coulomb + LJ of the solute with
+ * every physical molecule (cutoff elecut under the minimum image when
+ * periodic, none otherwise) */
+int32_t orc_realcal_bare(orc_ctx *c, int32_t solute, const double *solute_pos,
+                         double *ereal /* slvmax */);
 int32_t orc_realcal_self(orc_ctx *c, int32_t solute, const double *solute_pos,
                          double *pairep);
 int32_t orc_recpcal(orc_ctx *c, int32_t solute, const double *solute_pos,

@@ -61,5 +61,28 @@ def test_create_without_device_fails_loudly():
     p.slttype, p.splodr = 2, 4
     p.ms[:] = [16, 16, 16]
     p.elecut, p.lwljcut, p.upljcut, p.screen, p.block_threshold = 6.0, 5.0, 6.0, 0.5, 3.0
+    p.cltype, p.boxshp = 2, 1
     h = C.c_void_p()
     assert A.lib().erm_create(C.byref(p), C.byref(h)) == 12
+
+
+def test_create_checks_electrostatics_before_the_device():
+    """cltype / boxshp consistency as setconf.F90 enforces it: Ewald (1)
+    is unsupported, a non-periodic box needs bare Coulomb, soln mode and
+    NVT, PPPM an even spline order."""
+    def params(**kw):
+        p = A.ErmParams()
+        p.slttype, p.splodr, p.estype = 1, 4, 1
+        p.ms[:] = [16, 16, 16]
+        p.elecut, p.lwljcut, p.upljcut, p.screen, p.block_threshold = 6.0, 5.0, 6.0, 0.5, 3.0
+        p.cltype, p.boxshp = 2, 1
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+    h = C.c_void_p()
+    assert A.lib().erm_create(C.byref(params(cltype=1)), C.byref(h)) == 11
+    assert A.lib().erm_create(C.byref(params(cltype=5)), C.byref(h)) == 10
+    assert A.lib().erm_create(C.byref(params(boxshp=0)), C.byref(h)) == 10
+    assert A.lib().erm_create(C.byref(params(boxshp=0, cltype=0, slttype=2)), C.byref(h)) == 10
+    assert A.lib().erm_create(C.byref(params(boxshp=0, cltype=0, estype=2)), C.byref(h)) == 10
+    assert A.lib().erm_create(C.byref(params(cltype=3, splodr=5)), C.byref(h)) == 10

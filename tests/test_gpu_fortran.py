@@ -26,9 +26,9 @@ def write_input(path, sy, frames, nins, seed, ndiv=1, s=None):
     h = s.hists[0]
     with open(path, "wb") as f:
         f.write(struct.pack("<6i", s.natom, s.nmol, s.ntype, len(frames), nins, ndiv))
-        f.write(struct.pack("<4i3i4i6d3i", s.slttype, s.estype, s.ljswitch, s.splodr, *s.ms, s.corrcal,
+        f.write(struct.pack("<4i3i4i6d5i", s.slttype, s.estype, s.ljswitch, s.splodr, *s.ms, s.corrcal,
                             s.selfcal, s.wgtslf, s.force_calculation, s.elecut, s.lwljcut, s.upljcut,
-                            s.screen, s.temp, s.block_threshold, 0, 16, 0))
+                            s.screen, s.temp, s.block_threshold, 0, 16, 0, s.cltype, s.boxshp))
         f.write(struct.pack("<9di", h.ecdmin, h.ecfmns, h.ecdcen, h.eccore, h.ecdmax, h.eclbin, h.ecfbin,
                             h.ec0bin, h.finfac, h.pecore))
         for a, t in ((s.mol_begin, np.int32), (s.charge, np.float64), (s.ljtype, np.int32),
