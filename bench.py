@@ -18,6 +18,7 @@ of the same workload on this host.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -171,6 +172,7 @@ def main():
     ap.add_argument("--group-planes", type=int, default=0, help="fixed planes per x group (with --batch)")
     ap.add_argument("--slab-threads", type=int, default=0, help="fixed x-slab workgroup size (with --batch)")
     ap.add_argument("--column-threads", type=int, default=0, help="fixed z-column workgroup size (with --batch)")
+    ap.add_argument("--no-variants", action="store_true", help="skip the PPPM and bare-Coulomb passes")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the heterogeneous pass (four solute sizes in a mixed solvent, concurrently)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -197,8 +199,8 @@ def main():
     dframes = [torch.from_numpy(f.reshape(-1)).to(dev) for f in frames]
     torch.cuda.synchronize()
 
-    def run(prec, steps, warmup, graph=False):
-        g = ErmGPU(s, prec, max_batch=args.batch)
+    def run(prec, steps, warmup, graph=False, system=None):
+        g = ErmGPU(system or s, prec, max_batch=args.batch)
         tuning = None
         g.set_frame_device(dframes[0].data_ptr(), sy.cell)
         if not args.no_tune and args.batch == 0:
@@ -295,6 +297,22 @@ def main():
                        "stage_ms": {k: round(v, 3) for k, v in st32.items()}}
     if not args.no_mixed and args.precision == "fp64":
         out["heterogeneous"] = mixed_pass(torch, dev)
+    if not args.no_variants and args.precision == "fp64":
+        # the same workload with PPPM and with bare Coulomb (site cutoff at
+        # elecut; its cut molecules reach lower energies, hence the wider bins)
+        from ermod_amd.system import EL_COULOMB, EL_PPPM
+        var = {}
+        nst = max(1, args.steps // 4)
+        for name, cl in (("pppm", EL_PPPM), ("bare_coulomb", EL_COULOMB)):
+            sv = dataclasses.replace(s, cltype=cl)
+            if cl == EL_COULOMB:
+                sv = dataclasses.replace(sv, hists=[dataclasses.replace(h, ecdmin=-60.0) for h in s.hists])
+            dtv, stv, _, tunv, _, _ = run(args.precision, nst, 1, system=sv)
+            nv = nst * args.insertions
+            var[name] = {"value": nv * slvmax / dtv, "insertions_per_s": nv / dtv, "batch": tunv["max_batch"],
+                         "pme_path": tunv["pme_path"] if cl != EL_COULOMB else None,
+                         "stage_ms": {k: round(v, 3) for k, v in stv.items()}}
+        out["electrostatics"] = var
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sy, frames[0], 100000, args.cpu_seconds, args.seed)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

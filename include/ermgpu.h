@@ -76,7 +76,9 @@ typedef struct erm_params {
                              is refused as by setconf.F90:329-331 */
   int32_t boxshp;         /* ERM_SYS_PERIODIC, or ERM_SYS_NONPERIODIC (bare
                              Coulomb, estype 1 and slttype 1 only,
-                             setconf.F90:368-372, insertion.F90:155) */
+                             setconf.F90:368-372, insertion.F90:155); the
+                             cell is then not read.  Zero in both fields
+                             means bare Coulomb without a box */
 } erm_params;
 
 enum { ERM_EL_COULOMB = 0, ERM_EL_PME = 2, ERM_EL_PPPM = 3 };
@@ -131,11 +133,12 @@ int32_t erm_engclear(erm_ctx *ctx);
 
 /* One frame: cell inverse and volume (get_inverted_cell, engproc.F90:898-919),
  * Green's function when the cell changed (recpcal_spline_greenfunc,
- * recpcal.F90:111-152), solvent B-spline weights (recpcal_prepare_solvent,
- * recpcal.F90:328-340), coordinate normalisation (realcal_prepare,
- * realcal.F90:137-146) and the cell list (set_block_info / blockify /
- * sort_block, realcal.F90:373-525), built once per frame on the device.
- * sitepos: natom*3 host doubles (test-particle slots ignored). */
+ * recpcal.F90:111-152, or recpcal_pppm_greenfunc, recpcal.F90:205-325),
+ * solvent B-spline weights (recpcal_prepare_solvent, recpcal.F90:328-340),
+ * coordinate normalisation (realcal_prepare, realcal.F90:137-146) and the
+ * cell list (set_block_info / blockify / sort_block, realcal.F90:373-525),
+ * built once per frame on the device.  sitepos: natom*3 host doubles
+ * (test-particle slots ignored); cell may be NULL without a box. */
 int32_t erm_set_frame(erm_ctx *ctx, const double *sitepos, const double *cell,
                       double stat_weight_system);
 /* Same, with sitepos already in device memory (natom*3 doubles). */
