@@ -56,7 +56,7 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_get_uv_energy", "erm_realcal_proc", "erm_realcal_self", "erm_recpcal", "erm_engconst",
            "erm_insert_random", "erm_update_histogram", "erm_engstore", "erm_fetch", "erm_sizes",
            "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune",
-           "erm_insert_random_multi")
+           "erm_insert_random_multi", "erm_set_insertion")
 
 _lib = None
 
@@ -102,6 +102,8 @@ def lib():
                                               C.POINTER(C.c_uint64)]
         L.erm_solute_gradient.argtypes = [vp, C.c_int32, _i32p, _f64p, C.c_uint64, C.c_uint64, _f64p,
                                           C.c_int32, _f64p, _f64p, _f64p, _f64p]
+        L.erm_set_insertion.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                        _i32p]
         _lib = L
     return _lib
 
@@ -350,6 +352,16 @@ class ErmGPU:
                                               C.c_uint64(first_index), _p(w), int(cmbrule), _p(qq),
                                               _p(ee), _p(ss), _p(acc)), "solute_gradient")
         return acc[0], acc[1], acc[2:2 + ns], acc[2 + ns:2 + 2 * ns], acc[2 + 2 * ns:]
+
+    def set_insertion(self, insorigin, insposition, insorient=0, lwreg=0.0, upreg=0.0, hostspec=()):
+        """Placement of device-generated poses (instslt, insertion.F90:103-366):
+        the reference's insorigin / insposition / insorient numbering, region
+        bounds and the host species (uvspec numbering) of the aggregate
+        centre."""
+        hs = _i32(list(hostspec)) if len(hostspec) else None
+        self._check(lib().erm_set_insertion(self.h, int(insorigin), int(insposition), int(insorient),
+                                            float(lwreg), float(upreg), 0 if hs is None else len(hs),
+                                            _p(hs, _i32p)), "set_insertion")
 
     def set_tuning(self, max_batch=0, pme_path="auto", use_graph=False, slab_planes=0, group_planes=0,
                    slab_threads=0, column_threads=0):

@@ -326,6 +326,9 @@ struct erm_ctx {
   int zcol_T = NTHR;            // threads per z-column workgroup, 256 or 512 (autotuned)
   std::vector<int> zst;         // first z-sorted site of every plane (host copy)
   size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
+  // insertion modes: host molecules of the aggregate centre, and the centre
+  std::vector<int> hostmol;
+  DBuf d_hostmol, d_aggcen;
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
       d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
   DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
@@ -738,10 +741,12 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
   const int n = bt.n, ns = c->ns;
   if (bt.random) {
     if (!c->args_ready || !c->d_args.p) return fail(c, ERM_E_ENG_BUG, "random poses without generator arguments");
+    if (k.ins_org == ERM_INSORG_AGGCEN && !c->d_aggcen.p)
+      return fail(c, ERM_E_ENG_BUG, "aggregate centre missing");
     hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
                        (uint64_t)bt.first, dp<double>(c->d_bf),
                        dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template], dp<double>(c->d_cellm),
-                       dp<double>(c->d_spos));
+                       dp<double>(c->d_aggcen), dp<double>(c->d_spos));
     TRY(launch_check(c, "k_pose"));
   } else if (bt.host_pos) {
     HIPCHK(c, hipMemcpyAsync(c->d_spos.p, bt.host_pos, sizeof(double) * (size_t)n * ns * 3, hipMemcpyHostToDevice,
@@ -1712,16 +1717,73 @@ int32_t erm_engclear(erm_ctx *c) {
   return sync(c);
 }
 
+// aggregate centre of the frame's host molecules into d_aggcen
+static int aggregate_center(erm_ctx *c) {
+  TRY(ensure(c, c->d_aggcen, 3 * sizeof(double)));
+  hipLaunchKernelGGL(k_aggcen, dim3(1), dim3(1), 0, c->st, c->nmol, dp<int>(c->d_mol_begin), dp<int>(c->d_hostmol),
+                     dp<double>(c->d_sitemass), dp<double>(c->d_pos), dp<double>(c->d_aggcen));
+  return launch_check(c, "k_aggcen");
+}
+
 static int set_frame_impl(erm_ctx *c, const double *pos, bool device_ptr, const double *cell, double w) {
   if (!c || !pos) return ERM_E_ARG;
   if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
   TRY(set_frame_common(c, cell, w));
   HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * c->natom,
                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+  if (c->kp.ins_org == ERM_INSORG_AGGCEN) TRY(aggregate_center(c));
   int s = c->fp32 ? frame_prep<float>(c) : frame_prep<double>(c);
   if (s) return s;
   c->have_frame = true;
   return ERM_OK;
+}
+
+int32_t erm_set_insertion(erm_ctx *c, int32_t insorigin, int32_t insposition, int32_t insorient, double lwreg,
+                          double upreg, int32_t nhost, const int32_t *hostspec) {
+  TRY(need(c, false, false));
+  if (c->P.slttype == 1) return fail(c, ERM_E_ARG, "insertion modes apply to a test particle (slttype 2 or 3)");
+  if (insorigin == ERM_INSORG_REFSTR || insposition == ERM_INSPOS_RMSD || insposition == ERM_INSPOS_GAUSS)
+    return fail(c, ERM_E_UNSUPPORTED, "reference-structure insertion (RefInfo fit) is not implemented");
+  if (insposition == ERM_INSPOS_NOCHANGE)
+    return fail(c, ERM_E_UNSUPPORTED, "fixed positions: pass the coordinates to erm_engconst");
+  // pairing of origin and position (setconf.F90:399-428)
+  const bool agg = (insposition == ERM_INSPOS_SPHERE || insposition == ERM_INSPOS_SLAB_GENERIC ||
+                    insposition == ERM_INSPOS_SLAB_SYMMETRIC);
+  if (insposition == ERM_INSPOS_RANDOM ? insorigin != ERM_INSORG_ORIGIN : (!agg || insorigin != ERM_INSORG_AGGCEN))
+    return fail(c, ERM_E_ARG, "insorigin %d does not go with insposition %d (set_ins)", insorigin, insposition);
+  if (insorient != ERM_INSROT_RANDOM && insorient != ERM_INSROT_NOCHANGE)
+    return fail(c, ERM_E_ARG, "insorient %d", insorient);
+  // region bounds (set_reg)
+  if (insposition == ERM_INSPOS_SPHERE && (lwreg < 0.0 || upreg < 0.0 || lwreg > upreg))
+    return fail(c, ERM_E_ARG, "sphere needs 0 <= lwreg <= upreg (set_reg)");
+  if ((insposition == ERM_INSPOS_SLAB_GENERIC || insposition == ERM_INSPOS_SLAB_SYMMETRIC) && lwreg > upreg)
+    return fail(c, ERM_E_ARG, "slab needs lwreg <= upreg (set_reg)");
+  if (insposition == ERM_INSPOS_SLAB_SYMMETRIC && (lwreg < 0.0 || upreg < 0.0))
+    return fail(c, ERM_E_ARG, "symmetric slab needs lwreg, upreg >= 0 (set_reg)");
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (agg) {
+    // host molecules: every molecule of the listed species (com_aggregate)
+    if (nhost <= 0 || !hostspec) return fail(c, ERM_E_ARG, "the aggregate centre needs host species");
+    c->hostmol.assign(c->nmol, 0);
+    double mtot = 0.0;
+    for (int m = 0; m < c->nmol; ++m) {
+      if (c->sluvid[m] > 1) continue;
+      for (int h = 0; h < nhost; ++h)
+        if (c->uvspec[m] == hostspec[h] && hostspec[h] > 0) c->hostmol[m] = 1;
+      if (c->hostmol[m])
+        for (int a = c->mol_begin[m]; a < c->mol_begin[m + 1]; ++a) mtot += c->sitemass[a];
+    }
+    if (!(mtot > 0.0)) return fail(c, ERM_E_ARG, "host species without mass (bst_zrw)");
+    TRY(upload(c, c->d_hostmol, c->hostmol.data(), c->nmol));
+  }
+  Params &k = c->kp;
+  k.ins_org = insorigin;
+  k.ins_pos = insposition;
+  k.ins_rot = insorient;
+  k.lwreg = lwreg;
+  k.upreg = upreg;
+  if (agg && c->have_frame) TRY(aggregate_center(c));
+  return sync(c);
 }
 
 int32_t erm_set_frame(erm_ctx *c, const double *sitepos, const double *cell, double w) {

@@ -37,6 +37,10 @@ struct Params {
   // electrostatics (0 bare Coulomb, 2 PME, 3 PPPM) and boundary (0 none)
   int cltype, boxshp;
   double elecut;
+  // test-particle placement: insorigin, insposition, insorient and the
+  // region bounds lwreg / upreg (engmain.F90:320-327)
+  int ins_org, ins_pos, ins_rot;
+  double lwreg, upreg;
 };
 
 // --------------------------------------------------------------------------
@@ -199,59 +203,132 @@ __device__ void qprod_dev(const double *a, const double *b, double *p) {
   for (int i = 0; i < 3; ++i) p[1 + i] = cr[i] + a[0] * b[1 + i] + b[0] * a[1 + i];
 }
 
-// random test-particle pose (insertion.F90:133-163, 325-366) placed with the
-// reference's operation sequence (set_solute_origin, set_shift_com,
-// apply_orientation; bestfit.F90:49-100, 174-203)
+// com_aggregate (insertion.F90:539-566): centre of mass of every site of
+// the host molecules, summed in molecule order by a single thread so that the
+// rounding is that of the sequential sums (center_of_mass, bestfit.F90:158-172)
+__global__ void k_aggcen(int nmol, const int *__restrict__ mol_begin, const int *__restrict__ hostmol,
+                         const double *__restrict__ mass, const double *__restrict__ pos, double *c) {
+#pragma clang fp contract(off)
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double msum = 0.0;
+  for (int m = 0; m < nmol; ++m)
+    if (hostmol[m])
+      for (int a = mol_begin[m]; a < mol_begin[m + 1]; ++a) msum = msum + mass[a];
+  for (int i = 0; i < 3; ++i) {
+    double d = 0.0;
+    for (int m = 0; m < nmol; ++m)
+      if (hostmol[m])
+        for (int a = mol_begin[m]; a < mol_begin[m + 1]; ++a) d = d + pos[3 * a + i] * mass[a];
+    c[i] = d / msum;
+  }
+}
+
+// second generator stream, for the position draws of the sphere and slab
+// modes (the random-position draws 0-2 and the orientation draws 3 + 4a + i
+// stay in the first)
+constexpr uint64_t POS_STREAM = 0xD1B54A32D192ED03ULL;
+
+// test-particle pose (instslt, insertion.F90:43-101) placed with the
+// reference's operation sequence: set_solute_origin (ORIGIN / NOCHANGE /
+// AGGCEN, insertion.F90:103-131), set_shift_com (RANDOM / SPHERE / SLAB
+// generic and symmetric, insertion.F90:133-199) and apply_orientation
+// (RANDOM / NOCHANGE, insertion.F90:325-366; bestfit.F90:49-100, 174-203).
 // args = {seed, first index of the call}; insertion b of this launch is
 // number args[1] + offset + b of the generator stream (device-side arguments,
-// so a captured graph replays with new seeds)
+// so a captured graph replays with new seeds).  aggcen: the aggregate centre
+// of the frame (AGGCEN only).
 __global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
                        const double *__restrict__ bf, const double *__restrict__ mass,
-                       const double *__restrict__ cellm, double *spos) {
+                       const double *__restrict__ cellm, const double *__restrict__ aggcen, double *spos) {
 #pragma clang fp contract(off)
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int ns = P.ns;
-  const uint64_t seed = args[0];
+  const uint64_t seed = args[0], sp = seed ^ POS_STREAM;
   uint64_t index = args[1] + offset + (uint64_t)b;
-  double r[3], q[4] = {1.0, 0.0, 0.0, 0.0};
-  for (int i = 0; i < 3; ++i) r[i] = rng_uniform(seed, index, (uint32_t)i);
-  for (int a = 0; a < 63; ++a) {
-    double t[4];
-    for (int i = 0; i < 4; ++i) t[i] = rng_uniform(seed, index, (uint32_t)(3 + 4 * a + i)) * 2.0 - 1.0;
-    double s = t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + t[3] * t[3];
-    if (s < 1.0) {
-      double nrm = sqrt(s);
-      for (int i = 0; i < 4; ++i) q[i] = t[i] / nrm;
-      break;
-    }
-  }
   double *out = spos + (size_t)b * ns * 3;
   for (int i = 0; i < 3 * ns; ++i) out[i] = bf[i];
   double c[3];
-  com_dev(ns, out, mass, c);
-  for (int j = 0; j < ns; ++j)
-    for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + 0.0;
-  double com[3];
-  for (int i = 0; i < 3; ++i) com[i] = cellm[i] * r[0] + cellm[3 + i] * r[1] + cellm[6 + i] * r[2];
-  com_dev(ns, out, mass, c);
-  for (int j = 0; j < ns; ++j)
-    for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + com[i];
-  com_dev(ns, out, mass, c);
-  double cj[4] = {q[0], -q[1], -q[2], -q[3]};
-  for (int j = 0; j < ns; ++j) {
-    double q2[4] = {0.0, out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]};
-    double t[4], r2[4];
-    qprod_dev(q, q2, t);
-    qprod_dev(t, cj, r2);
-    for (int i = 0; i < 3; ++i) out[3 * j + i] = r2[1 + i] + c[i];
+  if (P.ins_org != 1) {
+    // COM to the origin or to the aggregate centre (set_solute_com)
+    const double o[3] = {P.ins_org == 2 ? aggcen[0] : 0.0, P.ins_org == 2 ? aggcen[1] : 0.0,
+                         P.ins_org == 2 ? aggcen[2] : 0.0};
+    com_dev(ns, out, mass, c);
+    for (int j = 0; j < ns; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + o[i];
+  }
+  if (P.ins_pos == 0) {
+    // fully random position in the cell: COM = cell . r
+    double r[3], com[3];
+    for (int i = 0; i < 3; ++i) r[i] = rng_uniform(seed, index, (uint32_t)i);
+    for (int i = 0; i < 3; ++i) com[i] = cellm[i] * r[0] + cellm[3 + i] * r[1] + cellm[6 + i] * r[2];
+    com_dev(ns, out, mass, c);
+    for (int j = 0; j < ns; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + com[i];
+  } else if (P.ins_pos == 2) {
+    // shell lwreg <= |shift| <= upreg, uniform in volume, direction by
+    // rejection from the cube
+    const double lw3 = P.lwreg * P.lwreg * P.lwreg, up3 = P.upreg * P.upreg * P.upreg;
+    double t = rng_uniform(sp, index, 0);
+    t = pow(t * (up3 - lw3) + lw3, 1.0 / 3.0);
+    double r[3] = {0.0, 0.0, 1.0}, nrm = 1.0;
+    for (int a = 0; a < 84; ++a) {
+      double u[3];
+      for (int i = 0; i < 3; ++i) u[i] = rng_uniform(sp, index, (uint32_t)(1 + 3 * a + i)) * 2.0 - 1.0;
+      double s2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+      if (0.0 < s2 && s2 <= 1.0) {
+        for (int i = 0; i < 3; ++i) r[i] = u[i];
+        nrm = s2;
+        break;
+      }
+    }
+    nrm = sqrt(nrm);
+    for (int j = 0; j < ns; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + t * r[i] / nrm;
+  } else if (P.ins_pos == 3 || P.ins_pos == 4) {
+    // slab: random in-plane position, lwreg <= z offset <= upreg (either
+    // sign with equal probability for the symmetric slab)
+    double r[3];
+    r[0] = rng_uniform(sp, index, 0);
+    r[1] = rng_uniform(sp, index, 1);
+    double t = rng_uniform(sp, index, 2);
+    t = t * (P.upreg - P.lwreg) + P.lwreg;
+    if (P.ins_pos == 4) {
+      double dir = rng_uniform(sp, index, 3);
+      if (dir > 0.5) t = -t;
+    }
+    const double len3 = sqrt(cellm[6] * cellm[6] + cellm[7] * cellm[7] + cellm[8] * cellm[8]);
+    r[2] = t / len3;
+    double com[3];
+    for (int i = 0; i < 3; ++i) com[i] = cellm[i] * r[0] + cellm[3 + i] * r[1] + cellm[6 + i] * r[2];
+    for (int j = 0; j < ns; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + com[i];
+  }
+  if (P.ins_rot == 0) {
+    // uniform rotation: quaternion by rejection from the 4-cube
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    for (int a = 0; a < 63; ++a) {
+      double t[4];
+      for (int i = 0; i < 4; ++i) t[i] = rng_uniform(seed, index, (uint32_t)(3 + 4 * a + i)) * 2.0 - 1.0;
+      double s = t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + t[3] * t[3];
+      if (s < 1.0) {
+        double nrm = sqrt(s);
+        for (int i = 0; i < 4; ++i) q[i] = t[i] / nrm;
+        break;
+      }
+    }
+    com_dev(ns, out, mass, c);
+    double cj[4] = {q[0], -q[1], -q[2], -q[3]};
+    for (int j = 0; j < ns; ++j) {
+      double q2[4] = {0.0, out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]};
+      double t[4], r2[4];
+      qprod_dev(q, q2, t);
+      qprod_dev(t, cj, r2);
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = r2[1 + i] + c[i];
+    }
   }
 }
 
-// Per configuration: reference point C (mean of the sites) and radius, the
-// B-spline data of every solute site, and its separable spectra
-//   what_d(m) = sum_s w_s exp(-2 pi i m ((g_d - s) mod K_d) / K_d)
-// for axis 1 (m < H) and axis 2 (m < K2).
 template <typename T>
 __global__ __launch_bounds__(128) void k_solute_prep(Params P, const double *__restrict__ spos,
                                                      const double *__restrict__ qsolute,

@@ -201,6 +201,17 @@ module ermgpu
                                     column_threads
      end function erm_set_tuning
 
+     ! insorigin / insposition / insorient / lwreg / upreg / hostspec of
+     ! instslt (insertion.F90:103-366), the reference's numbering
+     integer(c_int32_t) function erm_set_insertion(ctx, insorigin, insposition, insorient, lwreg, upreg, &
+          nhost, hostspec) bind(C, name="erm_set_insertion")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: insorigin, insposition, insorient, nhost
+       real(c_double), value :: lwreg, upreg
+       integer(c_int32_t), intent(in) :: hostspec(*)
+     end function erm_set_insertion
+
      integer(c_int32_t) function erm_get_tuning(ctx, out9) bind(C, name="erm_get_tuning")
        import :: c_ptr, c_int32_t
        type(c_ptr), value :: ctx

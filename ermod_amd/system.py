@@ -29,6 +29,11 @@ ES_NVT, ES_NPT = 1, 2
 # cltype and boxshp (engmain.F90:306,314); Ewald (1) is refused by the reference
 EL_COULOMB, EL_PME, EL_PPPM = 0, 2, 3
 SYS_NONPERIODIC, SYS_PERIODIC = 0, 1
+# insertion modes (engmain.F90:320-327)
+INSORG_ORIGIN, INSORG_NOCHANGE, INSORG_AGGCEN, INSORG_REFSTR = 0, 1, 2, 3
+(INSPOS_RANDOM, INSPOS_NOCHANGE, INSPOS_SPHERE, INSPOS_SLAB_GENERIC, INSPOS_SLAB_SYMMETRIC, INSPOS_RMSD,
+ INSPOS_GAUSS) = range(7)
+INSROT_RANDOM, INSROT_NOCHANGE = 0, 1
 PT_SOLVENT, PT_SOLUTE, PT_TEST_RIGID, PT_TEST_FLEX = 0, 1, 2, 3
 
 # getmass (setconf.F90:999-1053)

@@ -82,6 +82,13 @@ typedef struct erm_params {
 } erm_params;
 
 enum { ERM_EL_COULOMB = 0, ERM_EL_PME = 2, ERM_EL_PPPM = 3 };
+/* insertion modes, the reference's numbering (engmain.F90:320-327) */
+enum { ERM_INSORG_ORIGIN = 0, ERM_INSORG_NOCHANGE = 1, ERM_INSORG_AGGCEN = 2, ERM_INSORG_REFSTR = 3 };
+enum {
+  ERM_INSPOS_RANDOM = 0, ERM_INSPOS_NOCHANGE = 1, ERM_INSPOS_SPHERE = 2, ERM_INSPOS_SLAB_GENERIC = 3,
+  ERM_INSPOS_SLAB_SYMMETRIC = 4, ERM_INSPOS_RMSD = 5, ERM_INSPOS_GAUSS = 6
+};
+enum { ERM_INSROT_RANDOM = 0, ERM_INSROT_NOCHANGE = 1 };
 enum { ERM_SYS_NONPERIODIC = 0, ERM_SYS_PERIODIC = 1 };
 
 /* &hist namelist of one species (engproc.F90:66-67) */
@@ -130,6 +137,19 @@ int32_t erm_set_bins(erm_ctx *ctx, int32_t numslv, const int32_t *uvmax, const i
 /* Zeroes the accumulators of a division; replaces engclear
  * (engproc.F90:276-288). */
 int32_t erm_engclear(erm_ctx *ctx);
+
+/* Placement of device-generated test-particle poses; replaces the
+ * insorigin / insposition / insorient / lwreg / upreg / hostspec handling of
+ * instslt (set_solute_origin, set_shift_com, apply_orientation,
+ * com_aggregate; insertion.F90:103-366, 539-566) with the pairings and
+ * bounds setconf.F90:399-432 enforces.  Supported: ORIGIN + RANDOM (the
+ * default), AGGCEN + SPHERE / SLAB_GENERIC / SLAB_SYMMETRIC, each with
+ * RANDOM or NOCHANGE orientation; hostspec lists nhost solvent species
+ * (uvspec numbering) whose molecules define the aggregate centre, computed
+ * per frame.  REFSTR-based modes (RMSD, GAUSS) return ERM_E_UNSUPPORTED.
+ * Applies to erm_insert_random and erm_solute_gradient's random poses. */
+int32_t erm_set_insertion(erm_ctx *ctx, int32_t insorigin, int32_t insposition, int32_t insorient,
+                          double lwreg, double upreg, int32_t nhost, const int32_t *hostspec);
 
 /* One frame: cell inverse and volume (get_inverted_cell, engproc.F90:898-919),
  * Green's function when the cell changed (recpcal_spline_greenfunc,

@@ -40,6 +40,11 @@ class OrcSystem(C.Structure):
                 ("esmax", C.c_int32), ("escrd", _f64p), ("cltype", C.c_int32), ("boxshp", C.c_int32)]
 
 
+class OrcInsertion(C.Structure):
+    _fields_ = [("insorigin", C.c_int32), ("insposition", C.c_int32), ("insorient", C.c_int32),
+                ("lwreg", C.c_double), ("upreg", C.c_double)]
+
+
 class OrcAcc(C.Structure):
     _fields_ = [("edens", _f64p), ("ecorr", _f64p), ("eself", _f64p), ("minuv", _f64p),
                 ("maxuv", _f64p), ("slnuv", _f64p), ("engnorm", C.c_double),
@@ -75,6 +80,9 @@ def lib():
         L.orc_spline_axis.argtypes = [C.c_int32, C.c_int32, _f64p]
         L.orc_greenfunc.argtypes = [_i32p, C.c_int32, _f64p, C.c_double, C.c_double, _f64p, _f64p,
                                     _f64p, _f64p]
+        L.orc_aggregate_center.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_insertion_pose.argtypes = [C.c_int32, _f64p, _f64p, _f64p, C.POINTER(OrcInsertion), _f64p,
+                                         C.c_uint64, C.c_uint64, _f64p]
         L.orc_gfb_pppm.argtypes = [C.c_int32, _f64p]
         L.orc_pppm_greenfunc.argtypes = [_i32p, C.c_int32, _f64p, _f64p, C.c_double, C.c_double, _f64p]
         L.orc_realcal_bare.argtypes = [vp, C.c_int32, _f64p, _f64p]
@@ -136,6 +144,36 @@ def spline_coeff(order):
 def spline_value(order, x):
     c = spline_coeff(order)
     return np.array([lib().orc_spline_value(P(c), order, float(v)) for v in np.atleast_1d(x)])
+
+
+def host_molecules(system, hostspec):
+    """com_aggregate's molecule selection: every physical molecule whose
+    species (uvspec) is listed in hostspec."""
+    s = system
+    return i32([(int(s.sluvid[m]) <= 1 and int(s.uvspec[m]) in set(hostspec)) for m in range(s.nmol)])
+
+
+def aggregate_center(system, sitepos, hostspec):
+    """com_aggregate (insertion.F90:539-566) of a frame."""
+    s = system
+    out = np.zeros(3)
+    lib().orc_aggregate_center(s.nmol, P(i32(s.mol_begin), _i32p), P(host_molecules(s, hostspec), _i32p),
+                               P(f64(s.sitemass)), P(f64(sitepos)), P(out))
+    return out
+
+
+def insertion_pose(bfcoord, mass, cell, ins, seed, index, aggcen=None):
+    """instslt's placement of insertion `index` of stream `seed`; ins =
+    (insorigin, insposition, insorient, lwreg, upreg)."""
+    bf = f64(bfcoord)
+    n = bf.size // 3
+    st = OrcInsertion(int(ins[0]), int(ins[1]), int(ins[2]), float(ins[3]), float(ins[4]))
+    out = np.zeros((n, 3))
+    cm = f64(np.asarray(cell).T.copy())
+    ag = None if aggcen is None else f64(aggcen)
+    lib().orc_insertion_pose(n, P(bf), P(f64(mass)), P(cm), C.byref(st), P(ag), C.c_uint64(seed),
+                             C.c_uint64(index), P(out))
+    return out
 
 
 def gfb_pppm(order):

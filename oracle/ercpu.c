@@ -686,6 +686,93 @@ void orc_random_pose(uint64_t seed, uint64_t index, double *r, double *q) {
   q[1] = q[2] = q[3] = 0.0;
 }
 
+void orc_aggregate_center(int32_t nmol, const int32_t *mol_begin, const int32_t *hostmol, const double *sitemass,
+                          const double *pos, double *center) {
+  double msum = 0.0;
+  for (int m = 0; m < nmol; ++m)
+    if (hostmol[m])
+      for (int a = mol_begin[m]; a < mol_begin[m + 1]; ++a) msum = msum + sitemass[a];
+  for (int i = 0; i < 3; ++i) {
+    double d = 0.0;
+    for (int m = 0; m < nmol; ++m)
+      if (hostmol[m])
+        for (int a = mol_begin[m]; a < mol_begin[m + 1]; ++a) d = d + pos[3 * a + i] * sitemass[a];
+    center[i] = d / msum;
+  }
+}
+
+void orc_insertion_pose(int32_t n, const double *bfcoord, const double *mass, const double *cell,
+                        const orc_insertion *ins, const double *aggcen, uint64_t seed, uint64_t index,
+                        double *out) {
+  const uint64_t sp = seed ^ 0xD1B54A32D192ED03ULL;
+  double c[3];
+  memcpy(out, bfcoord, sizeof(double) * 3 * n);
+  /* set_solute_origin: ORIGIN or AGGCEN move the COM there (set_solute_com) */
+  if (ins->insorigin != 1) {
+    double o[3] = {0.0, 0.0, 0.0};
+    if (ins->insorigin == 2)
+      for (int i = 0; i < 3; ++i) o[i] = aggcen[i];
+    com_of(n, out, mass, c);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + o[i];
+  }
+  /* set_shift_com */
+  if (ins->insposition == 0) {
+    double r[3], com[3];
+    for (int i = 0; i < 3; ++i) r[i] = orc_rng_uniform(seed, index, (uint32_t)i);
+    for (int i = 0; i < 3; ++i) com[i] = cell[i] * r[0] + cell[3 + i] * r[1] + cell[6 + i] * r[2];
+    com_of(n, out, mass, c);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + com[i];
+  } else if (ins->insposition == 2) {
+    /* INSPOS_SPHERE (insertion.F90:165-180) */
+    double t = orc_rng_uniform(sp, index, 0);
+    t = pow(t * (ipow(ins->upreg, 3) - ipow(ins->lwreg, 3)) + ipow(ins->lwreg, 3), 1.0 / 3.0);
+    double r[3] = {0.0, 0.0, 1.0}, norm = 1.0;
+    for (int a = 0; a < 84; ++a) {
+      double u[3];
+      for (int i = 0; i < 3; ++i) u[i] = orc_rng_uniform(sp, index, (uint32_t)(1 + 3 * a + i)) * 2.0 - 1.0;
+      double s2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+      if (0.0 < s2 && s2 <= 1.0) {
+        memcpy(r, u, sizeof r);
+        norm = s2;
+        break;
+      }
+    }
+    norm = sqrt(norm);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + t * r[i] / norm;
+  } else if (ins->insposition == 3 || ins->insposition == 4) {
+    /* INSPOS_SLAB_GENERIC / SYMMETRIC (insertion.F90:181-198) */
+    double r[3], com[3];
+    r[0] = orc_rng_uniform(sp, index, 0);
+    r[1] = orc_rng_uniform(sp, index, 1);
+    double t = orc_rng_uniform(sp, index, 2);
+    t = t * (ins->upreg - ins->lwreg) + ins->lwreg;
+    if (ins->insposition == 4) {
+      double dir = orc_rng_uniform(sp, index, 3);
+      if (dir > 0.5) t = -t;
+    }
+    double len3 = sqrt(cell[6] * cell[6] + cell[7] * cell[7] + cell[8] * cell[8]);
+    r[2] = t / len3;
+    for (int i = 0; i < 3; ++i) com[i] = cell[i] * r[0] + cell[3 + i] * r[1] + cell[6 + i] * r[2];
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + com[i];
+  }
+  /* apply_orientation */
+  if (ins->insorient == 0) {
+    double r[3], q[4];
+    orc_random_pose(seed, index, r, q);
+    com_of(n, out, mass, c);
+    for (int j = 0; j < n; ++j) {
+      double v[3] = {out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]};
+      double t[3];
+      qrotate(v, q, t);
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = t[i] + c[i];
+    }
+  }
+}
+
 /* ====================================================================== */
 /* context                                                                */
 /* ====================================================================== */

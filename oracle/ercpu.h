@@ -131,6 +131,26 @@ void orc_place_solute(int32_t n, const double *bfcoord, const double *mass,
 double orc_rng_uniform(uint64_t seed, uint64_t index, uint32_t draw);
 void orc_random_pose(uint64_t seed, uint64_t index, double *r, double *q);
 
+/* insertion modes (engmain.F90:320-327): insorigin 0 ORIGIN / 2 AGGCEN,
+ * insposition 0 RANDOM / 2 SPHERE / 3 SLAB_GENERIC / 4 SLAB_SYMMETRIC,
+ * insorient 0 RANDOM / 1 NOCHANGE; lwreg, upreg the region bounds */
+typedef struct orc_insertion {
+  int32_t insorigin, insposition, insorient;
+  double lwreg, upreg;
+} orc_insertion;
+/* com_aggregate (insertion.F90:539-566): centre of mass of the sites of the
+ * molecules with hostmol[m] != 0, sequential sums in molecule order */
+void orc_aggregate_center(int32_t nmol, const int32_t *mol_begin, const int32_t *hostmol,
+                          const double *sitemass, const double *pos, double *center);
+/* instslt's placement (insertion.F90:103-366) of insertion `index` of the
+ * generator stream `seed` under the given modes: the draws of
+ * orc_random_pose for the random position and the orientation, a second
+ * stream (seed ^ 0xD1B54A32D192ED03) for the sphere and slab draws.
+ * aggcen: the aggregate centre (AGGCEN only). */
+void orc_insertion_pose(int32_t n, const double *bfcoord, const double *mass, const double *cell,
+                        const orc_insertion *ins, const double *aggcen, uint64_t seed, uint64_t index,
+                        double *out);
+
 /* --- context: the per-frame pipeline ---------------------------------- */
 orc_ctx *orc_create(const orc_system *sys, int32_t *status);
 void orc_destroy(orc_ctx *c);
