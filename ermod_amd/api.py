@@ -21,7 +21,8 @@ LIBPATH = os.path.join(HERE, "libermgpu.so")
 
 ERM_FP64, ERM_FP32 = 0, 1
 STATUS = {0: "ok", 1: "eng_min", 2: "eng_sft", 3: "mbin", 4: "eng_ecd", 5: "rcp_cns", 6: "noncuboid",
-          7: "eng_bug", 8: "eng_cns", 9: "eng_siz", 10: "arg", 11: "unsupported", 12: "device"}
+          7: "eng_bug", 8: "eng_cns", 9: "eng_siz", 10: "arg", 11: "unsupported", 12: "device", 13: "eof"}
+ERM_E_EOF = 13
 STAGES = ("prep", "pme_z", "pme_y", "pme_x_gather", "pair", "bin", "corr", "total")
 
 _i32p = C.POINTER(C.c_int32)
@@ -56,7 +57,8 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_get_uv_energy", "erm_realcal_proc", "erm_realcal_self", "erm_recpcal", "erm_engconst",
            "erm_insert_random", "erm_update_histogram", "erm_engstore", "erm_fetch", "erm_sizes",
            "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune",
-           "erm_insert_random_multi", "erm_set_insertion")
+           "erm_insert_random_multi", "erm_set_insertion", "erm_dcd_open", "erm_dcd_read", "erm_dcd_close",
+           "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close")
 
 _lib = None
 
@@ -104,6 +106,15 @@ def lib():
                                           C.c_int32, _f64p, _f64p, _f64p, _f64p]
         L.erm_set_insertion.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                         _i32p]
+        L.erm_dcd_open.argtypes = [C.c_char_p, C.POINTER(vp), _i32p, _i32p]
+        L.erm_dcd_read.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float), _f64p]
+        L.erm_dcd_close.argtypes = [vp]
+        L.erm_dcd_close.restype = None
+        L.erm_read_permutation.argtypes = [C.c_char_p, C.c_int32, _i32p, _i32p]
+        L.erm_traj_open.argtypes = [vp, C.c_char_p, C.c_int32, _i32p, C.c_char_p, C.c_int32, C.POINTER(vp)]
+        L.erm_traj_next.argtypes = [vp, _f64p, _f64p]
+        L.erm_traj_close.argtypes = [vp]
+        L.erm_traj_close.restype = None
         _lib = L
     return _lib
 
@@ -118,6 +129,78 @@ def _i32(a):
 
 def _p(a, t=_f64p):
     return None if a is None else a.ctypes.data_as(t)
+
+
+def read_dcd(path, natom=None, periodic=True):
+    """All frames of a DCD file through the library's host reader (as
+    trajectory-formats/dcd.f90 reads them): (coords [F, natom, 3] float32,
+    cells [F, 3, 3])."""
+    h = C.c_void_p()
+    nf, hc = C.c_int32(), C.c_int32()
+    st = lib().erm_dcd_open(os.fsencode(path), C.byref(h), C.byref(nf), C.byref(hc))
+    if st:
+        raise ErmError(st, "erm_dcd_open")
+    n = nf.value if natom is None else natom
+    frames, cells = [], []
+    try:
+        while True:
+            xyz = np.zeros(3 * n, dtype=np.float32)
+            cm = np.zeros(9)
+            st = lib().erm_dcd_read(h, n, int(bool(periodic)), xyz.ctypes.data_as(C.POINTER(C.c_float)), _p(cm))
+            if st == ERM_E_EOF:
+                break
+            if st:
+                raise ErmError(st, "erm_dcd_read")
+            frames.append(xyz.reshape(3, n).T.copy())
+            cells.append(cm.reshape(3, 3).T.copy())
+    finally:
+        lib().erm_dcd_close(h)
+    return np.array(frames).reshape(-1, n, 3), np.array(cells).reshape(-1, 3, 3)
+
+
+def read_permutation(path, natom):
+    """PermIndex (setconf.F90:866-905) as a 0-based array, or None when the
+    file does not exist."""
+    perm = np.zeros(natom, dtype=np.int32)
+    found = C.c_int32()
+    st = lib().erm_read_permutation(os.fsencode(path), natom, _p(perm, _i32p), C.byref(found))
+    if st:
+        raise ErmError(st, "erm_read_permutation")
+    return perm if found.value else None
+
+
+class Trajectory:
+    """Frames of a DCD file streamed into an ErmGPU context (erm_traj_*)."""
+
+    def __init__(self, engine, path, natom, perm=None, weights=None, skpcnf=1):
+        self.engine = engine
+        self._perm = None if perm is None else _i32(perm)
+        h = C.c_void_p()
+        engine._check(lib().erm_traj_open(engine.h, os.fsencode(path), int(natom), _p(self._perm, _i32p),
+                                          None if weights is None else os.fsencode(weights), int(skpcnf),
+                                          C.byref(h)), "traj_open")
+        self.h = h
+
+    def next(self):
+        """Loads the next frame; returns (cell, weight), or None at the end."""
+        cm = np.zeros(9)
+        w = C.c_double()
+        st = lib().erm_traj_next(self.h, _p(cm), C.byref(w))
+        if st == ERM_E_EOF:
+            return None
+        self.engine._check(st, "traj_next")
+        return cm.reshape(3, 3).T.copy(), w.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().erm_traj_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def hist_struct(h) -> ErmHist:

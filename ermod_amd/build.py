@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = "/opt/rocm/bin/amdflang"
 SOURCES = [os.path.join(HERE, "csrc", "ermgpu.hip")]
-HEADERS = [os.path.join(HERE, "csrc", f) for f in ("ermgpu_dev.h", "ermgpu_kern.h", "ermgpu_grad.h")] + [
+HEADERS = [os.path.join(HERE, "csrc", f) for f in ("ermgpu_dev.h", "ermgpu_kern.h", "ermgpu_grad.h",
+                                                         "ermgpu_traj.h")] + [
     os.path.join(ROOT, "include", "ermgpu.h")]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -10,9 +10,11 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ermgpu_grad.h"
+#include "ermgpu_traj.h"
 
 using namespace erm;
 
@@ -329,6 +331,8 @@ struct erm_ctx {
   // insertion modes: host molecules of the aggregate centre, and the centre
   std::vector<int> hostmol;
   DBuf d_hostmol, d_aggcen;
+  // trajectory frames: FP32 staging and the permutation
+  DBuf d_tstage, d_tperm;
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
       d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
   DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
@@ -1725,17 +1729,22 @@ static int aggregate_center(erm_ctx *c) {
   return launch_check(c, "k_aggcen");
 }
 
+// per-frame work after the coordinates are in d_pos
+static int frame_finish(erm_ctx *c) {
+  if (c->kp.ins_org == ERM_INSORG_AGGCEN) TRY(aggregate_center(c));
+  int s = c->fp32 ? frame_prep<float>(c) : frame_prep<double>(c);
+  if (s) return s;
+  c->have_frame = true;
+  return ERM_OK;
+}
+
 static int set_frame_impl(erm_ctx *c, const double *pos, bool device_ptr, const double *cell, double w) {
   if (!c || !pos) return ERM_E_ARG;
   if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
   TRY(set_frame_common(c, cell, w));
   HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * c->natom,
                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
-  if (c->kp.ins_org == ERM_INSORG_AGGCEN) TRY(aggregate_center(c));
-  int s = c->fp32 ? frame_prep<float>(c) : frame_prep<double>(c);
-  if (s) return s;
-  c->have_frame = true;
-  return ERM_OK;
+  return frame_finish(c);
 }
 
 int32_t erm_set_insertion(erm_ctx *c, int32_t insorigin, int32_t insposition, int32_t insorient, double lwreg,
@@ -2280,6 +2289,188 @@ int32_t erm_solute_gradient(erm_ctx *c, int32_t n, const int32_t *solutes, const
     if (st) return st;
   }
   return ERM_OK;
+}
+
+// ---- frame ingestion ----------------------------------------------------
+int32_t erm_dcd_open(const char *path, erm_dcd **out, int32_t *natom_file, int32_t *has_cell) {
+  if (!path || !out) return ERM_E_ARG;
+  *out = nullptr;
+  Dcd *d = new Dcd();
+  if (d->open(path)) {
+    d->close();
+    delete d;
+    return ERM_E_ARG;
+  }
+  if (natom_file) *natom_file = d->natom_file;
+  if (has_cell) *has_cell = d->have_cell ? 1 : 0;
+  *out = reinterpret_cast<erm_dcd *>(d);
+  return ERM_OK;
+}
+
+int32_t erm_dcd_read(erm_dcd *h, int32_t natom, int32_t periodic, float *xyz, double *cell) {
+  if (!h || !xyz || !cell || natom <= 0) return ERM_E_ARG;
+  Dcd *d = reinterpret_cast<Dcd *>(h);
+  if (periodic && !d->have_cell) return ERM_E_ARG;
+  int r = d->frame(natom, periodic != 0, xyz, cell);
+  return r == 0 ? ERM_OK : (r == 1 ? ERM_E_EOF : ERM_E_ARG);
+}
+
+void erm_dcd_close(erm_dcd *h) {
+  if (!h) return;
+  Dcd *d = reinterpret_cast<Dcd *>(h);
+  d->close();
+  delete d;
+}
+
+int32_t erm_read_permutation(const char *path, int32_t natom_traj, int32_t *perm, int32_t *found) {
+  if (!path || !perm || natom_traj <= 0) return ERM_E_ARG;
+  std::string msg;
+  int r = read_permutation(path, natom_traj, perm, msg);
+  if (found) *found = (r == 1) ? 0 : 1;
+  if (r == 1)
+    for (int i = 0; i < natom_traj; ++i) perm[i] = i;
+  return r < 0 ? ERM_E_ARG : ERM_OK;
+}
+
+}  // extern "C"
+
+// one frame read ahead: pinned coordinates, cell, weight and read status
+struct TrajSlot {
+  float *xyz = nullptr;
+  double cell[9] = {};
+  double weight = 1.0;
+  int status = 0;  // 0 ok, 1 end of file, -1 malformed frame, -2 bad SysWght
+};
+
+struct erm_traj {
+  erm_ctx *ctx = nullptr;
+  Dcd dcd;
+  WeightFile wf;
+  bool weights = false, periodic = true, have_perm = false;
+  int natom = 0, skip = 1, cur = 0;
+  TrajSlot slot[2];
+  std::thread reader;
+  bool reading = false;
+};
+
+// getconf_parallel's read loop for one frame (setconf.F90:917-927): skpcnf
+// frames, each with its SysWght line; the last one is kept
+static void traj_read(erm_traj *t, int k) {
+  TrajSlot &s = t->slot[k];
+  for (int i = 0; i < t->skip; ++i) {
+    s.status = t->dcd.frame(t->natom, t->periodic, s.xyz, s.cell);
+    if (s.status) return;
+    s.weight = 1.0;
+    if (t->weights && !t->wf.next(s.weight)) {
+      s.status = -2;
+      return;
+    }
+  }
+}
+
+static void traj_join(erm_traj *t) {
+  if (t->reading) t->reader.join();
+  t->reading = false;
+}
+
+extern "C" {
+
+int32_t erm_traj_open(erm_ctx *c, const char *dcd_path, int32_t natom_traj, const int32_t *perm,
+                      const char *weight_path, int32_t skpcnf, erm_traj **out) {
+  if (!out) return ERM_E_ARG;
+  *out = nullptr;
+  TRY(need(c, false, false));
+  if (!dcd_path) return fail(c, ERM_E_ARG, "no trajectory path");
+  if (skpcnf < 1) return fail(c, ERM_E_ARG, "skpcnf must be >= 1");
+  // the trajectory holds the physical molecules, which come first
+  int nphys = 0;
+  for (int m = 0; m < c->nmol; ++m) {
+    const bool phys = (c->sluvid[m] <= 1);
+    if (phys) nphys += c->mol_begin[m + 1] - c->mol_begin[m];
+    if (phys ? c->mol_begin[m + 1] > natom_traj : c->mol_begin[m] < natom_traj)
+      return fail(c, ERM_E_ARG, "the %d trajectory atoms must be those of the solvent and solute molecules, first",
+                  natom_traj);
+  }
+  if (nphys != natom_traj)
+    return fail(c, ERM_E_ARG, "the trajectory must hold the %d atoms of the solvent and solute molecules", nphys);
+  if (perm) {
+    std::vector<int> cnt(natom_traj, 0);
+    for (int i = 0; i < natom_traj; ++i) {
+      if (perm[i] < 0 || perm[i] >= natom_traj) return fail(c, ERM_E_ARG, "permutation entry out of range (set_pmt)");
+      ++cnt[perm[i]];
+    }
+    for (int i = 0; i < natom_traj; ++i)
+      if (cnt[i] != 1) return fail(c, ERM_E_ARG, "atom %d appears %d times in the permutation (set_pmt)", i + 1, cnt[i]);
+  }
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  erm_traj *t = new erm_traj();
+  t->ctx = c;
+  t->natom = natom_traj;
+  t->skip = skpcnf;
+  t->periodic = (c->P.boxshp == ERM_SYS_PERIODIC);
+  auto bail = [&](int code, const char *msg) {
+    for (auto &s : t->slot)
+      if (s.xyz) (void)hipHostFree(s.xyz);
+    t->dcd.close();
+    t->wf.close();
+    delete t;
+    return fail(c, code, "%s", msg);
+  };
+  if (const char *e = t->dcd.open(dcd_path)) return bail(ERM_E_ARG, e);
+  if (t->dcd.natom_file < natom_traj) return bail(ERM_E_ARG, "the DCD file has fewer atoms than the system");
+  if (t->periodic && !t->dcd.have_cell) return bail(ERM_E_ARG, "Cell info is requested, but does not exist!");
+  if (weight_path) {
+    if (!t->wf.open(weight_path)) return bail(ERM_E_ARG, "cannot open the weight file");
+    t->weights = true;
+  }
+  for (auto &s : t->slot)
+    if (hipHostMalloc((void **)&s.xyz, sizeof(float) * 3 * natom_traj, hipHostMallocDefault) != hipSuccess)
+      return bail(ERM_E_DEVICE, "hipHostMalloc");
+  if (ensure(c, c->d_tstage, sizeof(float) * 3 * natom_traj)) return bail(ERM_E_DEVICE, "staging buffer");
+  if (perm) {
+    if (upload(c, c->d_tperm, perm, natom_traj)) return bail(ERM_E_DEVICE, "permutation upload");
+    t->have_perm = true;
+  }
+  t->reader = std::thread(traj_read, t, 0);
+  t->reading = true;
+  *out = t;
+  return ERM_OK;
+}
+
+int32_t erm_traj_next(erm_traj *t, double *cell, double *weight) {
+  if (!t) return ERM_E_ARG;
+  erm_ctx *c = t->ctx;
+  traj_join(t);
+  TrajSlot &s = t->slot[t->cur];
+  if (s.status == 1) return ERM_E_EOF;
+  if (s.status == -1) return fail(c, ERM_E_ARG, "malformed DCD frame");
+  if (s.status == -2) return fail(c, ERM_E_ARG, "the weight file is ill-formed");
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  TRY(set_frame_common(c, s.cell, s.weight));
+  HIPCHK(c, hipMemcpyAsync(c->d_tstage.p, s.xyz, sizeof(float) * 3 * t->natom, hipMemcpyHostToDevice, c->st));
+  hipLaunchKernelGGL(k_frame_scatter, dim3((t->natom + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, t->natom,
+                     dp<float>(c->d_tstage), t->have_perm ? dp<int>(c->d_tperm) : (const int *)nullptr,
+                     dp<double>(c->d_pos));
+  TRY(launch_check(c, "k_frame_scatter"));
+  if (cell) memcpy(cell, s.cell, sizeof s.cell);
+  if (weight) *weight = s.weight;
+  // the next frame is read into the other pinned buffer while the device
+  // prepares this one
+  const int nxt = 1 - t->cur;
+  t->reader = std::thread(traj_read, t, nxt);
+  t->reading = true;
+  t->cur = nxt;
+  return frame_finish(c);
+}
+
+void erm_traj_close(erm_traj *t) {
+  if (!t) return;
+  traj_join(t);
+  for (auto &s : t->slot)
+    if (s.xyz) (void)hipHostFree(s.xyz);
+  t->dcd.close();
+  t->wf.close();
+  delete t;
 }
 
 }  // extern "C"

@@ -203,6 +203,18 @@ __device__ void qprod_dev(const double *a, const double *b, double *p) {
   for (int i = 0; i < 3; ++i) p[1 + i] = cr[i] + a[0] * b[1 + i] + b[0] * a[1 + i];
 }
 
+// a trajectory frame (FP32 SoA x[n], y[n], z[n]) into the FP64 AoS
+// coordinates through the permutation (setconf.F90:983-987): trajectory atom
+// i becomes atom perm[i]; the conversion to FP64 is exact
+__global__ void k_frame_scatter(int n, const float *__restrict__ xyz, const int *__restrict__ perm, double *pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = perm ? perm[i] : i;
+  pos[3 * a] = (double)xyz[i];
+  pos[3 * a + 1] = (double)xyz[n + i];
+  pos[3 * a + 2] = (double)xyz[2 * n + i];
+}
+
 // com_aggregate (insertion.F90:539-566): centre of mass of every site of
 // the host molecules, summed in molecule order by a single thread so that the
 // rounding is that of the sequential sums (center_of_mass, bestfit.F90:158-172)

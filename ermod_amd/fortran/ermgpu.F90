@@ -11,7 +11,7 @@ module ermgpu
   integer(c_int32_t), parameter :: ERM_OK = 0, ERM_E_ENG_MIN = 1, ERM_E_ENG_SFT = 2, &
        ERM_E_MBIN = 3, ERM_E_ENG_ECD = 4, ERM_E_RCP_CNS = 5, ERM_E_NONCUBOID = 6, &
        ERM_E_ENG_BUG = 7, ERM_E_ENG_CNS = 8, ERM_E_ENG_SIZ = 9, ERM_E_ARG = 10, &
-       ERM_E_UNSUPPORTED = 11, ERM_E_DEVICE = 12
+       ERM_E_UNSUPPORTED = 11, ERM_E_DEVICE = 12, ERM_E_EOF = 13
   integer(c_int32_t), parameter :: ERM_FP64 = 0, ERM_FP32 = 1
 
   type, bind(C) :: erm_params
@@ -234,6 +234,35 @@ module ermgpu
        integer(c_int64_t), value :: seed, first_index
        type(c_ptr), value :: solutes, solute_pos, weights, q, eps, sigma, acc
      end function erm_solute_gradient
+
+     ! frame ingestion (getconf_parallel, setconf.F90:860-996; dcd.f90)
+     integer(c_int32_t) function erm_traj_open_c(ctx, dcd_path, natom_traj, perm, weight_path, skpcnf, traj) &
+          bind(C, name="erm_traj_open")
+       import :: c_ptr, c_int32_t, c_char
+       type(c_ptr), value :: ctx, perm, weight_path
+       character(kind=c_char), intent(in) :: dcd_path(*)
+       integer(c_int32_t), value :: natom_traj, skpcnf
+       type(c_ptr), intent(out) :: traj
+     end function erm_traj_open_c
+
+     integer(c_int32_t) function erm_traj_next(traj, cell, weight) bind(C, name="erm_traj_next")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: traj
+       real(c_double), intent(out) :: cell(3, 3), weight
+     end function erm_traj_next
+
+     subroutine erm_traj_close(traj) bind(C, name="erm_traj_close")
+       import :: c_ptr
+       type(c_ptr), value :: traj
+     end subroutine erm_traj_close
+
+     integer(c_int32_t) function erm_read_permutation_c(path, natom_traj, perm, found) &
+          bind(C, name="erm_read_permutation")
+       import :: c_int32_t, c_char
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: natom_traj
+       integer(c_int32_t), intent(out) :: perm(*), found
+     end function erm_read_permutation_c
   end interface
 
 contains
@@ -252,6 +281,37 @@ contains
        msg(i:i) = s(i)
     end do
   end function erm_last_error
+
+  ! a DCD trajectory as the frame source of ctx; perm0 (0-based) and the
+  ! SysWght path are optional
+  integer(c_int32_t) function erm_traj_open(ctx, dcd_path, natom_traj, skpcnf, traj, perm0, weight_path) &
+       result(st)
+    use iso_c_binding, only: c_loc
+    type(c_ptr), intent(in) :: ctx
+    character(*), intent(in) :: dcd_path
+    integer(c_int32_t), intent(in) :: natom_traj, skpcnf
+    type(c_ptr), intent(out) :: traj
+    integer(c_int32_t), intent(in), target, optional :: perm0(:)
+    character(*), intent(in), optional :: weight_path
+    character(kind=c_char, len=:), allocatable, target :: wp
+    type(c_ptr) :: pp, wpp
+    pp = c_null_ptr
+    wpp = c_null_ptr
+    if (present(perm0)) pp = c_loc(perm0)
+    if (present(weight_path)) then
+       wp = trim(weight_path) // c_null_char
+       wpp = c_loc(wp)
+    end if
+    st = erm_traj_open_c(ctx, trim(dcd_path) // c_null_char, natom_traj, pp, wpp, skpcnf, traj)
+  end function erm_traj_open
+
+  ! PermIndex into the 0-based perm0; found = 0 (identity) without the file
+  integer(c_int32_t) function erm_read_permutation(path, natom_traj, perm0, found) result(st)
+    character(*), intent(in) :: path
+    integer(c_int32_t), intent(in) :: natom_traj
+    integer(c_int32_t), intent(out) :: perm0(*), found
+    st = erm_read_permutation_c(trim(path) // c_null_char, natom_traj, perm0, found)
+  end function erm_read_permutation
 
   ! engstore's file output (engproc.F90:560-656), called after erm_engstore
   ! for division `division` of `engdiv`: engsln|engref.NN and, with corrcal,

@@ -4,7 +4,7 @@
 ! per frame set_frame + random insertions (engconst), engstore and the
 ! reference's output files (erm_store_division, engproc.F90:560-656).
 !
-! usage: ermgpu_host <input stream file> <output directory>
+! usage: ermgpu_host <input stream file> <output directory> [trajectory.dcd]
 ! The input holds, in order and without padding: natom nmol ntype nframes
 ! nins ndiv (int32); the erm_params fields except pme_path (cltype and
 ! boxshp last); the erm_hist fields of species 1
@@ -12,7 +12,9 @@
 ! mol_begin(nmol+1) charge(natom) ljtype(natom) lj_eps(ntype^2)
 ! lj_sig2(ntype^2) sluvid(nmol) uvspec(nmol) mol_charge(nmol)
 ! sitemass(natom) bfcoord(3*ns) cell(9) seed(int64); then nframes frames
-! of 3*natom coordinates, split evenly over the ndiv divisions.
+! of 3*natom coordinates, split evenly over the ndiv divisions.  With a DCD
+! file the frames (and their cells) come from it instead, through
+! erm_traj_*, with <output directory>/PermIndex and SysWght when present.
 program ermgpu_host
   use iso_c_binding
   use ermgpu
@@ -30,10 +32,17 @@ program ermgpu_host
   real(c_double) :: cell(3, 3)
   integer(c_int64_t) :: seed
   integer :: f, div, fpd
-  character(len=1024) :: inp, outdir
+  character(len=1024) :: inp, outdir, dcd
+  type(c_ptr) :: traj
+  integer(c_int32_t) :: ntraj, found
+  integer(c_int32_t), allocatable :: perm0(:)
+  real(c_double) :: fw
+  logical :: use_dcd, have_w
 
   call get_command_argument(1, inp)
   call get_command_argument(2, outdir)
+  call get_command_argument(3, dcd)
+  use_dcd = len_trim(dcd) > 0
   open(10, file=trim(inp), access='stream', form='unformatted', status='old', action='read')
   read(10) natom, nmol, ntype, nframes, nins, ndiv
   read(10) prm%slttype, prm%estype, prm%ljswitch, prm%splodr, prm%ms, prm%corrcal, prm%selfcal, &
@@ -68,15 +77,38 @@ program ermgpu_host
   st = erm_sizes(ctx, slvmax, ermax)
   allocate(uv(slvmax + 1, nins), edens(ermax), ecorr(ermax, ermax))
   allocate(avediv(ndiv, 2), aveuv(ndiv, 1))
+  if (use_dcd) then
+     ! the trajectory holds every atom but the test particle's
+     ntraj = natom - ns
+     allocate(perm0(ntraj))
+     st = erm_read_permutation(trim(outdir)//'/PermIndex', ntraj, perm0, found)
+     call check(st, 'erm_read_permutation')
+     inquire(file=trim(outdir)//'/SysWght', exist=have_w)
+     if (found /= 0 .and. have_w) then
+        st = erm_traj_open(ctx, trim(dcd), ntraj, 1, traj, perm0, trim(outdir)//'/SysWght')
+     else if (found /= 0) then
+        st = erm_traj_open(ctx, trim(dcd), ntraj, 1, traj, perm0)
+     else if (have_w) then
+        st = erm_traj_open(ctx, trim(dcd), ntraj, 1, traj, weight_path=trim(outdir)//'/SysWght')
+     else
+        st = erm_traj_open(ctx, trim(dcd), ntraj, 1, traj)
+     end if
+     call check(st, 'erm_traj_open')
+  end if
 
   fpd = nframes / ndiv
   do div = 1, ndiv
      st = erm_engclear(ctx)
      call check(st, 'erm_engclear')
      do f = (div - 1) * fpd + 1, div * fpd
-        read(10) pos
-        st = erm_set_frame(ctx, pos, cell, 1.0d0)
-        call check(st, 'erm_set_frame')
+        if (use_dcd) then
+           st = erm_traj_next(traj, cell, fw)
+           call check(st, 'erm_traj_next')
+        else
+           read(10) pos
+           st = erm_set_frame(ctx, pos, cell, 1.0d0)
+           call check(st, 'erm_set_frame')
+        end if
         if (f == 1) then
            st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_loc(uv))
         else
@@ -99,6 +131,7 @@ program ermgpu_host
           prm%temp, 1, [pemax], [pesoft], ercrd, int(esmax), escrd, avediv, aveuv, trim(outdir))
   end do
   close(10)
+  if (use_dcd) call erm_traj_close(traj)
   call erm_destroy(ctx)
   print '(a,i0,a,i0,a,i0,a)', 'ermgpu_host: ', ndiv, ' divisions, ', nframes, ' frames x ', nins, &
        ' insertions done'

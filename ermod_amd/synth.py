@@ -318,3 +318,34 @@ def mixed(nsite=20, nwater=7000, ncosolv=800, nion=100, L=60.0, K=64, seed=SEED0
 
 
 WORKLOADS = {"soln12": soln12, "refs20": refs20, "soln300": soln300, "soln30": soln30, "mixed": mixed}
+
+
+def write_dcd(path, frames, cells=None, big_endian=False, title="ermod_amd synthetic"):
+    """A DCD trajectory as CHARMM writes it (Fortran unformatted records):
+    the 'CORD' header (icntrl(11) = 1 when cells are given), one title line,
+    the atom count, then per frame the unit-cell record (A, 0, B, 0, 0, C for
+    the diagonal cell, the layout trajectory-formats/dcd.f90 reads into
+    cell(1,1), (1,2), (2,2), (1,3), (2,3), (3,3)) and FP32 x, y, z records."""
+    import struct
+    e = ">" if big_endian else "<"
+    frames = np.asarray(frames, dtype=np.float32)
+    nf, n = frames.shape[0], frames.shape[1]
+
+    def rec(payload):
+        return struct.pack(e + "i", len(payload)) + payload + struct.pack(e + "i", len(payload))
+
+    icntrl = [0] * 20
+    icntrl[0] = nf
+    icntrl[2] = 1
+    icntrl[10] = 1 if cells is not None else 0
+    icntrl[19] = 24
+    hdr = b"CORD" + struct.pack(e + "20i", *icntrl)
+    out = [rec(hdr), rec(struct.pack(e + "i", 1) + title.encode().ljust(80)[:80]), rec(struct.pack(e + "i", n))]
+    for k in range(nf):
+        if cells is not None:
+            c = np.asarray(cells[k], dtype=np.float64)
+            out.append(rec(struct.pack(e + "6d", c[0, 0], c[0, 1], c[1, 1], c[0, 2], c[1, 2], c[2, 2])))
+        for d in range(3):
+            out.append(rec(frames[k, :, d].astype(e + "f4").tobytes()))
+    with open(path, "wb") as f:
+        f.write(b"".join(out))

@@ -46,7 +46,8 @@ enum erm_status {
   ERM_E_ENG_SIZ = 9,     /* too many bins                         engproc.F90:194       */
   ERM_E_ARG = 10,        /* invalid argument or call order */
   ERM_E_UNSUPPORTED = 11,/* option outside this implementation */
-  ERM_E_DEVICE = 12      /* HIP runtime error */
+  ERM_E_DEVICE = 12,     /* HIP runtime error */
+  ERM_E_EOF = 13         /* end of the trajectory (not an error) */
 };
 
 enum erm_precision { ERM_FP64 = 0, ERM_FP32 = 1 };
@@ -150,6 +151,41 @@ int32_t erm_engclear(erm_ctx *ctx);
  * Applies to erm_insert_random and erm_solute_gradient's random poses. */
 int32_t erm_set_insertion(erm_ctx *ctx, int32_t insorigin, int32_t insposition, int32_t insorient,
                           double lwreg, double upreg, int32_t nhost, const int32_t *hostspec);
+
+/* ---- frame ingestion ----------------------------------------------------
+ * DCD trajectories as trajectory-formats/dcd.f90:39-105 reads them (Fortran
+ * unformatted records, 'CORD' header, optional unit-cell record, FP32 x, y, z
+ * records), the PermIndex permutation and the SysWght frame weights
+ * (getconf_parallel / read_weight, setconf.F90:860-996). */
+typedef struct erm_dcd erm_dcd;
+/* Host-only.  natom_file: the atom-count record; has_cell: icntrl(11). */
+int32_t erm_dcd_open(const char *path, erm_dcd **out, int32_t *natom_file, int32_t *has_cell);
+/* Host-only.  Next frame: xyz = x[natom], y[natom], z[natom] (the first natom
+ * values of each record); with periodic != 0 the cell record fills the
+ * column-major cell's (1,1), (1,2), (2,2), (1,3), (2,3), (3,3) as dcd.f90
+ * does (the rest 0).  ERM_E_EOF at the end of the file. */
+int32_t erm_dcd_read(erm_dcd *h, int32_t natom, int32_t periodic, float *xyz, double *cell);
+void erm_dcd_close(erm_dcd *h);
+/* Host-only.  PermIndex ("i p" lines, 1-based; setconf.F90:866-905) into the
+ * 0-based perm[natom_traj]; identity and *found = 0 when the file does not
+ * exist; ERM_E_ARG for entries out of range or targets not used once. */
+int32_t erm_read_permutation(const char *path, int32_t natom_traj, int32_t *perm, int32_t *found);
+
+typedef struct erm_traj erm_traj;
+/* Frame source for a context.  The file holds the natom_traj atoms of the
+ * solvent and solute molecules (they come first; test-particle slots
+ * follow); perm (0-based, NULL for none) sends trajectory atom i to atom
+ * perm[i]; weight_path names a SysWght file (NULL: weight 1); every frame
+ * used is the last of skpcnf frames read.  Close it before the context. */
+int32_t erm_traj_open(erm_ctx *ctx, const char *dcd_path, int32_t natom_traj, const int32_t *perm,
+                      const char *weight_path, int32_t skpcnf, erm_traj **out);
+/* getconf_parallel + erm_set_frame for the next frame: the FP32 coordinates
+ * go from pinned memory to the device asynchronously and are scattered
+ * there through the permutation into FP64; the following frame is read by
+ * a host thread meanwhile.  cell (9) and weight are optional outputs.
+ * ERM_E_EOF after the last frame. */
+int32_t erm_traj_next(erm_traj *t, double *cell, double *weight);
+void erm_traj_close(erm_traj *t);
 
 /* One frame: cell inverse and volume (get_inverted_cell, engproc.F90:898-919),
  * Green's function when the cell changed (recpcal_spline_greenfunc,

@@ -21,11 +21,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "ermod_amd", "fortran", "ermgpu_host")
 
 
-def write_input(path, sy, frames, nins, seed, ndiv=1, s=None):
+def write_input(path, sy, frames, nins, seed, ndiv=1, s=None, nframes=None):
     s = s or sy.system
     h = s.hists[0]
+    nframes = len(frames) if nframes is None else nframes
     with open(path, "wb") as f:
-        f.write(struct.pack("<6i", s.natom, s.nmol, s.ntype, len(frames), nins, ndiv))
+        f.write(struct.pack("<6i", s.natom, s.nmol, s.ntype, nframes, nins, ndiv))
         f.write(struct.pack("<4i3i4i6d5i", s.slttype, s.estype, s.ljswitch, s.splodr, *s.ms, s.corrcal,
                             s.selfcal, s.wgtslf, s.force_calculation, s.elecut, s.lwljcut, s.upljcut,
                             s.screen, s.temp, s.block_threshold, 0, 16, 0, s.cltype, s.boxshp))
@@ -144,3 +145,30 @@ def test_fortran_host_divisions_and_files(tmp_path):
         assert int(line[:5]) == pti
         mn = float(line[5:20])
         assert mn == pytest.approx(a["minuv"][pti], abs=1e-5)
+
+
+def test_fortran_host_frames_from_dcd(tmp_path):
+    """The same run with the frames read from a DCD file in permuted atom
+    order (PermIndex beside it) through erm_traj_*: identical raw
+    accumulators and engref.tt."""
+    print("\n[gpu case] fortran host tiny_refs from DCD + PermIndex", flush=True)
+    sy = synth.tiny_refs()
+    s = sy.system
+    frames = [sy.frame(i) for i in range(3)]
+    nins, seed = 12, 31
+    a, b = tmp_path / "stream", tmp_path / "dcd"
+    a.mkdir()
+    b.mkdir()
+    write_input(a / "in.bin", sy, frames, nins, seed)
+    r = subprocess.run([EXE, str(a / "in.bin"), str(a)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ntraj = s.natom - s.nsite(sy.solute_mol)
+    perm = np.random.default_rng(2).permutation(ntraj)
+    synth.write_dcd(b / "t.dcd", [f[:ntraj][perm] for f in frames], [sy.cell] * 3)
+    (b / "PermIndex").write_text("".join(f"{i + 1} {p + 1}\n" for i, p in enumerate(perm)))
+    write_input(b / "in.bin", sy, [], nins, seed, nframes=3)
+    r = subprocess.run([EXE, str(b / "in.bin"), str(b), str(b / "t.dcd")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert (a / "raw.bin").read_bytes() == (b / "raw.bin").read_bytes()
+    assert (a / "engref.tt").read_text() == (b / "engref.tt").read_text()
