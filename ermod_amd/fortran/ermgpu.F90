@@ -313,6 +313,38 @@ contains
     st = erm_read_permutation_c(trim(path) // c_null_char, natom_traj, perm0, found)
   end function erm_read_permutation
 
+  ! per-frame fluctuation output of engconst (engproc.F90:394-438) on an open
+  ! unit: for solution systems (slttype 1) flcuv.tt, the per-species sums of
+  ! the solute-solvent energy of every solute, '(i9, 9999f15.5)' with the
+  ! frame number, or '(2i9, 9999f15.5)' with the solute's index and the frame
+  ! number when there are several; for test particles progress.tt, the frame
+  ! number.  uvall(0:slvmax, nconf) as erm_engconst returns it; specofk(k) is
+  ! uvspec(tagpt(k)) (0 for the solute itself, whose energy is skipped).
+  subroutine erm_write_flcuv(unit, slttype, frame_no, nconf, slvmax, numslv, uvall, specofk)
+    integer, intent(in) :: unit, slttype, frame_no, nconf, slvmax, numslv
+    real(c_double), intent(in), optional :: uvall(0:slvmax, *)
+    integer(c_int32_t), intent(in), optional :: specofk(*)
+    real(c_double) :: flc(numslv)
+    integer :: cnt, k, pti
+    if (slttype /= 1) then
+       write(unit, *) frame_no
+       return
+    end if
+    do cnt = 1, nconf
+       flc(:) = 0.0d0
+       do k = 1, slvmax
+          pti = specofk(k)
+          if (pti <= 0) cycle
+          flc(pti) = flc(pti) + uvall(k, cnt)
+       end do
+       if (nconf == 1) then
+          write(unit, '(i9, 9999f15.5)') frame_no, flc
+       else
+          write(unit, '(2i9, 9999f15.5)') cnt, frame_no, flc
+       end if
+    end do
+  end subroutine erm_write_flcuv
+
   ! engstore's file output (engproc.F90:560-656), called after erm_engstore
   ! for division `division` of `engdiv`: engsln|engref.NN and, with corrcal,
   ! corsln|corref.NN, with selfcal slfeng.NN ('.tt' when engdiv = 1); at the

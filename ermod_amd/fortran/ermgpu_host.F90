@@ -1,8 +1,9 @@
 ! ermgpu_host.F90 -- Fortran host of the energy pipeline through the
 ! ISO_C_BINDING module ermgpu: the division loop of ERmod's trjmain
-! (enganal.F90:88-97) for a test-particle system -- per division engclear,
-! per frame set_frame + random insertions (engconst), engstore and the
-! reference's output files (erm_store_division, engproc.F90:560-656).
+! (enganal.F90:88-97) -- per division engclear, per frame set_frame and
+! engconst (random insertions of the test particle, or every solute molecule
+! of a solution system) with its flcuv.tt / progress.tt line, engstore and
+! the reference's output files (erm_store_division, engproc.F90:560-656).
 !
 ! usage: ermgpu_host <input stream file> <output directory> [trajectory.dcd]
 ! The input holds, in order and without padding: natom nmol ntype nframes
@@ -27,6 +28,9 @@ program ermgpu_host
   integer(c_int32_t), allocatable :: mol_begin(:), ljtype(:), sluvid(:), uvspec(:)
   real(c_double), allocatable :: charge(:), lj_eps(:), lj_sig2(:), mol_charge(:), sitemass(:), bfcoord(:)
   real(c_double), allocatable, target :: pos(:), uv(:, :), edens(:), ecorr(:, :), ercrd(:), escrd(:)
+  integer(c_int32_t), allocatable, target :: sl0(:)
+  integer(c_int32_t), allocatable :: specofk(:)
+  integer(c_int32_t) :: nslt, ncfg, k, m, uflc
   real(c_double), allocatable :: avediv(:, :), aveuv(:, :)
   real(c_double), target :: scal(4)
   real(c_double) :: cell(3, 3)
@@ -75,7 +79,28 @@ program ermgpu_host
   st = erm_set_bins(ctx, 1, [pemax], [pesoft], ercrd, esmax, escrd)
   call check(st, 'erm_set_bins')
   st = erm_sizes(ctx, slvmax, ermax)
-  allocate(uv(slvmax + 1, nins), edens(ermax), ecorr(ermax, ermax))
+  ! solutes (sltlist) and the species of every physical molecule (tagpt)
+  nslt = count(sluvid == 1)
+  allocate(sl0(max(nslt, 1)), specofk(slvmax))
+  k = 0
+  nslt = 0
+  do m = 1, nmol
+     if (sluvid(m) > 1) cycle
+     k = k + 1
+     specofk(k) = uvspec(m)
+     if (sluvid(m) == 1) then
+        nslt = nslt + 1
+        sl0(nslt) = m - 1
+     end if
+  end do
+  ncfg = nins
+  if (prm%slttype == 1) ncfg = nslt
+  allocate(uv(slvmax + 1, ncfg), edens(ermax), ecorr(ermax, ermax))
+  if (prm%slttype == 1) then
+     open(newunit=uflc, file=trim(outdir)//'/flcuv.tt', status='replace', action='write')
+  else
+     open(newunit=uflc, file=trim(outdir)//'/progress.tt', status='replace', action='write')
+  end if
   allocate(avediv(ndiv, 2), aveuv(ndiv, 1))
   if (use_dcd) then
      ! the trajectory holds every atom but the test particle's
@@ -109,17 +134,30 @@ program ermgpu_host
            st = erm_set_frame(ctx, pos, cell, 1.0d0)
            call check(st, 'erm_set_frame')
         end if
-        if (f == 1) then
-           st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_loc(uv))
+        if (prm%slttype == 1) then
+           st = erm_engconst(ctx, nslt, c_loc(sl0), c_null_ptr, c_null_ptr, c_loc(uv))
+           call check(st, 'erm_engconst')
+           call erm_write_flcuv(uflc, 1, f, int(nslt), int(slvmax), 1, uv, specofk)
         else
-           st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_null_ptr)
+           if (f == 1) then
+              st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_loc(uv))
+           else
+              st = erm_insert_random(ctx, nins, seed, int((f - 1) * nins, c_int64_t), c_null_ptr)
+           end if
+           call check(st, 'erm_insert_random')
+           call erm_write_flcuv(uflc, int(prm%slttype), f, 0, int(slvmax), 1)
         end if
-        call check(st, 'erm_insert_random')
      end do
      if (div == ndiv) then
         ! raw accumulators of the last division, before normalisation
-        st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
-                       c_loc(scal))
+        ecorr = 0.0d0
+        if (prm%corrcal /= 0) then
+           st = erm_fetch(ctx, c_loc(edens), c_loc(ecorr), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                          c_loc(scal))
+        else
+           st = erm_fetch(ctx, c_loc(edens), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                          c_loc(scal))
+        end if
         call check(st, 'erm_fetch')
         open(11, file=trim(outdir)//'/raw.bin', access='stream', form='unformatted', status='replace')
         write(11) slvmax, ermax, scal, edens, ecorr, uv
@@ -131,6 +169,8 @@ program ermgpu_host
           prm%temp, 1, [pemax], [pesoft], ercrd, int(esmax), escrd, avediv, aveuv, trim(outdir))
   end do
   close(10)
+  endfile(uflc)
+  close(uflc)
   if (use_dcd) call erm_traj_close(traj)
   call erm_destroy(ctx)
   print '(a,i0,a,i0,a,i0,a)', 'ermgpu_host: ', ndiv, ' divisions, ', nframes, ' frames x ', nins, &

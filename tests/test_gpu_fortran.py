@@ -32,10 +32,12 @@ def write_input(path, sy, frames, nins, seed, ndiv=1, s=None, nframes=None):
                             s.screen, s.temp, s.block_threshold, 0, 16, 0, s.cltype, s.boxshp))
         f.write(struct.pack("<9di", h.ecdmin, h.ecfmns, h.ecdcen, h.eccore, h.ecdmax, h.eclbin, h.ecfbin,
                             h.ec0bin, h.finfac, h.pecore))
+        # the host reads 3 x (sites of the last molecule) body-frame values
+        bf = s.bfcoord if s.bfcoord is not None else np.zeros((s.natom - s.mol_begin[-2], 3))
         for a, t in ((s.mol_begin, np.int32), (s.charge, np.float64), (s.ljtype, np.int32),
                      (s.lj_eps, np.float64), (s.lj_sig2, np.float64), (s.sluvid, np.int32),
                      (s.uvspec, np.int32), (s.mol_charge, np.float64), (s.sitemass, np.float64),
-                     (s.bfcoord, np.float64), (sy.cell.T, np.float64)):
+                     (bf, np.float64), (sy.cell.T, np.float64)):
             f.write(np.ascontiguousarray(a, dtype=t).tobytes())
         f.write(struct.pack("<q", seed))
         for fr in frames:
@@ -90,6 +92,8 @@ def test_fortran_host_end_to_end(tmp_path):
         else:
             wr = e[iduv]
         assert abs(rep - wr) <= 1e-6 * max(1.0, abs(wr))
+    prog = open(tmp_path / "progress.tt").read().split()
+    assert [int(v) for v in prog] == [1, 2, 3]
     rec = open(tmp_path / "corref.tt", "rb").read()
     n1 = struct.unpack("<i", rec[:4])[0]
     assert n1 == 8 * ermax * ermax and struct.unpack("<i", rec[-4:])[0] == n1
@@ -172,3 +176,42 @@ def test_fortran_host_frames_from_dcd(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert (a / "raw.bin").read_bytes() == (b / "raw.bin").read_bytes()
     assert (a / "engref.tt").read_text() == (b / "engref.tt").read_text()
+
+
+def test_fortran_host_solution(tmp_path):
+    """A solution system: every frame's solute through erm_engconst, one
+    flcuv.tt line per frame ('(i9, 9999f15.5)', the per-species sum of the
+    solute-solvent energy), engsln.tt, weight_soln and aveuv.tt."""
+    print("\n[gpu case] fortran host tiny_soln", flush=True)
+    sy = synth.tiny_soln()
+    s = sy.system
+    frames = [sy.frame(i) for i in range(3)]
+    inp = tmp_path / "in.bin"
+    write_input(inp, sy, frames, 0, 1)
+    r = subprocess.run([EXE, str(inp), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = O.Oracle(s)
+    m = sy.solute_mol
+    flc = []
+    for fr in frames:
+        o.set_frame(fr, sy.cell)
+        u = o.uv_energy(m)
+        o.update_histogram(m, u)
+        flc.append(u[1:].sum())
+    lines = open(tmp_path / "flcuv.tt").read().splitlines()
+    assert len(lines) == 3 and all(len(l) == 24 for l in lines)
+    for f, (line, want) in enumerate(zip(lines, flc), start=1):
+        assert int(line[:9]) == f
+        assert abs(float(line[9:]) - want) <= 5e-6 + 1e-12 * abs(want)
+    a = o.acc()
+    raw = open(tmp_path / "raw.bin", "rb").read()
+    slvmax, ermax = struct.unpack("<2i", raw[:8])
+    edens = np.frombuffer(raw, np.float64, ermax, 8 + 32)
+    assert np.array_equal(edens, a["edens"])
+    dens = np.array([float(l[20:]) for l in open(tmp_path / "engsln.tt").read().splitlines()])
+    np.testing.assert_allclose(dens, a["edens"] / a["engnorm"], rtol=1e-14, atol=1e-300)
+    w = open(tmp_path / "weight_soln").read().splitlines()
+    assert len(w) == 1 and int(w[0][:5]) == 1
+    aveuv = open(tmp_path / "aveuv.tt").read().splitlines()
+    assert len(aveuv) == 1 and int(aveuv[0][:5]) == 1
+    assert abs(float(aveuv[0][5:]) - a["slnuv"][1] / a["engnorm"]) <= 5e-6
