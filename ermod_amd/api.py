@@ -58,7 +58,8 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_insert_random", "erm_update_histogram", "erm_engstore", "erm_fetch", "erm_sizes",
            "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune",
            "erm_insert_random_multi", "erm_set_insertion", "erm_dcd_open", "erm_dcd_read", "erm_dcd_close",
-           "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close")
+           "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close", "erm_set_intermediate",
+           "erm_fetch_intermediate")
 
 _lib = None
 
@@ -106,6 +107,8 @@ def lib():
                                           C.c_int32, _f64p, _f64p, _f64p, _f64p]
         L.erm_set_insertion.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                         _i32p]
+        L.erm_set_intermediate.argtypes = [vp, _i32p]
+        L.erm_fetch_intermediate.argtypes = [vp, _f64p, _f64p, _f64p]
         L.erm_dcd_open.argtypes = [C.c_char_p, C.POINTER(vp), _i32p, _i32p]
         L.erm_dcd_read.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float), _f64p]
         L.erm_dcd_close.argtypes = [vp]
@@ -435,6 +438,20 @@ class ErmGPU:
                                               C.c_uint64(first_index), _p(w), int(cmbrule), _p(qq),
                                               _p(ee), _p(ss), _p(acc)), "solute_gradient")
         return acc[0], acc[1], acc[2:2 + ns], acc[2 + ns:2 + 2 * ns], acc[2 + 2 * ns:]
+
+    def set_intermediate(self, zerodns):
+        """The intermediate-state method (extensions/Intermediate/engproc.F90)
+        with the bins empty in the reference solvent marked; None: off."""
+        self._zerodns = None if zerodns is None else _i32(zerodns)
+        self._check(lib().erm_set_intermediate(self.h, _p(self._zerodns, _i32p)), "set_intermediate")
+
+    def fetch_intermediate(self):
+        """(edintm, uvintm, norm_intm, soln_intm, soln_total)."""
+        ed = np.zeros(self.ermax)
+        uv = np.zeros(self.system.numslv + 1)
+        sc = np.zeros(3)
+        self._check(lib().erm_fetch_intermediate(self.h, _p(ed), _p(uv), _p(sc)), "fetch_intermediate")
+        return ed, uv, sc[0], sc[1], sc[2]
 
     def set_insertion(self, insorigin, insposition, insorient=0, lwreg=0.0, upreg=0.0, hostspec=()):
         """Placement of device-generated poses (instslt, insertion.F90:103-366):

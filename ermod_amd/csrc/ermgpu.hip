@@ -333,6 +333,8 @@ struct erm_ctx {
   DBuf d_hostmol, d_aggcen;
   // trajectory frames: FP32 staging and the permutation
   DBuf d_tstage, d_tperm;
+  // intermediate state: empty reference bins, accumulators, weights
+  DBuf d_zerodns, d_edintm, d_uvintm, d_intm_sc, d_wi;
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
       d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
   DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
@@ -344,7 +346,8 @@ struct erm_ctx {
             &d_w3h, &d_args, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
             &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
-            &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm};
+            &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm, &d_hostmol, &d_aggcen, &d_tstage,
+            &d_tperm, &d_zerodns, &d_edintm, &d_uvintm, &d_intm_sc, &d_wi};
   }
 };
 
@@ -968,6 +971,21 @@ int run_histogram(erm_ctx *c, int n, int first) {
     hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, dp<double>(c->d_flceng),
                        dp<double>(c->d_w), dp<double>(c->d_slnuv));
     TRY(launch_check(c, "k_slnuv"));
+  }
+  if (k.intm) {
+    TRY(ensure(c, c->d_wi, sizeof(double) * n));
+    hipLaunchKernelGGL(k_intm_weight, dim3(n), dim3(256), 0, c->st, E, dp<double>(c->d_hist), dp<int>(c->d_zerodns),
+                       dp<double>(c->d_w), dp<double>(c->d_wi));
+    TRY(launch_check(c, "k_intm_weight"));
+    hipLaunchKernelGGL(k_edens, dim3((E + 255) / 256), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
+                       dp<double>(c->d_wi), dp<double>(c->d_edintm));
+    TRY(launch_check(c, "k_edens"));
+    hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, dp<double>(c->d_flceng),
+                       dp<double>(c->d_wi), dp<double>(c->d_uvintm));
+    TRY(launch_check(c, "k_slnuv"));
+    hipLaunchKernelGGL(k_intm_scal, dim3(1), dim3(1), 0, c->st, n, dp<double>(c->d_w), dp<double>(c->d_wi),
+                       dp<double>(c->d_intm_sc));
+    TRY(launch_check(c, "k_intm_scal"));
   }
   TRY(mark(c, 6));
   if (c->P.corrcal) {
@@ -1718,6 +1736,13 @@ int32_t erm_engclear(erm_ctx *c) {
   HIPCHK(c, hipMemsetAsync(c->d_slnuv.p, 0, sizeof(double) * (c->numslv + 1), c->st));
   // avslf, engnorm, engsmpl (voffset persists across divisions)
   HIPCHK(c, hipMemsetAsync(c->d_scal.p, 0, sizeof(double) * 3, c->st));
+  if (c->kp.intm) {
+    // edintm, uvintm and norm_intm; soln_intm and soln_total run over all
+    // divisions (extensions/Intermediate/engproc.F90:348-352)
+    HIPCHK(c, hipMemsetAsync(c->d_edintm.p, 0, sizeof(double) * c->ermax, c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_uvintm.p, 0, sizeof(double) * (MAXSPEC + 1), c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_intm_sc.p, 0, sizeof(double), c->st));
+  }
   return sync(c);
 }
 
@@ -2016,6 +2041,47 @@ int32_t erm_engstore(erm_ctx *c) {
   }
   sc[2] = sc[2] / en;
   TRY(upload(c, c->d_scal, sc.data(), 8));
+  if (c->kp.intm) {
+    // edintm / norm_intm (extensions/Intermediate/engproc.F90:656)
+    std::vector<double> ed(E), isc(3);
+    HIPCHK(c, hipMemcpyAsync(ed.data(), c->d_edintm.p, sizeof(double) * E, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(isc.data(), c->d_intm_sc.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c->st));
+    TRY(sync(c));
+    for (double &v : ed) v = v / isc[0];
+    TRY(upload(c, c->d_edintm, ed.data(), E));
+  }
+  return sync(c);
+}
+
+int32_t erm_set_intermediate(erm_ctx *c, const int32_t *zerodns) {
+  TRY(need(c, false, true));
+  if (c->P.slttype != 1) return fail(c, ERM_E_ARG, "the intermediate state applies to solution systems");
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (!zerodns) {
+    c->kp.intm = 0;
+    return ERM_OK;
+  }
+  const int E = c->ermax;
+  std::vector<int> z(zerodns, zerodns + E);
+  for (int &v : z) v = v ? 1 : 0;
+  TRY(upload(c, c->d_zerodns, z.data(), E));
+  std::vector<double> zero(std::max(E, MAXSPEC + 1), 0.0);
+  TRY(upload(c, c->d_edintm, zero.data(), E));
+  TRY(upload(c, c->d_uvintm, zero.data(), MAXSPEC + 1));
+  TRY(upload(c, c->d_intm_sc, zero.data(), 3));
+  c->kp.intm = 1;
+  return sync(c);
+}
+
+int32_t erm_fetch_intermediate(erm_ctx *c, double *edintm, double *uvintm, double *scal) {
+  TRY(need(c, false, true));
+  if (!c->kp.intm) return fail(c, ERM_E_ARG, "the intermediate state is off");
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (edintm)
+    HIPCHK(c, hipMemcpyAsync(edintm, c->d_edintm.p, sizeof(double) * c->ermax, hipMemcpyDeviceToHost, c->st));
+  if (uvintm)
+    HIPCHK(c, hipMemcpyAsync(uvintm, c->d_uvintm.p, sizeof(double) * (c->numslv + 1), hipMemcpyDeviceToHost, c->st));
+  if (scal) HIPCHK(c, hipMemcpyAsync(scal, c->d_intm_sc.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c->st));
   return sync(c);
 }
 

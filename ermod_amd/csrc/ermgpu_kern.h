@@ -41,6 +41,8 @@ struct Params {
   // region bounds lwreg / upreg (engmain.F90:320-327)
   int ins_org, ins_pos, ins_rot;
   double lwreg, upreg;
+  // intermediate-state accumulation on (solution systems)
+  int intm;
 };
 
 // --------------------------------------------------------------------------
@@ -1662,6 +1664,40 @@ __global__ __launch_bounds__(256) void k_edens(int n, int ermax, const double *_
       if (h[u] != 0.0) e = e + w[b0 + u] * h[u];
   }
   edens[i] = e;
+}
+
+// intermediate state (extensions/Intermediate/engproc.F90:996-1009): the
+// weight of configuration b if its instantaneous histogram touches no bin
+// that is empty in the reference solvent, else 0; one workgroup per b
+__global__ __launch_bounds__(256) void k_intm_weight(int ermax, const double *__restrict__ hist,
+                                                     const int *__restrict__ zerodns, const double *__restrict__ w,
+                                                     double *wi) {
+  __shared__ int hit;
+  const int b = blockIdx.x;
+  if (threadIdx.x == 0) hit = 0;
+  __syncthreads();
+  const double *h = hist + (size_t)b * ermax;
+  int my = 0;
+  for (int i = threadIdx.x; i < ermax; i += blockDim.x) my |= (h[i] > 0.0 && zerodns[i]) ? 1 : 0;
+  if (my) atomicOr(&hit, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) wi[b] = hit ? 0.0 : w[b];
+}
+
+// soln_total += engnmfc; soln_intm and norm_intm += the intermediate weight
+// (adding the 0 of an excluded configuration leaves the sum unchanged), in
+// configuration order; sc = {norm_intm, soln_intm, soln_total}
+__global__ void k_intm_scal(int n, const double *__restrict__ w, const double *__restrict__ wi, double *sc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double a = sc[0], i = sc[1], t = sc[2];
+  for (int b = 0; b < n; ++b) {
+    t = t + w[b];
+    i = i + wi[b];
+    a = a + wi[b];
+  }
+  sc[0] = a;
+  sc[1] = i;
+  sc[2] = t;
 }
 
 // slnuv(:) += flceng(:, cntdst) * engnmfc (engproc.F90:832-834)

@@ -235,6 +235,20 @@ module ermgpu
        type(c_ptr), value :: solutes, solute_pos, weights, q, eps, sigma, acc
      end function erm_solute_gradient
 
+     ! the intermediate-state method (extensions/Intermediate/engproc.F90)
+     integer(c_int32_t) function erm_set_intermediate(ctx, zerodns) bind(C, name="erm_set_intermediate")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: zerodns(*)
+     end function erm_set_intermediate
+
+     integer(c_int32_t) function erm_fetch_intermediate(ctx, edintm, uvintm, scal) &
+          bind(C, name="erm_fetch_intermediate")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: edintm(*), uvintm(*), scal(3)
+     end function erm_fetch_intermediate
+
      ! frame ingestion (getconf_parallel, setconf.F90:860-996; dcd.f90)
      integer(c_int32_t) function erm_traj_open_c(ctx, dcd_path, natom_traj, perm, weight_path, skpcnf, traj) &
           bind(C, name="erm_traj_open")
@@ -344,6 +358,108 @@ contains
        end if
     end do
   end subroutine erm_write_flcuv
+
+  ! the bins empty in the reference solvent (extensions/Intermediate/
+  ! engproc.F90:286-315): refs_filename.tt, or every existing .01 ... .99
+  ! file, each of ermax lines whose third value is the density; a bin is
+  ! empty unless some file gives it more than 1e-30.  Returns ERM_E_ARG when
+  ! no file exists or one is short.
+  integer(c_int32_t) function erm_read_zerodns(refs_filename, ermax, zerodns) result(st)
+    character(*), intent(in) :: refs_filename
+    integer, intent(in) :: ermax
+    integer(c_int32_t), intent(out) :: zerodns(ermax)
+    character(len=len_trim(refs_filename) + 3) :: fname
+    character(len=3) :: suf
+    logical :: ex
+    integer :: nfile, k, iduv, u, idum, ios, nread
+    real(c_double) :: factor
+    st = ERM_OK
+    zerodns(:) = 1
+    fname = trim(refs_filename) // '.tt'
+    inquire(file=fname, exist=ex)
+    nfile = 99
+    if (ex) nfile = 1
+    nread = 0
+    do k = 1, nfile
+       if (nfile > 1) then
+          write(suf, '(a1,i2.2)') '.', k
+          fname = trim(refs_filename) // suf
+          inquire(file=fname, exist=ex)
+       end if
+       if (.not. ex) cycle
+       nread = nread + 1
+       open(newunit=u, file=fname, action='read')
+       do iduv = 1, ermax
+          read(u, *, iostat=ios) factor, idum, factor
+          if (ios /= 0) then
+             st = ERM_E_ARG
+             close(u)
+             return
+          end if
+          if (factor > 1.0d-30) zerodns(iduv) = 0
+       end do
+       close(u)
+    end do
+    if (nread == 0) st = ERM_E_ARG
+  end function erm_read_zerodns
+
+  ! the intermediate state's output of engstore (extensions/Intermediate/
+  ! engproc.F90:655-694), after erm_engstore: engint.NN ('.tt' for one
+  ! division) in the engsln format; at the last division Prob_Result (the
+  ! probability soln_intm / soln_total and temp * log of it) and avint.tt.
+  ! avintm(engdiv, numslv) carries the per-division averages across calls.
+  subroutine erm_store_intermediate(ctx, division, engdiv, temp, numslv, uvmax, uvsoft, uvcrd, avintm, dir)
+    use iso_c_binding, only: c_loc
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: division, engdiv, numslv
+    integer(c_int32_t), intent(in) :: uvmax(numslv), uvsoft(numslv)
+    real(c_double), intent(in) :: temp, uvcrd(*)
+    real(c_double), intent(inout) :: avintm(engdiv, numslv)
+    character(*), intent(in) :: dir
+    real(c_double), allocatable :: edintm(:), uvintm(:)
+    real(c_double) :: sc(3), factor
+    integer(c_int32_t) :: slvmax, ermax, st
+    character(len=3) :: suff
+    integer :: u, iduv, idmin, pti, k
+    st = erm_sizes(ctx, slvmax, ermax)
+    allocate(edintm(ermax), uvintm(0:numslv))
+    st = erm_fetch_intermediate(ctx, edintm, uvintm, sc)
+    if (engdiv == 1) then
+       suff = '.tt'
+    else
+       write(suff, '(a1,i2.2)') '.', mod(division, 100)
+    end if
+    open(newunit=u, file=dir//'/engint'//suff, form='formatted', action='write')
+    idmin = 0
+    pti = 1
+    do iduv = 1, ermax
+       do while (iduv > idmin + uvmax(pti))
+          idmin = idmin + uvmax(pti)
+          pti = pti + 1
+       end do
+       factor = erm_repval(uvcrd(idmin + 1), uvmax(pti), uvsoft(pti), iduv - idmin)
+       write(u, '(g15.7,i5,g25.15)') factor, pti, edintm(iduv)
+    end do
+    endfile(u)
+    close(u)
+    avintm(division, 1:numslv) = uvintm(1:numslv) / sc(1)
+    if (division == engdiv) then
+       factor = sc(2) / sc(3)
+       open(newunit=u, file=dir//'/Prob_Result', action='write')
+       write(u, '(a,f12.6)') ' Probability to find the system in the intermediate state = ', factor
+       write(u, '(a)') ' Free-energy change from the intermediate state to the simulated state'
+       write(u, '(a,f12.6,a)') '                                                = ', temp * log(factor), &
+            ' kcal/mol'
+       endfile(u)
+       close(u)
+       open(newunit=u, file=dir//'/avint.tt', action='write')
+       do k = 1, engdiv
+          write(u, '(i5, 9999f15.5)') k, avintm(k, 1:numslv)
+       end do
+       endfile(u)
+       close(u)
+    end if
+  end subroutine erm_store_intermediate
 
   ! engstore's file output (engproc.F90:560-656), called after erm_engstore
   ! for division `division` of `engdiv`: engsln|engref.NN and, with corrcal,

@@ -16,6 +16,8 @@
 ! of 3*natom coordinates, split evenly over the ndiv divisions.  With a DCD
 ! file the frames (and their cells) come from it instead, through
 ! erm_traj_*, with <output directory>/PermIndex and SysWght when present.
+! For a solution system, a file <output directory>/intermediate holding the
+! reference's refs_filename switches the intermediate-state method on.
 program ermgpu_host
   use iso_c_binding
   use ermgpu
@@ -31,7 +33,10 @@ program ermgpu_host
   integer(c_int32_t), allocatable, target :: sl0(:)
   integer(c_int32_t), allocatable :: specofk(:)
   integer(c_int32_t) :: nslt, ncfg, k, m, uflc
-  real(c_double), allocatable :: avediv(:, :), aveuv(:, :)
+  real(c_double), allocatable :: avediv(:, :), aveuv(:, :), avintm(:, :)
+  integer(c_int32_t), allocatable :: zerodns(:)
+  character(len=1024) :: refsname
+  logical :: intm
   real(c_double), target :: scal(4)
   real(c_double) :: cell(3, 3)
   integer(c_int64_t) :: seed
@@ -101,7 +106,19 @@ program ermgpu_host
   else
      open(newunit=uflc, file=trim(outdir)//'/progress.tt', status='replace', action='write')
   end if
-  allocate(avediv(ndiv, 2), aveuv(ndiv, 1))
+  allocate(avediv(ndiv, 2), aveuv(ndiv, 1), avintm(ndiv, 1))
+  inquire(file=trim(outdir)//'/intermediate', exist=intm)
+  intm = intm .and. prm%slttype == 1
+  if (intm) then
+     open(12, file=trim(outdir)//'/intermediate', action='read')
+     read(12, '(a)') refsname
+     close(12)
+     allocate(zerodns(ermax))
+     st = erm_read_zerodns(trim(refsname), int(ermax), zerodns)
+     call check(st, 'erm_read_zerodns')
+     st = erm_set_intermediate(ctx, zerodns)
+     call check(st, 'erm_set_intermediate')
+  end if
   if (use_dcd) then
      ! the trajectory holds every atom but the test particle's
      ntraj = natom - ns
@@ -167,6 +184,8 @@ program ermgpu_host
      call check(st, 'erm_engstore')
      call erm_store_division(ctx, div, int(ndiv), int(prm%slttype), int(prm%wgtslf), int(prm%corrcal), &
           prm%temp, 1, [pemax], [pesoft], ercrd, int(esmax), escrd, avediv, aveuv, trim(outdir))
+     if (intm) call erm_store_intermediate(ctx, div, int(ndiv), prm%temp, 1, [pemax], [pesoft], ercrd, &
+          avintm, trim(outdir))
   end do
   close(10)
   endfile(uflc)

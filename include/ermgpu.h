@@ -152,6 +152,19 @@ int32_t erm_engclear(erm_ctx *ctx);
 int32_t erm_set_insertion(erm_ctx *ctx, int32_t insorigin, int32_t insposition, int32_t insorient,
                           double lwreg, double upreg, int32_t nhost, const int32_t *hostspec);
 
+/* The intermediate-state method of extensions/Intermediate/engproc.F90
+ * (solution systems): zerodns[ermax] != 0 marks the bins whose density is
+ * at most 1e-30 in every reference-solvent engref file (lines 286-315);
+ * NULL switches it off.  Each configuration whose instantaneous histogram
+ * touches none of them adds its weight to soln_intm and norm_intm, its
+ * histogram to edintm and its per-species energy sums to uvintm; every
+ * configuration adds to soln_total (lines 996-1009).  erm_engclear clears
+ * edintm, uvintm and norm_intm, erm_engstore divides edintm by norm_intm. */
+int32_t erm_set_intermediate(erm_ctx *ctx, const int32_t *zerodns);
+/* edintm[ermax], uvintm[numslv+1] ([0] unused), scal = {norm_intm,
+ * soln_intm, soln_total}; any pointer may be NULL. */
+int32_t erm_fetch_intermediate(erm_ctx *ctx, double *edintm, double *uvintm, double *scal);
+
 /* ---- frame ingestion ----------------------------------------------------
  * DCD trajectories as trajectory-formats/dcd.f90:39-105 reads them (Fortran
  * unformatted records, 'CORD' header, optional unit-cell record, FP32 x, y, z

@@ -83,6 +83,8 @@ def lib():
         L.orc_aggregate_center.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.orc_insertion_pose.argtypes = [C.c_int32, _f64p, _f64p, _f64p, C.POINTER(OrcInsertion), _f64p,
                                          C.c_uint64, C.c_uint64, _f64p]
+        L.orc_set_intermediate.argtypes = [vp, _i32p]
+        L.orc_get_intermediate.argtypes = [vp, _f64p, _f64p, _f64p]
         L.orc_gfb_pppm.argtypes = [C.c_int32, _f64p]
         L.orc_pppm_greenfunc.argtypes = [_i32p, C.c_int32, _f64p, _f64p, C.c_double, C.c_double, _f64p]
         L.orc_realcal_bare.argtypes = [vp, C.c_int32, _f64p, _f64p]
@@ -375,6 +377,18 @@ class Oracle:
         check(lib().orc_update_histogram(self.h, int(solute), float(weight), P(u), P(ins, _i32p)),
               "update_histogram")
         return ins
+
+    def set_intermediate(self, zerodns):
+        self._zerodns = None if zerodns is None else i32(zerodns)
+        check(lib().orc_set_intermediate(self.h, P(self._zerodns, _i32p)), "set_intermediate")
+
+    def intermediate(self):
+        """(edintm, uvintm, norm_intm, soln_intm, soln_total)."""
+        ed = np.zeros(self.ermax)
+        uv = np.zeros(self.system.numslv + 1)
+        sc = np.zeros(3)
+        check(lib().orc_get_intermediate(self.h, P(ed), P(uv), P(sc)), "get_intermediate")
+        return ed, uv, sc[0], sc[1], sc[2]
 
     def engclear(self):
         lib().orc_engclear(self.h)

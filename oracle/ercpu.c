@@ -801,6 +801,9 @@ struct orc_ctx {
   double engnorm, engsmpl, avslf, voffset;
   int32_t voffset_init, warn_bin_first;
   int32_t *insdst;
+  /* intermediate state (extensions/Intermediate/engproc.F90:34-39) */
+  int32_t *zerodns;
+  double *edintm, *uvintm, norm_intm, soln_intm, soln_total;
   /* realcal_self cache (engproc.F90:712-730) */
   uint64_t solute_hash;
   double usreal;
@@ -935,7 +938,7 @@ void orc_destroy(orc_ctx *c) {
                   c->lj_sig2, c->mol_charge, c->uvcrd, c->escrd, c->tagpt, c->kof, c->slvtag, c->splslv,
                   c->grdslv, c->splfc[0], c->splfc[1], c->splfc[2], c->engfac, c->cnv, c->rcp, c->pos,
                   c->posn, c->spos, c->sposn, c->edens, c->ecorr, c->eself, c->minuv, c->maxuv, c->slnuv,
-                  c->flceng, c->insdst};
+                  c->flceng, c->insdst, c->zerodns, c->edintm, c->uvintm};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(c);
 }
@@ -949,6 +952,12 @@ void orc_engclear(orc_ctx *c) {
   c->avslf = 0.0;
   c->engnorm = 0.0;
   c->engsmpl = 0.0;
+  /* soln_intm and soln_total run over all divisions */
+  if (c->zerodns) {
+    memset(c->edintm, 0, sizeof(double) * c->ermax);
+    memset(c->uvintm, 0, sizeof(double) * (c->s.numslv + 1));
+    c->norm_intm = 0.0;
+  }
 }
 
 static double wrap1(double x, double L) { return x - L * floor(x * (1.0 / L)); }
@@ -1757,7 +1766,50 @@ int32_t orc_update_histogram(orc_ctx *c, int32_t solute, double stat_weight_solu
       }
     }
   }
+  /* the intermediate state: configurations with no molecule in a bin that
+   * is empty in the reference solvent (extensions/Intermediate/
+   * engproc.F90:996-1009) */
+  if (c->zerodns) {
+    c->soln_total = c->soln_total + engnmfc;
+    int hit = 0;
+    for (int i = 0; i < c->ermax; ++i)
+      if (c->insdst[i] > 0 && c->zerodns[i]) hit = 1;
+    if (!hit) {
+      c->soln_intm = c->soln_intm + engnmfc;
+      c->norm_intm = c->norm_intm + engnmfc;
+      for (int i = 0; i < c->ermax; ++i)
+        if (c->insdst[i] > 0) c->edintm[i] = c->edintm[i] + engnmfc * (double)c->insdst[i];
+      for (int i = 1; i <= s->numslv; ++i) c->uvintm[i] = c->uvintm[i] + c->flceng[i] * engnmfc;
+    }
+  }
   if (insdst) memcpy(insdst, c->insdst, sizeof(int32_t) * c->ermax);
+  return ORC_OK;
+}
+
+int32_t orc_set_intermediate(orc_ctx *c, const int32_t *zerodns) {
+  if (c->s.slttype != 1) return ORC_E_ARG;
+  free(c->zerodns);
+  free(c->edintm);
+  free(c->uvintm);
+  c->zerodns = NULL;
+  c->edintm = c->uvintm = NULL;
+  if (!zerodns) return ORC_OK;
+  c->zerodns = dupi(zerodns, c->ermax);
+  c->edintm = (double *)calloc(c->ermax ? c->ermax : 1, sizeof(double));
+  c->uvintm = (double *)calloc(c->s.numslv + 1, sizeof(double));
+  c->norm_intm = c->soln_intm = c->soln_total = 0.0;
+  return ORC_OK;
+}
+
+int32_t orc_get_intermediate(const orc_ctx *c, double *edintm, double *uvintm, double *scal) {
+  if (!c->zerodns) return ORC_E_ARG;
+  if (edintm) memcpy(edintm, c->edintm, sizeof(double) * c->ermax);
+  if (uvintm) memcpy(uvintm, c->uvintm, sizeof(double) * (c->s.numslv + 1));
+  if (scal) {
+    scal[0] = c->norm_intm;
+    scal[1] = c->soln_intm;
+    scal[2] = c->soln_total;
+  }
   return ORC_OK;
 }
 
@@ -1769,6 +1821,8 @@ int32_t orc_engstore(orc_ctx *c) {
   if (c->eself)
     for (int i = 0; i < c->s.esmax; ++i) c->eself[i] = c->eself[i] / c->engnorm;
   c->avslf = c->avslf / c->engnorm;
+  if (c->zerodns)
+    for (int i = 0; i < c->ermax; ++i) c->edintm[i] = c->edintm[i] / c->norm_intm;
   return ORC_OK;
 }
 

@@ -191,6 +191,14 @@ typedef struct orc_acc {
   int32_t ermax, esmax, numslv, voffset_initialized;
 } orc_acc;
 void orc_get_acc(orc_ctx *c, orc_acc *out);
+/* The intermediate-state method (extensions/Intermediate/engproc.F90,
+ * solution systems): zerodns[ermax] marks the bins empty in the reference
+ * solvent (NULL switches it off).  Accumulates edintm, uvintm (numslv+1,
+ * [0] unused), norm_intm (per division) and soln_intm, soln_total (over
+ * all divisions); scal = {norm_intm, soln_intm, soln_total}.  engstore
+ * divides edintm by norm_intm. */
+int32_t orc_set_intermediate(orc_ctx *c, const int32_t *zerodns);
+int32_t orc_get_intermediate(const orc_ctx *c, double *edintm, double *uvintm, double *scal);
 /* per-species sum of solute-solvent energy of the last update (flceng) */
 const double *orc_flceng(const orc_ctx *c);
 

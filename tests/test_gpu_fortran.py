@@ -215,3 +215,49 @@ def test_fortran_host_solution(tmp_path):
     aveuv = open(tmp_path / "aveuv.tt").read().splitlines()
     assert len(aveuv) == 1 and int(aveuv[0][:5]) == 1
     assert abs(float(aveuv[0][5:]) - a["slnuv"][1] / a["engnorm"]) <= 5e-6
+
+
+def test_fortran_host_intermediate(tmp_path):
+    """The intermediate-state method through the Fortran host: zerodns from
+    a reference engref.tt, engint.NN per division, Prob_Result and avint.tt
+    against the oracle."""
+    print("\n[gpu case] fortran host tiny_soln intermediate state", flush=True)
+    sy = synth.tiny_soln()
+    s = sy.system
+    m = sy.solute_mol
+    frames = [sy.frame(i) for i in range(4)]
+    o0 = O.Oracle(s)
+    occ = 0
+    for fr in frames:
+        o0.set_frame(fr, sy.cell)
+        occ = occ + (o0.update_histogram(m, o0.uv_energy(m)) > 0)
+    mask = np.zeros(o0.ermax, np.int32)
+    mask[np.nonzero((occ >= 1) & (occ <= 3))[0][0]] = 1
+    refs = tmp_path / "refs"
+    refs.mkdir()
+    with open(refs / "engref.tt", "w") as f:
+        for iduv in range(o0.ermax):
+            f.write(f"{0.0:15.7g}{1:5d}{0.0 if mask[iduv] else 0.25:25.15g}\n")
+    (tmp_path / "intermediate").write_text(str(refs / "engref") + "\n")
+    inp = tmp_path / "in.bin"
+    write_input(inp, sy, frames, 0, 1, ndiv=2)
+    r = subprocess.run([EXE, str(inp), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = O.Oracle(s)
+    o.set_intermediate(mask)
+    avint = []
+    for div in range(2):
+        o.engclear()
+        for fr in frames[2 * div:2 * div + 2]:
+            o.set_frame(fr, sy.cell)
+            o.update_histogram(m, o.uv_energy(m))
+        ed, uv, norm, sint, stot = o.intermediate()
+        dens = np.array([float(l[20:]) for l in open(tmp_path / f"engint.{div + 1:02d}").read().splitlines()])
+        np.testing.assert_allclose(dens, ed / norm, rtol=1e-14, atol=1e-300)
+        avint.append(uv[1] / norm)
+    prob = open(tmp_path / "Prob_Result").read().splitlines()
+    assert abs(float(prob[0].split("=")[1]) - sint / stot) <= 5e-7
+    av = open(tmp_path / "avint.tt").read().splitlines()
+    assert len(av) == 2
+    for line, want in zip(av, avint):
+        assert abs(float(line[5:]) - want) <= 5e-6
