@@ -21,7 +21,8 @@ LIBPATH = os.path.join(HERE, "libermgpu.so")
 
 ERM_FP64, ERM_FP32 = 0, 1
 STATUS = {0: "ok", 1: "eng_min", 2: "eng_sft", 3: "mbin", 4: "eng_ecd", 5: "rcp_cns", 6: "noncuboid",
-          7: "eng_bug", 8: "eng_cns", 9: "eng_siz", 10: "arg", 11: "unsupported", 12: "device", 13: "eof"}
+          7: "eng_bug", 8: "eng_cns", 9: "eng_siz", 10: "arg", 11: "unsupported", 12: "device", 13: "eof",
+          14: "ins_limit"}
 ERM_E_EOF = 13
 STAGES = ("prep", "pme_z", "pme_y", "pme_x_gather", "pair", "bin", "corr", "total")
 
@@ -59,7 +60,7 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune",
            "erm_insert_random_multi", "erm_set_insertion", "erm_dcd_open", "erm_dcd_read", "erm_dcd_close",
            "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close", "erm_set_intermediate",
-           "erm_fetch_intermediate")
+           "erm_fetch_intermediate", "erm_set_refstructure")
 
 _lib = None
 
@@ -107,6 +108,7 @@ def lib():
                                           C.c_int32, _f64p, _f64p, _f64p, _f64p]
         L.erm_set_insertion.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                         _i32p]
+        L.erm_set_refstructure.argtypes = [vp, C.c_int32, _i32p, _f64p, _f64p, _f64p, _f64p]
         L.erm_set_intermediate.argtypes = [vp, _i32p]
         L.erm_fetch_intermediate.argtypes = [vp, _f64p, _f64p, _f64p]
         L.erm_dcd_open.argtypes = [C.c_char_p, C.POINTER(vp), _i32p, _i32p]
@@ -438,6 +440,16 @@ class ErmGPU:
                                               C.c_uint64(first_index), _p(w), int(cmbrule), _p(qq),
                                               _p(ee), _p(ss), _p(acc)), "solute_gradient")
         return acc[0], acc[1], acc[2:2 + ns], acc[2 + ns:2 + 2 * ns], acc[2 + 2 * ns:]
+
+    def set_refstructure(self, host_atoms, host_crd, host_weight, slt_crd, slt_weight):
+        """The reference structure of the REFSTR insertion modes
+        (load_refstructure's arrays, insertion.F90:623-723): 0-based host
+        atoms with reference coordinates and weights, the test particle's
+        reference coordinates and weights."""
+        ha = _i32(host_atoms)
+        self._ref = [ha, _f64(host_crd), _f64(host_weight), _f64(slt_crd), _f64(slt_weight)]
+        self._check(lib().erm_set_refstructure(self.h, len(ha), _p(ha, _i32p), *[_p(x) for x in self._ref[1:]]),
+                    "set_refstructure")
 
     def set_intermediate(self, zerodns):
         """The intermediate-state method (extensions/Intermediate/engproc.F90)

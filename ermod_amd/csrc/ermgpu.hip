@@ -201,6 +201,124 @@ bool pppm_greenfunc(const int *K, int p, const double *cell, const double *invcl
   return true;
 }
 
+// ---- best fit (bestfit.F90): Horn's quaternion superposition ----------
+// eigenvector of the largest eigenvalue of a symmetric 4x4 matrix (cyclic
+// Jacobi rotations; the reference calls LAPACK dsyev)
+void sym4_top_eigvec(const double A_in[4][4], double v[4]) {
+  double A[4][4], V[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      A[i][j] = A_in[i][j];
+      V[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < 4; ++p)
+      for (int q = p + 1; q < 4; ++q) off += A[p][q] * A[p][q];
+    if (off == 0.0) break;
+    for (int p = 0; p < 4; ++p)
+      for (int q = p + 1; q < 4; ++q) {
+        if (A[p][q] == 0.0) continue;
+        const double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < 4; ++k) {
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = cs * akp - sn * akq;
+          A[k][q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < 4; ++k) {
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = cs * apk - sn * aqk;
+          A[q][k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < 4; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = cs * vkp - sn * vkq;
+          V[k][q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  int best = 0;
+  for (int i = 1; i < 4; ++i)
+    if (A[i][i] > A[best][best]) best = i;
+  for (int i = 0; i < 4; ++i) v[i] = V[i][best];
+}
+
+// find_rotation_quaternion (bestfit.F90:92-146) for n points (3 x n,
+// COM-centred) and weights
+void fit_quaternion(int n, const double *ref, const double *mov, const double *w, double q[4]) {
+  double ip[3][3];  // ip[j][i] = sum ref(i,:) mov(j,:) w
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc = acc + ref[3 * k + i] * mov[3 * k + j] * w[k];
+      ip[j][i] = acc;
+    }
+  double M[4][4];
+  M[0][0] = ip[0][0] + ip[1][1] + ip[2][2];
+  M[1][1] = ip[0][0] - ip[1][1] - ip[2][2];
+  M[2][2] = -ip[0][0] + ip[1][1] - ip[2][2];
+  M[3][3] = -ip[0][0] - ip[1][1] + ip[2][2];
+  M[0][1] = ip[1][2] - ip[2][1];
+  M[0][2] = ip[2][0] - ip[0][2];
+  M[0][3] = ip[0][1] - ip[1][0];
+  M[1][2] = ip[0][1] + ip[1][0];
+  M[2][3] = ip[1][2] + ip[2][1];
+  M[1][3] = ip[2][0] + ip[0][2];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < i; ++j) M[i][j] = M[j][i];
+  sym4_top_eigvec(M, q);
+}
+
+void com_weighted(int n, const double *x, const double *w, double c[3]) {
+  double s = 0.0;
+  for (int k = 0; k < n; ++k) s = s + w[k];
+  for (int i = 0; i < 3; ++i) {
+    double d = 0.0;
+    for (int k = 0; k < n; ++k) d = d + x[3 * k + i] * w[k];
+    c[i] = d / s;
+  }
+}
+
+void qrot(const double *v, const double *q, double *out) {
+  const double a[4] = {q[0], q[1], q[2], q[3]};
+  const double b[4] = {0.0, v[0], v[1], v[2]};
+  auto prod = [](const double *x, const double *y, double *p) {
+    double dot = x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+    p[0] = x[0] * y[0] - dot;
+    double cr[3] = {x[2] * y[3] - x[3] * y[2], x[3] * y[1] - x[1] * y[3], x[1] * y[2] - x[2] * y[1]};
+    for (int i = 0; i < 3; ++i) p[1 + i] = cr[i] + x[0] * y[1 + i] + y[0] * x[1 + i];
+  };
+  double t[4], r[4];
+  const double cj[4] = {q[0], -q[1], -q[2], -q[3]};
+  prod(a, b, t);
+  prod(t, cj, r);
+  for (int i = 0; i < 3; ++i) out[i] = r[1 + i];
+}
+
+// fit_a_rotate_b (bestfit.F90:214-241): the rotation that fits a onto refa
+// (weights w, both COM-centred first) applied to b around a's COM, placed
+// at refa's COM
+void fit_a_rotate_b(int na, const double *refa, const double *a, const double *w, int nb, const double *b,
+                    double *bout) {
+  std::vector<double> wr(refa, refa + 3 * na), wa(a, a + 3 * na);
+  double cr[3], ca[3];
+  com_weighted(na, wr.data(), w, cr);
+  for (int k = 0; k < na; ++k)
+    for (int i = 0; i < 3; ++i) wr[3 * k + i] = wr[3 * k + i] - cr[i];
+  com_weighted(na, wa.data(), w, ca);
+  for (int k = 0; k < na; ++k)
+    for (int i = 0; i < 3; ++i) wa[3 * k + i] = wa[3 * k + i] - ca[i];
+  double q[4];
+  fit_quaternion(na, wr.data(), wa.data(), w, q);
+  for (int k = 0; k < nb; ++k) {
+    double d[3] = {b[3 * k] - ca[0], b[3 * k + 1] - ca[1], b[3 * k + 2] - ca[2]}, r[3];
+    qrot(d, q, r);
+    for (int i = 0; i < 3; ++i) bout[3 * k + i] = r[i] + cr[i];
+  }
+}
+
 // get_inverted_cell (engproc.F90:898-919), column-major
 void inverted_cell(const double *c, double *inv, double *vol) {
 #define M(i, j) c[3 * ((j)-1) + (i)-1]
@@ -335,6 +453,13 @@ struct erm_ctx {
   DBuf d_tstage, d_tperm;
   // intermediate state: empty reference bins, accumulators, weights
   DBuf d_zerodns, d_edintm, d_uvintm, d_intm_sc, d_wi;
+  // reference structure (RefInfo, insertion.F90:623-723): host atoms and
+  // their reference coordinates and weights, the solute's reference
+  // coordinates and weights; per frame the fitted and best-fit solute
+  bool have_ref = false, gauss_init = false;
+  std::vector<int> ref_atoms;
+  std::vector<double> ref_host, ref_hw, ref_slt, ref_sw;
+  DBuf d_refatoms, d_refgather, d_refsite, d_refw;
   DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
       d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
   DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
@@ -347,7 +472,8 @@ struct erm_ctx {
             &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
             &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm, &d_hostmol, &d_aggcen, &d_tstage,
-            &d_tperm, &d_zerodns, &d_edintm, &d_uvintm, &d_intm_sc, &d_wi};
+            &d_tperm, &d_zerodns, &d_edintm, &d_uvintm, &d_intm_sc, &d_wi, &d_refatoms, &d_refgather,
+            &d_refsite, &d_refw};
   }
 };
 
@@ -748,13 +874,24 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
   const int n = bt.n, ns = c->ns;
   if (bt.random) {
     if (!c->args_ready || !c->d_args.p) return fail(c, ERM_E_ENG_BUG, "random poses without generator arguments");
-    if (k.ins_org == ERM_INSORG_AGGCEN && !c->d_aggcen.p)
-      return fail(c, ERM_E_ENG_BUG, "aggregate centre missing");
-    hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
-                       (uint64_t)bt.first, dp<double>(c->d_bf),
-                       dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template], dp<double>(c->d_cellm),
-                       dp<double>(c->d_aggcen), dp<double>(c->d_spos));
-    TRY(launch_check(c, "k_pose"));
+    if (k.ins_org == ERM_INSORG_REFSTR) {
+      if (!c->d_refsite.p) return fail(c, ERM_E_ENG_BUG, "reference-structure fit missing");
+      if (k.ins_pos == ERM_INSPOS_GAUSS) TRY(ensure(c, c->d_wsol, sizeof(double) * std::max(n, c->Bmax)));
+      hipLaunchKernelGGL(k_pose_refstr, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
+                         (uint64_t)bt.first, dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
+                         dp<double>(c->d_cellm), dp<double>(c->d_refsite), dp<double>(c->d_refw),
+                         dp<double>(c->d_spos), dp<double>(c->d_wsol), dp<int>(c->d_status));
+      TRY(launch_check(c, "k_pose_refstr"));
+      if (k.ins_pos == ERM_INSPOS_GAUSS) c->wsol_ones = 0;
+    } else {
+      if (k.ins_org == ERM_INSORG_AGGCEN && !c->d_aggcen.p)
+        return fail(c, ERM_E_ENG_BUG, "aggregate centre missing");
+      hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
+                         (uint64_t)bt.first, dp<double>(c->d_bf),
+                         dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template], dp<double>(c->d_cellm),
+                         dp<double>(c->d_aggcen), dp<double>(c->d_spos));
+      TRY(launch_check(c, "k_pose"));
+    }
   } else if (bt.host_pos) {
     HIPCHK(c, hipMemcpyAsync(c->d_spos.p, bt.host_pos, sizeof(double) * (size_t)n * ns * 3, hipMemcpyHostToDevice,
                              c->st));
@@ -1754,9 +1891,43 @@ static int aggregate_center(erm_ctx *c) {
   return launch_check(c, "k_aggcen");
 }
 
+// reffit (insertion.F90:598-622) for the frame in d_pos: the reference
+// host structure fitted onto the frame's host atoms carries the reference
+// solute along (the best-fit reference solute), and the test particle's
+// body is fitted onto that; both go to d_refsite.  The GAUSS constants are
+// set at the first frame (insertion.F90:240-246).
+static int refstr_frame(erm_ctx *c) {
+  const int nh = (int)c->ref_atoms.size(), ns = c->ns;
+  TRY(ensure(c, c->d_refgather, sizeof(double) * 3 * nh));
+  hipLaunchKernelGGL(k_gather_atoms, dim3((nh + 255) / 256), dim3(256), 0, c->st, nh, dp<int>(c->d_refatoms),
+                     dp<double>(c->d_pos), dp<double>(c->d_refgather));
+  TRY(launch_check(c, "k_gather_atoms"));
+  std::vector<double> hc(3 * nh), site(6 * ns);
+  HIPCHK(c, hipMemcpyAsync(hc.data(), c->d_refgather.p, sizeof(double) * 3 * nh, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  double *fitted = site.data(), *best = site.data() + 3 * ns;
+  fit_a_rotate_b(nh, hc.data(), c->ref_host.data(), c->ref_hw.data(), ns, c->ref_slt.data(), best);
+  fit_a_rotate_b(ns, best, c->bfcoord.data(), c->ref_sw.data(), ns, c->bfcoord.data(), fitted);
+  TRY(upload(c, c->d_refsite, site.data(), site.size()));
+  Params &k = c->kp;
+  if (k.ins_pos == ERM_INSPOS_GAUSS && !c->gauss_init) {
+    for (int d = 0; d < 3; ++d) {
+      const double len = std::sqrt(c->cell[3 * d] * c->cell[3 * d] + c->cell[3 * d + 1] * c->cell[3 * d + 1] +
+                                   c->cell[3 * d + 2] * c->cell[3 * d + 2]);
+      k.gl[d] = (len / 2.0) / k.upreg;
+    }
+    k.gz0 = 8.0 * k.gl[0] * k.gl[1] * k.gl[2];
+    const double sp = std::sqrt(PI_REF);
+    k.gz1 = (sp * sp * sp) * std::erf(k.gl[0]) * std::erf(k.gl[1]) * std::erf(k.gl[2]);
+    c->gauss_init = true;
+  }
+  return ERM_OK;
+}
+
 // per-frame work after the coordinates are in d_pos
 static int frame_finish(erm_ctx *c) {
   if (c->kp.ins_org == ERM_INSORG_AGGCEN) TRY(aggregate_center(c));
+  if (c->kp.ins_org == ERM_INSORG_REFSTR) TRY(refstr_frame(c));
   int s = c->fp32 ? frame_prep<float>(c) : frame_prep<double>(c);
   if (s) return s;
   c->have_frame = true;
@@ -1776,15 +1947,23 @@ int32_t erm_set_insertion(erm_ctx *c, int32_t insorigin, int32_t insposition, in
                           double upreg, int32_t nhost, const int32_t *hostspec) {
   TRY(need(c, false, false));
   if (c->P.slttype == 1) return fail(c, ERM_E_ARG, "insertion modes apply to a test particle (slttype 2 or 3)");
-  if (insorigin == ERM_INSORG_REFSTR || insposition == ERM_INSPOS_RMSD || insposition == ERM_INSPOS_GAUSS)
-    return fail(c, ERM_E_UNSUPPORTED, "reference-structure insertion (RefInfo fit) is not implemented");
   if (insposition == ERM_INSPOS_NOCHANGE)
     return fail(c, ERM_E_UNSUPPORTED, "fixed positions: pass the coordinates to erm_engconst");
   // pairing of origin and position (setconf.F90:399-428)
   const bool agg = (insposition == ERM_INSPOS_SPHERE || insposition == ERM_INSPOS_SLAB_GENERIC ||
                     insposition == ERM_INSPOS_SLAB_SYMMETRIC);
-  if (insposition == ERM_INSPOS_RANDOM ? insorigin != ERM_INSORG_ORIGIN : (!agg || insorigin != ERM_INSORG_AGGCEN))
+  const bool refm = (insposition == ERM_INSPOS_RMSD || insposition == ERM_INSPOS_GAUSS);
+  if (insposition < 0 || insposition > ERM_INSPOS_GAUSS || insorigin < 0 || insorigin > ERM_INSORG_REFSTR)
+    return fail(c, ERM_E_ARG, "insorigin %d / insposition %d", insorigin, insposition);
+  if (insposition == ERM_INSPOS_RANDOM ? insorigin != ERM_INSORG_ORIGIN
+      : agg ? insorigin != ERM_INSORG_AGGCEN : insorigin != ERM_INSORG_REFSTR)
     return fail(c, ERM_E_ARG, "insorigin %d does not go with insposition %d (set_ins)", insorigin, insposition);
+  if (refm && (lwreg < 0.0 || upreg < 0.0 || lwreg > upreg))
+    return fail(c, ERM_E_ARG, "the reference-structure modes need 0 <= lwreg <= upreg (set_reg)");
+  if (refm && !c->have_ref)
+    return fail(c, ERM_E_ARG, "the reference-structure modes need erm_set_refstructure first");
+  if (refm && c->P.slttype != 2)
+    return fail(c, ERM_E_UNSUPPORTED, "reference-structure insertion of a flexible solute: place it on the host");
   if (insorient != ERM_INSROT_RANDOM && insorient != ERM_INSROT_NOCHANGE)
     return fail(c, ERM_E_ARG, "insorient %d", insorient);
   // region bounds (set_reg)
@@ -1816,7 +1995,36 @@ int32_t erm_set_insertion(erm_ctx *c, int32_t insorigin, int32_t insposition, in
   k.ins_rot = insorient;
   k.lwreg = lwreg;
   k.upreg = upreg;
+  c->gauss_init = false;
   if (agg && c->have_frame) TRY(aggregate_center(c));
+  if (refm && c->have_frame) TRY(refstr_frame(c));
+  return sync(c);
+}
+
+int32_t erm_set_refstructure(erm_ctx *c, int32_t nhost, const int32_t *host_atoms, const double *host_crd,
+                             const double *host_weight, const double *slt_crd, const double *slt_weight) {
+  TRY(need(c, false, false));
+  if (c->P.slttype == 1) return fail(c, ERM_E_ARG, "a reference structure applies to a test particle");
+  if (nhost < 1 || !host_atoms || !host_crd || !host_weight || !slt_crd || !slt_weight)
+    return fail(c, ERM_E_ARG, "reference structure: host atoms, coordinates and weights are required");
+  const int ns = c->ns;
+  double hw = 0.0, sw = 0.0;
+  for (int i = 0; i < nhost; ++i) {
+    if (host_atoms[i] < 0 || host_atoms[i] >= c->natom) return fail(c, ERM_E_ARG, "host atom %d out of range", i);
+    hw += host_weight[i];
+  }
+  for (int i = 0; i < ns; ++i) sw += slt_weight[i];
+  if (!(hw > 0.0) || !(sw > 0.0)) return fail(c, ERM_E_ARG, "reference weights sum to zero (bst_zrw)");
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  c->ref_atoms.assign(host_atoms, host_atoms + nhost);
+  c->ref_host.assign(host_crd, host_crd + 3 * nhost);
+  c->ref_hw.assign(host_weight, host_weight + nhost);
+  c->ref_slt.assign(slt_crd, slt_crd + 3 * ns);
+  c->ref_sw.assign(slt_weight, slt_weight + ns);
+  TRY(upload(c, c->d_refatoms, c->ref_atoms.data(), nhost));
+  TRY(upload(c, c->d_refw, c->ref_sw.data(), ns));
+  c->have_ref = true;
+  if (c->have_frame && c->kp.ins_org == ERM_INSORG_REFSTR) TRY(refstr_frame(c));
   return sync(c);
 }
 
@@ -2281,6 +2489,8 @@ int32_t erm_solute_gradient(erm_ctx *c, int32_t n, const int32_t *solutes, const
   if (c->P.wgtslf != 0 || c->P.estype != 1)
     return fail(c, ERM_E_UNSUPPORTED, "the gradient needs parameter-independent weights (wgtslf 0, estype 1)");
   if (c->P.cltype == ERM_EL_COULOMB) return fail(c, ERM_E_UNSUPPORTED, "the gradient covers PME and PPPM");
+  if (c->kp.ins_pos == ERM_INSPOS_GAUSS)
+    return fail(c, ERM_E_UNSUPPORTED, "the gradient with GAUSS insertion weights");
   const int ns = c->ns;
   if (ns > 1024) return fail(c, ERM_E_UNSUPPORTED, "gradient for solutes of more than 1024 sites");
   const Params &k = c->kp;

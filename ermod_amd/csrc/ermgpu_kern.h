@@ -43,6 +43,9 @@ struct Params {
   double lwreg, upreg;
   // intermediate-state accumulation on (solution systems)
   int intm;
+  // reference-structure insertion (GAUSS): (L/2)/upreg per axis and the
+  // normalisations z0, z1 of the uniform and Gaussian parts
+  double gl[3], gz0, gz1;
 };
 
 // --------------------------------------------------------------------------
@@ -215,6 +218,13 @@ __global__ void k_frame_scatter(int n, const float *__restrict__ xyz, const int 
   pos[3 * a] = (double)xyz[i];
   pos[3 * a + 1] = (double)xyz[n + i];
   pos[3 * a + 2] = (double)xyz[2 * n + i];
+}
+
+// coordinates of the listed atoms (the reference-structure host atoms)
+__global__ void k_gather_atoms(int n, const int *__restrict__ idx, const double *__restrict__ pos, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; ++d) out[3 * i + d] = pos[3 * idx[i] + d];
 }
 
 // com_aggregate (insertion.F90:539-566): centre of mass of every site of
@@ -1493,6 +1503,113 @@ __device__ __forceinline__ void set_error(int *status, int code, int b, int j, d
     status[2] = j;
     *reinterpret_cast<double *>(&status[4]) = v;
   }
+}
+
+// generator streams of attempt a of a reference-structure insertion
+__device__ __forceinline__ uint64_t att_seed(uint64_t seed, int a, uint64_t salt) {
+  return seed ^ mix64((uint64_t)a * 0x9E3779B97F4A7C15ULL + salt);
+}
+constexpr uint64_t SALT_POS = 0x243F6A8885A308D3ULL, SALT_ROT = 0x13198A2E03707344ULL, SALT_MIX = 0xA4093822299F31D0ULL;
+constexpr int REF_ATTEMPTS = 4096;
+
+// Test-particle pose against a reference structure (insorigin REFSTR,
+// insposition RMSD or GAUSS; instslt, insertion.F90:84-99, 198-230, 236-288,
+// 569-595): from the solute fitted onto the frame's reference (ref[0..3ns)),
+// a random shift within upreg + 3 * (largest COM distance) (RMSD) or from the
+// uniform / Gaussian mixture (GAUSS, which divides the weight by the mixture
+// density at every attempt), the orientation, and acceptance when the RMSD
+// to the best-fit reference solute (ref[3ns..6ns), weights refw) lies in
+// [lwreg, upreg]; else another attempt.  wsol[b] receives the weight (GAUSS).
+__global__ void k_pose_refstr(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
+                              const double *__restrict__ mass, const double *__restrict__ cellm,
+                              const double *__restrict__ ref, const double *__restrict__ refw, double *spos,
+                              double *wsol, int *status) {
+#pragma clang fp contract(off)
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const int ns = P.ns;
+  const uint64_t seed = args[0];
+  const uint64_t index = args[1] + offset + (uint64_t)b;
+  double *out = spos + (size_t)b * ns * 3;
+  const double *fitted = ref, *best = ref + 3 * ns;
+  double weight = 1.0, c[3];
+  bool ok = false;
+  for (int a = 0; a < REF_ATTEMPTS && !ok; ++a) {
+    const uint64_t sp = att_seed(seed, a, SALT_POS), sr = att_seed(seed, a, SALT_ROT);
+    for (int i = 0; i < 3 * ns; ++i) out[i] = fitted[i];
+    double com[3];
+    if (P.ins_pos == 5) {
+      com_dev(ns, out, mass, c);
+      double maxdis = 0.0;
+      for (int j = 0; j < ns; ++j) {
+        double d0 = out[3 * j] - c[0], d1 = out[3 * j + 1] - c[1], d2 = out[3 * j + 2] - c[2];
+        double dst = d0 * d0 + d1 * d1 + d2 * d2;
+        if (dst > maxdis) maxdis = dst;
+      }
+      maxdis = sqrt(maxdis);
+      const double movmax = P.upreg + 3.0 * maxdis;
+      for (int i = 0; i < 3; ++i) com[i] = movmax * (2.0 * rng_uniform(sp, index, (uint32_t)i) - 1.0);
+    } else {
+      // uniform_gauss_mixture: uniform or Gaussian with probability 1/2 each
+      double sc[3];
+      const bool uni = rng_uniform(att_seed(seed, a, SALT_MIX), index, 0) < 0.5;
+      uint32_t d = 0;
+      for (int i = 0; i < 3; ++i) {
+        if (uni) {
+          sc[i] = (2.0 * rng_uniform(sp, index, d++) - 1.0) * P.gl[i];
+        } else {
+          double r = 0.0;
+          for (int t = 0; t < 40; ++t) {
+            double r1 = 1.0 - rng_uniform(sp, index, d++);
+            double r2 = rng_uniform(sp, index, d++);
+            r = sqrt(-2.0 * log(r1)) * cos(2.0 * 3.1415926535897932 * r2) / sqrt(2.0);
+            if (fabs(r) < P.gl[i]) break;
+          }
+          sc[i] = r;
+        }
+      }
+      double sq = sc[0] * sc[0] + sc[1] * sc[1] + sc[2] * sc[2];
+      weight = weight / (0.5 * 1.0 / P.gz0 + (1.0 - 0.5) * exp(-sq) / P.gz1);
+      double u[3];
+      for (int i = 0; i < 3; ++i) u[i] = sc[i] / (P.gl[i] * 2.0);
+      for (int i = 0; i < 3; ++i) com[i] = cellm[i] * u[0] + cellm[3 + i] * u[1] + cellm[6 + i] * u[2];
+    }
+    for (int j = 0; j < ns; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + com[i];
+    if (P.ins_rot == 0) {
+      double q[4] = {1.0, 0.0, 0.0, 0.0};
+      for (int t = 0; t < 63; ++t) {
+        double v[4];
+        for (int i = 0; i < 4; ++i) v[i] = rng_uniform(sr, index, (uint32_t)(4 * t + i)) * 2.0 - 1.0;
+        double s2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3];
+        if (s2 < 1.0) {
+          double nrm = sqrt(s2);
+          for (int i = 0; i < 4; ++i) q[i] = v[i] / nrm;
+          break;
+        }
+      }
+      com_dev(ns, out, mass, c);
+      double cj[4] = {q[0], -q[1], -q[2], -q[3]};
+      for (int j = 0; j < ns; ++j) {
+        double q2[4] = {0.0, out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]};
+        double t4[4], r2[4];
+        qprod_dev(q, q2, t4);
+        qprod_dev(t4, cj, r2);
+        for (int i = 0; i < 3; ++i) out[3 * j + i] = r2[1 + i] + c[i];
+      }
+    }
+    // rmsd_nofit (bestfit.F90:252-268) against the best-fit reference solute
+    double wsum = 0.0, disp = 0.0;
+    for (int j = 0; j < ns; ++j) wsum = wsum + refw[j];
+    for (int j = 0; j < ns; ++j) {
+      double d0 = out[3 * j] - best[3 * j], d1 = out[3 * j + 1] - best[3 * j + 1], d2 = out[3 * j + 2] - best[3 * j + 2];
+      disp = disp + refw[j] * (d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    const double rmsd = sqrt(disp / wsum);
+    ok = !(P.lwreg > rmsd || rmsd > P.upreg);
+  }
+  if (!ok) set_error(status, ERM_E_INS_LIMIT, (int)index, -1, 0.0);
+  if (P.ins_pos == 6) wsol[b] = weight;
 }
 
 __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict__ uv,

@@ -11,7 +11,7 @@ module ermgpu
   integer(c_int32_t), parameter :: ERM_OK = 0, ERM_E_ENG_MIN = 1, ERM_E_ENG_SFT = 2, &
        ERM_E_MBIN = 3, ERM_E_ENG_ECD = 4, ERM_E_RCP_CNS = 5, ERM_E_NONCUBOID = 6, &
        ERM_E_ENG_BUG = 7, ERM_E_ENG_CNS = 8, ERM_E_ENG_SIZ = 9, ERM_E_ARG = 10, &
-       ERM_E_UNSUPPORTED = 11, ERM_E_DEVICE = 12, ERM_E_EOF = 13
+       ERM_E_UNSUPPORTED = 11, ERM_E_DEVICE = 12, ERM_E_EOF = 13, ERM_E_INS_LIMIT = 14
   integer(c_int32_t), parameter :: ERM_FP64 = 0, ERM_FP32 = 1
 
   type, bind(C) :: erm_params
@@ -211,6 +211,16 @@ module ermgpu
        real(c_double), value :: lwreg, upreg
        integer(c_int32_t), intent(in) :: hostspec(*)
      end function erm_set_insertion
+
+     ! load_refstructure's arrays for the REFSTR insertion modes
+     integer(c_int32_t) function erm_set_refstructure(ctx, nhost, host_atoms, host_crd, host_weight, &
+          slt_crd, slt_weight) bind(C, name="erm_set_refstructure")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nhost
+       integer(c_int32_t), intent(in) :: host_atoms(*)
+       real(c_double), intent(in) :: host_crd(*), host_weight(*), slt_crd(*), slt_weight(*)
+     end function erm_set_refstructure
 
      integer(c_int32_t) function erm_get_tuning(ctx, out9) bind(C, name="erm_get_tuning")
        import :: c_ptr, c_int32_t

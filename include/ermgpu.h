@@ -47,7 +47,8 @@ enum erm_status {
   ERM_E_ARG = 10,        /* invalid argument or call order */
   ERM_E_UNSUPPORTED = 11,/* option outside this implementation */
   ERM_E_DEVICE = 12,     /* HIP runtime error */
-  ERM_E_EOF = 13         /* end of the trajectory (not an error) */
+  ERM_E_EOF = 13,        /* end of the trajectory (not an error) */
+  ERM_E_INS_LIMIT = 14   /* no reference-structure insertion accepted in 4096 attempts */
 };
 
 enum erm_precision { ERM_FP64 = 0, ERM_FP32 = 1 };
@@ -147,10 +148,24 @@ int32_t erm_engclear(erm_ctx *ctx);
  * default), AGGCEN + SPHERE / SLAB_GENERIC / SLAB_SYMMETRIC, each with
  * RANDOM or NOCHANGE orientation; hostspec lists nhost solvent species
  * (uvspec numbering) whose molecules define the aggregate centre, computed
- * per frame.  REFSTR-based modes (RMSD, GAUSS) return ERM_E_UNSUPPORTED.
- * Applies to erm_insert_random and erm_solute_gradient's random poses. */
+ * per frame; REFSTR + RMSD / GAUSS after erm_set_refstructure.  Applies
+ * to erm_insert_random and erm_solute_gradient's random poses. */
 int32_t erm_set_insertion(erm_ctx *ctx, int32_t insorigin, int32_t insposition, int32_t insorient,
                           double lwreg, double upreg, int32_t nhost, const int32_t *hostspec);
+/* The reference structure of the REFSTR modes as load_refstructure
+ * (insertion.F90:623-723) leaves it: nhost atoms (0-based indices) of the
+ * refspec species with their reference coordinates (3 x nhost) and fitting
+ * weights, the test particle's reference coordinates (3 x nsite) and
+ * weights.  With insorigin REFSTR and insposition RMSD or GAUSS (rigid test
+ * particle), every frame runs reffit (insertion.F90:598-622) on the host
+ * (Horn's quaternion fit, bestfit.F90:92-241) and every insertion draws
+ * attempts until the RMSD to the best-fit reference solute lies in
+ * [lwreg, upreg] (check_solute_configuration, insertion.F90:569-595);
+ * GAUSS divides the insertion's weight by the mixture density at each
+ * attempt, as set_shift_com does.  4096 attempts without acceptance return
+ * ERM_E_INS_LIMIT. */
+int32_t erm_set_refstructure(erm_ctx *ctx, int32_t nhost, const int32_t *host_atoms, const double *host_crd,
+                             const double *host_weight, const double *slt_crd, const double *slt_weight);
 
 /* The intermediate-state method of extensions/Intermediate/engproc.F90
  * (solution systems): zerodns[ermax] != 0 marks the bins whose density is

@@ -85,6 +85,11 @@ def lib():
                                          C.c_uint64, C.c_uint64, _f64p]
         L.orc_set_intermediate.argtypes = [vp, _i32p]
         L.orc_get_intermediate.argtypes = [vp, _f64p, _f64p, _f64p]
+        L.orc_fit_a_rotate_b.argtypes = [C.c_int32, _f64p, _f64p, _f64p, C.c_int32, _f64p, _f64p]
+        L.orc_rmsd_nofit.argtypes = [C.c_int32, _f64p, _f64p, _f64p]
+        L.orc_rmsd_nofit.restype = C.c_double
+        L.orc_refstr_pose.argtypes = [C.c_int32, _f64p, _f64p, C.POINTER(OrcInsertion), _f64p, _f64p, _f64p,
+                                      _f64p, C.c_uint64, C.c_uint64, _f64p, _f64p]
         L.orc_gfb_pppm.argtypes = [C.c_int32, _f64p]
         L.orc_pppm_greenfunc.argtypes = [_i32p, C.c_int32, _f64p, _f64p, C.c_double, C.c_double, _f64p]
         L.orc_realcal_bare.argtypes = [vp, C.c_int32, _f64p, _f64p]
@@ -176,6 +181,51 @@ def insertion_pose(bfcoord, mass, cell, ins, seed, index, aggcen=None):
     lib().orc_insertion_pose(n, P(bf), P(f64(mass)), P(cm), C.byref(st), P(ag), C.c_uint64(seed),
                              C.c_uint64(index), P(out))
     return out
+
+
+def fit_a_rotate_b(refa, a, w, b):
+    """bestfit's fit_a_rotate_b: b moved by the superposition of a onto refa."""
+    refa, a, b, w = f64(refa), f64(a), f64(b), f64(w)
+    out = np.zeros_like(b)
+    lib().orc_fit_a_rotate_b(len(a), P(refa), P(a), P(w), len(b), P(b), P(out))
+    return out
+
+
+def rmsd_nofit(a, b, w):
+    return lib().orc_rmsd_nofit(len(f64(a)), P(f64(a)), P(f64(b)), P(f64(w)))
+
+
+def reffit(frame_pos, host_atoms, host_ref, host_w, slt_ref, slt_w, bfcoord):
+    """reffit (insertion.F90:598-622): (fitted solute, best-fit reference
+    solute) for a frame."""
+    hc = f64(np.asarray(frame_pos)[np.asarray(host_atoms)])
+    best = fit_a_rotate_b(hc, host_ref, host_w, slt_ref)
+    fitted = fit_a_rotate_b(best, bfcoord, slt_w, bfcoord)
+    return fitted, best
+
+
+def gauss_constants(cell, upreg):
+    """uniform_gauss_mixture's first-call constants: (L/2)/upreg per axis,
+    z0, z1 (insertion.F90:240-246)."""
+    from math import erf, sqrt, pi
+    cell = np.asarray(cell, dtype=np.float64)
+    gl = [(float(np.sqrt(np.sum(cell[:, d] ** 2))) / 2.0) / upreg for d in range(3)]
+    sp = sqrt(pi)
+    return np.array(gl + [8.0 * gl[0] * gl[1] * gl[2], (sp * sp * sp) * erf(gl[0]) * erf(gl[1]) * erf(gl[2])])
+
+
+def refstr_pose(mass, cell, ins, fitted, best, w, gconst, seed, index):
+    """A reference-structure insertion: (coordinates, weight)."""
+    fitted = f64(fitted)
+    n = len(fitted)
+    st = OrcInsertion(int(ins[0]), int(ins[1]), int(ins[2]), float(ins[3]), float(ins[4]))
+    out = np.zeros((n, 3))
+    wt = C.c_double()
+    cm = f64(np.asarray(cell).T.copy())
+    check(lib().orc_refstr_pose(n, P(f64(mass)), P(cm), C.byref(st), P(fitted), P(f64(best)), P(f64(w)),
+                                P(f64(gconst)), C.c_uint64(seed), C.c_uint64(index), P(out), C.byref(wt)),
+          "refstr_pose")
+    return out, wt.value
 
 
 def gfb_pppm(order):

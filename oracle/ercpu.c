@@ -773,6 +773,190 @@ void orc_insertion_pose(int32_t n, const double *bfcoord, const double *mass, co
   }
 }
 
+/* ---- bestfit.F90: Horn's quaternion superposition ----------------------- */
+/* eigenvector of the largest eigenvalue of a symmetric 4x4 matrix by cyclic
+ * Jacobi rotations (the reference calls LAPACK dsyev) */
+static void sym4_top_eigvec(const double Ain[4][4], double v[4]) {
+  double A[4][4], V[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      A[i][j] = Ain[i][j];
+      V[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < 4; ++p)
+      for (int q = p + 1; q < 4; ++q) off += A[p][q] * A[p][q];
+    if (off == 0.0) break;
+    for (int p = 0; p < 4; ++p)
+      for (int q = p + 1; q < 4; ++q) {
+        if (A[p][q] == 0.0) continue;
+        double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+        double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < 4; ++k) {
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = cs * akp - sn * akq;
+          A[k][q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < 4; ++k) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = cs * apk - sn * aqk;
+          A[q][k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < 4; ++k) {
+          double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = cs * vkp - sn * vkq;
+          V[k][q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  int best = 0;
+  for (int i = 1; i < 4; ++i)
+    if (A[i][i] > A[best][best]) best = i;
+  for (int i = 0; i < 4; ++i) v[i] = V[i][best];
+}
+
+/* find_rotation_quaternion (bestfit.F90:92-146) */
+static void fit_quaternion(int n, const double *ref, const double *mov, const double *w, double q[4]) {
+  double ip[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc = acc + ref[3 * k + i] * mov[3 * k + j] * w[k];
+      ip[j][i] = acc;
+    }
+  double M[4][4];
+  M[0][0] = ip[0][0] + ip[1][1] + ip[2][2];
+  M[1][1] = ip[0][0] - ip[1][1] - ip[2][2];
+  M[2][2] = -ip[0][0] + ip[1][1] - ip[2][2];
+  M[3][3] = -ip[0][0] - ip[1][1] + ip[2][2];
+  M[0][1] = ip[1][2] - ip[2][1];
+  M[0][2] = ip[2][0] - ip[0][2];
+  M[0][3] = ip[0][1] - ip[1][0];
+  M[1][2] = ip[0][1] + ip[1][0];
+  M[2][3] = ip[1][2] + ip[2][1];
+  M[1][3] = ip[2][0] + ip[0][2];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < i; ++j) M[i][j] = M[j][i];
+  sym4_top_eigvec(M, q);
+}
+
+void orc_fit_a_rotate_b(int32_t na, const double *refa, const double *a, const double *w, int32_t nb, const double *b,
+                        double *bout) {
+  double *wr = (double *)malloc(sizeof(double) * 3 * (na ? na : 1));
+  double *wa = (double *)malloc(sizeof(double) * 3 * (na ? na : 1));
+  memcpy(wr, refa, sizeof(double) * 3 * na);
+  memcpy(wa, a, sizeof(double) * 3 * na);
+  double cr[3], ca[3];
+  com_of(na, wr, w, cr);
+  for (int k = 0; k < na; ++k)
+    for (int i = 0; i < 3; ++i) wr[3 * k + i] = wr[3 * k + i] - cr[i];
+  com_of(na, wa, w, ca);
+  for (int k = 0; k < na; ++k)
+    for (int i = 0; i < 3; ++i) wa[3 * k + i] = wa[3 * k + i] - ca[i];
+  double q[4];
+  fit_quaternion(na, wr, wa, w, q);
+  for (int k = 0; k < nb; ++k) {
+    double d[3] = {b[3 * k] - ca[0], b[3 * k + 1] - ca[1], b[3 * k + 2] - ca[2]}, r[3];
+    qrotate(d, q, r);
+    for (int i = 0; i < 3; ++i) bout[3 * k + i] = r[i] + cr[i];
+  }
+  free(wr);
+  free(wa);
+}
+
+double orc_rmsd_nofit(int32_t n, const double *a, const double *b, const double *w) {
+  double ws = 0.0, disp = 0.0;
+  for (int j = 0; j < n; ++j) ws = ws + w[j];
+  for (int j = 0; j < n; ++j) {
+    double d0 = b[3 * j] - a[3 * j], d1 = b[3 * j + 1] - a[3 * j + 1], d2 = b[3 * j + 2] - a[3 * j + 2];
+    disp = disp + w[j] * (d0 * d0 + d1 * d1 + d2 * d2);
+  }
+  return sqrt(disp / ws);
+}
+
+static uint64_t att_seed(uint64_t seed, int a, uint64_t salt) {
+  return seed ^ mix64((uint64_t)a * 0x9E3779B97F4A7C15ULL + salt);
+}
+
+int32_t orc_refstr_pose(int32_t n, const double *mass, const double *cell, const orc_insertion *ins,
+                        const double *fitted, const double *best, const double *w, const double *gconst,
+                        uint64_t seed, uint64_t index, double *out, double *weight) {
+  const double PI_F = 3.1415926535897932;
+  double c[3], wt = 1.0;
+  for (int a = 0; a < 4096; ++a) {
+    uint64_t sp = att_seed(seed, a, 0x243F6A8885A308D3ULL), sr = att_seed(seed, a, 0x13198A2E03707344ULL);
+    memcpy(out, fitted, sizeof(double) * 3 * n);
+    double com[3];
+    if (ins->insposition == 5) {
+      /* INSPOS_RMSD (insertion.F90:199-217) */
+      com_of(n, out, mass, c);
+      double maxdis = 0.0;
+      for (int j = 0; j < n; ++j) {
+        double d0 = out[3 * j] - c[0], d1 = out[3 * j + 1] - c[1], d2 = out[3 * j + 2] - c[2];
+        double dst = d0 * d0 + d1 * d1 + d2 * d2;
+        if (dst > maxdis) maxdis = dst;
+      }
+      maxdis = sqrt(maxdis);
+      double movmax = ins->upreg + 3.0 * maxdis;
+      for (int i = 0; i < 3; ++i) com[i] = movmax * (2.0 * orc_rng_uniform(sp, index, (uint32_t)i) - 1.0);
+    } else {
+      /* INSPOS_GAUSS: uniform_gauss_mixture (insertion.F90:236-288) */
+      double sc[3];
+      int uni = orc_rng_uniform(att_seed(seed, a, 0xA4093822299F31D0ULL), index, 0) < 0.5;
+      uint32_t d = 0;
+      for (int i = 0; i < 3; ++i) {
+        if (uni) {
+          sc[i] = (2.0 * orc_rng_uniform(sp, index, d++) - 1.0) * gconst[i];
+        } else {
+          double r = 0.0;
+          for (int t = 0; t < 40; ++t) {
+            double r1 = 1.0 - orc_rng_uniform(sp, index, d++);
+            double r2 = orc_rng_uniform(sp, index, d++);
+            r = sqrt(-2.0 * log(r1)) * cos(2.0 * PI_F * r2) / sqrt(2.0);
+            if (fabs(r) < gconst[i]) break;
+          }
+          sc[i] = r;
+        }
+      }
+      double sq = sc[0] * sc[0] + sc[1] * sc[1] + sc[2] * sc[2];
+      wt = wt / (0.5 * 1.0 / gconst[3] + (1.0 - 0.5) * exp(-sq) / gconst[4]);
+      double u[3];
+      for (int i = 0; i < 3; ++i) u[i] = sc[i] / (gconst[i] * 2.0);
+      for (int i = 0; i < 3; ++i) com[i] = cell[i] * u[0] + cell[3 + i] * u[1] + cell[6 + i] * u[2];
+    }
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + com[i];
+    if (ins->insorient == 0) {
+      double q[4] = {1.0, 0.0, 0.0, 0.0};
+      for (int t = 0; t < 63; ++t) {
+        double v[4];
+        for (int i = 0; i < 4; ++i) v[i] = orc_rng_uniform(sr, index, (uint32_t)(4 * t + i)) * 2.0 - 1.0;
+        double s2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3];
+        if (s2 < 1.0) {
+          double nrm = sqrt(s2);
+          for (int i = 0; i < 4; ++i) q[i] = v[i] / nrm;
+          break;
+        }
+      }
+      com_of(n, out, mass, c);
+      for (int j = 0; j < n; ++j) {
+        double v[3] = {out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]}, t[3];
+        qrotate(v, q, t);
+        for (int i = 0; i < 3; ++i) out[3 * j + i] = t[i] + c[i];
+      }
+    }
+    double rmsd = orc_rmsd_nofit(n, best, out, w);
+    if (!(ins->lwreg > rmsd || rmsd > ins->upreg)) {
+      *weight = wt;
+      return ORC_OK;
+    }
+  }
+  *weight = wt;
+  return 14;
+}
+
 /* ====================================================================== */
 /* context                                                                */
 /* ====================================================================== */

@@ -151,6 +151,21 @@ void orc_insertion_pose(int32_t n, const double *bfcoord, const double *mass, co
                         const orc_insertion *ins, const double *aggcen, uint64_t seed, uint64_t index,
                         double *out);
 
+/* bestfit.F90: fit_a_rotate_b (214-241; Horn's quaternion from the top
+ * eigenvector of the 4x4 key matrix, here by Jacobi rotations instead of
+ * dsyev) and rmsd_nofit (252-268); coordinates 3 x n */
+void orc_fit_a_rotate_b(int32_t na, const double *refa, const double *a, const double *w, int32_t nb,
+                        const double *b, double *bout);
+double orc_rmsd_nofit(int32_t n, const double *a, const double *b, const double *w);
+/* A reference-structure insertion (insorigin REFSTR, insposition 5 RMSD or
+ * 6 GAUSS) from the frame's fitted solute and best-fit reference solute:
+ * attempts of position + orientation until rmsd_nofit lies in [lwreg,
+ * upreg].  gconst = (L1/2)/upreg, (L2/2)/upreg, (L3/2)/upreg, z0, z1 (GAUSS).
+ * *weight: the GAUSS weight (1 for RMSD).  Returns 14 after 4096 attempts. */
+int32_t orc_refstr_pose(int32_t n, const double *mass, const double *cell, const orc_insertion *ins,
+                        const double *fitted, const double *best, const double *w, const double *gconst,
+                        uint64_t seed, uint64_t index, double *out, double *weight);
+
 /* --- context: the per-frame pipeline ---------------------------------- */
 orc_ctx *orc_create(const orc_system *sys, int32_t *status);
 void orc_destroy(orc_ctx *c);

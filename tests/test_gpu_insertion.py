@@ -116,7 +116,7 @@ def test_insertion_argument_checks():
                ((INSORG_AGGCEN, INSPOS_SLAB_SYMMETRIC, 0, -1.0, 2.0, [1]), 10),
                ((INSORG_AGGCEN, INSPOS_SPHERE, 0, 1.0, 2.0, []), 10),       # no host species
                ((INSORG_AGGCEN, INSPOS_SPHERE, 2, 1.0, 2.0, [1]), 10),      # insorient
-               ((INSORG_REFSTR, INSPOS_GAUSS, 0, 1.0, 2.0, [1]), 11)]       # reference structure
+               ((INSORG_REFSTR, INSPOS_GAUSS, 0, 1.0, 2.0, [1]), 10)]       # no reference structure yet
         for (org, p, rot, lw, up, host), code in bad:
             with pytest.raises(ErmError) as ei:
                 g.set_insertion(org, p, rot, lw, up, hostspec=host)
