@@ -60,7 +60,7 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune",
            "erm_insert_random_multi", "erm_set_insertion", "erm_dcd_open", "erm_dcd_read", "erm_dcd_close",
            "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close", "erm_set_intermediate",
-           "erm_fetch_intermediate", "erm_set_refstructure")
+           "erm_fetch_intermediate", "erm_set_refstructure", "erm_insert_flexible")
 
 _lib = None
 
@@ -109,6 +109,7 @@ def lib():
         L.erm_set_insertion.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                         _i32p]
         L.erm_set_refstructure.argtypes = [vp, C.c_int32, _i32p, _f64p, _f64p, _f64p, _f64p]
+        L.erm_insert_flexible.argtypes = [vp, C.c_int32, _f64p, _f64p, C.c_uint64, C.c_uint64, _f64p]
         L.erm_set_intermediate.argtypes = [vp, _i32p]
         L.erm_fetch_intermediate.argtypes = [vp, _f64p, _f64p, _f64p]
         L.erm_dcd_open.argtypes = [C.c_char_p, C.POINTER(vp), _i32p, _i32p]
@@ -385,6 +386,17 @@ class ErmGPU:
         out = np.zeros((n, self.slvmax + 1)) if want_uv else None
         self._check(lib().erm_insert_random(self.h, int(n), C.c_uint64(seed), C.c_uint64(first_index),
                                             _p(out)), "insert_random")
+        return out
+
+    def insert_flexible(self, confs, seed, first_index=0, weights=None, want_uv=False):
+        """Random placement of the given body configurations of a flexible
+        test particle (slttype 3) with their weights (SltWght)."""
+        cf = _f64(confs)
+        n = cf.size // (3 * self.system.nsite(self.test_mol))
+        w = None if weights is None else _f64(weights)
+        out = np.zeros((n, self.slvmax + 1)) if want_uv else None
+        self._check(lib().erm_insert_flexible(self.h, n, _p(cf), _p(w), C.c_uint64(seed), C.c_uint64(first_index),
+                                              _p(out)), "insert_flexible")
         return out
 
     def update_histogram(self, uvengy, solutes=None, weights=None):

@@ -456,6 +456,7 @@ struct erm_ctx {
   // reference structure (RefInfo, insertion.F90:623-723): host atoms and
   // their reference coordinates and weights, the solute's reference
   // coordinates and weights; per frame the fitted and best-fit solute
+  DBuf d_flexbf;  // body configurations of a flexible test particle
   bool have_ref = false, gauss_init = false;
   std::vector<int> ref_atoms;
   std::vector<double> ref_host, ref_hw, ref_slt, ref_sw;
@@ -473,7 +474,7 @@ struct erm_ctx {
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
             &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm, &d_hostmol, &d_aggcen, &d_tstage,
             &d_tperm, &d_zerodns, &d_edintm, &d_uvintm, &d_intm_sc, &d_wi, &d_refatoms, &d_refgather,
-            &d_refsite, &d_refw};
+            &d_refsite, &d_refw, &d_flexbf};
   }
 };
 
@@ -553,6 +554,7 @@ const char *status_name(int code) {
     case ERM_E_ENG_SFT: return "eng_sft: energy above the soft region";
     case ERM_E_MBIN: return "mbin: energy in the overflow bin below 1e3";
     case ERM_E_ENG_CNS: return "eng_cns: solvent molecule without species";
+    case ERM_E_INS_LIMIT: return "no reference-structure insertion accepted in 4096 attempts";
     default: return "error";
   }
 }
@@ -864,6 +866,7 @@ struct Batch {
   const double *weights;           // n or NULL
   bool random;                     // device poses
   uint64_t seed, first_index;
+  const double *flex = nullptr;    // random placement of n given body configurations (n * ns * 3)
 };
 
 
@@ -886,10 +889,11 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
     } else {
       if (k.ins_org == ERM_INSORG_AGGCEN && !c->d_aggcen.p)
         return fail(c, ERM_E_ENG_BUG, "aggregate centre missing");
+      if (bt.flex) TRY(upload(c, c->d_flexbf, bt.flex, (size_t)n * ns * 3));
       hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
-                         (uint64_t)bt.first, dp<double>(c->d_bf),
-                         dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template], dp<double>(c->d_cellm),
-                         dp<double>(c->d_aggcen), dp<double>(c->d_spos));
+                         (uint64_t)bt.first, bt.flex ? dp<double>(c->d_flexbf) : dp<double>(c->d_bf),
+                         bt.flex ? 3 * ns : 0, dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
+                         dp<double>(c->d_cellm), dp<double>(c->d_aggcen), dp<double>(c->d_spos));
       TRY(launch_check(c, "k_pose"));
     }
   } else if (bt.host_pos) {
@@ -1226,6 +1230,7 @@ int enqueue_batches(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
     if (all.host_pos) bt.host_pos = all.host_pos + (size_t)s0 * ns * 3;
     if (all.solutes) bt.solutes = all.solutes + s0;
     if (all.weights) bt.weights = all.weights + s0;
+    if (all.flex) bt.flex = all.flex + (size_t)s0 * ns * 3;
     c->cur_ev = (8 * (bi + 1) <= (int)c->bev.size()) ? &c->bev[8 * bi] : nullptr;
     if (bt.weights) {
       TRY(upload(c, c->d_wsol, bt.weights, bt.n));
@@ -2136,6 +2141,21 @@ int32_t erm_insert_random(erm_ctx *c, int32_t n, uint64_t seed, uint64_t first_i
   Batch bt{n, 0, nullptr, nullptr, nullptr, true, seed, first_index};
   const int flags = DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF | DO_HIST;
   if (c->use_graph && !uvengy && n > 0) return process_graph(c, bt, flags);
+  return process(c, bt, flags, uvengy);
+}
+
+int32_t erm_insert_flexible(erm_ctx *c, int32_t n, const double *confs, const double *weights, uint64_t seed,
+                            uint64_t first_index, double *uvengy) {
+  TRY(need(c, true, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (c->P.slttype != 3) return fail(c, ERM_E_ARG, "flexible insertion needs slttype 3");
+  if (n < 0) return fail(c, ERM_E_ARG, "negative count");
+  if (n > 0 && !confs) return fail(c, ERM_E_ARG, "no solute configurations");
+  if (c->kp.ins_org == ERM_INSORG_REFSTR)
+    return fail(c, ERM_E_UNSUPPORTED, "reference-structure insertion of a flexible solute: place it on the host");
+  Batch bt{n, 0, nullptr, nullptr, weights, true, seed, first_index};
+  bt.flex = confs;
+  const int flags = DO_RECIP | DO_REAL | DO_RESIDUAL | DO_SELF | DO_HIST;
   return process(c, bt, flags, uvengy);
 }
 

@@ -260,9 +260,11 @@ constexpr uint64_t POS_STREAM = 0xD1B54A32D192ED03ULL;
 // args = {seed, first index of the call}; insertion b of this launch is
 // number args[1] + offset + b of the generator stream (device-side arguments,
 // so a captured graph replays with new seeds).  aggcen: the aggregate centre
-// of the frame (AGGCEN only).
+// of the frame (AGGCEN only).  bf_stride 0: the rigid body bf for every
+// insertion; 3 * ns: body b of a flexible solute (getsolute's SltConf
+// configurations, insertion.F90:384-480).
 __global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
-                       const double *__restrict__ bf, const double *__restrict__ mass,
+                       const double *__restrict__ bf, int bf_stride, const double *__restrict__ mass,
                        const double *__restrict__ cellm, const double *__restrict__ aggcen, double *spos) {
 #pragma clang fp contract(off)
   int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -271,7 +273,8 @@ __global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint6
   const uint64_t seed = args[0], sp = seed ^ POS_STREAM;
   uint64_t index = args[1] + offset + (uint64_t)b;
   double *out = spos + (size_t)b * ns * 3;
-  for (int i = 0; i < 3 * ns; ++i) out[i] = bf[i];
+  const double *body = bf + (size_t)b * bf_stride;  // one body per insertion for a flexible solute
+  for (int i = 0; i < 3 * ns; ++i) out[i] = body[i];
   double c[3];
   if (P.ins_org != 1) {
     // COM to the origin or to the aggregate centre (set_solute_com)

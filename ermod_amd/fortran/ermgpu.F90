@@ -212,6 +212,16 @@ module ermgpu
        integer(c_int32_t), intent(in) :: hostspec(*)
      end function erm_set_insertion
 
+     ! random placement of flexible-solute bodies (SltConf) with weights
+     integer(c_int32_t) function erm_insert_flexible(ctx, n, confs, weights, seed, first_index, uvengy) &
+          bind(C, name="erm_insert_flexible")
+       import :: c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       type(c_ptr), value :: confs, weights, uvengy
+       integer(c_int64_t), value :: seed, first_index
+     end function erm_insert_flexible
+
      ! load_refstructure's arrays for the REFSTR insertion modes
      integer(c_int32_t) function erm_set_refstructure(ctx, nhost, host_atoms, host_crd, host_weight, &
           slt_crd, slt_weight) bind(C, name="erm_set_refstructure")

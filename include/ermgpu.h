@@ -152,6 +152,13 @@ int32_t erm_engclear(erm_ctx *ctx);
  * to erm_insert_random and erm_solute_gradient's random poses. */
 int32_t erm_set_insertion(erm_ctx *ctx, int32_t insorigin, int32_t insposition, int32_t insorient,
                           double lwreg, double upreg, int32_t nhost, const int32_t *hostspec);
+/* Random insertions of a flexible test particle (slttype 3): confs holds n
+ * body configurations (n * nsite * 3, as getsolute reads them from SltConf,
+ * insertion.F90:384-480), weights their SltWght values (NULL: 1); body b is
+ * placed as insertion first_index + b of the stream seed under the
+ * insertion modes (not REFSTR), then as erm_insert_random. */
+int32_t erm_insert_flexible(erm_ctx *ctx, int32_t n, const double *confs, const double *weights, uint64_t seed,
+                            uint64_t first_index, double *uvengy);
 /* The reference structure of the REFSTR modes as load_refstructure
  * (insertion.F90:623-723) leaves it: nhost atoms (0-based indices) of the
  * refspec species with their reference coordinates (3 x nhost) and fitting
