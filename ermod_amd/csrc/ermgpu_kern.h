@@ -847,6 +847,77 @@ __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename 
 // (consecutive lanes on distinct LDS banks; each real row fills exactly its
 // complex row's bytes).  Each thread then adds the planes'
 // contributions to its MAXR owned sites (FP64 sums).
+// gather of one group of nq real planes (first plane zz0, rows of K1 + 2)
+// into the per-site sums acc[] of the sites a thread owns.  PP: the spline
+// order at compile time (weights, row offsets and the p^3 products in
+// registers, loads independent); PP = 0 takes it from P.p.
+template <typename T, int K1, int MAXR, int PP>
+__device__ __forceinline__ void xslab_gather(const Params &P, const T *real, int zz0, int nq, int s_beg, size_t np,
+                                             const int *gz, const int *__restrict__ zg, const T *__restrict__ zq,
+                                             const T *__restrict__ zw, double *acc) {
+  constexpr int K1P = K1 + 2;
+  const int K2 = P.K2, p = PP ? PP : P.p;
+  const size_t plane = (size_t)K2 * (K1 / 2 + 1) * 2;  // reals per plane
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) {
+    const int g3 = gz[r];
+    const int clo = max(0, g3 - (zz0 + nq - 1)), chi = min(p - 1, g3 - zz0);
+    if (clo > chi) continue;
+    const int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+    const int gp = zg[idx];
+    const int g1 = gp & 1023, g2 = (gp >> 10) & 1023;
+    const T qv = zq[idx];
+    if constexpr (PP > 0) {
+      T w1[PP], w2[PP];
+      int xo[PP], yo[PP];
+#pragma unroll
+      for (int k = 0; k < PP; ++k) {
+        w1[k] = zw[(size_t)k * np + idx];
+        w2[k] = zw[(size_t)(PP + k) * np + idx];
+        int x = g1 - k, y = g2 - k;
+        xo[k] = x + ((x < 0) ? K1 : 0);
+        yo[k] = (y + ((y < 0) ? K2 : 0)) * K1P;
+      }
+      for (int c3 = clo; c3 <= chi; ++c3) {
+        const T *ph = real + (size_t)(g3 - c3 - zz0) * plane;
+        T sum = 0;
+#pragma unroll
+        for (int c2 = 0; c2 < PP; ++c2) {
+          const T *row = ph + yo[c2];
+          T rs = 0;
+#pragma unroll
+          for (int c1 = 0; c1 < PP; ++c1) rs += w1[c1] * row[xo[c1]];
+          sum += w2[c2] * rs;
+        }
+        acc[r] += (double)(qv * zw[(size_t)(2 * PP + c3) * np + idx] * sum);
+      }
+    } else {
+      T w1[MAXP], w2[MAXP];
+      for (int k = 0; k < p; ++k) {
+        w1[k] = zw[(size_t)k * np + idx];
+        w2[k] = zw[(size_t)(p + k) * np + idx];
+      }
+      for (int c3 = clo; c3 <= chi; ++c3) {
+        const T *ph = real + (size_t)(g3 - c3 - zz0) * plane;
+        T sum = 0;
+        for (int c2 = 0; c2 < p; ++c2) {
+          int y = g2 - c2;
+          y += (y < 0) ? K2 : 0;
+          const T *row = ph + (size_t)y * K1P;
+          T rs = 0;
+          for (int c1 = 0; c1 < p; ++c1) {
+            int xx = g1 - c1;
+            xx += (xx < 0) ? K1 : 0;
+            rs += w1[c1] * row[xx];
+          }
+          sum += w2[c2] * rs;
+        }
+        acc[r] += (double)(qv * zw[(size_t)(2 * p + c3) * np + idx] * sum);
+      }
+    }
+  }
+}
+
 template <typename T, int K1, int MAXR, int XT = NTHR>
 __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int nsites,
                                                     const typename C2<T>::type *__restrict__ Hk,
@@ -898,44 +969,14 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
       for (int m = 0; m < K1 / 2; ++m) d[m] = Z[m];
     }
     __syncthreads();
-    // gather: owned sites whose support g3 - c3 meets planes [zz0, zz0 + nq)
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const int g3 = gz[r];
-      const int clo = max(0, g3 - (zz0 + nq - 1)), chi = min(p - 1, g3 - zz0);
-      if (clo <= chi) {
-        const int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
-        const int gp = zg[idx];
-        const int g1 = gp & 1023, g2 = (gp >> 10) & 1023;
-        T w1[MAXP], w2[MAXP];
-        for (int k = 0; k < p; ++k) {
-          w1[k] = zw[(size_t)k * np + idx];
-          w2[k] = zw[(size_t)(p + k) * np + idx];
-        }
-        const T qv = zq[idx];
-        for (int c3 = clo; c3 <= chi; ++c3) {
-          const T *ph = real + (size_t)(g3 - c3 - zz0) * cplane * 2;
-          T sum = 0;
-          for (int c2 = 0; c2 < p; ++c2) {
-            int y = g2 - c2;
-            y += (y < 0) ? K2 : 0;
-            const T *row = ph + (size_t)y * K1P;
-            T rs = 0;
-            if (g1 >= p - 1) {
-              for (int c1 = 0; c1 < p; ++c1) rs += w1[c1] * row[g1 - c1];
-            } else {
-              for (int c1 = 0; c1 < p; ++c1) {
-                int xx = g1 - c1;
-                xx += (xx < 0) ? K1 : 0;
-                rs += w1[c1] * row[xx];
-              }
-            }
-            sum += w2[c2] * rs;
-          }
-          acc[r] += (double)(qv * zw[(size_t)(2 * p + c3) * np + idx] * sum);
-        }
-      }
-    }
+    // gather: owned sites whose support g3 - c3 meets planes [zz0, zz0 + nq);
+    // the common spline orders get fully unrolled register code
+    if (p == 4)
+      xslab_gather<T, K1, MAXR, 4>(P, real, zz0, nq, s_beg, np, gz, zg, zq, zw, acc);
+    else if (p == 6)
+      xslab_gather<T, K1, MAXR, 6>(P, real, zz0, nq, s_beg, np, gz, zg, zq, zw, acc);
+    else
+      xslab_gather<T, K1, MAXR, 0>(P, real, zz0, nq, s_beg, np, gz, zg, zq, zw, acc);
     __syncthreads();
   }
 #pragma unroll
