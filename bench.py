@@ -173,6 +173,8 @@ def main():
     ap.add_argument("--slab-threads", type=int, default=0, help="fixed x-slab workgroup size (with --batch)")
     ap.add_argument("--column-threads", type=int, default=0, help="fixed z-column workgroup size (with --batch)")
     ap.add_argument("--no-variants", action="store_true", help="skip the PPPM and bare-Coulomb passes")
+    ap.add_argument("--fold-y", type=int, default=-1, choices=[-1, 0, 1],
+                    help="inverse y transform inside the x slab (with --batch; -1 keeps the default)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the heterogeneous pass (four solute sizes in a mixed solvent, concurrently)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -207,6 +209,7 @@ def main():
             tuning, _ = g.autotune(n_trial=args.insertions, seed=args.seed + 1)
         else:
             g.set_tuning(max_batch=args.batch, slab_planes=args.slab_planes, group_planes=args.group_planes,
+                         fold_y=None if args.fold_y < 0 else bool(args.fold_y),
                          slab_threads=args.slab_threads, column_threads=args.column_threads)
         g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)  # keeps batch and slab shape
         for i in range(warmup):
@@ -278,7 +281,7 @@ def main():
                       "frames": args.steps, "insertions_per_frame": args.insertions,
                       "batch": bmax, "pme_path": tun["pme_path"], "slab_planes": tun["slab_planes"],
                       "group_planes": tun["group_planes"], "slab_threads": tun["slab_threads"],
-                      "column_threads": tun["column_threads"],
+                      "column_threads": tun["column_threads"], "fold_y": tun["fold_y"],
                       "precision": args.precision,
                       "tuning": "autotuned" if not args.no_tune and args.batch == 0 else "fixed"},
            "insertions_per_s": n_ins / dt,

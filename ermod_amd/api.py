@@ -98,8 +98,7 @@ def lib():
         L.erm_fetch.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.erm_sizes.argtypes = [vp, _i32p, _i32p]
         L.erm_stage_times.argtypes = [vp, _f64p]
-        L.erm_set_tuning.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                                     C.c_int32]
+        L.erm_set_tuning.argtypes = [vp] + [C.c_int32] * 8
         L.erm_get_tuning.argtypes = [vp, _i32p]
         L.erm_autotune.argtypes = [vp, C.c_int32, C.c_uint64, _f64p]
         L.erm_insert_random_multi.argtypes = [C.POINTER(vp), C.c_int32, _i32p, C.POINTER(C.c_uint64),
@@ -488,19 +487,23 @@ class ErmGPU:
                                             _p(hs, _i32p)), "set_insertion")
 
     def set_tuning(self, max_batch=0, pme_path="auto", use_graph=False, slab_planes=0, group_planes=0,
-                   slab_threads=0, column_threads=0):
+                   slab_threads=0, column_threads=0, fold_y=None):
+        """fold_y: True / False moves the inverse y transform into the x slab
+        or not; None keeps the current setting."""
         pp = {"auto": 0, "fused": 1, "unfused": 2, "register": 3}[pme_path]
+        fy = -1 if fold_y is None else int(bool(fold_y))
         self._check(lib().erm_set_tuning(self.h, int(max_batch), pp, int(bool(use_graph)), int(slab_planes),
-                                         int(group_planes), int(slab_threads), int(column_threads)),
+                                         int(group_planes), int(slab_threads), int(column_threads), fy),
                     "set_tuning")
 
     def tuning(self):
-        out = np.zeros(9, dtype=np.int32)
+        out = np.zeros(10, dtype=np.int32)
         self._check(lib().erm_get_tuning(self.h, _p(out, _i32p)), "get_tuning")
         names = {0: "auto", 1: "fused", 2: "unfused", 3: "register"}
         return dict(max_batch=int(out[0]), pme_path=names[int(out[1])], use_graph=bool(out[2]),
                     slab_threads=int(out[3]), slab_planes=int(out[4]), group_planes=int(out[5]),
-                    slab_cap=int(out[6]), group_cap=int(out[7]), column_threads=int(out[8]))
+                    slab_cap=int(out[6]), group_cap=int(out[7]), column_threads=int(out[8]),
+                    fold_y=bool(out[9]))
 
     def autotune(self, n_trial=512, seed=12345):
         """Times batch sizes, PME paths and graph replay on the current frame;

@@ -444,6 +444,7 @@ struct erm_ctx {
   int tune_S = 16, tune_Q = 8;  // caps on slab depth and planes per group (autotuned)
   int xslab_T = NTHR;           // threads per x-slab workgroup, 256 or 512 (autotuned)
   int zcol_T = NTHR;            // threads per z-column workgroup, 256 or 512 (autotuned)
+  bool xslab_yf = false;        // inverse y transform inside the x slab (autotuned)
   std::vector<int> zst;         // first z-sorted site of every plane (host copy)
   size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
   // insertion modes: host molecules of the aggregate centre, and the centre
@@ -690,19 +691,27 @@ template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, i
   hipLaunchKernelGGL((k_pme_ycol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_tw2),
                      dp<V>(c->d_Hk));
 }
-template <typename T, int K, int R, int XT = NTHR>
+template <typename T, int K, int R, int XT = NTHR, bool YF = false>
 int launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
   using V = typename C2<T>::type;
-  TRY(set_lds(c, (const void *)k_pme_xslab<T, K, R, XT>, lds, "k_pme_xslab"));
+  TRY(set_lds(c, (const void *)k_pme_xslab<T, K, R, XT, YF>, lds, "k_pme_xslab"));
   const int nslab = (k.K3 + c->slab_S - 1) / c->slab_S;
-  hipLaunchKernelGGL((k_pme_xslab<T, K, R, XT>), dim3(nslab, n), dim3(XT), lds, c->st, k, c->slab_S, c->xslab_Q,
+  hipLaunchKernelGGL((k_pme_xslab<T, K, R, XT, YF>), dim3(nslab, n), dim3(XT), lds, c->st, k, c->slab_S, c->xslab_Q,
                      c->nsites, dp<V>(c->d_Hk), dp<V>(c->d_tw1), dp<int>(c->d_zstart), dp<int>(c->d_zg),
                      dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite), dp<double>(c->d_esite));
   return ERM_OK;
 }
 // slab_R counts sites per thread of a 256-thread workgroup; a 512-thread
 // workgroup owns the same sites with half as many per thread
+// whether the x slab takes the inverse y transform (512-thread slabs, K2 = K1)
+bool xslab_folds_y(const erm_ctx *c) { return c->xslab_yf && c->xslab_T == 512 && c->kp.K2 == c->kp.K1; }
+
 template <typename T, int K> int launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds) {
+  if (xslab_folds_y(c)) {
+    if (c->slab_R == 8) return launch_xslab_r<T, K, 4, 512, true>(c, k, n, lds);
+    if (c->slab_R == 16) return launch_xslab_r<T, K, 8, 512, true>(c, k, n, lds);
+    return launch_xslab_r<T, K, 16, 512, true>(c, k, n, lds);
+  }
   if (c->xslab_T == 512) {
     if (c->slab_R == 8) return launch_xslab_r<T, K, 4, 512>(c, k, n, lds);
     if (c->slab_R == 16) return launch_xslab_r<T, K, 8, 512>(c, k, n, lds);
@@ -969,8 +978,10 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     TRY(rz);
     TRY(launch_check(c, "k_pme_zcol_reg"));
     TRY(mark(c, 2));
-    ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
-    TRY(launch_check(c, "k_pme_ycol_reg"));
+    if (!xslab_folds_y(c)) {
+      ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
+      TRY(launch_check(c, "k_pme_ycol_reg"));
+    }
     TRY(mark(c, 3));
     TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
     const size_t lx = 2 * sizeof(T) * (size_t)c->xslab_Q * H * K2;
@@ -2366,8 +2377,10 @@ int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
 }
 
 int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t use_graph, int32_t slab_planes,
-                       int32_t group_planes, int32_t slab_threads, int32_t column_threads) {
+                       int32_t group_planes, int32_t slab_threads, int32_t column_threads, int32_t fold_y) {
   if (!c) return ERM_E_ARG;
+  if (fold_y < -1 || fold_y > 1) return fail(c, ERM_E_ARG, "fold_y must be -1, 0 or 1");
+  if (fold_y >= 0) c->xslab_yf = (fold_y == 1);
   if (max_batch < 0 || max_batch > 4096 || pme_path < 0 || pme_path > 3 || use_graph < 0 || use_graph > 1 ||
       slab_planes < 0 || slab_planes > 16 || group_planes < 0 || group_planes > 16 ||
       (slab_threads != 0 && slab_threads != 256 && slab_threads != 512) ||
@@ -2384,9 +2397,10 @@ int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t 
   return ERM_OK;
 }
 
-int32_t erm_get_tuning(const erm_ctx *c, int32_t *out9) {
-  if (!c || !out9) return ERM_E_ARG;
-  int32_t *out8 = out9;
+int32_t erm_get_tuning(const erm_ctx *c, int32_t *out10) {
+  if (!c || !out10) return ERM_E_ARG;
+  int32_t *out8 = out10, *out9 = out10;
+  out10[9] = xslab_folds_y(c) ? 1 : 0;
   out9[8] = c->zcol_T;
   out8[0] = c->Bmax;
   out8[1] = c->P.pme_path;
@@ -2474,6 +2488,20 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
       }
     }
     c->zcol_T = bestZ ? bestZ : z0;
+    // inverse y transform inside the x slab, or its own kernel
+    if (c->xslab_T == 512 && c->kp.K2 == c->kp.K1) {
+      const bool y0 = c->xslab_yf;
+      int besty = -1;
+      for (int yf : {0, 1}) {
+        c->xslab_yf = (yf == 1);
+        TRY(timed(&ms));
+        if (ms < best) {
+          best = ms;
+          besty = yf;
+        }
+      }
+      c->xslab_yf = (besty >= 0) ? (besty == 1) : y0;
+    }
   }
   // graph replay against eager launches for the chosen shape
   c->use_graph = true;

@@ -918,7 +918,9 @@ __device__ __forceinline__ void xslab_gather(const Params &P, const T *real, int
   }
 }
 
-template <typename T, int K1, int MAXR, int XT = NTHR>
+// YF: the planes arrive before the inverse y transform, which is done here
+// per (plane, m1) column in LDS (K2 = K1), so k_pme_ycol_reg is skipped
+template <typename T, int K1, int MAXR, int XT = NTHR, bool YF = false>
 __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int nsites,
                                                     const typename C2<T>::type *__restrict__ Hk,
                                                     const typename C2<T>::type *__restrict__ tw1,
@@ -956,6 +958,21 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
       for (int e = threadIdx.x; e < cplane; e += blockDim.x) dst[e] = src[e];
     }
     __syncthreads();
+    if constexpr (YF) {
+      // inverse y transform of column (q, m1), in place (consecutive lanes
+      // take consecutive m1: distinct banks)
+      for (int t = threadIdx.x; t < nq * H; t += blockDim.x) {
+        const int q = t / H, m1 = t - q * H;
+        V *col = stage + (size_t)q * cplane + m1;
+        V X[K1];
+#pragma unroll
+        for (int m2 = 0; m2 < K1; ++m2) X[m2] = col[(size_t)m2 * H];
+        reg_fft<K1, +1>(X);
+#pragma unroll
+        for (int y = 0; y < K1; ++y) col[(size_t)y * H] = X[y];
+      }
+      __syncthreads();
+    }
     // the real row (q, y) occupies exactly the bytes of the complex row
     // (q, y) (K1P = 2H), which only this thread reads: no barrier between
     // the transform and its write, any number of rows per thread

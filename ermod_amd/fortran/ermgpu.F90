@@ -194,11 +194,11 @@ module ermgpu
      end function erm_stage_times
 
      integer(c_int32_t) function erm_set_tuning(ctx, max_batch, pme_path, use_graph, slab_planes, &
-          group_planes, slab_threads, column_threads) bind(C, name="erm_set_tuning")
+          group_planes, slab_threads, column_threads, fold_y) bind(C, name="erm_set_tuning")
        import :: c_ptr, c_int32_t
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: max_batch, pme_path, use_graph, slab_planes, group_planes, slab_threads, &
-                                    column_threads
+                                    column_threads, fold_y
      end function erm_set_tuning
 
      ! insorigin / insposition / insorient / lwreg / upreg / hostspec of
@@ -232,10 +232,10 @@ module ermgpu
        real(c_double), intent(in) :: host_crd(*), host_weight(*), slt_crd(*), slt_weight(*)
      end function erm_set_refstructure
 
-     integer(c_int32_t) function erm_get_tuning(ctx, out9) bind(C, name="erm_get_tuning")
+     integer(c_int32_t) function erm_get_tuning(ctx, out10) bind(C, name="erm_get_tuning")
        import :: c_ptr, c_int32_t
        type(c_ptr), value :: ctx
-       integer(c_int32_t), intent(out) :: out9(9)
+       integer(c_int32_t), intent(out) :: out10(10)
      end function erm_get_tuning
 
      integer(c_int32_t) function erm_autotune(ctx, n_trial, seed, best_ms) bind(C, name="erm_autotune")

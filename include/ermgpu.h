@@ -317,14 +317,16 @@ int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
  * buffers re-captures); slab_planes (1..16) caps the z planes of one fused
  * PME slab and group_planes (1..16) the planes the register path transforms
  * per group, slab_threads and column_threads (256 or 512) the workgroup
- * sizes of the register path's x slab and z columns (0 keeps any of them). */
+ * sizes of the register path's x slab and z columns (0 keeps any of them);
+ * fold_y = 1 moves the inverse y transform into 512-thread x slabs of a
+ * grid with K2 = K1, 0 keeps it in its own kernel, -1 keeps the setting. */
 int32_t erm_set_tuning(erm_ctx *ctx, int32_t max_batch, int32_t pme_path, int32_t use_graph,
                        int32_t slab_planes, int32_t group_planes, int32_t slab_threads,
-                       int32_t column_threads);
-/* out9 = {max_batch, pme_path, use_graph, slab threads (256 / 512), slab
+                       int32_t column_threads, int32_t fold_y);
+/* out10 = {max_batch, pme_path, use_graph, slab threads (256 / 512), slab
  * planes in use, group planes in use, slab cap, group cap, z-column
- * threads (256 / 512)} */
-int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out9);
+ * threads (256 / 512), y transform inside the x slab (0 / 1)} */
+int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out10);
 /* Times n_trial random insertions (seed) on the current frame for batch
  * sizes 128..1024, every applicable PME path and slab / group shapes, then
  * graph replay against eager launches, and keeps the fastest; accumulators
