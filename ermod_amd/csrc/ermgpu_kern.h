@@ -751,7 +751,9 @@ __global__ __launch_bounds__(XT) void k_pme_zcol_reg(Params P, const T *__restri
   using V = typename C2<T>::type;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int b = blockIdx.y, ns = P.ns, H = P.H, K2 = P.K2;
-  const int nsp = H + K2 + K3;
+  // staged spectra per site: x and y, and z in FP64 (FP32 reads the z
+  // spectrum through the scalar cache)
+  const int nsp = H + K2 + (sizeof(T) == 8 ? K3 : 0);
   V *l1 = reinterpret_cast<V *>(lds_raw);  // [ZREG_SCH][H]
   V *l2 = l1 + ZREG_SCH * H;               // [ZREG_SCH][K2]
   V *l3 = l2 + ZREG_SCH * K2;              // [ZREG_SCH][K3]
@@ -787,7 +789,11 @@ __global__ __launch_bounds__(XT) void k_pme_zcol_reg(Params P, const T *__restri
       const T q = lq[sl];
       a.x *= q;
       a.y *= q;
-      const V *w3 = l3 + sl * K3;
+      // the site's z spectrum is the same for every lane: FP32 reads it
+      // through the scalar cache into SGPRs (an LDS broadcast costs a vector
+      // read per four FMAs), FP64 from LDS (its 2 K3 values would not stay
+      // in SGPRs)
+      const V *w3 = (sizeof(T) == 4) ? w3h + ((size_t)b * ns + s0 + sl) * K3 : l3 + sl * K3;
 #pragma unroll
       for (int m3 = 0; m3 < K3; ++m3) {
         const V w = w3[m3];
