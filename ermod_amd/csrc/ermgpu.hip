@@ -849,12 +849,12 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(ensure(c, c->d_zw, sizeof(T) * (size_t)nsites * 3 * k.p));
   HIPCHK(c, hipMemsetAsync(c->d_zcount.p, 0, sizeof(int) * (K3 + 1), c->st));
   const int gs = (nsites + tb - 1) / tb;
-  hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, dp<int>(c->d_sg), dp<int>(c->d_zcount));
+  hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, K3, dp<int>(c->d_sg), dp<int>(c->d_zcount));
   TRY(launch_check(c, "k_zcount"));
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_zcount), dp<int>(c->d_zstart), K3);
   TRY(launch_check(c, "k_scan"));
   HIPCHK(c, hipMemcpyAsync(c->d_zfill.p, c->d_zstart.p, sizeof(int) * K3, hipMemcpyDeviceToDevice, c->st));
-  hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, k.p, dp<int>(c->d_sg), dp<T>(c->d_sq),
+  hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, K3, k.p, dp<int>(c->d_sg), dp<T>(c->d_sq),
                      dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zg), dp<T>(c->d_zq),
                      dp<T>(c->d_zw));
   TRY(launch_check(c, "k_zscatter"));
@@ -1116,7 +1116,7 @@ int run_histogram(erm_ctx *c, int n, int first) {
                      dp<double>(c->d_escrd), dp<double>(c->d_scal), dp<double>(c->d_eself),
                      dp<unsigned long long>(c->d_mnmx), dp<double>(c->d_w), dp<int>(c->d_status), first);
   TRY(launch_check(c, "k_scalars"));
-  hipLaunchKernelGGL(k_edens, dim3((E + 255) / 256), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
+  hipLaunchKernelGGL(k_edens, dim3((E + 31) / 32), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
                      dp<double>(c->d_w), dp<double>(c->d_edens));
   TRY(launch_check(c, "k_edens"));
   if (c->P.slttype == 1) {
@@ -1129,7 +1129,7 @@ int run_histogram(erm_ctx *c, int n, int first) {
     hipLaunchKernelGGL(k_intm_weight, dim3(n), dim3(256), 0, c->st, E, dp<double>(c->d_hist), dp<int>(c->d_zerodns),
                        dp<double>(c->d_w), dp<double>(c->d_wi));
     TRY(launch_check(c, "k_intm_weight"));
-    hipLaunchKernelGGL(k_edens, dim3((E + 255) / 256), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
+    hipLaunchKernelGGL(k_edens, dim3((E + 31) / 32), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
                        dp<double>(c->d_wi), dp<double>(c->d_edintm));
     TRY(launch_check(c, "k_edens"));
     hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, dp<double>(c->d_flceng),

@@ -1032,17 +1032,36 @@ __global__ void k_mol_sum(Params P, int nsites, const double *__restrict__ e_sit
 
 // per-frame sort of the solvent sites by grid base g3 (z-sorted copies of
 // base, charge and B-spline weights, the latter as [3p][nsites])
-__global__ void k_zcount(int nsites, const int *__restrict__ sg, int *zcount) {
+// counts per plane in LDS, one global add per plane and workgroup
+__global__ __launch_bounds__(256) void k_zcount(int nsites, int K3, const int *__restrict__ sg, int *zcount) {
+  __shared__ int cnt[512];
+  for (int z = threadIdx.x; z < K3; z += blockDim.x) cnt[z] = 0;
+  __syncthreads();
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nsites) atomicAdd(&zcount[(sg[t] >> 20) & 1023], 1);
+  if (t < nsites) atomicAdd(&cnt[(sg[t] >> 20) & 1023], 1);
+  __syncthreads();
+  for (int z = threadIdx.x; z < K3; z += blockDim.x)
+    if (cnt[z]) atomicAdd(&zcount[z], cnt[z]);
 }
 
+// slot of every site within its plane: a rank within the workgroup's
+// sites of that plane (LDS) plus the workgroup's base (one global add per
+// plane and workgroup)
 template <typename T>
-__global__ void k_zscatter(int nsites, int p, const int *__restrict__ sg, const T *__restrict__ sq,
-                           const T *__restrict__ sw, int *zfill, int *zsite, int *zg, T *zq, T *zw) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_zscatter(int nsites, int K3, int p, const int *__restrict__ sg,
+                                                  const T *__restrict__ sq, const T *__restrict__ sw, int *zfill,
+                                                  int *zsite, int *zg, T *zq, T *zw) {
+  __shared__ int cnt[512], base[512];
+  for (int z = threadIdx.x; z < K3; z += blockDim.x) cnt[z] = 0;
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = (t < nsites) ? ((sg[t] >> 20) & 1023) : 0;
+  const int rank = (t < nsites) ? atomicAdd(&cnt[g], 1) : 0;
+  __syncthreads();
+  for (int z = threadIdx.x; z < K3; z += blockDim.x) base[z] = cnt[z] ? atomicAdd(&zfill[z], cnt[z]) : 0;
+  __syncthreads();
   if (t >= nsites) return;
-  int zt = atomicAdd(&zfill[(sg[t] >> 20) & 1023], 1);
+  const int zt = base[g] + rank;
   zsite[zt] = t;
   zg[zt] = sg[t];
   zq[zt] = sq[t];
@@ -1755,15 +1774,14 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
   __shared__ double wsh[NTHR], twsh[NTHR];
   __shared__ int ish[NTHR];
   __shared__ double voff;
-  __shared__ unsigned long long mn, mx;
+  __shared__ unsigned long long rmn[NTHR], rmx[NTHR];
+  unsigned long long lmn = ~0ULL, lmx = 0ULL;  // this thread's extrema (ordered)
   if (threadIdx.x == 0) {
     if (P.wgtslf && scal[4] == 0.0 && n > 0) {
       scal[3] = uv[0];
       scal[4] = 1.0;
     }
     voff = scal[3];
-    mn = mnmx[0];
-    mx = mnmx[1];
   }
   __syncthreads();
   for (int b0 = 0; b0 < n; b0 += NTHR) {
@@ -1806,9 +1824,9 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
       twsh[threadIdx.x] = tw;
       ish[threadIdx.x] = bin;
       wout[b] = engnmfc;
-      unsigned long long o = dbl_to_ord(u0);
-      atomicMin(&mn, o);
-      atomicMax(&mx, o);
+      const unsigned long long o = dbl_to_ord(u0);
+      lmn = (o < lmn) ? o : lmn;
+      lmx = (o > lmx) ? o : lmx;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1826,28 +1844,50 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    mnmx[0] = mn;
-    mnmx[1] = mx;
+  rmn[threadIdx.x] = lmn;
+  rmx[threadIdx.x] = lmx;
+  __syncthreads();
+  for (int st = NTHR / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      rmn[threadIdx.x] = (rmn[threadIdx.x + st] < rmn[threadIdx.x]) ? rmn[threadIdx.x + st] : rmn[threadIdx.x];
+      rmx[threadIdx.x] = (rmx[threadIdx.x + st] > rmx[threadIdx.x]) ? rmx[threadIdx.x + st] : rmx[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && n > 0) {
+    mnmx[0] = (rmn[0] < mnmx[0]) ? rmn[0] : mnmx[0];
+    mnmx[1] = (rmx[0] > mnmx[1]) ? rmx[0] : mnmx[1];
   }
 }
 
-// edens(i) += engnmfc * insdst(i), configurations in order (engproc.F90:836-840)
+// edens(i) += engnmfc * insdst(i) (engproc.F90:836-840).  A workgroup takes
+// 32 bins; its 8 waves' halves sum 8 consecutive chunks of the
+// configurations (in order within a chunk, empty bins skipped as the
+// reference skips them), then the chunk sums are added in chunk order.
 __global__ __launch_bounds__(256) void k_edens(int n, int ermax, const double *__restrict__ hist,
                                                const double *__restrict__ w, double *edens) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ermax) return;
-  // the reference's sequential order per bin; 32 loads in flight per thread
-  double e = edens[i];
-  for (int b0 = 0; b0 < n; b0 += 32) {
-    double h[32];
+  __shared__ double part[8][32];
+  const int lb = threadIdx.x & 31, ch = threadIdx.x >> 5;
+  const int i = blockIdx.x * 32 + lb;
+  const int per = (n + 7) / 8, b0 = min(n, ch * per), b1 = min(n, b0 + per);
+  double e = 0.0;
+  if (i < ermax) {
+    for (int bb = b0; bb < b1; bb += 16) {
+      double h[16];
 #pragma unroll
-    for (int u = 0; u < 32; ++u) h[u] = (b0 + u < n) ? hist[(size_t)(b0 + u) * ermax + i] : 0.0;
+      for (int u = 0; u < 16; ++u) h[u] = (bb + u < b1) ? hist[(size_t)(bb + u) * ermax + i] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 32; ++u)
-      if (h[u] != 0.0) e = e + w[b0 + u] * h[u];
+      for (int u = 0; u < 16; ++u)
+        if (h[u] != 0.0) e = e + w[bb + u] * h[u];
+    }
   }
-  edens[i] = e;
+  part[ch][lb] = e;
+  __syncthreads();
+  if (ch == 0 && i < ermax) {
+    double t = edens[i];
+    for (int k = 0; k < 8; ++k) t = t + part[k][lb];
+    edens[i] = t;
+  }
 }
 
 // intermediate state (extensions/Intermediate/engproc.F90:996-1009): the
