@@ -670,6 +670,20 @@ template <typename T> bool reg_path_ok(const erm_ctx *c) {
   return reg_len<T>(k.K1) && reg_len<T>(k.K2) && reg_len<T>(k.K3) && (k.K1 % 2 == 0);
 }
 
+// the register path's z columns go through the z FFT for spline orders 4
+// and 6 (two lanes per column), else they sum the solute's z spectra
+bool zcol_fft(const erm_ctx *c, bool regp) { return regp && (c->kp.p == 4 || c->kp.p == 6); }
+// z-column workgroups per configuration
+int zcol_groups(const erm_ctx *c, bool regp, int vb) {
+  const int ncol = c->kp.H * c->kp.K2;
+  if (!regp) {
+    const int w = zcol_width(c->kp.K3, vb);
+    return (ncol + w - 1) / w;
+  }
+  const int zt = (c->zcol_T == 512) ? 512 : NTHR;
+  return ((zcol_fft(c, regp) ? 2 : 1) * ncol + zt - 1) / zt;
+}
+
 template <typename T, int K, int XT>
 int launch_zcol_reg_t(erm_ctx *c, const Params &k, int n, int nblk) {
   using V = typename C2<T>::type;
@@ -684,6 +698,25 @@ int launch_zcol_reg_t(erm_ctx *c, const Params &k, int n, int nblk) {
 template <typename T, int K> int launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk) {
   if (c->zcol_T == 512) return launch_zcol_reg_t<T, K, 512>(c, k, n, nblk);
   return launch_zcol_reg_t<T, K, NTHR>(c, k, n, nblk);
+}
+template <typename T, int K, int PP, int XT>
+int launch_zcol_fft_t(erm_ctx *c, const Params &k, int n, int nblk) {
+  using V = typename C2<T>::type;
+  const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT);
+  TRY(set_lds(c, (const void *)k_pme_zcol_fft<T, K, PP, XT>, lds, "k_pme_zcol_fft"));
+  hipLaunchKernelGGL((k_pme_zcol_fft<T, K, PP, XT>), dim3(nblk, n), dim3(XT), lds, c->st, k, dp<T>(c->d_engfac),
+                     dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<int>(c->d_sg3), dp<T>(c->d_sw3),
+                     dp<V>(c->d_Hk), dp<double>(c->d_selfp));
+  return ERM_OK;
+}
+// z columns spread and transformed along z (spline orders 4 and 6)
+template <typename T, int K> int launch_zcol_fft(erm_ctx *c, const Params &k, int n, int nblk) {
+  if (k.p == 4) {
+    if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 4, 512>(c, k, n, nblk);
+    return launch_zcol_fft_t<T, K, 4, NTHR>(c, k, n, nblk);
+  }
+  if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 6, 512>(c, k, n, nblk);
+  return launch_zcol_fft_t<T, K, 6, NTHR>(c, k, n, nblk);
 }
 template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, int n) {
   using V = typename C2<T>::type;
@@ -953,15 +986,15 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   TRY(ensure(c, c->d_w1h, sizeof(V) * (size_t)n * ns * H));
   TRY(ensure(c, c->d_w2h, sizeof(V) * (size_t)n * ns * K2));
   const bool regp = (flags & DO_RECIP) && c->xslab_Q > 0 && (c->P.pme_path == 0 || c->P.pme_path == 3);
-  if (regp) TRY(ensure(c, c->d_w3h, sizeof(V) * (size_t)n * ns * K3));
+  const bool zfft = zcol_fft(c, regp);  // else the preparation writes the z spectra
+  if (regp && !zfft) TRY(ensure(c, c->d_w3h, sizeof(V) * (size_t)n * ns * K3));
   TRY(ensure(c, c->d_uv, sizeof(double) * (size_t)n * (S + 1)));
   TRY(ensure(c, c->d_sltj, sizeof(int) * n));
   TRY(ensure(c, c->d_selfreal, sizeof(double) * n));
   TRY(ensure(c, c->d_usreal, sizeof(double)));
   const size_t vb = sizeof(V);
   const int ncolwg = zcol_width(K3, vb);
-  const int zt = (c->zcol_T == 512) ? 512 : NTHR;
-  const int nzg = regp ? (H * K2 + zt - 1) / zt : (H * K2 + ncolwg - 1) / ncolwg;
+  const int nzg = zcol_groups(c, regp, (int)vb);
   TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
   if (flags & DO_RECIP) TRY(ensure(c, c->d_Hk, vb * (size_t)n * K3 * K2 * H));
   TRY(mark(c, 0));
@@ -969,14 +1002,18 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   hipLaunchKernelGGL(k_solute_prep<T>, dim3(n), dim3(128), 0, c->st, k, dp<double>(c->d_spos),
                      dp<double>(c->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<V>(c->d_tw3),
                      dp<double>(c->d_sref), dp<int>(c->d_sg3), dp<T>(c->d_sw3), dp<T>(c->d_sqv),
-                     dp<V>(c->d_w1h), dp<V>(c->d_w2h), regp ? dp<V>(c->d_w3h) : (V *)nullptr);
+                     dp<V>(c->d_w1h), dp<V>(c->d_w2h), (regp && !zfft) ? dp<V>(c->d_w3h) : (V *)nullptr);
   TRY(launch_check(c, "k_solute_prep"));
   TRY(mark(c, 1));
   if (regp) {
     int rz = ERM_E_UNSUPPORTED;
-    ERM_KSWITCH(K3, (rz = launch_zcol_reg<T, KK>(c, k, n, nzg)));
+    if (zfft) {
+      ERM_KSWITCH(K3, (rz = launch_zcol_fft<T, KK>(c, k, n, nzg)));
+    } else {
+      ERM_KSWITCH(K3, (rz = launch_zcol_reg<T, KK>(c, k, n, nzg)));
+    }
     TRY(rz);
-    TRY(launch_check(c, "k_pme_zcol_reg"));
+    TRY(launch_check(c, zfft ? "k_pme_zcol_fft" : "k_pme_zcol_reg"));
     TRY(mark(c, 2));
     if (!xslab_folds_y(c)) {
       ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
@@ -2097,10 +2134,8 @@ int32_t erm_recpcal(erm_ctx *c, int32_t solute, const double *pos, double *erecp
   if (self_recp) {
     // sum of the column partials of the single configuration
     const int vb = c->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
-    const int ncolwg = zcol_width(c->kp.K3, vb);
     const bool regp = c->xslab_Q > 0 && (c->P.pme_path == 0 || c->P.pme_path == 3);
-    const int zt = (c->zcol_T == 512) ? 512 : NTHR;
-    const int nzg = regp ? (c->kp.H * c->kp.K2 + zt - 1) / zt : (c->kp.H * c->kp.K2 + ncolwg - 1) / ncolwg;
+    const int nzg = zcol_groups(c, regp, vb);
     std::vector<double> part(nzg);
     HIPCHK(c, hipMemcpy(part.data(), c->d_selfp.p, sizeof(double) * nzg, hipMemcpyDeviceToHost));
     double s = 0.0;

@@ -827,6 +827,200 @@ __global__ __launch_bounds__(XT) void k_pme_zcol_reg(Params P, const T *__restri
   }
 }
 
+// the value of the other lane of an even/odd lane pair (DPP quad_perm
+// [1, 0, 3, 2]; every lane of the wave must be active)
+__device__ __forceinline__ float pair_swap(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pair_swap(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// One value of the radix-2 step of a length-2M transform held by a lane
+// pair (lane h owns y[n + h M] at index n): s y + y' (y' the other lane's
+// value; s = 1 on lane 0, -1 on lane 1), times w on lane 1 (hs = h blends
+// w with 1).
+template <typename T, typename V> __device__ __forceinline__ void pair_step(V &y, V w, T sg, T hs) {
+  V t;
+  t.x = fma(sg, y.x, pair_swap(y.x));
+  t.y = fma(sg, y.y, pair_swap(y.y));
+  const T wr = fma(hs, w.x - (T)1, (T)1), wi = hs * w.y;
+  y.x = t.x * wr - t.y * wi;
+  y.y = t.x * wi + t.y * wr;
+}
+// (a scheduling barrier after every fourth value keeps the exchanged copies
+// from all being live at once)
+template <typename T, int M, int SIGN, typename V, int... J>
+__device__ __forceinline__ void pair_steps(V (&Y)[M], T sg, T hs, std::integer_sequence<int, J...>) {
+  ((pair_step<T>(Y[J], root_v<V, SIGN, J, 2 * M>(), sg, hs),
+    (J % 4 == 3) ? __builtin_amdgcn_sched_barrier(0) : (void)0),
+   ...);
+}
+// every value times w^J (lane 1) or 1 (lane 0), w = e^{SIGN 2 pi i / 2M}
+template <typename T, int M, int SIGN, typename V, int... J>
+__device__ __forceinline__ void pair_twiddle(V (&Y)[M], T hs, std::integer_sequence<int, J...>) {
+  auto one = [&](V &y, V w) {
+    const T wr = fma(hs, w.x - (T)1, (T)1), wi = hs * w.y;
+    const T yr = y.x * wr - y.y * wi;
+    y.y = y.x * wi + y.y * wr;
+    y.x = yr;
+  };
+  (one(Y[J], root_v<V, SIGN, J, 2 * M>()), ...);
+}
+
+// Y += a w[k] at the PP planes (G - k) mod K3 of a site at plane G, on the
+// lane pair that splits the column (lane h owns planes [h K3/2, (h+1) K3/2))
+template <int K3, int G, typename T, typename V, int... Kk>
+__device__ __forceinline__ void zspread_half(V (&Y)[K3 / 2], const V a, const T (&w)[sizeof...(Kk)], int h,
+                                             std::integer_sequence<int, Kk...>) {
+  constexpr int HK = K3 / 2;
+  auto one = [&](V &y, int own, T wk) {
+    const T ww = (h == own) ? wk : (T)0;
+    y.x = fma(a.x, ww, y.x);
+    y.y = fma(a.y, ww, y.y);
+  };
+  (one(Y[((G - Kk + K3) % K3) % HK], ((G - Kk + K3) % K3) / HK, w[Kk]), ...);
+}
+
+// dynamic LDS of k_pme_zcol_fft with XT threads: ZREG_SCH sites x (H + K2)
+// spectra + charges + the self-energy reduction
+template <typename T> __host__ __device__ inline size_t zcol_fft_lds(int H, int K2, int XT = NTHR) {
+  return (size_t)ZREG_SCH * (H + K2) * 2 * sizeof(T) + ZREG_SCH * sizeof(T) + XT * sizeof(double);
+}
+
+#define ERM_ZC(G)                                                                            \
+  case G:                                                                                    \
+    if constexpr (G < K3) zspread_half<K3, G, T>(Y, a, w, h, std::make_integer_sequence<int, PP>{}); \
+    break;
+
+// Column (m1, m2) of the same product as k_pme_zcol_reg, with the z axis
+// spread and transformed instead of summed spectrally, on a lane pair that
+// holds half of the column each:
+//  - every site adds q_s what1_s(m1) what2_s(m2) w3_s,k at its PP planes
+//    (g3 - k) mod K3 (the reference's spread along z, recpcal.F90:451-470);
+//    a site's plane and weights are the same for every lane (scalar loads)
+//    and select one of K3 unrolled register updates (a switch on a uniform
+//    value: scalar branches);
+//  - forward z FFT (the r2c transform's last axis): one radix-2 step across
+//    the pair, then a K3/2 FFT per lane (lane h: frequencies 2m + h);
+//  - Green's function and self energy (recpcal.F90:377-398);
+//  - inverse z FFT: a K3/2 FFT per lane, then the radix-2 step across the
+//    pair (lane h: planes h K3/2 + n) -> Hk.
+template <typename T, int K3, int PP, int XT = NTHR>
+__global__ __launch_bounds__(XT) void k_pme_zcol_fft(Params P, const T *__restrict__ engfac,
+                                                       const T *__restrict__ sqv,
+                                                       const typename C2<T>::type *__restrict__ w1h,
+                                                       const typename C2<T>::type *__restrict__ w2h,
+                                                       const int *__restrict__ sg3, const T *__restrict__ sw3,
+                                                       typename C2<T>::type *Hk, double *selfp) {
+  using V = typename C2<T>::type;
+  constexpr int HK = K3 / 2;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int b = blockIdx.y, ns = P.ns, H = P.H, K2 = P.K2;
+  const int nsp = H + K2;
+  V *l1 = reinterpret_cast<V *>(lds_raw);  // [ZREG_SCH][H]
+  V *l2 = l1 + ZREG_SCH * H;               // [ZREG_SCH][K2]
+  T *lq = reinterpret_cast<T *>(l2 + ZREG_SCH * K2);
+  double *red = reinterpret_cast<double *>(lds_raw + zcol_fft_lds<T>(H, K2, XT) - XT * sizeof(double));
+  const int ncol = H * K2;
+  const int h = threadIdx.x & 1;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+  const bool act = c < ncol;
+  const int m1 = act ? c % H : 0, m2 = act ? c / H : 0;
+  const T sg = h ? (T)-1 : (T)1, hs = (T)h;
+  V Y[HK];
+#pragma unroll
+  for (int z = 0; z < HK; ++z) {
+    Y[z].x = 0;
+    Y[z].y = 0;
+  }
+  for (int s0 = 0; s0 < ns; s0 += ZREG_SCH) {
+    const int nc = min(ZREG_SCH, ns - s0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nc * nsp; t += blockDim.x) {
+      const int sl = t / nsp, m = t - sl * nsp;
+      const size_t bs = (size_t)b * ns + s0 + sl;
+      if (m < H)
+        l1[sl * H + m] = w1h[bs * H + m];
+      else
+        l2[sl * K2 + (m - H)] = w2h[bs * K2 + (m - H)];
+    }
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) lq[t] = sqv[(size_t)b * ns + s0 + t];
+    __syncthreads();
+    for (int sl = 0; sl < nc; ++sl) {
+      V a = cmul(l1[sl * H + m1], l2[sl * K2 + m2]);
+      const T q = lq[sl];
+      a.x *= q;
+      a.y *= q;
+      const size_t bs = (size_t)b * ns + s0 + sl;
+      const int g3 = __builtin_amdgcn_readfirstlane(sg3[bs]);
+      T w[PP];
+#pragma unroll
+      for (int k = 0; k < PP; ++k) w[k] = sw3[bs * PP + k];
+      switch (g3) {
+        ERM_ZC(0) ERM_ZC(1) ERM_ZC(2) ERM_ZC(3) ERM_ZC(4) ERM_ZC(5) ERM_ZC(6) ERM_ZC(7)
+        ERM_ZC(8) ERM_ZC(9) ERM_ZC(10) ERM_ZC(11) ERM_ZC(12) ERM_ZC(13) ERM_ZC(14) ERM_ZC(15)
+        ERM_ZC(16) ERM_ZC(17) ERM_ZC(18) ERM_ZC(19) ERM_ZC(20) ERM_ZC(21) ERM_ZC(22) ERM_ZC(23)
+        ERM_ZC(24) ERM_ZC(25) ERM_ZC(26) ERM_ZC(27) ERM_ZC(28) ERM_ZC(29) ERM_ZC(30) ERM_ZC(31)
+        ERM_ZC(32) ERM_ZC(33) ERM_ZC(34) ERM_ZC(35) ERM_ZC(36) ERM_ZC(37) ERM_ZC(38) ERM_ZC(39)
+        ERM_ZC(40) ERM_ZC(41) ERM_ZC(42) ERM_ZC(43) ERM_ZC(44) ERM_ZC(45) ERM_ZC(46) ERM_ZC(47)
+        ERM_ZC(48) ERM_ZC(49) ERM_ZC(50) ERM_ZC(51) ERM_ZC(52) ERM_ZC(53) ERM_ZC(54) ERM_ZC(55)
+        ERM_ZC(56) ERM_ZC(57) ERM_ZC(58) ERM_ZC(59) ERM_ZC(60) ERM_ZC(61) ERM_ZC(62) ERM_ZC(63)
+        ERM_ZC(64) ERM_ZC(65) ERM_ZC(66) ERM_ZC(67) ERM_ZC(68) ERM_ZC(69) ERM_ZC(70) ERM_ZC(71)
+        ERM_ZC(72) ERM_ZC(73) ERM_ZC(74) ERM_ZC(75) ERM_ZC(76) ERM_ZC(77) ERM_ZC(78) ERM_ZC(79)
+        ERM_ZC(80) ERM_ZC(81) ERM_ZC(82) ERM_ZC(83) ERM_ZC(84) ERM_ZC(85) ERM_ZC(86) ERM_ZC(87)
+        ERM_ZC(88) ERM_ZC(89) ERM_ZC(90) ERM_ZC(91) ERM_ZC(92) ERM_ZC(93) ERM_ZC(94) ERM_ZC(95)
+        default: break;
+      }
+    }
+  }
+  // forward: radix-2 step across the pair, K3/2 FFT per lane
+  pair_steps<T, HK, -1>(Y, sg, hs, std::make_integer_sequence<int, HK>{});
+  reg_fft<HK, -1>(Y);
+  double self = 0.0;
+  const double wgt = (m1 == 0 || (P.K1 % 2 == 0 && m1 == H - 1)) ? 0.5 : 1.0;
+  // engfac[m3][m2][m1] at m3 = 2m + h: a uniform row per m plus the lane's
+  // 32-bit offset (scalar base + vector offset addressing)
+  const unsigned goff = act ? (unsigned)(h * ncol + c) : 0u;
+  const T gmask = act ? (T)1 : (T)0;
+#pragma unroll
+  for (int m = 0; m < HK; ++m) {
+    const T *grow = engfac + (size_t)(2 * m) * ncol;
+    const T g = gmask * grow[goff];
+    self += wgt * (double)g * ((double)Y[m].x * (double)Y[m].x + (double)Y[m].y * (double)Y[m].y);
+    Y[m].x *= g;
+    Y[m].y *= g;
+  }
+  red[threadIdx.x] = self;
+  // inverse: K3/2 FFT per lane, lane 1 times e^{+2 pi i n / K3}, radix-2
+  // step across the pair
+  reg_fft<HK, +1>(Y);
+  pair_twiddle<T, HK, +1>(Y, hs, std::make_integer_sequence<int, HK>{});
+#pragma unroll
+  for (int n = 0; n < HK; ++n) {
+    Y[n].x = fma(sg, Y[n].x, pair_swap(Y[n].x));
+    Y[n].y = fma(sg, Y[n].y, pair_swap(Y[n].y));
+    if (n % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+  }
+  __syncthreads();
+  for (int st = XT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) selfp[(size_t)b * gridDim.x + blockIdx.x] = red[0];
+  if (act) {
+    const unsigned hoff = (unsigned)(h * HK * ncol + c);
+#pragma unroll
+    for (int n = 0; n < HK; ++n) {
+      V *hrow = Hk + ((size_t)b * K3 + n) * ncol;
+      hrow[hoff] = Y[n];
+    }
+  }
+}
+#undef ERM_ZC
+
 template <typename T, int K2, int SIGN = +1>
 __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename C2<T>::type *__restrict__ tw2,
                                                        typename C2<T>::type *Hk) {
