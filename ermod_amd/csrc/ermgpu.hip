@@ -419,7 +419,7 @@ struct erm_ctx {
   DBuf d_pos, d_cellid, d_count, d_cell_start, d_fill, d_rm, d_mol_k, d_mol_ns, d_mol_first, d_mol_q,
       d_mol_spec, d_mol_m, d_jof;
   DBuf d_sx, d_sy, d_sz, d_sq, d_sct, d_sg, d_sw;
-  DBuf d_zcount, d_zstart, d_zfill, d_zsite, d_zg, d_zq, d_zw, d_esite;
+  DBuf d_zcount, d_zstart, d_zfill, d_zfine, d_zsite, d_zg, d_zq, d_zw, d_esite;
   int nsites = 0;             // sites of the physical molecules
   int slab_S = 0, slab_R = 0; // fused PME slab: planes per workgroup, registers per thread
   DBuf d_tw1, d_tw2, d_tw3, d_engfac;
@@ -469,7 +469,7 @@ struct erm_ctx {
     return {&d_mol_begin, &d_charge, &d_tagpt, &d_cvt, &d_mol_charge, &d_uvspec, &d_sitemass, &d_bf, &d_qslt, &d_qslt_t,
             &d_sltlj, &d_sltlj64, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
             &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
-            &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
+            &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zfine, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
             &d_w3h, &d_args, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
             &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
@@ -873,23 +873,30 @@ template <typename T> int frame_prep(erm_ctx *c) {
   if (c->P.cltype == ERM_EL_COULOMB) return mol_index(c);
   // sites sorted by PME grid base g3 for the fused slab kernel
   const int K3 = k.K3;
-  TRY(ensure(c, c->d_zcount, sizeof(int) * (K3 + 1)));
+  // buckets (g3, g2) when they fit the workgroup-local counts, else g3
+  const int KB = (K3 * k.K2 <= ZBUCKET_MAX) ? k.K2 : 1, nbk = K3 * KB;
+  TRY(ensure(c, c->d_zcount, sizeof(int) * (nbk + 1)));
+  TRY(ensure(c, c->d_zfine, sizeof(int) * (nbk + 1)));
+  TRY(ensure(c, c->d_zfill, sizeof(int) * (nbk + 1)));
   TRY(ensure(c, c->d_zstart, sizeof(int) * (K3 + 1)));
-  TRY(ensure(c, c->d_zfill, sizeof(int) * (K3 + 1)));
   TRY(ensure(c, c->d_zsite, sizeof(int) * nsites));
   TRY(ensure(c, c->d_zg, sizeof(int) * nsites));
   TRY(ensure(c, c->d_zq, sizeof(T) * nsites));
   TRY(ensure(c, c->d_zw, sizeof(T) * (size_t)nsites * 3 * k.p));
-  HIPCHK(c, hipMemsetAsync(c->d_zcount.p, 0, sizeof(int) * (K3 + 1), c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_zcount.p, 0, sizeof(int) * (nbk + 1), c->st));
   const int gs = (nsites + tb - 1) / tb;
-  hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, K3, dp<int>(c->d_sg), dp<int>(c->d_zcount));
+  hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, dp<int>(c->d_sg),
+                     dp<int>(c->d_zcount));
   TRY(launch_check(c, "k_zcount"));
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_zcount), dp<int>(c->d_zstart), K3);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_zcount), dp<int>(c->d_zfine), nbk);
   TRY(launch_check(c, "k_scan"));
-  HIPCHK(c, hipMemcpyAsync(c->d_zfill.p, c->d_zstart.p, sizeof(int) * K3, hipMemcpyDeviceToDevice, c->st));
-  hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, K3, k.p, dp<int>(c->d_sg), dp<T>(c->d_sq),
-                     dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zg), dp<T>(c->d_zq),
-                     dp<T>(c->d_zw));
+  hipLaunchKernelGGL(k_zplanes, dim3((K3 + 1 + 255) / 256), dim3(256), 0, c->st, K3, KB, dp<int>(c->d_zfine),
+                     dp<int>(c->d_zstart));
+  TRY(launch_check(c, "k_zplanes"));
+  HIPCHK(c, hipMemcpyAsync(c->d_zfill.p, c->d_zfine.p, sizeof(int) * nbk, hipMemcpyDeviceToDevice, c->st));
+  hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, k.p, dp<int>(c->d_sg),
+                     dp<T>(c->d_sq), dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zg),
+                     dp<T>(c->d_zq), dp<T>(c->d_zw));
   TRY(launch_check(c, "k_zscatter"));
   c->zst.assign(K3 + 1, 0);
   HIPCHK(c, hipMemcpyAsync(c->zst.data(), c->d_zstart.p, sizeof(int) * (K3 + 1), hipMemcpyDeviceToHost, c->st));

@@ -1148,14 +1148,20 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
   const size_t np = (size_t)nsites;
   for (int zz0 = zb - p + 1; zz0 < ze; zz0 += Q) {
     const int nq = min(Q, ze - zz0);
-    // coalesced copy of the nq complex planes (plane by plane, unrolled so
-    // that several loads are in flight)
-    for (int q = 0; q < nq; ++q) {
-      const int z = (((zz0 + q) % K3) + K3) % K3;
-      const V *src = Hk + ((size_t)b * K3 + z) * cplane;
-      V *dst = stage + (size_t)q * cplane;
-#pragma unroll 4
-      for (int e = threadIdx.x; e < cplane; e += blockDim.x) dst[e] = src[e];
+    // the nq complex planes straight into LDS: 16-byte global-to-LDS loads,
+    // one 1 KiB run per wave instruction, all in flight until the wait
+    {
+      const int ln = threadIdx.x & 63, nch = cplane * (int)sizeof(V) / 16;
+      for (int q = 0; q < nq; ++q) {
+        const int z = (((zz0 + q) % K3) + K3) % K3;
+        const uint4 *src = reinterpret_cast<const uint4 *>(Hk + ((size_t)b * K3 + z) * cplane);
+        uint4 *dst = reinterpret_cast<uint4 *>(stage + (size_t)q * cplane);
+        for (int e0 = (int)(threadIdx.x & ~63u); e0 < nch; e0 += blockDim.x)
+          if (e0 + ln < nch)
+            __builtin_amdgcn_global_load_lds(src + e0 + ln, (__attribute__((address_space(3))) void *)(dst + e0), 16,
+                                             0, 0);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
     if constexpr (YF) {
@@ -1225,34 +1231,48 @@ __global__ void k_mol_sum(Params P, int nsites, const double *__restrict__ e_sit
 }
 
 // per-frame sort of the solvent sites by grid base g3 (z-sorted copies of
-// base, charge and B-spline weights, the latter as [3p][nsites])
-// counts per plane in LDS, one global add per plane and workgroup
-__global__ __launch_bounds__(256) void k_zcount(int nsites, int K3, const int *__restrict__ sg, int *zcount) {
-  __shared__ int cnt[512];
-  for (int z = threadIdx.x; z < K3; z += blockDim.x) cnt[z] = 0;
+// base, charge and B-spline weights, the latter as [3p][nsites]); within a
+// plane by row g2 as well when KB = K2 (buckets g3 KB + g2), which puts the
+// sites of a wave on few rows of a plane for the slab gather
+constexpr int ZBUCKET_MAX = 4096;
+__device__ __forceinline__ int zbucket(int g, int KB) {
+  return ((g >> 20) & 1023) * KB + (KB > 1 ? ((g >> 10) & 1023) : 0);
+}
+
+// counts per bucket in LDS, one global add per bucket and workgroup
+__global__ __launch_bounds__(256) void k_zcount(int nsites, int nbk, int KB, const int *__restrict__ sg,
+                                                int *zcount) {
+  __shared__ int cnt[ZBUCKET_MAX];
+  for (int z = threadIdx.x; z < nbk; z += blockDim.x) cnt[z] = 0;
   __syncthreads();
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nsites) atomicAdd(&cnt[(sg[t] >> 20) & 1023], 1);
+  if (t < nsites) atomicAdd(&cnt[zbucket(sg[t], KB)], 1);
   __syncthreads();
-  for (int z = threadIdx.x; z < K3; z += blockDim.x)
+  for (int z = threadIdx.x; z < nbk; z += blockDim.x)
     if (cnt[z]) atomicAdd(&zcount[z], cnt[z]);
 }
 
-// slot of every site within its plane: a rank within the workgroup's
-// sites of that plane (LDS) plus the workgroup's base (one global add per
-// plane and workgroup)
+// first site of every plane from the bucket starts
+__global__ void k_zplanes(int K3, int KB, const int *__restrict__ fstart, int *zstart) {
+  int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z <= K3) zstart[z] = fstart[(size_t)z * KB];
+}
+
+// slot of every site within its bucket: a rank within the workgroup's sites
+// of that bucket (LDS) plus the workgroup's base (one global add per bucket
+// and workgroup)
 template <typename T>
-__global__ __launch_bounds__(256) void k_zscatter(int nsites, int K3, int p, const int *__restrict__ sg,
+__global__ __launch_bounds__(256) void k_zscatter(int nsites, int nbk, int KB, int p, const int *__restrict__ sg,
                                                   const T *__restrict__ sq, const T *__restrict__ sw, int *zfill,
                                                   int *zsite, int *zg, T *zq, T *zw) {
-  __shared__ int cnt[512], base[512];
-  for (int z = threadIdx.x; z < K3; z += blockDim.x) cnt[z] = 0;
+  __shared__ int cnt[ZBUCKET_MAX], base[ZBUCKET_MAX];
+  for (int z = threadIdx.x; z < nbk; z += blockDim.x) cnt[z] = 0;
   __syncthreads();
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = (t < nsites) ? ((sg[t] >> 20) & 1023) : 0;
+  const int g = (t < nsites) ? zbucket(sg[t], KB) : 0;
   const int rank = (t < nsites) ? atomicAdd(&cnt[g], 1) : 0;
   __syncthreads();
-  for (int z = threadIdx.x; z < K3; z += blockDim.x) base[z] = cnt[z] ? atomicAdd(&zfill[z], cnt[z]) : 0;
+  for (int z = threadIdx.x; z < nbk; z += blockDim.x) base[z] = cnt[z] ? atomicAdd(&zfill[z], cnt[z]) : 0;
   __syncthreads();
   if (t >= nsites) return;
   const int zt = base[g] + rank;
