@@ -410,6 +410,7 @@ struct erm_ctx {
   double prevcl[9] = {};
   double stat_weight_system = 1.0;
   std::vector<int> jof_k;  // sorted index of tagpt index k
+  bool jof_stale = true;   // jof_k not yet read back for this frame
   Params kp{};
   // refs rigid: realcal_self of the first configuration is reused
   bool have_usreal = false;
@@ -815,8 +816,13 @@ int mol_index(erm_ctx *c) {
   TRY(sync(c));
   c->jof_k.assign(S, -1);
   for (int j = 0; j < S; ++j) c->jof_k[molk[j]] = j;
+  c->jof_stale = false;
   return ERM_OK;
 }
+// jof_k for the current frame, read back on first use (soln-mode solutes,
+// host energies, the adjoint), so that frame preparation needs no readback
+// for inserted test particles
+int ensure_jof(erm_ctx *c) { return c->jof_stale ? mol_index(c) : ERM_OK; }
 
 template <typename T> int frame_prep(erm_ctx *c) {
   Params &k = c->kp;
@@ -870,7 +876,8 @@ template <typename T> int frame_prep(erm_ctx *c) {
                      dp<int>(c->d_mol_spec), dp<int>(c->d_mol_m));
   TRY(launch_check(c, "k_site_fill"));
   c->nsites = nsites;
-  if (c->P.cltype == ERM_EL_COULOMB) return mol_index(c);
+  c->jof_stale = true;
+  if (c->P.cltype == ERM_EL_COULOMB) return ERM_OK;
   // sites sorted by PME grid base g3 for the fused slab kernel
   const int K3 = k.K3;
   // buckets (g3, g2) when they fit the workgroup-local counts, else g3
@@ -900,7 +907,7 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(launch_check(c, "k_zscatter"));
   c->zst.assign(K3 + 1, 0);
   HIPCHK(c, hipMemcpyAsync(c->zst.data(), c->d_zstart.p, sizeof(int) * (K3 + 1), hipMemcpyDeviceToHost, c->st));
-  TRY(mol_index(c));
+  TRY(sync(c));
   return choose_slab<T>(c);
 }
 
@@ -939,7 +946,8 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
       if (k.ins_org == ERM_INSORG_AGGCEN && !c->d_aggcen.p)
         return fail(c, ERM_E_ENG_BUG, "aggregate centre missing");
       if (bt.flex) TRY(upload(c, c->d_flexbf, bt.flex, (size_t)n * ns * 3));
-      hipLaunchKernelGGL(k_pose, dim3((n + 63) / 64), dim3(64), 0, c->st, k, n, dp<uint64_t>(c->d_args),
+      const size_t lpose = (ns <= POSE_LDS_SITES) ? sizeof(double) * 4 * ns : 0;
+      hipLaunchKernelGGL(k_pose, dim3(n), dim3(64), lpose, c->st, k, n, dp<uint64_t>(c->d_args),
                          (uint64_t)bt.first, bt.flex ? dp<double>(c->d_flexbf) : dp<double>(c->d_bf),
                          bt.flex ? 3 * ns : 0, dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
                          dp<double>(c->d_cellm), dp<double>(c->d_aggcen), dp<double>(c->d_spos));
@@ -955,6 +963,7 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
                                sizeof(double) * ns * 3, hipMemcpyDeviceToDevice, c->st));
   }
   if (bt.solutes) {
+    TRY(ensure_jof(c));
     std::vector<int> sltj(n, -1);
     for (int b = 0; b < n; ++b) {
       int kk = c->kof[bt.solutes[b]];
@@ -2266,6 +2275,7 @@ int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, cons
   for (int s0 = 0; s0 < n; s0 += c->Bmax) {
     int nb = std::min(c->Bmax, n - s0);
     // energies into the sorted order
+    TRY(ensure_jof(c));
     std::vector<double> uvs((size_t)nb * (S + 1));
     for (int b = 0; b < nb; ++b) {
       const double *src = uvengy + (size_t)(s0 + b) * (S + 1);
@@ -2636,6 +2646,7 @@ int32_t erm_solute_gradient(erm_ctx *c, int32_t n, const int32_t *solutes, const
   if (c->P.slttype == 1) {
     int m = c->sltlist[0];
     qsolv = qsolv - c->mol_charge[m];
+    TRY(ensure_jof(c));
     g.excl = c->jof_k[c->kof[m]];
   }
   g.rf = -3.1415926535897932 * qsolv / c->P.screen / c->P.screen / c->volume;

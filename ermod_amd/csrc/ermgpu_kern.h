@@ -263,35 +263,52 @@ constexpr uint64_t POS_STREAM = 0xD1B54A32D192ED03ULL;
 // of the frame (AGGCEN only).  bf_stride 0: the rigid body bf for every
 // insertion; 3 * ns: body b of a flexible solute (getsolute's SltConf
 // configurations, insertion.F90:384-480).
-__global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
-                       const double *__restrict__ bf, int bf_stride, const double *__restrict__ mass,
-                       const double *__restrict__ cellm, const double *__restrict__ aggcen, double *spos) {
+// One 64-lane workgroup per insertion: the lanes update the sites in
+// parallel and every lane forms the same scalars (draws, centres of mass in
+// site order) from the positions, kept in LDS when 4 ns doubles fit
+// (POSE_LDS_SITES), else in spos itself.
+constexpr int POSE_LDS_SITES = 2048;
+__global__ __launch_bounds__(64) void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
+                                             const double *__restrict__ bf, int bf_stride,
+                                             const double *__restrict__ mass, const double *__restrict__ cellm,
+                                             const double *__restrict__ aggcen, double *spos) {
 #pragma clang fp contract(off)
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  extern __shared__ double pose_lds[];
+  const int b = blockIdx.x;
   if (b >= n) return;
-  const int ns = P.ns;
+  const int ns = P.ns, ln = threadIdx.x, nl = blockDim.x;
   const uint64_t seed = args[0], sp = seed ^ POS_STREAM;
-  uint64_t index = args[1] + offset + (uint64_t)b;
+  const uint64_t index = args[1] + offset + (uint64_t)b;
   double *out = spos + (size_t)b * ns * 3;
   const double *body = bf + (size_t)b * bf_stride;  // one body per insertion for a flexible solute
-  for (int i = 0; i < 3 * ns; ++i) out[i] = body[i];
+  const bool in_lds = ns <= POSE_LDS_SITES;
+  double *x = in_lds ? pose_lds : out;
+  const double *ms = in_lds ? pose_lds + 3 * ns : mass;
+  for (int i = ln; i < 3 * ns; i += nl) x[i] = body[i];
+  if (in_lds)
+    for (int j = ln; j < ns; j += nl) pose_lds[3 * ns + j] = mass[j];
+  __syncthreads();
   double c[3];
   if (P.ins_org != 1) {
     // COM to the origin or to the aggregate centre (set_solute_com)
     const double o[3] = {P.ins_org == 2 ? aggcen[0] : 0.0, P.ins_org == 2 ? aggcen[1] : 0.0,
                          P.ins_org == 2 ? aggcen[2] : 0.0};
-    com_dev(ns, out, mass, c);
-    for (int j = 0; j < ns; ++j)
-      for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + o[i];
+    com_dev(ns, x, ms, c);
+    __syncthreads();
+    for (int j = ln; j < ns; j += nl)
+      for (int i = 0; i < 3; ++i) x[3 * j + i] = (x[3 * j + i] - c[i]) + o[i];
+    __syncthreads();
   }
   if (P.ins_pos == 0) {
     // fully random position in the cell: COM = cell . r
     double r[3], com[3];
     for (int i = 0; i < 3; ++i) r[i] = rng_uniform(seed, index, (uint32_t)i);
     for (int i = 0; i < 3; ++i) com[i] = cellm[i] * r[0] + cellm[3 + i] * r[1] + cellm[6 + i] * r[2];
-    com_dev(ns, out, mass, c);
-    for (int j = 0; j < ns; ++j)
-      for (int i = 0; i < 3; ++i) out[3 * j + i] = (out[3 * j + i] - c[i]) + com[i];
+    com_dev(ns, x, ms, c);
+    __syncthreads();
+    for (int j = ln; j < ns; j += nl)
+      for (int i = 0; i < 3; ++i) x[3 * j + i] = (x[3 * j + i] - c[i]) + com[i];
+    __syncthreads();
   } else if (P.ins_pos == 2) {
     // shell lwreg <= |shift| <= upreg, uniform in volume, direction by
     // rejection from the cube
@@ -310,8 +327,9 @@ __global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint6
       }
     }
     nrm = sqrt(nrm);
-    for (int j = 0; j < ns; ++j)
-      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + t * r[i] / nrm;
+    for (int j = ln; j < ns; j += nl)
+      for (int i = 0; i < 3; ++i) x[3 * j + i] = x[3 * j + i] + t * r[i] / nrm;
+    __syncthreads();
   } else if (P.ins_pos == 3 || P.ins_pos == 4) {
     // slab: random in-plane position, lwreg <= z offset <= upreg (either
     // sign with equal probability for the symmetric slab)
@@ -328,8 +346,9 @@ __global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint6
     r[2] = t / len3;
     double com[3];
     for (int i = 0; i < 3; ++i) com[i] = cellm[i] * r[0] + cellm[3 + i] * r[1] + cellm[6 + i] * r[2];
-    for (int j = 0; j < ns; ++j)
-      for (int i = 0; i < 3; ++i) out[3 * j + i] = out[3 * j + i] + com[i];
+    for (int j = ln; j < ns; j += nl)
+      for (int i = 0; i < 3; ++i) x[3 * j + i] = x[3 * j + i] + com[i];
+    __syncthreads();
   }
   if (P.ins_rot == 0) {
     // uniform rotation: quaternion by rejection from the 4-cube
@@ -344,16 +363,20 @@ __global__ void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint6
         break;
       }
     }
-    com_dev(ns, out, mass, c);
+    com_dev(ns, x, ms, c);
+    __syncthreads();
     double cj[4] = {q[0], -q[1], -q[2], -q[3]};
-    for (int j = 0; j < ns; ++j) {
-      double q2[4] = {0.0, out[3 * j] - c[0], out[3 * j + 1] - c[1], out[3 * j + 2] - c[2]};
+    for (int j = ln; j < ns; j += nl) {
+      double q2[4] = {0.0, x[3 * j] - c[0], x[3 * j + 1] - c[1], x[3 * j + 2] - c[2]};
       double t[4], r2[4];
       qprod_dev(q, q2, t);
       qprod_dev(t, cj, r2);
-      for (int i = 0; i < 3; ++i) out[3 * j + i] = r2[1 + i] + c[i];
+      for (int i = 0; i < 3; ++i) x[3 * j + i] = r2[1 + i] + c[i];
     }
+    __syncthreads();
   }
+  if (in_lds)
+    for (int i = ln; i < 3 * ns; i += nl) out[i] = x[i];
 }
 
 template <typename T>
@@ -1936,6 +1959,19 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
   __syncthreads();
   const double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
   const int sj = slt_j[b];
+  // runs of equal species and equal bins along a thread's molecules are
+  // summed in registers and added to LDS once per run (the far molecules of
+  // a configuration mostly share a species and a bin near zero)
+  int rs = -1, rb = -1, rc = 0;
+  double rsum = 0.0;
+  unsigned long long rmn = ~0ULL, rmx = 0ULL;
+  auto flush_spec = [&]() {
+    if (rs >= 0) {
+      atomicAdd(&flc[rs], rsum);
+      atomicMin(&mn[rs], rmn);
+      atomicMax(&mx[rs], rmx);
+    }
+  };
   for (int j = threadIdx.x; j < P.slvmax; j += blockDim.x) {
     if (j == sj) continue;
     int pti = mol_spec[j];
@@ -1962,11 +1998,27 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
       continue;
     }
     if (lo == n && v < 1.0e3 && !P.force) set_error(status, 3, first + b, j, v);
-    atomicAdd(&h[off + lo - 1], 1);
-    atomicAdd(&flc[pti - 1], v);
-    atomicMin(&mn[pti - 1], dbl_to_ord(v));
-    atomicMax(&mx[pti - 1], dbl_to_ord(v));
+    const int bin = off + lo - 1;
+    if (bin != rb) {
+      if (rc) atomicAdd(&h[rb], rc);
+      rb = bin;
+      rc = 0;
+    }
+    ++rc;
+    if (pti - 1 != rs) {
+      flush_spec();
+      rs = pti - 1;
+      rsum = 0.0;
+      rmn = ~0ULL;
+      rmx = 0ULL;
+    }
+    rsum += v;
+    const unsigned long long o = dbl_to_ord(v);
+    rmn = (o < rmn) ? o : rmn;
+    rmx = (o > rmx) ? o : rmx;
   }
+  if (rc) atomicAdd(&h[rb], rc);
+  flush_spec();
   __syncthreads();
   for (int t = threadIdx.x; t < P.ermax; t += blockDim.x) hist[(size_t)b * P.ermax + t] = (double)h[t];
   if (threadIdx.x < P.numslv) {
@@ -2043,19 +2095,31 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
       lmx = (o > lmx) ? o : lmx;
     }
     __syncthreads();
+    const int m = min(NTHR, n - b0);
     if (threadIdx.x == 0) {
-      const int m = min(NTHR, n - b0);
       double en = scal[0], es = scal[1], av = scal[2];
       for (int i = 0; i < m; ++i) {
         en = en + wsh[i];
         es = es + 1.0;
         av = av + twsh[i];
-        if (ish[i] >= 0) eself[ish[i]] = eself[ish[i]] + wsh[i];
       }
       scal[0] = en;
       scal[1] = es;
       scal[2] = av;
     }
+    // self-energy bins, one thread per bin, each adding its configurations
+    // in configuration order (the same sums as the sequential loop)
+    if (P.selfcal)
+      for (int e = threadIdx.x; e < P.esmax; e += NTHR) {
+        double v = eself[e];
+        bool hit = false;
+        for (int i = 0; i < m; ++i)
+          if (ish[i] == e) {
+            v = v + wsh[i];
+            hit = true;
+          }
+        if (hit) eself[e] = v;
+      }
     __syncthreads();
   }
   rmn[threadIdx.x] = lmn;
