@@ -1,0 +1,62 @@
+"""A frame whose solvent is packed into a band of z planes after a uniform
+frame: the slab shape is chosen again for the packed frame (four times the
+sites per slab of the uniform one), and the per-molecule energies on it
+match the oracle: FP64 within 1e-9 kcal/mol + 1e-11 relative, FP32 within
+1e-4 kcal/mol (1e-5 relative above 100 kcal/mol)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from ermod_amd import synth
+from ermod_amd.api import ErmGPU
+
+pytestmark = pytest.mark.gpu
+
+
+def print_case(name):
+    print(f"\n[gpu case] {name}", flush=True)
+
+
+def packed_frame(sy, pos, frac):
+    """Every solvent molecule moved rigidly so that its first site's z lies in
+    [0, frac L)."""
+    s = sy.system
+    out = pos.copy()
+    L = sy.L
+    for m in range(s.nmol):
+        if m == sy.solute_mol:
+            continue
+        b, e = s.mol_begin[m], s.mol_begin[m + 1]
+        z0 = out[b, 2] % L
+        out[b:e, 2] += z0 * frac - out[b, 2]
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_dense_frame_after_uniform_frame(prec):
+    print_case(f"slab passes on a packed frame {prec}")
+    sy = synth.refs20()
+    s = sy.system
+    m = sy.solute_mol
+    dense = packed_frame(sy, sy.frame(1), 0.3)
+    # the test particle in the empty part of the box, beyond the cutoff of the band
+    confs = []
+    for i in range(6):
+        r, q = O.random_pose(5, i)
+        r = np.array([r[0], r[1], 0.66])
+        confs.append(O.place_solute(s.bfcoord, s.solute_mass(m), sy.cell, r, q))
+    confs = np.array(confs)
+    o = O.Oracle(s)
+    o.set_frame(dense, sy.cell)
+    with ErmGPU(s, prec, max_batch=8) as g:
+        g.set_frame(sy.frame(0), sy.cell)
+        g.set_frame(dense, sy.cell)
+        u = g.engconst(solute_pos=confs, want_uv=True)
+    for i in range(len(confs)):
+        want = o.uv_energy(m, confs[i])
+        if prec == "fp64":
+            np.testing.assert_allclose(u[i], want, rtol=1e-11, atol=1e-9)
+        else:
+            err = np.abs(u[i, 1:] - want[1:])
+            lim = np.where(np.abs(want[1:]) <= 100.0, 1e-4, 1e-5 * np.abs(want[1:]))
+            assert (err <= lim).all(), (i, err.max())
