@@ -719,11 +719,18 @@ template <typename T, int K> int launch_zcol_fft(erm_ctx *c, const Params &k, in
   if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 6, 512>(c, k, n, nblk);
   return launch_zcol_fft_t<T, K, 6, NTHR>(c, k, n, nblk);
 }
+// inverse y columns: two lanes per column in FP64 (one lane's column does
+// not leave room for a second wave), one in FP32
 template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, int n) {
   using V = typename C2<T>::type;
-  int nblk = (k.K3 * k.H + NTHR - 1) / NTHR;
-  hipLaunchKernelGGL((k_pme_ycol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_tw2),
-                     dp<V>(c->d_Hk));
+  if constexpr (sizeof(T) == 8) {
+    int nblk = (2 * k.K3 * k.H + NTHR - 1) / NTHR;
+    hipLaunchKernelGGL((k_pme_ycol_pair<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_Hk));
+  } else {
+    int nblk = (k.K3 * k.H + NTHR - 1) / NTHR;
+    hipLaunchKernelGGL((k_pme_ycol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_tw2),
+                       dp<V>(c->d_Hk));
+  }
 }
 template <typename T, int K, int R, int XT = NTHR, bool YF = false>
 int launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
@@ -1033,7 +1040,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     TRY(mark(c, 2));
     if (!xslab_folds_y(c)) {
       ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
-      TRY(launch_check(c, "k_pme_ycol_reg"));
+      TRY(launch_check(c, "y columns"));
     }
     TRY(mark(c, 3));
     TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));

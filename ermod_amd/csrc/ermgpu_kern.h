@@ -1044,6 +1044,33 @@ __global__ __launch_bounds__(XT) void k_pme_zcol_fft(Params P, const T *__restri
 }
 #undef ERM_ZC
 
+// Inverse y transform of column (z, m1) in place on a lane pair: lane h
+// loads m2 in [h K2/2, (h+1) K2/2), the radix-2 step across the pair (DPP)
+// and a K2/2 FFT per lane leave y = 2k + h in lane h (half the registers of
+// one lane per column, so more waves hide the loads)
+template <typename T, int K2>
+__global__ __launch_bounds__(NTHR) void k_pme_ycol_pair(Params P, typename C2<T>::type *Hk) {
+  using V = typename C2<T>::type;
+  constexpr int HK = K2 / 2;
+  const int b = blockIdx.y, H = P.H, K3 = P.K3;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = t & 1, c = t >> 1;  // c = (z, m1)
+  const bool act = c < K3 * H;
+  const int cc = act ? c : 0;  // idle lanes of the last workgroup read column 0 and store nothing
+  const int z = cc / H, m1 = cc - z * H;
+  V *col = Hk + ((size_t)b * K3 + z) * K2 * H + m1;
+  V Y[HK];
+#pragma unroll
+  for (int j = 0; j < HK; ++j) Y[j] = col[(size_t)(h * HK + j) * H];
+  const T sg = h ? (T)-1 : (T)1, hs = (T)h;
+  pair_steps<T, HK, +1>(Y, sg, hs, std::make_integer_sequence<int, HK>{});
+  reg_fft<HK, +1>(Y);
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < HK; ++k) col[(size_t)(2 * k + h) * H] = Y[k];
+  }
+}
+
 template <typename T, int K2, int SIGN = +1>
 __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename C2<T>::type *__restrict__ tw2,
                                                        typename C2<T>::type *Hk) {
