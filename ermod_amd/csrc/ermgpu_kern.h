@@ -895,16 +895,18 @@ __device__ __forceinline__ void pair_twiddle(V (&Y)[M], T hs, std::integer_seque
 
 // Y += a w[k] at the PP planes (G - k) mod K3 of a site at plane G, on the
 // lane pair that splits the column (lane h owns planes [h K3/2, (h+1) K3/2))
+// (hm[o] is 1 on the lane that owns half o, else 0: one multiply selects
+// the weight)
 template <int K3, int G, typename T, typename V, int... Kk>
-__device__ __forceinline__ void zspread_half(V (&Y)[K3 / 2], const V a, const T (&w)[sizeof...(Kk)], int h,
-                                             std::integer_sequence<int, Kk...>) {
+__device__ __forceinline__ void zspread_half(V (&Y)[K3 / 2], const V a, const T (&w)[sizeof...(Kk)],
+                                             const T (&hm)[2], std::integer_sequence<int, Kk...>) {
   constexpr int HK = K3 / 2;
-  auto one = [&](V &y, int own, T wk) {
-    const T ww = (h == own) ? wk : (T)0;
+  auto one = [&](V &y, T hmo, T wk) {
+    const T ww = hmo * wk;
     y.x = fma(a.x, ww, y.x);
     y.y = fma(a.y, ww, y.y);
   };
-  (one(Y[((G - Kk + K3) % K3) % HK], ((G - Kk + K3) % K3) / HK, w[Kk]), ...);
+  (one(Y[((G - Kk + K3) % K3) % HK], hm[((G - Kk + K3) % K3) / HK], w[Kk]), ...);
 }
 
 // dynamic LDS of k_pme_zcol_fft with XT threads: ZREG_SCH sites x (H + K2)
@@ -915,7 +917,7 @@ template <typename T> __host__ __device__ inline size_t zcol_fft_lds(int H, int 
 
 #define ERM_ZC(G)                                                                            \
   case G:                                                                                    \
-    if constexpr (G < K3) zspread_half<K3, G, T>(Y, a, w, h, std::make_integer_sequence<int, PP>{}); \
+    if constexpr (G < K3) zspread_half<K3, G, T>(Y, a, w, hm, std::make_integer_sequence<int, PP>{}); \
     break;
 
 // Column (m1, m2) of the same product as k_pme_zcol_reg, with the z axis
@@ -953,6 +955,7 @@ __global__ __launch_bounds__(XT) void k_pme_zcol_fft(Params P, const T *__restri
   const bool act = c < ncol;
   const int m1 = act ? c % H : 0, m2 = act ? c / H : 0;
   const T sg = h ? (T)-1 : (T)1, hs = (T)h;
+  const T hm[2] = {(T)(1 - h), (T)h};
   V Y[HK];
 #pragma unroll
   for (int z = 0; z < HK; ++z) {
@@ -1184,8 +1187,15 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
   const int K2 = P.K2, K3 = P.K3, p = P.p;
   const int cplane = K2 * H;                  // complex elements per plane
   V *stage = reinterpret_cast<V *>(lds_raw);  // [Q][K2][H] complex, then [Q][K2][K1P] real in place
-  const int b = blockIdx.y;
-  const int zb = blockIdx.x * S, ze = min(zb + S, K3);
+  // XCD-aware order: workgroups go to the 8 XCDs round-robin by linear
+  // index, so XCD x is given the x-th contiguous share of the (configuration,
+  // slab) items; consecutive slabs of a configuration then run on one XCD
+  // and find their p - 1 shared halo planes in its L2
+  const int nwg = gridDim.x * gridDim.y;
+  int L = blockIdx.y * gridDim.x + blockIdx.x;
+  if (nwg % 8 == 0) L = (L % 8) * (nwg / 8) + L / 8;
+  const int b = L / gridDim.x;
+  const int zb = (L - b * gridDim.x) * S, ze = min(zb + S, K3);
   const int s_beg = z_start[zb], s_end = z_start[ze];
   double acc[MAXR];
   int gz[MAXR];
