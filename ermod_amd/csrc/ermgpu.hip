@@ -859,21 +859,30 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(ensure(c, c->d_sct, sizeof(int) * nsites));
   TRY(ensure(c, c->d_sg, sizeof(int) * nsites));
   TRY(ensure(c, c->d_sw, sizeof(T) * (size_t)nsites * 3 * k.p));
-  HIPCHK(c, hipMemsetAsync(c->d_count.p, 0, sizeof(int) * (ncell + 1), c->st));
-  unsigned long long zero = 0;
-  HIPCHK(c, hipMemcpyAsync(c->d_rm.p, &zero, sizeof zero, hipMemcpyHostToDevice, c->st));
+  // z buckets of the site sort below: (g3, g2) when they fit the
+  // workgroup-local counts, else g3
+  const int K3 = k.K3;
+  const bool zsort = (c->P.cltype != ERM_EL_COULOMB);
+  const int KB = (K3 * k.K2 <= ZBUCKET_MAX) ? k.K2 : 1, nbk = K3 * KB;
+  if (zsort) TRY(ensure(c, c->d_zcount, sizeof(int) * (nbk + 1)));
+  const int nclr = std::max(ncell + 1, zsort ? nbk + 1 : 0);
+  hipLaunchKernelGGL(k_frame_clear, dim3((nclr + 255) / 256), dim3(256), 0, c->st, dp<int>(c->d_count), ncell + 1,
+                     zsort ? dp<int>(c->d_zcount) : (int *)nullptr, zsort ? nbk + 1 : 0,
+                     dp<unsigned long long>(c->d_rm));
+  TRY(launch_check(c, "k_frame_clear"));
   const int tb = 256, gb = (S + tb - 1) / tb;
   hipLaunchKernelGGL(k_mol_cell, dim3(gb), dim3(tb), 0, c->st, k, dp<double>(c->d_pos), dp<int>(c->d_tagpt),
                      dp<int>(c->d_mol_begin), dp<int>(c->d_cellid), dp<int>(c->d_count),
                      dp<unsigned long long>(c->d_rm));
   TRY(launch_check(c, "k_mol_cell"));
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_count), dp<int>(c->d_cell_start), ncell);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_count), dp<int>(c->d_cell_start), ncell,
+                     dp<int>(c->d_fill), (int *)nullptr, 0, 0);
   TRY(launch_check(c, "k_scan"));
-  HIPCHK(c, hipMemcpyAsync(c->d_fill.p, c->d_cell_start.p, sizeof(int) * ncell, hipMemcpyDeviceToDevice, c->st));
   hipLaunchKernelGGL(k_mol_scatter, dim3(gb), dim3(tb), 0, c->st, k, dp<int>(c->d_cellid), dp<int>(c->d_fill),
                      dp<int>(c->d_tagpt), dp<int>(c->d_mol_begin), dp<int>(c->d_mol_k), dp<int>(c->d_mol_ns));
   TRY(launch_check(c, "k_mol_scatter"));
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_mol_ns), dp<int>(c->d_mol_first), S);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_mol_ns), dp<int>(c->d_mol_first), S,
+                     (int *)nullptr, (int *)nullptr, 0, 0);
   TRY(launch_check(c, "k_scan"));
   hipLaunchKernelGGL(k_site_fill<T>, dim3(gb), dim3(tb), 0, c->st, k, dp<double>(c->d_pos), dp<int>(c->d_tagpt),
                      dp<int>(c->d_mol_begin), dp<double>(c->d_charge), dp<int>(c->d_cvt),
@@ -884,12 +893,8 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(launch_check(c, "k_site_fill"));
   c->nsites = nsites;
   c->jof_stale = true;
-  if (c->P.cltype == ERM_EL_COULOMB) return ERM_OK;
-  // sites sorted by PME grid base g3 for the fused slab kernel
-  const int K3 = k.K3;
-  // buckets (g3, g2) when they fit the workgroup-local counts, else g3
-  const int KB = (K3 * k.K2 <= ZBUCKET_MAX) ? k.K2 : 1, nbk = K3 * KB;
-  TRY(ensure(c, c->d_zcount, sizeof(int) * (nbk + 1)));
+  if (!zsort) return ERM_OK;
+  // sites sorted by PME grid base (g3, g2) for the slab kernels
   TRY(ensure(c, c->d_zfine, sizeof(int) * (nbk + 1)));
   TRY(ensure(c, c->d_zfill, sizeof(int) * (nbk + 1)));
   TRY(ensure(c, c->d_zstart, sizeof(int) * (K3 + 1)));
@@ -897,17 +902,13 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(ensure(c, c->d_zg, sizeof(int) * nsites));
   TRY(ensure(c, c->d_zq, sizeof(T) * nsites));
   TRY(ensure(c, c->d_zw, sizeof(T) * (size_t)nsites * 3 * k.p));
-  HIPCHK(c, hipMemsetAsync(c->d_zcount.p, 0, sizeof(int) * (nbk + 1), c->st));
   const int gs = (nsites + tb - 1) / tb;
   hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, dp<int>(c->d_sg),
                      dp<int>(c->d_zcount));
   TRY(launch_check(c, "k_zcount"));
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_zcount), dp<int>(c->d_zfine), nbk);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_zcount), dp<int>(c->d_zfine), nbk,
+                     dp<int>(c->d_zfill), dp<int>(c->d_zstart), K3, KB);
   TRY(launch_check(c, "k_scan"));
-  hipLaunchKernelGGL(k_zplanes, dim3((K3 + 1 + 255) / 256), dim3(256), 0, c->st, K3, KB, dp<int>(c->d_zfine),
-                     dp<int>(c->d_zstart));
-  TRY(launch_check(c, "k_zplanes"));
-  HIPCHK(c, hipMemcpyAsync(c->d_zfill.p, c->d_zfine.p, sizeof(int) * nbk, hipMemcpyDeviceToDevice, c->st));
   hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, k.p, dp<int>(c->d_sg),
                      dp<T>(c->d_sq), dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zg),
                      dp<T>(c->d_zq), dp<T>(c->d_zw));
@@ -1455,7 +1456,7 @@ int set_frame_common(erm_ctx *c, const double *cell, double w) {
   memcpy(c->prevcl, cell, sizeof c->prevcl);
   c->have_prevcl = true;
   frame_params(c);
-  TRY(upload(c, c->d_cellm, c->cell, 9));
+  if (changed) TRY(upload(c, c->d_cellm, c->cell, 9));
   return ERM_OK;
 }
 

@@ -91,8 +91,12 @@ __global__ void k_mol_cell(Params P, const double *__restrict__ pos, const int *
   atomicMax(rm_ord, dbl_to_ord(rm));
 }
 
-// exclusive scan of n ints by one workgroup of 1024 threads; out[n] = total
-__global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *out, int n) {
+// Exclusive scan of in[0, n) into out[0, n], out[n] the total (one
+// workgroup).  copy: a second copy of out[0, n) (the fill cursors of a
+// counting sort), or null; planes: planes[z] = out[z KB] for z <= K3, or
+// null.
+__global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *out, int n, int *copy, int *planes,
+                                               int K3, int KB) {
   __shared__ int part[1024];
   const int t = threadIdx.x;
   const int chunk = (n + 1023) / 1024;
@@ -113,9 +117,23 @@ __global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *
     if (b0 + i < n) {
       int v = in[b0 + i];
       out[b0 + i] = run;
+      if (copy) copy[b0 + i] = run;
       run += v;
     }
   if (t == 1023) out[n] = part[1023];
+  if (planes) {
+    __syncthreads();
+    for (int z = t; z <= K3; z += blockDim.x) planes[z] = out[(size_t)z * KB];
+  }
+}
+
+// the per-frame counters zeroed in one launch: cell counts, z-bucket
+// counts (nz may be 0) and the molecule-extent maximum
+__global__ void k_frame_clear(int *count, int nc, int *zcount, int nz, unsigned long long *rm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) count[i] = 0;
+  if (i < nz) zcount[i] = 0;
+  if (i == 0) *rm = 0ULL;
 }
 
 __global__ void k_mol_scatter(Params P, const int *__restrict__ cellid, int *fill,
@@ -1312,11 +1330,6 @@ __global__ __launch_bounds__(256) void k_zcount(int nsites, int nbk, int KB, con
     if (cnt[z]) atomicAdd(&zcount[z], cnt[z]);
 }
 
-// first site of every plane from the bucket starts
-__global__ void k_zplanes(int K3, int KB, const int *__restrict__ fstart, int *zstart) {
-  int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z <= K3) zstart[z] = fstart[(size_t)z * KB];
-}
 
 // slot of every site within its bucket: a rank within the workgroup's sites
 // of that bucket (LDS) plus the workgroup's base (one global add per bucket
