@@ -2286,12 +2286,29 @@ __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__
     int pr = p0 + kq + 4 * v, ic = i0 + r;
     acc[v] = (pr < ermax && ic < ermax) ? ecorr[(size_t)ic * ermax + pr] : 0.0;
   }
-  for (int b0 = 0; b0 < n; b0 += 4) {
+  // U steps of 4 configurations per round: all 2U operand loads issued
+  // before the U dependent MFMAs (the same MFMA sequence as one step at a
+  // time, so the same sums)
+  constexpr int U = 8;
+  const bool pin = p0 + r < ermax, iin = i0 + r < ermax;
+  int b0 = 0;
+  for (; b0 + 4 * U <= n; b0 += 4 * U) {
+    double a[U], bb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + 4 * u + kq;
+      a[u] = pin ? hist[(size_t)b * ermax + p0 + r] : 0.0;
+      bb[u] = iin ? w[b] * hist[(size_t)b * ermax + i0 + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[u], acc, 0, 0, 0);
+  }
+  for (; b0 < n; b0 += 4) {
     int b = b0 + kq;
     double a = 0.0, bb = 0.0;
     if (b < n) {
-      if (p0 + r < ermax) a = hist[(size_t)b * ermax + p0 + r];
-      if (i0 + r < ermax) bb = w[b] * hist[(size_t)b * ermax + i0 + r];
+      if (pin) a = hist[(size_t)b * ermax + p0 + r];
+      if (iin) bb = w[b] * hist[(size_t)b * ermax + i0 + r];
     }
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
   }
