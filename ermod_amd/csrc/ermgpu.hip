@@ -1203,8 +1203,8 @@ int run_histogram(erm_ctx *c, int n, int first) {
   TRY(mark(c, 6));
   if (c->P.corrcal) {
     int nt = (E + 15) / 16;
-    int waves = nt * nt;
-    hipLaunchKernelGGL(k_corr, dim3((waves + 3) / 4), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
+    int tiles = nt * (nt + 1) / 2;  // tiles on and above the diagonal, one workgroup each
+    hipLaunchKernelGGL(k_corr, dim3(tiles), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
                        dp<double>(c->d_w), dp<double>(c->d_ecorr));
     TRY(launch_check(c, "k_corr"));
   }

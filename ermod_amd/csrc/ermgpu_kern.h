@@ -2266,33 +2266,38 @@ __global__ void k_slnuv(int n, int numslv, const double *__restrict__ flceng, co
 // Correlation matrix on FP64 MFMA: ecorr(p, i) += sum_b (w_b h_b[i]) h_b[p]
 // (engproc.F90:841-853), one 16 x 16 tile per wave, v_mfma_f64_16x16x4f64
 // over the batch in steps of 4 configurations.  ecorr column-major
-// (element (p, i) at p + i * ermax).
+// (element (p, i) at p + i * ermax).  The matrix is symmetric: the
+// workgroups take the tiles on and above the diagonal (p tile <= i tile),
+// and an off-diagonal tile also writes its transpose.
 // --------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__restrict__ hist,
                                               const double *__restrict__ w, double *ecorr) {
-  const int lane = threadIdx.x & 63;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nt = (ermax + 15) / 16;
-  if (wave >= nt * nt) return;
-  const int tp = wave % nt, ti = wave / nt;  // tile rows p, columns i
+  __shared__ dbl4 part[3][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int tile = blockIdx.x;
+  // tile = ti (ti + 1) / 2 + tp with tp <= ti
+  int ti = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+  while (ti * (ti + 1) / 2 > tile) --ti;
+  while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+  const int tp = tile - ti * (ti + 1) / 2;  // tile rows p, columns i
   const int p0 = tp * 16, i0 = ti * 16;
   const int r = lane & 15, kq = lane >> 4;
-  // D layout of the f64 16x16x4 MFMA: register v of a lane holds row
-  // (lane >> 4) + 4 v, column lane & 15
-  dbl4 acc;
-  for (int v = 0; v < 4; ++v) {
-    int pr = p0 + kq + 4 * v, ic = i0 + r;
-    acc[v] = (pr < ermax && ic < ermax) ? ecorr[(size_t)ic * ermax + pr] : 0.0;
-  }
+  // the four waves of the workgroup take four consecutive quarters of the
+  // configurations (a quarter of the dependent MFMA chain each); the
+  // quarter sums are added in order at the end.  D layout of the f64
+  // 16x16x4 MFMA: register v of a lane holds row (lane >> 4) + 4 v, column
+  // lane & 15
+  const int nq4 = (((n + 3) / 4 + 3) / 4) * 4;  // configurations per quarter, a multiple of 4
+  const int bq0 = min(n, q * nq4), bq1 = min(n, bq0 + nq4);
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
   // U steps of 4 configurations per round: all 2U operand loads issued
-  // before the U dependent MFMAs (the same MFMA sequence as one step at a
-  // time, so the same sums)
+  // before the U dependent MFMAs
   constexpr int U = 8;
   const bool pin = p0 + r < ermax, iin = i0 + r < ermax;
-  int b0 = 0;
-  for (; b0 + 4 * U <= n; b0 += 4 * U) {
+  int b0 = bq0;
+  for (; b0 + 4 * U <= bq1; b0 += 4 * U) {
     double a[U], bb[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -2303,18 +2308,29 @@ __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__
 #pragma unroll
     for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[u], acc, 0, 0, 0);
   }
-  for (; b0 < n; b0 += 4) {
+  for (; b0 < bq1; b0 += 4) {
     int b = b0 + kq;
     double a = 0.0, bb = 0.0;
-    if (b < n) {
+    if (b < bq1) {
       if (pin) a = hist[(size_t)b * ermax + p0 + r];
       if (iin) bb = w[b] * hist[(size_t)b * ermax + i0 + r];
     }
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
   }
+  if (q > 0) part[q - 1][lane] = acc;
+  __syncthreads();
+  if (q != 0) return;
+  for (int k = 0; k < 3; ++k) {
+    const dbl4 o = part[k][lane];
+    for (int v = 0; v < 4; ++v) acc[v] = acc[v] + o[v];
+  }
   for (int v = 0; v < 4; ++v) {
     int pr = p0 + kq + 4 * v, ic = i0 + r;
-    if (pr < ermax && ic < ermax) ecorr[(size_t)ic * ermax + pr] = acc[v];
+    if (pr < ermax && ic < ermax) {
+      const double e = ecorr[(size_t)ic * ermax + pr] + acc[v];
+      ecorr[(size_t)ic * ermax + pr] = e;
+      if (tp != ti) ecorr[(size_t)pr * ermax + ic] = e;
+    }
   }
 }
 
