@@ -1,24 +1,31 @@
-"""Benchmark: solute-solvent mol-pair energies/s and insertions/s of the
-reference-solvent insertion workload on one MI355X.
+"""Benchmark: solute-solvent mol-pair energies/s and insertions/s of
+test-particle insertion on one MI355X.
 
-Workload (``refs20``): 4000 rigid 3-site waters in a 49.3 A periodic box;
-per frame, 1000 insertions of a rigid 20-site test particle at uniform random
-positions and orientations (device-generated, counter-based); real-space
-cutoff 12 A, Ewald alpha 0.35, PME 48^3 with order-4 B-splines; 600-bin
-energy coordinate; per-insertion histograms accumulated into the 600 x 600
-correlation matrix on FP64 MFMA.  A step is one frame (1000 insertions x
-4000 molecules = 4e6 mol-pair energies).  Frames are synthetic and uploaded
-to HBM before the timed region.
+Headline workload (``mixed5000``, FP32 pair/PME with FP64 sums): a mixed
+solvent of 7000 rigid 3-site waters, 800 rigid 10-site cosolvent molecules,
+100 Na+ and 100 Cl- (8000 molecules, 29200 sites) in a 60 A periodic box;
+per frame 5000 random insertions split evenly over four rigid test particles
+of 5, 20, 60 and 150 sites, each with its own 600-bin energy coordinate
+(four solvent species x 150 bins) and 600 x 600 correlation matrix on FP64
+MFMA; cutoff 12 A, Ewald alpha 0.35, PME 64^3 order 4.  A step is one frame
+(5000 insertions x 8000 molecules = 4e7 mol-pair energies).  Frames are
+synthetic and resident in HBM before the timed region.
 
-Prints one JSON line (rank 0).  ``roofline`` refers to the dominant kernel,
-timed with HIP events on the library's stream over the timed region;
-``cpu_baseline`` is the oracle (the FP64 CPU restatement) on a bounded sample
-of the same workload on this host.
+Also measured (secondary keys): the FP32 error of the headline workload
+against its FP64 run on the same poses (largest per-molecule |dE| and the
+bin-flip fraction), and the ``refs20`` workload (4000 waters in 49.3 A,
+20-site test particle, 1000 insertions per frame, PME 48^3, 600 bins) in
+FP64 (eager and graph replay) and FP32.
+
+Prints one JSON line.  ``roofline`` is the dominant kernel of the headline
+run, timed with HIP events on the library's stream over the timed region;
+``stages`` gives every stage against its roofline; ``cpu_baseline`` is the
+oracle (the FP64 C restatement) on the same frame and test particles over
+the host's cores (OpenMP over insertions), bounded in time.
 """
 from __future__ import annotations
 
 import argparse
-import dataclasses
 import json
 import os
 import sys
@@ -29,304 +36,343 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8000.0          # GB/s, MI355X
-FP64_PEAK = 78.6           # TFLOP/s dense FP64 (vector and MFMA)
-FP32_PEAK = 157.3          # TFLOP/s dense FP32
+HBM_PEAK = 8000.0          # GB/s, MI355X HBM3E
+FP64_MFMA_PEAK = 78.6      # TFLOP/s dense FP64 MFMA
+FP64_PEAK = 78.6           # TFLOP/s dense FP64 vector
+FP32_PEAK = 157.3          # TFLOP/s dense FP32 vector
 
-
-def stage_bytes(sy, prec_bytes, n_ins, slvmax, ermax):
-    """Algorithmic HBM bytes per launch (n_ins configurations) of the
-    register-path stages: Hk = the half-spectrum grid H*K2*K3 of complex T
-    per configuration (written by the z columns, read and rewritten by the y
-    transform, read by the x transform + gather, which also writes and
-    re-reads the per-site sums and writes the per-molecule energies; the
-    solvent's site data is read once per launch)."""
-    K1, K2, K3 = sy.system.ms
-    H = K1 // 2 + 1
-    T = prec_bytes
-    nsites = 3 * slvmax
-    p = sy.system.splodr
-    hk = 2 * T * H * K2 * K3
-    return {
-        "pme_z": n_ins * hk,
-        "pme_y": n_ins * 2 * hk,
-        "pme_x_gather": n_ins * (hk + 16 * nsites + 8 * slvmax) + nsites * (3 * p * T + 8 + T),
-        "bin": n_ins * (8 * slvmax + 8 * ermax),
-    }
-
-
-def measured_traffic(stage, precision, configs):
-    """HBM bytes per launch of the x-slab stage from the committed counter
-    summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, each
-    calibrated on a device copy of known size), scaled to `configs`
-    configurations per launch; None when no summary covers the case."""
-    path = os.path.join(ROOT, "profiles", "refs20_fp64_hbm_traffic_pmc.md")
-    if stage != "pme_x_gather" or precision != "fp64" or not os.path.exists(path):
-        return None
-    kv = {}
-    for line in open(path):
-        if "=" in line and " " not in line.strip():
-            k, v = line.strip().split("=", 1)
-            kv[k] = float(v)
-    keys = ("xslab_read_bytes_per_launch", "xslab_write_bytes_per_launch",
-            "mol_sum_read_bytes_per_launch", "mol_sum_write_bytes_per_launch")
-    if not all(k in kv for k in keys) or not kv.get("configs_per_launch"):
-        return None
-    return sum(kv[k] for k in keys) * configs / kv["configs_per_launch"]
+STAGE_KERNELS = {"prep": "k_pose + k_solute_prep", "pme_z": "k_pme_zcol_fft", "pme_y": "k_pme_ycol_reg",
+                 "pme_x": "k_pme_xslab", "mol_sum": "k_mol_sum", "pair": "k_pair", "self": "k_selfreal + k_u0",
+                 "bin": "k_bin + k_scalars + k_edens", "corr": "k_corr"}
 
 
 def pair_count(sy, pos, P, rc=12.0):
-    """Exact solute-site x solvent-site pairs within the cutoff for the
-    configurations P (numpy, minimum image)."""
+    """Mean number of solute-site x solvent-site pairs within the cutoff over
+    the configurations P (numpy, minimum image)."""
     s = sy.system
     L = sy.L
-    phys = s.physical()
-    idx = np.concatenate([np.arange(s.mol_begin[m], s.mol_begin[m + 1]) for m in phys])
+    idx = np.concatenate([np.arange(s.mol_begin[m], s.mol_begin[m + 1]) for m in s.physical()])
     v = pos[idx]
     tot = 0
     for sp in P:
-        d = sp[:, None, :] - v[None, :, :]
-        d -= L * np.round(d / L)
-        tot += int(np.count_nonzero((d * d).sum(-1) <= rc * rc))
+        for a in range(0, len(sp), 16):
+            d = sp[a:a + 16, None, :] - v[None, :, :]
+            d -= L * np.round(d / L)
+            tot += int(np.count_nonzero((d * d).sum(-1) <= rc * rc))
     return tot / len(P)
 
 
-def cpu_baseline(sy, frame0, n_ins_max, seconds, seed):
-    import oracle as O
+def sample_poses(sy, n, seed):
+    """Test-particle coordinates of insertions 0..n-1 of stream `seed`, placed
+    as the device places them (rotation about the centre of mass at a
+    uniform position); only used to count pairs for the roofline."""
     s = sy.system
-    o = O.Oracle(s)
-    o.set_frame(frame0, sy.cell)
     m = sy.solute_mol
-    t0 = time.perf_counter()
-    done = 0
-    while done < n_ins_max:
-        r, q = O.random_pose(seed, done)
-        sp = O.place_solute(s.bfcoord, s.solute_mass(m), sy.cell, r, q)
-        u = o.uv_energy(m, sp)
-        o.update_histogram(m, u)
-        done += 1
-        if time.perf_counter() - t0 > seconds:
-            break
-    dt = time.perf_counter() - t0
-    slv = o.slvmax
-    return {"value": done * slv / dt, "unit": "mol-pair energies/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (FP64 C restatement, 1 thread) on frame 0 of the same workload: "
-                      f"{done} insertions with update_histogram in {dt:.1f} s",
-            "insertions_per_s": done / dt}
+    mass = s.solute_mass(m)
+    bf = s.bfcoord - (mass[:, None] * s.bfcoord).sum(0) / mass.sum()
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        out.append(bf @ R.T + rng.random(3) * sy.L)
+    return out
 
 
-def mixed_pass(torch, dev, frames=4, per_frame=5000, prec="fp32"):
-    """The heterogeneous insertion workload: 7000 waters + 800 10-site
-    cosolvent molecules + 100 Na+ + 100 Cl- in a 60 A box (K = 64), per frame
-    5000 insertions split over four rigid solutes of 5, 20, 60 and 150 sites,
-    one context each (600 bins apiece), run concurrently on four streams
-    from one device copy of the frame; FP32.  Each context is autotuned on
-    the first frame (untimed)."""
+def stage_work(sy, T, slvmax, ermax, npair):
+    """Algorithmic work per configuration of each stage and per launch (the
+    solvent's site data a launch reads once), FP32 (T = 4) or FP64 (T = 8):
+    bytes for HBM-bound stages, flops for the pair kernel (40 per pair
+    within the cutoff) and the correlation matrix (2 ermax^2, FP64 MFMA)."""
+    s = sy.system
+    K1, K2, K3 = s.ms
+    H = K1 // 2 + 1
+    p = s.splodr
+    ns = s.nsite(sy.solute_mol)
+    nv = int(sum(s.nsite(m) for m in s.physical()))
+    hk = 2 * T * H * K2 * K3                          # half spectrum of one configuration
+    return {
+        # spread along z + z FFT + Green's function: the spectrum it writes,
+        # the per-site x / y spectra and z weights it reads
+        "pme_z": ("hbm", hk + ns * (2 * T * (H + K2) + (T + 4) * p + T), 0),
+        "pme_y": ("hbm", 2 * hk, 0),                   # inverse y transform in place
+        # x transform + gather: the potential grid (T K^3) and the per-site
+        # sums it writes; solvent site data (base, charge, 3p weights, index)
+        # once per launch
+        "pme_x": ("hbm", T * K1 * K2 * K3 + 8 * nv, nv * (8 + T * (3 * p + 1))),
+        "mol_sum": ("hbm", 8 * nv + 8 * (slvmax + 1), 16 * slvmax),
+        "pair": ("valu", 40.0 * npair, 0),
+        "bin": ("hbm", 8 * (slvmax + 1) + 8 * ermax, 4 * slvmax),
+        "corr": ("mfma", 2.0 * ermax * ermax, 0),
+    }
+
+
+def roofline_of(stage, bound, work_cfg, work_launch, ms_total, n_cfg, n_launch, T):
+    peak, unit = {"hbm": (HBM_PEAK, "GB/s"), "valu": (FP32_PEAK if T == 4 else FP64_PEAK, "TFLOP/s"),
+                  "mfma": (FP64_MFMA_PEAK, "TFLOP/s")}[bound]
+    total = work_cfg * n_cfg + work_launch * n_launch
+    per_launch = total / max(n_launch, 1)
+    avg_ms = ms_total / max(n_launch, 1)
+    ach = per_launch / (avg_ms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
+    return {"kernel": STAGE_KERNELS.get(stage, stage), "stage": stage, "bound": bound, "achieved": ach,
+            "peak": peak, "unit": unit, "frac": ach / peak, "avg_launch_ms": avg_ms,
+            ("bytes_per_launch" if unit == "GB/s" else "flops_per_launch"): per_launch,
+            "configs_per_launch": n_cfg / max(n_launch, 1), "launches": n_launch}
+
+
+def measured_traffic(workload, precision, kernel, configs):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes, each scaled by the
+    same counter on a device copy of known size), at `configs`
+    configurations per launch; None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_hbm_traffic_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.strip().split()
+        if len(f) >= 2 and f[0] == "traffic" and f[1] == kernel:
+            kv = dict(x.split("=", 1) for x in f[2:] if "=" in x)
+            try:
+                per_cfg = (float(kv["read_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])) / float(
+                    kv["configs_per_launch"])
+            except (KeyError, ValueError, ZeroDivisionError):
+                return None
+            return per_cfg * configs
+    return None
+
+
+# ------------------------------------------------------------ workloads
+def mixed_setup(torch, dev, nframes):
     from ermod_amd import synth
-    from ermod_amd.api import ErmGPU, insert_random_multi
-    sizes = synth.MIXED_SOLUTE_SITES
-    syss = [synth.mixed(ns) for ns in sizes]
-    big = max(syss, key=lambda sy: sy.system.natom)
-    host = [big.frame(i) for i in range(frames + 1)]
+    syss = [synth.mixed(ns) for ns in synth.MIXED_SOLUTE_SITES]
+    big = syss[-1]
+    host = [big.frame(i) for i in range(nframes)]
     dpos = [torch.from_numpy(f.reshape(-1)).to(dev) for f in host]
-    share = [per_frame // len(sizes)] * len(sizes)
+    torch.cuda.synchronize()
+    return syss, host, dpos
+
+
+def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True):
+    """Timed frames of the mixed-solvent workload: per frame, each solute's
+    context takes the device frame and its share of the insertions; the four
+    contexts run concurrently (one stream each).  Returns the timing, the
+    summed stage times and per-stage launch / configuration counts."""
+    from ermod_amd.api import ErmGPU, insert_random_multi
+    nfu = len(dpos)
+    share = [per_frame // len(syss)] * len(syss)
     engines = [ErmGPU(sy.system, prec) for sy in syss]
     try:
         for e, sy in zip(engines, syss):
             e.set_frame_device(dpos[0].data_ptr(), sy.cell)
-            e.autotune(n_trial=share[0], seed=99)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for f in range(1, frames + 1):
+            if tune:
+                e.autotune(n_trial=share[0], seed=99)
+        for e in engines:
+            e.set_tuning(max_batch=0, pme_path=e.tuning()["pme_path"], use_graph=False)
+        seeds = [11, 12, 13, 14]
+        for i in range(warmup):
             for e, sy in zip(engines, syss):
-                e.set_frame_device(dpos[f].data_ptr(), sy.cell)
-            insert_random_multi(engines, share, [11, 12, 13, 14], [f * s for s in share])
+                e.set_frame_device(dpos[i % nfu].data_ptr(), sy.cell)
+            insert_random_multi(engines, share, seeds, [i * s for s in share])
+        for e in engines:
+            e.engclear()
+        torch.cuda.synchronize()
+        st = [dict() for _ in engines]
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            for e, sy in zip(engines, syss):
+                e.set_frame_device(dpos[i % nfu].data_ptr(), sy.cell)
+            insert_random_multi(engines, share, seeds, [i * s for s in share])
+            for k, e in enumerate(engines):
+                for key, v in e.stage_times().items():
+                    st[k][key] = st[k].get(key, 0.0) + v
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        nmol = syss[0].system.nmol - 1
-        nins = frames * sum(share)
-        return {"workload": "7000 waters + 800 cosolvent + 100 Na+ + 100 Cl- in 60 A, K 64; solutes of "
-                            "5/20/60/150 sites, 1250 insertions each per frame, concurrent contexts",
-                "precision": prec, "frames": frames, "mol_pairs_per_s": nins * nmol / dt,
-                "insertions_per_s": nins / dt, "ms_per_frame": dt / frames * 1e3}
+        tun = [e.tuning() for e in engines]
+        acc = [e.fetch() for e in engines]
+        slvmax, ermax = engines[0].slvmax, engines[0].ermax
     finally:
         for e in engines:
             e.close()
+    return dict(dt=dt, stage=st, tuning=tun, acc=acc, share=share, slvmax=slvmax, ermax=ermax)
+
+
+def mixed_fp32_error(syss, dpos, per_frame):
+    """FP32 against FP64 on one frame's insertions (every solute, the same
+    poses): the largest per-molecule |dE| and the bin-flip fraction over the
+    four species' edges."""
+    from ermod_amd.api import ErmGPU, build_edges
+    share = per_frame // len(syss)
+    worst, flips, n, big = 0.0, 0, 0, 0.0
+    per = {}
+    for k, sy in enumerate(syss):
+        s = sy.system
+        out = {}
+        for prec in ("fp64", "fp32"):
+            with ErmGPU(s, prec) as g:
+                g.set_frame_device(dpos[-1].data_ptr(), sy.cell)
+                out[prec] = g.insert_random(share, 31 + k, first_index=0, want_uv=True)[:, 1:]
+        spec = s.uvspec[s.physical()]
+        err = np.abs(out["fp32"] - out["fp64"])
+        f = 0
+        for t, h in enumerate(s.hists, start=1):
+            edges = build_edges(h)[0]
+            sel = spec == t
+            f += int(np.count_nonzero(np.searchsorted(edges, out["fp32"][:, sel], side="left")
+                                      != np.searchsorted(edges, out["fp64"][:, sel], side="left")))
+        per[str(s.nsite(sy.solute_mol))] = {"max_abs_err": float(err.max()), "bin_flips": f}
+        worst = max(worst, float(err.max()))
+        mag = np.abs(out["fp64"])
+        if (mag > 100).any():
+            big = max(big, float(err[mag > 100].max()))
+        flips += f
+        n += err.size
+    return {"max_abs_err_kcal_mol": worst, "max_abs_err_above_100": big, "bin_flips": flips, "entries": n,
+            "bin_flip_fraction": flips / n, "insertions": share * len(syss), "per_solute_sites": per,
+            "bar_kcal_mol": 1e-4}
+
+
+def cpu_baseline_mixed(syss, host_frame, seconds):
+    """The oracle on the same frame and the four test particles (insertion i
+    of solute i mod 4), OpenMP over insertions on the host's cores."""
+    import oracle as O
+    mols = [sy.solute_mol for sy in syss]
+    done, dt, nt = O.bench_insertions([sy.system for sy in syss], mols, host_frame, syss[0].cell,
+                                      10 ** 7, seconds, seed=7)
+    nins = sum(done)
+    slv = syss[0].system.physical().size
+    return {"value": nins * slv / dt, "unit": "mol-pair energies/s", "cores": nt, "kind": "port",
+            "sample": f"oracle (FP64 C restatement), {nt} OpenMP threads over insertions, one context per "
+                      f"thread and solute; frame 0 of the mixed-solvent workload, {nins} insertions "
+                      f"({'/'.join(str(d) for d in done)} of the 5/20/60/150-site solutes) in {dt:.1f} s",
+            "insertions_per_s": nins / dt, "nproc": os.cpu_count()}
+
+
+def refs20_pass(torch, dev, prec, steps, warmup, insertions, graph=False, tune=True):
+    from ermod_amd import synth
+    from ermod_amd.api import ErmGPU
+    sy = synth.refs20()
+    nfu = min(steps + warmup, 8)
+    frames = [torch.from_numpy(sy.frame(i).reshape(-1)).to(dev) for i in range(nfu)]
+    g = ErmGPU(sy.system, prec)
+    try:
+        g.set_frame_device(frames[0].data_ptr(), sy.cell)
+        if tune:
+            g.autotune(n_trial=insertions, seed=20261204)
+        g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)
+        for i in range(warmup):
+            g.set_frame_device(frames[i % nfu].data_ptr(), sy.cell)
+            g.insert_random(insertions, 20261203, first_index=i * insertions)
+        g.engclear()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            g.set_frame_device(frames[i % nfu].data_ptr(), sy.cell)
+            g.insert_random(insertions, 20261203, first_index=i * insertions)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        tun = g.tuning()
+        slv = g.slvmax
+    finally:
+        g.close()
+    n = steps * insertions
+    return {"value": n * slv / dt, "insertions_per_s": n / dt, "ms_per_step": dt / steps * 1e3,
+            "batch": tun["max_batch"], "pme_path": tun["pme_path"], "use_graph": tun["use_graph"]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20, help="timed frames")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--insertions", type=int, default=1000)
-    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
+    ap.add_argument("--insertions", type=int, default=5000, help="insertions per frame (split over 4 solutes)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default batch and PME path)")
-    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay pass")
-    ap.add_argument("--slab-planes", type=int, default=0, help="fixed slab depth (with --batch)")
-    ap.add_argument("--group-planes", type=int, default=0, help="fixed planes per x group (with --batch)")
-    ap.add_argument("--slab-threads", type=int, default=0, help="fixed x-slab workgroup size (with --batch)")
-    ap.add_argument("--column-threads", type=int, default=0, help="fixed z-column workgroup size (with --batch)")
-    ap.add_argument("--no-variants", action="store_true", help="skip the PPPM and bare-Coulomb passes")
-    ap.add_argument("--fold-y", type=int, default=-1, choices=[-1, 0, 1],
-                    help="inverse y transform inside the x slab (with --batch; -1 keeps the default)")
-    ap.add_argument("--no-mixed", action="store_true",
-                    help="skip the heterogeneous pass (four solute sizes in a mixed solvent, concurrently)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
-                    help="after the run, one device copy of MB megabytes (HBM counter calibration)")
+    ap.add_argument("--no-error", action="store_true", help="skip the FP32-vs-FP64 error pass")
+    ap.add_argument("--no-refs20", action="store_true", help="skip the refs20 secondary passes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs (frame sharding with "
-                                   "an all-reduce of the accumulators) are not part of it",
+        print(json.dumps({"error": "this build measures one GPU; runs on several GPUs (frames sharded over "
+                                   "ranks, one all-reduce of the accumulators per division) are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
-    from ermod_amd import synth
-    from ermod_amd.api import ErmGPU
-
-    sy = synth.refs20()
-    s = sy.system
-    nf = args.warmup + args.steps
-    nfu = min(nf, 8)   # distinct synthetic frames, cycled
-    t_gen = time.perf_counter()
-    frames = [sy.frame(i) for i in range(nfu)]
-    gen_s = time.perf_counter() - t_gen
     dev = torch.device("cuda:0")
-    dframes = [torch.from_numpy(f.reshape(-1)).to(dev) for f in frames]
-    torch.cuda.synchronize()
+    nfu = min(args.steps + args.warmup, 6) + 1   # distinct frames (cycled) + one for the error pass
+    syss, host, dpos = mixed_setup(torch, dev, nfu)
+    r = run_mixed(torch, syss, dpos[:-1], args.precision, args.steps, args.warmup, args.insertions,
+                  tune=not args.no_tune)
+    slvmax, ermax = r["slvmax"], r["ermax"]
+    n_ins = args.steps * sum(r["share"])
+    value = n_ins * slvmax / r["dt"]
+    T = 4 if args.precision == "fp32" else 8
 
-    def run(prec, steps, warmup, graph=False, system=None):
-        g = ErmGPU(system or s, prec, max_batch=args.batch)
-        tuning = None
-        g.set_frame_device(dframes[0].data_ptr(), sy.cell)
-        if not args.no_tune and args.batch == 0:
-            tuning, _ = g.autotune(n_trial=args.insertions, seed=args.seed + 1)
-        else:
-            g.set_tuning(max_batch=args.batch, slab_planes=args.slab_planes, group_planes=args.group_planes,
-                         fold_y=None if args.fold_y < 0 else bool(args.fold_y),
-                         slab_threads=args.slab_threads, column_threads=args.column_threads)
-        g.set_tuning(max_batch=0, pme_path=g.tuning()["pme_path"], use_graph=graph)  # keeps batch and slab shape
-        for i in range(warmup):
-            g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
-            g.insert_random(args.insertions, args.seed, first_index=i * args.insertions)
-        g.engclear()
-        torch.cuda.synchronize()
-        st = {k: 0.0 for k in ("prep", "pme_z", "pme_y", "pme_x_gather", "pair", "bin", "corr", "total")}
-        t0 = time.perf_counter()
-        for i in range(warmup, warmup + steps):
-            g.set_frame_device(dframes[i % nfu].data_ptr(), sy.cell)
-            g.insert_random(args.insertions, args.seed, first_index=i * args.insertions)
-            for k, v in g.stage_times().items():
-                st[k] += v
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        acc = g.fetch()
-        tun = g.tuning()
-        slv, erm = g.slvmax, g.ermax
-        g.close()
-        return dt, st, acc, tun, slv, erm
-
-    dt, st, acc, tun, slvmax, ermax = run(args.precision, args.steps, args.warmup)
-    bmax = tun["max_batch"]
-    n_ins = args.steps * args.insertions
-    value = n_ins * slvmax / dt
-    launches = args.steps * int(np.ceil(args.insertions / bmax))
-    T = 8 if args.precision == "fp64" else 4
-    sb = stage_bytes(sy, T, bmax, slvmax, ermax)
-    # dominant stage over the timed region
-    stages = {k: v for k, v in st.items() if k not in ("total",)}
-    dom = max(stages, key=stages.get)
-    avg_ms = stages[dom] / launches
-    kname = {"pme_z": "k_pme_zcol_reg", "pme_y": "k_pme_ycol_reg", "pme_x_gather": "k_pme_xslab + k_mol_sum",
-             "bin": "k_bin", "prep": "k_pose + k_solute_prep", "corr": "k_corr"}.get(dom, dom)
-    if tun["pme_path"] != "register" and dom.startswith("pme"):
-        kname = dom + " (" + tun["pme_path"] + " path)"
-    if dom in sb:
-        ach = sb[dom] / (avg_ms * 1e-3) / 1e9
-        roof = {"kernel": kname, "stage": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": measured_traffic(dom, args.precision, bmax),
-                "avg_launch_ms": avg_ms,
-                "bytes_per_launch": sb[dom], "configs_per_launch": bmax}
-    else:
-        roof = {"kernel": dom, "avg_launch_ms": avg_ms}
-    if dom == "pair":
-        import oracle as O
-        Ps = []
-        for i in range(4):
-            r, q = O.random_pose(args.seed, i)
-            Ps.append(O.place_solute(s.bfcoord, s.solute_mass(sy.solute_mol), sy.cell, r, q))
-        npair = pair_count(sy, frames[0], Ps, 12.0)
-        flops = 40.0 * npair * bmax
-        peak = FP64_PEAK if args.precision == "fp64" else FP32_PEAK
-        ach = flops / (avg_ms * 1e-3) / 1e12
-        roof = {"kernel": "pair", "bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None, "avg_launch_ms": avg_ms,
-                "flops_per_launch": flops, "pairs_per_insertion": npair}
-    stage_share = {k: round(v / max(st["total"], 1e-9), 4) for k, v in stages.items()}
+    # stage rooflines over the timed region, summed over the four contexts
+    stage_rows = {}
+    tot = {}
+    for k, sy in enumerate(syss):
+        pos0 = host[0]
+        npair = pair_count(sy, pos0, sample_poses(sy, 3, 100 + k))
+        work = stage_work(sy, T, slvmax, ermax, npair)
+        B = r["tuning"][k]["max_batch"]
+        n_cfg = args.steps * r["share"][k]
+        n_launch = args.steps * int(np.ceil(r["share"][k] / B))
+        for stg, (bound, wc, wl) in work.items():
+            ms = r["stage"][k].get(stg, 0.0)
+            a = tot.setdefault(stg, dict(bound=bound, work=0.0, ms=0.0, cfg=0, launch=0))
+            a["work"] += wc * n_cfg + wl * n_launch
+            a["ms"] += ms
+            a["cfg"] += n_cfg
+            a["launch"] += n_launch
+    for stg, a in tot.items():
+        if a["ms"] <= 0:
+            continue
+        stage_rows[stg] = roofline_of(stg, a["bound"], a["work"] / a["cfg"], 0.0, a["ms"], a["cfg"], a["launch"], T)
+        stage_rows[stg]["ms_total"] = round(a["ms"], 3)
+    dom = max(stage_rows, key=lambda k: stage_rows[k]["ms_total"])
+    roof = dict(stage_rows[dom])
+    roof["traffic"] = (measured_traffic("mixed5000", args.precision, roof["kernel"], roof["configs_per_launch"])
+                       if roof["unit"] == "GB/s" else None)
+    stage_ms_total = {k: round(sum(st.get(k, 0.0) for st in r["stage"]), 3) for k in r["stage"][0]}
 
     out = {"metric": "solute-solvent mol-pair energies/s", "value": value, "unit": "mol-pair energies/s",
-           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["dt"] / args.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64" if args.precision == "fp64" else "f32 pair/PME, f64 sums",
+           "dtype": "f32 pair/PME, f64 sums" if args.precision == "fp32" else "f64",
            "data": "synthetic",
-           "config": {"workload": "refs20: 4000 waters in 49.3 A, rigid 20-site test particle, "
-                                  f"{args.insertions} random insertions per frame, cutoff 12 A, "
-                                  "alpha 0.35, PME 48^3 p4, 600 bins, correlation on FP64 MFMA",
-                      "frames": args.steps, "insertions_per_frame": args.insertions,
-                      "batch": bmax, "pme_path": tun["pme_path"], "slab_planes": tun["slab_planes"],
-                      "group_planes": tun["group_planes"], "slab_threads": tun["slab_threads"],
-                      "column_threads": tun["column_threads"], "fold_y": tun["fold_y"],
+           "config": {"workload": "mixed5000: 7000 waters + 800 10-site cosolvent + 100 Na+ + 100 Cl- "
+                                  "(8000 molecules) in 60 A; per frame 5000 random insertions split over "
+                                  "rigid 5/20/60/150-site test particles, 600 bins and a 600x600 correlation "
+                                  "matrix (FP64 MFMA) per solute; cutoff 12 A, alpha 0.35, PME 64^3 p4",
+                      "frames": args.steps, "insertions_per_frame": sum(r["share"]),
                       "precision": args.precision,
-                      "tuning": "autotuned" if not args.no_tune and args.batch == 0 else "fixed"},
-           "insertions_per_s": n_ins / dt,
-           "roofline": roof, "stage_ms": {k: round(v, 3) for k, v in st.items()},
-           "stage_share": stage_share, "engsmpl": acc["engsmpl"]}
-    if not args.no_graph:
-        dtg, _, accg, tung, _, _ = run(args.precision, args.steps, args.warmup, graph=True)
-        out["graph_replay"] = {"value": n_ins * slvmax / dtg, "insertions_per_s": n_ins / dtg,
-                               "ms_per_step": dtg / args.steps * 1e3, "use_graph": tung["use_graph"],
-                               "edens_equal": bool(np.array_equal(accg["edens"], acc["edens"]))}
-    if not args.no_fp32 and args.precision == "fp64":
-        dt32, st32, _, tun32, _, _ = run("fp32", max(1, args.steps // 2), 1)
-        n32 = max(1, args.steps // 2) * args.insertions
-        out["fp32"] = {"value": n32 * slvmax / dt32, "insertions_per_s": n32 / dt32,
-                       "batch": tun32["max_batch"], "pme_path": tun32["pme_path"],
-                       "stage_ms": {k: round(v, 3) for k, v in st32.items()}}
-    if not args.no_mixed and args.precision == "fp64":
-        out["heterogeneous"] = mixed_pass(torch, dev)
-    if not args.no_variants and args.precision == "fp64":
-        # the same workload with PPPM and with bare Coulomb (site cutoff at
-        # elecut; its cut molecules reach lower energies, hence the wider bins)
-        from ermod_amd.system import EL_COULOMB, EL_PPPM
-        var = {}
-        nst = max(1, args.steps // 4)
-        for name, cl in (("pppm", EL_PPPM), ("bare_coulomb", EL_COULOMB)):
-            sv = dataclasses.replace(s, cltype=cl)
-            if cl == EL_COULOMB:
-                sv = dataclasses.replace(sv, hists=[dataclasses.replace(h, ecdmin=-60.0) for h in s.hists])
-            dtv, stv, _, tunv, _, _ = run(args.precision, nst, 1, system=sv)
-            nv = nst * args.insertions
-            var[name] = {"value": nv * slvmax / dtv, "insertions_per_s": nv / dtv, "batch": tunv["max_batch"],
-                         "pme_path": tunv["pme_path"] if cl != EL_COULOMB else None,
-                         "stage_ms": {k: round(v, 3) for k, v in stv.items()}}
-        out["electrostatics"] = var
+                      "tuning": [{k: t[k] for k in ("max_batch", "pme_path", "slab_planes", "group_planes",
+                                                    "slab_threads", "column_threads", "fold_y")}
+                                 for t in r["tuning"]],
+                      "schedule": "four contexts, one stream each"},
+           "insertions_per_s": n_ins / r["dt"],
+           "roofline": roof, "stages": stage_rows, "stage_ms": stage_ms_total,
+           "engsmpl": [a["engsmpl"] for a in r["acc"]]}
+    if not args.no_error:
+        out["fp32_error"] = mixed_fp32_error(syss, dpos, args.insertions)
+    if not args.no_refs20:
+        st = max(2, args.steps // 2)
+        out["refs20"] = {"workload": "refs20: 4000 waters in 49.3 A, rigid 20-site test particle, 1000 random "
+                                     "insertions per frame, PME 48^3 p4, 600 bins",
+                         "fp64": refs20_pass(torch, dev, "fp64", st, 2, 1000, tune=not args.no_tune),
+                         "fp64_graph": refs20_pass(torch, dev, "fp64", st, 2, 1000, graph=True,
+                                                   tune=not args.no_tune),
+                         "fp32": refs20_pass(torch, dev, "fp32", st, 2, 1000, tune=not args.no_tune)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sy, frames[0], 100000, args.cpu_seconds, args.seed)
-        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
-    if args.copy_calibration > 0:
-        n = args.copy_calibration * (1 << 20) // 8
-        a = torch.ones(n, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = 2 * n * 8
+        cb = cpu_baseline_mixed(syss, host[0], args.cpu_seconds)
+        cb["gpu_over_cpu"] = value / cb["value"]
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

@@ -24,7 +24,7 @@ STATUS = {0: "ok", 1: "eng_min", 2: "eng_sft", 3: "mbin", 4: "eng_ecd", 5: "rcp_
           7: "eng_bug", 8: "eng_cns", 9: "eng_siz", 10: "arg", 11: "unsupported", 12: "device", 13: "eof",
           14: "ins_limit"}
 ERM_E_EOF = 13
-STAGES = ("prep", "pme_z", "pme_y", "pme_x_gather", "pair", "bin", "corr", "total")
+STAGES = ("prep", "pme_z", "pme_y", "pme_x", "mol_sum", "pair", "self", "bin", "corr", "total")
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -513,6 +513,6 @@ class ErmGPU:
         return self.tuning(), float(ms[0])
 
     def stage_times(self):
-        out = np.zeros(8)
+        out = np.zeros(len(STAGES))
         lib().erm_stage_times(self.h, _p(out))
         return dict(zip(STAGES, out))

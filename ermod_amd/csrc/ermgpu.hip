@@ -13,24 +13,17 @@
 #include <thread>
 #include <vector>
 
+#include "ermgpu_ctx.h"
 #include "ermgpu_grad.h"
 #include "ermgpu_traj.h"
 
 using namespace erm;
+using namespace ermh;
 
 namespace {
 size_t zcol_lds(int ncol, int K3, size_t vb);
 const double PI_REF = 3.1415926535897932;
 
-struct DBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-};
 
 // ---------------------------------------------------------------- host math
 // spline.F90:26-59 (coefficients) -- used for the B-spline moduli
@@ -380,208 +373,8 @@ size_t zcol_lds(int ncol, int K3, size_t vb) {
 }
 }  // namespace
 
-// ============================================================== the context
-struct erm_ctx {
-  erm_params P{};
-  std::string err;
-  hipStream_t st = nullptr;
-  hipEvent_t ev[8] = {};
-  double stage_ms[8] = {};
-  bool fp32 = false;
-  int Bmax = 256;
-  // topology
-  bool have_topo = false, have_bins = false, have_frame = false;
-  int natom = 0, nmol = 0, ntype = 0;
-  std::vector<int> mol_begin, ljtype, sluvid, uvspec;
-  std::vector<double> charge, lj_eps, lj_sig2, mol_charge, sitemass, bfcoord;
-  int slvmax = 0;
-  std::vector<int> tagpt, kof, sltlist;
-  int slt_template = -1, ns = 0;
-  int nvt = 0;
-  std::vector<int> vtypes;  // compact index -> ljtype
-  double total_charge = 0.0;
-  // bins
-  int numslv = 0, ermax = 0, esmax = 0;
-  std::vector<int> uvmax, uvsoft, specoff;
-  std::vector<double> uvcrd, escrd;
-  // frame
-  double cell[9] = {}, invcl[9] = {}, volume = 0.0, L[3] = {};
-  bool have_prevcl = false;
-  double prevcl[9] = {};
-  double stat_weight_system = 1.0;
-  std::vector<int> jof_k;  // sorted index of tagpt index k
-  bool jof_stale = true;   // jof_k not yet read back for this frame
-  Params kp{};
-  // refs rigid: realcal_self of the first configuration is reused
-  bool have_usreal = false;
-  // device buffers
-  DBuf d_mol_begin, d_charge, d_tagpt, d_cvt, d_mol_charge, d_uvspec, d_sitemass, d_bf, d_qslt, d_qslt_t, d_sltlj,
-      d_sltlj64;
-  DBuf d_pos, d_cellid, d_count, d_cell_start, d_fill, d_rm, d_mol_k, d_mol_ns, d_mol_first, d_mol_q,
-      d_mol_spec, d_mol_m, d_jof;
-  DBuf d_sx, d_sy, d_sz, d_sq, d_sct, d_sg, d_sw;
-  DBuf d_zcount, d_zstart, d_zfill, d_zfine, d_zsite, d_zg, d_zq, d_zw, d_esite;
-  int nsites = 0;             // sites of the physical molecules
-  int slab_S = 0, slab_R = 0; // fused PME slab: planes per workgroup, registers per thread
-  DBuf d_tw1, d_tw2, d_tw3, d_engfac;
-  DBuf d_edges, d_specoff, d_uvmax, d_uvsoft, d_escrd;
-  DBuf d_w3h;
-  // solute gradient: per-frame solvent potential grid and batch buffers
-  DBuf d_ggrid, d_gHk, d_gtab, d_gq_t, d_gqd, d_gepsd, d_gpart, d_gphi, d_gub, d_gw, d_gout;
-  uint64_t frame_serial = 0, ggrid_serial = ~0ULL;
-  // batching: device-side call arguments, per-batch stage events, graph replay
-  DBuf d_args;
-  uint64_t *h_args = nullptr;      // pinned {seed, first_index}
-  bool args_ready = false;         // d_args holds the current call's arguments
-  std::vector<hipEvent_t> bev;     // 8 events per batch of the last call
-  hipEvent_t *cur_ev = nullptr;    // events of the batch being enqueued
-  int wsol_ones = 0;               // d_wsol holds 1.0 for this many configurations
-  bool capturing = false, use_graph = false;
-  hipGraphExec_t gexec = nullptr;
-  std::string gkey;
-  uint64_t alloc_gen = 0;          // bumped by every device (re)allocation
-  int ggrid_excl = -2;
-  int xslab_Q = 0;            // planes per x-c2r group of the register path
-  int tune_S = 16, tune_Q = 8;  // caps on slab depth and planes per group (autotuned)
-  int xslab_T = NTHR;           // threads per x-slab workgroup, 256 or 512 (autotuned)
-  int zcol_T = NTHR;            // threads per z-column workgroup, 256 or 512 (autotuned)
-  bool xslab_yf = false;        // inverse y transform inside the x slab (autotuned)
-  std::vector<int> zst;         // first z-sorted site of every plane (host copy)
-  size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
-  // insertion modes: host molecules of the aggregate centre, and the centre
-  std::vector<int> hostmol;
-  DBuf d_hostmol, d_aggcen;
-  // trajectory frames: FP32 staging and the permutation
-  DBuf d_tstage, d_tperm;
-  // intermediate state: empty reference bins, accumulators, weights
-  DBuf d_zerodns, d_edintm, d_uvintm, d_intm_sc, d_wi;
-  // reference structure (RefInfo, insertion.F90:623-723): host atoms and
-  // their reference coordinates and weights, the solute's reference
-  // coordinates and weights; per frame the fitted and best-fit solute
-  DBuf d_flexbf;  // body configurations of a flexible test particle
-  bool have_ref = false, gauss_init = false;
-  std::vector<int> ref_atoms;
-  std::vector<double> ref_host, ref_hw, ref_slt, ref_sw;
-  DBuf d_refatoms, d_refgather, d_refsite, d_refw;
-  DBuf d_spos, d_sref, d_sg3, d_sw3, d_sqv, d_w1h, d_w2h, d_Hk, d_phi, d_selfp, d_uv, d_uvout, d_hist,
-      d_wsol, d_w, d_flceng, d_sltj, d_selfreal, d_usreal;
-  DBuf d_edens, d_ecorr, d_eself, d_slnuv, d_mnmx, d_scal, d_status, d_cellm;
-  std::vector<DBuf *> all() {
-    return {&d_mol_begin, &d_charge, &d_tagpt, &d_cvt, &d_mol_charge, &d_uvspec, &d_sitemass, &d_bf, &d_qslt, &d_qslt_t,
-            &d_sltlj, &d_sltlj64, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
-            &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
-            &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zfine, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
-            &d_w3h, &d_args, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
-            &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
-            &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
-            &d_eself, &d_slnuv, &d_mnmx, &d_scal, &d_status, &d_cellm, &d_hostmol, &d_aggcen, &d_tstage,
-            &d_tperm, &d_zerodns, &d_edintm, &d_uvintm, &d_intm_sc, &d_wi, &d_refatoms, &d_refgather,
-            &d_refsite, &d_refw, &d_flexbf};
-  }
-};
 
 namespace {
-
-int fail(erm_ctx *c, int code, const char *fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  if (c) c->err = buf;
-  return code;
-}
-
-#define HIPCHK(c, call)                                                                        \
-  do {                                                                                         \
-    hipError_t e_ = (call);                                                                    \
-    if (e_ != hipSuccess) return fail((c), ERM_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
-
-int ensure(erm_ctx *c, DBuf &b, size_t bytes) {
-  if (bytes == 0) bytes = 16;
-  if (b.bytes >= bytes) return ERM_OK;
-  b.release();
-  hipError_t e = hipMalloc(&b.p, bytes);
-  if (e != hipSuccess) {
-    b.p = nullptr;
-    return fail(c, ERM_E_DEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-  }
-  b.bytes = bytes;
-  ++c->alloc_gen;
-  return ERM_OK;
-}
-
-template <typename X> int upload(erm_ctx *c, DBuf &b, const X *src, size_t n) {
-  int s = ensure(c, b, n * sizeof(X));
-  if (s) return s;
-  if (n) HIPCHK(c, hipMemcpyAsync(b.p, src, n * sizeof(X), hipMemcpyHostToDevice, c->st));
-  return ERM_OK;
-}
-
-template <typename X> X *dp(DBuf &b) { return reinterpret_cast<X *>(b.p); }
-
-int sync(erm_ctx *c) {
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  return ERM_OK;
-}
-
-int launch_check(erm_ctx *c, const char *what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, ERM_E_DEVICE, "launch %s: %s", what, hipGetErrorString(e));
-  return ERM_OK;
-}
-
-// raises the dynamic LDS limit of kernel f to `bytes` when it exceeds the
-// default 64 KB; a refused request is cleared from the runtime's error state
-int set_lds(erm_ctx *c, const void *f, size_t bytes, const char *what) {
-  if (bytes <= 64 * 1024) return ERM_OK;
-  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(c, ERM_E_UNSUPPORTED, "%s: %zu bytes of LDS refused (%s)", what, bytes, hipGetErrorString(e));
-  }
-  return ERM_OK;
-}
-
-#define TRY(x)              \
-  do {                      \
-    int s_ = (x);           \
-    if (s_) return s_;      \
-  } while (0)
-
-const char *status_name(int code) {
-  switch (code) {
-    case ERM_E_ENG_MIN: return "eng_min: energy at or below the lowest bin edge";
-    case ERM_E_ENG_SFT: return "eng_sft: energy above the soft region";
-    case ERM_E_MBIN: return "mbin: energy in the overflow bin below 1e3";
-    case ERM_E_ENG_CNS: return "eng_cns: solvent molecule without species";
-    case ERM_E_INS_LIMIT: return "no reference-structure insertion accepted in 4096 attempts";
-    default: return "error";
-  }
-}
-
-// first error raised by the device during the call (configuration index
-// within the call); synchronises the stream
-int check_status(erm_ctx *c) {
-  int h[8];
-  HIPCHK(c, hipMemcpyAsync(h, c->d_status.p, sizeof h, hipMemcpyDeviceToHost, c->st));
-  TRY(sync(c));
-  if (h[0]) {
-    double v;
-    memcpy(&v, &h[4], sizeof v);
-    return fail(c, h[0], "%s (configuration %d, sorted molecule %d, energy %.6g)", status_name(h[0]), h[1],
-                h[2], v);
-  }
-  return ERM_OK;
-}
-
-// stage event i of the batch being enqueued (none while a graph is captured)
-int mark(erm_ctx *c, int i) {
-  if (c->capturing || !c->cur_ev) return ERM_OK;
-  HIPCHK(c, hipEventRecord(c->cur_ev[i], c->st));
-  return ERM_OK;
-}
 
 template <typename T> struct TP {
   using V = typename C2<T>::type;
@@ -683,84 +476,6 @@ int zcol_groups(const erm_ctx *c, bool regp, int vb) {
   }
   const int zt = (c->zcol_T == 512) ? 512 : NTHR;
   return ((zcol_fft(c, regp) ? 2 : 1) * ncol + zt - 1) / zt;
-}
-
-template <typename T, int K, int XT>
-int launch_zcol_reg_t(erm_ctx *c, const Params &k, int n, int nblk) {
-  using V = typename C2<T>::type;
-  const size_t lds = zcol_reg_lds<T>(k.H, k.K2, K, XT);
-  TRY(set_lds(c, (const void *)k_pme_zcol_reg<T, K, XT>, lds, "k_pme_zcol_reg"));
-  hipLaunchKernelGGL((k_pme_zcol_reg<T, K, XT>), dim3(nblk, n), dim3(XT), lds, c->st, k, dp<T>(c->d_engfac),
-                     dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<V>(c->d_w3h), dp<V>(c->d_tw3),
-                     dp<V>(c->d_Hk), dp<double>(c->d_selfp));
-  return ERM_OK;
-}
-// z columns with the tuned workgroup size (256 or 512)
-template <typename T, int K> int launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk) {
-  if (c->zcol_T == 512) return launch_zcol_reg_t<T, K, 512>(c, k, n, nblk);
-  return launch_zcol_reg_t<T, K, NTHR>(c, k, n, nblk);
-}
-template <typename T, int K, int PP, int XT>
-int launch_zcol_fft_t(erm_ctx *c, const Params &k, int n, int nblk) {
-  using V = typename C2<T>::type;
-  const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT);
-  TRY(set_lds(c, (const void *)k_pme_zcol_fft<T, K, PP, XT>, lds, "k_pme_zcol_fft"));
-  hipLaunchKernelGGL((k_pme_zcol_fft<T, K, PP, XT>), dim3(nblk, n), dim3(XT), lds, c->st, k, dp<T>(c->d_engfac),
-                     dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<int>(c->d_sg3), dp<T>(c->d_sw3),
-                     dp<V>(c->d_Hk), dp<double>(c->d_selfp));
-  return ERM_OK;
-}
-// z columns spread and transformed along z (spline orders 4 and 6)
-template <typename T, int K> int launch_zcol_fft(erm_ctx *c, const Params &k, int n, int nblk) {
-  if (k.p == 4) {
-    if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 4, 512>(c, k, n, nblk);
-    return launch_zcol_fft_t<T, K, 4, NTHR>(c, k, n, nblk);
-  }
-  if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 6, 512>(c, k, n, nblk);
-  return launch_zcol_fft_t<T, K, 6, NTHR>(c, k, n, nblk);
-}
-// inverse y columns: two lanes per column in FP64 (one lane's column does
-// not leave room for a second wave), one in FP32
-template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, int n) {
-  using V = typename C2<T>::type;
-  if constexpr (sizeof(T) == 8) {
-    int nblk = (2 * k.K3 * k.H + NTHR - 1) / NTHR;
-    hipLaunchKernelGGL((k_pme_ycol_pair<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_Hk));
-  } else {
-    int nblk = (k.K3 * k.H + NTHR - 1) / NTHR;
-    hipLaunchKernelGGL((k_pme_ycol_reg<T, K>), dim3(nblk, n), dim3(NTHR), 0, c->st, k, dp<V>(c->d_tw2),
-                       dp<V>(c->d_Hk));
-  }
-}
-template <typename T, int K, int R, int XT = NTHR, bool YF = false>
-int launch_xslab_r(erm_ctx *c, const Params &k, int n, size_t lds) {
-  using V = typename C2<T>::type;
-  TRY(set_lds(c, (const void *)k_pme_xslab<T, K, R, XT, YF>, lds, "k_pme_xslab"));
-  const int nslab = (k.K3 + c->slab_S - 1) / c->slab_S;
-  hipLaunchKernelGGL((k_pme_xslab<T, K, R, XT, YF>), dim3(nslab, n), dim3(XT), lds, c->st, k, c->slab_S, c->xslab_Q,
-                     c->nsites, dp<V>(c->d_Hk), dp<V>(c->d_tw1), dp<int>(c->d_zstart), dp<int>(c->d_zg),
-                     dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite), dp<double>(c->d_esite));
-  return ERM_OK;
-}
-// slab_R counts sites per thread of a 256-thread workgroup; a 512-thread
-// workgroup owns the same sites with half as many per thread
-// whether the x slab takes the inverse y transform (512-thread slabs, K2 = K1)
-bool xslab_folds_y(const erm_ctx *c) { return c->xslab_yf && c->xslab_T == 512 && c->kp.K2 == c->kp.K1; }
-
-template <typename T, int K> int launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds) {
-  if (xslab_folds_y(c)) {
-    if (c->slab_R == 8) return launch_xslab_r<T, K, 4, 512, true>(c, k, n, lds);
-    if (c->slab_R == 16) return launch_xslab_r<T, K, 8, 512, true>(c, k, n, lds);
-    return launch_xslab_r<T, K, 16, 512, true>(c, k, n, lds);
-  }
-  if (c->xslab_T == 512) {
-    if (c->slab_R == 8) return launch_xslab_r<T, K, 4, 512>(c, k, n, lds);
-    if (c->slab_R == 16) return launch_xslab_r<T, K, 8, 512>(c, k, n, lds);
-    return launch_xslab_r<T, K, 16, 512>(c, k, n, lds);
-  }
-  if (c->slab_R == 8) return launch_xslab_r<T, K, 8>(c, k, n, lds);
-  if (c->slab_R == 16) return launch_xslab_r<T, K, 16>(c, k, n, lds);
-  return launch_xslab_r<T, K, 32>(c, k, n, lds);
 }
 
 #define ERM_KSWITCH(K, CALL)                  \
@@ -1050,6 +765,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     ERM_KSWITCH(K1, (rc = launch_xslab<T, KK>(c, k, n, lx)));
     TRY(rc);
     TRY(launch_check(c, "k_pme_xslab"));
+    TRY(mark(c, 4));
     Params kg = k;
     if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
     hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
@@ -1081,6 +797,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                            dp<int>(c->d_zg), dp<T>(c->d_zq), dp<T>(c->d_zw), dp<int>(c->d_zsite),
                            dp<double>(c->d_esite));
       TRY(launch_check(c, "k_pme_slab"));
+      TRY(mark(c, 4));
       hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
                          dp<double>(c->d_esite), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
                          dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
@@ -1095,6 +812,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                          dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), dp<int>(c->d_sg),
                          dp<T>(c->d_sw), dp<T>(c->d_sq), dp<int>(c->d_sltj), dp<double>(c->d_uv));
       TRY(launch_check(c, "k_gather"));
+      TRY(mark(c, 4));
     }
   } else {
     const size_t nuv = (size_t)n * (S + 1);
@@ -1103,8 +821,9 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     TRY(launch_check(c, "k_fill"));
     TRY(mark(c, 2));
     TRY(mark(c, 3));
+    TRY(mark(c, 4));
   }
-  TRY(mark(c, 4));
+  TRY(mark(c, 5));
   if ((flags & DO_REAL) && bare && c->P.boxshp == ERM_SYS_NONPERIODIC) {
     size_t lo = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * c->nvt) + 64;
     TRY(set_lds(c, (const void *)k_pair_open<T>, lo, "k_pair_open"));
@@ -1133,7 +852,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                          dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
     TRY(launch_check(c, "k_pair"));
   }
-  TRY(mark(c, 5));
+  TRY(mark(c, 6));
   if (flags & DO_SELF) {
     // realcal_self: reused after the first configuration for a rigid test
     // particle (engproc.F90:712-730), recomputed per configuration otherwise
@@ -1157,6 +876,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                        dp<double>(c->d_uv));
     TRY(launch_check(c, "k_u0"));
   }
+  TRY(mark(c, 7));
   return ERM_OK;
 }
 
@@ -1200,7 +920,7 @@ int run_histogram(erm_ctx *c, int n, int first) {
                        dp<double>(c->d_intm_sc));
     TRY(launch_check(c, "k_intm_scal"));
   }
-  TRY(mark(c, 6));
+  TRY(mark(c, 8));
   if (c->P.corrcal) {
     int nt = (E + 15) / 16;
     int tiles = nt * (nt + 1) / 2;  // tiles on and above the diagonal, one workgroup each
@@ -1208,7 +928,7 @@ int run_histogram(erm_ctx *c, int n, int first) {
                        dp<double>(c->d_w), dp<double>(c->d_ecorr));
     TRY(launch_check(c, "k_corr"));
   }
-  TRY(mark(c, 7));
+  TRY(mark(c, 9));
   return ERM_OK;
 }
 
@@ -1222,26 +942,26 @@ int copy_out_uv(erm_ctx *c, int n, double *out) {
   return sync(c);
 }
 
-// stage times of the last call from the per-batch events (after a sync)
+// stage times of the last call from the per-batch events (after a sync):
+// NSTAGE_EV events per batch; stage i runs from event i to event i + 1, the
+// histogram stages (bin, correlation) only when the call accumulates
 int stage_times(erm_ctx *c, int nb, bool hist) {
-  for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
+  for (int i = 0; i < NSTAGE_OUT; ++i) c->stage_ms[i] = 0;
   float ms;
+  const int last = hist ? NSTAGE_EV - 1 : 7;
   for (int bi = 0; bi < nb; ++bi) {
-    hipEvent_t *ev = &c->bev[8 * bi];
-    for (int i = 0; i < 7; ++i) {
-      int a = i, b = i + 1;
-      if (!hist && b > 5) continue;
-      if (hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess) c->stage_ms[i] += ms;
-    }
-    if (hipEventElapsedTime(&ms, ev[0], ev[hist ? 7 : 5]) == hipSuccess) c->stage_ms[7] += ms;
+    hipEvent_t *ev = &c->bev[NSTAGE_EV * bi];
+    for (int i = 0; i < last; ++i)
+      if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) c->stage_ms[i] += ms;
+    if (hipEventElapsedTime(&ms, ev[0], ev[last]) == hipSuccess) c->stage_ms[NSTAGE_OUT - 1] += ms;
   }
   (void)hipGetLastError();
   return ERM_OK;
 }
 
-// 8 stage events for each of nb batches
+// NSTAGE_EV stage events for each of nb batches
 int batch_events(erm_ctx *c, int nb) {
-  while ((int)c->bev.size() < 8 * nb) {
+  while ((int)c->bev.size() < NSTAGE_EV * nb) {
     hipEvent_t e;
     HIPCHK(c, hipEventCreate(&e));
     c->bev.push_back(e);
@@ -1303,7 +1023,7 @@ int enqueue_batches(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
     if (all.solutes) bt.solutes = all.solutes + s0;
     if (all.weights) bt.weights = all.weights + s0;
     if (all.flex) bt.flex = all.flex + (size_t)s0 * ns * 3;
-    c->cur_ev = (8 * (bi + 1) <= (int)c->bev.size()) ? &c->bev[8 * bi] : nullptr;
+    c->cur_ev = (NSTAGE_EV * (bi + 1) <= (int)c->bev.size()) ? &c->bev[NSTAGE_EV * bi] : nullptr;
     if (bt.weights) {
       TRY(upload(c, c->d_wsol, bt.weights, bt.n));
       c->wsol_ones = 0;
@@ -1320,7 +1040,7 @@ int enqueue_batches(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
 // energies (and optionally histograms) over n configurations in batches;
 // one synchronisation at the end (the error status and the stage times)
 int process(erm_ctx *c, const Batch &all, int flags, double *uv_out) {
-  for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
+  for (int i = 0; i < NSTAGE_OUT; ++i) c->stage_ms[i] = 0;
   const int nb = (all.n + c->Bmax - 1) / c->Bmax;
   TRY(batch_events(c, nb));
   TRY(begin_call(c, all));
@@ -1348,14 +1068,14 @@ std::string graph_key(erm_ctx *c, const Batch &all, int flags) {
 int process_graph(erm_ctx *c, const Batch &all, int flags) {
   std::string key = graph_key(c, all, flags);
   if (c->gexec && key == c->gkey) {
-    for (int i = 0; i < 8; ++i) c->stage_ms[i] = 0;
+    for (int i = 0; i < NSTAGE_OUT; ++i) c->stage_ms[i] = 0;
     TRY(begin_call(c, all));
     HIPCHK(c, hipEventRecord(c->ev[0], c->st));
     HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
     HIPCHK(c, hipEventRecord(c->ev[7], c->st));
     TRY(check_status(c));
     float ms;
-    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[7]) == hipSuccess) c->stage_ms[7] = ms;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[7]) == hipSuccess) c->stage_ms[NSTAGE_OUT - 1] = ms;
     return ERM_OK;
   }
   TRY(process(c, all, flags, nullptr));
@@ -1461,31 +1181,6 @@ int set_frame_common(erm_ctx *c, const double *cell, double w) {
 }
 
 // ---- solute gradient ---------------------------------------------------
-template <typename T, int K> void launch_xr2c(erm_ctx *c, const Params &k) {
-  using V = typename C2<T>::type;
-  const int nt = k.K3 * ((k.K2 + 1) / 2);
-  hipLaunchKernelGGL((k_xrow_r2c<T, K>), dim3((nt + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k,
-                     dp<T>(c->d_ggrid), dp<V>(c->d_tw1), dp<V>(c->d_gHk));
-}
-template <typename T, int K> void launch_xc2r(erm_ctx *c, const Params &k) {
-  using V = typename C2<T>::type;
-  const int nt = k.K3 * ((k.K2 + 1) / 2);
-  hipLaunchKernelGGL((k_xrow_c2r<T, K>), dim3((nt + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k,
-                     dp<V>(c->d_gHk), dp<V>(c->d_tw1), dp<T>(c->d_ggrid));
-}
-template <typename T, int K, int SIGN> void launch_ydense(erm_ctx *c, const Params &k) {
-  using V = typename C2<T>::type;
-  const int nt = k.K3 * k.H;
-  hipLaunchKernelGGL((k_pme_ycol_reg<T, K, SIGN>), dim3((nt + NTHR - 1) / NTHR, 1), dim3(NTHR), 0, c->st, k,
-                     dp<V>(c->d_tw2), dp<V>(c->d_gHk));
-}
-template <typename T, int K> void launch_zgreen(erm_ctx *c, const Params &k) {
-  using V = typename C2<T>::type;
-  const int nt = k.H * k.K2;
-  hipLaunchKernelGGL((k_zcol_green<T, K>), dim3((nt + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, k,
-                     dp<T>(c->d_engfac), dp<V>(c->d_tw3), dp<V>(c->d_gHk));
-}
-
 // PME potential of every physical molecule except sorted molecule `excl`
 // on the real grid d_ggrid: spread, r2c, Green's function, c2r (the solvent
 // counterpart of recpcal_prepare_solute, recpcal.F90:342-406); once per frame
@@ -2251,7 +1946,7 @@ int32_t erm_insert_random_multi(erm_ctx *const *ctxs, int32_t nctx, const int32_
   for (int i = 0; i < nctx; ++i) {
     erm_ctx *c = ctxs[i];
     Batch bt{n[i], 0, nullptr, nullptr, nullptr, true, seed[i], first_index[i]};
-    for (int k = 0; k < 8; ++k) c->stage_ms[k] = 0;
+    for (int k = 0; k < NSTAGE_OUT; ++k) c->stage_ms[k] = 0;
     nbs[i] = (bt.n + c->Bmax - 1) / c->Bmax;
     int st = batch_events(c, nbs[i]);
     if (!st) st = begin_call(c, bt);
@@ -2430,9 +2125,9 @@ int32_t erm_sizes(const erm_ctx *c, int32_t *slvmax, int32_t *ermax) {
   return ERM_OK;
 }
 
-int32_t erm_stage_times(const erm_ctx *c, double *ms8) {
-  if (!c || !ms8) return ERM_E_ARG;
-  for (int i = 0; i < 8; ++i) ms8[i] = c->stage_ms[i];
+int32_t erm_stage_times(const erm_ctx *c, double *ms10) {
+  if (!c || !ms10) return ERM_E_ARG;
+  for (int i = 0; i < NSTAGE_OUT; ++i) ms10[i] = c->stage_ms[i];
   return ERM_OK;
 }
 

@@ -379,7 +379,7 @@ __global__ void k_grad_recip(Params P, int n, const double *__restrict__ spos, c
 }
 
 // per configuration: u_b = sum_s (u_real + q_s phi_s) + Q_slt * rf
-__global__ void k_grad_ub(Params P, int n, int nsplit, const double *__restrict__ part,
+static __global__ void k_grad_ub(Params P, int n, int nsplit, const double *__restrict__ part,
                           const double *__restrict__ phi, const double *__restrict__ qd, double rf, double qslt,
                           double *ub) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,7 +396,7 @@ __global__ void k_grad_ub(Params P, int n, int nsplit, const double *__restrict_
 
 // weighted batch sums: out = [sum w, sum w u, sum w du/dq (ns),
 // sum w du/deps (ns), sum w du/dsigma (ns)]; thread per output, b in order
-__global__ void k_grad_reduce(Params P, int n, int nsplit, const double *__restrict__ part,
+static __global__ void k_grad_reduce(Params P, int n, int nsplit, const double *__restrict__ part,
                               const double *__restrict__ phi, const double *__restrict__ ub,
                               const double *__restrict__ w, const double *__restrict__ epsd, double rf,
                               double *out) {

@@ -66,7 +66,7 @@ __device__ __forceinline__ int cell_index(const Params &P, const double *w) {
 
 // cell of each physical molecule (by its first site) and the largest
 // distance of a site from its molecule's first site
-__global__ void k_mol_cell(Params P, const double *__restrict__ pos, const int *__restrict__ tagpt,
+static __global__ void k_mol_cell(Params P, const double *__restrict__ pos, const int *__restrict__ tagpt,
                            const int *__restrict__ mol_begin, int *cellid, int *count,
                            unsigned long long *rm_ord) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,7 +95,7 @@ __global__ void k_mol_cell(Params P, const double *__restrict__ pos, const int *
 // workgroup).  copy: a second copy of out[0, n) (the fill cursors of a
 // counting sort), or null; planes: planes[z] = out[z KB] for z <= K3, or
 // null.
-__global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *out, int n, int *copy, int *planes,
+static __global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *out, int n, int *copy, int *planes,
                                                int K3, int KB) {
   __shared__ int part[1024];
   const int t = threadIdx.x;
@@ -129,14 +129,14 @@ __global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *
 
 // the per-frame counters zeroed in one launch: cell counts, z-bucket
 // counts (nz may be 0) and the molecule-extent maximum
-__global__ void k_frame_clear(int *count, int nc, int *zcount, int nz, unsigned long long *rm) {
+static __global__ void k_frame_clear(int *count, int nc, int *zcount, int nz, unsigned long long *rm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nc) count[i] = 0;
   if (i < nz) zcount[i] = 0;
   if (i == 0) *rm = 0ULL;
 }
 
-__global__ void k_mol_scatter(Params P, const int *__restrict__ cellid, int *fill,
+static __global__ void k_mol_scatter(Params P, const int *__restrict__ cellid, int *fill,
                               const int *__restrict__ tagpt, const int *__restrict__ mol_begin,
                               int *mol_k, int *mol_ns) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,7 +207,7 @@ __device__ __forceinline__ double rng_uniform(uint64_t seed, uint64_t index, uin
   return (double)(z >> 11) * 0x1.0p-53;
 }
 
-__device__ void com_dev(int n, const double *pts, const double *mass, double *c) {
+static __device__ void com_dev(int n, const double *pts, const double *mass, double *c) {
 #pragma clang fp contract(off)
   double msum = 0.0;
   for (int j = 0; j < n; ++j) msum = msum + mass[j];
@@ -218,7 +218,7 @@ __device__ void com_dev(int n, const double *pts, const double *mass, double *c)
   }
 }
 
-__device__ void qprod_dev(const double *a, const double *b, double *p) {
+static __device__ void qprod_dev(const double *a, const double *b, double *p) {
 #pragma clang fp contract(off)
   double dot = a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
   p[0] = a[0] * b[0] - dot;
@@ -229,7 +229,7 @@ __device__ void qprod_dev(const double *a, const double *b, double *p) {
 // a trajectory frame (FP32 SoA x[n], y[n], z[n]) into the FP64 AoS
 // coordinates through the permutation (setconf.F90:983-987): trajectory atom
 // i becomes atom perm[i]; the conversion to FP64 is exact
-__global__ void k_frame_scatter(int n, const float *__restrict__ xyz, const int *__restrict__ perm, double *pos) {
+static __global__ void k_frame_scatter(int n, const float *__restrict__ xyz, const int *__restrict__ perm, double *pos) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int a = perm ? perm[i] : i;
@@ -239,7 +239,7 @@ __global__ void k_frame_scatter(int n, const float *__restrict__ xyz, const int 
 }
 
 // coordinates of the listed atoms (the reference-structure host atoms)
-__global__ void k_gather_atoms(int n, const int *__restrict__ idx, const double *__restrict__ pos, double *out) {
+static __global__ void k_gather_atoms(int n, const int *__restrict__ idx, const double *__restrict__ pos, double *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int d = 0; d < 3; ++d) out[3 * i + d] = pos[3 * idx[i] + d];
@@ -248,7 +248,7 @@ __global__ void k_gather_atoms(int n, const int *__restrict__ idx, const double 
 // com_aggregate (insertion.F90:539-566): centre of mass of every site of
 // the host molecules, summed in molecule order by a single thread so that the
 // rounding is that of the sequential sums (center_of_mass, bestfit.F90:158-172)
-__global__ void k_aggcen(int nmol, const int *__restrict__ mol_begin, const int *__restrict__ hostmol,
+static __global__ void k_aggcen(int nmol, const int *__restrict__ mol_begin, const int *__restrict__ hostmol,
                          const double *__restrict__ mass, const double *__restrict__ pos, double *c) {
 #pragma clang fp contract(off)
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -286,7 +286,7 @@ constexpr uint64_t POS_STREAM = 0xD1B54A32D192ED03ULL;
 // site order) from the positions, kept in LDS when 4 ns doubles fit
 // (POSE_LDS_SITES), else in spos itself.
 constexpr int POSE_LDS_SITES = 2048;
-__global__ __launch_bounds__(64) void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
+static __global__ __launch_bounds__(64) void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
                                              const double *__restrict__ bf, int bf_stride,
                                              const double *__restrict__ mass, const double *__restrict__ cellm,
                                              const double *__restrict__ aggcen, double *spos) {
@@ -1290,7 +1290,7 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
 // per-molecule reciprocal energy from the per-site sums, plus the
 // neutralising background (residual_ene, engproc.F90:859-869):
 // uv[b][1 + j] = -pi Q_s Q_j / (alpha^2 V) + sum over the sites of j
-__global__ void k_mol_sum(Params P, int nsites, const double *__restrict__ e_site,
+static __global__ void k_mol_sum(Params P, int nsites, const double *__restrict__ e_site,
                           const int *__restrict__ mol_first, const int *__restrict__ mol_ns,
                           const double *__restrict__ mol_q, const int *__restrict__ slt_j, double *uv) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1318,7 +1318,7 @@ __device__ __forceinline__ int zbucket(int g, int KB) {
 }
 
 // counts per bucket in LDS, one global add per bucket and workgroup
-__global__ __launch_bounds__(256) void k_zcount(int nsites, int nbk, int KB, const int *__restrict__ sg,
+static __global__ __launch_bounds__(256) void k_zcount(int nsites, int nbk, int KB, const int *__restrict__ sg,
                                                 int *zcount) {
   __shared__ int cnt[ZBUCKET_MAX];
   for (int z = threadIdx.x; z < nbk; z += blockDim.x) cnt[z] = 0;
@@ -1505,7 +1505,7 @@ __device__ __forceinline__ T lj_bare_t(const Params &P, T eps, T sig2, T dis2, T
 }
 
 // inclusive prefix sum of a[0..n) in LDS, n <= 8 * blockDim.x; all threads call
-__device__ void block_inclusive_scan(int *a, int n) {
+static __device__ void block_inclusive_scan(int *a, int n) {
   __shared__ int part[NTHR];
   const int per = (n + blockDim.x - 1) / blockDim.x;
   const int b0 = threadIdx.x * per;
@@ -1537,7 +1537,7 @@ __device__ void block_inclusive_scan(int *a, int n) {
 // flatten the candidates in the same order.  seg_beg[i] is the first
 // molecule of segment i and seg_pre[i] the number of candidates before it;
 // returns the segment count nsg (seg_pre[nsg] = total).  All threads call.
-__device__ int pillar_segments(const Params &P, const double Cw[3], double R, const int *__restrict__ cell_start,
+static __device__ int pillar_segments(const Params &P, const double Cw[3], double R, const int *__restrict__ cell_start,
                                int *seg_beg, int *seg_pre) {
   const double R2 = R * R;
   int lo[3], cnt[3];
@@ -1819,7 +1819,7 @@ __global__ __launch_bounds__(NTHR) void k_pair_open(Params P, const double *__re
 }
 
 // realcal_self (realcal.F90:284-328) of configuration b, FP64
-__global__ __launch_bounds__(NTHR) void k_selfreal(Params P, const double *__restrict__ spos, int b,
+static __global__ __launch_bounds__(NTHR) void k_selfreal(Params P, const double *__restrict__ spos, int b,
                                                    const double *__restrict__ qsolute, double *out) {
   __shared__ double red[NTHR];
   const int ns = P.ns;
@@ -1852,7 +1852,7 @@ __global__ __launch_bounds__(NTHR) void k_selfreal(Params P, const double *__res
 }
 
 // uvengy(0) = self_recp + realcal_self + residual_self (engproc.F90:701-732)
-__global__ void k_u0(Params P, int n, int ngrp, const double *__restrict__ selfp,
+static __global__ void k_u0(Params P, int n, int ngrp, const double *__restrict__ selfp,
                      const double *__restrict__ selfreal, int per_item, double *uv) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
@@ -1893,7 +1893,7 @@ constexpr int REF_ATTEMPTS = 4096;
 // density at every attempt), the orientation, and acceptance when the RMSD
 // to the best-fit reference solute (ref[3ns..6ns), weights refw) lies in
 // [lwreg, upreg]; else another attempt.  wsol[b] receives the weight (GAUSS).
-__global__ void k_pose_refstr(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
+static __global__ void k_pose_refstr(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
                               const double *__restrict__ mass, const double *__restrict__ cellm,
                               const double *__restrict__ ref, const double *__restrict__ refw, double *spos,
                               double *wsol, int *status) {
@@ -1985,7 +1985,7 @@ __global__ void k_pose_refstr(Params P, int n, const uint64_t *__restrict__ args
   if (P.ins_pos == 6) wsol[b] = weight;
 }
 
-__global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict__ uv,
+static __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict__ uv,
                                               const double *__restrict__ edges,
                                               const int *__restrict__ spec_off, const int *__restrict__ uvmax,
                                               const int *__restrict__ uvsoft, const int *__restrict__ mol_spec,
@@ -2083,7 +2083,7 @@ __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__restrict
 // (wgtslf, volcorrect) and its self-energy bin; thread 0 then adds them in
 // configuration order, as the reference's sequential loop does.
 // scal: [0] engnorm [1] engsmpl [2] avslf [3] voffset [4] voffset set
-__global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double *__restrict__ uv,
+static __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double *__restrict__ uv,
                                                   const double *__restrict__ wsol,
                                                   const double *__restrict__ escrd, double *scal, double *eself,
                                                   unsigned long long *mnmx, double *wout, int *status, int first) {
@@ -2192,7 +2192,7 @@ __global__ __launch_bounds__(NTHR) void k_scalars(Params P, int n, const double 
 // 32 bins; its 8 waves' halves sum 8 consecutive chunks of the
 // configurations (in order within a chunk, empty bins skipped as the
 // reference skips them), then the chunk sums are added in chunk order.
-__global__ __launch_bounds__(256) void k_edens(int n, int ermax, const double *__restrict__ hist,
+static __global__ __launch_bounds__(256) void k_edens(int n, int ermax, const double *__restrict__ hist,
                                                const double *__restrict__ w, double *edens) {
   __shared__ double part[8][32];
   const int lb = threadIdx.x & 31, ch = threadIdx.x >> 5;
@@ -2221,7 +2221,7 @@ __global__ __launch_bounds__(256) void k_edens(int n, int ermax, const double *_
 // intermediate state (extensions/Intermediate/engproc.F90:996-1009): the
 // weight of configuration b if its instantaneous histogram touches no bin
 // that is empty in the reference solvent, else 0; one workgroup per b
-__global__ __launch_bounds__(256) void k_intm_weight(int ermax, const double *__restrict__ hist,
+static __global__ __launch_bounds__(256) void k_intm_weight(int ermax, const double *__restrict__ hist,
                                                      const int *__restrict__ zerodns, const double *__restrict__ w,
                                                      double *wi) {
   __shared__ int hit;
@@ -2239,7 +2239,7 @@ __global__ __launch_bounds__(256) void k_intm_weight(int ermax, const double *__
 // soln_total += engnmfc; soln_intm and norm_intm += the intermediate weight
 // (adding the 0 of an excluded configuration leaves the sum unchanged), in
 // configuration order; sc = {norm_intm, soln_intm, soln_total}
-__global__ void k_intm_scal(int n, const double *__restrict__ w, const double *__restrict__ wi, double *sc) {
+static __global__ void k_intm_scal(int n, const double *__restrict__ w, const double *__restrict__ wi, double *sc) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double a = sc[0], i = sc[1], t = sc[2];
   for (int b = 0; b < n; ++b) {
@@ -2253,7 +2253,7 @@ __global__ void k_intm_scal(int n, const double *__restrict__ w, const double *_
 }
 
 // slnuv(:) += flceng(:, cntdst) * engnmfc (engproc.F90:832-834)
-__global__ void k_slnuv(int n, int numslv, const double *__restrict__ flceng, const double *__restrict__ w,
+static __global__ void k_slnuv(int n, int numslv, const double *__restrict__ flceng, const double *__restrict__ w,
                         double *slnuv) {
   int s = threadIdx.x;
   if (s >= numslv) return;
@@ -2272,7 +2272,7 @@ __global__ void k_slnuv(int n, int numslv, const double *__restrict__ flceng, co
 // --------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__restrict__ hist,
+static __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__restrict__ hist,
                                               const double *__restrict__ w, double *ecorr) {
   __shared__ dbl4 part[3][64];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
@@ -2335,7 +2335,7 @@ __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const double *__
 }
 
 // uvengy in tagpt order from the sorted order
-__global__ void k_unsort(int n, int slvmax, const int *__restrict__ mol_k, const double *__restrict__ uv,
+static __global__ void k_unsort(int n, int slvmax, const int *__restrict__ mol_k, const double *__restrict__ uv,
                          double *out) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;

@@ -187,10 +187,10 @@ module ermgpu
        integer(c_int32_t), intent(out) :: slvmax, ermax
      end function erm_sizes
 
-     integer(c_int32_t) function erm_stage_times(ctx, ms8) bind(C, name="erm_stage_times")
+     integer(c_int32_t) function erm_stage_times(ctx, ms10) bind(C, name="erm_stage_times")
        import :: c_ptr, c_int32_t, c_double
        type(c_ptr), value :: ctx
-       real(c_double), intent(out) :: ms8(8)
+       real(c_double), intent(out) :: ms10(10)
      end function erm_stage_times
 
      integer(c_int32_t) function erm_set_tuning(ctx, max_batch, pme_path, use_graph, slab_planes, &

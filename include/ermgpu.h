@@ -302,11 +302,12 @@ int32_t erm_sizes(const erm_ctx *ctx, int32_t *slvmax, int32_t *ermax);
 
 /* Device time of the last engconst / insert_random call, split by stage
  * (milliseconds, HIP events on the context's stream, summed over batches):
- * [0] poses and solute splines, [1] PME z columns, [2] PME y transform
- * (planes, for the unfused path), [3] PME x transform and gather, [4] real
- * space and self terms, [5] binning, [6] correlation, [7] total.  A call
- * replayed from a graph reports the total only. */
-int32_t erm_stage_times(const erm_ctx *ctx, double *ms8);
+ * [0] poses and solute splines, [1] PME z columns (spread, z FFT, Green's
+ * function), [2] PME inverse y transform, [3] PME x transform and gather
+ * into per-site sums, [4] per-molecule sums, [5] real-space pairs, [6] self
+ * terms, [7] binning, weights and edens, [8] correlation matrix, [9] total.
+ * A call replayed from a graph reports the total only. */
+int32_t erm_stage_times(const erm_ctx *ctx, double *ms10);
 
 /* ---- batching and tuning ------------------------------------------------- */
 /* No counterpart in the reference (its engconst runs one insertion at a

@@ -115,6 +115,9 @@ def lib():
         L.orc_flceng.argtypes = [vp]
         L.orc_flceng.restype = _f64p
         L.orc_solute_gradient.argtypes = [vp, C.c_int32, _f64p, C.c_int32, _f64p, _f64p, _f64p, _f64p]
+        L.orc_bench_insertions.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(_f64p),
+                                           C.POINTER(_f64p), _f64p, C.c_int64, C.c_uint64, C.c_uint64, C.c_double,
+                                           C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -461,6 +464,49 @@ class Oracle:
             out["eself"] = np.ctypeslib.as_array(a.eself, (a.esmax,)).copy()
         out["flceng"] = np.ctypeslib.as_array(lib().orc_flceng(self.h), (ns + 1,)).copy()
         return out
+
+
+def host_threads():
+    """Cores this process may run on: the scheduler's affinity set, capped by
+    OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def bench_insertions(systems, solute_mols, frame, cell, n, seconds, seed=0, first_index=0, threads=None):
+    """Times engconst's test-particle loop on the CPU: up to n random
+    insertions (insertion i of systems[i % len(systems)]), one oracle context
+    per (thread, system) holding the frame, OpenMP over insertions, stopped
+    after `seconds`.  Returns (insertions done per system, seconds, threads)."""
+    import time
+    nt = threads or host_threads()
+    ntypes = len(systems)
+    ctxs = []
+    for _ in range(nt):
+        for s in systems:
+            o = Oracle(s)
+            o.set_frame(frame, cell)
+            ctxs.append(o)
+    handles = (C.c_void_p * len(ctxs))(*[o.h for o in ctxs])
+    bfs = [f64(s.bfcoord) for s in systems]
+    masses = [f64(s.solute_mass(m)) for s, m in zip(systems, solute_mols)]
+    bfp = (_f64p * ntypes)(*[P(b) for b in bfs])
+    mp = (_f64p * ntypes)(*[P(m) for m in masses])
+    mols = i32(solute_mols)
+    nsite = i32([len(b) for b in bfs])
+    cm = f64(np.asarray(cell).T.copy())
+    done = (C.c_int64 * ntypes)()
+    t0 = time.perf_counter()
+    st = lib().orc_bench_insertions(handles, nt, ntypes, P(mols, _i32p), P(nsite, _i32p), bfp, mp, P(cm),
+                                    int(n), C.c_uint64(seed), C.c_uint64(first_index), float(seconds), done)
+    dt = time.perf_counter() - t0
+    check(st, "bench_insertions")
+    for o in ctxs:
+        o.close()
+    return [int(d) for d in done], dt, nt
 
 
 def ref_lib():

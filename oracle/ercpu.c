@@ -28,6 +28,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -2201,4 +2202,68 @@ int32_t orc_solute_gradient(orc_ctx *c, int32_t solute, const double *solute_pos
   free(q);
   free(elj);
   return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU baseline: engconst's test-particle loop (engproc.F90:388-392: pose,  */
+/* get_uv_energy, update_histogram) over the host's cores.  Insertion i     */
+/* goes to solute type i mod ntypes and runs on the context of its OpenMP   */
+/* thread for that type: ctxs[thread * ntypes + type], each holding the     */
+/* frame, its own scratch and its own accumulators (the reference's threads */
+/* share one insertion and split its molecules; splitting the insertions    */
+/* instead needs no synchronisation and is the faster CPU schedule).        */
+/* ------------------------------------------------------------------------ */
+int32_t orc_bench_insertions(orc_ctx *const *ctxs, int32_t nthreads, int32_t ntypes,
+                             const int32_t *solute_mol, const int32_t *nsite,
+                             const double *const *bfcoord, const double *const *mass,
+                             const double *cell, int64_t n, uint64_t seed, uint64_t first_index,
+                             double seconds, int64_t *done_per_type) {
+  if (nthreads < 1 || ntypes < 1 || n < 0) return ORC_E_ARG;
+  int nsmax = 0, smax = 0;
+  for (int t = 0; t < ntypes; ++t) {
+    if (nsite[t] > nsmax) nsmax = nsite[t];
+    if (ctxs[t]->slvmax > smax) smax = ctxs[t]->slvmax;
+  }
+  for (int t = 0; t < ntypes; ++t) done_per_type[t] = 0;
+  int32_t status = ORC_OK;
+  const double t_end = omp_get_wtime() + seconds;
+  int stop = 0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    const int th = omp_get_thread_num();
+    double *pos = (double *)malloc(sizeof(double) * 3 * (size_t)nsmax);
+    double *uv = (double *)malloc(sizeof(double) * ((size_t)smax + 1));
+    int64_t *mine = (int64_t *)calloc((size_t)ntypes, sizeof(int64_t));
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t i = 0; i < n; ++i) {
+      int halt;
+#pragma omp atomic read
+      halt = stop;
+      if (halt) continue;
+      const int ty = (int)(i % ntypes);
+      orc_ctx *c = ctxs[(size_t)th * ntypes + ty];
+      double r[3], q[4];
+      orc_random_pose(seed, first_index + (uint64_t)i, r, q);
+      orc_place_solute(nsite[ty], bfcoord[ty], mass[ty], cell, r, q, pos);
+      int32_t st = orc_uv_energy(c, solute_mol[ty], pos, uv);
+      if (!st) st = orc_update_histogram(c, solute_mol[ty], 1.0, uv, NULL);
+      if (st) {
+#pragma omp critical
+        status = st;
+#pragma omp atomic write
+        stop = 1;
+      }
+      ++mine[ty];
+      if (omp_get_wtime() > t_end) {
+#pragma omp atomic write
+        stop = 1;
+      }
+    }
+#pragma omp critical
+    for (int t = 0; t < ntypes; ++t) done_per_type[t] += mine[t];
+    free(pos);
+    free(uv);
+    free(mine);
+  }
+  return status;
 }

@@ -227,6 +227,20 @@ int32_t orc_solute_gradient(orc_ctx *c, int32_t solute, const double *solute_pos
                             int32_t cmbrule, const double *q, const double *eps,
                             const double *sigma, double *out);
 
+
+/* CPU baseline (not part of the reference's interface): n random insertions
+ * of rigid test particles, insertion i of solute type i mod ntypes, pose from
+ * the counter-based generator (seed, first_index + i) as orc_random_pose +
+ * orc_place_solute, then uv_energy + update_histogram on
+ * ctxs[thread * ntypes + type] (each context holds the frame already).
+ * Stops early once `seconds` have passed; done_per_type[ntypes] counts the
+ * insertions completed. */
+int32_t orc_bench_insertions(orc_ctx *const *ctxs, int32_t nthreads, int32_t ntypes,
+                             const int32_t *solute_mol, const int32_t *nsite,
+                             const double *const *bfcoord, const double *const *mass,
+                             const double *cell, int64_t n, uint64_t seed, uint64_t first_index,
+                             double seconds, int64_t *done_per_type);
+
 #ifdef __cplusplus
 }
 #endif

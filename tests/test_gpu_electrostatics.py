@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from bars import FP32_ABS, fp32_ok, fp32_report
 from ermod_amd import synth
 from ermod_amd.api import ErmError, ErmGPU
 from ermod_amd.system import EL_COULOMB, EL_PPPM, SYS_NONPERIODIC, SYS_PERIODIC
@@ -26,10 +27,6 @@ def print_case(name):
     print(f"\n[gpu case] {name}", flush=True)
 
 
-def fp32_ok(e32, e64):
-    err = np.abs(e32 - e64)
-    lim = np.where(np.abs(e64) <= 100.0, 1e-4, 1e-5 * np.abs(e64))
-    return err <= lim, err
 
 
 def variant(kind, cltype, boxshp=SYS_PERIODIC):

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from bars import FP32_ABS, fp32_ok, fp32_report
 from ermod_amd import synth
 from ermod_amd.api import ErmError, ErmGPU
 from ermod_amd.system import (INSORG_AGGCEN, INSORG_ORIGIN, INSORG_REFSTR, INSPOS_GAUSS, INSPOS_RANDOM,
@@ -57,7 +58,7 @@ def test_insertion_modes_match_oracle(name, ins, prec):
                     o.update_histogram(m, want)
                 else:
                     err = np.abs(u[i, 1:] - want[1:])
-                    lim = np.where(np.abs(want[1:]) <= 100.0, 1e-4, 1e-5 * np.abs(want[1:]))
+                    lim = FP32_ABS
                     assert (err <= lim).all(), (i, err.max())
         b = g.fetch()
     if prec == "fp64":

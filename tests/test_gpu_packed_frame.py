@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from bars import FP32_ABS, fp32_ok, fp32_report
 from ermod_amd import synth
 from ermod_amd.api import ErmGPU
 
@@ -58,5 +59,5 @@ def test_dense_frame_after_uniform_frame(prec):
             np.testing.assert_allclose(u[i], want, rtol=1e-11, atol=1e-9)
         else:
             err = np.abs(u[i, 1:] - want[1:])
-            lim = np.where(np.abs(want[1:]) <= 100.0, 1e-4, 1e-5 * np.abs(want[1:]))
+            lim = FP32_ABS
             assert (err <= lim).all(), (i, err.max())
