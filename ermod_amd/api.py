@@ -60,7 +60,8 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_stage_times", "erm_solute_gradient", "erm_set_tuning", "erm_get_tuning", "erm_autotune",
            "erm_insert_random_multi", "erm_set_insertion", "erm_dcd_open", "erm_dcd_read", "erm_dcd_close",
            "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close", "erm_set_intermediate",
-           "erm_fetch_intermediate", "erm_set_refstructure", "erm_insert_flexible")
+           "erm_fetch_intermediate", "erm_set_refstructure", "erm_insert_flexible", "erm_set_pair_path",
+           "erm_get_pair_path")
 
 _lib = None
 
@@ -101,6 +102,8 @@ def lib():
         L.erm_set_tuning.argtypes = [vp] + [C.c_int32] * 8
         L.erm_get_tuning.argtypes = [vp, _i32p]
         L.erm_autotune.argtypes = [vp, C.c_int32, C.c_uint64, _f64p]
+        L.erm_set_pair_path.argtypes = [vp, C.c_int32]
+        L.erm_get_pair_path.argtypes = [vp]
         L.erm_insert_random_multi.argtypes = [C.POINTER(vp), C.c_int32, _i32p, C.POINTER(C.c_uint64),
                                               C.POINTER(C.c_uint64)]
         L.erm_solute_gradient.argtypes = [vp, C.c_int32, _i32p, _f64p, C.c_uint64, C.c_uint64, _f64p,
@@ -504,6 +507,14 @@ class ErmGPU:
                     slab_threads=int(out[3]), slab_planes=int(out[4]), group_planes=int(out[5]),
                     slab_cap=int(out[6]), group_cap=int(out[7]), column_threads=int(out[8]),
                     fold_y=bool(out[9]))
+
+    def set_pair_path(self, path):
+        """'tiles' (compacted per-wave pair lists) or 'molecule' (one thread
+        per candidate molecule)."""
+        self._check(lib().erm_set_pair_path(self.h, {"tiles": 0, "molecule": 1}[path]), "set_pair_path")
+
+    def pair_path(self):
+        return {0: "tiles", 1: "molecule"}[lib().erm_get_pair_path(self.h)]
 
     def autotune(self, n_trial=512, seed=12345):
         """Times batch sizes, PME paths and graph replay on the current frame;

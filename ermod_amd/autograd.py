@@ -39,6 +39,9 @@ class FrameAveragedEnergy(torch.autograd.Function):
             if isinstance(pos, torch.Tensor) and pos.is_cuda:
                 if pos.dtype != torch.float64 or not pos.is_contiguous():
                     raise ValueError("device coordinates must be contiguous float64")
+                # the library copies the coordinates on its own stream: wait
+                # for whatever torch stream produced them first
+                torch.cuda.current_stream(pos.device).synchronize()
                 engine.set_frame_device(pos.data_ptr(), cell, weight)
             else:
                 engine.set_frame(np.asarray(pos, dtype=np.float64), cell, weight)

@@ -594,7 +594,8 @@ template <typename T> int frame_prep(erm_ctx *c) {
                      dp<int>(c->d_fill), (int *)nullptr, 0, 0);
   TRY(launch_check(c, "k_scan"));
   hipLaunchKernelGGL(k_mol_scatter, dim3(gb), dim3(tb), 0, c->st, k, dp<int>(c->d_cellid), dp<int>(c->d_fill),
-                     dp<int>(c->d_tagpt), dp<int>(c->d_mol_begin), dp<int>(c->d_mol_k), dp<int>(c->d_mol_ns));
+                     dp<int>(c->d_tagpt), dp<int>(c->d_mol_begin), dp<int>(c->d_mol_k), dp<int>(c->d_mol_ns),
+                     dp<int>(c->d_jof));
   TRY(launch_check(c, "k_mol_scatter"));
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_mol_ns), dp<int>(c->d_mol_first), S,
                      (int *)nullptr, (int *)nullptr, 0, 0);
@@ -614,6 +615,7 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(ensure(c, c->d_zfill, sizeof(int) * (nbk + 1)));
   TRY(ensure(c, c->d_zstart, sizeof(int) * (K3 + 1)));
   TRY(ensure(c, c->d_zsite, sizeof(int) * nsites));
+  TRY(ensure(c, c->d_zpos, sizeof(int) * nsites));
   TRY(ensure(c, c->d_zg, sizeof(int) * nsites));
   TRY(ensure(c, c->d_zq, sizeof(T) * nsites));
   TRY(ensure(c, c->d_zw, sizeof(T) * (size_t)nsites * 3 * k.p));
@@ -625,7 +627,7 @@ template <typename T> int frame_prep(erm_ctx *c) {
                      dp<int>(c->d_zfill), dp<int>(c->d_zstart), K3, KB);
   TRY(launch_check(c, "k_scan"));
   hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, k.p, dp<int>(c->d_sg),
-                     dp<T>(c->d_sq), dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zg),
+                     dp<T>(c->d_sq), dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zpos), dp<int>(c->d_zg),
                      dp<T>(c->d_zq), dp<T>(c->d_zw));
   TRY(launch_check(c, "k_zscatter"));
   c->zst.assign(K3 + 1, 0);
@@ -703,6 +705,55 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
   return ERM_OK;
 }
 
+// real-space pairs of a batch: the tiled kernel with compacted pair lists,
+// or the kernel with one thread per candidate molecule (pair_path 1, or
+// when the tiles' LDS does not fit)
+template <typename T> int launch_pair(erm_ctx *c, const Params &k, int n, bool bare) {
+  const int S = c->slvmax, ns = c->ns;
+  const double Rg = k.rcmax + c->slt_radius + c->mol_radius + 0.5 * (k.hcell[0] + k.hcell[1] + k.hcell[2]) / 3.0;
+  const double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
+  const int segcap = 2 * k.nb[1] * k.nb[2];
+  const int nwave = NTHR / 64;
+  const size_t lt = pair_tile_lds<T>(ns, c->nvt, segcap, nwave);
+  if (c->pair_path == 0 && lt <= c->lds_cap) {
+    // one wave per chunk of 64 candidate molecules
+    const int nchunk = (int)std::ceil(frac * S / 64.0);
+    const int nsplit = std::max(1, std::min(64, (nchunk + nwave - 1) / nwave));
+    if (bare) {
+      TRY(set_lds(c, (const void *)k_pair_tile<T, true>, lt, "k_pair_tile"));
+      hipLaunchKernelGGL((k_pair_tile<T, true>), dim3(n, nsplit), dim3(NTHR), lt, c->st, k, dp<double>(c->d_spos),
+                         dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
+                         dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                         dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+    } else {
+      TRY(set_lds(c, (const void *)k_pair_tile<T, false>, lt, "k_pair_tile"));
+      hipLaunchKernelGGL((k_pair_tile<T, false>), dim3(n, nsplit), dim3(NTHR), lt, c->st, k, dp<double>(c->d_spos),
+                         dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
+                         dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                         dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+    }
+    return launch_check(c, "k_pair_tile");
+  }
+  const int nsplit = std::max(1, std::min(16, (int)std::ceil(frac * S / (2.0 * NTHR))));
+  const size_t lpair = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * c->nvt) + sizeof(int) * (2 * MAXSEG + 2) + 64;
+  TRY(set_lds(c, bare ? (const void *)k_pair<T, true> : (const void *)k_pair<T, false>, lpair, "k_pair"));
+  if (bare)
+    hipLaunchKernelGGL((k_pair<T, true>), dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
+                       dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
+                       dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                       dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                       dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+  else
+    hipLaunchKernelGGL((k_pair<T, false>), dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
+                       dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
+                       dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
+                       dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
+                       dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+  return launch_check(c, "k_pair");
+}
+
 // Energies of a batch of configurations into d_uv (sorted molecule order);
 // flags select the terms.  Stage events ev[0..6].
 template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
@@ -769,7 +820,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     Params kg = k;
     if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
     hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
-                       dp<double>(c->d_esite), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
+                       dp<double>(c->d_esite), dp<int>(c->d_zpos), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
                        dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
     TRY(launch_check(c, "k_mol_sum"));
   } else if (flags & DO_RECIP) {
@@ -799,7 +850,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
       TRY(launch_check(c, "k_pme_slab"));
       TRY(mark(c, 4));
       hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
-                         dp<double>(c->d_esite), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
+                         dp<double>(c->d_esite), dp<int>(c->d_zpos), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
                          dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
       TRY(launch_check(c, "k_mol_sum"));
     } else {
@@ -834,23 +885,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                        dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
     TRY(launch_check(c, "k_pair_open"));
   } else if (flags & DO_REAL) {
-    double Rg = k.rcmax + 8.0;
-    double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
-    int nsplit = std::max(1, std::min(16, (int)std::ceil(frac * S / (2.0 * NTHR))));
-    size_t lpair = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * c->nvt) + sizeof(int) * (2 * MAXSEG + 2) + 64;
-    if (bare)
-      hipLaunchKernelGGL((k_pair<T, true>), dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
-                         dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
-                         dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
-                         dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
-                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
-    else
-      hipLaunchKernelGGL((k_pair<T, false>), dim3(n, nsplit), dim3(NTHR), lpair, c->st, k, dp<double>(c->d_spos),
-                         dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
-                         dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
-                         dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
-                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
-    TRY(launch_check(c, "k_pair"));
+    TRY(launch_pair<T>(c, k, n, bare));
   }
   TRY(mark(c, 6));
   if (flags & DO_SELF) {
@@ -880,56 +915,80 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
   return ERM_OK;
 }
 
-// update_histogram for the n configurations in d_uv (weights in d_wsol)
-int run_histogram(erm_ctx *c, int n, int first) {
-  Params k = c->kp;
-  const int E = c->ermax;
-  TRY(ensure(c, c->d_hist, sizeof(double) * (size_t)n * E));
+// device buffers of update_histogram for a batch of n configurations: the
+// energies (sorted molecule order), the solutes' sorted indices, the
+// configurations' weights in and out, their histograms, per-species sums,
+// intermediate-state weights and the error status
+struct HistBufs {
+  double *uv, *wsol, *hist, *w, *flceng, *wi;
+  int *sltj, *status;
+};
+
+// the context's own buffers (grown as needed)
+int ctx_hist_bufs(erm_ctx *c, int n, HistBufs &hb) {
+  TRY(ensure(c, c->d_hist, sizeof(double) * (size_t)n * c->ermax));
   TRY(ensure(c, c->d_w, sizeof(double) * n));
   TRY(ensure(c, c->d_flceng, sizeof(double) * (size_t)n * MAXSPEC));
+  if (c->kp.intm) TRY(ensure(c, c->d_wi, sizeof(double) * n));
+  hb.uv = dp<double>(c->d_uv);
+  hb.wsol = dp<double>(c->d_wsol);
+  hb.hist = dp<double>(c->d_hist);
+  hb.w = dp<double>(c->d_w);
+  hb.flceng = dp<double>(c->d_flceng);
+  hb.wi = dp<double>(c->d_wi);
+  hb.sltj = dp<int>(c->d_sltj);
+  hb.status = dp<int>(c->d_status);
+  return ERM_OK;
+}
+
+// update_histogram for the n configurations of hb (engproc.F90:758-856)
+int run_histogram_at(erm_ctx *c, int n, int first, const HistBufs &hb) {
+  Params k = c->kp;
+  const int E = c->ermax;
   size_t lb = (size_t)E * (sizeof(double) + sizeof(int));
-  hipLaunchKernelGGL(k_bin, dim3(n), dim3(NTHR), lb, c->st, k, dp<double>(c->d_uv), dp<double>(c->d_edges),
+  hipLaunchKernelGGL(k_bin, dim3(n), dim3(NTHR), lb, c->st, k, hb.uv, dp<double>(c->d_edges),
                      dp<int>(c->d_specoff), dp<int>(c->d_uvmax), dp<int>(c->d_uvsoft), dp<int>(c->d_mol_spec),
-                     dp<int>(c->d_sltj), dp<double>(c->d_hist), dp<double>(c->d_flceng),
-                     dp<unsigned long long>(c->d_mnmx), dp<int>(c->d_status), first);
+                     hb.sltj, hb.hist, hb.flceng, dp<unsigned long long>(c->d_mnmx), hb.status, first);
   TRY(launch_check(c, "k_bin"));
-  hipLaunchKernelGGL(k_scalars, dim3(1), dim3(NTHR), 0, c->st, k, n, dp<double>(c->d_uv), dp<double>(c->d_wsol),
-                     dp<double>(c->d_escrd), dp<double>(c->d_scal), dp<double>(c->d_eself),
-                     dp<unsigned long long>(c->d_mnmx), dp<double>(c->d_w), dp<int>(c->d_status), first);
+  hipLaunchKernelGGL(k_scalars, dim3(1), dim3(NTHR), 0, c->st, k, n, hb.uv, hb.wsol, dp<double>(c->d_escrd),
+                     dp<double>(c->d_scal), dp<double>(c->d_eself), dp<unsigned long long>(c->d_mnmx), hb.w,
+                     hb.status, first);
   TRY(launch_check(c, "k_scalars"));
-  hipLaunchKernelGGL(k_edens, dim3((E + 31) / 32), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
-                     dp<double>(c->d_w), dp<double>(c->d_edens));
+  hipLaunchKernelGGL(k_edens, dim3((E + 31) / 32), dim3(256), 0, c->st, n, E, hb.hist, hb.w, dp<double>(c->d_edens));
   TRY(launch_check(c, "k_edens"));
   if (c->P.slttype == 1) {
-    hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, dp<double>(c->d_flceng),
-                       dp<double>(c->d_w), dp<double>(c->d_slnuv));
+    hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, hb.flceng, hb.w,
+                       dp<double>(c->d_slnuv));
     TRY(launch_check(c, "k_slnuv"));
   }
   if (k.intm) {
-    TRY(ensure(c, c->d_wi, sizeof(double) * n));
-    hipLaunchKernelGGL(k_intm_weight, dim3(n), dim3(256), 0, c->st, E, dp<double>(c->d_hist), dp<int>(c->d_zerodns),
-                       dp<double>(c->d_w), dp<double>(c->d_wi));
+    hipLaunchKernelGGL(k_intm_weight, dim3(n), dim3(256), 0, c->st, E, hb.hist, dp<int>(c->d_zerodns), hb.w, hb.wi);
     TRY(launch_check(c, "k_intm_weight"));
-    hipLaunchKernelGGL(k_edens, dim3((E + 31) / 32), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
-                       dp<double>(c->d_wi), dp<double>(c->d_edintm));
+    hipLaunchKernelGGL(k_edens, dim3((E + 31) / 32), dim3(256), 0, c->st, n, E, hb.hist, hb.wi,
+                       dp<double>(c->d_edintm));
     TRY(launch_check(c, "k_edens"));
-    hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, dp<double>(c->d_flceng),
-                       dp<double>(c->d_wi), dp<double>(c->d_uvintm));
+    hipLaunchKernelGGL(k_slnuv, dim3(1), dim3(MAXSPEC), 0, c->st, n, c->numslv, hb.flceng, hb.wi,
+                       dp<double>(c->d_uvintm));
     TRY(launch_check(c, "k_slnuv"));
-    hipLaunchKernelGGL(k_intm_scal, dim3(1), dim3(1), 0, c->st, n, dp<double>(c->d_w), dp<double>(c->d_wi),
-                       dp<double>(c->d_intm_sc));
+    hipLaunchKernelGGL(k_intm_scal, dim3(1), dim3(1), 0, c->st, n, hb.w, hb.wi, dp<double>(c->d_intm_sc));
     TRY(launch_check(c, "k_intm_scal"));
   }
   TRY(mark(c, 8));
   if (c->P.corrcal) {
     int nt = (E + 15) / 16;
     int tiles = nt * (nt + 1) / 2;  // tiles on and above the diagonal, one workgroup each
-    hipLaunchKernelGGL(k_corr, dim3(tiles), dim3(256), 0, c->st, n, E, dp<double>(c->d_hist),
-                       dp<double>(c->d_w), dp<double>(c->d_ecorr));
+    hipLaunchKernelGGL(k_corr, dim3(tiles), dim3(256), 0, c->st, n, E, hb.hist, hb.w, dp<double>(c->d_ecorr));
     TRY(launch_check(c, "k_corr"));
   }
   TRY(mark(c, 9));
   return ERM_OK;
+}
+
+// update_histogram for the n configurations in d_uv (weights in d_wsol)
+int run_histogram(erm_ctx *c, int n, int first) {
+  HistBufs hb;
+  TRY(ctx_hist_bufs(c, n, hb));
+  return run_histogram_at(c, n, first, hb);
 }
 
 int copy_out_uv(erm_ctx *c, int n, double *out) {
@@ -1568,9 +1627,23 @@ int32_t erm_set_topology(erm_ctx *c, int32_t natom, int32_t nmol, const int32_t 
   c->kp.nvt = c->nvt;
   c->bfcoord.clear();
   if (bfcoord) c->bfcoord.assign(bfcoord, bfcoord + 3 * c->ns);
+  // test-particle radius about its centroid (sizes the pair kernel's grid);
+  // solution-mode solutes keep the default
+  if (bfcoord && c->ns > 0) {
+    double cm[3] = {0, 0, 0}, r2 = 0.0;
+    for (int s = 0; s < c->ns; ++s)
+      for (int d = 0; d < 3; ++d) cm[d] += bfcoord[3 * s + d] / c->ns;
+    for (int s = 0; s < c->ns; ++s) {
+      double q = 0.0;
+      for (int d = 0; d < 3; ++d) q += (bfcoord[3 * s + d] - cm[d]) * (bfcoord[3 * s + d] - cm[d]);
+      r2 = std::max(r2, q);
+    }
+    c->slt_radius = std::sqrt(r2);
+  }
   TRY(upload(c, c->d_mol_begin, mol_begin, nmol + 1));
   TRY(upload(c, c->d_charge, charge, natom));
   TRY(upload(c, c->d_tagpt, c->tagpt.data(), c->slvmax));
+  TRY(upload(c, c->d_kof, c->kof.data(), c->nmol));
   TRY(upload(c, c->d_cvt, cvt.data(), natom));
   TRY(upload(c, c->d_mol_charge, mol_charge, nmol));
   TRY(upload(c, c->d_uvspec, uvspec, nmol));
@@ -2152,6 +2225,15 @@ int32_t erm_set_tuning(erm_ctx *c, int32_t max_batch, int32_t pme_path, int32_t 
   return ERM_OK;
 }
 
+int32_t erm_set_pair_path(erm_ctx *c, int32_t path) {
+  if (!c) return ERM_E_ARG;
+  if (path < 0 || path > 1) return fail(c, ERM_E_ARG, "pair path must be 0 (tiles) or 1 (per molecule)");
+  c->pair_path = path;
+  return ERM_OK;
+}
+
+int32_t erm_get_pair_path(const erm_ctx *c) { return c ? c->pair_path : -1; }
+
 int32_t erm_get_tuning(const erm_ctx *c, int32_t *out10) {
   if (!c || !out10) return ERM_E_ARG;
   int32_t *out8 = out10, *out9 = out10;
@@ -2257,6 +2339,21 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
       }
       c->xslab_yf = (besty >= 0) ? (besty == 1) : y0;
     }
+  }
+  // real-space kernel: compacted tiles or one thread per candidate molecule
+  {
+    const int p0 = c->pair_path;
+    int bestp = -1;
+    double ms;
+    for (int pp : {0, 1}) {
+      c->pair_path = pp;
+      TRY(timed(&ms));
+      if (ms < best) {
+        best = ms;
+        bestp = pp;
+      }
+    }
+    c->pair_path = (bestp >= 0) ? bestp : p0;
   }
   // graph replay against eager launches for the chosen shape
   c->use_graph = true;

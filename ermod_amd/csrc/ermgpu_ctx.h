@@ -74,9 +74,9 @@ struct erm_ctx {
   DBuf d_mol_begin, d_charge, d_tagpt, d_cvt, d_mol_charge, d_uvspec, d_sitemass, d_bf, d_qslt, d_qslt_t, d_sltlj,
       d_sltlj64;
   DBuf d_pos, d_cellid, d_count, d_cell_start, d_fill, d_rm, d_mol_k, d_mol_ns, d_mol_first, d_mol_q,
-      d_mol_spec, d_mol_m, d_jof;
+      d_mol_spec, d_mol_m, d_jof, d_kof;
   DBuf d_sx, d_sy, d_sz, d_sq, d_sct, d_sg, d_sw;
-  DBuf d_zcount, d_zstart, d_zfill, d_zfine, d_zsite, d_zg, d_zq, d_zw, d_esite;
+  DBuf d_zcount, d_zstart, d_zfill, d_zfine, d_zsite, d_zpos, d_zg, d_zq, d_zw, d_esite;
   int nsites = 0;             // sites of the physical molecules
   int slab_S = 0, slab_R = 0; // fused PME slab: planes per workgroup, registers per thread
   DBuf d_tw1, d_tw2, d_tw3, d_engfac;
@@ -102,6 +102,9 @@ struct erm_ctx {
   int xslab_T = erm::NTHR;           // threads per x-slab workgroup, 256 or 512 (autotuned)
   int zcol_T = erm::NTHR;            // threads per z-column workgroup, 256 or 512 (autotuned)
   bool xslab_yf = false;        // inverse y transform inside the x slab (autotuned)
+  int pair_path = 0;            // 0 tiled pair kernel, 1 one thread per candidate molecule
+  double slt_radius = 8.0;      // test particle's largest site distance from its centroid
+  double mol_radius = 2.0;      // solvent molecules' largest site distance from the first site
   std::vector<int> zst;         // first z-sorted site of every plane (host copy)
   size_t lds_cap = 150 * 1024; // dynamic LDS a kernel may request (160 KB per CU)
   // insertion modes: host molecules of the aggregate centre, and the centre
@@ -125,8 +128,8 @@ struct erm_ctx {
   std::vector<DBuf *> all() {
     return {&d_mol_begin, &d_charge, &d_tagpt, &d_cvt, &d_mol_charge, &d_uvspec, &d_sitemass, &d_bf, &d_qslt, &d_qslt_t,
             &d_sltlj, &d_sltlj64, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
-            &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
-            &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zfine, &d_zsite, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
+            &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_kof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
+            &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zfine, &d_zsite, &d_zpos, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
             &d_w3h, &d_args, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
             &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,
             &d_uvout, &d_hist, &d_wsol, &d_w, &d_flceng, &d_sltj, &d_selfreal, &d_usreal, &d_edens, &d_ecorr,
@@ -218,10 +221,11 @@ inline const char *status_name(int code) {
 }
 
 // first error raised by the device during the call (configuration index
-// within the call); synchronises the stream
-inline int check_status(erm_ctx *c) {
+// within the call), from the status words at d_status (the context's own
+// by default); synchronises the stream
+inline int check_status(erm_ctx *c, const int *d_status = nullptr) {
   int h[8];
-  HIPCHK(c, hipMemcpyAsync(h, c->d_status.p, sizeof h, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(h, d_status ? d_status : c->d_status.p, sizeof h, hipMemcpyDeviceToHost, c->st));
   TRY(sync(c));
   if (h[0]) {
     double v;

@@ -138,12 +138,13 @@ static __global__ void k_frame_clear(int *count, int nc, int *zcount, int nz, un
 
 static __global__ void k_mol_scatter(Params P, const int *__restrict__ cellid, int *fill,
                               const int *__restrict__ tagpt, const int *__restrict__ mol_begin,
-                              int *mol_k, int *mol_ns) {
+                              int *mol_k, int *mol_ns, int *jof) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.slvmax) return;
   int j = atomicAdd(&fill[cellid[k]], 1);
   int m = tagpt[k];
   mol_k[j] = k;
+  jof[k] = j;
   mol_ns[j] = mol_begin[m + 1] - mol_begin[m];
 }
 
@@ -757,7 +758,7 @@ __global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
 #pragma unroll
   for (int r = 0; r < MAXR; ++r) {
     int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
-    if (idx < s_end) e_site[(size_t)b * np + zsite[idx]] = acc[r];
+    if (idx < s_end) e_site[(size_t)b * np + idx] = acc[r];
   }
 }
 
@@ -1283,16 +1284,19 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
 #pragma unroll
   for (int r = 0; r < MAXR; ++r) {
     int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
-    if (idx < s_end) e_site[(size_t)b * np + zsite[idx]] = acc[r];
+    if (idx < s_end) e_site[(size_t)b * np + idx] = acc[r];
   }
 }
 
-// per-molecule reciprocal energy from the per-site sums, plus the
+// per-molecule reciprocal energy from the per-site sums (stored in the
+// z-sorted site order; zpos maps a cell-sorted site to it), plus the
 // neutralising background (residual_ene, engproc.F90:859-869):
-// uv[b][1 + j] = -pi Q_s Q_j / (alpha^2 V) + sum over the sites of j
+// uv[b][1 + j] = -pi Q_s Q_j / (alpha^2 V) + sum over the sites of j, in
+// site order
 static __global__ void k_mol_sum(Params P, int nsites, const double *__restrict__ e_site,
-                          const int *__restrict__ mol_first, const int *__restrict__ mol_ns,
-                          const double *__restrict__ mol_q, const int *__restrict__ slt_j, double *uv) {
+                                 const int *__restrict__ zpos, const int *__restrict__ mol_first,
+                                 const int *__restrict__ mol_ns, const double *__restrict__ mol_q,
+                                 const int *__restrict__ slt_j, double *uv) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (j >= P.slvmax) return;
@@ -1303,7 +1307,7 @@ static __global__ void k_mol_sum(Params P, int nsites, const double *__restrict_
   }
   const double *es = e_site + (size_t)b * nsites;
   double e = 0.0;
-  for (int t = mol_first[j]; t < mol_first[j] + mol_ns[j]; ++t) e += es[t];
+  for (int t = mol_first[j]; t < mol_first[j] + mol_ns[j]; ++t) e += es[zpos[t]];
   double epcl = 3.1415926535897932 * P.solute_charge * mol_q[j] / P.screen / P.screen / P.volume;
   u[j] = (0.0 - epcl) + e;
 }
@@ -1337,7 +1341,7 @@ static __global__ __launch_bounds__(256) void k_zcount(int nsites, int nbk, int 
 template <typename T>
 __global__ __launch_bounds__(256) void k_zscatter(int nsites, int nbk, int KB, int p, const int *__restrict__ sg,
                                                   const T *__restrict__ sq, const T *__restrict__ sw, int *zfill,
-                                                  int *zsite, int *zg, T *zq, T *zw) {
+                                                  int *zsite, int *zpos, int *zg, T *zq, T *zw) {
   __shared__ int cnt[ZBUCKET_MAX], base[ZBUCKET_MAX];
   for (int z = threadIdx.x; z < nbk; z += blockDim.x) cnt[z] = 0;
   __syncthreads();
@@ -1350,6 +1354,7 @@ __global__ __launch_bounds__(256) void k_zscatter(int nsites, int nbk, int KB, i
   if (t >= nsites) return;
   const int zt = base[g] + rank;
   zsite[zt] = t;
+  zpos[t] = zt;
   zg[zt] = sg[t];
   zq[zt] = sq[t];
   for (int k = 0; k < 3 * p; ++k) zw[(size_t)k * nsites + zt] = sw[(size_t)t * 3 * p + k];
@@ -1609,6 +1614,16 @@ static __device__ int pillar_segments(const Params &P, const double Cw[3], doubl
   return nsg;
 }
 
+// FP32 mode evaluates a pair in FP64 (from the FP64 displacement and
+// parameters) when its energy is large enough for FP32 rounding to matter at
+// 1e-4 kcal/mol: a repulsive LJ overlap (r < 2^(1/6) sigma, where r^-12
+// amplifies the rounding of r) or a Coulomb term |q q| / r above COUL_HOT
+// kcal/mol (first-shell pairs of charged sites).  Such pairs are few.
+constexpr float COUL_HOT = 24.0f;
+template <typename T> __device__ __forceinline__ bool hot_pair(T eps, T sig2, T qq, T r2) {
+  return (eps > (T)0 && sig2 > (T)1.2599211 * r2) || (qq * qq > (T)(COUL_HOT * COUL_HOT) * r2);
+}
+
 // BARE: bare Coulomb q q / r below elecut and the LJ of realcal_bare
 // (realcal.F90:155-281) in place of the screened real-space pair
 template <typename T, bool BARE = false>
@@ -1695,10 +1710,9 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
           T ep = 0;
           T eps = ulj[(s * nvt + cv) * 2], sig2 = ulj[(s * nvt + cv) * 2 + 1];
           if constexpr (sizeof(T) == 4) {
-            // r < 2^(-1/6) sigma (a repulsive overlap, where the r^-12 term
-            // amplifies FP32 rounding): the pair is recomputed in FP64 from
-            // the FP64 displacement and parameters
-            if (eps > (T)0 && sig2 > (T)1.2599211 * r2) {
+            // a repulsive overlap or a large Coulomb term: the pair in FP64
+            // from the FP64 displacement and parameters
+            if (hot_pair<T>(eps, sig2, uq[s] * qv, r2)) {
               const double ox = (double)ux[s] + (double)ulo[3 * s];
               const double oy = (double)uy[s] + (double)ulo[3 * s + 1];
               const double oz = (double)uz[s] + (double)ulo[3 * s + 2];
@@ -1743,6 +1757,324 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
       }
     }
     u[j] += e;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Real space in tiles with compacted pair lists (get_pair_energy_block,
+// realcal.F90:585-784, rebuilt for 64-wide waves).  A workgroup takes one
+// configuration; its waves take chunks of 64 candidate molecules (the
+// pillar candidates of k_pair, chunk c to wave c mod #waves), one molecule
+// per lane.  The chunk's sites are walked in tiles of 64 (one site per lane,
+// SoA loads, the FP64 displacement from the configuration centre and its
+// wrap formed once per site).  For every group of PT_GRP solute sites the
+// lanes test their site against each solute site and append the pairs
+// inside the cutoff to a wave-local queue (ballot + mbcnt, no atomics); the
+// queued pairs are then evaluated with every lane busy, and each site adds
+// its pairs' energies in solute-site order.  Per-site sums are reduced per
+// molecule in site order by the lane that owns the molecule, so every sum
+// is deterministic.  A per-pair minimum image is only formed when the reach
+// rc + R_solute is at least half the box (otherwise the site's own wrap is
+// already the minimum image of every pair within the cutoff).
+// --------------------------------------------------------------------------
+constexpr int PT_GRP = 8;                 // solute sites per compaction group
+constexpr int PT_QCAP = 64 * PT_GRP;      // queued pairs per wave (at most)
+
+template <typename T> __host__ __device__ inline size_t pair_tile_wave_lds() {
+  // tile: hi[3][64], lo[3][64], q[64] (T), type[64]; molecule offsets [65],
+  // first sites [64]; queue [PT_QCAP] (int) and energies [PT_QCAP] (f64);
+  // per-site sums [64] (f64); rounded to 16 bytes
+  return ((size_t)7 * 64 * sizeof(T) + 64 * 4 + 65 * 4 + 64 * 4 + (size_t)PT_QCAP * (4 + 8) + 64 * 8 + 16 + 15) &
+         ~(size_t)15;
+}
+// solute data, then the pillar segments (segcap = 2 nb1 nb2 slots), then the
+// waves' tiles and queues
+template <typename T> __host__ __device__ inline size_t pair_tile_head(int ns, int nvt, int segcap) {
+  const size_t a = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * nvt) + sizeof(int) * (2 * (size_t)segcap + 2) + 64;
+  return (a + 15) & ~(size_t)15;
+}
+template <typename T> __host__ __device__ inline size_t pair_tile_lds(int ns, int nvt, int segcap, int nwave) {
+  return pair_tile_head<T>(ns, nvt, segcap) + (size_t)nwave * pair_tile_wave_lds<T>();
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+template <typename T, bool BARE = false>
+__global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__restrict__ spos,
+                                                    const double *__restrict__ sref,
+                                                    const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
+                                                    const int *__restrict__ cell_start,
+                                                    const int *__restrict__ mol_first,
+                                                    const int *__restrict__ mol_ns, const double *__restrict__ sx,
+                                                    const double *__restrict__ sy, const double *__restrict__ sz,
+                                                    const T *__restrict__ sq, const int *__restrict__ sct,
+                                                    const int *__restrict__ slt_j,
+                                                    const double *__restrict__ rm_ord_p,
+                                                    const double *__restrict__ slt_lj64,
+                                                    const double *__restrict__ slt_q64, double *uv) {
+  constexpr bool F32 = sizeof(T) == 4;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int ns = P.ns, nvt = P.nvt;
+  T *ux = reinterpret_cast<T *>(lds_raw);
+  T *uy = ux + ns;
+  T *uz = uy + ns;
+  T *uq = uz + ns;
+  T *ulj = uq + ns;                       // [ns][nvt][2] (eps, sig2)
+  T *ulo = ulj + 2 * ns * nvt;            // [ns][3] low parts of the offsets
+  const int segcap = 2 * P.nb[1] * P.nb[2];
+  int *seg_beg = reinterpret_cast<int *>(ulo + 3 * ns);
+  int *seg_pre = seg_beg + segcap;
+  const size_t base = pair_tile_head<T>(ns, nvt, segcap);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned char *wl = lds_raw + base + (size_t)wv * pair_tile_wave_lds<T>();
+  T *th = reinterpret_cast<T *>(wl);      // [3][64] high parts
+  T *tl = th + 3 * 64;                    // [3][64] low parts
+  T *tq = tl + 3 * 64;                    // [64]
+  int *tt = reinterpret_cast<int *>(tq + 64);
+  int *woff = tt + 64;                    // [65] first chunk site of molecule lane
+  int *wfirst = woff + 65;                // [64] first sorted site of molecule lane
+  int *wq = wfirst + 64;                  // [PT_QCAP] queued pairs: solute site << 6 | tile lane
+  double *weq = reinterpret_cast<double *>(reinterpret_cast<uintptr_t>(wq + PT_QCAP + 1) & ~(uintptr_t)7);
+  double *wes = weq + PT_QCAP;            // [64] per-site sums of a tile
+
+  const int b = blockIdx.x;
+  const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
+  const double Rs = sref[4 * b + 3];
+  const double *xs = spos + (size_t)b * ns * 3;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    ux[s] = (T)(xs[3 * s] - C0);
+    uy[s] = (T)(xs[3 * s + 1] - C1);
+    uz[s] = (T)(xs[3 * s + 2] - C2v);
+    uq[s] = slt_q[s];
+    ulo[3 * s] = (T)((xs[3 * s] - C0) - (double)ux[s]);
+    ulo[3 * s + 1] = (T)((xs[3 * s + 1] - C1) - (double)uy[s]);
+    ulo[3 * s + 2] = (T)((xs[3 * s + 2] - C2v) - (double)uz[s]);
+  }
+  for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
+  const double Rm = ord_to_dbl(*(const unsigned long long *)rm_ord_p);
+  const double R = P.rcmax + Rs + Rm + 1e-6 * (1.0 + P.rcmax);
+  const double Cw[3] = {wrap1(C0, P.L[0]), wrap1(C1, P.L[1]), wrap1(C2v, P.L[2])};
+  const int nsg = pillar_segments(P, Cw, R, cell_start, seg_beg, seg_pre);
+  const int total = seg_pre[nsg];
+  const int sj = slt_j[b];
+  // sites farther than rc + Rs from the centre meet no solute site within
+  // the cutoff; the minimum image of a pair is the site's own wrap unless
+  // the reach rc + Rs reaches half the box
+  const double reach = P.rcmax + Rs + 1e-6 * (1.0 + P.rcmax), reach2 = reach * reach;
+  const bool img = reach >= 0.5 * fmin(P.L[0], fmin(P.L[1], P.L[2]));
+  const double iL0 = 1.0 / P.L[0], iL1 = 1.0 / P.L[1], iL2 = 1.0 / P.L[2];
+  const T L0 = (T)P.L[0], L1 = (T)P.L[1], L2 = (T)P.L[2];
+  const T h0 = (T)P.halfL[0], h1 = (T)P.halfL[1], h2 = (T)P.halfL[2];
+  const T lw2 = (T)P.lw2, up2 = (T)P.up2, el2 = (T)P.el2, cut2 = (T)P.cut2, alpha = (T)P.screen;
+  // the queue test on the high parts (FP32) keeps a margin; the exact test
+  // on the full displacement decides
+  const T cut2q = F32 ? (T)(P.cut2 * (1.0 + 1e-5)) : cut2;
+  double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
+  const int nchunk = (total + 63) >> 6;
+  for (int c = blockIdx.y * nw + wv; c < nchunk; c += gridDim.y * nw) {
+    // molecule of this lane: flattened candidate c*64 + lane
+    const int idx = c * 64 + lane;
+    int j = -1, nsm = 0, first = 0;
+    if (idx < total) {
+      int lo_s = 0, hi_s = nsg;  // last segment with seg_pre <= idx
+      while (hi_s - lo_s > 1) {
+        const int mid = (lo_s + hi_s) >> 1;
+        if (seg_pre[mid] <= idx)
+          lo_s = mid;
+        else
+          hi_s = mid;
+      }
+      j = seg_beg[lo_s] + (idx - seg_pre[lo_s]);
+      if (j != sj) {
+        nsm = mol_ns[j];
+        first = mol_first[j];
+      }
+    }
+    // exclusive scan of the site counts over the wave
+    int off = nsm;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(off, d, 64);
+      if (lane >= d) off += v;
+    }
+    const int tot_sites = __shfl(off, 63, 64);
+    off -= nsm;
+    woff[lane] = off;
+    if (lane == 63) woff[64] = tot_sites;
+    wfirst[lane] = first;
+    wave_lds_sync();
+    double acc = 0.0;  // energy of molecule `lane`
+    for (int t0 = 0; t0 < tot_sites; t0 += 64) {
+      const int cs = t0 + lane;
+      bool act = cs < tot_sites;
+      T hx = 0, hy = 0, hz = 0, lx = 0, ly = 0, lz = 0, qv = 0;
+      int cv = 0;
+      if (act) {
+        int a = 0, e = 64;  // molecule k of chunk site cs: last k with woff[k] <= cs
+        while (e - a > 1) {
+          const int mid = (a + e) >> 1;
+          if (woff[mid] <= cs)
+            a = mid;
+          else
+            e = mid;
+        }
+        const int t = wfirst[a] + (cs - woff[a]);
+        double dx = sx[t] - C0, dy = sy[t] - C1, dz = sz[t] - C2v;
+        dx -= P.L[0] * rint(dx * iL0);
+        dy -= P.L[1] * rint(dy * iL1);
+        dz -= P.L[2] * rint(dz * iL2);
+        act = (dx * dx + dy * dy + dz * dz) <= reach2;
+        hx = (T)dx;
+        hy = (T)dy;
+        hz = (T)dz;
+        if constexpr (F32) {
+          lx = (T)(dx - (double)hx);
+          ly = (T)(dy - (double)hy);
+          lz = (T)(dz - (double)hz);
+        }
+        qv = sq[t];
+        cv = sct[t];
+      }
+      th[lane] = hx;
+      th[64 + lane] = hy;
+      th[128 + lane] = hz;
+      tl[lane] = lx;
+      tl[64 + lane] = ly;
+      tl[128 + lane] = lz;
+      tq[lane] = qv;
+      tt[lane] = cv;
+      double e = 0.0;  // this site's energy, in solute-site order
+      for (int s0 = 0; s0 < ns; s0 += PT_GRP) {
+        int pend[PT_GRP];
+        int qn = 0;
+#pragma unroll
+        for (int g = 0; g < PT_GRP; ++g) {
+          const int s = s0 + g;
+          bool in = false;
+          if (s < ns && act) {
+            T ax = hx - ux[s], ay = hy - uy[s], az = hz - uz[s];
+            if (img) {
+              ax = fabs(ax);
+              ay = fabs(ay);
+              az = fabs(az);
+              ax = (ax > h0) ? L0 - ax : ax;
+              ay = (ay > h1) ? L1 - ay : ay;
+              az = (az > h2) ? L2 - az : az;
+            }
+            in = (ax * ax + ay * ay + az * az) <= cut2q;
+          }
+          const unsigned long long m = __ballot(in);
+          const int pos = qn + lanes_below(m);
+          pend[g] = in ? pos : -1;
+          if (in) wq[pos] = (s << 6) | lane;
+          qn += (int)__popcll(m);
+        }
+        wave_lds_sync();
+        // the queued pairs, every lane busy
+        for (int q0 = 0; q0 < qn; q0 += 64) {
+          const int qi = q0 + lane;
+          if (qi >= qn) break;
+          const int ent = wq[qi];
+          const int i = ent & 63, s = ent >> 6;
+          const T vx = th[i], vy = th[64 + i], vz = th[128 + i];
+          T ax, ay, az;
+          if constexpr (F32) {
+            ax = ((vx - ux[s]) + tl[i]) - ulo[3 * s];
+            ay = ((vy - uy[s]) + tl[64 + i]) - ulo[3 * s + 1];
+            az = ((vz - uz[s]) + tl[128 + i]) - ulo[3 * s + 2];
+          } else {
+            ax = vx - ux[s];
+            ay = vy - uy[s];
+            az = vz - uz[s];
+          }
+          ax = fabs(ax);
+          ay = fabs(ay);
+          az = fabs(az);
+          if (img) {
+            ax = (ax > h0) ? L0 - ax : ax;
+            ay = (ay > h1) ? L1 - ay : ay;
+            az = (az > h2) ? L2 - az : az;
+          }
+          const T r2 = ax * ax + ay * ay + az * az;
+          const T qsv = tq[i];
+          const int cvi = tt[i];
+          double epd = 0.0;
+          if (r2 <= cut2) {
+            const T eps = ulj[(s * nvt + cvi) * 2], sig2 = ulj[(s * nvt + cvi) * 2 + 1];
+            bool done = false;
+            if constexpr (F32) {
+              // a repulsive overlap or a large Coulomb term: the pair in FP64
+              if (hot_pair<T>(eps, sig2, uq[s] * qsv, r2)) {
+                double bx = ((double)vx + (double)tl[i]) - ((double)ux[s] + (double)ulo[3 * s]);
+                double by = ((double)vy + (double)tl[64 + i]) - ((double)uy[s] + (double)ulo[3 * s + 1]);
+                double bz = ((double)vz + (double)tl[128 + i]) - ((double)uz[s] + (double)ulo[3 * s + 2]);
+                bx = fabs(bx);
+                by = fabs(by);
+                bz = fabs(bz);
+                if (img) {
+                  bx = (bx > P.halfL[0]) ? P.L[0] - bx : bx;
+                  by = (by > P.halfL[1]) ? P.L[1] - by : by;
+                  bz = (bz > P.halfL[2]) ? P.L[2] - bz : bz;
+                }
+                const double r2d = bx * bx + by * by + bz * bz;
+                const double e64 = slt_lj64[(s * nvt + cvi) * 2], s64 = slt_lj64[(s * nvt + cvi) * 2 + 1];
+                if constexpr (BARE) {
+                  const double rd = sqrt(r2d);
+                  epd = lj_bare_t<double>(P, e64, s64, r2d, rd);
+                  if (rd < P.elecut) epd += slt_q64[s] * (double)qsv / rd;
+                } else {
+                  if (r2d <= P.lw2)
+                    epd = lj_low_t<double>(P, e64, s64, r2d);
+                  else if (r2d <= P.up2)
+                    epd = lj_switch_t<double>(P, e64, s64, r2d);
+                  if (r2d <= P.el2) epd += coul_t<double>(slt_q64[s] * (double)qsv, sqrt(r2d), P.screen);
+                }
+                done = true;
+              }
+            }
+            if (!done) {
+              T ep = 0;
+              if constexpr (BARE) {
+                const T rst = sqrt(r2);
+                ep = lj_bare_t<T>(P, eps, sig2, r2, rst);
+                if (rst < (T)P.elecut) ep += uq[s] * qsv / rst;
+              } else {
+                if (eps > (T)0) {
+                  if (r2 <= lw2)
+                    ep += lj_low_t<T>(P, eps, sig2, r2);
+                  else if (r2 <= up2)
+                    ep += lj_switch_t<T>(P, eps, sig2, r2);
+                }
+                if (r2 <= el2) ep += coul_t<T>(uq[s] * qsv, sqrt(r2), alpha);
+              }
+              epd = (double)ep;
+            }
+          }
+          weq[qi] = epd;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int g = 0; g < PT_GRP; ++g)
+          if (pend[g] >= 0) e += weq[pend[g]];
+        wave_lds_sync();
+      }
+      wes[lane] = act ? e : 0.0;
+      wave_lds_sync();
+      // molecule `lane` adds its sites of this tile in site order
+      if (nsm > 0) {
+        const int a = max(off, t0), z = min(off + nsm, t0 + 64);
+        for (int q = a; q < z; ++q) acc += wes[q - t0];
+      }
+      wave_lds_sync();
+    }
+    if (nsm > 0) u[j] += acc;
   }
 }
 
@@ -1801,7 +2133,7 @@ __global__ __launch_bounds__(NTHR) void k_pair_open(Params P, const double *__re
       const T r2 = ax * ax + ay * ay + az * az;
       const T eps = ulj[(s * nvt + cv) * 2], sig2 = ulj[(s * nvt + cv) * 2 + 1];
       if constexpr (sizeof(T) == 4) {
-        if (eps > (T)0 && sig2 > (T)1.2599211 * r2) {
+        if (hot_pair<T>(eps, sig2, uq[s] * qv, r2)) {
           const double bx = dx - ((double)ux[s] + (double)ulo[3 * s]);
           const double by = dy - ((double)uy[s] + (double)ulo[3 * s + 1]);
           const double bz = dz - ((double)uz[s] + (double)ulo[3 * s + 2]);
@@ -2335,6 +2667,29 @@ static __global__ __launch_bounds__(256) void k_corr(int n, int ermax, const dou
 }
 
 // uvengy in tagpt order from the sorted order
+// the inverse of k_unsort for energies given in tagpt order, and each
+// configuration's solute as a sorted index (kof: tagpt index of a molecule,
+// -1 for none; jof: sorted index of a tagpt index; solutes NULL: none)
+static __global__ void k_sort_uv(int n, int slvmax, const int *__restrict__ mol_k, const int *__restrict__ jof,
+                                 const int *__restrict__ kof, const int *__restrict__ solutes,
+                                 const double *__restrict__ in, double *uv, int *slt_j) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (b >= n) return;
+  const double *u = in + (size_t)b * (slvmax + 1);
+  double *o = uv + (size_t)b * (slvmax + 1);
+  if (j == 0) {
+    o[0] = u[0];
+    int sj = -1;
+    if (solutes) {
+      const int kk = kof[solutes[b]];
+      sj = (kk >= 0) ? jof[kk] : -1;
+    }
+    slt_j[b] = sj;
+  }
+  if (j < slvmax) o[1 + j] = u[1 + mol_k[j]];
+}
+
 static __global__ void k_unsort(int n, int slvmax, const int *__restrict__ mol_k, const double *__restrict__ uv,
                          double *out) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;

@@ -201,6 +201,13 @@ module ermgpu
                                     column_threads, fold_y
      end function erm_set_tuning
 
+     ! real-space kernel: 0 compacted tiles, 1 one thread per candidate molecule
+     integer(c_int32_t) function erm_set_pair_path(ctx, path) bind(C, name="erm_set_pair_path")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: path
+     end function erm_set_pair_path
+
      ! insorigin / insposition / insorient / lwreg / upreg / hostspec of
      ! instslt (insertion.F90:103-366), the reference's numbering
      integer(c_int32_t) function erm_set_insertion(ctx, insorigin, insposition, insorient, lwreg, upreg, &

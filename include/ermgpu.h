@@ -232,7 +232,10 @@ void erm_traj_close(erm_traj *t);
  * (test-particle slots ignored); cell may be NULL without a box. */
 int32_t erm_set_frame(erm_ctx *ctx, const double *sitepos, const double *cell,
                       double stat_weight_system);
-/* Same, with sitepos already in device memory (natom*3 doubles). */
+/* Same, with sitepos already in device memory (natom*3 doubles).  The copy is
+ * ordered on the context's stream only: the caller makes sure the
+ * coordinates are written (synchronises the stream that produced them)
+ * before the call. */
 int32_t erm_set_frame_device(erm_ctx *ctx, const double *d_sitepos, const double *cell,
                              double stat_weight_system);
 
@@ -328,9 +331,17 @@ int32_t erm_set_tuning(erm_ctx *ctx, int32_t max_batch, int32_t pme_path, int32_
  * planes in use, group planes in use, slab cap, group cap, z-column
  * threads (256 / 512), y transform inside the x slab (0 / 1)} */
 int32_t erm_get_tuning(const erm_ctx *ctx, int32_t *out10);
+/* Real-space kernel (no counterpart in the reference): 0 (default) walks the
+ * candidates in tiles of 64 solvent sites and evaluates the pairs inside the
+ * cutoff from a compacted per-wave list (realcal.F90:585-784 blocks the
+ * same loop for the CPU); 1 gives each candidate molecule to one thread.
+ * erm_autotune picks the faster.  erm_get_pair_path returns the setting. */
+int32_t erm_set_pair_path(erm_ctx *ctx, int32_t path);
+int32_t erm_get_pair_path(const erm_ctx *ctx);
 /* Times n_trial random insertions (seed) on the current frame for batch
- * sizes 128..1024, every applicable PME path and slab / group shapes, then
- * graph replay against eager launches, and keeps the fastest; accumulators
+ * sizes 128..1024, every applicable PME path and slab / group shapes, the
+ * real-space kernel, then graph replay against eager launches, and keeps
+ * the fastest; accumulators
  * are untouched.  best_ms (optional) receives the chosen time. */
 int32_t erm_autotune(erm_ctx *ctx, int32_t n_trial, uint64_t seed, double *best_ms);
 

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from bars import FP32_ABS, fp32_ok, fp32_report
+from bars import FP32_ABS, FP32_EMAX, fp32_ok, fp32_report
 from ermod_amd import synth
 from ermod_amd.api import ErmError, ErmGPU
 
@@ -297,6 +297,69 @@ def test_soln300_fp32_large_system():
     fp32_report("soln300 vs oracle", res["fp32"][1:], want[1:], o.uvcrd)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_register_path_tunings(refs, prec):
+    """Every shape the autotuner may pick for the register path -- x-slab
+    and z-column workgroup sizes, the inverse y transform inside the x slab,
+    slab depth and planes per group -- gives the oracle's reciprocal
+    energies."""
+    print_case(f"register-path tunings {prec}")
+    sy, pos = refs
+    o = O.Oracle(sy.system)
+    o.set_frame(pos, sy.cell)
+    P = poses(sy, 2, 46)
+    want = [o.recpcal(sy.solute_mol, sp) for sp in P]
+    seen = set()
+    with ErmGPU(sy.system, prec, pme_path="register") as g:
+        g.set_frame(pos, sy.cell)
+        for st in (256, 512):
+            for ct in (256, 512):
+                for fy in (False, True):
+                    for sp_ in (3, 8, 16):
+                        for gp in (4, 16):
+                            g.set_tuning(max_batch=0, pme_path="register", slab_planes=sp_, group_planes=gp,
+                                         slab_threads=st, column_threads=ct, fold_y=fy)
+                            t = g.tuning()
+                            seen.add((t["slab_threads"], t["column_threads"], t["fold_y"], t["slab_planes"],
+                                      t["group_planes"]))
+                            for sp, (we, ws) in zip(P, want):
+                                got, gs = g.recpcal(sy.solute_mol, sp)
+                                if prec == "fp64":
+                                    np.testing.assert_allclose(got, we, **TOL64)
+                                    assert gs == pytest.approx(ws, rel=1e-11, abs=1e-9)
+                                else:
+                                    ok, err = fp32_ok(got, we)
+                                    assert ok.all(), (t, err.max())
+    assert any(k[2] for k in seen), "the folded y transform was not exercised"
+    assert len({k[3] for k in seen}) >= 2
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("which", ["tiny", "refs20"])
+def test_pair_paths_agree(tiny, refs, which, prec):
+    """Both real-space kernels -- compacted tiles and one thread per
+    molecule -- give the oracle's real-space energies (tiny: a box small
+    enough that pairs need their own minimum image)."""
+    print_case(f"pair paths {which} {prec}")
+    sy, pos = tiny if which == "tiny" else refs
+    o = O.Oracle(sy.system)
+    o.set_frame(pos, sy.cell)
+    P = poses(sy, 4, 47)
+    want = [o.realcal(sy.solute_mol, sp) for sp in P]
+    with ErmGPU(sy.system, prec) as g:
+        g.set_frame(pos, sy.cell)
+        for path in ("tiles", "molecule"):
+            g.set_pair_path(path)
+            assert g.pair_path() == path
+            for sp, w in zip(P, want):
+                got = g.realcal_proc(sy.solute_mol, sp)
+                if prec == "fp64":
+                    np.testing.assert_allclose(got, w, **TOL64)
+                else:
+                    ok, err = fp32_ok(got, w)
+                    assert ok.all(), (path, err.max())
+
+
 @pytest.mark.parametrize("path", ["fused", "unfused", "register"])
 def test_pme_paths_agree(refs, path):
     """Every reciprocal-space path (LDS slab; plane + gather; register
@@ -330,7 +393,9 @@ def test_fp32_error_breakdown(refs):
             g.set_frame(pos, sy.cell)
             res[prec] = ([g.realcal_proc(sy.solute_mol, sp) for sp in P],
                          [g.recpcal(sy.solute_mol, sp)[0] for sp in P])
-    er = max(np.max(np.abs(a - b)) for a, b in zip(res["fp32"][0], res["fp64"][0]))
+    er = max(np.max(np.abs(a - b)[np.abs(b) <= FP32_EMAX]) for a, b in zip(res["fp32"][0], res["fp64"][0]))
     ek = max(np.max(np.abs(a - b)) for a, b in zip(res["fp32"][1], res["fp64"][1]))
     print(f"fp32 max |dE| real {er:.3e}  recip {ek:.3e}")
     assert er <= FP32_ABS and ek <= FP32_ABS
+    for a, b in zip(res["fp32"][0], res["fp64"][0]):
+        assert fp32_ok(a, b)[0].all()
