@@ -156,36 +156,51 @@ def mixed_setup(torch, dev, nframes):
     return syss, host, dpos
 
 
-def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True):
+# the shape the autotuner settles on for the mixed-solvent workload, used
+# when it is skipped (profiling runs)
+FIXED_TUNING = dict(max_batch=1024, pme_path="register", slab_planes=8, group_planes=8, slab_threads=512,
+                    column_threads=256, fold_y=True)
+
+
+def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sequential=False):
     """Timed frames of the mixed-solvent workload: per frame, each solute's
     context takes the device frame and its share of the insertions; the four
-    contexts run concurrently (one stream each).  Returns the timing, the
+    contexts run concurrently (one stream each), or one after another
+    (sequential, for clean per-kernel traces).  Returns the timing, the
     summed stage times and per-stage launch / configuration counts."""
     from ermod_amd.api import ErmGPU, insert_random_multi
     nfu = len(dpos)
     share = [per_frame // len(syss)] * len(syss)
     engines = [ErmGPU(sy.system, prec) for sy in syss]
+    seeds = [11, 12, 13, 14]
+
+    def frame(i):
+        for e, sy in zip(engines, syss):
+            e.set_frame_device(dpos[i % nfu].data_ptr(), sy.cell)
+        if sequential:
+            for k, e in enumerate(engines):
+                e.insert_random(share[k], seeds[k], first_index=i * share[k])
+        else:
+            insert_random_multi(engines, share, seeds, [i * s for s in share])
+
     try:
         for e, sy in zip(engines, syss):
             e.set_frame_device(dpos[0].data_ptr(), sy.cell)
             if tune:
                 e.autotune(n_trial=share[0], seed=99)
+            else:
+                e.set_tuning(**FIXED_TUNING)
         for e in engines:
             e.set_tuning(max_batch=0, pme_path=e.tuning()["pme_path"], use_graph=False)
-        seeds = [11, 12, 13, 14]
         for i in range(warmup):
-            for e, sy in zip(engines, syss):
-                e.set_frame_device(dpos[i % nfu].data_ptr(), sy.cell)
-            insert_random_multi(engines, share, seeds, [i * s for s in share])
+            frame(i)
         for e in engines:
             e.engclear()
         torch.cuda.synchronize()
         st = [dict() for _ in engines]
         t0 = time.perf_counter()
         for i in range(warmup, warmup + steps):
-            for e, sy in zip(engines, syss):
-                e.set_frame_device(dpos[i % nfu].data_ptr(), sy.cell)
-            insert_random_multi(engines, share, seeds, [i * s for s in share])
+            frame(i)
             for k, e in enumerate(engines):
                 for key, v in e.stage_times().items():
                     st[k][key] = st[k].get(key, 0.0) + v
@@ -313,6 +328,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default batch and PME path)")
     ap.add_argument("--no-error", action="store_true", help="skip the FP32-vs-FP64 error pass")
     ap.add_argument("--no-refs20", action="store_true", help="skip the refs20 secondary passes")
+    ap.add_argument("--sequential", action="store_true",
+                    help="run the four solutes' contexts one after another (clean per-kernel traces)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="after the run, one device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; runs on several GPUs (frames sharded over "
@@ -325,7 +344,7 @@ def main():
     nfu = min(args.steps + args.warmup, 6) + 1   # distinct frames (cycled) + one for the error pass
     syss, host, dpos = mixed_setup(torch, dev, nfu)
     r = run_mixed(torch, syss, dpos[:-1], args.precision, args.steps, args.warmup, args.insertions,
-                  tune=not args.no_tune)
+                  tune=not args.no_tune, sequential=args.sequential)
     slvmax, ermax = r["slvmax"], r["ermax"]
     n_ins = args.steps * sum(r["share"])
     value = n_ins * slvmax / r["dt"]
@@ -375,7 +394,8 @@ def main():
                       "tuning": [{k: t[k] for k in ("max_batch", "pme_path", "slab_planes", "group_planes",
                                                     "slab_threads", "column_threads", "fold_y")}
                                  for t in r["tuning"]],
-                      "schedule": "four contexts, one stream each"},
+                      "schedule": "four contexts, one after another" if args.sequential
+                      else "four contexts, one stream each"},
            "insertions_per_s": n_ins / r["dt"],
            "roofline": roof, "stages": stage_rows, "stage_ms": stage_ms_total,
            "engsmpl": [a["engsmpl"] for a in r["acc"]]}
@@ -393,6 +413,14 @@ def main():
         cb = cpu_baseline_mixed(syss, host[0], args.cpu_seconds)
         cb["gpu_over_cpu"] = value / cb["value"]
         out["cpu_baseline"] = cb
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 2 * n * 8
     print(json.dumps(out), flush=True)
 
 

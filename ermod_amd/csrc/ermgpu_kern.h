@@ -1780,17 +1780,22 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
 constexpr int PT_GRP = 8;                 // solute sites per compaction group
 constexpr int PT_QCAP = 64 * PT_GRP;      // queued pairs per wave (at most)
 
+template <typename T> struct V4T;
+template <> struct V4T<float> { using type = float4; };
+template <> struct V4T<double> { using type = double4; };
+
+// LDS of k_pair_tile.  Head: the solute's sites as (x, y, z, q) and their
+// low parts (T4 each), the LJ rows [ns][nvt] (eps, sig2), the pillar segments
+// (segcap = 2 nb1 nb2 slots).  Per wave: the tile's sites (x, y, z, q) and
+// (low parts, LJ type), molecule offsets [65] and first sites [64], the queue
+// [PT_QCAP] (int) and its energies (f64), the per-site sums [64] (f64), the
+// slots of the queued pairs evaluated in FP64 [PT_QCAP] (int).
 template <typename T> __host__ __device__ inline size_t pair_tile_wave_lds() {
-  // tile: hi[3][64], lo[3][64], q[64] (T), type[64]; molecule offsets [65],
-  // first sites [64]; queue [PT_QCAP] (int) and energies [PT_QCAP] (f64);
-  // per-site sums [64] (f64); rounded to 16 bytes
-  return ((size_t)7 * 64 * sizeof(T) + 64 * 4 + 65 * 4 + 64 * 4 + (size_t)PT_QCAP * (4 + 8) + 64 * 8 + 16 + 15) &
+  return ((size_t)2 * 64 * 4 * sizeof(T) + 65 * 4 + 64 * 4 + (size_t)PT_QCAP * (4 + 8 + 4) + 64 * 8 + 16 + 15) &
          ~(size_t)15;
 }
-// solute data, then the pillar segments (segcap = 2 nb1 nb2 slots), then the
-// waves' tiles and queues
 template <typename T> __host__ __device__ inline size_t pair_tile_head(int ns, int nvt, int segcap) {
-  const size_t a = sizeof(T) * ((size_t)7 * ns + (size_t)2 * ns * nvt) + sizeof(int) * (2 * (size_t)segcap + 2) + 64;
+  const size_t a = sizeof(T) * ((size_t)8 * ns + (size_t)2 * ns * nvt) + sizeof(int) * (2 * (size_t)segcap + 2) + 64;
   return (a + 15) & ~(size_t)15;
 }
 template <typename T> __host__ __device__ inline size_t pair_tile_lds(int ns, int nvt, int segcap, int nwave) {
@@ -1820,44 +1825,52 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
                                                     const double *__restrict__ slt_lj64,
                                                     const double *__restrict__ slt_q64, double *uv) {
   constexpr bool F32 = sizeof(T) == 4;
+  using T4 = typename V4T<T>::type;
+  using T2 = typename C2<T>::type;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int ns = P.ns, nvt = P.nvt;
-  T *ux = reinterpret_cast<T *>(lds_raw);
-  T *uy = ux + ns;
-  T *uz = uy + ns;
-  T *uq = uz + ns;
-  T *ulj = uq + ns;                       // [ns][nvt][2] (eps, sig2)
-  T *ulo = ulj + 2 * ns * nvt;            // [ns][3] low parts of the offsets
+  T4 *us4 = reinterpret_cast<T4 *>(lds_raw);  // [ns] (x, y, z, q) offsets from the centre
+  T4 *ul4 = us4 + ns;                         // [ns] low parts (x, y, z, 0)
+  T2 *ulj = reinterpret_cast<T2 *>(ul4 + ns); // [ns][nvt] (eps, sig2)
   const int segcap = 2 * P.nb[1] * P.nb[2];
-  int *seg_beg = reinterpret_cast<int *>(ulo + 3 * ns);
+  int *seg_beg = reinterpret_cast<int *>(ulj + ns * nvt);
   int *seg_pre = seg_beg + segcap;
   const size_t base = pair_tile_head<T>(ns, nvt, segcap);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   unsigned char *wl = lds_raw + base + (size_t)wv * pair_tile_wave_lds<T>();
-  T *th = reinterpret_cast<T *>(wl);      // [3][64] high parts
-  T *tl = th + 3 * 64;                    // [3][64] low parts
-  T *tq = tl + 3 * 64;                    // [64]
-  int *tt = reinterpret_cast<int *>(tq + 64);
-  int *woff = tt + 64;                    // [65] first chunk site of molecule lane
+  T4 *tu4 = reinterpret_cast<T4 *>(wl);   // [64] (x, y, z, q) of the tile's sites
+  T4 *tl4 = tu4 + 64;                     // [64] (low x, low y, low z, LJ type bits)
+  int *woff = reinterpret_cast<int *>(tl4 + 64);  // [65] first chunk site of molecule lane
   int *wfirst = woff + 65;                // [64] first sorted site of molecule lane
   int *wq = wfirst + 64;                  // [PT_QCAP] queued pairs: solute site << 6 | tile lane
   double *weq = reinterpret_cast<double *>(reinterpret_cast<uintptr_t>(wq + PT_QCAP + 1) & ~(uintptr_t)7);
   double *wes = weq + PT_QCAP;            // [64] per-site sums of a tile
+  int *whq = reinterpret_cast<int *>(wes + 64);  // [PT_QCAP] queue slots of the pairs evaluated in FP64
 
   const int b = blockIdx.x;
   const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
   const double Rs = sref[4 * b + 3];
   const double *xs = spos + (size_t)b * ns * 3;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-    ux[s] = (T)(xs[3 * s] - C0);
-    uy[s] = (T)(xs[3 * s + 1] - C1);
-    uz[s] = (T)(xs[3 * s + 2] - C2v);
-    uq[s] = slt_q[s];
-    ulo[3 * s] = (T)((xs[3 * s] - C0) - (double)ux[s]);
-    ulo[3 * s + 1] = (T)((xs[3 * s + 1] - C1) - (double)uy[s]);
-    ulo[3 * s + 2] = (T)((xs[3 * s + 2] - C2v) - (double)uz[s]);
+    const double d0 = xs[3 * s] - C0, d1 = xs[3 * s + 1] - C1, d2 = xs[3 * s + 2] - C2v;
+    T4 u, l;
+    u.x = (T)d0;
+    u.y = (T)d1;
+    u.z = (T)d2;
+    u.w = slt_q[s];
+    l.x = (T)(d0 - (double)u.x);
+    l.y = (T)(d1 - (double)u.y);
+    l.z = (T)(d2 - (double)u.z);
+    l.w = 0;
+    us4[s] = u;
+    ul4[s] = l;
   }
-  for (int t = threadIdx.x; t < 2 * ns * nvt; t += blockDim.x) ulj[t] = slt_lj[t];
+  for (int t = threadIdx.x; t < ns * nvt; t += blockDim.x) {
+    T2 v;
+    v.x = slt_lj[2 * t];
+    v.y = slt_lj[2 * t + 1];
+    ulj[t] = v;
+  }
   const double Rm = ord_to_dbl(*(const unsigned long long *)rm_ord_p);
   const double R = P.rcmax + Rs + Rm + 1e-6 * (1.0 + P.rcmax);
   const double Cw[3] = {wrap1(C0, P.L[0]), wrap1(C1, P.L[1]), wrap1(C2v, P.L[2])};
@@ -1914,8 +1927,9 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
     for (int t0 = 0; t0 < tot_sites; t0 += 64) {
       const int cs = t0 + lane;
       bool act = cs < tot_sites;
-      T hx = 0, hy = 0, hz = 0, lx = 0, ly = 0, lz = 0, qv = 0;
-      int cv = 0;
+      T4 tu, tl;
+      tu.x = tu.y = tu.z = tu.w = 0;
+      tl.x = tl.y = tl.z = tl.w = 0;
       if (act) {
         int a = 0, e = 64;  // molecule k of chunk site cs: last k with woff[k] <= cs
         while (e - a > 1) {
@@ -1931,25 +1945,21 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
         dy -= P.L[1] * rint(dy * iL1);
         dz -= P.L[2] * rint(dz * iL2);
         act = (dx * dx + dy * dy + dz * dz) <= reach2;
-        hx = (T)dx;
-        hy = (T)dy;
-        hz = (T)dz;
+        tu.x = (T)dx;
+        tu.y = (T)dy;
+        tu.z = (T)dz;
+        tu.w = sq[t];
         if constexpr (F32) {
-          lx = (T)(dx - (double)hx);
-          ly = (T)(dy - (double)hy);
-          lz = (T)(dz - (double)hz);
+          tl.x = (T)(dx - (double)tu.x);
+          tl.y = (T)(dy - (double)tu.y);
+          tl.z = (T)(dz - (double)tu.z);
+          tl.w = __int_as_float(sct[t]);
+        } else {
+          tl.w = __longlong_as_double((long long)sct[t]);
         }
-        qv = sq[t];
-        cv = sct[t];
       }
-      th[lane] = hx;
-      th[64 + lane] = hy;
-      th[128 + lane] = hz;
-      tl[lane] = lx;
-      tl[64 + lane] = ly;
-      tl[128 + lane] = lz;
-      tq[lane] = qv;
-      tt[lane] = cv;
+      tu4[lane] = tu;
+      tl4[lane] = tl;
       double e = 0.0;  // this site's energy, in solute-site order
       for (int s0 = 0; s0 < ns; s0 += PT_GRP) {
         int pend[PT_GRP];
@@ -1958,8 +1968,9 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
         for (int g = 0; g < PT_GRP; ++g) {
           const int s = s0 + g;
           bool in = false;
-          if (s < ns && act) {
-            T ax = hx - ux[s], ay = hy - uy[s], az = hz - uz[s];
+          if (s < ns) {
+            const T4 us = us4[s];
+            T ax = tu.x - us.x, ay = tu.y - us.y, az = tu.z - us.z;
             if (img) {
               ax = fabs(ax);
               ay = fabs(ay);
@@ -1968,7 +1979,7 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
               ay = (ay > h1) ? L1 - ay : ay;
               az = (az > h2) ? L2 - az : az;
             }
-            in = (ax * ax + ay * ay + az * az) <= cut2q;
+            in = act && (ax * ax + ay * ay + az * az) <= cut2q;
           }
           const unsigned long long m = __ballot(in);
           const int pos = qn + lanes_below(m);
@@ -1977,22 +1988,26 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
           qn += (int)__popcll(m);
         }
         wave_lds_sync();
-        // the queued pairs, every lane busy
+        // the queued pairs, every lane busy; in FP32 the pairs that need FP64
+        // (hot_pair) go to a second queue evaluated afterwards, so the FP64
+        // code runs on full waves too
+        int nh = 0;
         for (int q0 = 0; q0 < qn; q0 += 64) {
           const int qi = q0 + lane;
-          if (qi >= qn) break;
-          const int ent = wq[qi];
+          const bool valid = qi < qn;
+          const int ent = valid ? wq[qi] : 0;
           const int i = ent & 63, s = ent >> 6;
-          const T vx = th[i], vy = th[64 + i], vz = th[128 + i];
+          const T4 v = tu4[i], vl = tl4[i], us = us4[s];
           T ax, ay, az;
           if constexpr (F32) {
-            ax = ((vx - ux[s]) + tl[i]) - ulo[3 * s];
-            ay = ((vy - uy[s]) + tl[64 + i]) - ulo[3 * s + 1];
-            az = ((vz - uz[s]) + tl[128 + i]) - ulo[3 * s + 2];
+            const T4 ls = ul4[s];
+            ax = ((v.x - us.x) + vl.x) - ls.x;
+            ay = ((v.y - us.y) + vl.y) - ls.y;
+            az = ((v.z - us.z) + vl.z) - ls.z;
           } else {
-            ax = vx - ux[s];
-            ay = vy - uy[s];
-            az = vz - uz[s];
+            ax = v.x - us.x;
+            ay = v.y - us.y;
+            az = v.z - us.z;
           }
           ax = fabs(ax);
           ay = fabs(ay);
@@ -2003,18 +2018,68 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
             az = (az > h2) ? L2 - az : az;
           }
           const T r2 = ax * ax + ay * ay + az * az;
-          const T qsv = tq[i];
-          const int cvi = tt[i];
-          double epd = 0.0;
-          if (r2 <= cut2) {
-            const T eps = ulj[(s * nvt + cvi) * 2], sig2 = ulj[(s * nvt + cvi) * 2 + 1];
-            bool done = false;
-            if constexpr (F32) {
-              // a repulsive overlap or a large Coulomb term: the pair in FP64
-              if (hot_pair<T>(eps, sig2, uq[s] * qsv, r2)) {
-                double bx = ((double)vx + (double)tl[i]) - ((double)ux[s] + (double)ulo[3 * s]);
-                double by = ((double)vy + (double)tl[64 + i]) - ((double)uy[s] + (double)ulo[3 * s + 1]);
-                double bz = ((double)vz + (double)tl[128 + i]) - ((double)uz[s] + (double)ulo[3 * s + 2]);
+          const T qsv = v.w;
+          int cvi;
+          if constexpr (F32)
+            cvi = __float_as_int(vl.w);
+          else
+            cvi = (int)__double_as_longlong(vl.w);
+          const T2 lj = ulj[s * nvt + cvi];
+          const T eps = lj.x, sig2 = lj.y;
+          const bool inr = valid && r2 <= cut2;
+          bool hot = false;
+          if constexpr (F32) hot = inr && hot_pair<T>(eps, sig2, us.w * qsv, r2);
+          const unsigned long long mh = __ballot(hot);
+          if (hot) whq[nh + lanes_below(mh)] = qi;
+          nh += (int)__popcll(mh);
+          if (valid && !hot) {
+            T ep = 0;
+            if (inr) {
+              if constexpr (BARE) {
+                const T rst = sqrt(r2);
+                ep = lj_bare_t<T>(P, eps, sig2, r2, rst);
+                if (rst < (T)P.elecut) ep += us.w * qsv / rst;
+              } else {
+                if (P.ljswitch == 0) {
+                  // CHARMM potential switch without branches: the low-range
+                  // term, scaled by the switch beyond lwljcut, zero beyond
+                  // upljcut or for eps = 0 (the expressions of lj_low_t and
+                  // lj_switch_t)
+                  const T invr2 = sig2 / r2;
+                  const T invr6 = invr2 * invr2 * invr2;
+                  const T elo = (T)4 * eps * invr6 * (invr6 - (T)1);
+                  const T d = r2 - up2, w = up2 - lw2;
+                  const T esw = elo * ((T)2 * r2 + up2 - (T)3 * lw2) * (d * d) / (w * w * w);
+                  T elj = (r2 <= lw2) ? elo : esw;
+                  elj = (r2 <= up2 && eps > (T)0) ? elj : (T)0;
+                  ep = elj;
+                } else if (eps > (T)0) {
+                  if (r2 <= lw2)
+                    ep += lj_low_t<T>(P, eps, sig2, r2);
+                  else if (r2 <= up2)
+                    ep += lj_switch_t<T>(P, eps, sig2, r2);
+                }
+                if (r2 <= el2) ep += coul_t<T>(us.w * qsv, sqrt(r2), alpha);
+              }
+            }
+            weq[qi] = (double)ep;
+          }
+        }
+        if constexpr (F32) {
+          if (nh > 0) {
+            wave_lds_sync();
+            for (int h0 = 0; h0 < nh; h0 += 64) {
+              const int hi = h0 + lane;
+              if (hi < nh) {
+                const int qi = whq[hi];
+                const int ent = wq[qi];
+                const int i = ent & 63, s = ent >> 6;
+                const T4 v = tu4[i], vl = tl4[i], us = us4[s], ls = ul4[s];
+                const int cvi = __float_as_int(vl.w);
+                const T qsv = v.w;
+                double bx = ((double)v.x + (double)vl.x) - ((double)us.x + (double)ls.x);
+                double by = ((double)v.y + (double)vl.y) - ((double)us.y + (double)ls.y);
+                double bz = ((double)v.z + (double)vl.z) - ((double)us.z + (double)ls.z);
                 bx = fabs(bx);
                 by = fabs(by);
                 bz = fabs(bz);
@@ -2025,6 +2090,7 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
                 }
                 const double r2d = bx * bx + by * by + bz * bz;
                 const double e64 = slt_lj64[(s * nvt + cvi) * 2], s64 = slt_lj64[(s * nvt + cvi) * 2 + 1];
+                double epd = 0.0;
                 if constexpr (BARE) {
                   const double rd = sqrt(r2d);
                   epd = lj_bare_t<double>(P, e64, s64, r2d, rd);
@@ -2036,28 +2102,10 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
                     epd = lj_switch_t<double>(P, e64, s64, r2d);
                   if (r2d <= P.el2) epd += coul_t<double>(slt_q64[s] * (double)qsv, sqrt(r2d), P.screen);
                 }
-                done = true;
+                weq[qi] = epd;
               }
-            }
-            if (!done) {
-              T ep = 0;
-              if constexpr (BARE) {
-                const T rst = sqrt(r2);
-                ep = lj_bare_t<T>(P, eps, sig2, r2, rst);
-                if (rst < (T)P.elecut) ep += uq[s] * qsv / rst;
-              } else {
-                if (eps > (T)0) {
-                  if (r2 <= lw2)
-                    ep += lj_low_t<T>(P, eps, sig2, r2);
-                  else if (r2 <= up2)
-                    ep += lj_switch_t<T>(P, eps, sig2, r2);
-                }
-                if (r2 <= el2) ep += coul_t<T>(uq[s] * qsv, sqrt(r2), alpha);
-              }
-              epd = (double)ep;
             }
           }
-          weq[qi] = epd;
         }
         wave_lds_sync();
 #pragma unroll
