@@ -61,7 +61,7 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_insert_random_multi", "erm_set_insertion", "erm_dcd_open", "erm_dcd_read", "erm_dcd_close",
            "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close", "erm_set_intermediate",
            "erm_fetch_intermediate", "erm_set_refstructure", "erm_insert_flexible", "erm_set_pair_path",
-           "erm_get_pair_path")
+           "erm_get_pair_path", "erm_update_histogram_static")
 
 _lib = None
 
@@ -95,6 +95,7 @@ def lib():
         L.erm_engconst.argtypes = [vp, C.c_int32, _i32p, _f64p, _f64p, _f64p]
         L.erm_insert_random.argtypes = [vp, C.c_int32, C.c_uint64, C.c_uint64, _f64p]
         L.erm_update_histogram.argtypes = [vp, C.c_int32, _i32p, _f64p, _f64p]
+        L.erm_update_histogram_static.argtypes = [vp, C.c_int32, _i32p, _f64p, _f64p]
         L.erm_engstore.argtypes = [vp]
         L.erm_fetch.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.erm_sizes.argtypes = [vp, _i32p, _i32p]
@@ -408,6 +409,16 @@ class ErmGPU:
         w = None if weights is None else _f64(weights)
         self._check(lib().erm_update_histogram(self.h, n, _p(sl, _i32p), _p(w), _p(u)),
                     "update_histogram")
+
+    def update_histogram_static(self, uvengy, solutes=None, weights=None):
+        """update_histogram with the library's static scratch (allocates no
+        device memory; n limited by the scratch size)."""
+        u = _f64(uvengy)
+        n = u.shape[0]
+        sl = None if solutes is None else _i32(solutes)
+        w = None if weights is None else _f64(weights)
+        self._check(lib().erm_update_histogram_static(self.h, n, _p(sl, _i32p), _p(w), _p(u)),
+                    "update_histogram_static")
 
     def engstore(self):
         self._check(lib().erm_engstore(self.h), "engstore")

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -19,6 +20,40 @@
 
 using namespace erm;
 using namespace ermh;
+
+// Scratch of erm_update_histogram_static: a file-scope device array, one copy
+// per GPU (each at its own address), held until the process ends.  Host code
+// never passes `scratch` itself to a kernel or copy: its device address is
+// resolved once per device (scratch_init, from erm_create) and used instead.
+// Calls share their GPU's copy and it keeps the last call's contents, so
+// every call clears what must start at zero, and calls that overlap on one
+// GPU race.
+constexpr size_t SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[SCRATCH_DEVICES] = {};
+
+// device address of `scratch` on `device`, which must be current
+int scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= SCRATCH_DEVICES) return ERM_E_UNSUPPORTED;
+  if (g_scratch[device]) return ERM_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return ERM_E_DEVICE;
+  }
+  g_scratch[device] = static_cast<unsigned char *>(p);
+  return ERM_OK;
+}
+unsigned char *scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < SCRATCH_DEVICES) ? g_scratch[device] : nullptr;
+}
+}  // namespace
 
 namespace {
 size_t zcol_lds(int ncol, int K3, size_t vb);
@@ -1436,6 +1471,7 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ERM_E_DEVICE;
   if (P.device < 0 || P.device >= ndev) return ERM_E_ARG;
   if (hipSetDevice(P.device) != hipSuccess) return ERM_E_DEVICE;
+  if (scratch_init(P.device) != ERM_OK) return ERM_E_DEVICE;
   erm_ctx *c = new erm_ctx();
   c->P = P;
   for (int d = 0; d < 3; ++d) c->P.ms[d] = msv[d];
@@ -2078,6 +2114,70 @@ int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, cons
     TRY(run_histogram(c, nb, s0));
   }
   return check_status(c);
+}
+
+int32_t erm_update_histogram_static(erm_ctx *c, int32_t n, const int32_t *solutes, const double *weights,
+                                    const double *uvengy) {
+  TRY(need(c, true, true));
+  if (hipSetDevice(c->P.device) != hipSuccess) return fail(c, ERM_E_DEVICE, "hipSetDevice");
+  if (n < 0 || (n > 0 && !uvengy)) return fail(c, ERM_E_ARG, "uvengy is NULL or n negative");
+  if (c->P.slttype == 1 && n > 0 && !solutes) return fail(c, ERM_E_ARG, "soln mode needs solute indices");
+  if (solutes)
+    for (int b = 0; b < n; ++b)
+      if (solutes[b] < 0 || solutes[b] >= c->nmol) return fail(c, ERM_E_ARG, "solute index out of range");
+  unsigned char *base = scratch_of(c->P.device);
+  if (!base) return fail(c, ERM_E_DEVICE, "static scratch not resolved on device %d", c->P.device);
+  // layout: status words, then per configuration the energies as given and
+  // sorted, its histogram, per-species sums, weights (in, out, intermediate)
+  // and the solute's indices, each region 256-byte aligned
+  const int S = c->slvmax, E = c->ermax;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t per = 2 * sizeof(double) * ((size_t)S + 1) + sizeof(double) * ((size_t)E + MAXSPEC + 3) +
+                     2 * sizeof(int);
+  const size_t fixed = 256 + 9 * 256;  // status + alignment of the nine regions
+  const size_t cap = (SCRATCH_BYTES > fixed) ? (SCRATCH_BYTES - fixed) / per : 0;
+  if ((size_t)n > cap)
+    return fail(c, ERM_E_ARG,
+                "erm_update_histogram_static: %d configurations exceed the %zu that its %zu-byte scratch holds "
+                "for %d molecules and %d bins",
+                n, cap, SCRATCH_BYTES, S, E);
+  if (n == 0) return ERM_OK;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    unsigned char *p = base + off;
+    off += al(bytes);
+    return p;
+  };
+  int *status = reinterpret_cast<int *>(carve(64));
+  double *uvin = reinterpret_cast<double *>(carve(sizeof(double) * (size_t)n * (S + 1)));
+  HistBufs hb;
+  hb.status = status;
+  hb.uv = reinterpret_cast<double *>(carve(sizeof(double) * (size_t)n * (S + 1)));
+  hb.hist = reinterpret_cast<double *>(carve(sizeof(double) * (size_t)n * E));
+  hb.flceng = reinterpret_cast<double *>(carve(sizeof(double) * (size_t)n * MAXSPEC));
+  hb.wsol = reinterpret_cast<double *>(carve(sizeof(double) * n));
+  hb.w = reinterpret_cast<double *>(carve(sizeof(double) * n));
+  hb.wi = reinterpret_cast<double *>(carve(sizeof(double) * n));
+  hb.sltj = reinterpret_cast<int *>(carve(sizeof(int) * n));
+  int *sol = solutes ? reinterpret_cast<int *>(carve(sizeof(int) * n)) : nullptr;
+  if (off > SCRATCH_BYTES) return fail(c, ERM_E_ENG_BUG, "static scratch layout overflows");
+  // the array keeps the previous call's contents: the status starts at zero
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(32), 0, c->st, status, (size_t)16, 0);
+  TRY(launch_check(c, "k_fill"));
+  HIPCHK(c, hipMemcpyAsync(uvin, uvengy, sizeof(double) * (size_t)n * (S + 1), hipMemcpyHostToDevice, c->st));
+  if (sol) HIPCHK(c, hipMemcpyAsync(sol, solutes, sizeof(int) * n, hipMemcpyHostToDevice, c->st));
+  if (weights) {
+    HIPCHK(c, hipMemcpyAsync(hb.wsol, weights, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+  } else {
+    hipLaunchKernelGGL(k_fill<double>, dim3((n + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, hb.wsol, (size_t)n, 1.0);
+    TRY(launch_check(c, "k_fill"));
+  }
+  hipLaunchKernelGGL(k_sort_uv, dim3((S + 255) / 256, n), dim3(256), 0, c->st, n, S, dp<int>(c->d_mol_k),
+                     dp<int>(c->d_jof), dp<int>(c->d_kof), sol, uvin, hb.uv, hb.sltj);
+  TRY(launch_check(c, "k_sort_uv"));
+  c->cur_ev = nullptr;
+  TRY(run_histogram_at(c, n, 0, hb));
+  return check_status(c, status);
 }
 
 int32_t erm_engstore(erm_ctx *c) {

@@ -289,6 +289,21 @@ int32_t erm_insert_random_multi(erm_ctx *const *ctxs, int32_t nctx, const int32_
 int32_t erm_update_histogram(erm_ctx *ctx, int32_t n, const int32_t *solutes,
                              const double *weights, const double *uvengy);
 
+/* erm_update_histogram without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as erm_update_histogram; its scratch (the
+ * energies as given and sorted, per-configuration histograms, weights and
+ * per-species sums) lives in a 64 MiB device array of the library, one per
+ * GPU, instead of context buffers.  n must fit that array:
+ * n * (16 (slvmax + 1) + 8 (ermax + 35) + 8) bytes plus alignment, e.g. about
+ * 950 configurations for 4000 molecules and 600 bins; larger n returns
+ * ERM_E_ARG with the limit in the message.  Calls on one GPU share the array:
+ * calls that overlap on one GPU (several host threads) race.  The context
+ * must already hold its frame and bins (erm_set_frame, erm_set_bins). */
+int32_t erm_update_histogram_static(erm_ctx *ctx, int32_t n, const int32_t *solutes,
+                                    const double *weights, const double *uvengy);
+
 /* engstore's normalisation (engproc.F90:539-553) on the device accumulators
  * (one process: no reduction step). */
 int32_t erm_engstore(erm_ctx *ctx);
