@@ -61,7 +61,9 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_insert_random_multi", "erm_set_insertion", "erm_dcd_open", "erm_dcd_read", "erm_dcd_close",
            "erm_read_permutation", "erm_traj_open", "erm_traj_next", "erm_traj_close", "erm_set_intermediate",
            "erm_fetch_intermediate", "erm_set_refstructure", "erm_insert_flexible", "erm_set_pair_path",
-           "erm_get_pair_path", "erm_update_histogram_static")
+           "erm_get_pair_path", "erm_update_histogram_static",
+           "erm_group_create", "erm_group_destroy", "erm_group_last_error", "erm_group_set_frame",
+           "erm_group_set_frame_device", "erm_group_insert_random", "erm_group_set_tuning", "erm_group_stage_times")
 
 _lib = None
 
@@ -104,6 +106,16 @@ def lib():
         L.erm_get_tuning.argtypes = [vp, _i32p]
         L.erm_autotune.argtypes = [vp, C.c_int32, C.c_uint64, _f64p]
         L.erm_set_pair_path.argtypes = [vp, C.c_int32]
+        L.erm_group_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+        L.erm_group_destroy.argtypes = [vp]
+        L.erm_group_destroy.restype = None
+        L.erm_group_last_error.argtypes = [vp]
+        L.erm_group_last_error.restype = C.c_char_p
+        L.erm_group_set_frame.argtypes = [vp, _f64p, _f64p, C.c_double]
+        L.erm_group_set_frame_device.argtypes = [vp, C.c_void_p, _f64p, C.c_double]
+        L.erm_group_insert_random.argtypes = [vp, _i32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.erm_group_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+        L.erm_group_stage_times.argtypes = [vp, _f64p]
         L.erm_get_pair_path.argtypes = [vp]
         L.erm_insert_random_multi.argtypes = [C.POINTER(vp), C.c_int32, _i32p, C.POINTER(C.c_uint64),
                                               C.POINTER(C.c_uint64)]
@@ -241,6 +253,75 @@ def insert_random_multi(engines, n, seeds, first_indices):
     if st:
         msgs = [lib().erm_last_error(e.h) for e in engines]
         raise ErmError(st, "insert_random_multi: " + "; ".join(m.decode() for m in msgs if m))
+
+
+class ErmGroup:
+    """Several rigid test particles (one ErmGPU context each, same solvent)
+    inserted in one launch sequence (erm_group_*)."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        k = len(self.engines)
+        hs = (C.c_void_p * k)(*[e.h for e in self.engines])
+        out = C.c_void_p()
+        st = lib().erm_group_create(hs, k, C.byref(out))
+        if st:
+            raise ErmError(st, "group_create: " + lib().erm_last_error(self.engines[0].h).decode())
+        self.h = out.value
+
+    def _check(self, st, where):
+        if st:
+            raise ErmError(st, f"{where}: {lib().erm_group_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().erm_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_frame(self, sitepos, cell, weight=1.0):
+        pos = _f64(sitepos)
+        cm = _f64(np.asarray(cell).T.copy())
+        self._check(lib().erm_group_set_frame(self.h, _p(pos), _p(cm), float(weight)), "group_set_frame")
+
+    def set_frame_device(self, dev_ptr: int, cell, weight=1.0):
+        cm = _f64(np.asarray(cell).T.copy())
+        self._check(lib().erm_group_set_frame_device(self.h, C.c_void_p(dev_ptr), _p(cm), float(weight)),
+                    "group_set_frame_device")
+
+    def insert_random(self, n, seeds, first_indices):
+        k = len(self.engines)
+        nn = np.ascontiguousarray(n, dtype=np.int32)
+        ss = (C.c_uint64 * k)(*[int(x) for x in seeds])
+        ff = (C.c_uint64 * k)(*[int(x) for x in first_indices])
+        self._check(lib().erm_group_insert_random(self.h, _p(nn, _i32p), ss, ff), "group_insert_random")
+
+    def set_tuning(self, max_batch=0, persistent=None):
+        """persistent: None keeps it; True / False every / no heavy stage
+        as one persistent grid; or the bit mask (1 z columns, 2 pairs)."""
+        if persistent is None:
+            ps = -1
+        elif isinstance(persistent, bool):
+            ps = 3 if persistent else 0
+        else:
+            ps = int(persistent)
+        self._check(lib().erm_group_set_tuning(self.h, int(max_batch), ps), "group_set_tuning")
+
+    def stage_times(self):
+        out = np.zeros(len(STAGES))
+        lib().erm_group_stage_times(self.h, _p(out))
+        return dict(zip(STAGES, out))
 
 
 def hash_double(v):

@@ -957,6 +957,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
 struct HistBufs {
   double *uv, *wsol, *hist, *w, *flceng, *wi;
   int *sltj, *status;
+  const int *mol_spec;  // species of the sorted molecules (the frame's owner)
 };
 
 // the context's own buffers (grown as needed)
@@ -973,17 +974,31 @@ int ctx_hist_bufs(erm_ctx *c, int n, HistBufs &hb) {
   hb.wi = dp<double>(c->d_wi);
   hb.sltj = dp<int>(c->d_sltj);
   hb.status = dp<int>(c->d_status);
+  hb.mol_spec = dp<int>(c->d_mol_spec);
   return ERM_OK;
 }
 
+// the correlation matrix part of update_histogram (engproc.F90:835-853)
+int run_corr_at(erm_ctx *c, int n, const HistBufs &hb) {
+  if (!c->P.corrcal) return ERM_OK;
+  const int E = c->ermax;
+  const int nt = (E + 15) / 16;
+  const int tiles = nt * (nt + 1) / 2;  // tiles on and above the diagonal, one workgroup each
+  hipLaunchKernelGGL(k_corr, dim3(tiles), dim3(256), 0, c->st, n, E, hb.hist, hb.w, dp<double>(c->d_ecorr));
+  return launch_check(c, "k_corr");
+}
+
 // update_histogram for the n configurations of hb (engproc.F90:758-856)
-int run_histogram_at(erm_ctx *c, int n, int first, const HistBufs &hb) {
+// (without_corr: the correlation matrix is left to run_corr_at)
+int run_histogram_at(erm_ctx *c, int n, int first, const HistBufs &hb, bool without_corr = false) {
   Params k = c->kp;
   const int E = c->ermax;
-  size_t lb = (size_t)E * (sizeof(double) + sizeof(int));
+  const size_t lb = (size_t)E * (sizeof(double) + sizeof(int)) + sizeof(double) * (size_t)c->numslv * NTHR;
+  TRY(set_lds(c, (const void *)k_bin, lb, "k_bin"));
+  if (!hb.mol_spec) return fail(c, ERM_E_ENG_BUG, "histogram without the frame's molecule species");
   hipLaunchKernelGGL(k_bin, dim3(n), dim3(NTHR), lb, c->st, k, hb.uv, dp<double>(c->d_edges),
-                     dp<int>(c->d_specoff), dp<int>(c->d_uvmax), dp<int>(c->d_uvsoft), dp<int>(c->d_mol_spec),
-                     hb.sltj, hb.hist, hb.flceng, dp<unsigned long long>(c->d_mnmx), hb.status, first);
+                     dp<int>(c->d_specoff), dp<int>(c->d_uvmax), dp<int>(c->d_uvsoft), hb.mol_spec, hb.sltj, hb.hist,
+                     hb.flceng, dp<unsigned long long>(c->d_mnmx), hb.status, first);
   TRY(launch_check(c, "k_bin"));
   hipLaunchKernelGGL(k_scalars, dim3(1), dim3(NTHR), 0, c->st, k, n, hb.uv, hb.wsol, dp<double>(c->d_escrd),
                      dp<double>(c->d_scal), dp<double>(c->d_eself), dp<unsigned long long>(c->d_mnmx), hb.w,
@@ -1009,12 +1024,7 @@ int run_histogram_at(erm_ctx *c, int n, int first, const HistBufs &hb) {
     TRY(launch_check(c, "k_intm_scal"));
   }
   TRY(mark(c, 8));
-  if (c->P.corrcal) {
-    int nt = (E + 15) / 16;
-    int tiles = nt * (nt + 1) / 2;  // tiles on and above the diagonal, one workgroup each
-    hipLaunchKernelGGL(k_corr, dim3(tiles), dim3(256), 0, c->st, n, E, hb.hist, hb.w, dp<double>(c->d_ecorr));
-    TRY(launch_check(c, "k_corr"));
-  }
+  if (!without_corr) TRY(run_corr_at(c, n, hb));
   TRY(mark(c, 9));
   return ERM_OK;
 }
@@ -1379,6 +1389,280 @@ template <typename T> int gradient_batch(erm_ctx *c, const Batch &bt, const Grad
   return ERM_OK;
 }
 
+// ---- several test-particle species on one solvent ------------------------
+// (no counterpart in the reference, which runs one solute species per run,
+// engproc.F90:89-91).  Members keep their own topology tables, bins and
+// accumulators; the first member holds the frame's solvent data and the
+// batch buffers.  A batch mixes configurations of every member in runs
+// (largest solutes first); the solute-dependent heavy stages (z columns,
+// real-space pairs) run as one persistent launch over all runs or as one
+// launch per run, the solvent-side PME stages once over the whole batch,
+// the light per-run stages per run on the same stream.
+constexpr int GROUP_MAXRUN = 16;
+}  // namespace
+
+struct erm_group {
+  std::vector<erm_ctx *> ctx;
+  int Bmax = 1024;
+  // persistent grids over all runs: bit 0 the z columns, bit 1 the pairs
+  // (a clear bit: one launch per run)
+  int persistent = 2;
+  uint64_t frame_serial = 0;
+  ermh::DBuf d_runs, d_counter;
+  GRun *h_runs = nullptr;  // pinned run tables of the batches of a call
+  int h_runs_cap = 0;
+  double stage_ms[ermh::NSTAGE_OUT] = {};
+  std::string err;
+};
+
+namespace {
+
+// members' streams replaced by the first member's for the length of a group
+// call, so every launch of the call is ordered on one stream
+struct StreamSwap {
+  erm_group *g;
+  std::vector<hipStream_t> saved;
+  explicit StreamSwap(erm_group *gg) : g(gg) {
+    for (erm_ctx *c : g->ctx) saved.push_back(c->st);
+    for (erm_ctx *c : g->ctx) c->st = g->ctx[0]->st;
+  }
+  ~StreamSwap() {
+    for (size_t i = 0; i < g->ctx.size(); ++i) g->ctx[i]->st = saved[i];
+  }
+};
+
+int gfail(erm_group *g, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g->err = buf;
+  return code;
+}
+
+// the members describe the same solvent, grid and pair parameters
+int group_compatible(erm_group *g) {
+  erm_ctx *a = g->ctx[0];
+  for (size_t i = 0; i < g->ctx.size(); ++i) {
+    erm_ctx *c = g->ctx[i];
+    if (!c->have_topo || !c->have_bins) return gfail(g, ERM_E_ARG, "member %zu lacks topology or bins", i);
+    if (c->P.slttype != 2 || c->bfcoord.empty())
+      return gfail(g, ERM_E_ARG, "member %zu is not a rigid test particle with bfcoord", i);
+    if (c->kp.ins_org != ERM_INSORG_ORIGIN || c->kp.ins_pos != ERM_INSPOS_RANDOM || c->kp.ins_rot != ERM_INSROT_RANDOM)
+      return gfail(g, ERM_E_UNSUPPORTED, "member %zu: groups insert at random positions and orientations only", i);
+    if (c->P.cltype == ERM_EL_COULOMB) return gfail(g, ERM_E_UNSUPPORTED, "groups need PME or PPPM");
+    if (c->P.device != a->P.device || c->fp32 != a->fp32 || c->P.splodr != a->P.splodr ||
+        memcmp(c->P.ms, a->P.ms, sizeof a->P.ms) != 0 || c->P.cltype != a->P.cltype ||
+        c->P.ljswitch != a->P.ljswitch || c->P.elecut != a->P.elecut || c->P.lwljcut != a->P.lwljcut ||
+        c->P.upljcut != a->P.upljcut || c->P.screen != a->P.screen ||
+        c->P.block_threshold != a->P.block_threshold || c->P.estype != a->P.estype || c->P.wgtslf != a->P.wgtslf)
+      return gfail(g, ERM_E_ARG, "member %zu: device, precision or parameters differ from member 0", i);
+    if (c->slvmax != a->slvmax || c->nvt != a->nvt || c->vtypes != a->vtypes)
+      return gfail(g, ERM_E_ARG, "member %zu: solvent differs from member 0", i);
+    for (int k = 0; k < a->slvmax; ++k) {
+      const int m = c->tagpt[k], ma = a->tagpt[k];
+      const int n = c->mol_begin[m + 1] - c->mol_begin[m];
+      if (n != a->mol_begin[ma + 1] - a->mol_begin[ma] || c->uvspec[m] != a->uvspec[ma] ||
+          c->mol_charge[m] != a->mol_charge[ma])
+        return gfail(g, ERM_E_ARG, "member %zu: solvent molecule %d differs from member 0", i, k);
+      for (int s = 0; s < n; ++s)
+        if (c->charge[c->mol_begin[m] + s] != a->charge[a->mol_begin[ma] + s] ||
+            c->ljtype[c->mol_begin[m] + s] != a->ljtype[a->mol_begin[ma] + s])
+          return gfail(g, ERM_E_ARG, "member %zu: solvent site of molecule %d differs from member 0", i, k);
+    }
+  }
+  return ERM_OK;
+}
+
+// one batch of the runs R[0..nrun) (configurations [0, n) of the batch)
+template <typename T> int group_batch(erm_group *g, const GRun *R, int nrun, const uint64_t *offset, int n,
+                                      const GRun *d_runs, int nitem_z, int nitem_p) {
+  using V = typename C2<T>::type;
+  erm_ctx *c = g->ctx[0];
+  Params k = c->kp;
+  const int S = c->slvmax, H = k.H, K2 = k.K2, K3 = k.K3, p = k.p;
+  size_t sites = 0;
+  int Emax = 0;
+  for (int r = 0; r < nrun; ++r) sites += (size_t)R[r].ncfg * R[r].ns;
+  for (erm_ctx *m : g->ctx) Emax = std::max(Emax, m->ermax);
+  const int nzg = zcol_groups(c, true, (int)sizeof(V));
+  TRY(ensure(c, c->d_spos, sizeof(double) * sites * 3));
+  TRY(ensure(c, c->d_sref, sizeof(double) * 4 * n));
+  TRY(ensure(c, c->d_sg3, sizeof(int) * sites));
+  TRY(ensure(c, c->d_sw3, sizeof(T) * sites * p));
+  TRY(ensure(c, c->d_sqv, sizeof(T) * sites));
+  TRY(ensure(c, c->d_w1h, sizeof(V) * sites * H));
+  TRY(ensure(c, c->d_w2h, sizeof(V) * sites * K2));
+  TRY(ensure(c, c->d_uv, sizeof(double) * (size_t)n * (S + 1)));
+  TRY(ensure(c, c->d_sltj, sizeof(int) * n));
+  TRY(ensure(c, c->d_selfp, sizeof(double) * (size_t)n * nzg));
+  TRY(ensure(c, c->d_Hk, sizeof(V) * (size_t)n * K3 * K2 * H));
+  TRY(ensure(c, c->d_esite, sizeof(double) * (size_t)n * c->nsites));
+  TRY(ensure(c, c->d_hist, sizeof(double) * (size_t)n * Emax));
+  TRY(ensure(c, c->d_w, sizeof(double) * n));
+  TRY(ensure(c, c->d_flceng, sizeof(double) * (size_t)n * MAXSPEC));
+  TRY(ensure(c, g->d_counter, sizeof(int) * 4));
+  TRY(mark(c, 0));
+  hipLaunchKernelGGL(k_fill<int>, dim3((n + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, dp<int>(c->d_sltj), (size_t)n,
+                     -1);
+  TRY(launch_check(c, "k_fill"));
+  // poses and solute splines, per run with the member's tables
+  for (int r = 0; r < nrun; ++r) {
+    erm_ctx *m = g->ctx[R[r].member];
+    Params km = m->kp;
+    km.ns = m->ns;
+    const size_t so = R[r].site0;
+    const size_t lpose = (m->ns <= POSE_LDS_SITES) ? sizeof(double) * 4 * m->ns : 0;
+    hipLaunchKernelGGL(k_pose, dim3(R[r].ncfg), dim3(64), lpose, c->st, km, R[r].ncfg, dp<uint64_t>(m->d_args),
+                       offset[r], dp<double>(m->d_bf), 0, dp<double>(m->d_sitemass) + m->mol_begin[m->slt_template],
+                       dp<double>(m->d_cellm), dp<double>(m->d_aggcen), dp<double>(c->d_spos) + so * 3);
+    TRY(launch_check(c, "k_pose"));
+    hipLaunchKernelGGL(k_solute_prep<T>, dim3(R[r].ncfg), dim3(128), 0, c->st, km, dp<double>(c->d_spos) + so * 3,
+                       dp<double>(m->d_qslt), dp<V>(c->d_tw1), dp<V>(c->d_tw2), dp<V>(c->d_tw3),
+                       dp<double>(c->d_sref) + 4 * (size_t)R[r].cfg0, dp<int>(c->d_sg3) + so, dp<T>(c->d_sw3) + so * p,
+                       dp<T>(c->d_sqv) + so, dp<V>(c->d_w1h) + so * H, dp<V>(c->d_w2h) + so * K2, (V *)nullptr);
+    TRY(launch_check(c, "k_solute_prep"));
+  }
+  TRY(mark(c, 1));
+  // the work counters of the persistent launches ([0] z columns, [1] pairs)
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(32), 0, c->st, dp<int>(g->d_counter), (size_t)4, 0);
+  TRY(launch_check(c, "k_fill"));
+  // z columns over every run: one persistent launch, or one launch per run
+  if (g->persistent & 1) {
+    int rz = ERM_E_UNSUPPORTED;
+    ERM_KSWITCH(K3, (rz = launch_zcol_fft_runs<T, KK>(c, k, nzg, d_runs, nrun, nitem_z, dp<int>(g->d_counter))));
+    TRY(rz);
+    TRY(launch_check(c, "k_pme_zcol_fft_runs"));
+  } else {
+    for (int r = 0; r < nrun; ++r) {
+      Params kr = k;
+      kr.ns = R[r].ns;
+      int rz = ERM_E_UNSUPPORTED;
+      ERM_KSWITCH(K3, (rz = launch_zcol_fft<T, KK>(c, kr, R[r].ncfg, nzg, (size_t)R[r].site0, (size_t)R[r].cfg0)));
+      TRY(rz);
+      TRY(launch_check(c, "k_pme_zcol_fft"));
+    }
+  }
+  TRY(mark(c, 2));
+  if (!xslab_folds_y(c)) {
+    ERM_KSWITCH(K2, (launch_ycol_reg<T, KK>(c, k, n)));
+    TRY(launch_check(c, "y columns"));
+  }
+  TRY(mark(c, 3));
+  {
+    const size_t lx = 2 * sizeof(T) * (size_t)c->xslab_Q * H * K2;
+    int rc = ERM_E_UNSUPPORTED;
+    ERM_KSWITCH(k.K1, (rc = launch_xslab<T, KK>(c, k, n, lx)));
+    TRY(rc);
+    TRY(launch_check(c, "k_pme_xslab"));
+  }
+  TRY(mark(c, 4));
+  for (int r = 0; r < nrun; ++r) {
+    erm_ctx *m = g->ctx[R[r].member];
+    Params kg = k;
+    kg.solute_charge = m->mol_charge[m->slt_template];
+    const size_t c0 = R[r].cfg0;
+    hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, R[r].ncfg), dim3(NTHR), 0, c->st, kg, c->nsites,
+                       dp<double>(c->d_esite) + c0 * c->nsites, dp<int>(c->d_zpos), dp<int>(c->d_mol_first),
+                       dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), dp<int>(c->d_sltj) + c0,
+                       dp<double>(c->d_uv) + c0 * (S + 1));
+    TRY(launch_check(c, "k_mol_sum"));
+  }
+  TRY(mark(c, 5));
+  {
+    int nsmax = 0;
+    for (int r = 0; r < nrun; ++r) nsmax = std::max(nsmax, R[r].ns);
+    const int segcap = 2 * k.nb[1] * k.nb[2];
+    const size_t lt = pair_tile_lds<T>(nsmax, c->nvt, segcap, NTHR / 64);
+    if (lt > c->lds_cap) return fail(c, ERM_E_UNSUPPORTED, "solute too large for the pair kernel's LDS");
+    if (g->persistent & 2) {
+      const void *f = (const void *)k_pair_tile_runs<T, false>;
+      TRY(set_lds(c, f, lt, "k_pair_tile_runs"));
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, NTHR, lt) != hipSuccess || occ < 1) {
+        (void)hipGetLastError();
+        occ = 1;
+      }
+      const int grid = std::min(nitem_p, std::max(1, occ * c->ncu));
+      hipLaunchKernelGGL((k_pair_tile_runs<T, false>), dim3(grid), dim3(NTHR), lt, c->st, k, dp<double>(c->d_spos),
+                         dp<double>(c->d_sref), dp<int>(c->d_cell_start), dp<int>(c->d_mol_first),
+                         dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy), dp<double>(c->d_sz),
+                         dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj), dp<double>(c->d_rm), d_runs, nrun,
+                         nitem_p, dp<int>(g->d_counter) + 1, dp<double>(c->d_uv));
+      TRY(launch_check(c, "k_pair_tile_runs"));
+    } else {
+      TRY(set_lds(c, (const void *)k_pair_tile<T, false>, lt, "k_pair_tile"));
+      for (int r = 0; r < nrun; ++r) {
+        Params kr = k;
+        kr.ns = R[r].ns;
+        const size_t c0 = R[r].cfg0;
+        hipLaunchKernelGGL((k_pair_tile<T, false>), dim3(R[r].ncfg, R[r].nsplit), dim3(NTHR), lt, c->st, kr,
+                           dp<double>(c->d_spos) + (size_t)R[r].site0 * 3, dp<double>(c->d_sref) + 4 * c0,
+                           static_cast<const T *>(R[r].q_t), static_cast<const T *>(R[r].lj_t),
+                           dp<int>(c->d_cell_start), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
+                           dp<double>(c->d_sx), dp<double>(c->d_sy), dp<double>(c->d_sz), dp<T>(c->d_sq),
+                           dp<int>(c->d_sct), dp<int>(c->d_sltj) + c0, dp<double>(c->d_rm), R[r].lj64, R[r].q64,
+                           dp<double>(c->d_uv) + c0 * (S + 1));
+        TRY(launch_check(c, "k_pair_tile"));
+      }
+    }
+  }
+  TRY(mark(c, 6));
+  // self terms: the member's realcal_self from its first configuration
+  // (reused for the rigid solute, engproc.F90:712-730), then uvengy(0)
+  for (int r = 0; r < nrun; ++r) {
+    erm_ctx *m = g->ctx[R[r].member];
+    Params km = m->kp;
+    km.ns = m->ns;
+    km.solute_charge = m->mol_charge[m->slt_template];
+    TRY(ensure(m, m->d_usreal, sizeof(double)));
+    if (!m->have_usreal) {
+      hipLaunchKernelGGL(k_selfreal, dim3(1), dim3(NTHR), 0, c->st, km, dp<double>(c->d_spos) + (size_t)R[r].site0 * 3,
+                         0, dp<double>(m->d_qslt), dp<double>(m->d_usreal));
+      TRY(launch_check(c, "k_selfreal"));
+      m->have_usreal = true;
+    }
+    const size_t c0 = R[r].cfg0;
+    hipLaunchKernelGGL(k_u0, dim3((R[r].ncfg + 63) / 64), dim3(64), 0, c->st, km, R[r].ncfg, nzg,
+                       dp<double>(c->d_selfp) + c0 * nzg, dp<double>(m->d_usreal), 0,
+                       dp<double>(c->d_uv) + c0 * (S + 1));
+    TRY(launch_check(c, "k_u0"));
+  }
+  TRY(mark(c, 7));
+  // update_histogram per run into the member's accumulators, then the
+  // correlation matrices
+  HistBufs hbs[GROUP_MAXRUN];
+  for (int r = 0; r < nrun; ++r) {
+    erm_ctx *m = g->ctx[R[r].member];
+    const size_t c0 = R[r].cfg0;
+    HistBufs hb;
+    hb.uv = dp<double>(c->d_uv) + c0 * (S + 1);
+    hb.wsol = dp<double>(c->d_wsol);  // ones
+    hb.hist = dp<double>(c->d_hist) + c0 * Emax;
+    hb.w = dp<double>(c->d_w) + c0;
+    hb.flceng = dp<double>(c->d_flceng) + c0 * MAXSPEC;
+    hb.wi = nullptr;
+    hb.sltj = dp<int>(c->d_sltj) + c0;
+    hb.status = dp<int>(c->d_status);
+    hb.mol_spec = dp<int>(c->d_mol_spec);  // the energies are in member 0's sorted order
+    hipEvent_t *keep = m->cur_ev;
+    m->cur_ev = nullptr;
+    const int st = run_histogram_at(m, R[r].ncfg, (int)offset[r], hb, true);
+    m->cur_ev = keep;
+    if (st) return fail(c, st, "member %d: %s", R[r].member, m->err.c_str());
+    hbs[r] = hb;
+  }
+  TRY(mark(c, 8));
+  for (int r = 0; r < nrun; ++r) {
+    erm_ctx *m = g->ctx[R[r].member];
+    const int st = run_corr_at(m, R[r].ncfg, hbs[r]);
+    if (st) return fail(c, st, "member %d: %s", R[r].member, m->err.c_str());
+  }
+  TRY(mark(c, 9));
+  return ERM_OK;
+}
+
 }  // namespace
 
 // ===================================================================== C-ABI
@@ -1485,6 +1769,10 @@ int32_t erm_create(const erm_params *prm, erm_ctx **out) {
       optin = 0;
     (void)hipGetLastError();
     c->lds_cap = std::max<size_t>(64 * 1024, std::min<size_t>(150 * 1024, optin > 8192 ? (size_t)optin - 8192 : 0));
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P.device) == hipSuccess && ncu > 0)
+      c->ncu = ncu;
+    (void)hipGetLastError();
   }
   if (hipStreamCreate(&c->st) != hipSuccess) {
     delete c;
@@ -1711,7 +1999,9 @@ int32_t erm_set_bins(erm_ctx *c, int32_t numslv, const int32_t *uvmax, const int
     c->specoff[i] = E;
     E += uvmax[i];
   }
-  if ((size_t)E * 12 > 120 * 1024) return fail(c, ERM_E_UNSUPPORTED, "ermax %d too large", E);
+  // k_bin holds the edges, the counts and per-thread species sums in LDS
+  if ((size_t)E * 12 + (size_t)numslv * NTHR * 8 > c->lds_cap)
+    return fail(c, ERM_E_UNSUPPORTED, "ermax %d with %d species too large", E, numslv);
   c->ermax = E;
   c->uvcrd.assign(uvcrd, uvcrd + E);
   c->esmax = c->P.selfcal ? esmax : 0;
@@ -2073,6 +2363,189 @@ int32_t erm_insert_random_multi(erm_ctx *const *ctxs, int32_t nctx, const int32_
   return first_err;
 }
 
+// ---- several test-particle species on one solvent (erm_group_*) ---------
+int32_t erm_group_create(erm_ctx *const *ctxs, int32_t nctx, erm_group **out) {
+  if (!ctxs || nctx < 1 || nctx > GROUP_MAXRUN || !out) return ERM_E_ARG;
+  *out = nullptr;
+  for (int i = 0; i < nctx; ++i) {
+    if (!ctxs[i]) return ERM_E_ARG;
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(ctxs[i], ERM_E_ARG, "a context appears twice in the group");
+  }
+  erm_group *g = new erm_group();
+  g->ctx.assign(ctxs, ctxs + nctx);
+  const int st = group_compatible(g);
+  if (st) {
+    ctxs[0]->err = g->err;
+    delete g;
+    return st;
+  }
+  *out = g;
+  return ERM_OK;
+}
+
+void erm_group_destroy(erm_group *g) {
+  if (!g) return;
+  if (hipSetDevice(g->ctx[0]->P.device) == hipSuccess) {
+    g->d_runs.release();
+    g->d_counter.release();
+    if (g->h_runs) (void)hipHostFree(g->h_runs);
+  }
+  delete g;
+}
+
+const char *erm_group_last_error(const erm_group *g) { return g ? g->err.c_str() : "null group"; }
+
+static int group_set_frame(erm_group *g, const double *pos, bool device_ptr, const double *cell, double w) {
+  if (!g || !pos) return ERM_E_ARG;
+  erm_ctx *c = g->ctx[0];
+  int st = set_frame_impl(c, pos, device_ptr, cell, w);
+  if (st) return gfail(g, st, "member 0: %s", c->err.c_str());
+  for (size_t i = 1; i < g->ctx.size(); ++i) {
+    erm_ctx *m = g->ctx[i];
+    st = set_frame_common(m, cell, w);
+    if (st) return gfail(g, st, "member %zu: %s", i, m->err.c_str());
+    // the solvent's device data of this frame live in member 0 only
+    m->have_frame = false;
+    m->jof_stale = true;
+    // the member's cell upload (its own stream) precedes the group's launches
+    if (hipStreamSynchronize(m->st) != hipSuccess) {
+      (void)hipGetLastError();
+      return gfail(g, ERM_E_DEVICE, "member %zu: stream synchronisation", i);
+    }
+  }
+  g->frame_serial = c->frame_serial;
+  return ERM_OK;
+}
+
+int32_t erm_group_set_frame(erm_group *g, const double *sitepos, const double *cell, double w) {
+  return group_set_frame(g, sitepos, false, cell, w);
+}
+
+int32_t erm_group_set_frame_device(erm_group *g, const double *d_sitepos, const double *cell, double w) {
+  return group_set_frame(g, d_sitepos, true, cell, w);
+}
+
+int32_t erm_group_set_tuning(erm_group *g, int32_t max_batch, int32_t persistent) {
+  if (!g) return ERM_E_ARG;
+  if (max_batch < 0 || max_batch > 65536 || persistent < -1 || persistent > 3)
+    return gfail(g, ERM_E_ARG, "group tuning out of range");
+  if (max_batch > 0) g->Bmax = max_batch;
+  if (persistent >= 0) g->persistent = persistent;
+  return ERM_OK;
+}
+
+int32_t erm_group_stage_times(const erm_group *g, double *ms10) {
+  if (!g || !ms10) return ERM_E_ARG;
+  for (int i = 0; i < NSTAGE_OUT; ++i) ms10[i] = g->stage_ms[i];
+  return ERM_OK;
+}
+
+int32_t erm_group_insert_random(erm_group *g, const int32_t *n, const uint64_t *seed, const uint64_t *first_index) {
+  if (!g || !n || !seed || !first_index) return ERM_E_ARG;
+  erm_ctx *c = g->ctx[0];
+  const int nm = (int)g->ctx.size();
+  if (hipSetDevice(c->P.device) != hipSuccess) return gfail(g, ERM_E_DEVICE, "hipSetDevice");
+  if (!c->have_frame || g->frame_serial != c->frame_serial)
+    return gfail(g, ERM_E_ARG, "erm_group_set_frame has not been called for the current frame");
+  for (int i = 0; i < nm; ++i)
+    if (n[i] < 0) return gfail(g, ERM_E_ARG, "negative count for member %d", i);
+  const bool ok_path = (c->fp32 ? reg_path_ok<float>(c) : reg_path_ok<double>(c)) && zcol_fft(c, true) &&
+                       c->xslab_Q > 0 && (c->P.pme_path == 0 || c->P.pme_path == 3);
+  if (!ok_path) return gfail(g, ERM_E_UNSUPPORTED, "groups need the register PME path with spline order 4 or 6");
+  for (int i = 0; i < NSTAGE_OUT; ++i) g->stage_ms[i] = 0;
+  int64_t N = 0;
+  for (int i = 0; i < nm; ++i) N += n[i];
+  if (N == 0) return ERM_OK;
+  StreamSwap swap(g);
+  const int B = std::max(1, g->Bmax);
+  const int nb = (int)((N + B - 1) / B);
+  // members by decreasing size: the heavy work items are handed out first
+  std::vector<int> order(nm);
+  for (int i = 0; i < nm; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->ctx[a]->ns > g->ctx[b]->ns; });
+  int st = ensure(c, c->d_status, 64);
+  if (!st) {
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(32), 0, c->st, dp<int>(c->d_status), (size_t)8, 0);
+    st = launch_check(c, "k_fill");
+  }
+  for (int i = 0; i < nm && !st; ++i) st = upload_args(g->ctx[i], seed[i], first_index[i]);
+  if (!st) st = ensure(c, c->d_wsol, sizeof(double) * B);
+  if (!st) {
+    hipLaunchKernelGGL(k_fill<double>, dim3((B + NTHR - 1) / NTHR), dim3(NTHR), 0, c->st, dp<double>(c->d_wsol),
+                       (size_t)B, 1.0);
+    st = launch_check(c, "k_fill");
+    c->wsol_ones = 0;  // the ones may be overwritten by a weighted call of member 0
+  }
+  if (!st) st = batch_events(c, nb);
+  const int need_runs = nb * GROUP_MAXRUN;
+  if (!st && g->h_runs_cap < need_runs) {
+    if (g->h_runs) (void)hipHostFree(g->h_runs);
+    g->h_runs = nullptr;
+    g->h_runs_cap = 0;
+    if (hipHostMalloc((void **)&g->h_runs, sizeof(GRun) * need_runs, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      st = fail(c, ERM_E_DEVICE, "hipHostMalloc of the run tables");
+    } else {
+      g->h_runs_cap = need_runs;
+    }
+  }
+  if (!st) st = ensure(c, g->d_runs, sizeof(GRun) * need_runs);
+  const Params &k = c->kp;
+  const int vb = c->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
+  const int nzg = zcol_groups(c, true, vb);
+  for (int bi = 0; bi < nb && !st; ++bi) {
+    GRun *R = g->h_runs + (size_t)bi * GROUP_MAXRUN;
+    uint64_t offset[GROUP_MAXRUN];
+    int nrun = 0, cfg = 0, site = 0, zi = 0, pi = 0;
+    for (int oi = 0; oi < nm; ++oi) {
+      const int t = order[oi];
+      erm_ctx *m = g->ctx[t];
+      const int64_t a = (int64_t)n[t] * bi / nb, e = (int64_t)n[t] * (bi + 1) / nb;
+      const int cnt = (int)(e - a);
+      if (cnt <= 0) continue;
+      const double Rg = k.rcmax + m->slt_radius + c->mol_radius + 0.5 * (k.hcell[0] + k.hcell[1] + k.hcell[2]) / 3.0;
+      const double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
+      const int nchunk = (int)std::ceil(frac * c->slvmax / 64.0);
+      const int nsplit = std::max(1, std::min(64, (nchunk + NTHR / 64 - 1) / (NTHR / 64)));
+      GRun &r = R[nrun];
+      r.cfg0 = cfg;
+      r.ncfg = cnt;
+      r.ns = m->ns;
+      r.site0 = site;
+      r.zitem0 = zi;
+      r.pitem0 = pi;
+      r.nsplit = nsplit;
+      r.member = t;
+      r.q_t = m->d_qslt_t.p;
+      r.lj_t = m->d_sltlj.p;
+      r.q64 = dp<double>(m->d_qslt);
+      r.lj64 = dp<double>(m->d_sltlj64);
+      offset[nrun] = (uint64_t)a;
+      cfg += cnt;
+      site += cnt * m->ns;
+      zi += cnt * nzg;
+      pi += cnt * nsplit;
+      ++nrun;
+    }
+    GRun *d_runs = dp<GRun>(g->d_runs) + (size_t)bi * GROUP_MAXRUN;
+    if (hipMemcpyAsync(d_runs, R, sizeof(GRun) * nrun, hipMemcpyHostToDevice, c->st) != hipSuccess) {
+      (void)hipGetLastError();
+      st = fail(c, ERM_E_DEVICE, "run table upload");
+      break;
+    }
+    c->cur_ev = &c->bev[NSTAGE_EV * bi];
+    st = c->fp32 ? group_batch<float>(g, R, nrun, offset, cfg, d_runs, zi, pi)
+                 : group_batch<double>(g, R, nrun, offset, cfg, d_runs, zi, pi);
+  }
+  c->cur_ev = nullptr;
+  if (!st) st = check_status(c);
+  if (!st) st = stage_times(c, nb, true);
+  if (st) return gfail(g, st, "%s", c->err.c_str());
+  for (int i = 0; i < NSTAGE_OUT; ++i) g->stage_ms[i] = c->stage_ms[i];
+  return ERM_OK;
+}
+
 int32_t erm_update_histogram(erm_ctx *c, int32_t n, const int32_t *solutes, const double *weights,
                              const double *uvengy) {
   TRY(need(c, true, true));
@@ -2160,6 +2633,7 @@ int32_t erm_update_histogram_static(erm_ctx *c, int32_t n, const int32_t *solute
   hb.wi = reinterpret_cast<double *>(carve(sizeof(double) * n));
   hb.sltj = reinterpret_cast<int *>(carve(sizeof(int) * n));
   int *sol = solutes ? reinterpret_cast<int *>(carve(sizeof(int) * n)) : nullptr;
+  hb.mol_spec = dp<int>(c->d_mol_spec);
   if (off > SCRATCH_BYTES) return fail(c, ERM_E_ENG_BUG, "static scratch layout overflows");
   // the array keeps the previous call's contents: the status starts at zero
   hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(32), 0, c->st, status, (size_t)16, 0);

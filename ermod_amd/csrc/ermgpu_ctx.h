@@ -103,6 +103,7 @@ struct erm_ctx {
   int zcol_T = erm::NTHR;            // threads per z-column workgroup, 256 or 512 (autotuned)
   bool xslab_yf = false;        // inverse y transform inside the x slab (autotuned)
   int pair_path = 0;            // 0 tiled pair kernel, 1 one thread per candidate molecule
+  int ncu = 256;                // compute units of the device (persistent grids)
   double slt_radius = 8.0;      // test particle's largest site distance from its centroid
   double mol_radius = 2.0;      // solvent molecules' largest site distance from the first site
   std::vector<int> zst;         // first z-sorted site of every plane (host copy)
@@ -251,7 +252,13 @@ inline bool xslab_folds_y(const erm_ctx *c) { return c->xslab_yf && c->xslab_T =
 // function, or the spectral z sum), inverse y columns, the x slab with the
 // gather, and the solvent-potential transforms of the adjoint.
 template <typename T, int K> int launch_zcol_reg(erm_ctx *c, const Params &k, int n, int nblk);
-template <typename T, int K> int launch_zcol_fft(erm_ctx *c, const Params &k, int n, int nblk);
+// (launch_zcol_fft from site site0 and configuration cfg0 of the batch
+// buffers, k.ns sites per configuration; launch_zcol_fft_runs over a run
+// table with a persistent grid)
+template <typename T, int K>
+int launch_zcol_fft(erm_ctx *c, const Params &k, int n, int nblk, size_t site0 = 0, size_t cfg0 = 0);
+template <typename T, int K>
+int launch_zcol_fft_runs(erm_ctx *c, const Params &k, int nblk, const GRun *runs, int nrun, int nitem, int *counter);
 template <typename T, int K> void launch_ycol_reg(erm_ctx *c, const Params &k, int n);
 template <typename T, int K> int launch_xslab(erm_ctx *c, const Params &k, int n, size_t lds);
 template <typename T, int K> void launch_xr2c(erm_ctx *c, const Params &k);

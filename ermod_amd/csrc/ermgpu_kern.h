@@ -912,67 +912,100 @@ __device__ __forceinline__ void pair_twiddle(V (&Y)[M], T hs, std::integer_seque
   (one(Y[J], root_v<V, SIGN, J, 2 * M>()), ...);
 }
 
-// Y += a w[k] at the PP planes (G - k) mod K3 of a site at plane G, on the
-// lane pair that splits the column (lane h owns planes [h K3/2, (h+1) K3/2))
-// (hm[o] is 1 on the lane that owns half o, else 0: one multiply selects
-// the weight)
+// Y += a w[k] at the PP planes (G - k) mod K3 of a site at plane G, on
+// the lane pair that splits the column (lane h owns planes [h K3/2,
+// (h+1) K3/2)); hm[o] is 1 on the lane that owns half o, else 0.  When the
+// PP planes lie in one half (all but the buckets next to a half's edge)
+// one multiply masks a for every plane; else each weight is masked.
 template <int K3, int G, typename T, typename V, int... Kk>
 __device__ __forceinline__ void zspread_half(V (&Y)[K3 / 2], const V a, const T (&w)[sizeof...(Kk)],
                                              const T (&hm)[2], std::integer_sequence<int, Kk...>) {
   constexpr int HK = K3 / 2;
-  auto one = [&](V &y, T hmo, T wk) {
-    const T ww = hmo * wk;
-    y.x = fma(a.x, ww, y.x);
-    y.y = fma(a.y, ww, y.y);
-  };
-  (one(Y[((G - Kk + K3) % K3) % HK], hm[((G - Kk + K3) % K3) / HK], w[Kk]), ...);
+  constexpr int h0 = G / HK;
+  constexpr bool same = ((((G - Kk + K3) % K3) / HK == h0) && ...);
+  if constexpr (same) {
+    V am;
+    am.x = a.x * hm[h0];
+    am.y = a.y * hm[h0];
+    auto one = [&](V &y, T wk) {
+      y.x = fma(am.x, wk, y.x);
+      y.y = fma(am.y, wk, y.y);
+    };
+    (one(Y[((G - Kk + K3) % K3) % HK], w[Kk]), ...);
+  } else {
+    auto one = [&](V &y, T hmo, T wk) {
+      const T ww = hmo * wk;
+      y.x = fma(a.x, ww, y.x);
+      y.y = fma(a.y, ww, y.y);
+    };
+    (one(Y[((G - Kk + K3) % K3) % HK], hm[((G - Kk + K3) % K3) / HK], w[Kk]), ...);
+  }
 }
 
-// dynamic LDS of k_pme_zcol_fft with XT threads: ZREG_SCH sites x (H + K2)
-// spectra + charges + the self-energy reduction
+// k_pme_zcol_fft stages its sites in chunks of ZFFT_SCH, sorted by z plane
+constexpr int ZFFT_SCH = 64;
+constexpr int ZFFT_KMAX = 96;  // largest z grid length of the register kernels
+
+// m2 rows a workgroup of XT threads (XT / 2 columns, m1 fastest) touches
+__host__ __device__ inline int zcol_fft_rows(int H, int K2, int XT) {
+  const int r = (XT / 2 - 1) / H + 2;
+  return r < K2 ? r : K2;
+}
+
+// dynamic LDS of k_pme_zcol_fft with XT threads: ZFFT_SCH sites x (H + the
+// block's m2 rows) spectra, charge-scaled z weights, planes and ranks, the
+// bucket starts, and the self-energy reduction
 template <typename T> __host__ __device__ inline size_t zcol_fft_lds(int H, int K2, int XT = NTHR) {
-  return (size_t)ZREG_SCH * (H + K2) * 2 * sizeof(T) + ZREG_SCH * sizeof(T) + XT * sizeof(double);
+  const size_t spec = (size_t)ZFFT_SCH * (H + zcol_fft_rows(H, K2, XT)) * 2 * sizeof(T);
+  const size_t small = (size_t)ZFFT_SCH * (sizeof(T) * MAXP + 2 * sizeof(int)) + (ZFFT_KMAX + 1) * sizeof(int);
+  return ((spec + small + 15) & ~(size_t)15) + XT * sizeof(double);
 }
-
-#define ERM_ZC(G)                                                                            \
-  case G:                                                                                    \
-    if constexpr (G < K3) zspread_half<K3, G, T>(Y, a, w, hm, std::make_integer_sequence<int, PP>{}); \
-    break;
 
 // Column (m1, m2) of the same product as k_pme_zcol_reg, with the z axis
 // spread and transformed instead of summed spectrally, on a lane pair that
 // holds half of the column each:
 //  - every site adds q_s what1_s(m1) what2_s(m2) w3_s,k at its PP planes
-//    (g3 - k) mod K3 (the reference's spread along z, recpcal.F90:451-470);
-//    a site's plane and weights are the same for every lane (scalar loads)
-//    and select one of K3 unrolled register updates (a switch on a uniform
-//    value: scalar branches);
+//    (g3 - k) mod K3 (the reference's spread along z, recpcal.F90:451-470).
+//    The sites of a chunk are sorted by plane g3 in LDS; one loop per plane
+//    (unrolled over the K3 planes, each over its own sites) updates the
+//    PP registers of that plane at compile-time indices, so the column
+//    stays in place in registers;
 //  - forward z FFT (the r2c transform's last axis): one radix-2 step across
 //    the pair, then a K3/2 FFT per lane (lane h: frequencies 2m + h);
 //  - Green's function and self energy (recpcal.F90:377-398);
 //  - inverse z FFT: a K3/2 FFT per lane, then the radix-2 step across the
 //    pair (lane h: planes h K3/2 + n) -> Hk.
-template <typename T, int K3, int PP, int XT = NTHR>
-__global__ __launch_bounds__(XT) void k_pme_zcol_fft(Params P, const T *__restrict__ engfac,
-                                                       const T *__restrict__ sqv,
-                                                       const typename C2<T>::type *__restrict__ w1h,
-                                                       const typename C2<T>::type *__restrict__ w2h,
-                                                       const int *__restrict__ sg3, const T *__restrict__ sw3,
-                                                       typename C2<T>::type *Hk, double *selfp) {
+// Column block blk of one configuration (ns sites; the site arrays start at
+// the configuration's first site, Hk at its spectrum); the block's
+// self-energy sum goes to *self_out.  All threads of the workgroup call it.
+template <typename T, int K3, int PP, int XT>
+__device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, const T *__restrict__ engfac,
+                                              const T *__restrict__ sqv,
+                                              const typename C2<T>::type *__restrict__ w1h,
+                                              const typename C2<T>::type *__restrict__ w2h,
+                                              const int *__restrict__ sg3, const T *__restrict__ sw3,
+                                              typename C2<T>::type *Hk, double *self_out) {
+  static_assert(K3 <= ZFFT_KMAX, "z grid length");
   using V = typename C2<T>::type;
   constexpr int HK = K3 / 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int b = blockIdx.y, ns = P.ns, H = P.H, K2 = P.K2;
-  const int nsp = H + K2;
-  V *l1 = reinterpret_cast<V *>(lds_raw);  // [ZREG_SCH][H]
-  V *l2 = l1 + ZREG_SCH * H;               // [ZREG_SCH][K2]
-  T *lq = reinterpret_cast<T *>(l2 + ZREG_SCH * K2);
+  const int H = P.H, K2 = P.K2;
+  const int rows = zcol_fft_rows(H, K2, XT);
+  V *l1 = reinterpret_cast<V *>(lds_raw);           // [ZFFT_SCH][H], sorted order
+  V *l2 = l1 + ZFFT_SCH * H;                        // [ZFFT_SCH][nm2] the block's m2 rows
+  T *lw = reinterpret_cast<T *>(l2 + ZFFT_SCH * rows);  // [ZFFT_SCH][PP] z weights x charge
+  int *lg = reinterpret_cast<int *>(lw + ZFFT_SCH * MAXP);  // [ZFFT_SCH] planes, chunk order
+  int *lr = lg + ZFFT_SCH;                          // [ZFFT_SCH] rank of each site by plane
+  int *lst = lr + ZFFT_SCH;                         // [K3 + 1] first sorted site of each plane
   double *red = reinterpret_cast<double *>(lds_raw + zcol_fft_lds<T>(H, K2, XT) - XT * sizeof(double));
   const int ncol = H * K2;
   const int h = threadIdx.x & 1;
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+  const int c = (blk * XT + (int)threadIdx.x) >> 1;
   const bool act = c < ncol;
-  const int m1 = act ? c % H : 0, m2 = act ? c / H : 0;
+  const int m1 = act ? c % H : 0;
+  const int m2lo = (blk * (XT / 2)) / H, m2hi = (min(ncol, (blk + 1) * (XT / 2)) - 1) / H;
+  const int nm2 = m2hi - m2lo + 1;
+  const int m2 = act ? c / H - m2lo : 0;
   const T sg = h ? (T)-1 : (T)1, hs = (T)h;
   const T hm[2] = {(T)(1 - h), (T)h};
   V Y[HK];
@@ -981,45 +1014,60 @@ __global__ __launch_bounds__(XT) void k_pme_zcol_fft(Params P, const T *__restri
     Y[z].x = 0;
     Y[z].y = 0;
   }
-  for (int s0 = 0; s0 < ns; s0 += ZREG_SCH) {
-    const int nc = min(ZREG_SCH, ns - s0);
+  const int nsp = H + nm2;
+  // the sites of plane G (bucket [lst[G], lst[G + 1]) of the sorted chunk)
+  // add to the registers of planes G, G - 1, ..., G - PP + 1
+  auto bucket = [&](auto gc) {
+    constexpr int G = decltype(gc)::value;
+    const int b0 = __builtin_amdgcn_readfirstlane(lst[G]);
+    const int b1 = __builtin_amdgcn_readfirstlane(lst[G + 1]);
+    for (int i = b0; i < b1; ++i) {
+      const V a = cmul(l1[i * H + m1], l2[i * nm2 + m2]);
+      T w[PP];
+#pragma unroll
+      for (int k = 0; k < PP; ++k) w[k] = lw[i * PP + k];  // one broadcast read
+      zspread_half<K3, G, T>(Y, a, w, hm, std::make_integer_sequence<int, PP>{});
+    }
+  };
+  for (int s0 = 0; s0 < ns; s0 += ZFFT_SCH) {
+    const int nc = min(ZFFT_SCH, ns - s0);
+    __syncthreads();  // the previous chunk is consumed
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) lg[t] = sg3[(size_t)s0 + t];
+    __syncthreads();
+    // rank by (plane, index); bucket starts = sites on lower planes (a
+    // plane outside [0, K3) falls in no bucket)
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+      const int g = lg[t];
+      int r = 0;
+      for (int u = 0; u < nc; ++u) {
+        const int gu = lg[u];
+        r += (gu < g) || (gu == g && u < t);
+      }
+      lr[t] = r;
+    }
+    for (int t = threadIdx.x; t <= K3; t += blockDim.x) {
+      int r = 0;
+      for (int u = 0; u < nc; ++u) r += lg[u] < t;
+      lst[t] = r;
+    }
     __syncthreads();
     for (int t = threadIdx.x; t < nc * nsp; t += blockDim.x) {
       const int sl = t / nsp, m = t - sl * nsp;
-      const size_t bs = (size_t)b * ns + s0 + sl;
+      const size_t bs = (size_t)s0 + sl;
+      const int r = lr[sl];
       if (m < H)
-        l1[sl * H + m] = w1h[bs * H + m];
+        l1[r * H + m] = w1h[bs * H + m];
       else
-        l2[sl * K2 + (m - H)] = w2h[bs * K2 + (m - H)];
+        l2[r * nm2 + (m - H)] = w2h[bs * K2 + m2lo + (m - H)];
     }
-    for (int t = threadIdx.x; t < nc; t += blockDim.x) lq[t] = sqv[(size_t)b * ns + s0 + t];
+    for (int t = threadIdx.x; t < nc * PP; t += blockDim.x) {
+      const int sl = t / PP, k = t - sl * PP;
+      lw[lr[sl] * PP + k] = sqv[(size_t)s0 + sl] * sw3[(size_t)s0 * PP + t];
+    }
     __syncthreads();
-    for (int sl = 0; sl < nc; ++sl) {
-      V a = cmul(l1[sl * H + m1], l2[sl * K2 + m2]);
-      const T q = lq[sl];
-      a.x *= q;
-      a.y *= q;
-      const size_t bs = (size_t)b * ns + s0 + sl;
-      const int g3 = __builtin_amdgcn_readfirstlane(sg3[bs]);
-      T w[PP];
-#pragma unroll
-      for (int k = 0; k < PP; ++k) w[k] = sw3[bs * PP + k];
-      switch (g3) {
-        ERM_ZC(0) ERM_ZC(1) ERM_ZC(2) ERM_ZC(3) ERM_ZC(4) ERM_ZC(5) ERM_ZC(6) ERM_ZC(7)
-        ERM_ZC(8) ERM_ZC(9) ERM_ZC(10) ERM_ZC(11) ERM_ZC(12) ERM_ZC(13) ERM_ZC(14) ERM_ZC(15)
-        ERM_ZC(16) ERM_ZC(17) ERM_ZC(18) ERM_ZC(19) ERM_ZC(20) ERM_ZC(21) ERM_ZC(22) ERM_ZC(23)
-        ERM_ZC(24) ERM_ZC(25) ERM_ZC(26) ERM_ZC(27) ERM_ZC(28) ERM_ZC(29) ERM_ZC(30) ERM_ZC(31)
-        ERM_ZC(32) ERM_ZC(33) ERM_ZC(34) ERM_ZC(35) ERM_ZC(36) ERM_ZC(37) ERM_ZC(38) ERM_ZC(39)
-        ERM_ZC(40) ERM_ZC(41) ERM_ZC(42) ERM_ZC(43) ERM_ZC(44) ERM_ZC(45) ERM_ZC(46) ERM_ZC(47)
-        ERM_ZC(48) ERM_ZC(49) ERM_ZC(50) ERM_ZC(51) ERM_ZC(52) ERM_ZC(53) ERM_ZC(54) ERM_ZC(55)
-        ERM_ZC(56) ERM_ZC(57) ERM_ZC(58) ERM_ZC(59) ERM_ZC(60) ERM_ZC(61) ERM_ZC(62) ERM_ZC(63)
-        ERM_ZC(64) ERM_ZC(65) ERM_ZC(66) ERM_ZC(67) ERM_ZC(68) ERM_ZC(69) ERM_ZC(70) ERM_ZC(71)
-        ERM_ZC(72) ERM_ZC(73) ERM_ZC(74) ERM_ZC(75) ERM_ZC(76) ERM_ZC(77) ERM_ZC(78) ERM_ZC(79)
-        ERM_ZC(80) ERM_ZC(81) ERM_ZC(82) ERM_ZC(83) ERM_ZC(84) ERM_ZC(85) ERM_ZC(86) ERM_ZC(87)
-        ERM_ZC(88) ERM_ZC(89) ERM_ZC(90) ERM_ZC(91) ERM_ZC(92) ERM_ZC(93) ERM_ZC(94) ERM_ZC(95)
-        default: break;
-      }
-    }
+    [&]<int... Gs>(std::integer_sequence<int, Gs...>) {
+      (bucket(std::integral_constant<int, Gs>{}), ...);
+    }(std::make_integer_sequence<int, K3>{});
   }
   // forward: radix-2 step across the pair, K3/2 FFT per lane
   pair_steps<T, HK, -1>(Y, sg, hs, std::make_integer_sequence<int, HK>{});
@@ -1054,17 +1102,81 @@ __global__ __launch_bounds__(XT) void k_pme_zcol_fft(Params P, const T *__restri
     if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) selfp[(size_t)b * gridDim.x + blockIdx.x] = red[0];
+  if (threadIdx.x == 0) *self_out = red[0];
   if (act) {
     const unsigned hoff = (unsigned)(h * HK * ncol + c);
 #pragma unroll
     for (int n = 0; n < HK; ++n) {
-      V *hrow = Hk + ((size_t)b * K3 + n) * ncol;
+      V *hrow = Hk + (size_t)n * ncol;
       hrow[hoff] = Y[n];
     }
   }
 }
-#undef ERM_ZC
+
+template <typename T, int K3, int PP, int XT = NTHR>
+__global__ __launch_bounds__(XT) void k_pme_zcol_fft(Params P, const T *__restrict__ engfac,
+                                                       const T *__restrict__ sqv,
+                                                       const typename C2<T>::type *__restrict__ w1h,
+                                                       const typename C2<T>::type *__restrict__ w2h,
+                                                       const int *__restrict__ sg3, const T *__restrict__ sw3,
+                                                       typename C2<T>::type *Hk, double *selfp) {
+  const int b = blockIdx.y, ns = P.ns;
+  const size_t so = (size_t)b * ns, ncol = (size_t)P.H * P.K2;
+  zcol_fft_body<T, K3, PP, XT>(P, ns, blockIdx.x, engfac, sqv + so, w1h + so * P.H, w2h + so * P.K2, sg3 + so,
+                               sw3 + so * PP, Hk + (size_t)b * K3 * ncol, selfp + (size_t)b * gridDim.x + blockIdx.x);
+}
+
+// One run of configurations of one solute type within a batch of several
+// (the heterogeneous insertion batch): configurations [cfg0, cfg0 + ncfg)
+// of the batch, ns sites each from site site0 of the batch's site arrays;
+// the work items of the persistent kernels the run owns start at zitem0
+// (z columns: ncfg x column blocks) and pitem0 (pairs: ncfg x nsplit); the
+// type's charges and LJ rows in T and in FP64; member: the host's index of
+// the type.
+struct GRun {
+  int cfg0, ncfg, ns, site0;
+  int zitem0, pitem0, nsplit, member;
+  const void *q_t, *lj_t;
+  const double *q64, *lj64;
+};
+
+// Work items of the persistent kernels: a workgroup takes ITEM_CHUNK
+// consecutive items per atomic (the same for every thread: the barrier
+// after the read keeps the slot until all have it) until the counter
+// passes nitem.
+constexpr int ITEM_CHUNK = 4;
+__device__ __forceinline__ int next_items(int *counter) {
+  __shared__ int item_sh;
+  __syncthreads();
+  if (threadIdx.x == 0) item_sh = atomicAdd(counter, ITEM_CHUNK);
+  __syncthreads();
+  return item_sh;
+}
+
+// z columns of a heterogeneous batch as one persistent launch: workgroups
+// pull (configuration, column block) items, the runs in the order given
+// (largest solutes first), until nitem are taken
+template <typename T, int K3, int PP, int XT = NTHR>
+__global__ __launch_bounds__(XT) void k_pme_zcol_fft_runs(Params P, const T *__restrict__ engfac,
+                                                            const T *__restrict__ sqv,
+                                                            const typename C2<T>::type *__restrict__ w1h,
+                                                            const typename C2<T>::type *__restrict__ w2h,
+                                                            const int *__restrict__ sg3, const T *__restrict__ sw3,
+                                                            typename C2<T>::type *Hk, double *selfp,
+                                                            const GRun *__restrict__ runs, int nrun, int nblk,
+                                                            int nitem, int *counter) {
+  const size_t ncol = (size_t)P.H * P.K2;
+  for (int base = next_items(counter); base < nitem; base = next_items(counter))
+  for (int it = base; it < min(nitem, base + ITEM_CHUNK); ++it) {
+    int r = 0;
+    while (r + 1 < nrun && runs[r + 1].zitem0 <= it) ++r;
+    const int ns = runs[r].ns, rel = it - runs[r].zitem0;
+    const int bl = rel / nblk, blk = rel - bl * nblk, b = runs[r].cfg0 + bl;
+    const size_t so = (size_t)runs[r].site0 + (size_t)bl * ns;
+    zcol_fft_body<T, K3, PP, XT>(P, ns, blk, engfac, sqv + so, w1h + so * P.H, w2h + so * P.K2, sg3 + so,
+                                 sw3 + so * PP, Hk + (size_t)b * K3 * ncol, selfp + (size_t)b * nblk + blk);
+  }
+}
 
 // Inverse y transform of column (z, m1) in place on a lane pair: lane h
 // loads m2 in [h K2/2, (h+1) K2/2), the radix-2 step across the pair (DPP)
@@ -1811,24 +1923,26 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-template <typename T, bool BARE = false>
-__global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__restrict__ spos,
-                                                    const double *__restrict__ sref,
-                                                    const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
-                                                    const int *__restrict__ cell_start,
-                                                    const int *__restrict__ mol_first,
-                                                    const int *__restrict__ mol_ns, const double *__restrict__ sx,
-                                                    const double *__restrict__ sy, const double *__restrict__ sz,
-                                                    const T *__restrict__ sq, const int *__restrict__ sct,
-                                                    const int *__restrict__ slt_j,
-                                                    const double *__restrict__ rm_ord_p,
-                                                    const double *__restrict__ slt_lj64,
-                                                    const double *__restrict__ slt_q64, double *uv) {
+// Configuration b (ns sites at xs), split `split` of nsplit workgroups;
+// all threads of the workgroup call it
+template <typename T, bool BARE>
+__device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, int split, int nsplit,
+                                               const double *__restrict__ xs, const double *__restrict__ sref,
+                                               const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
+                                               const int *__restrict__ cell_start,
+                                               const int *__restrict__ mol_first,
+                                               const int *__restrict__ mol_ns, const double *__restrict__ sx,
+                                               const double *__restrict__ sy, const double *__restrict__ sz,
+                                               const T *__restrict__ sq, const int *__restrict__ sct,
+                                               const int *__restrict__ slt_j,
+                                               const double *__restrict__ rm_ord_p,
+                                               const double *__restrict__ slt_lj64,
+                                               const double *__restrict__ slt_q64, double *uv) {
   constexpr bool F32 = sizeof(T) == 4;
   using T4 = typename V4T<T>::type;
   using T2 = typename C2<T>::type;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int ns = P.ns, nvt = P.nvt;
+  const int nvt = P.nvt;
   T4 *us4 = reinterpret_cast<T4 *>(lds_raw);  // [ns] (x, y, z, q) offsets from the centre
   T4 *ul4 = us4 + ns;                         // [ns] low parts (x, y, z, 0)
   T2 *ulj = reinterpret_cast<T2 *>(ul4 + ns); // [ns][nvt] (eps, sig2)
@@ -1847,10 +1961,8 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
   double *wes = weq + PT_QCAP;            // [64] per-site sums of a tile
   int *whq = reinterpret_cast<int *>(wes + 64);  // [PT_QCAP] queue slots of the pairs evaluated in FP64
 
-  const int b = blockIdx.x;
   const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
   const double Rs = sref[4 * b + 3];
-  const double *xs = spos + (size_t)b * ns * 3;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     const double d0 = xs[3 * s] - C0, d1 = xs[3 * s + 1] - C1, d2 = xs[3 * s + 2] - C2v;
     T4 u, l;
@@ -1891,7 +2003,7 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
   const T cut2q = F32 ? (T)(P.cut2 * (1.0 + 1e-5)) : cut2;
   double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
   const int nchunk = (total + 63) >> 6;
-  for (int c = blockIdx.y * nw + wv; c < nchunk; c += gridDim.y * nw) {
+  for (int c = split * nw + wv; c < nchunk; c += nsplit * nw) {
     // molecule of this lane: flattened candidate c*64 + lane
     const int idx = c * 64 + lane;
     int j = -1, nsm = 0, first = 0;
@@ -2123,6 +2235,54 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
       wave_lds_sync();
     }
     if (nsm > 0) u[j] += acc;
+  }
+}
+
+template <typename T, bool BARE = false>
+__global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__restrict__ spos,
+                                                    const double *__restrict__ sref,
+                                                    const T *__restrict__ slt_q, const T *__restrict__ slt_lj,
+                                                    const int *__restrict__ cell_start,
+                                                    const int *__restrict__ mol_first,
+                                                    const int *__restrict__ mol_ns, const double *__restrict__ sx,
+                                                    const double *__restrict__ sy, const double *__restrict__ sz,
+                                                    const T *__restrict__ sq, const int *__restrict__ sct,
+                                                    const int *__restrict__ slt_j,
+                                                    const double *__restrict__ rm_ord_p,
+                                                    const double *__restrict__ slt_lj64,
+                                                    const double *__restrict__ slt_q64, double *uv) {
+  const int b = blockIdx.x;
+  pair_tile_body<T, BARE>(P, P.ns, b, blockIdx.y, gridDim.y, spos + (size_t)b * P.ns * 3, sref, slt_q, slt_lj,
+                          cell_start, mol_first, mol_ns, sx, sy, sz, sq, sct, slt_j, rm_ord_p, slt_lj64, slt_q64, uv);
+}
+
+// real space of a heterogeneous batch as one persistent launch:
+// workgroups pull (configuration, split) items, the runs in the order given
+// (largest solutes first), each with its solute type's tables
+template <typename T, bool BARE = false>
+__global__ __launch_bounds__(NTHR) void k_pair_tile_runs(Params P, const double *__restrict__ spos,
+                                                         const double *__restrict__ sref,
+                                                         const int *__restrict__ cell_start,
+                                                         const int *__restrict__ mol_first,
+                                                         const int *__restrict__ mol_ns,
+                                                         const double *__restrict__ sx,
+                                                         const double *__restrict__ sy,
+                                                         const double *__restrict__ sz, const T *__restrict__ sq,
+                                                         const int *__restrict__ sct, const int *__restrict__ slt_j,
+                                                         const double *__restrict__ rm_ord_p,
+                                                         const GRun *__restrict__ runs, int nrun, int nitem,
+                                                         int *counter, double *uv) {
+  for (int base = next_items(counter); base < nitem; base = next_items(counter))
+  for (int it = base; it < min(nitem, base + ITEM_CHUNK); ++it) {
+    __syncthreads();  // the previous item's LDS is free
+    int r = 0;
+    while (r + 1 < nrun && runs[r + 1].pitem0 <= it) ++r;
+    const int ns = runs[r].ns, nsplit = runs[r].nsplit, rel = it - runs[r].pitem0;
+    const int bl = rel / nsplit, split = rel - bl * nsplit, b = runs[r].cfg0 + bl;
+    const double *xs = spos + ((size_t)runs[r].site0 + (size_t)bl * ns) * 3;
+    pair_tile_body<T, BARE>(P, ns, b, split, nsplit, xs, sref, static_cast<const T *>(runs[r].q_t),
+                            static_cast<const T *>(runs[r].lj_t), cell_start, mol_first, mol_ns, sx, sy, sz, sq, sct,
+                            slt_j, rm_ord_p, runs[r].lj64, runs[r].q64, uv);
   }
 }
 
@@ -2373,16 +2533,19 @@ static __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__r
                                               unsigned long long *mnmx, int *status, int first) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double *e = reinterpret_cast<double *>(lds_raw);
-  int *h = reinterpret_cast<int *>(e + P.ermax);
-  __shared__ double flc[MAXSPEC];
+  // per-thread, per-species energy sums [numslv][blockDim.x] (each thread
+  // adds only to its own slots; they are summed in thread order below, so
+  // flceng does not depend on the order the threads run in)
+  double *part = e + P.ermax;
+  int *h = reinterpret_cast<int *>(part + (size_t)P.numslv * blockDim.x);
   __shared__ unsigned long long mn[MAXSPEC], mx[MAXSPEC];
   const int b = blockIdx.x;
   for (int t = threadIdx.x; t < P.ermax; t += blockDim.x) {
     e[t] = edges[t];
     h[t] = 0;
   }
+  for (int t = 0; t < P.numslv; ++t) part[(size_t)t * blockDim.x + threadIdx.x] = 0.0;
   if (threadIdx.x < MAXSPEC) {
-    flc[threadIdx.x] = 0.0;
     mn[threadIdx.x] = ~0ULL;
     mx[threadIdx.x] = 0ULL;
   }
@@ -2397,7 +2560,7 @@ static __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__r
   unsigned long long rmn = ~0ULL, rmx = 0ULL;
   auto flush_spec = [&]() {
     if (rs >= 0) {
-      atomicAdd(&flc[rs], rsum);
+      part[(size_t)rs * blockDim.x + threadIdx.x] += rsum;
       atomicMin(&mn[rs], rmn);
       atomicMax(&mx[rs], rmx);
     }
@@ -2452,7 +2615,9 @@ static __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__r
   __syncthreads();
   for (int t = threadIdx.x; t < P.ermax; t += blockDim.x) hist[(size_t)b * P.ermax + t] = (double)h[t];
   if (threadIdx.x < P.numslv) {
-    flceng[(size_t)b * MAXSPEC + threadIdx.x] = flc[threadIdx.x];
+    double sum = 0.0;
+    for (int t = 0; t < (int)blockDim.x; ++t) sum += part[(size_t)threadIdx.x * blockDim.x + t];
+    flceng[(size_t)b * MAXSPEC + threadIdx.x] = sum;
     atomicMin(&mnmx[2 * (threadIdx.x + 1)], mn[threadIdx.x]);
     atomicMax(&mnmx[2 * (threadIdx.x + 1) + 1], mx[threadIdx.x]);
   }

@@ -3,6 +3,8 @@
 // adjoint's solvent-potential transforms.  Compiled once per (ERM_T, ERM_K)
 // so that the instantiations build in parallel; ermgpu.hip dispatches on the
 // grid length at run time.
+#include <algorithm>
+
 #include "ermgpu_ctx.h"
 #include "ermgpu_grad.h"
 
@@ -28,23 +30,55 @@ template <typename T, int K> int launch_zcol_reg(erm_ctx *c, const Params &k, in
   return launch_zcol_reg_t<T, K, NTHR>(c, k, n, nblk);
 }
 template <typename T, int K, int PP, int XT>
-int launch_zcol_fft_t(erm_ctx *c, const Params &k, int n, int nblk) {
+int launch_zcol_fft_t(erm_ctx *c, const Params &k, int n, int nblk, size_t site0, size_t cfg0) {
   using V = typename C2<T>::type;
   const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT);
+  const size_t hk = cfg0 * k.K3 * (size_t)k.K2 * k.H;
   TRY(set_lds(c, (const void *)k_pme_zcol_fft<T, K, PP, XT>, lds, "k_pme_zcol_fft"));
   hipLaunchKernelGGL((k_pme_zcol_fft<T, K, PP, XT>), dim3(nblk, n), dim3(XT), lds, c->st, k, dp<T>(c->d_engfac),
-                     dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<int>(c->d_sg3), dp<T>(c->d_sw3),
-                     dp<V>(c->d_Hk), dp<double>(c->d_selfp));
+                     dp<T>(c->d_sqv) + site0, dp<V>(c->d_w1h) + site0 * k.H, dp<V>(c->d_w2h) + site0 * k.K2,
+                     dp<int>(c->d_sg3) + site0, dp<T>(c->d_sw3) + site0 * k.p, dp<V>(c->d_Hk) + hk,
+                     dp<double>(c->d_selfp) + cfg0 * nblk);
   return ERM_OK;
 }
 // z columns spread and transformed along z (spline orders 4 and 6)
-template <typename T, int K> int launch_zcol_fft(erm_ctx *c, const Params &k, int n, int nblk) {
+template <typename T, int K>
+int launch_zcol_fft(erm_ctx *c, const Params &k, int n, int nblk, size_t site0, size_t cfg0) {
   if (k.p == 4) {
-    if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 4, 512>(c, k, n, nblk);
-    return launch_zcol_fft_t<T, K, 4, NTHR>(c, k, n, nblk);
+    if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 4, 512>(c, k, n, nblk, site0, cfg0);
+    return launch_zcol_fft_t<T, K, 4, NTHR>(c, k, n, nblk, site0, cfg0);
   }
-  if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 6, 512>(c, k, n, nblk);
-  return launch_zcol_fft_t<T, K, 6, NTHR>(c, k, n, nblk);
+  if (c->zcol_T == 512) return launch_zcol_fft_t<T, K, 6, 512>(c, k, n, nblk, site0, cfg0);
+  return launch_zcol_fft_t<T, K, 6, NTHR>(c, k, n, nblk, site0, cfg0);
+}
+// z columns of a heterogeneous batch (run table in device memory): a
+// persistent grid (as many workgroups as fit the device at once) pulling
+// items through the counter
+template <typename T, int K, int PP, int XT>
+int launch_zcol_runs_t(erm_ctx *c, const Params &k, int nblk, const GRun *runs, int nrun, int nitem, int *counter) {
+  using V = typename C2<T>::type;
+  const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT);
+  const void *f = (const void *)k_pme_zcol_fft_runs<T, K, PP, XT>;
+  TRY(set_lds(c, f, lds, "k_pme_zcol_fft_runs"));
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, XT, lds) != hipSuccess || occ < 1) {
+    (void)hipGetLastError();
+    occ = 1;
+  }
+  const int grid = std::min(nitem, std::max(1, occ * c->ncu));
+  hipLaunchKernelGGL((k_pme_zcol_fft_runs<T, K, PP, XT>), dim3(grid), dim3(XT), lds, c->st, k, dp<T>(c->d_engfac),
+                     dp<T>(c->d_sqv), dp<V>(c->d_w1h), dp<V>(c->d_w2h), dp<int>(c->d_sg3), dp<T>(c->d_sw3),
+                     dp<V>(c->d_Hk), dp<double>(c->d_selfp), runs, nrun, nblk, nitem, counter);
+  return ERM_OK;
+}
+template <typename T, int K>
+int launch_zcol_fft_runs(erm_ctx *c, const Params &k, int nblk, const GRun *runs, int nrun, int nitem, int *counter) {
+  if (k.p == 4) {
+    if (c->zcol_T == 512) return launch_zcol_runs_t<T, K, 4, 512>(c, k, nblk, runs, nrun, nitem, counter);
+    return launch_zcol_runs_t<T, K, 4, NTHR>(c, k, nblk, runs, nrun, nitem, counter);
+  }
+  if (c->zcol_T == 512) return launch_zcol_runs_t<T, K, 6, 512>(c, k, nblk, runs, nrun, nitem, counter);
+  return launch_zcol_runs_t<T, K, 6, NTHR>(c, k, nblk, runs, nrun, nitem, counter);
 }
 // inverse y columns: two lanes per column in FP64 (one lane's column does
 // not leave room for a second wave), one in FP32
@@ -115,7 +149,8 @@ template <typename T, int K> void launch_zgreen(erm_ctx *c, const Params &k) {
 
 
 template int launch_zcol_reg<ERM_T, ERM_K>(erm_ctx *, const Params &, int, int);
-template int launch_zcol_fft<ERM_T, ERM_K>(erm_ctx *, const Params &, int, int);
+template int launch_zcol_fft<ERM_T, ERM_K>(erm_ctx *, const Params &, int, int, size_t, size_t);
+template int launch_zcol_fft_runs<ERM_T, ERM_K>(erm_ctx *, const Params &, int, const GRun *, int, int, int *);
 template void launch_ycol_reg<ERM_T, ERM_K>(erm_ctx *, const Params &, int);
 template int launch_xslab<ERM_T, ERM_K>(erm_ctx *, const Params &, int, size_t);
 template void launch_xr2c<ERM_T, ERM_K>(erm_ctx *, const Params &);

@@ -304,6 +304,54 @@ module ermgpu
        integer(c_int32_t), value :: natom_traj
        integer(c_int32_t), intent(out) :: perm(*), found
      end function erm_read_permutation_c
+
+     ! several rigid test particles on one solvent as one launch sequence
+     ! (no counterpart in the reference: one solute species per run there)
+     integer(c_int32_t) function erm_group_create(ctxs, nctx, grp) bind(C, name="erm_group_create")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), intent(in) :: ctxs(*)
+       integer(c_int32_t), value :: nctx
+       type(c_ptr), intent(out) :: grp
+     end function erm_group_create
+
+     subroutine erm_group_destroy(grp) bind(C, name="erm_group_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: grp
+     end subroutine erm_group_destroy
+
+     type(c_ptr) function erm_group_last_error_c(grp) bind(C, name="erm_group_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: grp
+     end function erm_group_last_error_c
+
+     integer(c_int32_t) function erm_group_set_frame(grp, sitepos, cell, stat_weight_system) &
+          bind(C, name="erm_group_set_frame")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: grp
+       real(c_double), intent(in) :: sitepos(*), cell(3, 3)
+       real(c_double), value :: stat_weight_system
+     end function erm_group_set_frame
+
+     integer(c_int32_t) function erm_group_insert_random(grp, n, seed, first_index) &
+          bind(C, name="erm_group_insert_random")
+       import :: c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: grp
+       integer(c_int32_t), intent(in) :: n(*)
+       integer(c_int64_t), intent(in) :: seed(*), first_index(*)
+     end function erm_group_insert_random
+
+     integer(c_int32_t) function erm_group_set_tuning(grp, max_batch, persistent) &
+          bind(C, name="erm_group_set_tuning")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: grp
+       integer(c_int32_t), value :: max_batch, persistent
+     end function erm_group_set_tuning
+
+     integer(c_int32_t) function erm_group_stage_times(grp, ms10) bind(C, name="erm_group_stage_times")
+       import :: c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: grp
+       real(c_double), intent(out) :: ms10(10)
+     end function erm_group_stage_times
   end interface
 
 contains

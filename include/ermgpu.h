@@ -284,6 +284,42 @@ int32_t erm_insert_random(erm_ctx *ctx, int32_t n, uint64_t seed, uint64_t first
 int32_t erm_insert_random_multi(erm_ctx *const *ctxs, int32_t nctx, const int32_t *n, const uint64_t *seed,
                                 const uint64_t *first_index);
 
+/* ---- several test-particle species on one solvent ----------------------
+ * No counterpart in the reference, which runs one solute species per run
+ * (engproc.F90:89-91).  A group binds nctx (<= 16) contexts that describe the
+ * same solvent (physical molecules, charges, LJ types, species), grid and
+ * pair parameters, each with its own rigid test particle (slttype 2,
+ * bfcoord, random positions and orientations), bins and accumulators.
+ * erm_group_insert_random(g, n, seed, first_index) leaves in every member
+ * the accumulators erm_insert_random(ctx[i], n[i], seed[i], first_index[i])
+ * leaves: the same poses and energies, the counts bit for bit.  It runs as
+ * one sequence on member 0's stream: batches mix the members'
+ * configurations in runs (largest solutes first); the y and x transforms
+ * and the gather of a batch are one launch each over every run; the z
+ * columns and the real-space pairs are one persistent launch each whose
+ * workgroups pull (configuration, block) items of every run, or one launch
+ * per run (erm_group_set_tuning); the solvent's frame data and the batch
+ * buffers are member 0's, and so is the PME tuning.  After
+ * erm_group_set_frame the other members hold the frame's cell but not its
+ * solvent data: erm_set_frame them before using one alone.  Needs the
+ * register PME path (grid lengths 16-64, 96 in FP32; spline order 4 or 6). */
+typedef struct erm_group erm_group;
+int32_t erm_group_create(erm_ctx *const *ctxs, int32_t nctx, erm_group **out);
+void erm_group_destroy(erm_group *g);
+const char *erm_group_last_error(const erm_group *g);
+int32_t erm_group_set_frame(erm_group *g, const double *sitepos, const double *cell, double stat_weight_system);
+int32_t erm_group_set_frame_device(erm_group *g, const double *d_sitepos, const double *cell,
+                                   double stat_weight_system);
+int32_t erm_group_insert_random(erm_group *g, const int32_t *n, const uint64_t *seed, const uint64_t *first_index);
+/* max_batch: configurations per batch over all members, up to 65536 (0
+ * keeps it; 1024 by default); persistent: a bit mask of the stages that run
+ * as one persistent grid over every run (workgroups pull items through an
+ * atomic counter), bit 0 the z columns, bit 1 the pairs; a stage whose bit
+ * is clear runs one launch per run; 2 by default, -1 keeps it. */
+int32_t erm_group_set_tuning(erm_group *g, int32_t max_batch, int32_t persistent);
+/* stage times of the last erm_group_insert_random, as erm_stage_times */
+int32_t erm_group_stage_times(const erm_group *g, double *ms10);
+
 /* update_histogram alone, for energies computed elsewhere (n configurations,
  * uvengy n*(slvmax+1); solutes as in erm_engconst, NULL in refs mode). */
 int32_t erm_update_histogram(erm_ctx *ctx, int32_t n, const int32_t *solutes,
