@@ -1998,6 +1998,7 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   const T L0 = (T)P.L[0], L1 = (T)P.L[1], L2 = (T)P.L[2];
   const T h0 = (T)P.halfL[0], h1 = (T)P.halfL[1], h2 = (T)P.halfL[2];
   const T lw2 = (T)P.lw2, up2 = (T)P.up2, el2 = (T)P.el2, cut2 = (T)P.cut2, alpha = (T)P.screen;
+  const T w3 = (up2 - lw2) * (up2 - lw2) * (up2 - lw2), iw3 = (T)1 / w3;  // switch denominator
   // the queue test on the high parts (FP32) keeps a margin; the exact test
   // on the full displacement decides
   const T cut2q = F32 ? (T)(P.cut2 * (1.0 + 1e-5)) : cut2;
@@ -2152,16 +2153,23 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
                 ep = lj_bare_t<T>(P, eps, sig2, r2, rst);
                 if (rst < (T)P.elecut) ep += us.w * qsv / rst;
               } else {
+                // FP32: one hardware reciprocal square root (1 ulp) per pair
+                // in place of the divisions and the rounded square root;
+                // the pairs it reaches are not hot, so their terms are below
+                // 24 kcal/mol (Coulomb) and eps (LJ)
+                const T rs = F32 ? (T)__builtin_amdgcn_rsqf((float)r2) : (T)0;
+                const T ir2 = F32 ? rs * rs : (T)1 / r2;
                 if (P.ljswitch == 0) {
                   // CHARMM potential switch without branches: the low-range
                   // term, scaled by the switch beyond lwljcut, zero beyond
                   // upljcut or for eps = 0 (the expressions of lj_low_t and
                   // lj_switch_t)
-                  const T invr2 = sig2 / r2;
+                  const T invr2 = F32 ? sig2 * ir2 : sig2 / r2;
                   const T invr6 = invr2 * invr2 * invr2;
                   const T elo = (T)4 * eps * invr6 * (invr6 - (T)1);
-                  const T d = r2 - up2, w = up2 - lw2;
-                  const T esw = elo * ((T)2 * r2 + up2 - (T)3 * lw2) * (d * d) / (w * w * w);
+                  const T d = r2 - up2;
+                  const T sw = ((T)2 * r2 + up2 - (T)3 * lw2) * (d * d);
+                  const T esw = F32 ? elo * sw * iw3 : elo * sw / (w3);
                   T elj = (r2 <= lw2) ? elo : esw;
                   elj = (r2 <= up2 && eps > (T)0) ? elj : (T)0;
                   ep = elj;
@@ -2171,7 +2179,13 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
                   else if (r2 <= up2)
                     ep += lj_switch_t<T>(P, eps, sig2, r2);
                 }
-                if (r2 <= el2) ep += coul_t<T>(us.w * qsv, sqrt(r2), alpha);
+                if (r2 <= el2) {
+                  if constexpr (F32) {
+                    ep += us.w * qsv * erfcf(alpha * (r2 * rs)) * rs;
+                  } else {
+                    ep += coul_t<T>(us.w * qsv, sqrt(r2), alpha);
+                  }
+                }
               }
             }
             weq[qi] = (double)ep;
