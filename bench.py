@@ -42,7 +42,7 @@ FP64_PEAK = 78.6           # TFLOP/s dense FP64 vector
 FP32_PEAK = 157.3          # TFLOP/s dense FP32 vector
 
 STAGE_KERNELS = {"prep": "k_pose + k_solute_prep", "pme_z": "k_pme_zcol_fft", "pme_y": "k_pme_ycol_reg",
-                 "pme_x": "k_pme_xslab", "mol_sum": "k_mol_sum", "pair": "k_pair", "self": "k_selfreal + k_u0",
+                 "pme_x": "k_pme_xslab", "mol_sum": "k_mol_sum", "pair": "k_pair_tile", "self": "k_selfreal + k_u0",
                  "bin": "k_bin + k_scalars + k_edens", "corr": "k_corr"}
 
 
@@ -111,6 +111,55 @@ def stage_work(sy, T, slvmax, ermax, npair):
     }
 
 
+def measure_peaks(torch, dev):
+    """Peak rates measured on this device: a 1 GiB device-to-device copy
+    (bytes read + written per second), FP32 FMA (v_fma_f32; the packed FP32
+    forms are disabled in this build, so the published 157.3 TFLOP/s, which
+    counts them, is not reachable by the pair kernel) and FP64 MFMA
+    (v_mfma_f64_16x16x4f64), the last two from libermpeak.so's
+    microbenchmarks; each the best of five timed launches."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "ermod_amd", "libermpeak.so"))
+    lib.erm_peak_fma_f32.restype = C.c_double
+    lib.erm_peak_mfma_f64.restype = C.c_double
+    for f in (lib.erm_peak_fma_f32, lib.erm_peak_mfma_f64):
+        f.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+    st = torch.cuda.current_stream(dev)
+
+    def best(launch):
+        ms = []
+        work = launch()
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            work = launch()
+            e1.record(st)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        if work <= 0:
+            raise RuntimeError("peak microbenchmark launch failed")
+        return work / (min(ms) * 1e-3)
+
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    copy = best(lambda: (b.copy_(a), 2.0 * n * 8)[1]) / 1e9
+    del a, b
+    blocks = 2048
+    o32 = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    o64 = torch.empty(blocks * 256, dtype=torch.float64, device=dev)
+    fma = best(lambda: lib.erm_peak_fma_f32(C.c_void_p(st.cuda_stream), blocks, 50, C.c_void_p(o32.data_ptr())))
+    mfma = best(lambda: lib.erm_peak_mfma_f64(C.c_void_p(st.cuda_stream), blocks, 20, C.c_void_p(o64.data_ptr())))
+    torch.cuda.synchronize()
+    return {"hbm_copy_GBps": copy, "fp32_fma_TFLOPs": fma / 1e12, "fp64_mfma_TFLOPs": mfma / 1e12,
+            "method": "best of 5 launches timed with HIP events: 1 GiB float64 device copy (2 x bytes); "
+                      "2048 x 256 threads of 8 independent v_fma_f32 chains; 2048 x 4 waves of 4 independent "
+                      "v_mfma_f64_16x16x4f64 chains"}
+
+
+MEASURED = {}   # filled from measure_peaks() when it runs
+
+
 def roofline_of(stage, bound, work_cfg, work_launch, ms_total, n_cfg, n_launch, T):
     peak, unit = {"hbm": (HBM_PEAK, "GB/s"), "valu": (FP32_PEAK if T == 4 else FP64_PEAK, "TFLOP/s"),
                   "mfma": (FP64_MFMA_PEAK, "TFLOP/s")}[bound]
@@ -118,10 +167,15 @@ def roofline_of(stage, bound, work_cfg, work_launch, ms_total, n_cfg, n_launch, 
     per_launch = total / max(n_launch, 1)
     avg_ms = ms_total / max(n_launch, 1)
     ach = per_launch / (avg_ms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
-    return {"kernel": STAGE_KERNELS.get(stage, stage), "stage": stage, "bound": bound, "achieved": ach,
-            "peak": peak, "unit": unit, "frac": ach / peak, "avg_launch_ms": avg_ms,
-            ("bytes_per_launch" if unit == "GB/s" else "flops_per_launch"): per_launch,
-            "configs_per_launch": n_cfg / max(n_launch, 1), "launches": n_launch}
+    out = {"kernel": STAGE_KERNELS.get(stage, stage), "stage": stage, "bound": bound, "achieved": ach,
+           "peak": peak, "unit": unit, "frac": ach / peak, "avg_launch_ms": avg_ms,
+           ("bytes_per_launch" if unit == "GB/s" else "flops_per_launch"): per_launch,
+           "configs_per_launch": n_cfg / max(n_launch, 1), "launches": n_launch}
+    key = {"hbm": "hbm_copy_GBps", "mfma": "fp64_mfma_TFLOPs", "valu": "fp32_fma_TFLOPs" if T == 4 else None}[bound]
+    if key and key in MEASURED:
+        out["peak_measured"] = MEASURED[key]
+        out["frac_of_measured"] = ach / MEASURED[key]
+    return out
 
 
 def measured_traffic(workload, precision, kernel, configs):
@@ -162,27 +216,37 @@ FIXED_TUNING = dict(max_batch=1024, pme_path="register", slab_planes=8, group_pl
                     column_threads=256, fold_y=True)
 
 
-def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sequential=False):
-    """Timed frames of the mixed-solvent workload: per frame, each solute's
-    context takes the device frame and its share of the insertions; the four
-    contexts run concurrently (one stream each), or one after another
-    (sequential, for clean per-kernel traces).  Returns the timing, the
-    summed stage times and per-stage launch / configuration counts."""
-    from ermod_amd.api import ErmGPU, insert_random_multi
+def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, schedule="group"):
+    """Timed frames of the mixed-solvent workload.  schedule "group": the four
+    solutes' contexts bound in one group, each frame one launch sequence on
+    one stream (z-column and pair launches persistent over all solutes or
+    one per solute; y/x transforms and gather one launch over all);
+    "multi": the four contexts concurrently, one stream each; "sequential":
+    one after another (clean per-kernel traces).  Each context is autotuned
+    on the first frame (untimed); the group's batch size and launch mode
+    are those the warm-up frames time fastest.  Returns the timing, the
+    summed stage times and the tunings."""
+    from ermod_amd.api import ErmGPU, ErmGroup, insert_random_multi
     nfu = len(dpos)
     share = [per_frame // len(syss)] * len(syss)
     engines = [ErmGPU(sy.system, prec) for sy in syss]
     seeds = [11, 12, 13, 14]
+    group = None
 
     def frame(i):
+        if group is not None:
+            group.set_frame_device(dpos[i % nfu].data_ptr(), syss[0].cell)
+            group.insert_random(share, seeds, [i * s for s in share])
+            return
         for e, sy in zip(engines, syss):
             e.set_frame_device(dpos[i % nfu].data_ptr(), sy.cell)
-        if sequential:
+        if schedule == "sequential":
             for k, e in enumerate(engines):
                 e.insert_random(share[k], seeds[k], first_index=i * share[k])
         else:
             insert_random_multi(engines, share, seeds, [i * s for s in share])
 
+    group_tuning = None
     try:
         for e, sy in zip(engines, syss):
             e.set_frame_device(dpos[0].data_ptr(), sy.cell)
@@ -192,6 +256,26 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sequ
                 e.set_tuning(**FIXED_TUNING)
         for e in engines:
             e.set_tuning(max_batch=0, pme_path=e.tuning()["pme_path"], use_graph=False)
+        if schedule == "group":
+            group = ErmGroup(engines)
+            total = sum(share)
+            if tune:
+                best = None
+                for bmax in ((total + 1) // 2, total):
+                    for pers in (0, 1, 2, 3):
+                        group.set_tuning(max_batch=bmax, persistent=pers)
+                        frame(0)
+                        torch.cuda.synchronize()
+                        t0 = time.perf_counter()
+                        frame(1)
+                        torch.cuda.synchronize()
+                        t = time.perf_counter() - t0
+                        if best is None or t < best[0]:
+                            best = (t, bmax, pers)
+                group_tuning = {"max_batch": best[1], "persistent": best[2]}
+            else:
+                group_tuning = {"max_batch": total, "persistent": 2}
+            group.set_tuning(**group_tuning)
         for i in range(warmup):
             frame(i)
         for e in engines:
@@ -201,18 +285,25 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sequ
         t0 = time.perf_counter()
         for i in range(warmup, warmup + steps):
             frame(i)
-            for k, e in enumerate(engines):
-                for key, v in e.stage_times().items():
-                    st[k][key] = st[k].get(key, 0.0) + v
+            if group is not None:
+                for key, v in group.stage_times().items():
+                    st[0][key] = st[0].get(key, 0.0) + v
+            else:
+                for k, e in enumerate(engines):
+                    for key, v in e.stage_times().items():
+                        st[k][key] = st[k].get(key, 0.0) + v
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         tun = [e.tuning() for e in engines]
         acc = [e.fetch() for e in engines]
         slvmax, ermax = engines[0].slvmax, engines[0].ermax
     finally:
+        if group is not None:
+            group.close()
         for e in engines:
             e.close()
-    return dict(dt=dt, stage=st, tuning=tun, acc=acc, share=share, slvmax=slvmax, ermax=ermax)
+    return dict(dt=dt, stage=st, tuning=tun, acc=acc, share=share, slvmax=slvmax, ermax=ermax,
+                group_tuning=group_tuning)
 
 
 FP32_EMAX = 1e4   # kcal/mol: the last finite bin edge of every energy coordinate here
@@ -328,8 +419,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default batch and PME path)")
     ap.add_argument("--no-error", action="store_true", help="skip the FP32-vs-FP64 error pass")
     ap.add_argument("--no-refs20", action="store_true", help="skip the refs20 secondary passes")
-    ap.add_argument("--sequential", action="store_true",
-                    help="run the four solutes' contexts one after another (clean per-kernel traces)")
+    ap.add_argument("--schedule", default="group", choices=["group", "multi", "sequential"],
+                    help="group: one launch sequence over the four solutes (default); multi: four contexts on "
+                         "four streams; sequential: one context after another")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the peak-rate microbenchmarks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="after the run, one device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -344,15 +437,19 @@ def main():
     nfu = min(args.steps + args.warmup, 6) + 1   # distinct frames (cycled) + one for the error pass
     syss, host, dpos = mixed_setup(torch, dev, nfu)
     r = run_mixed(torch, syss, dpos[:-1], args.precision, args.steps, args.warmup, args.insertions,
-                  tune=not args.no_tune, sequential=args.sequential)
+                  tune=not args.no_tune, schedule=args.schedule)
     slvmax, ermax = r["slvmax"], r["ermax"]
     n_ins = args.steps * sum(r["share"])
     value = n_ins * slvmax / r["dt"]
     T = 4 if args.precision == "fp32" else 8
 
-    # stage rooflines over the timed region, summed over the four contexts
+    # stage rooflines over the timed region: the work of every solute's
+    # configurations against the stage's summed time; a launch is one batch
+    # of the group (all solutes), or one batch of a context
     stage_rows = {}
     tot = {}
+    grouped = args.schedule == "group"
+    nb_group = int(np.ceil(sum(r["share"]) / r["group_tuning"]["max_batch"])) if grouped else 0
     for k, sy in enumerate(syss):
         pos0 = host[0]
         npair = pair_count(sy, pos0, sample_poses(sy, 3, 100 + k))
@@ -360,20 +457,36 @@ def main():
         B = r["tuning"][k]["max_batch"]
         n_cfg = args.steps * r["share"][k]
         n_launch = args.steps * int(np.ceil(r["share"][k] / B))
-        if r["tuning"][k]["fold_y"]:
+        folded = r["tuning"][0 if grouped else k]["fold_y"]
+        if folded:
             work.pop("pme_y")   # the inverse y transform runs inside the x slab
         for stg, (bound, wc, wl) in work.items():
-            ms = r["stage"][k].get(stg, 0.0)
-            a = tot.setdefault(stg, dict(bound=bound, work=0.0, ms=0.0, cfg=0, launch=0))
-            a["work"] += wc * n_cfg + wl * n_launch
-            a["ms"] += ms
+            a = tot.setdefault(stg, dict(bound=bound, work=0.0, ms=0.0, cfg=0, launch=0, wl=wl))
+            a["work"] += wc * n_cfg
             a["cfg"] += n_cfg
-            a["launch"] += n_launch
+            if not grouped:
+                a["work"] += wl * n_launch
+                a["ms"] += r["stage"][k].get(stg, 0.0)
+                a["launch"] += n_launch
+    pmask = r["group_tuning"]["persistent"] if grouped else 0
+    if grouped:
+        # a group batch launches the y/x transforms once, the z columns and
+        # pairs once (persistent) or once per solute, the rest once per solute
+        nrun = sum(1 for x in r["share"] if x > 0)
+        for stg, a in tot.items():
+            once = stg in ("pme_y", "pme_x") or (stg == "pme_z" and pmask & 1) or (stg == "pair" and pmask & 2)
+            a["launch"] = args.steps * nb_group * (1 if once else nrun)
+            a["work"] += a["wl"] * args.steps * nb_group
+            a["ms"] = r["stage"][0].get(stg, 0.0)
+    if not args.no_peaks:
+        MEASURED.update(measure_peaks(torch, dev))
     for stg, a in tot.items():
         if a["ms"] <= 0:
             continue
         stage_rows[stg] = roofline_of(stg, a["bound"], a["work"] / a["cfg"], 0.0, a["ms"], a["cfg"], a["launch"], T)
         stage_rows[stg]["ms_total"] = round(a["ms"], 3)
+        if (stg == "pme_z" and pmask & 1) or (stg == "pair" and pmask & 2):
+            stage_rows[stg]["kernel"] += "_runs"
     dom = max(stage_rows, key=lambda k: stage_rows[k]["ms_total"])
     roof = dict(stage_rows[dom])
     roof["traffic"] = (measured_traffic("mixed5000", args.precision, roof["kernel"], roof["configs_per_launch"])
@@ -394,10 +507,18 @@ def main():
                       "tuning": [{k: t[k] for k in ("max_batch", "pme_path", "slab_planes", "group_planes",
                                                     "slab_threads", "column_threads", "fold_y")}
                                  for t in r["tuning"]],
-                      "schedule": "four contexts, one after another" if args.sequential
-                      else "four contexts, one stream each"},
+                      "schedule": {"group": "one group on one stream: y/x transforms and gather one launch over "
+                                            "the four solutes; z columns "
+                                            + ("one persistent launch over the four" if pmask & 1 else
+                                               "one launch per solute") + "; pairs "
+                                            + ("one persistent launch over the four" if pmask & 2 else
+                                               "one launch per solute"),
+                                   "multi": "four contexts, one stream each",
+                                   "sequential": "four contexts, one after another"}[args.schedule],
+                      "group_tuning": r["group_tuning"]},
            "insertions_per_s": n_ins / r["dt"],
            "roofline": roof, "stages": stage_rows, "stage_ms": stage_ms_total,
+           "peaks_measured": dict(MEASURED) or None,
            "engsmpl": [a["engsmpl"] for a in r["acc"]]}
     if not args.no_error:
         out["fp32_error"] = mixed_fp32_error(syss, dpos, args.insertions)

@@ -16,6 +16,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = "/opt/rocm/bin/amdflang"
 MAIN = os.path.join(HERE, "csrc", "ermgpu.hip")
 PME = os.path.join(HERE, "csrc", "ermgpu_pme.hip")
+# peak-rate microbenchmarks for the roofline (libermpeak.so)
+PEAK = os.path.join(HERE, "csrc", "ermpeak.hip")
 # register-FFT grid lengths per precision (the run-time dispatch in
 # ermgpu.hip covers exactly these)
 PME_UNITS = [("double", k) for k in (16, 24, 32, 48, 64)] + [("float", k) for k in (16, 24, 32, 48, 64, 96)]
@@ -39,17 +41,21 @@ def _run(cmd, verbose):
 
 def build_lib(verbose: bool = False, jobs: int = 0) -> str:
     """Compiles the host/launch unit and one unit per (precision, grid
-    length) of the register-FFT kernels in parallel, then links them."""
+    length) of the register-FFT kernels in parallel, then links them; the
+    peak-rate microbenchmarks build into libermpeak.so alongside."""
     from concurrent.futures import ThreadPoolExecutor
     os.makedirs(BUILD_DIR, exist_ok=True)
     units = [([MAIN], os.path.join(BUILD_DIR, "ermgpu.o"))]
     for t, k in PME_UNITS:
         units.append(([f"-DERM_T={t}", f"-DERM_K={k}", PME], os.path.join(BUILD_DIR, f"pme_{t[0]}{k}.o")))
-    jobs = jobs or min(len(units), max(1, os.cpu_count() or 1), 16)
+    jobs = jobs or min(len(units) + 1, max(1, os.cpu_count() or 1), 16)
     cmds = [[HIPCC, *FLAGS, "-c", *src, "-o", obj] for src, obj in units]
+    peak = os.path.join(HERE, "libermpeak.so")
+    cmds.append([HIPCC, *FLAGS, "-shared", PEAK, "-o", peak + ".tmp"])
     with ThreadPoolExecutor(jobs) as ex:
         for f in [ex.submit(_run, c, verbose) for c in cmds]:
             f.result()
+    os.replace(peak + ".tmp", peak)
     out = os.path.join(HERE, "libermgpu.so")
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *[o for _, o in units], "-o", tmp], verbose)
