@@ -489,8 +489,9 @@ def main():
             stage_rows[stg]["kernel"] += "_runs"
     dom = max(stage_rows, key=lambda k: stage_rows[k]["ms_total"])
     roof = dict(stage_rows[dom])
-    roof["traffic"] = (measured_traffic("mixed5000", args.precision, roof["kernel"], roof["configs_per_launch"])
-                       if roof["unit"] == "GB/s" else None)
+    roof["traffic"] = measured_traffic("mixed5000", args.precision, roof["kernel"], roof["configs_per_launch"])
+    for row in stage_rows.values():
+        row["traffic"] = measured_traffic("mixed5000", args.precision, row["kernel"], row["configs_per_launch"])
     stage_ms_total = {k: round(sum(st.get(k, 0.0) for st in r["stage"]), 3) for k in r["stage"][0]}
 
     out = {"metric": "solute-solvent mol-pair energies/s", "value": value, "unit": "mol-pair energies/s",
