@@ -2037,13 +2037,16 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
     wfirst[lane] = first;
     wave_lds_sync();
     double acc = 0.0;  // energy of molecule `lane`
-    for (int t0 = 0; t0 < tot_sites; t0 += 64) {
-      const int cs = t0 + lane;
-      bool act = cs < tot_sites;
-      T4 tu, tl;
-      tu.x = tu.y = tu.z = tu.w = 0;
-      tl.x = tl.y = tl.z = tl.w = 0;
-      if (act) {
+    // the site data of a tile are loaded one tile ahead: the loads of tile
+    // t0 + 64 are in flight while tile t0's groups run
+    double px = 0.0, py = 0.0, pz = 0.0;
+    T pq = 0;
+    int pc = 0;
+    bool pv = false;
+    auto fetch = [&](int t0n) {
+      const int cs = t0n + lane;
+      pv = cs < tot_sites;
+      if (pv) {
         int a = 0, e = 64;  // molecule k of chunk site cs: last k with woff[k] <= cs
         while (e - a > 1) {
           const int mid = (a + e) >> 1;
@@ -2053,7 +2056,21 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
             e = mid;
         }
         const int t = wfirst[a] + (cs - woff[a]);
-        double dx = sx[t] - C0, dy = sy[t] - C1, dz = sz[t] - C2v;
+        px = sx[t];
+        py = sy[t];
+        pz = sz[t];
+        pq = sq[t];
+        pc = sct[t];
+      }
+    };
+    fetch(0);
+    for (int t0 = 0; t0 < tot_sites; t0 += 64) {
+      bool act = pv;
+      T4 tu, tl;
+      tu.x = tu.y = tu.z = tu.w = 0;
+      tl.x = tl.y = tl.z = tl.w = 0;
+      if (act) {
+        double dx = px - C0, dy = py - C1, dz = pz - C2v;
         dx -= P.L[0] * rint(dx * iL0);
         dy -= P.L[1] * rint(dy * iL1);
         dz -= P.L[2] * rint(dz * iL2);
@@ -2061,39 +2078,42 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
         tu.x = (T)dx;
         tu.y = (T)dy;
         tu.z = (T)dz;
-        tu.w = sq[t];
+        tu.w = pq;
         if constexpr (F32) {
           tl.x = (T)(dx - (double)tu.x);
           tl.y = (T)(dy - (double)tu.y);
           tl.z = (T)(dz - (double)tu.z);
-          tl.w = __int_as_float(sct[t]);
+          tl.w = __int_as_float(pc);
         } else {
-          tl.w = __longlong_as_double((long long)sct[t]);
+          tl.w = __longlong_as_double((long long)pc);
         }
       }
       tu4[lane] = tu;
       tl4[lane] = tl;
+      if (t0 + 64 < tot_sites) fetch(t0 + 64);
       double e = 0.0;  // this site's energy, in solute-site order
       for (int s0 = 0; s0 < ns; s0 += PT_GRP) {
         int pend[PT_GRP];
         int qn = 0;
+        // the group's solute sites, all read before the queue writes below
+        // (one wait for the eight broadcast reads)
+        T4 usg[PT_GRP];
+#pragma unroll
+        for (int g = 0; g < PT_GRP; ++g) usg[g] = us4[min(s0 + g, ns - 1)];
 #pragma unroll
         for (int g = 0; g < PT_GRP; ++g) {
           const int s = s0 + g;
-          bool in = false;
-          if (s < ns) {
-            const T4 us = us4[s];
-            T ax = tu.x - us.x, ay = tu.y - us.y, az = tu.z - us.z;
-            if (img) {
-              ax = fabs(ax);
-              ay = fabs(ay);
-              az = fabs(az);
-              ax = (ax > h0) ? L0 - ax : ax;
-              ay = (ay > h1) ? L1 - ay : ay;
-              az = (az > h2) ? L2 - az : az;
-            }
-            in = act && (ax * ax + ay * ay + az * az) <= cut2q;
+          const T4 us = usg[g];
+          T ax = tu.x - us.x, ay = tu.y - us.y, az = tu.z - us.z;
+          if (img) {
+            ax = fabs(ax);
+            ay = fabs(ay);
+            az = fabs(az);
+            ax = (ax > h0) ? L0 - ax : ax;
+            ay = (ay > h1) ? L1 - ay : ay;
+            az = (az > h2) ? L2 - az : az;
           }
+          const bool in = act && s < ns && (ax * ax + ay * ay + az * az) <= cut2q;
           const unsigned long long m = __ballot(in);
           const int pos = qn + lanes_below(m);
           pend[g] = in ? pos : -1;
