@@ -114,8 +114,8 @@ def stage_work(sy, T, slvmax, ermax, npair):
 def measure_peaks(torch, dev):
     """Peak rates measured on this device: a 1 GiB device-to-device copy
     (bytes read + written per second), FP32 FMA (v_fma_f32; the packed FP32
-    forms are disabled in this build, so the published 157.3 TFLOP/s, which
-    counts them, is not reachable by the pair kernel) and FP64 MFMA
+    forms are disabled at compile time in this library, so the published
+    157.3 TFLOP/s, which counts them, is not reachable by the pair kernel) and FP64 MFMA
     (v_mfma_f64_16x16x4f64), the last two from libermpeak.so's
     microbenchmarks; each the best of five timed launches."""
     import ctypes as C
@@ -427,8 +427,8 @@ def main():
                     help="after the run, one device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; runs on several GPUs (frames sharded over "
-                                   "ranks, one all-reduce of the accumulators per division) are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU; runs on several GPUs (frames sharded over "
+                                   "ranks, one all-reduce of the accumulators per division) are not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 // Peak-rate microbenchmarks for the roofline: the device's FP32 FMA rate
-// (v_fma_f32, one lane op per FMA: the packed FP32 forms are disabled in
-// this build) and its FP64 matrix rate (v_mfma_f64_16x16x4f64, the
+// (v_fma_f32, one lane op per FMA: the library is compiled with the packed
+// FP32 forms disabled) and its FP64 matrix rate (v_mfma_f64_16x16x4f64, the
 // instruction the correlation kernel uses).  Each thread or wave runs
 // independent dependency chains long enough to hide the instruction
 // latency; the results are stored so that nothing is optimised away.
