@@ -1409,8 +1409,14 @@ static __global__ void k_mol_sum(Params P, int nsites, const double *__restrict_
                                  const int *__restrict__ zpos, const int *__restrict__ mol_first,
                                  const int *__restrict__ mol_ns, const double *__restrict__ mol_q,
                                  const int *__restrict__ slt_j, double *uv) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
+  // XCD-aware order (workgroups go to the 8 XCDs round-robin by linear
+  // index): the workgroups of a configuration run on one XCD, whose L2 then
+  // holds the configuration's per-site sums for all of them
+  const int nwg = gridDim.x * gridDim.y;
+  int L = blockIdx.y * gridDim.x + blockIdx.x;
+  if (nwg % 8 == 0) L = (L % 8) * (nwg / 8) + L / 8;
+  const int b = L / gridDim.x;
+  const int j = (L - b * gridDim.x) * blockDim.x + threadIdx.x;
   if (j >= P.slvmax) return;
   double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
   if (j == slt_j[b]) {
@@ -1957,7 +1963,10 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   int *woff = reinterpret_cast<int *>(tl4 + 64);  // [65] first chunk site of molecule lane
   int *wfirst = woff + 65;                // [64] first sorted site of molecule lane
   int *wq = wfirst + 64;                  // [PT_QCAP] queued pairs: solute site << 6 | tile lane
-  double *weq = reinterpret_cast<double *>(reinterpret_cast<uintptr_t>(wq + PT_QCAP + 1) & ~(uintptr_t)7);
+  // (a byte offset from wl, rounded down to 8 over the spare slot of wq, so
+  // the pointers stay LDS pointers for the compiler)
+  double *weq = reinterpret_cast<double *>(
+      wl + (((size_t)128 * sizeof(T4) + (size_t)(65 + 64 + PT_QCAP + 1) * sizeof(int)) & ~(size_t)7));
   double *wes = weq + PT_QCAP;            // [64] per-site sums of a tile
   int *whq = reinterpret_cast<int *>(wes + 64);  // [PT_QCAP] queue slots of the pairs evaluated in FP64
 
