@@ -707,7 +707,7 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
         return fail(c, ERM_E_ENG_BUG, "aggregate centre missing");
       if (bt.flex) TRY(upload(c, c->d_flexbf, bt.flex, (size_t)n * ns * 3));
       const size_t lpose = (ns <= POSE_LDS_SITES) ? sizeof(double) * 4 * ns : 0;
-      hipLaunchKernelGGL(k_pose, dim3(n), dim3(64), lpose, c->st, k, n, dp<uint64_t>(c->d_args),
+      hipLaunchKernelGGL(lpose ? k_pose<true> : k_pose<false>, dim3(n), dim3(64), lpose, c->st, k, n, dp<uint64_t>(c->d_args),
                          (uint64_t)bt.first, bt.flex ? dp<double>(c->d_flexbf) : dp<double>(c->d_bf),
                          bt.flex ? 3 * ns : 0, dp<double>(c->d_sitemass) + c->mol_begin[c->slt_template],
                          dp<double>(c->d_cellm), dp<double>(c->d_aggcen), dp<double>(c->d_spos));
@@ -1514,7 +1514,8 @@ template <typename T> int group_batch(erm_group *g, const GRun *R, int nrun, con
     km.ns = m->ns;
     const size_t so = R[r].site0;
     const size_t lpose = (m->ns <= POSE_LDS_SITES) ? sizeof(double) * 4 * m->ns : 0;
-    hipLaunchKernelGGL(k_pose, dim3(R[r].ncfg), dim3(64), lpose, c->st, km, R[r].ncfg, dp<uint64_t>(m->d_args),
+    hipLaunchKernelGGL(lpose ? k_pose<true> : k_pose<false>, dim3(R[r].ncfg), dim3(64), lpose, c->st, km, R[r].ncfg,
+                       dp<uint64_t>(m->d_args),
                        offset[r], dp<double>(m->d_bf), 0, dp<double>(m->d_sitemass) + m->mol_begin[m->slt_template],
                        dp<double>(m->d_cellm), dp<double>(m->d_aggcen), dp<double>(c->d_spos) + so * 3);
     TRY(launch_check(c, "k_pose"));

@@ -287,6 +287,10 @@ constexpr uint64_t POS_STREAM = 0xD1B54A32D192ED03ULL;
 // site order) from the positions, kept in LDS when 4 ns doubles fit
 // (POSE_LDS_SITES), else in spos itself.
 constexpr int POSE_LDS_SITES = 2048;
+// IN_LDS: the sites and masses are staged in dynamic LDS (ns <=
+// POSE_LDS_SITES), else the sites are updated in place in the output; a
+// compile-time choice keeps each pointer in one address space
+template <bool IN_LDS>
 static __global__ __launch_bounds__(64) void k_pose(Params P, int n, const uint64_t *__restrict__ args, uint64_t offset,
                                              const double *__restrict__ bf, int bf_stride,
                                              const double *__restrict__ mass, const double *__restrict__ cellm,
@@ -300,11 +304,11 @@ static __global__ __launch_bounds__(64) void k_pose(Params P, int n, const uint6
   const uint64_t index = args[1] + offset + (uint64_t)b;
   double *out = spos + (size_t)b * ns * 3;
   const double *body = bf + (size_t)b * bf_stride;  // one body per insertion for a flexible solute
-  const bool in_lds = ns <= POSE_LDS_SITES;
+  constexpr bool in_lds = IN_LDS;
   double *x = in_lds ? pose_lds : out;
   const double *ms = in_lds ? pose_lds + 3 * ns : mass;
   for (int i = ln; i < 3 * ns; i += nl) x[i] = body[i];
-  if (in_lds)
+  if constexpr (in_lds)
     for (int j = ln; j < ns; j += nl) pose_lds[3 * ns + j] = mass[j];
   __syncthreads();
   double c[3];
