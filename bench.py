@@ -216,7 +216,7 @@ FIXED_TUNING = dict(max_batch=1024, pme_path="register", slab_planes=8, group_pl
                     column_threads=256, fold_y=True)
 
 
-def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, schedule="group"):
+def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, schedule="group", graph_steps=0):
     """Timed frames of the mixed-solvent workload.  schedule "group": the four
     solutes' contexts bound in one group, each frame one launch sequence on
     one stream (z-column and pair launches persistent over all solutes or
@@ -297,13 +297,25 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
         tun = [e.tuning() for e in engines]
         acc = [e.fetch() for e in engines]
         slvmax, ermax = engines[0].slvmax, engines[0].ermax
+        graph_dt = None
+        if group is not None and graph_steps > 0:
+            # the same frames replayed from the group's captured graph (one
+            # eager call captures it, untimed)
+            group.set_graph(True)
+            frame(warmup + steps)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(warmup + steps + 1, warmup + steps + 1 + graph_steps):
+                frame(i)
+            torch.cuda.synchronize()
+            graph_dt = time.perf_counter() - t0
     finally:
         if group is not None:
             group.close()
         for e in engines:
             e.close()
     return dict(dt=dt, stage=st, tuning=tun, acc=acc, share=share, slvmax=slvmax, ermax=ermax,
-                group_tuning=group_tuning)
+                group_tuning=group_tuning, graph_dt=graph_dt, graph_steps=graph_steps)
 
 
 FP32_EMAX = 1e4   # kcal/mol: the last finite bin edge of every energy coordinate here
@@ -423,6 +435,7 @@ def main():
                     help="group: one launch sequence over the four solutes (default); multi: four contexts on "
                          "four streams; sequential: one context after another")
     ap.add_argument("--no-peaks", action="store_true", help="skip the peak-rate microbenchmarks")
+    ap.add_argument("--no-graph", action="store_true", help="skip the group's graph-replay pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="after the run, one device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -437,7 +450,8 @@ def main():
     nfu = min(args.steps + args.warmup, 6) + 1   # distinct frames (cycled) + one for the error pass
     syss, host, dpos = mixed_setup(torch, dev, nfu)
     r = run_mixed(torch, syss, dpos[:-1], args.precision, args.steps, args.warmup, args.insertions,
-                  tune=not args.no_tune, schedule=args.schedule)
+                  tune=not args.no_tune, schedule=args.schedule,
+                  graph_steps=0 if args.no_graph else max(2, args.steps // 2))
     slvmax, ermax = r["slvmax"], r["ermax"]
     n_ins = args.steps * sum(r["share"])
     value = n_ins * slvmax / r["dt"]
@@ -520,6 +534,10 @@ def main():
            "insertions_per_s": n_ins / r["dt"],
            "roofline": roof, "stages": stage_rows, "stage_ms": stage_ms_total,
            "peaks_measured": dict(MEASURED) or None,
+           "group_graph": ({"value": r["graph_steps"] * sum(r["share"]) * slvmax / r["graph_dt"],
+                            "ms_per_step": r["graph_dt"] / r["graph_steps"] * 1e3, "steps": r["graph_steps"],
+                            "note": "the same workload replayed from the group's captured HIP graph"}
+                           if r.get("graph_dt") else None),
            "engsmpl": [a["engsmpl"] for a in r["acc"]]}
     if not args.no_error:
         out["fp32_error"] = mixed_fp32_error(syss, dpos, args.insertions)

@@ -63,7 +63,8 @@ EXPORTS = ("erm_build_edges", "erm_hash_double", "erm_create", "erm_destroy", "e
            "erm_fetch_intermediate", "erm_set_refstructure", "erm_insert_flexible", "erm_set_pair_path",
            "erm_get_pair_path", "erm_update_histogram_static",
            "erm_group_create", "erm_group_destroy", "erm_group_last_error", "erm_group_set_frame",
-           "erm_group_set_frame_device", "erm_group_insert_random", "erm_group_set_tuning", "erm_group_stage_times")
+           "erm_group_set_frame_device", "erm_group_insert_random", "erm_group_set_tuning", "erm_group_set_graph",
+           "erm_group_stage_times")
 
 _lib = None
 
@@ -115,6 +116,7 @@ def lib():
         L.erm_group_set_frame_device.argtypes = [vp, C.c_void_p, _f64p, C.c_double]
         L.erm_group_insert_random.argtypes = [vp, _i32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.erm_group_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+        L.erm_group_set_graph.argtypes = [vp, C.c_int32]
         L.erm_group_stage_times.argtypes = [vp, _f64p]
         L.erm_get_pair_path.argtypes = [vp]
         L.erm_insert_random_multi.argtypes = [C.POINTER(vp), C.c_int32, _i32p, C.POINTER(C.c_uint64),
@@ -317,6 +319,11 @@ class ErmGroup:
         else:
             ps = int(persistent)
         self._check(lib().erm_group_set_tuning(self.h, int(max_batch), ps), "group_set_tuning")
+
+    def set_graph(self, on: bool):
+        """Replay the batch sequence from a captured HIP graph when the
+        shape repeats (the first call of a shape runs eagerly and captures)."""
+        self._check(lib().erm_group_set_graph(self.h, int(bool(on))), "group_set_graph")
 
     def stage_times(self):
         out = np.zeros(len(STAGES))

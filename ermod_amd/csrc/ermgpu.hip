@@ -1413,6 +1413,11 @@ struct erm_group {
   int h_runs_cap = 0;
   double stage_ms[ermh::NSTAGE_OUT] = {};
   std::string err;
+  // graph replay of the batch sequence: the first call of a shape runs
+  // eagerly and captures; calls with the same key launch the graph
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::string gkey;
 };
 
 namespace {
@@ -2388,6 +2393,7 @@ int32_t erm_group_create(erm_ctx *const *ctxs, int32_t nctx, erm_group **out) {
 void erm_group_destroy(erm_group *g) {
   if (!g) return;
   if (hipSetDevice(g->ctx[0]->P.device) == hipSuccess) {
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
     g->d_runs.release();
     g->d_counter.release();
     if (g->h_runs) (void)hipHostFree(g->h_runs);
@@ -2495,9 +2501,12 @@ int32_t erm_group_insert_random(erm_group *g, const int32_t *n, const uint64_t *
   const Params &k = c->kp;
   const int vb = c->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
   const int nzg = zcol_groups(c, true, vb);
+  // the run tables of every batch (host), then the batches' launches
+  std::vector<int> nrun_b(nb), ncfg_b(nb), nz_b(nb), np_b(nb);
+  std::vector<uint64_t> off_b((size_t)nb * GROUP_MAXRUN, 0);
   for (int bi = 0; bi < nb && !st; ++bi) {
     GRun *R = g->h_runs + (size_t)bi * GROUP_MAXRUN;
-    uint64_t offset[GROUP_MAXRUN];
+    uint64_t *offset = off_b.data() + (size_t)bi * GROUP_MAXRUN;
     int nrun = 0, cfg = 0, site = 0, zi = 0, pi = 0;
     for (int oi = 0; oi < nm; ++oi) {
       const int t = order[oi];
@@ -2529,21 +2538,102 @@ int32_t erm_group_insert_random(erm_group *g, const int32_t *n, const uint64_t *
       pi += cnt * nsplit;
       ++nrun;
     }
-    GRun *d_runs = dp<GRun>(g->d_runs) + (size_t)bi * GROUP_MAXRUN;
-    if (hipMemcpyAsync(d_runs, R, sizeof(GRun) * nrun, hipMemcpyHostToDevice, c->st) != hipSuccess) {
-      (void)hipGetLastError();
-      st = fail(c, ERM_E_DEVICE, "run table upload");
-      break;
+    nrun_b[bi] = nrun;
+    ncfg_b[bi] = cfg;
+    nz_b[bi] = zi;
+    np_b[bi] = pi;
+  }
+  auto enqueue = [&]() -> int {
+    for (int bi = 0; bi < nb; ++bi) {
+      GRun *R = g->h_runs + (size_t)bi * GROUP_MAXRUN;
+      GRun *d_runs = dp<GRun>(g->d_runs) + (size_t)bi * GROUP_MAXRUN;
+      if (hipMemcpyAsync(d_runs, R, sizeof(GRun) * nrun_b[bi], hipMemcpyHostToDevice, c->st) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, ERM_E_DEVICE, "run table upload");
+      }
+      c->cur_ev = c->capturing ? nullptr : &c->bev[NSTAGE_EV * bi];
+      const uint64_t *offset = off_b.data() + (size_t)bi * GROUP_MAXRUN;
+      TRY(c->fp32 ? group_batch<float>(g, R, nrun_b[bi], offset, ncfg_b[bi], d_runs, nz_b[bi], np_b[bi])
+                  : group_batch<double>(g, R, nrun_b[bi], offset, ncfg_b[bi], d_runs, nz_b[bi], np_b[bi]));
     }
-    c->cur_ev = &c->bev[NSTAGE_EV * bi];
-    st = c->fp32 ? group_batch<float>(g, R, nrun, offset, cfg, d_runs, zi, pi)
-                 : group_batch<double>(g, R, nrun, offset, cfg, d_runs, zi, pi);
+    return ERM_OK;
+  };
+  // everything the captured launches take by value or by address
+  auto key = [&]() {
+    std::string kk(reinterpret_cast<const char *>(g->h_runs), sizeof(GRun) * (size_t)nb * GROUP_MAXRUN);
+    kk.append(reinterpret_cast<const char *>(off_b.data()), sizeof(uint64_t) * off_b.size());
+    char buf[160];
+    snprintf(buf, sizeof buf, "|%d|%d|%d|%d|%d|%d|%d|%d|%d", nb, g->persistent, c->slab_S, c->slab_R, c->xslab_Q,
+             c->xslab_T, c->zcol_T, (int)c->xslab_yf, c->wsol_ones);
+    kk += buf;
+    for (erm_ctx *m : g->ctx) {
+      kk.append(reinterpret_cast<const char *>(&m->kp), sizeof(Params));
+      snprintf(buf, sizeof buf, "|%llu|%d|%d", (unsigned long long)m->alloc_gen, (int)m->have_usreal, m->ns);
+      kk += buf;
+    }
+    return kk;
+  };
+  bool replayed = false;
+  if (!st && g->use_graph && g->gexec && key() == g->gkey) {
+    HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+    if (hipGraphLaunch(g->gexec, c->st) != hipSuccess) {
+      (void)hipGetLastError();
+      st = fail(c, ERM_E_DEVICE, "group graph launch");
+    }
+    HIPCHK(c, hipEventRecord(c->ev[7], c->st));
+    replayed = true;
+  } else if (!st) {
+    st = enqueue();
+    if (!st && g->use_graph) {
+      // capture the same launches (the eager results above stand)
+      if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+      g->gexec = nullptr;
+      g->gkey = key();
+      std::vector<uint64_t> gen;
+      for (erm_ctx *m : g->ctx) gen.push_back(m->alloc_gen);
+      hipGraph_t gr = nullptr;
+      if (hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed) == hipSuccess) {
+        c->capturing = true;
+        const int cs = enqueue();
+        c->capturing = false;
+        const hipError_t e = hipStreamEndCapture(c->st, &gr);
+        bool ok = (cs == ERM_OK && e == hipSuccess && gr);
+        for (size_t i = 0; i < g->ctx.size() && ok; ++i) ok = (g->ctx[i]->alloc_gen == gen[i]);
+        if (ok) ok = (hipGraphInstantiate(&g->gexec, gr, nullptr, nullptr, 0) == hipSuccess);
+        if (gr) (void)hipGraphDestroy(gr);
+        if (!ok) {
+          g->gexec = nullptr;
+          g->use_graph = false;  // later calls run eagerly
+        }
+      } else {
+        g->use_graph = false;
+      }
+      (void)hipGetLastError();
+    }
   }
   c->cur_ev = nullptr;
   if (!st) st = check_status(c);
+  if (replayed) {
+    for (int i = 0; i < NSTAGE_OUT; ++i) g->stage_ms[i] = 0;
+    float ms;
+    if (!st && hipEventElapsedTime(&ms, c->ev[0], c->ev[7]) == hipSuccess) g->stage_ms[NSTAGE_OUT - 1] = ms;
+    if (st) return gfail(g, st, "%s", c->err.c_str());
+    return ERM_OK;
+  }
   if (!st) st = stage_times(c, nb, true);
   if (st) return gfail(g, st, "%s", c->err.c_str());
   for (int i = 0; i < NSTAGE_OUT; ++i) g->stage_ms[i] = c->stage_ms[i];
+  return ERM_OK;
+}
+
+int32_t erm_group_set_graph(erm_group *g, int32_t on) {
+  if (!g) return ERM_E_ARG;
+  if (on < 0 || on > 1) return gfail(g, ERM_E_ARG, "graph flag must be 0 or 1");
+  g->use_graph = (on == 1);
+  if (!g->use_graph && g->gexec) {
+    (void)hipGraphExecDestroy(g->gexec);
+    g->gexec = nullptr;
+  }
   return ERM_OK;
 }
 

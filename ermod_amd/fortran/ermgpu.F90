@@ -347,6 +347,12 @@ module ermgpu
        integer(c_int32_t), value :: max_batch, persistent
      end function erm_group_set_tuning
 
+     integer(c_int32_t) function erm_group_set_graph(grp, on) bind(C, name="erm_group_set_graph")
+       import :: c_ptr, c_int32_t
+       type(c_ptr), value :: grp
+       integer(c_int32_t), value :: on
+     end function erm_group_set_graph
+
      integer(c_int32_t) function erm_group_stage_times(grp, ms10) bind(C, name="erm_group_stage_times")
        import :: c_ptr, c_int32_t, c_double
        type(c_ptr), value :: grp

@@ -317,6 +317,13 @@ int32_t erm_group_insert_random(erm_group *g, const int32_t *n, const uint64_t *
  * atomic counter), bit 0 the z columns, bit 1 the pairs; a stage whose bit
  * is clear runs one launch per run; 2 by default, -1 keeps it. */
 int32_t erm_group_set_tuning(erm_group *g, int32_t max_batch, int32_t persistent);
+/* on = 1: the first erm_group_insert_random of a shape runs eagerly and
+ * captures its batches' launches in a HIP graph; later calls of the same
+ * shape (counts, batch size, persistence, the members' parameters and
+ * buffers, the slab shape of the frame) copy the generator arguments and
+ * launch the graph; any change captures again.  Stage times of a replayed
+ * call hold the total only.  0 (the default): every call runs eagerly. */
+int32_t erm_group_set_graph(erm_group *g, int32_t on);
 /* stage times of the last erm_group_insert_random, as erm_stage_times */
 int32_t erm_group_stage_times(const erm_group *g, double *ms10);
 

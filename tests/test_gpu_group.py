@@ -110,3 +110,42 @@ def test_group_refuses_mismatched_members(mixed):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_group_graph_replay(mixed, prec):
+    """Graph replay of the group's batch sequence: three calls of one shape
+    over two frames (the first eager and captured, the others replayed),
+    with eager calls of the contexts and synchronisations in between; after
+    every call each member's accumulators equal the contexts' run alone,
+    bit for bit."""
+    print_case(f"group graph replay, {prec}")
+    syss, _ = mixed
+    big = syss[-1]
+    frames = [big.frame(3), big.frame(4)]
+    n, seeds = [40, 25, 30, 12], [31, 32, 33, 34]
+    alone = [ErmGPU(sy.system, prec, max_batch=64) for sy in syss]
+    grouped = [ErmGPU(sy.system, prec) for sy in syss]
+    other = ErmGPU(syss[1].system, prec)
+    try:
+        with ErmGroup(grouped) as g:
+            g.set_tuning(max_batch=64, persistent=2)
+            g.set_graph(True)
+            for call, f in enumerate([0, 0, 1, 1]):
+                pos = frames[f]
+                firsts = [1000 * call + k for k in range(4)]
+                g.set_frame(pos, syss[0].cell)
+                g.insert_random(n, seeds, firsts)
+                # unrelated eager work on the device between replays
+                other.set_frame(pos[:syss[1].system.natom], syss[1].cell)
+                other.insert_random(17, 5, first_index=call)
+                for e, sy, k in zip(alone, syss, range(4)):
+                    e.set_frame(pos[:sy.system.natom], sy.cell)
+                    e.insert_random(n[k], seeds[k], first_index=firsts[k])
+                for a, b in zip(alone, grouped):
+                    same_acc(a.fetch(), b.fetch())
+                st = g.stage_times()
+                assert st["total"] > 0
+    finally:
+        for e in alone + grouped + [other]:
+            e.close()
