@@ -212,7 +212,7 @@ def mixed_setup(torch, dev, nframes):
 
 # the shape the autotuner settles on for the mixed-solvent workload, used
 # when it is skipped (profiling runs)
-FIXED_TUNING = dict(max_batch=1024, pme_path="register", slab_planes=8, group_planes=8, slab_threads=512,
+FIXED_TUNING = dict(max_batch=1024, pme_path="register", slab_planes=16, group_planes=8, slab_threads=512,
                     column_threads=256, fold_y=True)
 
 
@@ -260,22 +260,36 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
             group = ErmGroup(engines)
             total = sum(share)
             if tune:
+                # the group's y/x transforms and gather run in the slab shape
+                # of its first member, autotuned alone on a quarter of the
+                # insertions: the shapes of the fixed tuning compete with it
+                t0m = engines[0].tuning()
+                own = dict(slab_planes=t0m["slab_cap"], group_planes=t0m["group_cap"],
+                           slab_threads=t0m["slab_threads"], fold_y=t0m["fold_y"])
+                # (slab_planes caps the slab depth: 16 gives 8-plane slabs of
+                # 16 sites per thread on this workload, 8 gives 4-plane slabs)
+                shapes = [own] + [dict(slab_planes=sp, group_planes=8, slab_threads=512, fold_y=fy)
+                                  for sp, fy in ((16, True), (16, False), (8, True))]
                 best = None
-                for bmax in ((total + 1) // 2, total):
-                    for pers in (0, 1, 2, 3):
-                        group.set_tuning(max_batch=bmax, persistent=pers)
-                        frame(0)
-                        torch.cuda.synchronize()
-                        t0 = time.perf_counter()
-                        frame(1)
-                        torch.cuda.synchronize()
-                        t = time.perf_counter() - t0
-                        if best is None or t < best[0]:
-                            best = (t, bmax, pers)
-                group_tuning = {"max_batch": best[1], "persistent": best[2]}
+                for si, shape in enumerate(shapes):
+                    engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shape)
+                    for bmax in ((total + 1) // 2, total):
+                        for pers in (0, 1, 2, 3):
+                            group.set_tuning(max_batch=bmax, persistent=pers)
+                            frame(0)
+                            torch.cuda.synchronize()
+                            t0 = time.perf_counter()
+                            frame(1)
+                            torch.cuda.synchronize()
+                            t = time.perf_counter() - t0
+                            if best is None or t < best[0]:
+                                best = (t, bmax, pers, si)
+                engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shapes[best[3]])
+                group_tuning = {"max_batch": best[1], "persistent": best[2],
+                                "slab_shape": "first member's own" if best[3] == 0 else shapes[best[3]]}
             else:
                 group_tuning = {"max_batch": total, "persistent": 2}
-            group.set_tuning(**group_tuning)
+            group.set_tuning(max_batch=group_tuning["max_batch"], persistent=group_tuning["persistent"])
         for i in range(warmup):
             frame(i)
         for e in engines:

@@ -738,7 +738,8 @@ __global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
         const int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
         const int gp = zg[idx];
         const int g1 = gp & 1023, g2 = (gp >> 10) & 1023;
-        const T f1 = zq[idx] * zw[(size_t)(2 * p + c3) * np + idx];
+        const T *wz = zw + (size_t)idx * 3 * p;  // the site's weights [3][p]
+        const T f1 = zq[idx] * wz[2 * p + c3];
         T sum = 0;
         for (int c2 = 0; c2 < p; ++c2) {
           int y = g2 - c2;
@@ -750,9 +751,9 @@ __global__ __launch_bounds__(NTHR) void k_pme_slab(Params P, int S, int nsites,
             int x = g1 - c1;
             x += (x < 0) ? K1 : 0;
             V v = row[x];
-            rs += zw[(size_t)c1 * np + idx] * (odd ? v.y : v.x);
+            rs += wz[c1] * (odd ? v.y : v.x);
           }
-          sum += zw[(size_t)(p + c2) * np + idx] * rs;
+          sum += wz[p + c2] * rs;
         }
         acc[r] += (double)(f1 * sum);
       }
@@ -1233,14 +1234,15 @@ __global__ __launch_bounds__(NTHR) void k_pme_ycol_reg(Params P, const typename 
 // complex FFT (c2r_half), then writes the real row in place over its own
 // complex row as (x[2m], x[2m+1]) pairs with a row stride of K1 + 2 values
 // (consecutive lanes on distinct LDS banks; each real row fills exactly its
-// complex row's bytes).  Each thread then adds the planes'
-// contributions to its MAXR owned sites (FP64 sums).
+// complex row's bytes), and repeats x[0], x[1] in the row's two spare values
+// K1, K1 + 1.  Each thread then adds the planes' contributions to its MAXR
+// owned sites (FP64 sums).
 // gather of one group of nq real planes (first plane zz0, rows of K1 + 2)
 // into the per-site sums acc[] of the sites a thread owns.  PP: the spline
 // order at compile time (weights, row offsets and the p^3 products in
 // registers, loads independent); PP = 0 takes it from P.p.
-template <typename T, int K1, int MAXR, int PP>
-__device__ __forceinline__ void xslab_gather(const Params &P, const T *real, int zz0, int nq, int s_beg, size_t np,
+template <typename T, int K1, int MAXR, int PP, int XT>
+__device__ __forceinline__ void xslab_gather(const Params &P, const T *real, int zz0, int nq, int s_beg,
                                              const int *gz, const int *__restrict__ zg, const T *__restrict__ zq,
                                              const T *__restrict__ zw, double *acc) {
   constexpr int K1P = K1 + 2;
@@ -1251,17 +1253,22 @@ __device__ __forceinline__ void xslab_gather(const Params &P, const T *real, int
     const int g3 = gz[r];
     const int clo = max(0, g3 - (zz0 + nq - 1)), chi = min(p - 1, g3 - zz0);
     if (clo > chi) continue;
-    const int idx = s_beg + r * (int)blockDim.x + (int)threadIdx.x;
+    int idx = s_beg + r * XT + (int)threadIdx.x;
+    // formed here, not hoisted out of the caller's plane-group loop (which
+    // would hold the load addresses of every site in registers across the
+    // transforms)
+    asm volatile("" : "+v"(idx));
     const int gp = zg[idx];
     const int g1 = gp & 1023, g2 = (gp >> 10) & 1023;
     const T qv = zq[idx];
+    const T *wz = zw + (size_t)idx * 3 * p;  // the site's weights [3][p]
     if constexpr (PP > 0) {
       T w1[PP], w2[PP];
       int xo[PP], yo[PP];
 #pragma unroll
       for (int k = 0; k < PP; ++k) {
-        w1[k] = zw[(size_t)k * np + idx];
-        w2[k] = zw[(size_t)(PP + k) * np + idx];
+        w1[k] = wz[k];
+        w2[k] = wz[PP + k];
         int x = g1 - k, y = g2 - k;
         xo[k] = x + ((x < 0) ? K1 : 0);
         yo[k] = (y + ((y < 0) ? K2 : 0)) * K1P;
@@ -1277,13 +1284,13 @@ __device__ __forceinline__ void xslab_gather(const Params &P, const T *real, int
           for (int c1 = 0; c1 < PP; ++c1) rs += w1[c1] * row[xo[c1]];
           sum += w2[c2] * rs;
         }
-        acc[r] += (double)(qv * zw[(size_t)(2 * PP + c3) * np + idx] * sum);
+        acc[r] += (double)(qv * wz[2 * PP + c3] * sum);
       }
     } else {
       T w1[MAXP], w2[MAXP];
       for (int k = 0; k < p; ++k) {
-        w1[k] = zw[(size_t)k * np + idx];
-        w2[k] = zw[(size_t)(p + k) * np + idx];
+        w1[k] = wz[k];
+        w2[k] = wz[p + k];
       }
       for (int c3 = clo; c3 <= chi; ++c3) {
         const T *ph = real + (size_t)(g3 - c3 - zz0) * plane;
@@ -1300,7 +1307,89 @@ __device__ __forceinline__ void xslab_gather(const Params &P, const T *real, int
           }
           sum += w2[c2] * rs;
         }
-        acc[r] += (double)(qv * zw[(size_t)(2 * p + c3) * np + idx] * sum);
+        acc[r] += (double)(qv * wz[2 * p + c3] * sum);
+      }
+    }
+  }
+}
+
+// The gather for spline order 4.  The weights of the owned sites that touch
+// the group are loaded four sites at a time (each site's twelve weights are
+// one 16-byte-aligned record: three 16-byte loads in FP32), so their loads
+// are in flight together (one wait per four sites rather than one per
+// site).  The four x values g1 - 3 .. g1 of a row are two pairs of adjacent
+// values, (xa, xa + 1) and (xb, xb + 1), each one two-value LDS read: with
+// x[0], x[1] repeated at K1, K1 + 1 every wrap of the stencil over the row's
+// end is such a pair (g1 = 0: (K1-3, K1-2), (K1-1, K1); g1 = 1: (K1-2, K1-1),
+// (0, 1); g1 = 2: (K1-1, K1), (1, 2)).  Products and sums are formed in the
+// order of xslab_gather, so both give the same values.
+template <typename T, int K1, int MAXR, int XT>
+__device__ __forceinline__ void xslab_gather4(const Params &P, const T *real, int zz0, int nq, int s_beg,
+                                              size_t np, const int *gz, const int *__restrict__ zg,
+                                              const T *__restrict__ zq, const T *__restrict__ zw, double *acc) {
+  constexpr int K1P = K1 + 2;
+  const int K2 = P.K2;
+  const ptrdiff_t plane = (ptrdiff_t)K2 * (K1 / 2 + 1) * 2;  // reals per plane
+  constexpr int RB = MAXR < 4 ? MAXR : 4;  // sites whose loads fly together
+#pragma unroll
+  for (int r0 = 0; r0 < MAXR; r0 += RB) {
+    int gp[RB];
+    T qv[RB], w[RB][12];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      // site r meets the group's planes [zz0, zz0 + nq) when
+      // g3 - 3 <= zz0 + nq - 1 and g3 >= zz0 (none for the padding, g3 < 0)
+      const int g3 = gz[r0 + i];
+      const bool need = g3 >= zz0 && g3 <= zz0 + nq + 2;
+      gp[i] = 0;
+      qv[i] = 0;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) w[i][k] = 0;
+      if (__builtin_amdgcn_ballot_w64(need)) {  // uniform: some lane's site r
+        int idx = min(s_beg + (r0 + i) * XT + (int)threadIdx.x, (int)np - 1);
+        asm volatile("" : "+v"(idx));  // not hoisted out of the plane-group loop
+        gp[i] = zg[idx];
+        qv[i] = zq[idx];
+        const T *wz = static_cast<const T *>(__builtin_assume_aligned(zw + (size_t)idx * 12, 16));
+#pragma unroll
+        for (int k = 0; k < 12; ++k) w[i][k] = wz[k];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int r = r0 + i;
+      const int g3 = gz[r];
+      const int clo = max(0, g3 - (zz0 + nq - 1)), chi = min(3, g3 - zz0);
+      if (clo > chi) continue;
+      const int g1 = gp[i] & 1023, g2 = (gp[i] >> 10) & 1023;
+      const int xa = g1 - 3 + ((g1 < 3) ? K1 : 0);
+      const int xb = g1 - 1 + ((g1 < 1) ? K1 : 0);
+      int ra[4], rb[4];  // the row's two pairs, offsets within a plane
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int y = g2 - k;
+        const int ro = (y + ((y < 0) ? K2 : 0)) * K1P;
+        ra[k] = ro + xa;
+        rb[k] = ro + xb;
+      }
+      const T *ph0 = real + (ptrdiff_t)(g3 - zz0) * plane;  // plane of c3 = 0
+#pragma unroll
+      for (int c3 = 0; c3 < 4; ++c3) {
+        if (c3 < clo || c3 > chi) continue;
+        const T *ph = ph0 - (ptrdiff_t)c3 * plane;
+        T sum = 0;
+#pragma unroll
+        for (int c2 = 0; c2 < 4; ++c2) {
+          const T *a = ph + ra[c2], *bb = ph + rb[c2];
+          const T a0 = a[0], a1 = a[1], b0 = bb[0], b1 = bb[1];
+          T rs = 0;
+          rs += w[i][0] * b1;
+          rs += w[i][1] * b0;
+          rs += w[i][2] * a1;
+          rs += w[i][3] * a0;
+          sum += w[i][4 + c2] * rs;
+        }
+        acc[r] += (double)(qv[i] * w[i][8 + c3] * sum);
       }
     }
   }
@@ -1385,16 +1474,17 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
       V *d = reinterpret_cast<V *>(real + (size_t)q * cplane * 2 + (size_t)y * K1P);
 #pragma unroll
       for (int m = 0; m < K1 / 2; ++m) d[m] = Z[m];
+      d[K1 / 2] = Z[0];  // x[0], x[1] again at K1, K1 + 1 (the gather's wrap)
     }
     __syncthreads();
     // gather: owned sites whose support g3 - c3 meets planes [zz0, zz0 + nq);
     // the common spline orders get fully unrolled register code
     if (p == 4)
-      xslab_gather<T, K1, MAXR, 4>(P, real, zz0, nq, s_beg, np, gz, zg, zq, zw, acc);
+      xslab_gather4<T, K1, MAXR, XT>(P, real, zz0, nq, s_beg, np, gz, zg, zq, zw, acc);
     else if (p == 6)
-      xslab_gather<T, K1, MAXR, 6>(P, real, zz0, nq, s_beg, np, gz, zg, zq, zw, acc);
+      xslab_gather<T, K1, MAXR, 6, XT>(P, real, zz0, nq, s_beg, gz, zg, zq, zw, acc);
     else
-      xslab_gather<T, K1, MAXR, 0>(P, real, zz0, nq, s_beg, np, gz, zg, zq, zw, acc);
+      xslab_gather<T, K1, MAXR, 0, XT>(P, real, zz0, nq, s_beg, gz, zg, zq, zw, acc);
     __syncthreads();
   }
 #pragma unroll
@@ -1435,7 +1525,8 @@ static __global__ void k_mol_sum(Params P, int nsites, const double *__restrict_
 }
 
 // per-frame sort of the solvent sites by grid base g3 (z-sorted copies of
-// base, charge and B-spline weights, the latter as [3p][nsites]); within a
+// base, charge and B-spline weights, the latter one record [3][p] per site,
+// [nsites][3][p]); within a
 // plane by row g2 as well when KB = K2 (buckets g3 KB + g2), which puts the
 // sites of a wave on few rows of a plane for the slab gather
 constexpr int ZBUCKET_MAX = 4096;
@@ -1479,7 +1570,7 @@ __global__ __launch_bounds__(256) void k_zscatter(int nsites, int nbk, int KB, i
   zpos[t] = zt;
   zg[zt] = sg[t];
   zq[zt] = sq[t];
-  for (int k = 0; k < 3 * p; ++k) zw[(size_t)k * nsites + zt] = sw[(size_t)t * 3 * p + k];
+  for (int k = 0; k < 3 * p; ++k) zw[(size_t)zt * 3 * p + k] = sw[(size_t)t * 3 * p + k];
 }
 
 // --------------------------------------------------------------------------
@@ -1901,13 +1992,16 @@ __global__ __launch_bounds__(NTHR) void k_pair(Params P, const double *__restric
 // --------------------------------------------------------------------------
 constexpr int PT_GRP = 8;                 // solute sites per compaction group
 constexpr int PT_QCAP = 64 * PT_GRP;      // queued pairs per wave (at most)
+constexpr float PT_FAR = 1e18f;           // sentinel displacement (A): (2 PT_FAR)^2 < FLT_MAX
 
 template <typename T> struct V4T;
 template <> struct V4T<float> { using type = float4; };
 template <> struct V4T<double> { using type = double4; };
 
 // LDS of k_pair_tile.  Head: the solute's sites as (x, y, z, q) and their
-// low parts (T4 each), the LJ rows [ns][nvt] (eps, sig2), the pillar segments
+// low parts (T4 each, ns rounded up to whole groups of PT_GRP: the slots
+// past ns hold a sentinel site far outside every cutoff), the LJ rows
+// [ns][nvt] (eps, sig2), the pillar segments
 // (segcap = 2 nb1 nb2 slots).  Per wave: the tile's sites (x, y, z, q) and
 // (low parts, LJ type), molecule offsets [65] and first sites [64], the queue
 // [PT_QCAP] (int) and its energies (f64), the per-site sums [64] (f64), the
@@ -1917,7 +2011,8 @@ template <typename T> __host__ __device__ inline size_t pair_tile_wave_lds() {
          ~(size_t)15;
 }
 template <typename T> __host__ __device__ inline size_t pair_tile_head(int ns, int nvt, int segcap) {
-  const size_t a = sizeof(T) * ((size_t)8 * ns + (size_t)2 * ns * nvt) + sizeof(int) * (2 * (size_t)segcap + 2) + 64;
+  const size_t nsp = (size_t)(ns + PT_GRP - 1) / PT_GRP * PT_GRP;
+  const size_t a = sizeof(T) * ((size_t)8 * nsp + (size_t)2 * ns * nvt) + sizeof(int) * (2 * (size_t)segcap + 2) + 64;
   return (a + 15) & ~(size_t)15;
 }
 template <typename T> __host__ __device__ inline size_t pair_tile_lds(int ns, int nvt, int segcap, int nwave) {
@@ -1953,9 +2048,10 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   using T2 = typename C2<T>::type;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int nvt = P.nvt;
-  T4 *us4 = reinterpret_cast<T4 *>(lds_raw);  // [ns] (x, y, z, q) offsets from the centre
-  T4 *ul4 = us4 + ns;                         // [ns] low parts (x, y, z, 0)
-  T2 *ulj = reinterpret_cast<T2 *>(ul4 + ns); // [ns][nvt] (eps, sig2)
+  const int nsp = (ns + PT_GRP - 1) / PT_GRP * PT_GRP;
+  T4 *us4 = reinterpret_cast<T4 *>(lds_raw);  // [nsp] (x, y, z, q) offsets from the centre
+  T4 *ul4 = us4 + nsp;                        // [nsp] low parts (x, y, z, 0)
+  T2 *ulj = reinterpret_cast<T2 *>(ul4 + nsp); // [ns][nvt] (eps, sig2)
   const int segcap = 2 * P.nb[1] * P.nb[2];
   int *seg_beg = reinterpret_cast<int *>(ulj + ns * nvt);
   int *seg_pre = seg_beg + segcap;
@@ -1976,17 +2072,25 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
 
   const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
   const double Rs = sref[4 * b + 3];
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-    const double d0 = xs[3 * s] - C0, d1 = xs[3 * s + 1] - C1, d2 = xs[3 * s + 2] - C2v;
+  for (int s = threadIdx.x; s < nsp; s += blockDim.x) {
     T4 u, l;
-    u.x = (T)d0;
-    u.y = (T)d1;
-    u.z = (T)d2;
-    u.w = slt_q[s];
-    l.x = (T)(d0 - (double)u.x);
-    l.y = (T)(d1 - (double)u.y);
-    l.z = (T)(d2 - (double)u.z);
-    l.w = 0;
+    l.x = l.y = l.z = l.w = 0;
+    if (s < ns) {
+      const double d0 = xs[3 * s] - C0, d1 = xs[3 * s + 1] - C1, d2 = xs[3 * s + 2] - C2v;
+      u.x = (T)d0;
+      u.y = (T)d1;
+      u.z = (T)d2;
+      u.w = slt_q[s];
+      l.x = (T)(d0 - (double)u.x);
+      l.y = (T)(d1 - (double)u.y);
+      l.z = (T)(d2 - (double)u.z);
+    } else {
+      // sentinel: x displacement -PT_FAR, so its distance to any tile site
+      // (at most PT_FAR away, the inactive lanes' sentinel) squares to at
+      // least PT_FAR^2, far outside every cutoff and finite in FP32
+      u.x = -(T)PT_FAR;
+      u.y = u.z = u.w = 0;
+    }
     us4[s] = u;
     ul4[s] = l;
   }
@@ -2101,6 +2205,9 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
           tl.w = __longlong_as_double((long long)pc);
         }
       }
+      // a lane without an active site takes a sentinel displacement, so the
+      // distance compare alone decides every pair below
+      if (!act) tu.x = (T)PT_FAR;
       tu4[lane] = tu;
       tl4[lane] = tl;
       if (t0 + 64 < tot_sites) fetch(t0 + 64);
@@ -2112,7 +2219,7 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
         // (one wait for the eight broadcast reads)
         T4 usg[PT_GRP];
 #pragma unroll
-        for (int g = 0; g < PT_GRP; ++g) usg[g] = us4[min(s0 + g, ns - 1)];
+        for (int g = 0; g < PT_GRP; ++g) usg[g] = us4[s0 + g];
 #pragma unroll
         for (int g = 0; g < PT_GRP; ++g) {
           const int s = s0 + g;
@@ -2126,8 +2233,10 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
             ay = (ay > h1) ? L1 - ay : ay;
             az = (az > h2) ? L2 - az : az;
           }
-          const bool in = act && s < ns && (ax * ax + ay * ay + az * az) <= cut2q;
-          const unsigned long long m = __ballot(in);
+          // inactive lanes and the padding sites past ns fail this compare
+          // through their sentinels; a bare compare is the ballot's mask
+          const bool in = (ax * ax + ay * ay + az * az) <= cut2q;
+          const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
           const int pos = qn + lanes_below(m);
           pend[g] = in ? pos : -1;
           if (in) wq[pos] = (s << 6) | lane;
