@@ -1998,9 +1998,10 @@ template <typename T> struct V4T;
 template <> struct V4T<float> { using type = float4; };
 template <> struct V4T<double> { using type = double4; };
 
-// LDS of k_pair_tile.  Head: the solute's sites as (x, y, z, q) and their
-// low parts (T4 each, ns rounded up to whole groups of PT_GRP: the slots
-// past ns hold a sentinel site far outside every cutoff), the LJ rows
+// LDS of k_pair_tile.  Head: the solute's sites as (x, y, z, q), their low
+// parts and their test rows (-2x, -2y, -2z, threshold) (T4 each, ns rounded
+// up to whole groups of PT_GRP: the slots past ns hold sentinels that fail
+// every test), the LJ rows
 // [ns][nvt] (eps, sig2), the pillar segments
 // (segcap = 2 nb1 nb2 slots).  Per wave: the tile's sites (x, y, z, q) and
 // (low parts, LJ type), molecule offsets [65] and first sites [64], the queue
@@ -2012,7 +2013,7 @@ template <typename T> __host__ __device__ inline size_t pair_tile_wave_lds() {
 }
 template <typename T> __host__ __device__ inline size_t pair_tile_head(int ns, int nvt, int segcap) {
   const size_t nsp = (size_t)(ns + PT_GRP - 1) / PT_GRP * PT_GRP;
-  const size_t a = sizeof(T) * ((size_t)8 * nsp + (size_t)2 * ns * nvt) + sizeof(int) * (2 * (size_t)segcap + 2) + 64;
+  const size_t a = sizeof(T) * ((size_t)12 * nsp + (size_t)2 * ns * nvt) + sizeof(int) * (2 * (size_t)segcap + 2) + 64;
   return (a + 15) & ~(size_t)15;
 }
 template <typename T> __host__ __device__ inline size_t pair_tile_lds(int ns, int nvt, int segcap, int nwave) {
@@ -2051,7 +2052,8 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   const int nsp = (ns + PT_GRP - 1) / PT_GRP * PT_GRP;
   T4 *us4 = reinterpret_cast<T4 *>(lds_raw);  // [nsp] (x, y, z, q) offsets from the centre
   T4 *ul4 = us4 + nsp;                        // [nsp] low parts (x, y, z, 0)
-  T2 *ulj = reinterpret_cast<T2 *>(ul4 + nsp); // [ns][nvt] (eps, sig2)
+  T4 *ut4 = ul4 + nsp;                        // [nsp] test rows (-2x, -2y, -2z, threshold)
+  T2 *ulj = reinterpret_cast<T2 *>(ut4 + nsp); // [ns][nvt] (eps, sig2)
   const int segcap = 2 * P.nb[1] * P.nb[2];
   int *seg_beg = reinterpret_cast<int *>(ulj + ns * nvt);
   int *seg_pre = seg_beg + segcap;
@@ -2072,9 +2074,24 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
 
   const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
   const double Rs = sref[4 * b + 3];
+  // sites farther than rc + Rs from the centre meet no solute site within
+  // the cutoff; the minimum image of a pair is the site's own wrap unless
+  // the reach rc + Rs reaches half the box
+  const double reach = P.rcmax + Rs + 1e-6 * (1.0 + P.rcmax), reach2 = reach * reach;
+  const bool img = reach >= 0.5 * fmin(P.L[0], fmin(P.L[1], P.L[2]));
+  // Without per-pair images a tile site t (|t| <= reach) and a solute site s
+  // are tested as |t|^2 - 2 t.s <= cut2 - |s|^2 (three FMAs).  The bound on
+  // its rounding, a few units in the last place of (reach + Rs)^2 (about
+  // 4e-7 relative in FP32), sits well inside the margin added to cut2; the
+  // exact distance of every queued pair decides.
+  const double xr = reach + Rs;
+  const double cut2x = F32 ? P.cut2 + fmax(P.cut2 * 1e-5, 4e-6 * xr * xr) : P.cut2 + 1e-12 * xr * xr;
   for (int s = threadIdx.x; s < nsp; s += blockDim.x) {
-    T4 u, l;
+    T4 u, l, t;
     l.x = l.y = l.z = l.w = 0;
+    // padding: never below the threshold (|t|^2 >= 0 > -1)
+    t.x = t.y = t.z = 0;
+    t.w = (T)-1;
     if (s < ns) {
       const double d0 = xs[3 * s] - C0, d1 = xs[3 * s + 1] - C1, d2 = xs[3 * s + 2] - C2v;
       u.x = (T)d0;
@@ -2084,6 +2101,10 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
       l.x = (T)(d0 - (double)u.x);
       l.y = (T)(d1 - (double)u.y);
       l.z = (T)(d2 - (double)u.z);
+      t.x = (T)-2 * u.x;
+      t.y = (T)-2 * u.y;
+      t.z = (T)-2 * u.z;
+      t.w = (T)(cut2x - ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z));
     } else {
       // sentinel: x displacement -PT_FAR, so its distance to any tile site
       // (at most PT_FAR away, the inactive lanes' sentinel) squares to at
@@ -2093,6 +2114,7 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
     }
     us4[s] = u;
     ul4[s] = l;
+    ut4[s] = t;
   }
   for (int t = threadIdx.x; t < ns * nvt; t += blockDim.x) {
     T2 v;
@@ -2106,18 +2128,14 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   const int nsg = pillar_segments(P, Cw, R, cell_start, seg_beg, seg_pre);
   const int total = seg_pre[nsg];
   const int sj = slt_j[b];
-  // sites farther than rc + Rs from the centre meet no solute site within
-  // the cutoff; the minimum image of a pair is the site's own wrap unless
-  // the reach rc + Rs reaches half the box
-  const double reach = P.rcmax + Rs + 1e-6 * (1.0 + P.rcmax), reach2 = reach * reach;
-  const bool img = reach >= 0.5 * fmin(P.L[0], fmin(P.L[1], P.L[2]));
   const double iL0 = 1.0 / P.L[0], iL1 = 1.0 / P.L[1], iL2 = 1.0 / P.L[2];
   const T L0 = (T)P.L[0], L1 = (T)P.L[1], L2 = (T)P.L[2];
   const T h0 = (T)P.halfL[0], h1 = (T)P.halfL[1], h2 = (T)P.halfL[2];
   const T lw2 = (T)P.lw2, up2 = (T)P.up2, el2 = (T)P.el2, cut2 = (T)P.cut2, alpha = (T)P.screen;
   const T w3 = (up2 - lw2) * (up2 - lw2) * (up2 - lw2), iw3 = (T)1 / w3;  // switch denominator
-  // the queue test on the high parts (FP32) keeps a margin; the exact test
-  // on the full displacement decides
+  // with per-pair images the queue test is the plain distance on the high
+  // parts (FP32, with a margin); the exact test on the full displacement
+  // decides
   const T cut2q = F32 ? (T)(P.cut2 * (1.0 + 1e-5)) : cut2;
   double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
   const int nchunk = (total + 63) >> 6;
@@ -2211,36 +2229,47 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
       tu4[lane] = tu;
       tl4[lane] = tl;
       if (t0 + 64 < tot_sites) fetch(t0 + 64);
+      const T t2 = tu.x * tu.x + tu.y * tu.y + tu.z * tu.z;  // PT_FAR^2 for an inactive lane
       double e = 0.0;  // this site's energy, in solute-site order
       for (int s0 = 0; s0 < ns; s0 += PT_GRP) {
         int pend[PT_GRP];
         int qn = 0;
-        // the group's solute sites, all read before the queue writes below
-        // (one wait for the eight broadcast reads)
-        T4 usg[PT_GRP];
-#pragma unroll
-        for (int g = 0; g < PT_GRP; ++g) usg[g] = us4[s0 + g];
-#pragma unroll
-        for (int g = 0; g < PT_GRP; ++g) {
-          const int s = s0 + g;
-          const T4 us = usg[g];
-          T ax = tu.x - us.x, ay = tu.y - us.y, az = tu.z - us.z;
-          if (img) {
-            ax = fabs(ax);
-            ay = fabs(ay);
-            az = fabs(az);
-            ax = (ax > h0) ? L0 - ax : ax;
-            ay = (ay > h1) ? L1 - ay : ay;
-            az = (az > h2) ? L2 - az : az;
-          }
-          // inactive lanes and the padding sites past ns fail this compare
-          // through their sentinels; a bare compare is the ballot's mask
-          const bool in = (ax * ax + ay * ay + az * az) <= cut2q;
+        // appends pair (s0 + g, lane) to the queue where `in` holds; inactive
+        // lanes and the padding sites past ns fail every test through their
+        // sentinels, so a bare compare is the ballot's mask
+        auto enqueue = [&](int g, bool in) {
           const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
           const int pos = qn + lanes_below(m);
           pend[g] = in ? pos : -1;
-          if (in) wq[pos] = (s << 6) | lane;
+          if (in) wq[pos] = ((s0 + g) << 6) | lane;
           qn += (int)__popcll(m);
+        };
+        // the group's solute rows, all read before the queue writes below
+        // (one wait for the eight broadcast reads)
+        T4 usg[PT_GRP];
+        if (!img) {
+#pragma unroll
+          for (int g = 0; g < PT_GRP; ++g) usg[g] = ut4[s0 + g];
+#pragma unroll
+          for (int g = 0; g < PT_GRP; ++g) {
+            const T4 ut = usg[g];
+            T v = fma(tu.x, ut.x, t2);
+            v = fma(tu.y, ut.y, v);
+            v = fma(tu.z, ut.z, v);
+            enqueue(g, v <= ut.w);
+          }
+        } else {
+#pragma unroll
+          for (int g = 0; g < PT_GRP; ++g) usg[g] = us4[s0 + g];
+#pragma unroll
+          for (int g = 0; g < PT_GRP; ++g) {
+            const T4 us = usg[g];
+            T ax = fabs(tu.x - us.x), ay = fabs(tu.y - us.y), az = fabs(tu.z - us.z);
+            ax = (ax > h0) ? L0 - ax : ax;
+            ay = (ay > h1) ? L1 - ay : ay;
+            az = (az > h2) ? L2 - az : az;
+            enqueue(g, (ax * ax + ay * ay + az * az) <= cut2q);
+          }
         }
         wave_lds_sync();
         // the queued pairs, every lane busy; in FP32 the pairs that need FP64
