@@ -21,7 +21,7 @@ PEAK = os.path.join(HERE, "csrc", "ermpeak.hip")
 # register-FFT grid lengths per precision (the run-time dispatch in
 # ermgpu.hip covers exactly these)
 PME_UNITS = [("double", k) for k in (16, 24, 32, 48, 64)] + [("float", k) for k in (16, 24, 32, 48, 64, 96)]
-HEADERS = [os.path.join(HERE, "csrc", f) for f in ("ermgpu_dev.h", "ermgpu_kern.h", "ermgpu_grad.h",
+HEADERS = [os.path.join(HERE, "csrc", f) for f in ("ermgpu_dev.h", "ermgpu_kern.h", "ermgpu_grad.h", "ermgpu_erfc.h",
                                                          "ermgpu_traj.h", "ermgpu_ctx.h")] + [
     os.path.join(ROOT, "include", "ermgpu.h")]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",

@@ -15,6 +15,7 @@
 #pragma once
 
 #include "ermgpu_dev.h"
+#include "ermgpu_erfc.h"
 
 namespace erm {
 
@@ -1833,6 +1834,21 @@ static __device__ int pillar_segments(const Params &P, const double Cw[3], doubl
 // amplifies the rounding of r) or a Coulomb term |q q| / r above COUL_HOT
 // kcal/mol (first-shell pairs of charged sites).  Such pairs are few.
 constexpr float COUL_HOT = 24.0f;
+// erfc(x) for 0 <= x <= ERFC_XMAX in FP32: exp(-x^2) times the fitted
+// polynomial R(u), u = x / ERFC_XMAX (ermgpu_erfc.h, from erfc_fit.py);
+// largest absolute error about 1e-7 over the range (one FP32 unit of erfc
+// near 0), in 15 operations
+__device__ __forceinline__ float erfc_fit(float x) {
+  const float u = x * (1.0f / ERFC_XMAX);
+  float r = ERFC_R[ERFC_DEG];
+#pragma unroll
+  for (int k = ERFC_DEG - 1; k >= 0; --k) r = __builtin_fmaf(r, u, ERFC_R[k]);
+  // exp(-x^2) = 2^(-x^2 log2 e); the argument stays above -29.3, no
+  // denormal or range handling needed
+  const float e = __builtin_amdgcn_exp2f(-(x * x) * 1.44269504088896341f);
+  return e * r;
+}
+
 template <typename T> __device__ __forceinline__ bool hot_pair(T eps, T sig2, T qq, T r2) {
   return (eps > (T)0 && sig2 > (T)1.2599211 * r2) || (qq * qq > (T)(COUL_HOT * COUL_HOT) * r2);
 }
@@ -2137,6 +2153,8 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   // parts (FP32, with a margin); the exact test on the full displacement
   // decides
   const T cut2q = F32 ? (T)(P.cut2 * (1.0 + 1e-5)) : cut2;
+  // FP32 Coulomb: the fitted erfc where alpha r stays within its range
+  const bool erfc_in_fit = P.screen * sqrt(P.el2) <= (double)ERFC_XMAX;
   double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
   const int nchunk = (total + 63) >> 6;
   for (int c = split * nw + wv; c < nchunk; c += nsplit * nw) {
@@ -2352,7 +2370,8 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
                 }
                 if (r2 <= el2) {
                   if constexpr (F32) {
-                    ep += us.w * qsv * erfcf(alpha * (r2 * rs)) * rs;
+                    const float xe = alpha * (r2 * rs);
+                    ep += us.w * qsv * (erfc_in_fit ? erfc_fit(xe) : erfcf(xe)) * rs;
                   } else {
                     ep += coul_t<T>(us.w * qsv, sqrt(r2), alpha);
                   }
