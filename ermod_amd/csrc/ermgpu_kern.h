@@ -2021,10 +2021,12 @@ template <> struct V4T<double> { using type = double4; };
 // [ns][nvt] (eps, sig2), the pillar segments
 // (segcap = 2 nb1 nb2 slots).  Per wave: the tile's sites (x, y, z, q) and
 // (low parts, LJ type), molecule offsets [65] and first sites [64], the queue
-// [PT_QCAP] (int) and its energies (f64), the per-site sums [64] (f64), the
-// slots of the queued pairs evaluated in FP64 [PT_QCAP] (int).
+// [PT_QCAP + 64] (int) and its energies [PT_QCAP + 64] (f64; the last 64 are
+// the lanes' spare slots, energy 0), the per-site sums [64] (f64), the slots
+// of the queued pairs evaluated in FP64 [PT_QCAP] (int).
 template <typename T> __host__ __device__ inline size_t pair_tile_wave_lds() {
-  return ((size_t)2 * 64 * 4 * sizeof(T) + 65 * 4 + 64 * 4 + (size_t)PT_QCAP * (4 + 8 + 4) + 64 * 8 + 16 + 15) &
+  return ((size_t)2 * 64 * 4 * sizeof(T) + 65 * 4 + 64 * 4 + (size_t)PT_QCAP * (4 + 8 + 4) + 64 * (4 + 8) + 64 * 8 +
+          16 + 15) &
          ~(size_t)15;
 }
 template <typename T> __host__ __device__ inline size_t pair_tile_head(int ns, int nvt, int segcap) {
@@ -2080,12 +2082,12 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   T4 *tl4 = tu4 + 64;                     // [64] (low x, low y, low z, LJ type bits)
   int *woff = reinterpret_cast<int *>(tl4 + 64);  // [65] first chunk site of molecule lane
   int *wfirst = woff + 65;                // [64] first sorted site of molecule lane
-  int *wq = wfirst + 64;                  // [PT_QCAP] queued pairs: solute site << 6 | tile lane
+  int *wq = wfirst + 64;                  // [PT_QCAP + 64] queued pairs: solute site << 6 | tile lane
   // (a byte offset from wl, rounded down to 8 over the spare slot of wq, so
   // the pointers stay LDS pointers for the compiler)
   double *weq = reinterpret_cast<double *>(
-      wl + (((size_t)128 * sizeof(T4) + (size_t)(65 + 64 + PT_QCAP + 1) * sizeof(int)) & ~(size_t)7));
-  double *wes = weq + PT_QCAP;            // [64] per-site sums of a tile
+      wl + (((size_t)128 * sizeof(T4) + (size_t)(65 + 64 + PT_QCAP + 64 + 1) * sizeof(int)) & ~(size_t)7));
+  double *wes = weq + PT_QCAP + 64;       // [64] per-site sums of a tile
   int *whq = reinterpret_cast<int *>(wes + 64);  // [PT_QCAP] queue slots of the pairs evaluated in FP64
 
   const double C0 = sref[4 * b], C1 = sref[4 * b + 1], C2v = sref[4 * b + 2];
@@ -2156,6 +2158,7 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   // FP32 Coulomb: the fitted erfc where alpha r stays within its range
   const bool erfc_in_fit = P.screen * sqrt(P.el2) <= (double)ERFC_XMAX;
   double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
+  weq[PT_QCAP + lane] = 0.0;  // the lanes' spare slots: never written again
   const int nchunk = (total + 63) >> 6;
   for (int c = split * nw + wv; c < nchunk; c += nsplit * nw) {
     // molecule of this lane: flattened candidate c*64 + lane
@@ -2258,8 +2261,13 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
         auto enqueue = [&](int g, bool in) {
           const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
           const int pos = qn + lanes_below(m);
-          pend[g] = in ? pos : -1;
-          if (in) wq[pos] = ((s0 + g) << 6) | lane;
+          // a lane without the pair writes its own spare slot, whose energy
+          // stays 0: no branch here or in the sums (a select on the ballot
+          // mask, kept from becoming a branch around pos)
+          int slot;
+          asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(slot) : "v"(PT_QCAP + lane), "v"(pos), "s"(m));
+          pend[g] = slot;
+          wq[slot] = ((s0 + g) << 6) | lane;
           qn += (int)__popcll(m);
         };
         // the group's solute rows, all read before the queue writes below
@@ -2426,7 +2434,7 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
         wave_lds_sync();
 #pragma unroll
         for (int g = 0; g < PT_GRP; ++g)
-          if (pend[g] >= 0) e += weq[pend[g]];
+          e += weq[pend[g]];
         wave_lds_sync();
       }
       wes[lane] = act ? e : 0.0;
