@@ -2158,7 +2158,11 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
   // FP32 Coulomb: the fitted erfc where alpha r stays within its range
   const bool erfc_in_fit = P.screen * sqrt(P.el2) <= (double)ERFC_XMAX;
   double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
-  weq[PT_QCAP + lane] = 0.0;  // the lanes' spare slots: never written again
+  // the lanes' spare energy slots hold 0 throughout (only 0 is written
+  // there); every queue slot holds a valid entry (solute site 0), so the
+  // evaluation reads and writes a full round of slots without branches
+  weq[PT_QCAP + lane] = 0.0;
+  for (int k = lane; k < PT_QCAP + 64; k += 64) wq[k] = lane;
   const int nchunk = (total + 63) >> 6;
   for (int c = split * nw + wv; c < nchunk; c += nsplit * nw) {
     // molecule of this lane: flattened candidate c*64 + lane
@@ -2303,9 +2307,12 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
         // code runs on full waves too
         int nh = 0;
         for (int q0 = 0; q0 < qn; q0 += 64) {
+          // a round of 64 slots: the lanes past qn read a stale (valid)
+          // entry and write energy 0 (slots below PT_QCAP + 64, so a spare
+          // slot keeps its 0)
           const int qi = q0 + lane;
           const bool valid = qi < qn;
-          const int ent = valid ? wq[qi] : 0;
+          const int ent = wq[qi];
           const int i = ent & 63, s = ent >> 6;
           const T4 v = tu4[i], vl = tl4[i], us = us4[s];
           T ax, ay, az;
@@ -2342,52 +2349,69 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
           const unsigned long long mh = __ballot(hot);
           if (hot) whq[nh + lanes_below(mh)] = qi;
           nh += (int)__popcll(mh);
-          if (valid && !hot) {
-            T ep = 0;
-            if (inr) {
-              if constexpr (BARE) {
-                const T rst = sqrt(r2);
-                ep = lj_bare_t<T>(P, eps, sig2, r2, rst);
-                if (rst < (T)P.elecut) ep += us.w * qsv / rst;
-              } else {
-                // FP32: one hardware reciprocal square root (1 ulp) per pair
-                // in place of the divisions and the rounded square root;
-                // the pairs it reaches are not hot, so their terms are below
-                // 24 kcal/mol (Coulomb) and eps (LJ)
-                const T rs = F32 ? (T)__builtin_amdgcn_rsqf((float)r2) : (T)0;
-                const T ir2 = F32 ? rs * rs : (T)1 / r2;
-                if (P.ljswitch == 0) {
-                  // CHARMM potential switch without branches: the low-range
-                  // term, scaled by the switch beyond lwljcut, zero beyond
-                  // upljcut or for eps = 0 (the expressions of lj_low_t and
-                  // lj_switch_t)
-                  const T invr2 = F32 ? sig2 * ir2 : sig2 / r2;
-                  const T invr6 = invr2 * invr2 * invr2;
-                  const T elo = (T)4 * eps * invr6 * (invr6 - (T)1);
-                  const T d = r2 - up2;
-                  const T sw = ((T)2 * r2 + up2 - (T)3 * lw2) * (d * d);
-                  const T esw = F32 ? elo * sw * iw3 : elo * sw / (w3);
-                  T elj = (r2 <= lw2) ? elo : esw;
-                  elj = (r2 <= up2 && eps > (T)0) ? elj : (T)0;
-                  ep = elj;
-                } else if (eps > (T)0) {
-                  if (r2 <= lw2)
-                    ep += lj_low_t<T>(P, eps, sig2, r2);
-                  else if (r2 <= up2)
-                    ep += lj_switch_t<T>(P, eps, sig2, r2);
-                }
-                if (r2 <= el2) {
-                  if constexpr (F32) {
-                    const float xe = alpha * (r2 * rs);
-                    ep += us.w * qsv * (erfc_in_fit ? erfc_fit(xe) : erfcf(xe)) * rs;
-                  } else {
-                    ep += coul_t<T>(us.w * qsv, sqrt(r2), alpha);
-                  }
-                }
+          T ep = 0;
+          if constexpr (F32 && !BARE) {
+            if (P.ljswitch == 0) {
+              // every lane computes, one select keeps the pairs inside the
+              // cutoff: one hardware reciprocal square root (1 ulp) per pair
+              // in place of the divisions and the rounded square root (the
+              // pairs it reaches are not hot, so their terms are below
+              // 24 kcal/mol (Coulomb) and eps (LJ)); the CHARMM potential
+              // switch as the low-range term, scaled by the switch beyond
+              // lwljcut, zero beyond upljcut or for eps = 0 (the expressions
+              // of lj_low_t and lj_switch_t)
+              const T rs = (T)__builtin_amdgcn_rsqf((float)r2);
+              const T invr2 = sig2 * (rs * rs);
+              const T invr6 = invr2 * invr2 * invr2;
+              const T elo = (T)4 * eps * invr6 * (invr6 - (T)1);
+              const T d = r2 - up2;
+              const T sw = ((T)2 * r2 + up2 - (T)3 * lw2) * (d * d);
+              const T esw = elo * sw * iw3;
+              T elj = (r2 <= lw2) ? elo : esw;
+              elj = (r2 <= up2 && eps > (T)0) ? elj : (T)0;
+              const float xe = alpha * (r2 * rs);
+              const T ec = us.w * qsv * (erfc_in_fit ? erfc_fit(xe) : erfcf(xe)) * rs;
+              const T et = elj + ((r2 <= el2) ? ec : (T)0);
+              ep = (inr && !hot) ? et : (T)0;
+            } else if (inr && !hot) {
+              const T rs = (T)__builtin_amdgcn_rsqf((float)r2);
+              if (eps > (T)0) {
+                if (r2 <= lw2)
+                  ep += lj_low_t<T>(P, eps, sig2, r2);
+                else if (r2 <= up2)
+                  ep += lj_switch_t<T>(P, eps, sig2, r2);
+              }
+              if (r2 <= el2) {
+                const float xe = alpha * (r2 * rs);
+                ep += us.w * qsv * (erfc_in_fit ? erfc_fit(xe) : erfcf(xe)) * rs;
               }
             }
-            weq[qi] = (double)ep;
+          } else if (inr && !hot) {
+            if constexpr (BARE) {
+              const T rst = sqrt(r2);
+              ep = lj_bare_t<T>(P, eps, sig2, r2, rst);
+              if (rst < (T)P.elecut) ep += us.w * qsv / rst;
+            } else {
+              if (P.ljswitch == 0) {
+                const T invr2 = sig2 / r2;
+                const T invr6 = invr2 * invr2 * invr2;
+                const T elo = (T)4 * eps * invr6 * (invr6 - (T)1);
+                const T d = r2 - up2;
+                const T sw = ((T)2 * r2 + up2 - (T)3 * lw2) * (d * d);
+                const T esw = elo * sw / (w3);
+                T elj = (r2 <= lw2) ? elo : esw;
+                elj = (r2 <= up2 && eps > (T)0) ? elj : (T)0;
+                ep = elj;
+              } else if (eps > (T)0) {
+                if (r2 <= lw2)
+                  ep += lj_low_t<T>(P, eps, sig2, r2);
+                else if (r2 <= up2)
+                  ep += lj_switch_t<T>(P, eps, sig2, r2);
+              }
+              if (r2 <= el2) ep += coul_t<T>(us.w * qsv, sqrt(r2), alpha);
+            }
           }
+          weq[qi] = (double)ep;  // a hot pair's slot is rewritten in FP64 below
         }
         if constexpr (F32) {
           if (nh > 0) {
