@@ -609,11 +609,12 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(ensure(c, c->d_sct, sizeof(int) * nsites));
   TRY(ensure(c, c->d_sg, sizeof(int) * nsites));
   TRY(ensure(c, c->d_sw, sizeof(T) * (size_t)nsites * 3 * k.p));
-  // z buckets of the site sort below: (g3, g2) when they fit the
-  // workgroup-local counts, else g3
+  // z buckets of the site sort below: (g3, LDS bank class of the gather)
+  // when they fit the workgroup-local counts, else g3
   const int K3 = k.K3;
   const bool zsort = (c->P.cltype != ERM_EL_COULOMB);
-  const int KB = (K3 * k.K2 <= ZBUCKET_MAX) ? k.K2 : 1, nbk = K3 * KB;
+  const int KB = (K3 * ZCLASS <= ZBUCKET_MAX) ? ZCLASS : 1, nbk = K3 * KB;
+  const int rowp = k.K1 + 2;
   if (zsort) TRY(ensure(c, c->d_zcount, sizeof(int) * (nbk + 1)));
   const int nclr = std::max(ncell + 1, zsort ? nbk + 1 : 0);
   hipLaunchKernelGGL(k_frame_clear, dim3((nclr + 255) / 256), dim3(256), 0, c->st, dp<int>(c->d_count), ncell + 1,
@@ -645,7 +646,8 @@ template <typename T> int frame_prep(erm_ctx *c) {
   c->nsites = nsites;
   c->jof_stale = true;
   if (!zsort) return ERM_OK;
-  // sites sorted by PME grid base (g3, g2) for the slab kernels
+  // sites sorted by PME plane g3, dealt over the gather's LDS bank classes
+  // within a plane, for the slab kernels
   TRY(ensure(c, c->d_zfine, sizeof(int) * (nbk + 1)));
   TRY(ensure(c, c->d_zfill, sizeof(int) * (nbk + 1)));
   TRY(ensure(c, c->d_zstart, sizeof(int) * (K3 + 1)));
@@ -655,14 +657,15 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(ensure(c, c->d_zq, sizeof(T) * nsites));
   TRY(ensure(c, c->d_zw, sizeof(T) * (size_t)nsites * 3 * k.p));
   const int gs = (nsites + tb - 1) / tb;
-  hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, dp<int>(c->d_sg),
+  hipLaunchKernelGGL(k_zcount, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, rowp, dp<int>(c->d_sg),
                      dp<int>(c->d_zcount));
   TRY(launch_check(c, "k_zcount"));
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->st, dp<int>(c->d_zcount), dp<int>(c->d_zfine), nbk,
                      dp<int>(c->d_zfill), dp<int>(c->d_zstart), K3, KB);
   TRY(launch_check(c, "k_scan"));
-  hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, k.p, dp<int>(c->d_sg),
-                     dp<T>(c->d_sq), dp<T>(c->d_sw), dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zpos), dp<int>(c->d_zg),
+  hipLaunchKernelGGL(k_zscatter<T>, dim3(gs), dim3(tb), 0, c->st, nsites, nbk, KB, rowp, k.p, dp<int>(c->d_sg),
+                     dp<T>(c->d_sq), dp<T>(c->d_sw), dp<int>(c->d_zcount), dp<int>(c->d_zfine),
+                     dp<int>(c->d_zfill), dp<int>(c->d_zsite), dp<int>(c->d_zpos), dp<int>(c->d_zg),
                      dp<T>(c->d_zq), dp<T>(c->d_zw));
   TRY(launch_check(c, "k_zscatter"));
   c->zst.assign(K3 + 1, 0);

@@ -1527,46 +1527,70 @@ static __global__ void k_mol_sum(Params P, int nsites, const double *__restrict_
 
 // per-frame sort of the solvent sites by grid base g3 (z-sorted copies of
 // base, charge and B-spline weights, the latter one record [3][p] per site,
-// [nsites][3][p]); within a
-// plane by row g2 as well when KB = K2 (buckets g3 KB + g2), which puts the
-// sites of a wave on few rows of a plane for the slab gather
+// [nsites][3][p]).  Within a plane (KB = ZCLASS) the sites are dealt over
+// the 32 LDS bank classes (rowp g2 + g1) mod 32 of their gather reads (rowp
+// = K1 + 2 values per real row; a plane is a multiple of 32 values on the
+// grids used, so the class does not depend on the plane or on the stencil
+// offsets): round r takes the r-th site of every class that has one, the
+// classes in the order 0, 2, ..., 30, 1, 3, ..., 31.  Any 32 consecutive
+// sites of full rounds then read 32 distinct banks, and any 16 read distinct
+// bank pairs, so a half-wave's gather reads do not conflict.  KB = 1: by
+// plane only.
 constexpr int ZBUCKET_MAX = 4096;
-__device__ __forceinline__ int zbucket(int g, int KB) {
-  return ((g >> 20) & 1023) * KB + (KB > 1 ? ((g >> 10) & 1023) : 0);
+constexpr int ZCLASS = 32;
+// bucket of a site: plane g3, then the dealing position of its class
+__device__ __forceinline__ int zbucket(int g, int KB, int rowp) {
+  const int g3 = (g >> 20) & 1023;
+  if (KB == 1) return g3;
+  const int c = (rowp * ((g >> 10) & 1023) + (g & 1023)) & (ZCLASS - 1);
+  return g3 * ZCLASS + ((c & 1) ? ZCLASS / 2 + (c >> 1) : (c >> 1));
 }
 
 // counts per bucket in LDS, one global add per bucket and workgroup
-static __global__ __launch_bounds__(256) void k_zcount(int nsites, int nbk, int KB, const int *__restrict__ sg,
-                                                int *zcount) {
+static __global__ __launch_bounds__(256) void k_zcount(int nsites, int nbk, int KB, int rowp,
+                                                       const int *__restrict__ sg, int *zcount) {
   __shared__ int cnt[ZBUCKET_MAX];
   for (int z = threadIdx.x; z < nbk; z += blockDim.x) cnt[z] = 0;
   __syncthreads();
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nsites) atomicAdd(&cnt[zbucket(sg[t], KB)], 1);
+  if (t < nsites) atomicAdd(&cnt[zbucket(sg[t], KB, rowp)], 1);
   __syncthreads();
   for (int z = threadIdx.x; z < nbk; z += blockDim.x)
     if (cnt[z]) atomicAdd(&zcount[z], cnt[z]);
 }
 
 
-// slot of every site within its bucket: a rank within the workgroup's sites
-// of that bucket (LDS) plus the workgroup's base (one global add per bucket
-// and workgroup)
+// slot of every site: its rank within its bucket (a rank within the
+// workgroup's sites of that bucket (LDS) plus the workgroup's base, one
+// global add per bucket and workgroup), then with class buckets its place in
+// the plane's dealing order: plane start + sum over the classes of
+// min(count, rank) + the classes before its own with more than rank sites
 template <typename T>
-__global__ __launch_bounds__(256) void k_zscatter(int nsites, int nbk, int KB, int p, const int *__restrict__ sg,
-                                                  const T *__restrict__ sq, const T *__restrict__ sw, int *zfill,
-                                                  int *zsite, int *zpos, int *zg, T *zq, T *zw) {
+__global__ __launch_bounds__(256) void k_zscatter(int nsites, int nbk, int KB, int rowp, int p,
+                                                  const int *__restrict__ sg, const T *__restrict__ sq,
+                                                  const T *__restrict__ sw, const int *__restrict__ zcount,
+                                                  const int *__restrict__ zfine, int *zfill, int *zsite, int *zpos,
+                                                  int *zg, T *zq, T *zw) {
   __shared__ int cnt[ZBUCKET_MAX], base[ZBUCKET_MAX];
   for (int z = threadIdx.x; z < nbk; z += blockDim.x) cnt[z] = 0;
   __syncthreads();
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = (t < nsites) ? zbucket(sg[t], KB) : 0;
+  const int g = (t < nsites) ? zbucket(sg[t], KB, rowp) : 0;
   const int rank = (t < nsites) ? atomicAdd(&cnt[g], 1) : 0;
   __syncthreads();
   for (int z = threadIdx.x; z < nbk; z += blockDim.x) base[z] = cnt[z] ? atomicAdd(&zfill[z], cnt[z]) : 0;
   __syncthreads();
   if (t >= nsites) return;
-  const int zt = base[g] + rank;
+  int zt = base[g] + rank;
+  if (KB > 1) {
+    const int rho = zt - zfine[g];  // rank within the bucket
+    const int g0 = g - g % KB;      // the plane's first bucket
+    zt = zfine[g0];
+    for (int q = 0; q < KB; ++q) {
+      const int n = zcount[g0 + q];
+      zt += min(n, rho) + ((g0 + q < g && n > rho) ? 1 : 0);
+    }
+  }
   zsite[zt] = t;
   zpos[t] = zt;
   zg[zt] = sg[t];
