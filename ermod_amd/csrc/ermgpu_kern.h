@@ -1021,12 +1021,17 @@ __device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, 
     Y[z].y = 0;
   }
   const int nsp = H + nm2;
+  // the bucket starts of the current chunk, one per lane: lane l holds
+  // lst[l] in lsa and lst[64 + l] in lsb (lst spans ZFFT_KMAX + 1 words)
+  const int lane = threadIdx.x & 63;
+  int lsa = 0, lsb = 0;
   // the sites of plane G (bucket [lst[G], lst[G + 1]) of the sorted chunk)
-  // add to the registers of planes G, G - 1, ..., G - PP + 1
+  // add to the registers of planes G, G - 1, ..., G - PP + 1; the bounds come
+  // from the lanes' copies (no LDS round trip per plane)
   auto bucket = [&](auto gc) {
     constexpr int G = decltype(gc)::value;
-    const int b0 = __builtin_amdgcn_readfirstlane(lst[G]);
-    const int b1 = __builtin_amdgcn_readfirstlane(lst[G + 1]);
+    const int b0 = __builtin_amdgcn_readlane(G < 64 ? lsa : lsb, G & 63);
+    const int b1 = __builtin_amdgcn_readlane(G + 1 < 64 ? lsa : lsb, (G + 1) & 63);
     for (int i = b0; i < b1; ++i) {
       const V a = cmul(l1[i * H + m1], l2[i * nm2 + m2]);
       T w[PP];
@@ -1071,6 +1076,8 @@ __device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, 
       lw[lr[sl] * PP + k] = sqv[(size_t)s0 + sl] * sw3[(size_t)s0 * PP + t];
     }
     __syncthreads();
+    lsa = lst[lane];
+    if constexpr (K3 >= 64) lsb = lst[64 + lane < ZFFT_KMAX + 1 ? 64 + lane : ZFFT_KMAX];
     [&]<int... Gs>(std::integer_sequence<int, Gs...>) {
       (bucket(std::integral_constant<int, Gs>{}), ...);
     }(std::make_integer_sequence<int, K3>{});
