@@ -948,8 +948,10 @@ __device__ __forceinline__ void zspread_half(V (&Y)[K3 / 2], const V a, const T 
   }
 }
 
-// k_pme_zcol_fft stages its sites in chunks of ZFFT_SCH, sorted by z plane
-constexpr int ZFFT_SCH = 64;
+// k_pme_zcol_fft stages its sites in chunks of zfft_sch<T>, sorted by z
+// plane (128 in FP32: the 150-site solutes take two chunks, and three
+// 256-thread workgroups still fit a CU's LDS)
+template <typename T> constexpr int zfft_sch = sizeof(T) == 4 ? 128 : 64;
 constexpr int ZFFT_KMAX = 96;  // largest z grid length of the register kernels
 
 // m2 rows a workgroup of XT threads (XT / 2 columns, m1 fastest) touches
@@ -958,12 +960,12 @@ __host__ __device__ inline int zcol_fft_rows(int H, int K2, int XT) {
   return r < K2 ? r : K2;
 }
 
-// dynamic LDS of k_pme_zcol_fft with XT threads: ZFFT_SCH sites x (H + the
+// dynamic LDS of k_pme_zcol_fft with XT threads: zfft_sch<T> sites x (H + the
 // block's m2 rows) spectra, charge-scaled z weights, planes and ranks, the
 // bucket starts, and the self-energy reduction
 template <typename T> __host__ __device__ inline size_t zcol_fft_lds(int H, int K2, int XT = NTHR) {
-  const size_t spec = (size_t)ZFFT_SCH * (H + zcol_fft_rows(H, K2, XT)) * 2 * sizeof(T);
-  const size_t small = (size_t)ZFFT_SCH * (sizeof(T) * MAXP + 2 * sizeof(int)) + (ZFFT_KMAX + 1) * sizeof(int);
+  const size_t spec = (size_t)zfft_sch<T> * (H + zcol_fft_rows(H, K2, XT)) * 2 * sizeof(T);
+  const size_t small = (size_t)zfft_sch<T> * (sizeof(T) * MAXP + 2 * sizeof(int)) + (ZFFT_KMAX + 1) * sizeof(int);
   return ((spec + small + 15) & ~(size_t)15) + XT * sizeof(double);
 }
 
@@ -997,12 +999,12 @@ __device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, 
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int H = P.H, K2 = P.K2;
   const int rows = zcol_fft_rows(H, K2, XT);
-  V *l1 = reinterpret_cast<V *>(lds_raw);           // [ZFFT_SCH][H], sorted order
-  V *l2 = l1 + ZFFT_SCH * H;                        // [ZFFT_SCH][nm2] the block's m2 rows
-  T *lw = reinterpret_cast<T *>(l2 + ZFFT_SCH * rows);  // [ZFFT_SCH][PP] z weights x charge
-  int *lg = reinterpret_cast<int *>(lw + ZFFT_SCH * MAXP);  // [ZFFT_SCH] planes, chunk order
-  int *lr = lg + ZFFT_SCH;                          // [ZFFT_SCH] rank of each site by plane
-  int *lst = lr + ZFFT_SCH;                         // [K3 + 1] first sorted site of each plane
+  V *l1 = reinterpret_cast<V *>(lds_raw);           // [zfft_sch<T>][H], sorted order
+  V *l2 = l1 + zfft_sch<T> * H;                        // [zfft_sch<T>][nm2] the block's m2 rows
+  T *lw = reinterpret_cast<T *>(l2 + zfft_sch<T> * rows);  // [zfft_sch<T>][PP] z weights x charge
+  int *lg = reinterpret_cast<int *>(lw + zfft_sch<T> * MAXP);  // [zfft_sch<T>] planes, chunk order
+  int *lr = lg + zfft_sch<T>;                          // [zfft_sch<T>] rank of each site by plane
+  int *lst = lr + zfft_sch<T>;                         // [K3 + 1] first sorted site of each plane
   double *red = reinterpret_cast<double *>(lds_raw + zcol_fft_lds<T>(H, K2, XT) - XT * sizeof(double));
   const int ncol = H * K2;
   const int h = threadIdx.x & 1;
@@ -1040,8 +1042,8 @@ __device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, 
       zspread_half<K3, G, T>(Y, a, w, hm, std::make_integer_sequence<int, PP>{});
     }
   };
-  for (int s0 = 0; s0 < ns; s0 += ZFFT_SCH) {
-    const int nc = min(ZFFT_SCH, ns - s0);
+  for (int s0 = 0; s0 < ns; s0 += zfft_sch<T>) {
+    const int nc = min(zfft_sch<T>, ns - s0);
     __syncthreads();  // the previous chunk is consumed
     for (int t = threadIdx.x; t < nc; t += blockDim.x) lg[t] = sg3[(size_t)s0 + t];
     __syncthreads();
