@@ -2359,10 +2359,10 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
             ay = v.y - us.y;
             az = v.z - us.z;
           }
-          ax = fabs(ax);
-          ay = fabs(ay);
-          az = fabs(az);
-          if (img) {
+          if (img) {  // the distance only squares the components otherwise
+            ax = fabs(ax);
+            ay = fabs(ay);
+            az = fabs(az);
             ax = (ax > h0) ? L0 - ax : ax;
             ay = (ay > h1) ? L1 - ay : ay;
             az = (az > h2) ? L2 - az : az;
@@ -2374,11 +2374,11 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
             cvi = __float_as_int(vl.w);
           else
             cvi = (int)__double_as_longlong(vl.w);
-          const T2 lj = ulj[s * nvt + cvi];
+          const T2 lj = ulj[__umul24(s, nvt) + cvi];  // s < 4096, nvt < 12800
           const T eps = lj.x, sig2 = lj.y;
           const bool inr = valid && r2 <= cut2;
           bool hot = false;
-          if constexpr (F32) hot = inr && hot_pair<T>(eps, sig2, us.w * qsv, r2);
+          if constexpr (F32) hot = inr & hot_pair<T>(eps, sig2, us.w * qsv, r2);  // both evaluated
           const unsigned long long mh = __ballot(hot);
           if (hot) whq[nh + lanes_below(mh)] = qi;
           nh += (int)__popcll(mh);
