@@ -743,6 +743,14 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
   return ERM_OK;
 }
 
+// workgroups per configuration of the tiled pair kernel: nchunk chunks of 64
+// candidate molecules over nwave waves, PAIR_CHUNKS_PER_WAVE chunks each
+constexpr int PAIR_CHUNKS_PER_WAVE = 2;
+inline int pair_splits(int nchunk, int nwave) {
+  const int per = nwave * PAIR_CHUNKS_PER_WAVE;
+  return std::max(1, std::min(64, (nchunk + per - 1) / per));
+}
+
 // real-space pairs of a batch: the tiled kernel with compacted pair lists,
 // or the kernel with one thread per candidate molecule (pair_path 1, or
 // when the tiles' LDS does not fit)
@@ -754,9 +762,10 @@ template <typename T> int launch_pair(erm_ctx *c, const Params &k, int n, bool b
   const int nwave = NTHR / 64;
   const size_t lt = pair_tile_lds<T>(ns, c->nvt, segcap, nwave);
   if (c->pair_path == 0 && lt <= c->lds_cap) {
-    // one wave per chunk of 64 candidate molecules
+    // chunks of 64 candidate molecules, PAIR_CHUNKS_PER_WAVE per wave (the
+    // per-configuration set-up of a workgroup is shared by its chunks)
     const int nchunk = (int)std::ceil(frac * S / 64.0);
-    const int nsplit = std::max(1, std::min(64, (nchunk + nwave - 1) / nwave));
+    const int nsplit = pair_splits(nchunk, nwave);
     if (bare) {
       TRY(set_lds(c, (const void *)k_pair_tile<T, true>, lt, "k_pair_tile"));
       hipLaunchKernelGGL((k_pair_tile<T, true>), dim3(n, nsplit), dim3(NTHR), lt, c->st, k, dp<double>(c->d_spos),
@@ -2520,7 +2529,7 @@ int32_t erm_group_insert_random(erm_group *g, const int32_t *n, const uint64_t *
       const double Rg = k.rcmax + m->slt_radius + c->mol_radius + 0.5 * (k.hcell[0] + k.hcell[1] + k.hcell[2]) / 3.0;
       const double frac = std::min(1.0, 4.18879 * Rg * Rg * Rg / c->volume);
       const int nchunk = (int)std::ceil(frac * c->slvmax / 64.0);
-      const int nsplit = std::max(1, std::min(64, (nchunk + NTHR / 64 - 1) / (NTHR / 64)));
+      const int nsplit = pair_splits(nchunk, NTHR / 64);
       GRun &r = R[nrun];
       r.cfg0 = cfg;
       r.ncfg = cnt;
