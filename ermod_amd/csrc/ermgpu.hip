@@ -745,7 +745,7 @@ int stage_configs(erm_ctx *c, const Batch &bt, const Params &k) {
 
 // workgroups per configuration of the tiled pair kernel: nchunk chunks of 64
 // candidate molecules over nwave waves, PAIR_CHUNKS_PER_WAVE chunks each
-constexpr int PAIR_CHUNKS_PER_WAVE = 2;
+constexpr int PAIR_CHUNKS_PER_WAVE = 4;
 inline int pair_splits(int nchunk, int nwave) {
   const int per = nwave * PAIR_CHUNKS_PER_WAVE;
   return std::max(1, std::min(64, (nchunk + per - 1) / per));
