@@ -277,11 +277,14 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
                         for pers in (0, 1, 2, 3):
                             group.set_tuning(max_batch=bmax, persistent=pers)
                             frame(0)
-                            torch.cuda.synchronize()
-                            t0 = time.perf_counter()
-                            frame(1)
-                            torch.cuda.synchronize()
-                            t = time.perf_counter() - t0
+                            t = None
+                            for f in (1, 2):  # the faster of two frames
+                                torch.cuda.synchronize()
+                                t0 = time.perf_counter()
+                                frame(f)
+                                torch.cuda.synchronize()
+                                tf = time.perf_counter() - t0
+                                t = tf if t is None else min(t, tf)
                             if best is None or t < best[0]:
                                 best = (t, bmax, pers, si)
                 engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shapes[best[3]])
@@ -553,8 +556,10 @@ def main():
                             "note": "the same workload replayed from the group's captured HIP graph"}
                            if r.get("graph_dt") else None),
            "engsmpl": [a["engsmpl"] for a in r["acc"]]}
-    if not args.no_error:
-        out["fp32_error"] = mixed_fp32_error(syss, dpos, args.insertions)
+    # refs20 before the FP32-error pass: at the end of the full sequence
+    # (peak-rate buffers, graph replays, then that pass, which allocates and
+    # frees some 11 GB of FP64 spectra) the FP64 eager refs20 batches ran at
+    # a third of their speed in isolation; the graph replays were unaffected
     if not args.no_refs20:
         st = max(2, args.steps // 2)
         out["refs20"] = {"workload": "refs20: 4000 waters in 49.3 A, rigid 20-site test particle, 1000 random "
@@ -563,6 +568,8 @@ def main():
                          "fp64_graph": refs20_pass(torch, dev, "fp64", st, 2, 1000, graph=True,
                                                    tune=not args.no_tune),
                          "fp32": refs20_pass(torch, dev, "fp32", st, 2, 1000, tune=not args.no_tune)}
+    if not args.no_error:
+        out["fp32_error"] = mixed_fp32_error(syss, dpos, args.insertions)
     if not args.no_cpu:
         cb = cpu_baseline_mixed(syss, host[0], args.cpu_seconds)
         cb["gpu_over_cpu"] = value / cb["value"]

@@ -601,6 +601,7 @@ template <typename T> int frame_prep(erm_ctx *c) {
   TRY(ensure(c, c->d_mol_q, sizeof(double) * S));
   TRY(ensure(c, c->d_mol_spec, sizeof(int) * S));
   TRY(ensure(c, c->d_mol_m, sizeof(int) * S));
+  TRY(ensure(c, c->d_mol_d1, sizeof(double) * S));
   TRY(ensure(c, c->d_jof, sizeof(int) * S));
   TRY(ensure(c, c->d_sx, sizeof(double) * nsites));
   TRY(ensure(c, c->d_sy, sizeof(double) * nsites));
@@ -641,7 +642,7 @@ template <typename T> int frame_prep(erm_ctx *c) {
                      dp<double>(c->d_mol_charge), dp<int>(c->d_uvspec), dp<int>(c->d_mol_k),
                      dp<int>(c->d_mol_first), dp<double>(c->d_sx), dp<double>(c->d_sy), dp<double>(c->d_sz),
                      dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sg), dp<T>(c->d_sw), dp<double>(c->d_mol_q),
-                     dp<int>(c->d_mol_spec), dp<int>(c->d_mol_m));
+                     dp<int>(c->d_mol_spec), dp<int>(c->d_mol_m), dp<double>(c->d_mol_d1));
   TRY(launch_check(c, "k_site_fill"));
   c->nsites = nsites;
   c->jof_stale = true;
@@ -772,14 +773,16 @@ template <typename T> int launch_pair(erm_ctx *c, const Params &k, int n, bool b
                          dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
                          dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
                          dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
-                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt),
+                         dp<double>(c->d_mol_d1), dp<double>(c->d_uv));
     } else {
       TRY(set_lds(c, (const void *)k_pair_tile<T, false>, lt, "k_pair_tile"));
       hipLaunchKernelGGL((k_pair_tile<T, false>), dim3(n, nsplit), dim3(NTHR), lt, c->st, k, dp<double>(c->d_spos),
                          dp<double>(c->d_sref), dp<T>(c->d_qslt_t), dp<T>(c->d_sltlj), dp<int>(c->d_cell_start),
                          dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy),
                          dp<double>(c->d_sz), dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj),
-                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt), dp<double>(c->d_uv));
+                         dp<double>(c->d_rm), dp<double>(c->d_sltlj64), dp<double>(c->d_qslt),
+                         dp<double>(c->d_mol_d1), dp<double>(c->d_uv));
     }
     return launch_check(c, "k_pair_tile");
   }
@@ -1607,7 +1610,7 @@ template <typename T> int group_batch(erm_group *g, const GRun *R, int nrun, con
                          dp<double>(c->d_sref), dp<int>(c->d_cell_start), dp<int>(c->d_mol_first),
                          dp<int>(c->d_mol_ns), dp<double>(c->d_sx), dp<double>(c->d_sy), dp<double>(c->d_sz),
                          dp<T>(c->d_sq), dp<int>(c->d_sct), dp<int>(c->d_sltj), dp<double>(c->d_rm), d_runs, nrun,
-                         nitem_p, dp<int>(g->d_counter) + 1, dp<double>(c->d_uv));
+                         nitem_p, dp<int>(g->d_counter) + 1, dp<double>(c->d_mol_d1), dp<double>(c->d_uv));
       TRY(launch_check(c, "k_pair_tile_runs"));
     } else {
       TRY(set_lds(c, (const void *)k_pair_tile<T, false>, lt, "k_pair_tile"));
@@ -1621,7 +1624,7 @@ template <typename T> int group_batch(erm_group *g, const GRun *R, int nrun, con
                            dp<int>(c->d_cell_start), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
                            dp<double>(c->d_sx), dp<double>(c->d_sy), dp<double>(c->d_sz), dp<T>(c->d_sq),
                            dp<int>(c->d_sct), dp<int>(c->d_sltj) + c0, dp<double>(c->d_rm), R[r].lj64, R[r].q64,
-                           dp<double>(c->d_uv) + c0 * (S + 1));
+                           dp<double>(c->d_mol_d1), dp<double>(c->d_uv) + c0 * (S + 1));
         TRY(launch_check(c, "k_pair_tile"));
       }
     }

@@ -74,7 +74,7 @@ struct erm_ctx {
   DBuf d_mol_begin, d_charge, d_tagpt, d_cvt, d_mol_charge, d_uvspec, d_sitemass, d_bf, d_qslt, d_qslt_t, d_sltlj,
       d_sltlj64;
   DBuf d_pos, d_cellid, d_count, d_cell_start, d_fill, d_rm, d_mol_k, d_mol_ns, d_mol_first, d_mol_q,
-      d_mol_spec, d_mol_m, d_jof, d_kof;
+      d_mol_spec, d_mol_m, d_mol_d1, d_jof, d_kof;
   DBuf d_sx, d_sy, d_sz, d_sq, d_sct, d_sg, d_sw;
   DBuf d_zcount, d_zstart, d_zfill, d_zfine, d_zsite, d_zpos, d_zg, d_zq, d_zw, d_esite;
   int nsites = 0;             // sites of the physical molecules
@@ -129,7 +129,7 @@ struct erm_ctx {
   std::vector<DBuf *> all() {
     return {&d_mol_begin, &d_charge, &d_tagpt, &d_cvt, &d_mol_charge, &d_uvspec, &d_sitemass, &d_bf, &d_qslt, &d_qslt_t,
             &d_sltlj, &d_sltlj64, &d_pos, &d_cellid, &d_count, &d_cell_start, &d_fill, &d_rm, &d_mol_k, &d_mol_ns,
-            &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_jof, &d_kof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
+            &d_mol_first, &d_mol_q, &d_mol_spec, &d_mol_m, &d_mol_d1, &d_jof, &d_kof, &d_sx, &d_sy, &d_sz, &d_sq, &d_sct, &d_sg,
             &d_sw, &d_zcount, &d_zstart, &d_zfill, &d_zfine, &d_zsite, &d_zpos, &d_zg, &d_zq, &d_zw, &d_esite, &d_tw1, &d_tw2, &d_tw3, &d_engfac, &d_edges, &d_specoff, &d_uvmax, &d_uvsoft, &d_escrd,
             &d_w3h, &d_args, &d_ggrid, &d_gHk, &d_gtab, &d_gq_t, &d_gqd, &d_gepsd, &d_gpart, &d_gphi, &d_gub, &d_gw,
             &d_gout, &d_spos, &d_sref, &d_sg3, &d_sw3, &d_sqv, &d_w1h, &d_w2h, &d_Hk, &d_phi, &d_selfp, &d_uv,

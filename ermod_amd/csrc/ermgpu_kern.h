@@ -159,7 +159,7 @@ __global__ void k_site_fill(Params P, const double *__restrict__ pos, const int 
                             const int *__restrict__ uvspec, const int *__restrict__ mol_k,
                             const int *__restrict__ mol_first, double *sx, double *sy, double *sz,
                             T *sq, int *sct, int *sg, T *sw, double *mol_q, int *mol_spec,
-                            int *mol_m) {
+                            int *mol_m, double *mol_d1) {
 #pragma clang fp contract(off)
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= P.slvmax) return;
@@ -167,6 +167,23 @@ __global__ void k_site_fill(Params P, const double *__restrict__ pos, const int 
   mol_q[j] = mol_charge[m];
   mol_spec[j] = uvspec[m];
   mol_m[j] = m;
+  {
+    // the molecule's reach from its first site (minimum images in a
+    // periodic box, the pair kernels' convention): no site of the molecule
+    // lies farther than mol_d1 from its first
+    const double *x0 = pos + 3 * mol_begin[m];
+    double d1 = 0.0;
+    for (int a = mol_begin[m] + 1; a < mol_begin[m + 1]; ++a) {
+      double dd = 0.0;
+      for (int e = 0; e < 3; ++e) {
+        double d = pos[3 * a + e] - x0[e];
+        if (P.boxshp) d = d - P.L[e] * rint(d / P.L[e]);
+        dd = dd + d * d;
+      }
+      d1 = fmax(d1, sqrt(dd));
+    }
+    mol_d1[j] = d1;
+  }
   int t = mol_first[j];
   const int Ks[3] = {P.K1, P.K2, P.K3};
   for (int a = mol_begin[m]; a < mol_begin[m + 1]; ++a, ++t) {
@@ -2094,7 +2111,8 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
                                                const int *__restrict__ slt_j,
                                                const double *__restrict__ rm_ord_p,
                                                const double *__restrict__ slt_lj64,
-                                               const double *__restrict__ slt_q64, double *uv) {
+                                               const double *__restrict__ slt_q64,
+                                               const double *__restrict__ mol_d1, double *uv) {
   constexpr bool F32 = sizeof(T) == 4;
   using T4 = typename V4T<T>::type;
   using T2 = typename C2<T>::type;
@@ -2212,8 +2230,16 @@ __device__ __forceinline__ void pair_tile_body(const Params &P, int ns, int b, i
       }
       j = seg_beg[lo_s] + (idx - seg_pre[lo_s]);
       if (j != sj) {
-        nsm = mol_ns[j];
         first = mol_first[j];
+        // a molecule whose first site lies farther than reach + mol_d1 from
+        // the centre has no site within reach (triangle inequality): it
+        // takes no tile lanes
+        double dx = sx[first] - C0, dy = sy[first] - C1, dz = sz[first] - C2v;
+        dx -= P.L[0] * rint(dx * iL0);
+        dy -= P.L[1] * rint(dy * iL1);
+        dz -= P.L[2] * rint(dz * iL2);
+        const double kr = reach + mol_d1[j] + 1e-6 * (1.0 + reach);
+        if (img || dx * dx + dy * dy + dz * dz <= kr * kr) nsm = mol_ns[j];
       }
     }
     // exclusive scan of the site counts over the wave
@@ -2519,10 +2545,12 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile(Params P, const double *__re
                                                     const int *__restrict__ slt_j,
                                                     const double *__restrict__ rm_ord_p,
                                                     const double *__restrict__ slt_lj64,
-                                                    const double *__restrict__ slt_q64, double *uv) {
+                                                    const double *__restrict__ slt_q64,
+                                                    const double *__restrict__ mol_d1, double *uv) {
   const int b = blockIdx.x;
   pair_tile_body<T, BARE>(P, P.ns, b, blockIdx.y, gridDim.y, spos + (size_t)b * P.ns * 3, sref, slt_q, slt_lj,
-                          cell_start, mol_first, mol_ns, sx, sy, sz, sq, sct, slt_j, rm_ord_p, slt_lj64, slt_q64, uv);
+                          cell_start, mol_first, mol_ns, sx, sy, sz, sq, sct, slt_j, rm_ord_p, slt_lj64, slt_q64,
+                          mol_d1, uv);
 }
 
 // real space of a heterogeneous batch as one persistent launch:
@@ -2540,7 +2568,8 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile_runs(Params P, const double 
                                                          const int *__restrict__ sct, const int *__restrict__ slt_j,
                                                          const double *__restrict__ rm_ord_p,
                                                          const GRun *__restrict__ runs, int nrun, int nitem,
-                                                         int *counter, double *uv) {
+                                                         int *counter, const double *__restrict__ mol_d1,
+                                                         double *uv) {
   for (int base = next_items(counter); base < nitem; base = next_items(counter))
   for (int it = base; it < min(nitem, base + ITEM_CHUNK); ++it) {
     __syncthreads();  // the previous item's LDS is free
@@ -2551,7 +2580,7 @@ __global__ __launch_bounds__(NTHR) void k_pair_tile_runs(Params P, const double 
     const double *xs = spos + ((size_t)runs[r].site0 + (size_t)bl * ns) * 3;
     pair_tile_body<T, BARE>(P, ns, b, split, nsplit, xs, sref, static_cast<const T *>(runs[r].q_t),
                             static_cast<const T *>(runs[r].lj_t), cell_start, mol_first, mol_ns, sx, sy, sz, sq, sct,
-                            slt_j, rm_ord_p, runs[r].lj64, runs[r].q64, uv);
+                            slt_j, rm_ord_p, runs[r].lj64, runs[r].q64, mol_d1, uv);
   }
 }
 
