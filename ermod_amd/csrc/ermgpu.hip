@@ -536,7 +536,7 @@ template <typename T> int choose_slab(erm_ctx *c) {
   c->slab_S = 0;
   c->slab_R = 0;
   // R = 8 only for the register path with shallow slabs (fewer live registers)
-  const bool reg8 = c->tune_S <= 8 && (c->P.pme_path == 0 || c->P.pme_path == 3);
+  const bool reg8 = c->tune_S <= 4 && (c->P.pme_path == 0 || c->P.pme_path == 3);
   for (int R : {8, 16, 32}) {
     if (c->P.pme_path == 2) break;
     if (R == 8 && !reg8) continue;
