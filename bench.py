@@ -271,26 +271,47 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
                 # 16 sites per thread on this workload, 8 gives 4-plane slabs)
                 shapes = [own] + [dict(slab_planes=sp, group_planes=gp, slab_threads=512, fold_y=fy)
                                   for sp, gp, fy in ((16, 8, True), (16, 8, False), (8, 8, True), (6, 9, True))]
+
+                def frames_min(fs, measure):
+                    # the smallest of a few frames' measures
+                    t = None
+                    for f in fs:
+                        torch.cuda.synchronize()
+                        t0 = time.perf_counter()
+                        frame(f)
+                        torch.cuda.synchronize()
+                        tf = measure(time.perf_counter() - t0)
+                        t = tf if t is None else min(t, tf)
+                    return t
+
+                # batch size and persistence by wall time per frame, in the
+                # first member's own shape
                 best = None
+                engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **own)
+                for bmax in ((total + 1) // 2, total):
+                    for pers in (0, 1, 2, 3):
+                        group.set_tuning(max_batch=bmax, persistent=pers)
+                        frame(0)
+                        t = frames_min((1, 2), lambda w: w)
+                        if best is None or t < best[0]:
+                            best = (t, bmax, pers)
+                group.set_tuning(max_batch=best[1], persistent=best[2])
+                # the slab shape by the device time of the stages it changes
+                # (y and x transforms and the gather, HIP events on the
+                # stream): frame wall times vary by more than the shapes
+                # differ
+                xy = lambda w: (lambda st: st["pme_y"] + st["pme_x"])(group.stage_times())
+                sbest = None
                 for si, shape in enumerate(shapes):
                     engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shape)
-                    for bmax in ((total + 1) // 2, total):
-                        for pers in (0, 1, 2, 3):
-                            group.set_tuning(max_batch=bmax, persistent=pers)
-                            frame(0)
-                            t = None
-                            for f in (1, 2):  # the faster of two frames
-                                torch.cuda.synchronize()
-                                t0 = time.perf_counter()
-                                frame(f)
-                                torch.cuda.synchronize()
-                                tf = time.perf_counter() - t0
-                                t = tf if t is None else min(t, tf)
-                            if best is None or t < best[0]:
-                                best = (t, bmax, pers, si)
-                engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shapes[best[3]])
+                    frame(0)
+                    t = frames_min((1, 2, 3), xy)
+                    if sbest is None or t < sbest[0]:
+                        sbest = (t, si)
+                engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shapes[sbest[1]])
                 group_tuning = {"max_batch": best[1], "persistent": best[2],
-                                "slab_shape": "first member's own" if best[3] == 0 else shapes[best[3]]}
+                                "slab_shape": "first member's own" if sbest[1] == 0 else shapes[sbest[1]],
+                                "slab_shape_xy_ms": round(sbest[0], 3)}
             else:
                 group_tuning = {"max_batch": total, "persistent": 2}
             group.set_tuning(max_batch=group_tuning["max_batch"], persistent=group_tuning["persistent"])

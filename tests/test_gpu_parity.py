@@ -316,8 +316,8 @@ def test_register_path_tunings(refs, prec):
         for st in (256, 512):
             for ct in (256, 512):
                 for fy in (False, True):
-                    for sp_ in (3, 6, 8, 16):
-                        for gp in (4, 9, 16):
+                    for sp_ in (3, 5, 6, 8, 16):
+                        for gp in (4, 8, 9, 16):
                             g.set_tuning(max_batch=0, pme_path="register", slab_planes=sp_, group_planes=gp,
                                          slab_threads=st, column_threads=ct, fold_y=fy)
                             t = g.tuning()
