@@ -524,6 +524,17 @@ int zcol_groups(const erm_ctx *c, bool regp, int vb) {
     default: break;                           \
   }
 
+// per-molecule reciprocal sums of n configurations (k_mol_sum: a thread per
+// molecule and group of MOLSUM_CFG configurations)
+int launch_mol_sum(erm_ctx *c, const Params &kg, int n, const double *esite, const int *sltj, double *uv) {
+  if (n <= 0) return ERM_OK;
+  const int S = c->slvmax;
+  hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, (n + MOLSUM_CFG - 1) / MOLSUM_CFG), dim3(NTHR), 0,
+                     c->st, kg, c->nsites, n, esite, dp<int>(c->d_zpos), dp<int>(c->d_mol_first),
+                     dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), sltj, uv);
+  return launch_check(c, "k_mol_sum");
+}
+
 // Slab shape of the fused PME kernels from the frame's z-sorted site counts:
 // the deepest slab (at most tune_S planes) whose busiest slab fits the
 // per-thread site registers (R = 16 or 32), with at most 3x halo planes;
@@ -869,10 +880,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
     TRY(mark(c, 4));
     Params kg = k;
     if (!(flags & DO_RESIDUAL)) kg.solute_charge = 0.0;
-    hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
-                       dp<double>(c->d_esite), dp<int>(c->d_zpos), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
-                       dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
-    TRY(launch_check(c, "k_mol_sum"));
+    TRY(launch_mol_sum(c, kg, n, dp<double>(c->d_esite), dp<int>(c->d_sltj), dp<double>(c->d_uv)));
   } else if (flags & DO_RECIP) {
     size_t lz = zcol_lds(ncolwg, K3, vb);
     hipLaunchKernelGGL(k_pme_zcol<T>, dim3(nzg, n), dim3(NTHR), lz, c->st, k, ncolwg, dp<T>(c->d_engfac),
@@ -899,10 +907,7 @@ template <typename T> int run_energies(erm_ctx *c, const Batch &bt, int flags) {
                            dp<double>(c->d_esite));
       TRY(launch_check(c, "k_pme_slab"));
       TRY(mark(c, 4));
-      hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, n), dim3(NTHR), 0, c->st, kg, c->nsites,
-                         dp<double>(c->d_esite), dp<int>(c->d_zpos), dp<int>(c->d_mol_first), dp<int>(c->d_mol_ns),
-                         dp<double>(c->d_mol_q), dp<int>(c->d_sltj), dp<double>(c->d_uv));
-      TRY(launch_check(c, "k_mol_sum"));
+      TRY(launch_mol_sum(c, kg, n, dp<double>(c->d_esite), dp<int>(c->d_sltj), dp<double>(c->d_uv)));
     } else {
       TRY(ensure(c, c->d_phi, sizeof(T) * (size_t)n * K3 * K2 * K1));
       hipLaunchKernelGGL(k_pme_plane<T>, dim3(K3, n), dim3(NTHR), lp, c->st, k, dp<V>(c->d_Hk), dp<V>(c->d_tw1),
@@ -1584,11 +1589,8 @@ template <typename T> int group_batch(erm_group *g, const GRun *R, int nrun, con
     Params kg = k;
     kg.solute_charge = m->mol_charge[m->slt_template];
     const size_t c0 = R[r].cfg0;
-    hipLaunchKernelGGL(k_mol_sum, dim3((S + NTHR - 1) / NTHR, R[r].ncfg), dim3(NTHR), 0, c->st, kg, c->nsites,
-                       dp<double>(c->d_esite) + c0 * c->nsites, dp<int>(c->d_zpos), dp<int>(c->d_mol_first),
-                       dp<int>(c->d_mol_ns), dp<double>(c->d_mol_q), dp<int>(c->d_sltj) + c0,
-                       dp<double>(c->d_uv) + c0 * (S + 1));
-    TRY(launch_check(c, "k_mol_sum"));
+    TRY(launch_mol_sum(c, kg, R[r].ncfg, dp<double>(c->d_esite) + c0 * c->nsites, dp<int>(c->d_sltj) + c0,
+                       dp<double>(c->d_uv) + c0 * (S + 1)));
   }
   TRY(mark(c, 5));
   {
@@ -2968,8 +2970,8 @@ int32_t erm_autotune(erm_ctx *c, int32_t n_trial, uint64_t seed, double *best_ms
   for (int B : {128, 256, 512, 1024}) {
     if (B > 4 * n_trial && B > 128) break;
     for (int pp : paths) {
-      for (int Sc : {16, 8, 5, 4, 3}) {
-        for (int Qc : {16, 8, 5, 4, 3}) {
+      for (int Sc : {16, 8, 6, 5, 4, 3}) {
+        for (int Qc : {16, 9, 8, 5, 4, 3}) {
           for (int Tc : {256, 512}) {
             if (pp != 3 && (Qc != 8 || Tc != 256)) continue;
             if (pp == 2 && Sc != 16) continue;

@@ -512,9 +512,13 @@ __global__ __launch_bounds__(128) void k_solute_prep(Params P, const double *__r
       V acc;
       acc.x = 0;
       acc.y = 0;
+      // twiddle index m (g - k) mod K, stepped by -m from m g mod K (m, g
+      // < K <= 512, so m g fits an int)
+      int ti = (m * g) % K;
       for (int k = 0; k < p; ++k) {
-        int gi = ((g - k) % K + K) % K;
-        V w = tw[(int)(((long long)m * gi) % K)];
+        V w = tw[ti];
+        ti -= m;
+        ti += (ti < 0) ? K : 0;
         T wt = (T)wsh[(sl * 3 + d) * MAXP + k];
         acc.x += wt * w.x;
         acc.y += wt * w.y;
@@ -1525,30 +1529,54 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
 // z-sorted site order; zpos maps a cell-sorted site to it), plus the
 // neutralising background (residual_ene, engproc.F90:859-869):
 // uv[b][1 + j] = -pi Q_s Q_j / (alpha^2 V) + sum over the sites of j, in
-// site order
-static __global__ void k_mol_sum(Params P, int nsites, const double *__restrict__ e_site,
+// site order.  A thread takes molecule j for MOLSUM_CFG consecutive
+// configurations and its sites four at a time: the four zpos reads, then
+// the per-site sums of every configuration, all in flight together (one
+// molecule per thread and configuration was a chain of dependent loads).
+constexpr int MOLSUM_CFG = 4;
+static __global__ void k_mol_sum(Params P, int nsites, int ncfg, const double *__restrict__ e_site,
                                  const int *__restrict__ zpos, const int *__restrict__ mol_first,
                                  const int *__restrict__ mol_ns, const double *__restrict__ mol_q,
                                  const int *__restrict__ slt_j, double *uv) {
   // XCD-aware order (workgroups go to the 8 XCDs round-robin by linear
-  // index): the workgroups of a configuration run on one XCD, whose L2 then
-  // holds the configuration's per-site sums for all of them
+  // index): the workgroups of a configuration group run on one XCD, whose
+  // L2 then holds the group's per-site sums for all of them
   const int nwg = gridDim.x * gridDim.y;
   int L = blockIdx.y * gridDim.x + blockIdx.x;
   if (nwg % 8 == 0) L = (L % 8) * (nwg / 8) + L / 8;
-  const int b = L / gridDim.x;
-  const int j = (L - b * gridDim.x) * blockDim.x + threadIdx.x;
+  const int bg = L / gridDim.x;
+  const int j = (L - bg * gridDim.x) * blockDim.x + threadIdx.x;
   if (j >= P.slvmax) return;
-  double *u = uv + (size_t)b * (P.slvmax + 1) + 1;
-  if (j == slt_j[b]) {
-    u[j] = 0.0;
-    return;
+  const int b0 = bg * MOLSUM_CFG;
+  const int nb = min(MOLSUM_CFG, ncfg - b0);
+  const int t0 = mol_first[j], t1 = t0 + mol_ns[j];
+  const double *es = e_site + (size_t)b0 * nsites;
+  double e[MOLSUM_CFG];
+#pragma unroll
+  for (int i = 0; i < MOLSUM_CFG; ++i) e[i] = 0.0;
+  for (int t = t0; t < t1; t += 4) {
+    int pz[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pz[k] = zpos[min(t + k, t1 - 1)];  // a molecule has a site
+    double v[MOLSUM_CFG][4];
+#pragma unroll
+    for (int i = 0; i < MOLSUM_CFG; ++i) {
+      const double *ei = es + (size_t)min(i, nb - 1) * nsites;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[i][k] = ei[pz[k]];
+    }
+#pragma unroll
+    for (int i = 0; i < MOLSUM_CFG; ++i)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) e[i] = (t + k < t1) ? e[i] + v[i][k] : e[i];
   }
-  const double *es = e_site + (size_t)b * nsites;
-  double e = 0.0;
-  for (int t = mol_first[j]; t < mol_first[j] + mol_ns[j]; ++t) e += es[zpos[t]];
-  double epcl = 3.1415926535897932 * P.solute_charge * mol_q[j] / P.screen / P.screen / P.volume;
-  u[j] = (0.0 - epcl) + e;
+  const double epcl = 3.1415926535897932 * P.solute_charge * mol_q[j] / P.screen / P.screen / P.volume;
+#pragma unroll
+  for (int i = 0; i < MOLSUM_CFG; ++i) {
+    if (i >= nb) break;
+    double *u = uv + (size_t)(b0 + i) * (P.slvmax + 1) + 1;
+    u[j] = (j == slt_j[b0 + i]) ? 0.0 : (0.0 - epcl) + e[i];
+  }
 }
 
 // per-frame sort of the solvent sites by grid base g3 (z-sorted copies of
