@@ -272,6 +272,9 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
                 shapes = [own] + [dict(slab_planes=sp, group_planes=gp, slab_threads=512, fold_y=fy)
                                   for sp, gp, fy in ((16, 8, True), (16, 8, False), (8, 8, True), (6, 9, True))]
 
+                def set_shape(shape):
+                    engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shape)
+
                 def frames_min(fs, measure):
                     # the smallest of a few frames' measures
                     t = None
@@ -287,7 +290,7 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
                 # batch size and persistence by wall time per frame, in the
                 # first member's own shape
                 best = None
-                engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **own)
+                set_shape(own)
                 for bmax in ((total + 1) // 2, total):
                     for pers in (0, 1, 2, 3):
                         group.set_tuning(max_batch=bmax, persistent=pers)
@@ -302,16 +305,19 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
                 # differ
                 xy = lambda w: (lambda st: st["pme_y"] + st["pme_x"])(group.stage_times())
                 sbest = None
+                shape_ms = []
                 for si, shape in enumerate(shapes):
-                    engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shape)
+                    set_shape(shape)
                     frame(0)
                     t = frames_min((1, 2, 3), xy)
+                    shape_ms.append(round(t, 3))
                     if sbest is None or t < sbest[0]:
                         sbest = (t, si)
-                engines[0].set_tuning(max_batch=0, pme_path=t0m["pme_path"], use_graph=False, **shapes[sbest[1]])
+                set_shape(shapes[sbest[1]])
                 group_tuning = {"max_batch": best[1], "persistent": best[2],
                                 "slab_shape": "first member's own" if sbest[1] == 0 else shapes[sbest[1]],
-                                "slab_shape_xy_ms": round(sbest[0], 3)}
+                                "slab_shape_xy_ms": round(sbest[0], 3),
+                                "shapes_xy_ms": [dict(shape, xy_ms=t) for shape, t in zip(shapes, shape_ms)]}
             else:
                 group_tuning = {"max_batch": total, "persistent": 2}
             group.set_tuning(max_batch=group_tuning["max_batch"], persistent=group_tuning["persistent"])

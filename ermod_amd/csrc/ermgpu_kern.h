@@ -1426,6 +1426,68 @@ __device__ __forceinline__ void xslab_gather4(const Params &P, const T *real, in
   }
 }
 
+// Planes [zz0, zz0 + nq) (periodic in K3) of configuration b's spectrum
+// Hk[b][z][m2][m1] into LDS and transformed there: (with YF the inverse y
+// transform per (plane, m1) column, then) the inverse x transform of every
+// row, each real row written over its own complex row with x[0], x[1]
+// repeated at K1, K1 + 1.  Returns the real planes [nq][K2][K1 + 2]; every
+// thread of the workgroup takes part, and the last barrier has been passed.
+template <typename T, int K1, bool YF>
+__device__ __forceinline__ T *xslab_stage(const typename C2<T>::type *__restrict__ Hk,
+                                          typename C2<T>::type *stage, int b, int zz0, int nq, int K2,
+                                          int K3) {
+  using V = typename C2<T>::type;
+  constexpr int H = K1 / 2 + 1;
+  constexpr int K1P = K1 + 2;
+  const int cplane = K2 * H;
+  // the nq complex planes straight into LDS: 16-byte global-to-LDS loads,
+  // one 1 KiB run per wave instruction, all in flight until the wait
+  {
+    const int ln = threadIdx.x & 63, nch = cplane * (int)sizeof(V) / 16;
+    for (int q = 0; q < nq; ++q) {
+      const int z = (((zz0 + q) % K3) + K3) % K3;
+      const uint4 *src = reinterpret_cast<const uint4 *>(Hk + ((size_t)b * K3 + z) * cplane);
+      uint4 *dst = reinterpret_cast<uint4 *>(stage + (size_t)q * cplane);
+      for (int e0 = (int)(threadIdx.x & ~63u); e0 < nch; e0 += blockDim.x)
+        if (e0 + ln < nch)
+          __builtin_amdgcn_global_load_lds(src + e0 + ln, (__attribute__((address_space(3))) void *)(dst + e0), 16,
+                                           0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+  }
+  __syncthreads();
+  if constexpr (YF) {
+    // inverse y transform of column (q, m1), in place (consecutive lanes
+    // take consecutive m1: distinct banks)
+    for (int t = threadIdx.x; t < nq * H; t += blockDim.x) {
+      const int q = t / H, m1 = t - q * H;
+      V *col = stage + (size_t)q * cplane + m1;
+      V X[K1];
+#pragma unroll
+      for (int m2 = 0; m2 < K1; ++m2) X[m2] = col[(size_t)m2 * H];
+      reg_fft<K1, +1>(X);
+#pragma unroll
+      for (int y = 0; y < K1; ++y) col[(size_t)y * H] = X[y];
+    }
+    __syncthreads();
+  }
+  // the real row (q, y) occupies exactly the bytes of the complex row
+  // (q, y) (K1P = 2H), which only this thread reads: no barrier between
+  // the transform and its write, any number of rows per thread
+  T *real = reinterpret_cast<T *>(stage);
+  for (int t = threadIdx.x; t < nq * K2; t += blockDim.x) {
+    const int q = t / K2, y = t - q * K2;
+    V Z[K1 / 2];
+    c2r_half<K1>(stage + (size_t)q * cplane + (size_t)y * H, Z);
+    V *d = reinterpret_cast<V *>(real + (size_t)q * cplane * 2 + (size_t)y * K1P);
+#pragma unroll
+    for (int m = 0; m < K1 / 2; ++m) d[m] = Z[m];
+    d[K1 / 2] = Z[0];  // x[0], x[1] again at K1, K1 + 1 (the gather's wrap)
+  }
+  __syncthreads();
+  return real;
+}
+
 // YF: the planes arrive before the inverse y transform, which is done here
 // per (plane, m1) column in LDS (K2 = K1), so k_pme_ycol_reg is skipped
 template <typename T, int K1, int MAXR, int XT = NTHR, bool YF = false>
@@ -1436,12 +1498,9 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
                                                     const T *__restrict__ zq, const T *__restrict__ zw,
                                                     const int *__restrict__ zsite, double *e_site) {
   using V = typename C2<T>::type;
-  constexpr int H = K1 / 2 + 1;
-  constexpr int K1P = K1 + 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int K2 = P.K2, K3 = P.K3, p = P.p;
-  const int cplane = K2 * H;                  // complex elements per plane
-  V *stage = reinterpret_cast<V *>(lds_raw);  // [Q][K2][H] complex, then [Q][K2][K1P] real in place
+  V *stage = reinterpret_cast<V *>(lds_raw);  // [Q][K2][H] complex, then [Q][K2][K1 + 2] real in place
   // XCD-aware order: workgroups go to the 8 XCDs round-robin by linear
   // index, so XCD x is given the x-th contiguous share of the (configuration,
   // slab) items; consecutive slabs of a configuration then run on one XCD
@@ -1463,51 +1522,7 @@ __global__ __launch_bounds__(XT) void k_pme_xslab(Params P, int S, int Q, int ns
   const size_t np = (size_t)nsites;
   for (int zz0 = zb - p + 1; zz0 < ze; zz0 += Q) {
     const int nq = min(Q, ze - zz0);
-    // the nq complex planes straight into LDS: 16-byte global-to-LDS loads,
-    // one 1 KiB run per wave instruction, all in flight until the wait
-    {
-      const int ln = threadIdx.x & 63, nch = cplane * (int)sizeof(V) / 16;
-      for (int q = 0; q < nq; ++q) {
-        const int z = (((zz0 + q) % K3) + K3) % K3;
-        const uint4 *src = reinterpret_cast<const uint4 *>(Hk + ((size_t)b * K3 + z) * cplane);
-        uint4 *dst = reinterpret_cast<uint4 *>(stage + (size_t)q * cplane);
-        for (int e0 = (int)(threadIdx.x & ~63u); e0 < nch; e0 += blockDim.x)
-          if (e0 + ln < nch)
-            __builtin_amdgcn_global_load_lds(src + e0 + ln, (__attribute__((address_space(3))) void *)(dst + e0), 16,
-                                             0, 0);
-      }
-      __builtin_amdgcn_s_waitcnt(0);
-    }
-    __syncthreads();
-    if constexpr (YF) {
-      // inverse y transform of column (q, m1), in place (consecutive lanes
-      // take consecutive m1: distinct banks)
-      for (int t = threadIdx.x; t < nq * H; t += blockDim.x) {
-        const int q = t / H, m1 = t - q * H;
-        V *col = stage + (size_t)q * cplane + m1;
-        V X[K1];
-#pragma unroll
-        for (int m2 = 0; m2 < K1; ++m2) X[m2] = col[(size_t)m2 * H];
-        reg_fft<K1, +1>(X);
-#pragma unroll
-        for (int y = 0; y < K1; ++y) col[(size_t)y * H] = X[y];
-      }
-      __syncthreads();
-    }
-    // the real row (q, y) occupies exactly the bytes of the complex row
-    // (q, y) (K1P = 2H), which only this thread reads: no barrier between
-    // the transform and its write, any number of rows per thread
-    T *real = reinterpret_cast<T *>(stage);
-    for (int t = threadIdx.x; t < nq * K2; t += blockDim.x) {
-      const int q = t / K2, y = t - q * K2;
-      V Z[K1 / 2];
-      c2r_half<K1>(stage + (size_t)q * cplane + (size_t)y * H, Z);
-      V *d = reinterpret_cast<V *>(real + (size_t)q * cplane * 2 + (size_t)y * K1P);
-#pragma unroll
-      for (int m = 0; m < K1 / 2; ++m) d[m] = Z[m];
-      d[K1 / 2] = Z[0];  // x[0], x[1] again at K1, K1 + 1 (the gather's wrap)
-    }
-    __syncthreads();
+    T *real = xslab_stage<T, K1, YF>(Hk, stage, b, zz0, nq, K2, K3);
     // gather: owned sites whose support g3 - c3 meets planes [zz0, zz0 + nq);
     // the common spline orders get fully unrolled register code
     if (p == 4)
@@ -2891,50 +2906,66 @@ static __global__ __launch_bounds__(NTHR) void k_bin(Params P, const double *__r
       atomicMax(&mx[rs], rmx);
     }
   };
-  for (int j = threadIdx.x; j < P.slvmax; j += blockDim.x) {
-    if (j == sj) continue;
-    int pti = mol_spec[j];
-    if (pti <= 0 || pti > P.numslv) {
-      set_error(status, 8, first + b, j, 0.0);
-      continue;
+  // a thread's molecules j = tid, tid + blockDim.x, ... in order, their
+  // energies and species loaded four at a time (all in flight together)
+  constexpr int JB = 4;
+  for (int j0 = threadIdx.x; j0 < P.slvmax; j0 += JB * (int)blockDim.x) {
+    double vb[JB];
+    int sb[JB];
+#pragma unroll
+    for (int k = 0; k < JB; ++k) {
+      const int jk = min(j0 + k * (int)blockDim.x, P.slvmax - 1);
+      vb[k] = u[jk];
+      sb[k] = mol_spec[jk];
     }
-    double v = u[j];
-    int off = spec_off[pti - 1], n = uvmax[pti - 1];
-    int lo = 0, hi = n;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (e[off + mid] < v)
-        lo = mid + 1;
-      else
-        hi = mid;
+#pragma unroll
+    for (int k = 0; k < JB; ++k) {
+      const int j = j0 + k * (int)blockDim.x;
+      if (j >= P.slvmax) break;
+      if (j == sj) continue;
+      int pti = sb[k];
+      if (pti <= 0 || pti > P.numslv) {
+        set_error(status, 8, first + b, j, 0.0);
+        continue;
+      }
+      double v = vb[k];
+      int off = spec_off[pti - 1], n = uvmax[pti - 1];
+      int lo = 0, hi = n;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (e[off + mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo <= 0) {
+        set_error(status, 1, first + b, j, v);
+        continue;
+      }
+      if (P.slttype == 1 && lo > uvsoft[pti - 1]) {
+        set_error(status, 2, first + b, j, v);
+        continue;
+      }
+      if (lo == n && v < 1.0e3 && !P.force) set_error(status, 3, first + b, j, v);
+      const int bin = off + lo - 1;
+      if (bin != rb) {
+        if (rc) atomicAdd(&h[rb], rc);
+        rb = bin;
+        rc = 0;
+      }
+      ++rc;
+      if (pti - 1 != rs) {
+        flush_spec();
+        rs = pti - 1;
+        rsum = 0.0;
+        rmn = ~0ULL;
+        rmx = 0ULL;
+      }
+      rsum += v;
+      const unsigned long long o = dbl_to_ord(v);
+      rmn = (o < rmn) ? o : rmn;
+      rmx = (o > rmx) ? o : rmx;
     }
-    if (lo <= 0) {
-      set_error(status, 1, first + b, j, v);
-      continue;
-    }
-    if (P.slttype == 1 && lo > uvsoft[pti - 1]) {
-      set_error(status, 2, first + b, j, v);
-      continue;
-    }
-    if (lo == n && v < 1.0e3 && !P.force) set_error(status, 3, first + b, j, v);
-    const int bin = off + lo - 1;
-    if (bin != rb) {
-      if (rc) atomicAdd(&h[rb], rc);
-      rb = bin;
-      rc = 0;
-    }
-    ++rc;
-    if (pti - 1 != rs) {
-      flush_spec();
-      rs = pti - 1;
-      rsum = 0.0;
-      rmn = ~0ULL;
-      rmx = 0ULL;
-    }
-    rsum += v;
-    const unsigned long long o = dbl_to_ord(v);
-    rmn = (o < rmn) ? o : rmn;
-    rmx = (o > rmx) ? o : rmx;
   }
   if (rc) atomicAdd(&h[rb], rc);
   flush_spec();
