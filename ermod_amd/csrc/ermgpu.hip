@@ -510,7 +510,10 @@ int zcol_groups(const erm_ctx *c, bool regp, int vb) {
     return (ncol + w - 1) / w;
   }
   const int zt = (c->zcol_T == 512) ? 512 : NTHR;
-  return ((zcol_fft(c, regp) ? 2 : 1) * ncol + zt - 1) / zt;
+  // lanes per column: the spectral form one, the spread-and-FFT form
+  // zfft_lanes (vb = 2 sizeof(T))
+  const int L = zcol_fft(c, regp) ? zfft_lanes(vb / 2, c->kp.K3) : 1;
+  return (L * ncol + zt - 1) / zt;
 }
 
 #define ERM_KSWITCH(K, CALL)                  \

@@ -975,19 +975,166 @@ __device__ __forceinline__ void zspread_half(V (&Y)[K3 / 2], const V a, const T 
 template <typename T> constexpr int zfft_sch = sizeof(T) == 4 ? 128 : 64;
 constexpr int ZFFT_KMAX = 96;  // largest z grid length of the register kernels
 
-// m2 rows a workgroup of XT threads (XT / 2 columns, m1 fastest) touches
-__host__ __device__ inline int zcol_fft_rows(int H, int K2, int XT) {
-  const int r = (XT / 2 - 1) / H + 2;
+// lanes per z column in k_pme_zcol_fft: one in FP32 up to K3 = 64 (the
+// whole column, 2 K3 floats, in one lane's registers), else a lane pair
+__host__ __device__ constexpr int zfft_lanes(int tbytes, int K3) { return (tbytes == 4 && K3 <= 64) ? 1 : 2; }
+
+// m2 rows a workgroup of XT threads (XT / L columns, m1 fastest) touches
+__host__ __device__ inline int zcol_fft_rows(int H, int K2, int XT, int L = 2) {
+  const int r = (XT / L - 1) / H + 2;
   return r < K2 ? r : K2;
 }
 
-// dynamic LDS of k_pme_zcol_fft with XT threads: zfft_sch<T> sites x (H + the
-// block's m2 rows) spectra, charge-scaled z weights, planes and ranks, the
-// bucket starts, and the self-energy reduction
-template <typename T> __host__ __device__ inline size_t zcol_fft_lds(int H, int K2, int XT = NTHR) {
-  const size_t spec = (size_t)zfft_sch<T> * (H + zcol_fft_rows(H, K2, XT)) * 2 * sizeof(T);
+// dynamic LDS of k_pme_zcol_fft with XT threads and L lanes per column:
+// zfft_sch<T> sites x (H + the block's m2 rows) spectra, charge-scaled z
+// weights, planes and ranks, the bucket starts, and the self-energy
+// reduction
+template <typename T>
+__host__ __device__ inline size_t zcol_fft_lds(int H, int K2, int XT = NTHR, int L = 2) {
+  const size_t spec = (size_t)zfft_sch<T> * (H + zcol_fft_rows(H, K2, XT, L)) * 2 * sizeof(T);
   const size_t small = (size_t)zfft_sch<T> * (sizeof(T) * MAXP + 2 * sizeof(int)) + (ZFFT_KMAX + 1) * sizeof(int);
   return ((spec + small + 15) & ~(size_t)15) + XT * sizeof(double);
+}
+
+// Site chunk of the z-column kernels in LDS: ranks by (plane, index), the
+// bucket starts per plane, the sorted spectra what1 (all m1) and what2 (the
+// block's nm2 rows from m2lo) and the charge-scaled z weights.  All threads
+// call it; the caller's barrier precedes the next chunk's call.
+template <typename T, int PP>
+__device__ __forceinline__ void zcol_stage_chunk(int s0, int nc, int K3, int H, int K2, int m2lo, int nm2,
+                                                 const T *__restrict__ sqv,
+                                                 const typename C2<T>::type *__restrict__ w1h,
+                                                 const typename C2<T>::type *__restrict__ w2h,
+                                                 const int *__restrict__ sg3, const T *__restrict__ sw3,
+                                                 typename C2<T>::type *l1, typename C2<T>::type *l2, T *lw,
+                                                 int *lg, int *lr, int *lst) {
+  __syncthreads();  // the previous chunk is consumed
+  for (int t = threadIdx.x; t < nc; t += blockDim.x) lg[t] = sg3[(size_t)s0 + t];
+  __syncthreads();
+  // rank by (plane, index); bucket starts = sites on lower planes (a plane
+  // outside [0, K3) falls in no bucket)
+  for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+    const int g = lg[t];
+    int r = 0;
+    for (int u = 0; u < nc; ++u) {
+      const int gu = lg[u];
+      r += (gu < g) || (gu == g && u < t);
+    }
+    lr[t] = r;
+  }
+  for (int t = threadIdx.x; t <= K3; t += blockDim.x) {
+    int r = 0;
+    for (int u = 0; u < nc; ++u) r += lg[u] < t;
+    lst[t] = r;
+  }
+  __syncthreads();
+  const int nsp = H + nm2;
+  for (int t = threadIdx.x; t < nc * nsp; t += blockDim.x) {
+    const int sl = t / nsp, m = t - sl * nsp;
+    const size_t bs = (size_t)s0 + sl;
+    const int r = lr[sl];
+    if (m < H)
+      l1[r * H + m] = w1h[bs * H + m];
+    else
+      l2[r * nm2 + (m - H)] = w2h[bs * K2 + m2lo + (m - H)];
+  }
+  for (int t = threadIdx.x; t < nc * PP; t += blockDim.x) {
+    const int sl = t / PP, k = t - sl * PP;
+    lw[lr[sl] * PP + k] = sqv[(size_t)s0 + sl] * sw3[(size_t)s0 * PP + t];
+  }
+  __syncthreads();
+}
+
+// One lane per column (m1, m2) (FP32, K3 <= 64): the same spread, Green's
+// function and transforms as the lane-pair form below, with the whole column
+// in one lane's registers in plane order.  Every site adds to its PP planes
+// directly (no lane masks), the forward and inverse z FFTs are one K3-point
+// register FFT each, and a wave's stores of one plane are one contiguous
+// run of 64 columns.
+template <typename T, int K3, int PP, int XT>
+__device__ __forceinline__ void zcol_fft_body1(const Params &P, int ns, int blk, const T *__restrict__ engfac,
+                                               const T *__restrict__ sqv,
+                                               const typename C2<T>::type *__restrict__ w1h,
+                                               const typename C2<T>::type *__restrict__ w2h,
+                                               const int *__restrict__ sg3, const T *__restrict__ sw3,
+                                               typename C2<T>::type *Hk, double *self_out) {
+  using V = typename C2<T>::type;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int H = P.H, K2 = P.K2;
+  const int rows = zcol_fft_rows(H, K2, XT, 1);
+  V *l1 = reinterpret_cast<V *>(lds_raw);
+  V *l2 = l1 + zfft_sch<T> * H;
+  T *lw = reinterpret_cast<T *>(l2 + zfft_sch<T> * rows);
+  int *lg = reinterpret_cast<int *>(lw + zfft_sch<T> * MAXP);
+  int *lr = lg + zfft_sch<T>;
+  int *lst = lr + zfft_sch<T>;
+  double *red = reinterpret_cast<double *>(lds_raw + zcol_fft_lds<T>(H, K2, XT, 1) - XT * sizeof(double));
+  const int ncol = H * K2;
+  const int c = blk * XT + (int)threadIdx.x;
+  const bool act = c < ncol;
+  const int m1 = act ? c % H : 0;
+  const int m2lo = (blk * XT) / H, m2hi = (min(ncol, (blk + 1) * XT) - 1) / H;
+  const int nm2 = m2hi - m2lo + 1;
+  const int m2 = act ? c / H - m2lo : 0;
+  V Y[K3];
+#pragma unroll
+  for (int z = 0; z < K3; ++z) {
+    Y[z].x = 0;
+    Y[z].y = 0;
+  }
+  const int lane = threadIdx.x & 63;
+  int lsa = 0, lsb = 0;
+  auto bucket = [&](auto gc) {
+    constexpr int G = decltype(gc)::value;
+    const int b0 = __builtin_amdgcn_readlane(G < 64 ? lsa : lsb, G & 63);
+    const int b1 = __builtin_amdgcn_readlane(G + 1 < 64 ? lsa : lsb, (G + 1) & 63);
+    for (int i = b0; i < b1; ++i) {
+      const V a = cmul(l1[i * H + m1], l2[i * nm2 + m2]);
+      T w[PP];
+#pragma unroll
+      for (int k = 0; k < PP; ++k) w[k] = lw[i * PP + k];  // one broadcast read
+#pragma unroll
+      for (int k = 0; k < PP; ++k) {
+        V &y = Y[(G - k + K3) % K3];
+        y.x = fma(a.x, w[k], y.x);
+        y.y = fma(a.y, w[k], y.y);
+      }
+    }
+  };
+  for (int s0 = 0; s0 < ns; s0 += zfft_sch<T>) {
+    const int nc = min(zfft_sch<T>, ns - s0);
+    zcol_stage_chunk<T, PP>(s0, nc, K3, H, K2, m2lo, nm2, sqv, w1h, w2h, sg3, sw3, l1, l2, lw, lg, lr, lst);
+    lsa = lst[lane];
+    if constexpr (K3 >= 64) lsb = lst[64 + lane < ZFFT_KMAX + 1 ? 64 + lane : ZFFT_KMAX];
+    [&]<int... Gs>(std::integer_sequence<int, Gs...>) {
+      (bucket(std::integral_constant<int, Gs>{}), ...);
+    }(std::make_integer_sequence<int, K3>{});
+  }
+  reg_fft<K3, -1>(Y);
+  double self = 0.0;
+  const double wgt = (m1 == 0 || (P.K1 % 2 == 0 && m1 == H - 1)) ? 0.5 : 1.0;
+  // engfac[m3][m2][m1]: a uniform row per m3 plus the lane's offset c
+  const unsigned goff = act ? (unsigned)c : 0u;
+  const T gmask = act ? (T)1 : (T)0;
+#pragma unroll
+  for (int m = 0; m < K3; ++m) {
+    const T g = gmask * engfac[(size_t)m * ncol + goff];
+    self += wgt * (double)g * ((double)Y[m].x * (double)Y[m].x + (double)Y[m].y * (double)Y[m].y);
+    Y[m].x *= g;
+    Y[m].y *= g;
+  }
+  red[threadIdx.x] = self;
+  reg_fft<K3, +1>(Y);
+  __syncthreads();
+  for (int st = XT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *self_out = red[0];
+  if (act) {
+#pragma unroll
+    for (int n = 0; n < K3; ++n) Hk[(size_t)n * ncol + c] = Y[n];
+  }
 }
 
 // Column (m1, m2) of the same product as k_pme_zcol_reg, with the z axis
@@ -1015,6 +1162,10 @@ __device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, 
                                               const int *__restrict__ sg3, const T *__restrict__ sw3,
                                               typename C2<T>::type *Hk, double *self_out) {
   static_assert(K3 <= ZFFT_KMAX, "z grid length");
+  if constexpr (zfft_lanes(sizeof(T), K3) == 1) {
+    zcol_fft_body1<T, K3, PP, XT>(P, ns, blk, engfac, sqv, w1h, w2h, sg3, sw3, Hk, self_out);
+    return;
+  }
   using V = typename C2<T>::type;
   constexpr int HK = K3 / 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];

@@ -32,7 +32,7 @@ template <typename T, int K> int launch_zcol_reg(erm_ctx *c, const Params &k, in
 template <typename T, int K, int PP, int XT>
 int launch_zcol_fft_t(erm_ctx *c, const Params &k, int n, int nblk, size_t site0, size_t cfg0) {
   using V = typename C2<T>::type;
-  const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT);
+  const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT, zfft_lanes(sizeof(T), K));
   const size_t hk = cfg0 * k.K3 * (size_t)k.K2 * k.H;
   TRY(set_lds(c, (const void *)k_pme_zcol_fft<T, K, PP, XT>, lds, "k_pme_zcol_fft"));
   hipLaunchKernelGGL((k_pme_zcol_fft<T, K, PP, XT>), dim3(nblk, n), dim3(XT), lds, c->st, k, dp<T>(c->d_engfac),
@@ -57,7 +57,7 @@ int launch_zcol_fft(erm_ctx *c, const Params &k, int n, int nblk, size_t site0, 
 template <typename T, int K, int PP, int XT>
 int launch_zcol_runs_t(erm_ctx *c, const Params &k, int nblk, const GRun *runs, int nrun, int nitem, int *counter) {
   using V = typename C2<T>::type;
-  const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT);
+  const size_t lds = zcol_fft_lds<T>(k.H, k.K2, XT, zfft_lanes(sizeof(T), K));
   const void *f = (const void *)k_pme_zcol_fft_runs<T, K, PP, XT>;
   TRY(set_lds(c, f, lds, "k_pme_zcol_fft_runs"));
   int occ = 0;
