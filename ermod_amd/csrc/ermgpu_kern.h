@@ -1194,7 +1194,6 @@ __device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, 
     Y[z].x = 0;
     Y[z].y = 0;
   }
-  const int nsp = H + nm2;
   // the bucket starts of the current chunk, one per lane: lane l holds
   // lst[l] in lsa and lst[64 + l] in lsb (lst spans ZFFT_KMAX + 1 words)
   const int lane = threadIdx.x & 63;
@@ -1216,40 +1215,7 @@ __device__ __forceinline__ void zcol_fft_body(const Params &P, int ns, int blk, 
   };
   for (int s0 = 0; s0 < ns; s0 += zfft_sch<T>) {
     const int nc = min(zfft_sch<T>, ns - s0);
-    __syncthreads();  // the previous chunk is consumed
-    for (int t = threadIdx.x; t < nc; t += blockDim.x) lg[t] = sg3[(size_t)s0 + t];
-    __syncthreads();
-    // rank by (plane, index); bucket starts = sites on lower planes (a
-    // plane outside [0, K3) falls in no bucket)
-    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
-      const int g = lg[t];
-      int r = 0;
-      for (int u = 0; u < nc; ++u) {
-        const int gu = lg[u];
-        r += (gu < g) || (gu == g && u < t);
-      }
-      lr[t] = r;
-    }
-    for (int t = threadIdx.x; t <= K3; t += blockDim.x) {
-      int r = 0;
-      for (int u = 0; u < nc; ++u) r += lg[u] < t;
-      lst[t] = r;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < nc * nsp; t += blockDim.x) {
-      const int sl = t / nsp, m = t - sl * nsp;
-      const size_t bs = (size_t)s0 + sl;
-      const int r = lr[sl];
-      if (m < H)
-        l1[r * H + m] = w1h[bs * H + m];
-      else
-        l2[r * nm2 + (m - H)] = w2h[bs * K2 + m2lo + (m - H)];
-    }
-    for (int t = threadIdx.x; t < nc * PP; t += blockDim.x) {
-      const int sl = t / PP, k = t - sl * PP;
-      lw[lr[sl] * PP + k] = sqv[(size_t)s0 + sl] * sw3[(size_t)s0 * PP + t];
-    }
-    __syncthreads();
+    zcol_stage_chunk<T, PP>(s0, nc, K3, H, K2, m2lo, nm2, sqv, w1h, w2h, sg3, sw3, l1, l2, lw, lg, lr, lst);
     lsa = lst[lane];
     if constexpr (K3 >= 64) lsb = lst[64 + lane < ZFFT_KMAX + 1 ? 64 + lane : ZFFT_KMAX];
     [&]<int... Gs>(std::integer_sequence<int, Gs...>) {
