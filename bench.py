@@ -319,7 +319,9 @@ def run_mixed(torch, syss, dpos, prec, steps, warmup, per_frame, tune=True, sche
                                 "slab_shape_xy_ms": round(sbest[0], 3),
                                 "shapes_xy_ms": [dict(shape, xy_ms=t) for shape, t in zip(shapes, shape_ms)]}
             else:
-                group_tuning = {"max_batch": total, "persistent": 2}
+                # the group tuning the warm-up timing settles on for this
+                # workload (profiling runs): one batch, both grids persistent
+                group_tuning = {"max_batch": total, "persistent": 3}
             group.set_tuning(max_batch=group_tuning["max_batch"], persistent=group_tuning["persistent"])
         for i in range(warmup):
             frame(i)
